@@ -1,0 +1,69 @@
+// xc_internal.h -- shared definitions of libxcore_hip (not part of the ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "xcore_hip.h"
+
+#define XC_MAX_SCRATCH 32
+
+struct xc_ctx {
+    int device;
+    unsigned flags;
+    hipStream_t own_stream;   // created by xc_create
+    hipStream_t stream;       // where work is enqueued (own_stream or the caller's)
+    char err[1024];
+    void* scratch[XC_MAX_SCRATCH];
+    size_t scratch_bytes[XC_MAX_SCRATCH];
+    void* ext[8];             // per-subsystem state (FFT plans, solver hierarchies, ...)
+    void (*ext_free[8])(xc_ctx*, void*);  // releases ext[i] in xc_destroy
+};
+
+enum { XC_EXT_NOISE = 0, XC_EXT_STOKES = 1, XC_EXT_UNET = 2, XC_EXT_STEP = 3 };
+
+// Scratch slots: one growing device buffer per slot, owned by the context.
+enum {
+    XC_SLOT_REDUCE = 0,
+    XC_SLOT_HOSTRED = 1,
+    XC_SLOT_FIELDS = 2,
+    XC_SLOT_NOISE = 8,
+    XC_SLOT_STOKES = 16,
+};
+
+int xc_fail(xc_ctx* ctx, int code, const char* fmt, ...);
+// After enqueueing kernels: report launch errors; synchronize in XC_CTX_SYNC mode.
+int xc_finish(xc_ctx* ctx, const char* what);
+// Device scratch of at least `bytes` in `slot` (contents undefined).
+int xc_scratch(xc_ctx* ctx, int slot, size_t bytes, void** out);
+
+#define XC_TRY_HIP(ctx, call)                                                              \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) return xc_fail((ctx), XC_EHIP, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+#define XC_CHECK_CTX(ctx)                                                                  \
+    do {                                                                                   \
+        if (!(ctx)) return XC_EINVAL_ARG;                                                  \
+    } while (0)
+
+// Column-major accessor
+#define XM(m, i, j) ((m).p[(int64_t)(i) + (int64_t)(j) * (m).ld])
+
+static inline bool xc_mat_ok(const xc_mat& m) {
+    return m.p != nullptr && m.rows >= 0 && m.cols >= 0 && (m.cols <= 1 || m.ld >= m.rows);
+}
+static inline bool xc_shape(const xc_mat& m, int64_t r, int64_t c) {
+    return m.rows == r && m.cols == c && xc_mat_ok(m);
+}
+
+// Hardware dispatches workgroup b of a 1-D grid to XCD (b % 8).  Remap so each
+// XCD walks a contiguous run of logical tiles: neighbouring tiles then share an
+// L2 and their halos are re-read from cache rather than from HBM.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+    const int xcd = b & 7, slot = b >> 3;
+    const int per = nb >> 3, rem = nb & 7;
+    return xcd * per + (xcd < rem ? xcd : rem) + slot;
+}

@@ -1,0 +1,16 @@
+"""xcore_amd -- MI355X (gfx950) implementation of xcore.jl's per-timestep hot path.
+
+Host-side mirror of the reference's Julia interface over the C ABI of
+libxcore_hip.so (include/xcore_hip.h).  Device fields are float64 torch
+tensors in the reference's column-major layout (see ``arrays``).
+"""
+from ._lib import LIB_PATH, NoConvergence, XCoreError, declared_symbols, lib
+from .arrays import as_mat, interior, to_device, to_host, xcore_zeros
+from .kernels import (advect_, advect_centered_, advect_centered_with_flux_, advect_with_flux_, context,
+                      diffus_, diffus_centered_, diffus_centered_with_flux_, diffus_with_flux_,
+                      embed_interior_, fill_ghosts_, scheme_halo)
+
+
+def load():
+    """Load libxcore_hip.so; raises if it is missing (there is no fallback)."""
+    return lib()
