@@ -52,7 +52,8 @@ enum { XC_PERIODIC = 0, XC_CLOSED = 1 };
  *      backend.jl:12-13, as the owner of device state) ------------------- */
 int         xc_create(xc_ctx** ctx, int device, unsigned flags);
 int         xc_destroy(xc_ctx* ctx);
-int         xc_set_stream(xc_ctx* ctx, void* hip_stream);  /* NULL = the context's own stream */
+int         xc_set_stream(xc_ctx* ctx, void* hip_stream);  /* enqueue on this stream (NULL = default stream) */
+int         xc_use_own_stream(xc_ctx* ctx);                /* back to the stream xc_create made */
 void*       xc_get_stream(xc_ctx* ctx);
 int         xc_set_flags(xc_ctx* ctx, unsigned flags);
 int         xc_sync(xc_ctx* ctx);
@@ -92,6 +93,81 @@ int xc_diffus_with_flux(xc_ctx* ctx, xc_mat dff, xc_mat qx, xc_mat qz, xc_mat f_
 /* phsevo.jl:67-79 diffus_centered! (qx == qz == NULL) and 135-149 diffus_centered_with_flux! */
 int xc_diffus_centered(xc_ctx* ctx, xc_mat dff, const xc_mat* qx, const xc_mat* qz,
                        xc_mat f, xc_mat k, double h, int xbc, int zbc);
+
+
+/* ---- state containers: field-by-field mirrors of src/state.jl -------------
+ * Shapes (Nz x Nx grid): cell (Nz,Nx); W-type (Nz+1,Nx+2); U-type (Nz+2,Nx+1);
+ * P (Nz+2,Nx+2); corner (Nz+1,Nx+1); z-face (Nz+1,Nx); x-face (Nz,Nx+1). */
+typedef struct { uint8_t* p; int64_t rows, cols, ld; } xc_bmat;   /* Matrix{Bool} */
+
+typedef struct {  /* state.jl:74-93 FluidState (the host-side solver cache is not part of it) */
+    xc_mat W, U, P, eta, etaco, rho, rhow, rhou, Drho, rhoo, rhooo,
+           drhodt, drhodto, drhodtoo, MFS, rhoWo, rhoWoo, rhoUo, rhoUoo;
+} xc_fluid;
+
+typedef struct {  /* state.jl:165-209 PhaseState */
+    xc_mat x, m, X, M, Xo, Xoo, Mo, Moo, chi, mu;
+    xc_bmat hasx, hasm;
+    xc_mat chiw, muw, chiu, muu, x_w, m_w, x_u, m_u, Drhom, Drhox, Pl, Pt,
+           etamix, etae, ke, kx, etat, etas, etasw, ks, V, Div_V, exx, ezz, exz, eII,
+           vx, vm, txx, tzz, txz, tII, fReL, ReL, ReD, Ra, fRel, Rel, Red, Rc,
+           Wx, Wm, Ux, Um, wx, wm, bndtaperw, bndshape, advn_X, advn_M, advn_rho,
+           qx_advn_X, qz_advn_X, qx_advn_M, qz_advn_M, qx_dffn_X, qz_dffn_X, qx_dffn_M, qz_dffn_M,
+           dffn_X, Gx, dXdt, dXdto, dXdtoo;
+} xc_phase;
+
+typedef struct {  /* noise.jl:38-53 NoiseState; the Gaussian filters and the RNG live in xc_ctx */
+    xc_mat psie, psix, psis, psieo, psixo, psiso, re, rs;
+    int64_t padL0, padl0;
+    double fL, fl;
+    xc_mat xiew, xieu, xixw, xixu, xisw, xisu, xie, xix, xis;
+    uint64_t seed;   /* the NoiseState's RNG (noise.jl:91): Philox4x32-10 key ... */
+    uint64_t draws;  /* ... and its counter, advanced by one per white-noise draw */
+} xc_noise;
+
+typedef struct { int64_t Nz, Nx; double h, D, L; } xc_grid;  /* grid.jl:24-43 (scalars) */
+
+typedef struct {  /* parameters.jl:87-164: the knobs the hot path reads */
+    double rhom0, rhox0, d0, g0, etam0, etax0;
+    double AA[4], BB[4], CC[4];  /* row-major 2x2: [11, 12, 21, 22] */
+    double Xi, Ptop, alpha, gamma, CFL, dtmax, etacntr;
+} xc_par;
+
+typedef struct { double L0, L0h, l0, l0h, D0, G0, txi0; } xc_scales;  /* scales.jl:290-334 subset */
+
+/* ---- field updates -------------------------------------------------------- */
+/* phsevo.jl:164-214 phsevo!(phase, fluid, grid, par, scales; ADVN, xBC, zBC, a1..b3, dt, alpha);
+ * coef = {a1, a2, a3, b1, b2, b3} from time_coefs (timing.jl:18-36) */
+int xc_phsevo(xc_ctx* ctx, xc_phase* phase, xc_fluid* fluid, const xc_grid* grid, const xc_par* par,
+              const xc_scales* scales, int scheme, int xbc, int zbc, const double coef[6], double dt,
+              double alpha);
+/* update.jl:525-538 update!(phase, fluid, grid, par, scales; xBC, zBC) */
+int xc_update(xc_ctx* ctx, xc_phase* phase, xc_fluid* fluid, const xc_grid* grid, const xc_par* par,
+              const xc_scales* scales, int xbc, int zbc);
+/* update.jl:494-502 update_dt(phase, fluid, grid, par, dt_prev): synchronizes, writes *dt_out */
+int xc_update_dt(xc_ctx* ctx, const xc_phase* phase, const xc_fluid* fluid, const xc_grid* grid,
+                 const xc_par* par, double dt_prev, double* dt_out);
+/* store.jl:13-30 store_previous!(fluid, phase) */
+int xc_store_previous(xc_ctx* ctx, xc_fluid* fluid, xc_phase* phase);
+/* store.jl:80-105 update_phase_velocities!(phase, fluid, ns, grid, par; xBC, zBC),
+ * including update_segregation_speed! (store.jl:47-68) */
+int xc_update_phase_velocities(xc_ctx* ctx, xc_phase* phase, const xc_fluid* fluid, const xc_noise* ns,
+                               const xc_grid* grid, const xc_par* par, int xbc, int zbc);
+/* noise.jl:138-239 noise!(ns, phase, grid, par, scales, dt; first_iter).  When first_iter
+ * is nonzero fresh white noise re, rs is drawn (Philox4x32-10 + Box-Muller, keyed by
+ * ns->seed, counter ns->draws, which is then incremented).  The Gaussian filter runs as
+ * separable batched mixed-radix FFTs in LDS (line lengths Nx and Nz+pad up to 2048). */
+int xc_noise_step(xc_ctx* ctx, xc_noise* ns, const xc_phase* phase, const xc_grid* grid, const xc_par* par,
+                  const xc_scales* scales, double dt, int first_iter);
+/* noise.jl:115-120 store_noise!(ns) */
+int xc_store_noise(xc_ctx* ctx, xc_noise* ns);
+/* the white-noise generator alone: standard normals into a (column-major) matrix */
+int xc_normal_fill(xc_ctx* ctx, xc_mat a, uint64_t seed, uint32_t stream, uint32_t draw);
+/* noise.jl:244-270 compute_fft_filter of one field (psi -> out, out != psi) */
+int xc_fft_filter(xc_ctx* ctx, xc_mat psi, xc_mat out, int64_t pad, double h, double sigma);
+
+/* timing.jl:18-36 time_coefs(T, TINT, step); tint 0 :be1im, 1 :bd2im, 2 :cn2si, 3 :bd2si */
+int xc_time_coefs(int tint, int64_t step, double coef_out[6]);
 
 #ifdef __cplusplus
 }

@@ -124,3 +124,315 @@ def diffus_centered(dff, f, k, h, xbc="periodic", zbc="closed", qx=None, qz=None
     _chk(lib().or_diffus_centered(om(dff), pqx, pqz, om(f), om(k), _h(h),
                                   BCS.get(xbc, 9), BCS.get(zbc, 9)), "diffus_centered")
     return dff
+
+
+# ---- state containers (numpy, Fortran order) and field updates --------------
+# Field order mirrors or_fluid / or_phase / or_noise in oracle.h (state.jl order).
+
+def _shape(code, Nz, Nx):
+    return {"c": (Nz, Nx), "w": (Nz + 1, Nx + 2), "u": (Nz + 2, Nx + 1), "p": (Nz + 2, Nx + 2),
+            "k": (Nz + 1, Nx + 1), "zf": (Nz + 1, Nx), "xf": (Nz, Nx + 1), "b": (Nz, Nx)}[code]
+
+
+FLUID = [("W", "w"), ("U", "u"), ("P", "p"), ("eta", "c"), ("etaco", "k"), ("rho", "c"), ("rhow", "zf"),
+         ("rhou", "xf"), ("Drho", "zf"), ("rhoo", "c"), ("rhooo", "c"), ("drhodt", "c"), ("drhodto", "c"),
+         ("drhodtoo", "c"), ("MFS", "c"), ("rhoWo", "zf"), ("rhoWoo", "zf"), ("rhoUo", "xf"), ("rhoUoo", "xf")]
+PHASE = [("x", "c"), ("m", "c"), ("X", "c"), ("M", "c"), ("Xo", "c"), ("Xoo", "c"), ("Mo", "c"), ("Moo", "c"),
+         ("chi", "c"), ("mu", "c"), ("hasx", "b"), ("hasm", "b"), ("chiw", "w"), ("muw", "w"), ("chiu", "u"),
+         ("muu", "u"), ("x_w", "zf"), ("m_w", "zf"), ("x_u", "xf"), ("m_u", "xf"), ("Drhom", "zf"), ("Drhox", "zf"),
+         ("Pl", "c"), ("Pt", "c"), ("etamix", "c"), ("etae", "c"), ("ke", "c"), ("kx", "c"), ("etat", "c"),
+         ("etas", "c"), ("etasw", "zf"), ("ks", "c"), ("V", "c"), ("Div_V", "c"), ("exx", "c"), ("ezz", "c"),
+         ("exz", "k"), ("eII", "c"), ("vx", "c"), ("vm", "c"), ("txx", "c"), ("tzz", "c"), ("txz", "k"),
+         ("tII", "c"), ("fReL", "c"), ("ReL", "c"), ("ReD", "c"), ("Ra", "c"), ("fRel", "c"), ("Rel", "c"),
+         ("Red", "c"), ("Rc", "c"), ("Wx", "w"), ("Wm", "w"), ("Ux", "u"), ("Um", "u"), ("wx", "w"), ("wm", "w"),
+         ("bndtaperw", "w"), ("bndshape", "c"), ("advn_X", "c"), ("advn_M", "c"), ("advn_rho", "c"),
+         ("qx_advn_X", "u"), ("qz_advn_X", "w"), ("qx_advn_M", "u"), ("qz_advn_M", "w"), ("qx_dffn_X", "u"),
+         ("qz_dffn_X", "w"), ("qx_dffn_M", "u"), ("qz_dffn_M", "w"), ("dffn_X", "c"), ("Gx", "c"), ("dXdt", "c"),
+         ("dXdto", "c"), ("dXdtoo", "c")]
+NOISE_A = [("psie", "c"), ("psix", "c"), ("psis", "c"), ("psieo", "c"), ("psixo", "c"), ("psiso", "c"),
+           ("re", "c"), ("rs", "c")]
+NOISE_B = [("xiew", "w"), ("xieu", "u"), ("xixw", "w"), ("xixu", "u"), ("xisw", "w"), ("xisu", "u"),
+           ("xie", "c"), ("xix", "c"), ("xis", "c")]
+
+
+class OBMat(ctypes.Structure):
+    _fields_ = [("p", ctypes.c_void_p), ("rows", ctypes.c_int64), ("cols", ctypes.c_int64), ("ld", ctypes.c_int64)]
+
+
+def _mk(name, flist):
+    return type(name, (ctypes.Structure,), {"_fields_": [(n, OBMat if c == "b" else OMat) for n, c in flist]})
+
+
+OFluid = _mk("or_fluid", FLUID)
+OPhase = _mk("or_phase", PHASE)
+
+
+class ONoise(ctypes.Structure):
+    _fields_ = ([(n, OMat) for n, _ in NOISE_A] + [("padL0", ctypes.c_int64), ("padl0", ctypes.c_int64),
+                                                   ("fL", ctypes.c_double), ("fl", ctypes.c_double)] +
+                [(n, OMat) for n, _ in NOISE_B])
+
+
+class OGrid(ctypes.Structure):
+    _fields_ = [("Nz", ctypes.c_int64), ("Nx", ctypes.c_int64), ("h", ctypes.c_double), ("D", ctypes.c_double),
+                ("L", ctypes.c_double)]
+
+
+class OPar(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("rhom0", "rhox0", "d0", "g0", "etam0", "etax0")] + \
+               [("AA", ctypes.c_double * 4), ("BB", ctypes.c_double * 4), ("CC", ctypes.c_double * 4)] + \
+               [(n, ctypes.c_double) for n in ("Xi", "Ptop", "alpha", "gamma", "CFL", "dtmax", "etacntr")]
+
+
+class OScales(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("L0", "L0h", "l0", "l0h", "D0", "G0", "txi0")]
+
+
+def _obm(a):
+    assert a.dtype == np.uint8 and a.flags.f_contiguous
+    return OBMat(a.ctypes.data, a.shape[0], a.shape[1], max(a.shape[0], 1))
+
+
+class HostState:
+    """A numpy (Fortran-order) state container with the reference's shapes."""
+
+    def __init__(self, flist, Nz, Nx, struct):
+        self.flist, self.Nz, self.Nx, self.struct = flist, Nz, Nx, struct
+        for n, c in flist:
+            r, k = _shape(c, Nz, Nx)
+            setattr(self, n, np.zeros((r, k), dtype=np.uint8 if c == "b" else np.float64, order="F"))
+
+    def pack(self):
+        s = self.struct()
+        for n, c in self.flist:
+            a = getattr(self, n)
+            setattr(s, n, _obm(a) if c == "b" else om(a))
+        return s
+
+    def arrays(self):
+        return {n: getattr(self, n) for n, _ in self.flist}
+
+
+def fluid_state(Nz, Nx):
+    return HostState(FLUID, Nz, Nx, OFluid)
+
+
+def phase_state(Nz, Nx):
+    return HostState(PHASE, Nz, Nx, OPhase)
+
+
+class HostNoise:
+    def __init__(self, Nz, Nx, padL0, padl0, fL, fl):
+        self.Nz, self.Nx = Nz, Nx
+        self.padL0, self.padl0, self.fL, self.fl = padL0, padl0, fL, fl
+        for n, c in NOISE_A + NOISE_B:
+            setattr(self, n, np.zeros(_shape(c, Nz, Nx), order="F"))
+
+    def pack(self):
+        s = ONoise()
+        for n, _ in NOISE_A + NOISE_B:
+            setattr(s, n, om(getattr(self, n)))
+        s.padL0, s.padl0, s.fL, s.fl = int(self.padL0), int(self.padl0), float(self.fL), float(self.fl)
+        return s
+
+    def arrays(self):
+        return {n: getattr(self, n) for n, _ in NOISE_A + NOISE_B}
+
+
+def ogrid(g):
+    return OGrid(int(g.Nz), int(g.Nx), float(g.h), float(g.D), float(g.L))
+
+
+def opar(p):
+    s = OPar()
+    for n in ("rhom0", "rhox0", "d0", "g0", "etam0", "etax0", "Xi", "Ptop", "alpha", "gamma", "CFL", "dtmax",
+              "etacntr"):
+        setattr(s, n, float(getattr(p, n)))
+    s.AA = (ctypes.c_double * 4)(*map(float, p.AA))
+    s.BB = (ctypes.c_double * 4)(*map(float, p.BB))
+    s.CC = (ctypes.c_double * 4)(*map(float, p.CC))
+    return s
+
+
+def oscales(sc):
+    return OScales(*(float(getattr(sc, n)) for n in ("L0", "L0h", "l0", "l0h", "D0", "G0", "txi0")))
+
+
+_R = ctypes.byref
+TINTS = {"be1im": 0, "bd2im": 1, "cn2si": 2, "bd2si": 3}
+
+
+def time_coefs(tint, step):
+    c = (ctypes.c_double * 6)()
+    _chk(lib().or_time_coefs(TINTS.get(str(tint).lstrip(":"), 9), ctypes.c_int64(int(step)), c), "time_coefs")
+    return tuple(c)
+
+
+def update(ph, fl, g, par, sc, xbc="periodic", zbc="closed"):
+    p, f = ph.pack(), fl.pack()
+    _chk(lib().or_update(_R(p), _R(f), _R(ogrid(g)), _R(opar(par)), _R(oscales(sc)), BCS.get(xbc, 9),
+                         BCS.get(zbc, 9)), "update")
+
+
+def update_stage(name, ph, fl, g, par, sc, xbc="periodic", zbc="closed"):
+    """Run one update! sub-step (for per-stage tests)."""
+    p, f = ph.pack(), fl.pack()
+    L = lib()
+    if name == "volume_fractions":
+        st = L.or_update_volume_fractions(_R(p), _R(f), _R(opar(par)))
+    elif name == "interpolations":
+        st = L.or_update_interpolations(_R(p), _R(f), _R(ogrid(g)), _R(opar(par)), BCS[xbc], BCS[zbc])
+    elif name == "pressure":
+        st = L.or_update_pressure(_R(p), _R(f), _R(ogrid(g)), _R(opar(par)))
+    elif name == "rheology":
+        st = L.or_update_rheology(_R(p), _R(opar(par)))
+    elif name == "kinematics":
+        st = L.or_update_kinematics(_R(p), _R(f), _R(ogrid(g)))
+    else:
+        raise ValueError(name)
+    _chk(st, name)
+
+
+def phsevo(ph, fl, g, par, sc, scheme="weno5", xbc="periodic", zbc="closed", coef=(1, 1, 0, 1, 0, 0), dt=1.0,
+           alpha=None):
+    alpha = par.alpha if alpha is None else alpha
+    c = (ctypes.c_double * 6)(*map(float, coef))
+    p, f = ph.pack(), fl.pack()
+    _chk(lib().or_phsevo(_R(p), _R(f), _R(ogrid(g)), _R(opar(par)), _R(oscales(sc)), SCHEMES.get(scheme, 99),
+                         BCS.get(xbc, 9), BCS.get(zbc, 9), c, ctypes.c_double(dt), ctypes.c_double(alpha)),
+         "phsevo")
+
+
+def store_previous(fl, ph):
+    f, p = fl.pack(), ph.pack()
+    _chk(lib().or_store_previous(_R(f), _R(p)), "store_previous")
+
+
+def update_phase_velocities(ph, fl, ns, g, par, xbc="periodic", zbc="closed"):
+    p, f, n = ph.pack(), fl.pack(), ns.pack()
+    _chk(lib().or_update_phase_velocities(_R(p), _R(f), _R(n), _R(ogrid(g)), _R(opar(par)), BCS.get(xbc, 9),
+                                          BCS.get(zbc, 9)), "update_phase_velocities")
+
+
+def update_dt(ph, fl, g, par, dt_prev):
+    f = lib().or_update_dt
+    f.restype = ctypes.c_double
+    p, fs = ph.pack(), fl.pack()
+    return f(_R(p), _R(fs), _R(ogrid(g)), _R(opar(par)), ctypes.c_double(dt_prev))
+
+
+def rheology_cell(chi, mu, par):
+    """etamix of one cell (update.jl:144-177) via a 1x1 PhaseState."""
+    ph = phase_state(1, 1)
+    ph.chi[0, 0], ph.mu[0, 0] = chi, mu
+    update_stage("rheology", ph, fluid_state(1, 1), None, par, None)
+    return ph.etamix[0, 0]
+
+
+# ---- noise (noise.jl) -------------------------------------------------------
+
+def philox4x32_10(ctr, key):
+    c = (ctypes.c_uint32 * 4)(*ctr)
+    k = (ctypes.c_uint32 * 2)(*key)
+    o = (ctypes.c_uint32 * 4)()
+    lib().or_philox4x32_10(c, k, o)
+    return tuple(o)
+
+
+def normal_fill(a, seed, stream, draw):
+    _chk(lib().or_normal_fill(om(a), ctypes.c_uint64(seed), ctypes.c_uint32(stream), ctypes.c_uint32(draw)),
+         "normal_fill")
+    return a
+
+
+def fft_filter(psi, pad, h, sigma):
+    out = zeros(*psi.shape)
+    _chk(lib().or_fft_filter(om(psi), om(out), ctypes.c_int64(pad), ctypes.c_double(h), ctypes.c_double(sigma)),
+         "fft_filter")
+    return out
+
+
+def noise(ns, ph, g, par, sc, dt, first_iter=True, seed=0, draw=0):
+    n, p = ns.pack(), ph.pack()
+    _chk(lib().or_noise_step(_R(n), _R(p), _R(ogrid(g)), _R(opar(par)), _R(oscales(sc)), ctypes.c_double(dt),
+                        int(bool(first_iter)), ctypes.c_uint64(seed), ctypes.c_uint32(draw)), "noise")
+
+
+def store_noise(ns):
+    n = ns.pack()
+    _chk(lib().or_store_noise(_R(n)), "store_noise")
+
+
+# ---- Stokes (fluidmech.jl) --------------------------------------------------
+
+class OFmOpts(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("sds", "top_cnv", "bot_cnv", "open_cnv", "xbc", "zbc", "scheme")] + \
+               [(n, ctypes.c_double) for n in ("dt", "a1", "a2", "a3", "b1", "b2", "b3", "alpha", "gamma", "g0")]
+
+
+def fm_opts(par, sds=-1, top_cnv=1, bot_cnv=1, open_cnv=False, xbc="periodic", zbc="closed", dt=1e32,
+            a1=1.0, a2=1.0, a3=0.0, b1=1.0, b2=0.0, b3=0.0, scheme=None):
+    return OFmOpts(int(sds), int(top_cnv), int(bot_cnv), int(bool(open_cnv)), BCS[xbc], BCS[zbc],
+                   SCHEMES[scheme or par.ADVN], float(dt), float(a1), float(a2), float(a3), float(b1), float(b2),
+                   float(b3), float(par.alpha), float(par.gamma), float(par.g0))
+
+
+def fluidmech_assemble(fl, ph, g, opts, bnchm=None):
+    """LL (scipy CSR, duplicates summed) and RR of fluidmech.jl after the pins; mutates MFS/drhodt."""
+    import scipy.sparse as sp
+    Nz, Nx = g.Nz, g.Nx
+    n = (Nz + 1) * (Nx + 2) + (Nz + 2) * (Nx + 1) + (Nz + 2) * (Nx + 2)
+    cap = 32 * n
+    I = np.zeros(cap, dtype=np.int64)
+    J = np.zeros(cap, dtype=np.int64)
+    V = np.zeros(cap)
+    RR = np.zeros(n)
+    nnz = ctypes.c_int64(0)
+    f = fl.pack()
+    p = ph.pack() if ph is not None else phase_state(Nz, Nx).pack()
+    bn = None
+    if bnchm is not None:
+        arr = (OMat * 6)(*[om(bnchm[k]) for k in ("src_W", "src_U", "src_P", "W_exact", "U_exact", "P_exact")])
+        bn = arr
+    _chk(lib().or_fluidmech_assemble(_R(f), _R(p), _R(ogrid(g)), _R(opts), bn,
+                                     I.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                     J.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                     V.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.c_int64(cap),
+                                     _R(nnz), RR.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "fluidmech_assemble")
+    k = nnz.value
+    LL = sp.coo_matrix((V[:k], (I[:k], J[:k])), shape=(n, n)).tocsr()
+    LL.sum_duplicates()
+    return LL, RR
+
+
+def stokes_vector(fl):
+    return np.concatenate([fl.W.ravel(order="F"), fl.U.ravel(order="F"), fl.P.ravel(order="F")])
+
+
+def set_stokes_vector(fl, sol):
+    nw, nu = fl.W.size, fl.U.size
+    fl.W[:] = sol[:nw].reshape(fl.W.shape, order="F")
+    fl.U[:] = sol[nw:nw + nu].reshape(fl.U.shape, order="F")
+    fl.P[:] = sol[nw + nu:].reshape(fl.P.shape, order="F")
+
+
+def fluidmech(fl, ph, g, opts, bnchm=None):
+    """fluidmech.jl:71-534: assemble, scale (495-507), sparse LU solve (SuperLU in place of
+    UMFPACK), Newton-style update.  Returns (LL, RR) of the solved system."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    LL, RR = fluidmech_assemble(fl, ph, g, opts, bnchm)
+    Nz, Nx = g.Nz, g.Nx
+    nv = (Nz + 1) * (Nx + 2) + (Nz + 2) * (Nx + 1)
+    diagLL = np.abs(LL.diagonal())
+    scl_p = np.ones((Nz + 2, Nx + 2), order="F")
+    scl_p[1:-1, 1:-1] = fl.rho / fl.eta
+    extra = np.concatenate([np.zeros(nv), np.sqrt(1.0 / scl_p.ravel(order="F"))])
+    scl = 1.0 / (np.sqrt(diagLL) + extra)
+    SOL = stokes_vector(fl)
+    FF = scl * (LL @ SOL - RR)
+    S = sp.diags(scl)
+    LLs = (S @ LL @ S).tocsc()
+    UPD = scl * spla.splu(LLs).solve(FF)
+    set_stokes_vector(fl, SOL - UPD)
+    return LL, RR

@@ -87,7 +87,13 @@ int xc_destroy(xc_ctx* c) {
 
 int xc_set_stream(xc_ctx* c, void* s) {
     XC_CHECK_CTX(c);
-    c->stream = s ? (hipStream_t)s : c->own_stream;
+    c->stream = (hipStream_t)s;  // NULL is the legacy default stream, not the context's own
+    return XC_OK;
+}
+
+int xc_use_own_stream(xc_ctx* c) {
+    XC_CHECK_CTX(c);
+    c->stream = c->own_stream;
     return XC_OK;
 }
 

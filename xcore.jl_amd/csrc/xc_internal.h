@@ -49,6 +49,17 @@ int xc_scratch(xc_ctx* ctx, int slot, size_t bytes, void** out);
         if (!(ctx)) return XC_EINVAL_ARG;                                                  \
     } while (0)
 
+// Diffusion with diffusivity k1 .* k2 (phsevo.jl:192-194, Xkx = X .* kx) -- product formed per cell.
+int xc_diffus_centered_kprod(xc_ctx* ctx, xc_mat dff, const xc_mat* qx, const xc_mat* qz, xc_mat f, xc_mat k1,
+                             xc_mat k2, double h, int xbc, int zbc);
+
+// 2-D launch domain of R x C elements, 64 (z) x 4 (x) lanes per workgroup, XCD-aware order
+struct xc_dom { int R, C, nbz, nb; };
+static inline xc_dom xc_make_dom(int64_t R, int64_t C) {
+    xc_dom d; d.R = (int)R; d.C = (int)C; d.nbz = (int)((R + 63) / 64);
+    d.nb = d.nbz * (int)((C + 3) / 4); if (d.nb < 1) d.nb = 1; return d;
+}
+
 // Column-major accessor
 #define XM(m, i, j) ((m).p[(int64_t)(i) + (int64_t)(j) * (m).ld])
 
@@ -66,4 +77,9 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
     const int xcd = b & 7, slot = b >> 3;
     const int per = nb >> 3, rem = nb & 7;
     return xcd * per + (xcd < rem ? xcd : rem) + slot;
+}
+__device__ __forceinline__ void xc_dom_ij(const xc_dom& d, int* j, int* i) {
+    const int t = xcd_tile(blockIdx.x, d.nb);
+    *j = (t % d.nbz) * 64 + (threadIdx.x & 63);
+    *i = (t / d.nbz) * 4 + (threadIdx.x >> 6);
 }

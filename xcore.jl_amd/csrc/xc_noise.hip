@@ -1,0 +1,712 @@
+// xc_noise.hip -- Ornstein-Uhlenbeck noise with the zero-padded Gaussian FFT
+// filter (src/noise.jl) for gfx950, FP64.
+//
+// * White noise: counter-based Philox4x32-10 + Box-Muller, one lane per pair
+//   of normals; any (seed, draw, element) is reproducible without state.
+// * Filter: exp(-s^2 (kz^2 + kx^2)) = exp(-s^2 kz^2) exp(-s^2 kx^2), so the 2-D
+//   FFT -> multiply -> inverse FFT of noise.jl:254-256 runs as two batched
+//   1-D passes (x periodic over Nx; z over the zero-padded Nz+pad).  Each
+//   workgroup keeps its lines in LDS for the forward transform, the filter
+//   multiply and the inverse transform (Stockham autosort, mixed radix
+//   2/3/4/5/7/11/13, twiddles from a table computed on the host in long
+//   double).  Two real lines ride in one complex transform: the multiplier is
+//   real and even, so real and imaginary parts are filtered independently.
+// * Mean/std rescale (noise.jl:261-267): two-pass reductions (mean, then the
+//   centred sum of squares) over all six raw/filtered fields.
+// * Flux differencing, tapers and magnitudes (noise.jl:195-236) as two
+//   elementwise kernels that apply the rescale on the fly.
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "xc_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+constexpr int NT = 256;
+constexpr double EPS = 2.220446049250313e-16;
+constexpr int MAXLINE = 2048;  // complex points per LDS buffer (64 KiB ping-pong)
+constexpr int MAXST = 24;
+
+// ---------------------------------------------------------------- RNG
+__device__ __forceinline__ void philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+        c[1] = (uint32_t)p1;
+        c[3] = (uint32_t)p0;
+        c[0] = n0;
+        c[2] = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+// element pair p of a column-major matrix -> two standard normals (same mapping as the oracle)
+__global__ __launch_bounds__(NT) void k_normals(double* a0, int64_t ld0, double* a1, int64_t ld1, int rows,
+                                                int64_t n, uint64_t seed, uint32_t draw, uint32_t stream0) {
+    const int64_t p = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (2 * p >= n) return;
+    const int s = blockIdx.y;
+    double* a = s == 0 ? a0 : a1;
+    const int64_t ld = s == 0 ? ld0 : ld1;
+    uint32_t c[4] = {(uint32_t)p, (uint32_t)((uint64_t)p >> 32), draw, stream0 + (uint32_t)s};
+    philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const uint64_t A = ((uint64_t)c[1] << 32) | c[0], B = ((uint64_t)c[3] << 32) | c[2];
+    const double u1 = ((double)(A >> 11) + 1.0) * 0x1.0p-53;
+    const double u2 = (double)(B >> 11) * 0x1.0p-53;
+    const double r = sqrt(-2.0 * log(u1));
+    const double th = 2.0 * 3.14159265358979323846 * u2;
+    const int64_t e0 = 2 * p, e1 = e0 + 1;
+    a[(e0 % rows) + (e0 / rows) * ld] = r * cos(th);
+    if (e1 < n) a[(e1 % rows) + (e1 / rows) * ld] = r * sin(th);
+}
+
+// ---------------------------------------------------------------- OU update (noise.jl:159-183)
+struct OUArgs {
+    const double *V, *vx, *fReL, *ke, *chi, *ks;
+    int64_t ldV, ldvx, ldf, ldk, ldc, ldks;
+    double *psie, *psix, *psis;
+    int64_t lde, ldx, lds;
+    const double *psieo, *psixo, *psiso, *re, *rs;
+    int64_t ldeo, ldxo, ldso, ldre, ldrs;
+    double Xi, L0h, l0h, txi0, dt, fL, fl;
+    int Nz, Nx;
+    xc_dom d;
+};
+__global__ __launch_bounds__(NT) void k_ou(OUArgs a) {
+    int j, i;
+    xc_dom_ij(a.d, &j, &i);
+    if (j >= a.Nz || i >= a.Nx) return;
+    double taue = a.L0h / (2.0 * (a.V[j + i * a.ldV] + EPS));
+    double taus = a.l0h / (2.0 * (a.vx[j + i * a.ldvx] + EPS));
+    const double St = a.txi0 / (taue + EPS);
+    const double fReL = a.fReL[j + i * a.ldf], ke = a.ke[j + i * a.ldk], chi = a.chi[j + i * a.ldc];
+    const double q1 = fReL * ke / taue;
+    const double sge = a.Xi * sqrt(q1 > 0.0 ? q1 : 0.0);
+    const double q2 = chi * fReL * ke / taue * St / (1.0 + St * St);
+    const double sgx = a.Xi * sqrt(q2 > 0.0 ? q2 : 0.0);
+    const double q3 = chi * a.ks[j + i * a.ldks] / taus;
+    const double sgs = a.Xi * sqrt(q3 > 0.0 ? q3 : 0.0);
+    taue = 1.0 / (1.0 / taue + 1.0 / (1e2 * a.dt)) + a.dt;
+    taus = 1.0 / (1.0 / taus + 1.0 / (1e2 * a.dt)) + a.dt;
+    const double Fte = exp(-a.dt / taue), Fts = exp(-a.dt / taus);
+    double ge = 1.0 - Fte * Fte, gs = 1.0 - Fts * Fts;
+    ge = ge > 0.0 ? ge : 0.0;
+    gs = gs > 0.0 ? gs : 0.0;
+    const double re = a.re[j + i * a.ldre], rs = a.rs[j + i * a.ldrs];
+    a.psie[j + i * a.lde] = Fte * a.psieo[j + i * a.ldeo] + sqrt(ge) * sge * a.fL * re;
+    a.psix[j + i * a.ldx] = Fte * a.psixo[j + i * a.ldxo] + sqrt(ge) * sgx * a.fL * re;
+    a.psis[j + i * a.lds] = Fts * a.psiso[j + i * a.ldso] + sqrt(gs) * sgs * a.fl * rs;
+}
+
+// ---------------------------------------------------------------- LDS FFT
+struct Plan {
+    int n, nst;
+    int rad[MAXST];
+    const double2* tw;  // W_n^t = exp(-2 pi i t / n), t in [0, n)
+    const double* g;    // filter multiplier exp(-s^2 k_t^2) / n, t in [0, n)
+};
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+
+// One Stockham stage of radix R over nl lines of length n (src -> dst).
+template <int R, bool INV>
+__device__ __forceinline__ void stage(const double2* __restrict__ src, double2* __restrict__ dst, int n, int nl,
+                                      int Ns, const double2* __restrict__ tw) {
+    const int nR = n / R;
+    const int strideT = n / (Ns * R);
+    double2 wr[R];
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+        double2 w = tw[m * nR];  // W_R^m = W_n^{m n / R}
+        wr[m] = INV ? make_double2(w.x, -w.y) : w;
+    }
+    for (int k = threadIdx.x; k < nl * nR; k += NT) {
+        const int l = k / nR, j = k - l * nR;
+        const int base = l * n;
+        const int jm = j % Ns;
+        const int t0 = jm * strideT;
+        double2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            double2 x = src[base + j + r * nR];
+            if (r > 0) {
+                double2 w = tw[(t0 * r) % n];
+                if (INV) w.y = -w.y;
+                x = cmul(x, w);
+            }
+            v[r] = x;
+        }
+        double2 o[R];
+        if constexpr (R == 2) {
+            o[0] = cadd(v[0], v[1]);
+            o[1] = csub(v[0], v[1]);
+        } else if constexpr (R == 4) {
+            const double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+            const double2 s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
+            // forward: W_4 = -i
+            const double2 jd13 = INV ? make_double2(-d13.y, d13.x) : make_double2(d13.y, -d13.x);
+            o[0] = cadd(s02, s13);
+            o[2] = csub(s02, s13);
+            o[1] = cadd(d02, jd13);
+            o[3] = csub(d02, jd13);
+        } else {
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                double2 s = v[0];
+#pragma unroll
+                for (int r = 1; r < R; ++r) s = cadd(s, cmul(v[r], wr[(r * q) % R]));
+                o[q] = s;
+            }
+        }
+        const int idxD = (j / Ns) * Ns * R + jm;
+#pragma unroll
+        for (int r = 0; r < R; ++r) dst[base + idxD + r * Ns] = o[r];
+    }
+}
+
+template <bool INV>
+__device__ double2* lds_fft(double2* a, double2* b, int nl, const Plan& p) {
+    double2* src = a;
+    double2* dst = b;
+    int Ns = 1;
+    for (int s = 0; s < p.nst; ++s) {
+        const int R = p.rad[s];
+        switch (R) {
+            case 2: stage<2, INV>(src, dst, p.n, nl, Ns, p.tw); break;
+            case 3: stage<3, INV>(src, dst, p.n, nl, Ns, p.tw); break;
+            case 4: stage<4, INV>(src, dst, p.n, nl, Ns, p.tw); break;
+            case 5: stage<5, INV>(src, dst, p.n, nl, Ns, p.tw); break;
+            case 7: stage<7, INV>(src, dst, p.n, nl, Ns, p.tw); break;
+            case 11: stage<11, INV>(src, dst, p.n, nl, Ns, p.tw); break;
+            default: stage<13, INV>(src, dst, p.n, nl, Ns, p.tw); break;
+        }
+        __syncthreads();
+        double2* t = src;
+        src = dst;
+        dst = t;
+        Ns *= R;
+    }
+    return src;
+}
+
+struct FieldIO {
+    const double* in;
+    int64_t ld_in;
+    double* out;
+    int64_t ld_out;
+};
+
+// x pass: rows of a (Nz x Nx) field, periodic over Nx, 2*nl rows per workgroup.
+// blockIdx.y = field.
+struct XPassArgs {
+    FieldIO io[3];
+    Plan plan[3];
+    int Nz, Nx, nl, nblk;
+};
+__global__ __launch_bounds__(NT) void k_filter_x(XPassArgs a) {
+    extern __shared__ double2 sm[];
+    const int f = blockIdx.y;
+    const Plan& p = a.plan[f];
+    const FieldIO io = a.io[f];
+    const int n = a.Nx, nl = a.nl, rows = 2 * nl;
+    const int r0 = blockIdx.x * rows;
+    double2* A = sm;
+    double2* B = sm + nl * n;
+    for (int k = threadIdx.x; k < rows * n; k += NT) {
+        const int rr = k % rows, c = k / rows;
+        const int r = r0 + rr;
+        const double v = r < a.Nz ? io.in[r + (int64_t)c * io.ld_in] : 0.0;
+        double* slot = reinterpret_cast<double*>(&A[(rr >> 1) * n + c]);
+        slot[rr & 1] = v;
+    }
+    __syncthreads();
+    double2* X = lds_fft<false>(A, B, nl, p);
+    for (int k = threadIdx.x; k < nl * n; k += NT) {
+        const double g = p.g[k % n];
+        X[k].x *= g;
+        X[k].y *= g;
+    }
+    __syncthreads();
+    double2* Y = lds_fft<true>(X, X == A ? B : A, nl, p);
+    for (int k = threadIdx.x; k < rows * n; k += NT) {
+        const int rr = k % rows, c = k / rows;
+        const int r = r0 + rr;
+        if (r < a.Nz) {
+            const double* slot = reinterpret_cast<const double*>(&Y[(rr >> 1) * n + c]);
+            io.out[r + (int64_t)c * io.ld_out] = slot[rr & 1];
+        }
+    }
+}
+
+// z pass: columns zero-padded to Nz+pad (hp = pad/2 zeros on top), 2*nl columns per workgroup.
+struct ZPassArgs {
+    FieldIO io[3];
+    Plan plan[3];
+    int hp[3];
+    int nl[3];
+    int Nz, Nx;
+};
+__global__ __launch_bounds__(NT) void k_filter_z(ZPassArgs a) {
+    extern __shared__ double2 sm[];
+    const int f = blockIdx.y;
+    const Plan& p = a.plan[f];
+    const FieldIO io = a.io[f];
+    const int n = p.n, nl = a.nl[f], cols = 2 * nl, hp = a.hp[f];
+    const int c0 = blockIdx.x * cols;
+    if (c0 >= a.Nx) return;  // whole workgroup (uniform)
+    double2* A = sm;
+    double2* B = sm + nl * n;
+    for (int k = threadIdx.x; k < cols * n; k += NT) {
+        const int z = k % n, cc = k / n;
+        const int c = c0 + cc;
+        const int r = z - hp;
+        const double v = (c < a.Nx && r >= 0 && r < a.Nz) ? io.in[r + (int64_t)c * io.ld_in] : 0.0;
+        double* slot = reinterpret_cast<double*>(&A[(cc >> 1) * n + z]);
+        slot[cc & 1] = v;
+    }
+    __syncthreads();
+    double2* X = lds_fft<false>(A, B, nl, p);
+    for (int k = threadIdx.x; k < nl * n; k += NT) {
+        const double g = p.g[k % n];
+        X[k].x *= g;
+        X[k].y *= g;
+    }
+    __syncthreads();
+    double2* Y = lds_fft<true>(X, X == A ? B : A, nl, p);
+    for (int k = threadIdx.x; k < cols * a.Nz; k += NT) {
+        const int r = k % a.Nz, cc = k / a.Nz;
+        const int c = c0 + cc;
+        if (c < a.Nx) {
+            const double* slot = reinterpret_cast<const double*>(&Y[(cc >> 1) * n + r + hp]);
+            io.out[r + (int64_t)c * io.ld_out] = slot[cc & 1];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- statistics
+// six (Nz x Nx) arrays: raw e, x, s and filtered e, x, s
+struct StatArgs {
+    const double* a[6];
+    int64_t ld[6];
+    int Nz, Nx;
+    xc_dom d;
+    double* part;   // [6][nb]
+    double* stats;  // [6] means, [6] sums of squares -> see k_stat_final
+};
+template <bool CENTRED>
+__global__ __launch_bounds__(NT) void k_stat_partial(StatArgs s) {
+    int j, i;
+    xc_dom_ij(s.d, &j, &i);
+    const bool in = j < s.Nz && i < s.Nx;
+    __shared__ double red[6][NT];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        double v = 0.0;
+        if (in) {
+            v = s.a[q][j + (int64_t)i * s.ld[q]];
+            if (CENTRED) {
+                v -= s.stats[q];
+                v = v * v;
+            }
+        }
+        red[q][threadIdx.x] = v;
+    }
+    __syncthreads();
+    for (int st = NT / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+#pragma unroll
+            for (int q = 0; q < 6; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) s.part[(int64_t)q * s.d.nb + blockIdx.x] = red[q][0];
+}
+// sums partials; mode 0: stats[q] = mean; mode 1: stats[6+q] = std (n-1)
+__global__ __launch_bounds__(NT) void k_stat_final(const double* part, int nb, double n, int mode, double* stats) {
+    __shared__ double red[6][NT];
+    for (int q = 0; q < 6; ++q) {
+        double acc = 0.0;
+        for (int b = threadIdx.x; b < nb; b += NT) acc += part[(int64_t)q * nb + b];
+        red[q][threadIdx.x] = acc;
+    }
+    __syncthreads();
+    for (int st = NT / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int q = 0; q < 6; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 6; ++q) {
+            if (mode == 0) stats[q] = red[q][0] / n;
+            else stats[6 + q] = sqrt(red[q][0] / (n - 1.0));
+        }
+}
+
+// rescaled filtered value (noise.jl:265-267); st: [0..5] means, [6..11] stds
+__device__ __forceinline__ double resc(const double* flt, int64_t ld, int j, int i, const double* st, int f) {
+    const double v = flt[j + (int64_t)i * ld];
+    const double rm = st[f], fm = st[3 + f], rs = st[6 + f], fs = st[9 + f];
+    if (fs > EPS) return (v - fm) * rs / fs + rm;
+    return v;
+}
+
+// ---------------------------------------------------------------- fluxes (noise.jl:190-220)
+struct FluxArgs {
+    const double* flt[3];
+    int64_t ldf[3];
+    const double* st;
+    const double *chiw, *muw, *chiu, *muu;
+    int64_t ldchiw, ldmuw, ldchiu, ldmuu;
+    double *xiew, *xieu, *xixw, *xixu, *xisw, *xisu;
+    int64_t ldew, ldeu, ldxw, ldxu, ldsw, ldsu;
+    int Nz, Nx;
+    xc_dom d;
+};
+__global__ __launch_bounds__(NT) void k_noise_flux(FluxArgs a) {
+    int j, i;
+    xc_dom_ij(a.d, &j, &i);
+    const int Nz = a.Nz, Nx = a.Nx;
+    if (j >= Nz + 2 || i >= Nx + 2) return;
+    // ext(f, j, i) = flt_f[icz[j], icx[i]] with grid.icz closed, grid.icx periodic (grid.jl:68-69)
+    auto ext = [&](int f, int jj, int ii) -> double {
+        const int z = jj - 1 < 0 ? 0 : (jj - 1 >= Nz ? Nz - 1 : jj - 1);
+        const int x = ii - 1 < 0 ? Nx - 1 : (ii - 1 >= Nx ? 0 : ii - 1);
+        return resc(a.flt[f], a.ldf[f], z, x, a.st, f);
+    };
+    auto psiec = [&](int jj, int ii) -> double {
+        return (ext(0, jj, ii) + ext(0, jj, ii + 1) + ext(0, jj + 1, ii) + ext(0, jj + 1, ii + 1)) / 4.0;
+    };
+    if (i <= Nx) {  // U-type (Nz+2, Nx+1)
+        const int z = j - 1 < 0 ? 0 : (j - 1 >= Nz ? Nz - 1 : j - 1);
+        a.xieu[j + (int64_t)i * a.ldeu] = psiec(z + 1, i) - psiec(z, i);
+        const double mu = a.muu[j + (int64_t)i * a.ldmuu] - 0.4;
+        const double tap = (1.0 - exp(-a.chiu[j + (int64_t)i * a.ldchiu] / 1e-4)) *
+                           (1.0 - exp(-(mu > 0.0 ? mu : 0.0) / 0.05));
+        a.xixu[j + (int64_t)i * a.ldxu] = (-(ext(1, j, i + 1) - ext(1, j, i))) * tap;
+        a.xisu[j + (int64_t)i * a.ldsu] = (-(ext(2, j, i + 1) - ext(2, j, i))) * tap;
+    }
+    if (j <= Nz) {  // W-type (Nz+1, Nx+2)
+        const int x = i - 1 < 0 ? Nx - 1 : (i - 1 >= Nx ? 0 : i - 1);
+        a.xiew[j + (int64_t)i * a.ldew] = -(psiec(j, x + 1) - psiec(j, x));
+        const double mu = a.muw[j + (int64_t)i * a.ldmuw] - 0.4;
+        const double tap = (1.0 - exp(-a.chiw[j + (int64_t)i * a.ldchiw] / 1e-4)) *
+                           (1.0 - exp(-(mu > 0.0 ? mu : 0.0) / 0.05));
+        a.xixw[j + (int64_t)i * a.ldxw] = (-(ext(1, j + 1, i) - ext(1, j, i))) * tap;
+        a.xisw[j + (int64_t)i * a.ldsw] = (-(ext(2, j + 1, i) - ext(2, j, i))) * tap;
+    }
+}
+
+// noise.jl:223-236
+__global__ __launch_bounds__(NT) void k_noise_mag(const double* sw, int64_t ldsw, const double* su, int64_t ldsu,
+                                                  const double* xw, int64_t ldxw, const double* xu, int64_t ldxu,
+                                                  const double* ew, int64_t ldew, const double* eu, int64_t ldeu,
+                                                  double* xis, int64_t lds, double* xix, int64_t ldx, double* xie,
+                                                  int64_t lde, int Nz, int Nx, xc_dom d) {
+    int j, i;
+    xc_dom_ij(d, &j, &i);
+    if (j >= Nz || i >= Nx) return;
+    auto mag = [&](const double* w, int64_t lw, const double* u, int64_t lu) {
+        const double wz = (w[j + (int64_t)(i + 1) * lw] + w[j + 1 + (int64_t)(i + 1) * lw]) / 2.0;
+        const double wx = (u[j + 1 + (int64_t)i * lu] + u[j + 1 + (int64_t)(i + 1) * lu]) / 2.0;
+        return sqrt(wz * wz + wx * wx);
+    };
+    xis[j + (int64_t)i * lds] = mag(sw, ldsw, su, ldsu);
+    xix[j + (int64_t)i * ldx] = mag(xw, ldxw, xu, ldxu);
+    xie[j + (int64_t)i * lde] = mag(ew, ldew, eu, ldeu);
+}
+
+__global__ __launch_bounds__(NT) void k_resc_inplace(double* o, int64_t ldo, const double* st, int Nz, int Nx,
+                                                     xc_dom d) {
+    int j, i;
+    xc_dom_ij(d, &j, &i);
+    if (j >= Nz || i >= Nx) return;
+    o[j + (int64_t)i * ldo] = resc(o, ldo, j, i, st, 0);
+}
+
+__global__ __launch_bounds__(NT) void k_copy3(const double* a, int64_t lda, double* ao, int64_t ldao, const double* b,
+                                              int64_t ldb, double* bo, int64_t ldbo, const double* c, int64_t ldc,
+                                              double* co, int64_t ldco, int Nz, int Nx, xc_dom d) {
+    int j, i;
+    xc_dom_ij(d, &j, &i);
+    if (j >= Nz || i >= Nx) return;
+    ao[j + (int64_t)i * ldao] = a[j + (int64_t)i * lda];
+    bo[j + (int64_t)i * ldbo] = b[j + (int64_t)i * ldb];
+    co[j + (int64_t)i * ldco] = c[j + (int64_t)i * ldc];
+}
+
+// ---------------------------------------------------------------- host: plans
+struct PlanEntry {
+    int n;
+    double sigma, h;
+    std::vector<int> rad;
+    double2* tw;  // device
+    double* g;    // device
+};
+struct NoiseCtx {
+    std::vector<PlanEntry> plans;
+};
+
+void noise_free(xc_ctx*, void* p) {
+    NoiseCtx* nc = (NoiseCtx*)p;
+    for (auto& e : nc->plans) {
+        (void)hipFree(e.tw);
+        (void)hipFree(e.g);
+    }
+    delete nc;
+}
+
+bool factorize(int n, std::vector<int>& rad) {
+    rad.clear();
+    static const int order[] = {4, 2, 3, 5, 7, 11, 13};
+    for (int r : order)
+        while (n % r == 0) {
+            rad.push_back(r);
+            n /= r;
+        }
+    return n == 1 && (int)rad.size() <= MAXST;
+}
+
+// fftfreq(n, 1/h)[t] * 2 pi (AbstractFFTs ordering)
+double kfreq(int t, int n, double h) {
+    const int nn = (n + 1) / 2;
+    const double mult = (1.0 / h) / (double)n;
+    return 2.0 * 3.14159265358979323846 * ((double)(t < nn ? t : t - n) * mult);
+}
+
+int get_plan(xc_ctx* ctx, int n, double sigma, double h, Plan* out) {
+    NoiseCtx* nc = (NoiseCtx*)ctx->ext[XC_EXT_NOISE];
+    if (!nc) {
+        nc = new NoiseCtx();
+        ctx->ext[XC_EXT_NOISE] = nc;
+        ctx->ext_free[XC_EXT_NOISE] = noise_free;
+    }
+    for (auto& e : nc->plans)
+        if (e.n == n && e.sigma == sigma && e.h == h) {
+            out->n = n;
+            out->nst = (int)e.rad.size();
+            for (int s = 0; s < out->nst; ++s) out->rad[s] = e.rad[s];
+            out->tw = e.tw;
+            out->g = e.g;
+            return XC_OK;
+        }
+    PlanEntry e;
+    e.n = n;
+    e.sigma = sigma;
+    e.h = h;
+    if (n > MAXLINE || !factorize(n, e.rad))
+        return xc_fail(ctx, XC_EINVAL_SHAPE,
+                       "noise filter: line length %d must be <= %d with prime factors in {2,3,5,7,11,13}", n, MAXLINE);
+    std::vector<double2> tw(n);
+    std::vector<double> g(n);
+    for (int t = 0; t < n; ++t) {
+        const long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)t / (long double)n;
+        tw[t] = make_double2((double)cosl(ang), (double)sinl(ang));
+        const double k = kfreq(t, n, h);
+        g[t] = exp(-(sigma * sigma) * (k * k)) / (double)n;  // inverse-transform normalisation folded in
+    }
+    XC_TRY_HIP(ctx, hipMalloc(&e.tw, n * sizeof(double2)));
+    XC_TRY_HIP(ctx, hipMalloc(&e.g, n * sizeof(double)));
+    XC_TRY_HIP(ctx, hipMemcpy(e.tw, tw.data(), n * sizeof(double2), hipMemcpyHostToDevice));
+    XC_TRY_HIP(ctx, hipMemcpy(e.g, g.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    nc->plans.push_back(e);
+    return get_plan(ctx, n, sigma, h, out);
+}
+
+// Filter three (or fewer) fields: in[f] -> out[f] using scratch tmp[f] between the passes.
+int filter_fields(xc_ctx* ctx, int nf, const xc_mat* in, xc_mat* out, xc_mat* tmp, const int64_t* pad, double h,
+                  const double* sigma) {
+    const int Nz = (int)in[0].rows, Nx = (int)in[0].cols;
+    int st;
+    XPassArgs xa{};
+    ZPassArgs za{};
+    for (int f = 0; f < nf; ++f) {
+        if ((st = get_plan(ctx, Nx, sigma[f], h, &xa.plan[f]))) return st;
+        if ((st = get_plan(ctx, Nz + (int)pad[f], sigma[f], h, &za.plan[f]))) return st;
+        xa.io[f] = FieldIO{in[f].p, in[f].ld, tmp[f].p, tmp[f].ld};
+        za.io[f] = FieldIO{tmp[f].p, tmp[f].ld, out[f].p, out[f].ld};
+        za.hp[f] = (int)(pad[f] / 2);
+        za.nl[f] = MAXLINE / za.plan[f].n;
+        if (za.nl[f] > 4) za.nl[f] = 4;
+    }
+    xa.Nz = za.Nz = Nz;
+    xa.Nx = za.Nx = Nx;
+    xa.nl = MAXLINE / Nx;
+    if (xa.nl > 8) xa.nl = 8;
+    const int xblocks = (Nz + 2 * xa.nl - 1) / (2 * xa.nl);
+    hipLaunchKernelGGL(k_filter_x, dim3(xblocks, nf), dim3(NT), 2 * xa.nl * Nx * sizeof(double2), ctx->stream, xa);
+    int nlmin = za.nl[0], nmax = za.plan[0].n * za.nl[0];
+    for (int f = 1; f < nf; ++f) {
+        nlmin = za.nl[f] < nlmin ? za.nl[f] : nlmin;
+        nmax = za.plan[f].n * za.nl[f] > nmax ? za.plan[f].n * za.nl[f] : nmax;
+    }
+    const int zblocks = (Nx + 2 * nlmin - 1) / (2 * nlmin);
+    hipLaunchKernelGGL(k_filter_z, dim3(zblocks, nf), dim3(NT), 2 * nmax * sizeof(double2), ctx->stream, za);
+    return XC_OK;
+}
+
+bool shp(const xc_mat& m, int64_t r, int64_t c) { return xc_shape(m, r, c); }
+
+}  // namespace
+
+extern "C" {
+
+int xc_normal_fill(xc_ctx* ctx, xc_mat a, uint64_t seed, uint32_t stream, uint32_t draw) {
+    XC_CHECK_CTX(ctx);
+    if (!xc_mat_ok(a)) return xc_fail(ctx, XC_EINVAL_ARG, "normal_fill: bad matrix");
+    const int64_t n = a.rows * a.cols;
+    if (n == 0) return XC_OK;
+    const int64_t pairs = (n + 1) / 2;
+    hipLaunchKernelGGL(k_normals, dim3((unsigned)((pairs + NT - 1) / NT), 1), dim3(NT), 0, ctx->stream, a.p, a.ld,
+                       a.p, a.ld, (int)a.rows, n, seed, draw, stream);
+    return xc_finish(ctx, "normal_fill");
+}
+
+int xc_fft_filter(xc_ctx* ctx, xc_mat psi, xc_mat out, int64_t pad, double h, double sigma) {
+    XC_CHECK_CTX(ctx);
+    if (!xc_mat_ok(psi) || !xc_shape(out, psi.rows, psi.cols) || psi.rows < 2 || psi.cols < 1 || pad < 0)
+        return xc_fail(ctx, XC_EINVAL_SHAPE, "compute_fft_filter: shapes");
+    const int Nz = (int)psi.rows, Nx = (int)psi.cols;
+    double* sc;
+    const xc_dom d = xc_make_dom(Nz, Nx);
+    const size_t ncell = (size_t)Nz * Nx;
+    int st;
+    if ((st = xc_scratch(ctx, XC_SLOT_NOISE, (ncell + 6 * (size_t)d.nb + 12) * sizeof(double), (void**)&sc))) return st;
+    xc_mat tmp{sc, Nz, Nx, Nz};
+    if ((st = filter_fields(ctx, 1, &psi, &out, &tmp, &pad, h, &sigma))) return st;
+    // statistics over (raw, filtered) pairs; unused slots point at the same arrays
+    StatArgs sa{};
+    for (int q = 0; q < 3; ++q) {
+        sa.a[q] = psi.p; sa.ld[q] = psi.ld;
+        sa.a[3 + q] = out.p; sa.ld[3 + q] = out.ld;
+    }
+    sa.Nz = Nz; sa.Nx = Nx; sa.d = d;
+    sa.part = sc + ncell;
+    sa.stats = sa.part + 6 * d.nb;
+    hipLaunchKernelGGL(k_stat_partial<false>, dim3(d.nb), dim3(NT), 0, ctx->stream, sa);
+    hipLaunchKernelGGL(k_stat_final, dim3(1), dim3(NT), 0, ctx->stream, sa.part, d.nb, (double)ncell, 0, sa.stats);
+    hipLaunchKernelGGL(k_stat_partial<true>, dim3(d.nb), dim3(NT), 0, ctx->stream, sa);
+    hipLaunchKernelGGL(k_stat_final, dim3(1), dim3(NT), 0, ctx->stream, sa.part, d.nb, (double)ncell, 1, sa.stats);
+    // rescale to the raw mean/std (field 0's statistics), elementwise in place
+    hipLaunchKernelGGL(k_resc_inplace, dim3(d.nb), dim3(NT), 0, ctx->stream, out.p, out.ld, sa.stats, Nz, Nx, d);
+    return xc_finish(ctx, "compute_fft_filter");
+}
+
+int xc_store_noise(xc_ctx* ctx, xc_noise* ns) {
+    XC_CHECK_CTX(ctx);
+    const int64_t Nz = ns->psie.rows, Nx = ns->psie.cols;
+    if (!shp(ns->psieo, Nz, Nx) || !shp(ns->psix, Nz, Nx) || !shp(ns->psixo, Nz, Nx) || !shp(ns->psis, Nz, Nx) ||
+        !shp(ns->psiso, Nz, Nx))
+        return xc_fail(ctx, XC_EINVAL_SHAPE, "store_noise!: NoiseState shapes");
+    const xc_dom d = xc_make_dom(Nz, Nx);
+    hipLaunchKernelGGL(k_copy3, dim3(d.nb), dim3(NT), 0, ctx->stream, ns->psie.p, ns->psie.ld, ns->psieo.p,
+                       ns->psieo.ld, ns->psix.p, ns->psix.ld, ns->psixo.p, ns->psixo.ld, ns->psis.p, ns->psis.ld,
+                       ns->psiso.p, ns->psiso.ld, (int)Nz, (int)Nx, d);
+    return xc_finish(ctx, "store_noise!");
+}
+
+int xc_noise_step(xc_ctx* ctx, xc_noise* ns, const xc_phase* ph, const xc_grid* g, const xc_par* par,
+                  const xc_scales* sc, double dt, int first_iter) {
+    XC_CHECK_CTX(ctx);
+    if (!ns || !ph || !g || !par || !sc) return xc_fail(ctx, XC_EINVAL_ARG, "noise!: NULL argument");
+    const int64_t Nz = g->Nz, Nx = g->Nx;
+    const xc_mat* cellm[] = {&ns->psie, &ns->psix, &ns->psis, &ns->psieo, &ns->psixo, &ns->psiso, &ns->re, &ns->rs,
+                             &ns->xie, &ns->xix, &ns->xis, &ph->V, &ph->vx, &ph->fReL, &ph->ke, &ph->chi, &ph->ks};
+    for (auto m : cellm)
+        if (!shp(*m, Nz, Nx)) return xc_fail(ctx, XC_EINVAL_SHAPE, "noise!: cell-centred fields must be (Nz, Nx)");
+    if (!shp(ns->xiew, Nz + 1, Nx + 2) || !shp(ns->xixw, Nz + 1, Nx + 2) || !shp(ns->xisw, Nz + 1, Nx + 2) ||
+        !shp(ns->xieu, Nz + 2, Nx + 1) || !shp(ns->xixu, Nz + 2, Nx + 1) || !shp(ns->xisu, Nz + 2, Nx + 1) ||
+        !shp(ph->chiw, Nz + 1, Nx + 2) || !shp(ph->muw, Nz + 1, Nx + 2) || !shp(ph->chiu, Nz + 2, Nx + 1) ||
+        !shp(ph->muu, Nz + 2, Nx + 1))
+        return xc_fail(ctx, XC_EINVAL_SHAPE, "noise!: staggered noise/volume-fraction fields");
+    if (Nz < 2 || ns->padL0 < 0 || ns->padl0 < 0) return xc_fail(ctx, XC_EINVAL_ARG, "noise!: bad grid/padding");
+    const xc_dom d = xc_make_dom(Nz, Nx);
+    // 1. white noise (noise.jl:154-157)
+    if (first_iter) {
+        const int64_t n = Nz * Nx, pairs = (n + 1) / 2;
+        if (ns->re.ld != ns->rs.ld) {
+            hipLaunchKernelGGL(k_normals, dim3((unsigned)((pairs + NT - 1) / NT), 1), dim3(NT), 0, ctx->stream,
+                               ns->re.p, ns->re.ld, ns->re.p, ns->re.ld, (int)Nz, n, ns->seed, (uint32_t)ns->draws, 0u);
+            hipLaunchKernelGGL(k_normals, dim3((unsigned)((pairs + NT - 1) / NT), 1), dim3(NT), 0, ctx->stream,
+                               ns->rs.p, ns->rs.ld, ns->rs.p, ns->rs.ld, (int)Nz, n, ns->seed, (uint32_t)ns->draws, 1u);
+        } else {
+            hipLaunchKernelGGL(k_normals, dim3((unsigned)((pairs + NT - 1) / NT), 2), dim3(NT), 0, ctx->stream,
+                               ns->re.p, ns->re.ld, ns->rs.p, ns->rs.ld, (int)Nz, n, ns->seed, (uint32_t)ns->draws, 0u);
+        }
+        ns->draws += 1;
+    }
+    // 2. OU update (noise.jl:159-183)
+    const double h = g->h, L0h = sc->L0h, l0h = sc->l0h;
+    OUArgs oa{};
+    oa.V = ph->V.p; oa.ldV = ph->V.ld; oa.vx = ph->vx.p; oa.ldvx = ph->vx.ld;
+    oa.fReL = ph->fReL.p; oa.ldf = ph->fReL.ld; oa.ke = ph->ke.p; oa.ldk = ph->ke.ld;
+    oa.chi = ph->chi.p; oa.ldc = ph->chi.ld; oa.ks = ph->ks.p; oa.ldks = ph->ks.ld;
+    oa.psie = ns->psie.p; oa.lde = ns->psie.ld; oa.psix = ns->psix.p; oa.ldx = ns->psix.ld;
+    oa.psis = ns->psis.p; oa.lds = ns->psis.ld;
+    oa.psieo = ns->psieo.p; oa.ldeo = ns->psieo.ld; oa.psixo = ns->psixo.p; oa.ldxo = ns->psixo.ld;
+    oa.psiso = ns->psiso.p; oa.ldso = ns->psiso.ld; oa.re = ns->re.p; oa.ldre = ns->re.ld;
+    oa.rs = ns->rs.p; oa.ldrs = ns->rs.ld;
+    oa.Xi = par->Xi; oa.L0h = L0h; oa.l0h = l0h; oa.txi0 = sc->txi0; oa.dt = dt;
+    oa.fL = 2.0 / sqrt(1.0 - exp(-(h * h) / (2 * (L0h * L0h))));
+    oa.fl = 2.0 / sqrt(1.0 - exp(-(h * h) / (2 * (l0h * l0h))));
+    oa.Nz = (int)Nz; oa.Nx = (int)Nx; oa.d = d;
+    hipLaunchKernelGGL(k_ou, dim3(d.nb), dim3(NT), 0, ctx->stream, oa);
+    // 3. filters (noise.jl:186-188)
+    const size_t ncell = (size_t)Nz * Nx;
+    double* sb;
+    int st;
+    if ((st = xc_scratch(ctx, XC_SLOT_NOISE, (6 * ncell + 6 * (size_t)d.nb + 12) * sizeof(double), (void**)&sb)))
+        return st;
+    xc_mat in[3] = {ns->psie, ns->psix, ns->psis};
+    xc_mat tmp[3], flt[3];
+    for (int f = 0; f < 3; ++f) {
+        tmp[f] = xc_mat{sb + f * ncell, Nz, Nx, Nz};
+        flt[f] = xc_mat{sb + (3 + f) * ncell, Nz, Nx, Nz};
+    }
+    const int64_t pads[3] = {ns->padL0, ns->padL0, ns->padl0};
+    const double sig[3] = {L0h, L0h, l0h};
+    if ((st = filter_fields(ctx, 3, in, flt, tmp, pads, h, sig))) return st;
+    StatArgs sa{};
+    for (int f = 0; f < 3; ++f) {
+        sa.a[f] = in[f].p; sa.ld[f] = in[f].ld;
+        sa.a[3 + f] = flt[f].p; sa.ld[3 + f] = flt[f].ld;
+    }
+    sa.Nz = (int)Nz; sa.Nx = (int)Nx; sa.d = d;
+    sa.part = sb + 6 * ncell;
+    sa.stats = sa.part + 6 * d.nb;
+    hipLaunchKernelGGL(k_stat_partial<false>, dim3(d.nb), dim3(NT), 0, ctx->stream, sa);
+    hipLaunchKernelGGL(k_stat_final, dim3(1), dim3(NT), 0, ctx->stream, sa.part, d.nb, (double)ncell, 0, sa.stats);
+    hipLaunchKernelGGL(k_stat_partial<true>, dim3(d.nb), dim3(NT), 0, ctx->stream, sa);
+    hipLaunchKernelGGL(k_stat_final, dim3(1), dim3(NT), 0, ctx->stream, sa.part, d.nb, (double)ncell, 1, sa.stats);
+    // 4. fluxes and magnitudes (noise.jl:190-236)
+    FluxArgs fa{};
+    for (int f = 0; f < 3; ++f) { fa.flt[f] = flt[f].p; fa.ldf[f] = flt[f].ld; }
+    fa.st = sa.stats;
+    fa.chiw = ph->chiw.p; fa.ldchiw = ph->chiw.ld; fa.muw = ph->muw.p; fa.ldmuw = ph->muw.ld;
+    fa.chiu = ph->chiu.p; fa.ldchiu = ph->chiu.ld; fa.muu = ph->muu.p; fa.ldmuu = ph->muu.ld;
+    fa.xiew = ns->xiew.p; fa.ldew = ns->xiew.ld; fa.xieu = ns->xieu.p; fa.ldeu = ns->xieu.ld;
+    fa.xixw = ns->xixw.p; fa.ldxw = ns->xixw.ld; fa.xixu = ns->xixu.p; fa.ldxu = ns->xixu.ld;
+    fa.xisw = ns->xisw.p; fa.ldsw = ns->xisw.ld; fa.xisu = ns->xisu.p; fa.ldsu = ns->xisu.ld;
+    fa.Nz = (int)Nz; fa.Nx = (int)Nx; fa.d = xc_make_dom(Nz + 2, Nx + 2);
+    hipLaunchKernelGGL(k_noise_flux, dim3(fa.d.nb), dim3(NT), 0, ctx->stream, fa);
+    hipLaunchKernelGGL(k_noise_mag, dim3(d.nb), dim3(NT), 0, ctx->stream, ns->xisw.p, ns->xisw.ld, ns->xisu.p,
+                       ns->xisu.ld, ns->xixw.p, ns->xixw.ld, ns->xixu.p, ns->xixu.ld, ns->xiew.p, ns->xiew.ld,
+                       ns->xieu.p, ns->xieu.ld, ns->xis.p, ns->xis.ld, ns->xix.p, ns->xix.ld, ns->xie.p, ns->xie.ld,
+                       (int)Nz, (int)Nx, d);
+    return xc_finish(ctx, "noise!");
+}
+
+}  // extern "C"

@@ -347,6 +347,7 @@ struct DifArgs {
     double* qz; int64_t ld_qz;
     const double* f; int64_t ld_f;
     const double* k; int64_t ld_k;
+    const double* k2; int64_t ld_k2;   // optional: diffusivity is k .* k2 (phsevo.jl:192)
     int Nz, Nx, halo, nbz, nb;
     double invh, invh2;
     int xbc, zbc;
@@ -364,12 +365,19 @@ __global__ __launch_bounds__(NT) void k_diffus(DifArgs a) {
         if constexpr (HIN) return p[(z + a.halo) + (int64_t)(x + a.halo) * ld];
         else return p[map_cell(z, Nz, a.zbc) + (int64_t)map_cell(x, Nx, a.xbc) * ld];
     };
+    auto kat = [&](int z, int x) -> double {
+        if constexpr (HIN) return at(a.k, a.ld_k, z, x);
+        else {
+            const int64_t o = map_cell(z, Nz, a.zbc), c = map_cell(x, Nx, a.xbc);
+            return a.k2 ? a.k[o + c * a.ld_k] * a.k2[o + c * a.ld_k2] : a.k[o + c * a.ld_k];
+        }
+    };
     const double fc = at(a.f, a.ld_f, iz, ix);
     const double fmx = at(a.f, a.ld_f, iz, ix - 1), fpx = at(a.f, a.ld_f, iz, ix + 1);
     const double fmz = at(a.f, a.ld_f, iz - 1, ix), fpz = at(a.f, a.ld_f, iz + 1, ix);
-    const double kc = at(a.k, a.ld_k, iz, ix);
-    const double kmx = at(a.k, a.ld_k, iz, ix - 1), kpx = at(a.k, a.ld_k, iz, ix + 1);
-    const double kmz = at(a.k, a.ld_k, iz - 1, ix), kpz = at(a.k, a.ld_k, iz + 1, ix);
+    const double kc = kat(iz, ix);
+    const double kmx = kat(iz, ix - 1), kpx = kat(iz, ix + 1);
+    const double kmz = kat(iz - 1, ix), kpz = kat(iz + 1, ix);
     // diffus.jl:23-29
     const double flux_xr = (kpx + kc) * (fpx - fc);
     const double flux_xl = (kc + kmx) * (fc - fmx);
@@ -401,7 +409,7 @@ __global__ __launch_bounds__(NT) void k_diffus(DifArgs a) {
 }
 
 int diffus_common(xc_ctx* ctx, bool hin, int halo, xc_mat dff, const xc_mat* qx, const xc_mat* qz,
-                  xc_mat f, xc_mat k, double h, int xbc, int zbc, const char* name) {
+                  xc_mat f, xc_mat k, double h, int xbc, int zbc, const char* name, const xc_mat* k2 = nullptr) {
     XC_CHECK_CTX(ctx);
     if (xbc != XC_PERIODIC && xbc != XC_CLOSED) return xc_fail(ctx, XC_EINVAL_BC, "%s: unsupported xBC = %d", name, xbc);
     if (zbc != XC_PERIODIC && zbc != XC_CLOSED) return xc_fail(ctx, XC_EINVAL_BC, "%s: unsupported zBC = %d", name, zbc);
@@ -416,6 +424,7 @@ int diffus_common(xc_ctx* ctx, bool hin, int halo, xc_mat dff, const xc_mat* qx,
     } else {
         halo = 1;
         if (!xc_shape(f, Nz, Nx) || !xc_shape(k, Nz, Nx)) return xc_fail(ctx, XC_EINVAL_SHAPE, "%s: f and k must be (Nz, Nx)", name);
+        if (k2 && !xc_shape(*k2, Nz, Nx)) return xc_fail(ctx, XC_EINVAL_SHAPE, "%s: k2 must be (Nz, Nx)", name);
     }
     if ((qx != nullptr) != (qz != nullptr)) return xc_fail(ctx, XC_EINVAL_ARG, "%s: pass both qx and qz or neither", name);
     const bool flux = qx && qz;
@@ -427,6 +436,7 @@ int diffus_common(xc_ctx* ctx, bool hin, int halo, xc_mat dff, const xc_mat* qx,
     a.dff = dff.p; a.ld_d = dff.ld;
     if (flux) { a.qx = qx->p; a.ld_qx = qx->ld; a.qz = qz->p; a.ld_qz = qz->ld; }
     a.f = f.p; a.ld_f = f.ld; a.k = k.p; a.ld_k = k.ld;
+    if (k2) { a.k2 = k2->p; a.ld_k2 = k2->ld; }
     a.Nz = (int)Nz; a.Nx = (int)Nx; a.halo = halo;
     a.nbz = (int)((Nz + 63) / 64);
     a.nb = a.nbz * (int)((Nx + 3) / 4);
@@ -478,6 +488,11 @@ __global__ void k_embed(double* fh, int64_t ldh, const double* f, int64_t ldf, i
 }
 
 }  // namespace
+
+int xc_diffus_centered_kprod(xc_ctx* ctx, xc_mat dff, const xc_mat* qx, const xc_mat* qz, xc_mat f, xc_mat k1,
+                             xc_mat k2, double h, int xbc, int zbc) {
+    return diffus_common(ctx, false, 1, dff, qx, qz, f, k1, h, xbc, zbc, "diffus_centered_with_flux!", &k2);
+}
 
 extern "C" {
 

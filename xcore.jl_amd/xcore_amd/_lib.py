@@ -53,6 +53,7 @@ _PROTOS = {
     "xc_create": [ctypes.POINTER(_P), _I, ctypes.c_uint],
     "xc_destroy": [_P],
     "xc_set_stream": [_P, _P],
+    "xc_use_own_stream": [_P],
     "xc_get_stream": [_P],
     "xc_set_flags": [_P, ctypes.c_uint],
     "xc_sync": [_P],
