@@ -177,6 +177,32 @@ __device__ __forceinline__ void stage(const double2* __restrict__ src, double2* 
     }
 }
 
+// Stage of any radix R (remaining prime factors): one output per lane,
+// out[q] = sum_r x[j + r n/R] W_n^{r (t0 + q n/R)}.
+template <bool INV>
+__device__ __forceinline__ void stage_generic(const double2* __restrict__ src, double2* __restrict__ dst, int n,
+                                              int nl, int Ns, int R, const double2* __restrict__ tw) {
+    const int nR = n / R;
+    const int strideT = n / (Ns * R);
+    for (int k = threadIdx.x; k < nl * n; k += NT) {
+        const int l = k / n, rem = k - l * n;
+        const int j = rem / R, q = rem - j * R;
+        const int base = l * n;
+        const int jm = j % Ns;
+        const int e = jm * strideT + q * nR;
+        double2 s = make_double2(0.0, 0.0);
+        int t = 0;
+        for (int r = 0; r < R; ++r) {
+            double2 w = tw[t];
+            if (INV) w.y = -w.y;
+            s = cadd(s, cmul(src[base + j + r * nR], w));
+            t += e;
+            if (t >= n) t -= n * (t / n);
+        }
+        dst[base + (j / Ns) * Ns * R + jm + q * Ns] = s;
+    }
+}
+
 template <bool INV>
 __device__ double2* lds_fft(double2* a, double2* b, int nl, const Plan& p) {
     double2* src = a;
@@ -191,7 +217,8 @@ __device__ double2* lds_fft(double2* a, double2* b, int nl, const Plan& p) {
             case 5: stage<5, INV>(src, dst, p.n, nl, Ns, p.tw); break;
             case 7: stage<7, INV>(src, dst, p.n, nl, Ns, p.tw); break;
             case 11: stage<11, INV>(src, dst, p.n, nl, Ns, p.tw); break;
-            default: stage<13, INV>(src, dst, p.n, nl, Ns, p.tw); break;
+            case 13: stage<13, INV>(src, dst, p.n, nl, Ns, p.tw); break;
+            default: stage_generic<INV>(src, dst, p.n, nl, Ns, R, p.tw); break;
         }
         __syncthreads();
         double2* t = src;
@@ -477,6 +504,11 @@ bool factorize(int n, std::vector<int>& rad) {
             rad.push_back(r);
             n /= r;
         }
+    for (int p = 17; n > 1 && p <= n; p += 2)  // remaining primes: generic stages
+        while (n % p == 0) {
+            rad.push_back(p);
+            n /= p;
+        }
     return n == 1 && (int)rad.size() <= MAXST;
 }
 
@@ -509,7 +541,7 @@ int get_plan(xc_ctx* ctx, int n, double sigma, double h, Plan* out) {
     e.h = h;
     if (n > MAXLINE || !factorize(n, e.rad))
         return xc_fail(ctx, XC_EINVAL_SHAPE,
-                       "noise filter: line length %d must be <= %d with prime factors in {2,3,5,7,11,13}", n, MAXLINE);
+                       "noise filter: line length %d must be <= %d", n, MAXLINE);
     std::vector<double2> tw(n);
     std::vector<double> g(n);
     for (int t = 0; t < n; ++t) {
