@@ -47,6 +47,8 @@ enum { XC_CENTR = 0, XC_UPWD1 = 1, XC_QUICK = 2, XC_FROMM = 3, XC_WENO3 = 4, XC_
 enum { XC_PERIODIC = 0, XC_CLOSED = 1 };
 
 #define XC_CTX_SYNC 1u   /* synchronize the stream before every entry point returns */
+#define XC_CTX_EXACT 2u  /* reference operation order everywhere (bit-identical to a scalar
+                            restatement); default: WENO5 weights with one division + FMA (<= 1e-12) */
 
 /* ---- context (no reference counterpart: replaces the KA backend object,
  *      backend.jl:12-13, as the owner of device state) ------------------- */
@@ -165,6 +167,82 @@ int xc_store_noise(xc_ctx* ctx, xc_noise* ns);
 int xc_normal_fill(xc_ctx* ctx, xc_mat a, uint64_t seed, uint32_t stream, uint32_t draw);
 /* noise.jl:244-270 compute_fft_filter of one field (psi -> out, out != psi) */
 int xc_fft_filter(xc_ctx* ctx, xc_mat psi, xc_mat out, int64_t pad, double h, double sigma);
+
+/* ---- Stokes (src/fluidmech.jl) ------------------------------------------- */
+typedef struct {  /* fluidmech! keywords (fluidmech.jl:71-80) and the iterative-solver controls */
+    int sds, top_cnv, bot_cnv, open_cnv, xbc, zbc;
+    int scheme;       /* par.ADVN, used for the momentum advection on the right-hand side */
+    double dt, a1, a2, a3, b1, b2, b3;
+    double tol;       /* stop at ||RR - LL x||_2 <= tol ||RR||_2 (1e-8) */
+    int maxit;        /* FGMRES iteration cap (all restarts) */
+    int restart;      /* FGMRES restart length */
+    int mg_pre, mg_post, mg_coarse;  /* multigrid smoothing sweeps (pre, post, coarsest level) */
+    double mg_omega;  /* damped-Jacobi weight of the multigrid smoother */
+} xc_fm_opts;
+
+typedef struct {  /* state.jl:131-134 BnchmData */
+    xc_mat src_W, src_U, src_P, W_exact, U_exact, P_exact;
+} xc_bnchm;
+
+typedef struct {
+    int iters;             /* FGMRES iterations used */
+    int converged;         /* 1 when the residual target was met */
+    double rel_resid;      /* final ||RR - LL x|| / ||RR|| */
+    double init_rel_resid; /* the same for the initial guess (previous W, U, P) */
+    int mg_levels;         /* multigrid levels of the velocity preconditioner */
+    int fallback_used;     /* 1 if the learned coarse correction was rejected (plain multigrid used) */
+} xc_solve_stats;
+
+/* fluidmech.jl:71-534 fluidmech!(state, grid, par; phase, bnchm_data, sds, top_cnv, bot_cnv, open_cnv,
+ * xBC, zBC, dt, a1..b3).  Builds the same saddle-point system LL x = RR (MFS update, momentum
+ * advection, BC rows, pins) matrix-free and solves it by right-preconditioned FGMRES with a
+ * block-triangular preconditioner (geometric multigrid on the velocity block, rho/eta Schur
+ * diagonal), starting from the current (W, U, P).  phase == NULL with bnchm != NULL is the MMS path.
+ * Returns XC_ENOCONV (state holds the best iterate) if tol is not met within maxit. */
+int xc_fluidmech(xc_ctx* ctx, xc_fluid* fluid, const xc_phase* phase, const xc_grid* grid, const xc_par* par,
+                 const xc_fm_opts* opts, const xc_bnchm* bnchm, xc_solve_stats* stats);
+/* y = LL x for stacked device vectors [vec(W); vec(U); vec(P)] (no reference counterpart: exposes
+ * the matrix-free operator of fluidmech.jl:145-488 for checking against an assembled matrix) */
+int xc_stokes_apply(xc_ctx* ctx, const xc_fluid* fluid, const xc_grid* grid, const xc_par* par,
+                    const xc_fm_opts* opts, int bnchm, const double* x, double* y);
+/* RR of fluidmech.jl (after its MFS update when phase != NULL) into a stacked device vector */
+int xc_stokes_rhs(xc_ctx* ctx, xc_fluid* fluid, const xc_phase* phase, const xc_grid* grid, const xc_par* par,
+                  const xc_fm_opts* opts, const xc_bnchm* bnchm, double* rr);
+
+/* ---- time loop (src/run.jl, src/diagnose.jl, src/history.jl) ------------ */
+typedef struct {  /* run! keywords (run.jl:167-181) and Picard controls (parameters.jl:150-154) */
+    int tint;          /* TINT: 0 :be1im, 1 :bd2im, 2 :cn2si, 3 :bd2si */
+    int scheme;        /* ADVN */
+    int xbc, zbc;
+    double rtol, atol; /* Picard stopping (par.rtol, par.atol) */
+    int maxit;         /* par.maxit */
+    int fixed_sweeps;  /* > 0: exactly this many Picard sweeps (for reproducible comparisons) */
+    xc_fm_opts fm;     /* sds, top_cnv, bot_cnv, open_cnv + Stokes solver controls (dt, a*, b* are set per step) */
+} xc_step_opts;
+
+typedef struct { double time, dt; int64_t step; } xc_step_state;
+
+typedef struct {
+    int sweeps;                     /* Picard sweeps taken */
+    double resnorm;                 /* last Picard residual */
+    int stokes_iters;               /* FGMRES iterations over all sweeps */
+    double stokes_worst_rel_resid;  /* largest final Stokes relative residual */
+    int stokes_fallbacks;           /* sweeps where the learned correction was rejected */
+} xc_step_stats;
+
+/* run.jl:204-275: one outer step of run! -- time_coefs, store_previous!, store_noise!, then the
+ * Picard loop (>= 3 sweeps of snapshot!, phsevo!, fluidmech!, noise!, update_phase_velocities!,
+ * update!, compute_resnorm), time += dt, dt = update_dt(...).  state->step/time/dt advance. */
+int xc_run_step(xc_ctx* ctx, xc_phase* phase, xc_fluid* fluid, xc_noise* ns, const xc_grid* grid,
+                const xc_par* par, const xc_scales* scales, const xc_step_opts* opts, xc_step_state* state,
+                xc_step_stats* stats);
+/* diagnose.jl:42-49 snapshot!(res, phase, fluid) into the context */
+int xc_snapshot(xc_ctx* ctx, const xc_phase* phase, const xc_fluid* fluid);
+/* diagnose.jl:57-71 compute_resnorm(res, phase, fluid, dt) -> out = {resnorm, res_mass, res_mmnt} */
+int xc_compute_resnorm(xc_ctx* ctx, const xc_phase* phase, const xc_fluid* fluid, double dt, double out[3]);
+/* history.jl:122-151 raw sums: {sum rho, sum M, sum X, sum Gx, top/bottom line sums of qz_advn_X,
+ * qz_advn_M, qz_dffn_X, qz_dffn_M over the interior columns} (12 values, synchronizes) */
+int xc_history_sums(xc_ctx* ctx, const xc_phase* phase, const xc_fluid* fluid, double out[12]);
 
 /* timing.jl:18-36 time_coefs(T, TINT, step); tint 0 :be1im, 1 :bd2im, 2 :cn2si, 3 :bd2si */
 int xc_time_coefs(int tint, int64_t step, double coef_out[6]);

@@ -436,3 +436,50 @@ def fluidmech(fl, ph, g, opts, bnchm=None):
     UPD = scl * spla.splu(LLs).solve(FF)
     set_stokes_vector(fl, SOL - UPD)
     return LL, RR
+
+
+# ---- time loop (run.jl:204-275) with the sparse-LU Stokes solve ---------------
+
+def snapshot(ph, fl):
+    return {k: getattr(fl, k).copy(order="F") for k in ("W", "U", "P", "MFS")} | {"X": ph.X.copy(order="F")}
+
+
+def compute_resnorm(snap, ph, fl, dt):
+    """diagnose.jl:57-71"""
+    ep = 2.220446049250313e-16
+    nrho = np.linalg.norm(fl.rho)
+    rm = np.linalg.norm(ph.X - snap["X"]) / (nrho + ep) + np.linalg.norm(fl.MFS - snap["MFS"]) / (nrho / dt + ep)
+    rp = sum(np.linalg.norm(getattr(fl, k) - snap[k]) / (np.linalg.norm(getattr(fl, k)) + ep) for k in ("W", "U", "P"))
+    return rm + rp, rm, rp
+
+
+def run_step(ph, fl, ns, g, par, sc, state, tint="bd2im", scheme="weno5", xbc="periodic", zbc="closed",
+             fixed_sweeps=0, sds=-1, top_cnv=1, bot_cnv=1, open_cnv=False, seed=0):
+    """One outer step of run! on the oracle; state = dict(time, dt, step, draws).  Noise draws use the same
+    counter-based generator as the HIP path (seed, draw = state['draws'])."""
+    state["step"] += 1
+    c = time_coefs(tint, state["step"])
+    dt = state["dt"]
+    store_previous(fl, ph)
+    store_noise(ns)
+    resnorm, resnorm0, it = 1.0, 1.0, 0
+    while (it < fixed_sweeps) if fixed_sweeps > 0 else (
+            (resnorm / resnorm0 >= par.rtol and resnorm >= par.atol and it < par.maxit) or it < 3):
+        it += 1
+        snap = snapshot(ph, fl)
+        phsevo(ph, fl, g, par, sc, scheme=scheme, xbc=xbc, zbc=zbc, coef=c, dt=dt)
+        opts = fm_opts(par, sds=sds, top_cnv=top_cnv, bot_cnv=bot_cnv, open_cnv=open_cnv, xbc=xbc, zbc=zbc, dt=dt,
+                       a1=c[0], a2=c[1], a3=c[2], b1=c[3], b2=c[4], b3=c[5], scheme=scheme)
+        fluidmech(fl, ph, g, opts)
+        first = it <= 1
+        noise(ns, ph, g, par, sc, dt, first_iter=first, seed=seed, draw=state["draws"])
+        if first:
+            state["draws"] += 1
+        update_phase_velocities(ph, fl, ns, g, par, xbc, zbc)
+        update(ph, fl, g, par, sc, xbc, zbc)
+        resnorm = compute_resnorm(snap, ph, fl, dt)[0]
+        if it == 1 or resnorm > resnorm0:
+            resnorm0 = resnorm + 1e-32
+    state["time"] += dt
+    state["dt"] = update_dt(ph, fl, g, par, dt)
+    return it

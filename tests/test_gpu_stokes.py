@@ -1,0 +1,122 @@
+"""GPU Stokes (fluidmech!) against the oracle's assembled system and sparse-LU solution.
+
+Bars: the matrix-free operator and right-hand side equal the oracle's assembled
+LL x and RR to 1e-12 (normwise); the FGMRES solution has relative residual
+||RR - LL x|| / ||RR|| <= 1e-8 on the oracle's own assembled system, and the
+MMS convergence orders of test_mms.jl (> 1.8) hold on the device path.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from fieldsetup import plausible_state, to_device_state
+from mms import mms_sources
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(g, w):
+    return np.linalg.norm(g - w) / max(np.linalg.norm(w), 1e-300)
+
+
+def _mms(orc, xc, N, L=10.0):
+    m = mms_sources(N, N, L)
+    par = xc.Parameters(D=L, N=N, L=L, etam0=1e3, g0=10.0, rhom0=2700, rhox0=3200, d0=1e-2, Da=0.0, Xi=0.0,
+                        gamma=0.0)
+    g = xc.Grid(N, N, L / N, L, L)
+    fl = orc.fluid_state(N, N)
+    for k in ("eta", "etaco", "rho", "rhow", "rhou", "Drho", "MFS"):
+        getattr(fl, k)[:] = m[k]
+    kw = dict(sds=-1, top_cnv=-1, bot_cnv=-1, open_cnv=False, dt=1e32, a1=1, a2=1, a3=0)
+    return m, par, g, fl, kw
+
+
+def _dev_bnchm(xc, m):
+    return xc.BnchmData(*(xc.to_device(m[k]) for k in ("src_W", "src_U", "src_P", "W_exact", "U_exact", "P_exact")))
+
+
+def test_operator_and_rhs_mms(xc, orc):
+    m, par, g, fl, kw = _mms(orc, xc, 24)
+    LL, RR = orc.fluidmech_assemble(fl, None, g, orc.fm_opts(par, **{k: v for k, v in kw.items()}), bnchm=m)
+    dfl = to_device_state(xc, fl, "fluid")
+    x = np.random.default_rng(0).standard_normal(LL.shape[0])
+    y = xc.stokes_apply(dfl, g, par, torch.as_tensor(x, device="cuda"), bnchm=True, **kw).cpu().numpy()
+    assert _rel(y, LL @ x) <= 1e-12
+    rr = xc.stokes_rhs(dfl, g, par, bnchm_data=_dev_bnchm(xc, m), **kw).cpu().numpy()
+    assert _rel(rr, RR) <= 1e-12
+
+
+@pytest.mark.parametrize("open_cnv", [False, True])
+def test_operator_and_rhs_production(xc, orc, open_cnv):
+    par, g, sc, ph, fl, ns = plausible_state(orc, xc, 20, 24, seed=4)
+    orc.update(ph, fl, g, par, sc)
+    orc.phsevo(ph, fl, g, par, sc, coef=orc.time_coefs("bd2im", 3), dt=sc.dt0)
+    kw = dict(dt=sc.dt0, a1=1.5, a2=2.0, a3=-0.5, b1=1, b2=0, b3=0, open_cnv=open_cnv)
+    dph, dfl = to_device_state(xc, ph, "phase"), to_device_state(xc, fl, "fluid")
+    LL, RR = orc.fluidmech_assemble(fl, ph, g, orc.fm_opts(par, **kw))
+    x = np.random.default_rng(1).standard_normal(LL.shape[0])
+    y = xc.stokes_apply(dfl, g, par, torch.as_tensor(x, device="cuda"), bnchm=False, **kw).cpu().numpy()
+    assert _rel(y, LL @ x) <= 1e-12
+    rr = xc.stokes_rhs(dfl, g, par, phase=dph, **kw).cpu().numpy()
+    assert _rel(rr, RR) <= 1e-12
+    # the MFS update happened identically on both sides
+    assert _rel(xc.to_host(dfl.MFS), fl.MFS) <= 1e-13
+
+
+@pytest.mark.parametrize("N", [32, 64])
+def test_mms_solve_vs_lu(xc, orc, N):
+    m, par, g, fl, kw = _mms(orc, xc, N)
+    dfl = to_device_state(xc, fl, "fluid")
+    stats = xc.fluidmech_(dfl, g, par, bnchm_data=_dev_bnchm(xc, m), tol=1e-8, **kw)
+    print(f"MMS N={N}: {stats}", flush=True)
+    assert stats["converged"] and stats["rel_resid"] <= 1e-8
+    LL, RR = orc.fluidmech(fl, None, g, orc.fm_opts(par, **kw), bnchm=m)
+    sol = np.concatenate([xc.to_host(getattr(dfl, k)).ravel(order="F") for k in ("W", "U", "P")])
+    # residual judged on the oracle's assembled system
+    assert np.linalg.norm(LL @ sol - RR) / np.linalg.norm(RR) <= 1.5e-8
+    lu = orc.stokes_vector(fl)
+    print(f"  |x - x_LU| / |x_LU| = {_rel(sol, lu):.3e}", flush=True)
+    assert _rel(sol, lu) <= 1e-5
+
+
+def test_mms_orders_on_device(xc, orc):
+    errs = {"W": [], "U": [], "P": []}
+    for N in (32, 64, 128):
+        m, par, g, fl, kw = _mms(orc, xc, N)
+        dfl = to_device_state(xc, fl, "fluid")
+        stats = xc.fluidmech_(dfl, g, par, bnchm_data=_dev_bnchm(xc, m), tol=1e-10, **kw)
+        print(f"MMS N={N}: {stats}", flush=True)
+        for k in errs:
+            ex = m[k + "_exact"]
+            errs[k].append(np.linalg.norm(xc.to_host(getattr(dfl, k)) - ex) / np.linalg.norm(ex))
+    for k, e in errs.items():
+        assert math.log2(e[0] / e[1]) > 1.8 and math.log2(e[1] / e[2]) > 1.8, (k, e)
+
+
+@pytest.mark.parametrize("shape", [(32, 40), (64, 64)])
+def test_production_solve(xc, orc, shape):
+    Nz, Nx = shape
+    par, g, sc, ph, fl, ns = plausible_state(orc, xc, Nz, Nx, seed=7)
+    orc.update(ph, fl, g, par, sc)
+    orc.phsevo(ph, fl, g, par, sc, coef=orc.time_coefs("bd2im", 3), dt=sc.dt0)
+    kw = dict(dt=sc.dt0, a1=1.5, a2=2.0, a3=-0.5, b1=1, b2=0, b3=0)
+    dph, dfl = to_device_state(xc, ph, "phase"), to_device_state(xc, fl, "fluid")
+    stats = xc.fluidmech_(dfl, g, par, phase=dph, tol=1e-8, **kw)
+    print(f"production {shape}: {stats}", flush=True)
+    assert stats["converged"]
+    LL, RR = orc.fluidmech(fl, ph, g, orc.fm_opts(par, **kw))
+    sol = np.concatenate([xc.to_host(getattr(dfl, k)).ravel(order="F") for k in ("W", "U", "P")])
+    assert np.linalg.norm(LL @ sol - RR) / np.linalg.norm(RR) <= 1.5e-8
+    print(f"  |x - x_LU| / |x_LU| = {_rel(sol, orc.stokes_vector(fl)):.3e}", flush=True)
+
+
+def test_noconv_reports(xc, orc):
+    m, par, g, fl, kw = _mms(orc, xc, 32)
+    dfl = to_device_state(xc, fl, "fluid")
+    stats = xc.fluidmech_(dfl, g, par, bnchm_data=_dev_bnchm(xc, m), tol=1e-14, maxit=3, raise_on_noconv=False,
+                          **kw)
+    assert not stats["converged"] and stats["iters"] == 3
+    with pytest.raises(xc.NoConvergence):
+        xc.fluidmech_(dfl, g, par, bnchm_data=_dev_bnchm(xc, m), tol=1e-14, maxit=2, **kw)

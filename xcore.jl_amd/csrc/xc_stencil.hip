@@ -63,6 +63,27 @@ __device__ __forceinline__ double weno5_poly(double fmm, double fm, double fc, d
     return (w1 * p1 + w2 * p2 + w3 * p3) / (w1 + w2 + w3);
 }
 
+// Same reconstruction with one division: the weights d_k/(b_k+eps)^2 are scaled
+// by prod_k (b_k+eps)^2 before normalising (mathematically identical; differs
+// from the reference's rounding in the last bits).  FMA contraction allowed.
+__device__ __forceinline__ double weno5_poly_fast(double fmm, double fm, double fc, double fp, double fpp) {
+#pragma clang fp contract(fast)
+    const double e = 2.220446049250313e-16;
+    const double sixth = 1.0 / 6.0, quarter = 0.25, tt = 13.0 / 12.0;
+    const double p1 = sixth * (2.0 * fmm - 7.0 * fm + 11.0 * fc);
+    const double p2 = sixth * (-fm + 5.0 * fc + 2.0 * fp);
+    const double p3 = sixth * (2.0 * fc + 5.0 * fp - fpp);
+    const double s1 = fmm - 2.0 * fm + fc, t1 = fmm - 4.0 * fm + 3.0 * fc;
+    const double s2 = fm - 2.0 * fc + fp, t2 = fm - fp;
+    const double s3 = fc - 2.0 * fp + fpp, t3 = 3.0 * fc - 4.0 * fp + fpp;
+    const double a1 = tt * (s1 * s1) + quarter * (t1 * t1) + e;
+    const double a2 = tt * (s2 * s2) + quarter * (t2 * t2) + e;
+    const double a3 = tt * (s3 * s3) + quarter * (t3 * t3) + e;
+    const double q1 = a1 * a1, q2 = a2 * a2, q3 = a3 * a3;
+    const double w1 = 0.1 * (q2 * q3), w2 = 0.6 * (q1 * q3), w3 = 0.3 * (q1 * q2);
+    return (w1 * p1 + w2 * p2 + w3 * p3) / (w1 + w2 + w3);
+}
+
 // advect.jl:216-229 (eps fixed at 1e-6)
 __device__ __forceinline__ double weno3_poly(double fm, double fc, double fp) {
     const double e = 1e-6;
@@ -89,7 +110,7 @@ __device__ __forceinline__ double tvd_flux(double fm, double fc, double fp, doub
 // (k = 0: the cell after the face, k = -1: the cell before it).  vl/vr are the
 // neighbouring face velocities (TVD only).  Same algebra as the reference's
 // per-face flux kernels, advect.jl:444-695.
-template <int S>
+template <int S, bool FAST>
 __device__ __forceinline__ double face_flux(const double* c, int st, double vl, double v, double vr) {
 #define F(k) c[(k) * st]
     if constexpr (S == XC_CENTR) {
@@ -114,8 +135,13 @@ __device__ __forceinline__ double face_flux(const double* c, int st, double vl, 
             fppos = weno3_poly(F(-2), F(-1), F(0));
             fpneg = weno3_poly(F(1), F(0), F(-1));
         } else {
-            fppos = weno5_poly(F(-3), F(-2), F(-1), F(0), F(1));
-            fpneg = weno5_poly(F(2), F(1), F(0), F(-1), F(-2));
+            if constexpr (FAST) {
+                fppos = weno5_poly_fast(F(-3), F(-2), F(-1), F(0), F(1));
+                fpneg = weno5_poly_fast(F(2), F(1), F(0), F(-1), F(-2));
+            } else {
+                fppos = weno5_poly(F(-3), F(-2), F(-1), F(0), F(1));
+                fpneg = weno5_poly(F(2), F(1), F(0), F(-1), F(-2));
+            }
         }
         return vpos(v) * fppos + vneg(v) * fpneg;
     }
@@ -148,7 +174,7 @@ struct AdvArgs {
 
 // HIN: f is a halo'd buffer (advect!); otherwise f is (Nz, Nx) and the BCs are
 // an index map (advect_centered!).  FLUX: also write qx/qz with padding.
-template <int S, bool HIN, bool FLUX>
+template <int S, bool HIN, bool FLUX, bool FAST>
 __global__ __launch_bounds__(NT) void k_advect(AdvArgs a) {
     constexpr int G = halo_of(S);
     constexpr int PZ = TZ + 2 * G;
@@ -206,7 +232,7 @@ __global__ __launch_bounds__(NT) void k_advect(AdvArgs a) {
             } else {
                 v = a.u[iz + (int64_t)F * a.ld_u];
             }
-            q = face_flux<S>(&fs[(fx + G) * PZ + lz + G], PZ, vl, v, vr);
+            q = face_flux<S, FAST>(&fs[(fx + G) * PZ + lz + G], PZ, vl, v, vr);
             if constexpr (FLUX) {
                 if (fx < TX || F == Nx) {
                     double* col = a.qx + (int64_t)F * a.ld_qx;
@@ -238,7 +264,7 @@ __global__ __launch_bounds__(NT) void k_advect(AdvArgs a) {
             } else {
                 v = a.w[F + (int64_t)ix * a.ld_w];
             }
-            q = face_flux<S>(&fs[(lx + G) * PZ + fz + G], 1, vl, v, vr);
+            q = face_flux<S, FAST>(&fs[(lx + G) * PZ + fz + G], 1, vl, v, vr);
             if constexpr (FLUX) {
                 if (fz < TZ || F == Nz) {
                     a.qz[F + (int64_t)(ix + 1) * a.ld_qz] = q;
@@ -264,22 +290,24 @@ __global__ __launch_bounds__(NT) void k_advect(AdvArgs a) {
 }
 
 template <int S, bool HIN, bool FLUX>
-hipError_t launch_advect_t(const AdvArgs& a0, hipStream_t s) {
+hipError_t launch_advect_t(const AdvArgs& a0, hipStream_t s, bool exact) {
     AdvArgs a = a0;
-    hipLaunchKernelGGL((k_advect<S, HIN, FLUX>), dim3(a.nb), dim3(NT), 0, s, a);
+    // only WENO5 has a distinct fast form; every other scheme is the reference arithmetic
+    if (S == XC_WENO5 && !exact) hipLaunchKernelGGL((k_advect<S, HIN, FLUX, true>), dim3(a.nb), dim3(NT), 0, s, a);
+    else hipLaunchKernelGGL((k_advect<S, HIN, FLUX, false>), dim3(a.nb), dim3(NT), 0, s, a);
     return hipGetLastError();
 }
 
 template <bool HIN, bool FLUX>
-hipError_t launch_advect_s(int scheme, const AdvArgs& a, hipStream_t s) {
+hipError_t launch_advect_s(int scheme, const AdvArgs& a, hipStream_t s, bool exact) {
     switch (scheme) {
-        case XC_CENTR: return launch_advect_t<XC_CENTR, HIN, FLUX>(a, s);
-        case XC_UPWD1: return launch_advect_t<XC_UPWD1, HIN, FLUX>(a, s);
-        case XC_QUICK: return launch_advect_t<XC_QUICK, HIN, FLUX>(a, s);
-        case XC_FROMM: return launch_advect_t<XC_FROMM, HIN, FLUX>(a, s);
-        case XC_WENO3: return launch_advect_t<XC_WENO3, HIN, FLUX>(a, s);
-        case XC_WENO5: return launch_advect_t<XC_WENO5, HIN, FLUX>(a, s);
-        default: return launch_advect_t<XC_TVDIM, HIN, FLUX>(a, s);
+        case XC_CENTR: return launch_advect_t<XC_CENTR, HIN, FLUX>(a, s, exact);
+        case XC_UPWD1: return launch_advect_t<XC_UPWD1, HIN, FLUX>(a, s, exact);
+        case XC_QUICK: return launch_advect_t<XC_QUICK, HIN, FLUX>(a, s, exact);
+        case XC_FROMM: return launch_advect_t<XC_FROMM, HIN, FLUX>(a, s, exact);
+        case XC_WENO3: return launch_advect_t<XC_WENO3, HIN, FLUX>(a, s, exact);
+        case XC_WENO5: return launch_advect_t<XC_WENO5, HIN, FLUX>(a, s, exact);
+        default: return launch_advect_t<XC_TVDIM, HIN, FLUX>(a, s, exact);
     }
 }
 
@@ -330,8 +358,9 @@ int advect_common(xc_ctx* ctx, int scheme, bool hin, xc_mat adv, const xc_mat* q
     a.invh = 1.0 / h;
     a.xbc = xbc; a.zbc = zbc;
     hipError_t e;
-    if (hin) e = flux ? launch_advect_s<true, true>(scheme, a, ctx->stream) : launch_advect_s<true, false>(scheme, a, ctx->stream);
-    else     e = flux ? launch_advect_s<false, true>(scheme, a, ctx->stream) : launch_advect_s<false, false>(scheme, a, ctx->stream);
+    const bool ex = (ctx->flags & XC_CTX_EXACT) != 0;
+    if (hin) e = flux ? launch_advect_s<true, true>(scheme, a, ctx->stream, ex) : launch_advect_s<true, false>(scheme, a, ctx->stream, ex);
+    else     e = flux ? launch_advect_s<false, true>(scheme, a, ctx->stream, ex) : launch_advect_s<false, false>(scheme, a, ctx->stream, ex);
     if (e != hipSuccess) return xc_fail(ctx, XC_EHIP, "%s: launch: %s", name, hipGetErrorString(e));
     return xc_finish(ctx, name);
 }

@@ -1,0 +1,971 @@
+// xc_stokes.hip -- the Stokes saddle-point solve of src/fluidmech.jl on gfx950.
+//
+// The reference assembles LL = [KV GG; DM KP] as a sparse matrix and factors it
+// with UMFPACK every Picard sweep (fluidmech.jl:329-522).  Here LL is never
+// stored: one kernel evaluates any row of LL x from the coefficient fields
+// (eta, etaco, rho, rhow, rhou) with exactly the reference's stencil, BC rows
+// and pins, and the system is solved by right-preconditioned FGMRES.
+//
+// Preconditioner (applied to a residual (r_v, r_p)):
+//   z_p = S^-1 r_p   with S ~ KP - DM A^-1 GG ~ +rho/eta on interior cells (the
+//                    reference's own scaling uses sqrt(eta/rho) there,
+//                    fluidmech.jl:497-500); ghost rows solved exactly from KP;
+//   z_v = A^-1 (r_v - GG z_p) by one geometric-multigrid V-cycle on the
+//                    velocity block A = KV: coarse operators rediscretised from
+//                    geometric-mean viscosities, staggered full-weighting
+//                    restriction, bilinear prolongation, damped-Jacobi smoothing.
+// All levels, vectors and the Krylov basis live in the context; nothing is
+// allocated after the first call at a given grid size.
+#include <math.h>
+#include <string.h>
+
+#include <vector>
+
+#include "xc_internal.h"
+
+namespace {
+
+constexpr int NT = 256;
+
+struct Flags {
+    int bnchm, sds, top_cnv, bot_cnv, open_cnv;
+    double a1, gamma, dt, g0;
+    int64_t nu0;          // MMS U-pin velocity index (fine level) or -1
+    int pin_kind;         // 1: single P pin (pr, pc); 2: open_cnv pins on P row Nz
+    int pr, pc;
+};
+
+struct LevD {  // coefficients of one level, dense column-major (ld = rows)
+    int Nz, Nx;
+    double h, invh, invh2;
+    const double *eta, *etaco, *rho, *rhow, *rhou;
+    __device__ __forceinline__ int64_t nW() const { return (int64_t)(Nz + 1) * (Nx + 2); }
+    __device__ __forceinline__ int64_t nU() const { return (int64_t)(Nz + 2) * (Nx + 1); }
+    __device__ __forceinline__ int64_t iW(int r, int c) const { return r + (int64_t)c * (Nz + 1); }
+    __device__ __forceinline__ int64_t iU(int r, int c) const { return nW() + r + (int64_t)c * (Nz + 2); }
+    __device__ __forceinline__ int64_t iP(int r, int c) const { return nW() + nU() + r + (int64_t)c * (Nz + 2); }
+    __device__ __forceinline__ double ETA(int r, int c) const { return eta[r + (int64_t)c * Nz]; }
+    __device__ __forceinline__ double ETACO(int r, int c) const { return etaco[r + (int64_t)c * (Nz + 1)]; }
+    __device__ __forceinline__ double RHO(int r, int c) const { return rho[r + (int64_t)c * Nz]; }
+    __device__ __forceinline__ double RHOW(int r, int c) const { return rhow[r + (int64_t)c * (Nz + 1)]; }
+    __device__ __forceinline__ double RHOU(int r, int c) const { return rhou[r + (int64_t)c * Nz]; }
+    __device__ __forceinline__ int icx(int k) const { return k == 0 ? Nx - 1 : (k == Nx + 1 ? 0 : k - 1); }
+    __device__ __forceinline__ int ifx(int k) const { return k == 0 ? Nx - 1 : (k == Nx + 2 ? 1 : k - 1); }
+};
+
+// ---- rows of the velocity block KV (fluidmech.jl:145-324).  Return sum_j KV_ij x_j, diag in *dg.
+__device__ double w_row(const LevD& L, const Flags& F, const double* x, int r, int c, double* dg, bool* bc) {
+    const int Nz = L.Nz, Nx = L.Nx;
+    *bc = true;
+    if (c == 0) { *dg = 1.0; return x[L.iW(r, 0)] + F.sds * x[L.iW(r, Nx)]; }
+    if (c == Nx + 1) { *dg = 1.0; return x[L.iW(r, Nx + 1)] + F.sds * x[L.iW(r, 1)]; }
+    if (r == 0) { *dg = 1.0; return x[L.iW(0, c)]; }
+    if (r == Nz) {
+        const double oc = F.open_cnv ? 1.0 : 0.0;
+        const double d = L.RHOW(Nz, c - 1) * L.invh;
+        *dg = d;
+        return d * x[L.iW(Nz, c)] - oc * L.RHOW(Nz - 1, c - 1) * L.invh * x[L.iW(Nz - 1, c)] +
+               oc * L.RHOU(Nz - 1, c) * L.invh * x[L.iU(Nz, c)] - oc * L.RHOU(Nz - 1, c - 1) * L.invh * x[L.iU(Nz, c - 1)];
+    }
+    *bc = false;
+    const double i2 = L.invh2;
+    const double C1 = L.ETACO(r, c - 1), C2 = L.ETACO(r, c), P1 = L.ETA(r - 1, c - 1), P2 = L.ETA(r, c - 1);
+    double d = F.a1 * L.RHOW(r, c - 1) / F.dt + ((2.0 / 3.0) * (P1 + P2) * i2 + 0.5 * (C1 + C2) * i2);
+    if (!F.bnchm) d += (L.RHO(r, c - 1) - L.RHO(r - 1, c - 1)) * L.invh * F.g0 * F.dt;
+    *dg = d;
+    return d * x[L.iW(r, c)] - (2.0 / 3.0) * P1 * i2 * x[L.iW(r - 1, c)] - (2.0 / 3.0) * P2 * i2 * x[L.iW(r + 1, c)] -
+           0.5 * C1 * i2 * x[L.iW(r, c - 1)] - 0.5 * C2 * i2 * x[L.iW(r, c + 1)] -
+           (0.5 * C1 - P1 / 3.0) * i2 * x[L.iU(r, c - 1)] + (0.5 * C1 - P2 / 3.0) * i2 * x[L.iU(r + 1, c - 1)] +
+           (0.5 * C2 - P1 / 3.0) * i2 * x[L.iU(r, c)] - (0.5 * C2 - P2 / 3.0) * i2 * x[L.iU(r + 1, c)];
+}
+
+__device__ double u_row(const LevD& L, const Flags& F, const double* x, int r, int c, double* dg, bool* bc) {
+    const int Nz = L.Nz;
+    *bc = true;
+    if (r == 0) { *dg = 1.0; return x[L.iU(0, c)] + F.top_cnv * x[L.iU(1, c)]; }
+    if (r == Nz + 1) { *dg = 1.0; return x[L.iU(Nz + 1, c)] + F.bot_cnv * x[L.iU(Nz, c)]; }
+    *bc = false;
+    const double i2 = L.invh2;
+    const double C1 = L.ETACO(r - 1, c), C2 = L.ETACO(r, c);
+    const double P1 = L.ETA(r - 1, L.icx(c)), P2 = L.ETA(r - 1, L.icx(c + 1));
+    double d = (F.a1 + F.gamma) * L.RHOU(r - 1, c) / F.dt + ((2.0 / 3.0) * (P1 + P2) * i2 + 0.5 * (C1 + C2) * i2);
+    if (!F.bnchm) d += (L.RHO(r - 1, L.icx(c + 1)) - L.RHO(r - 1, L.icx(c))) * L.invh * F.g0 * F.dt;
+    *dg = d;
+    return d * x[L.iU(r, c)] - (2.0 / 3.0) * P1 * i2 * x[L.iU(r, L.ifx(c))] -
+           (2.0 / 3.0) * P2 * i2 * x[L.iU(r, L.ifx(c + 2))] - 0.5 * C1 * i2 * x[L.iU(r - 1, c)] -
+           0.5 * C2 * i2 * x[L.iU(r + 1, c)] - (0.5 * C1 - P1 / 3.0) * i2 * x[L.iW(r - 1, c)] +
+           (0.5 * C1 - P2 / 3.0) * i2 * x[L.iW(r - 1, c + 1)] + (0.5 * C2 - P1 / 3.0) * i2 * x[L.iW(r, c)] -
+           (0.5 * C2 - P2 / 3.0) * i2 * x[L.iW(r, c + 1)];
+}
+
+// velocity row k of KV (index into the W|U vector): value and diagonal
+__device__ __forceinline__ double vel_row(const LevD& L, const Flags& F, const double* x, int64_t k, double* dg,
+                                          bool* bc, bool fine) {
+    if (fine && k == F.nu0) { *dg = 1.0; *bc = true; return x[k]; }
+    const int64_t nW = L.nW();
+    if (k < nW) {
+        const int r = (int)(k % (L.Nz + 1)), c = (int)(k / (L.Nz + 1));
+        return w_row(L, F, x, r, c, dg, bc);
+    }
+    const int64_t q = k - nW;
+    const int r = (int)(q % (L.Nz + 2)), c = (int)(q / (L.Nz + 2));
+    return u_row(L, F, x, r, c, dg, bc);
+}
+
+__device__ __forceinline__ bool is_pin(const Flags& F, int r, int c, int Nz) {
+    if (F.pin_kind == 1) return r == F.pr && c == F.pc;
+    if (F.pin_kind == 2) return r == Nz;
+    return false;
+}
+
+// full row k of LL x (fine level), fluidmech.jl:145-488
+__device__ double ll_row(const LevD& L, const Flags& F, const double* x, int64_t k) {
+    const int64_t nW = L.nW(), nU = L.nU();
+    const int Nz = L.Nz, Nx = L.Nx;
+    double dg;
+    bool bc;
+    if (k < nW + nU) {
+        double s = vel_row(L, F, x, k, &dg, &bc, true);
+        if (!bc) {  // GG (fluidmech.jl:343-356)
+            if (k < nW) {
+                const int r = (int)(k % (Nz + 1)), c = (int)(k / (Nz + 1));
+                s += -L.invh * x[L.iP(r, c)] + L.invh * x[L.iP(r + 1, c)];
+            } else {
+                const int64_t q = k - nW;
+                const int r = (int)(q % (Nz + 2)), c = (int)(q / (Nz + 2));
+                s += -L.invh * x[L.iP(r, c)] + L.invh * x[L.iP(r, c + 1)];
+            }
+        }
+        return s;
+    }
+    const int64_t q = k - nW - nU;
+    const int r = (int)(q % (Nz + 2)), c = (int)(q / (Nz + 2));
+    const bool pin = is_pin(F, r, c, Nz);
+    const bool interior = r >= 1 && r <= Nz && c >= 1 && c <= Nx;
+    if (pin && (F.bnchm || !interior)) return x[k];
+    if (r == 0) return x[L.iP(0, c)] - x[L.iP(1, c)];
+    if (r == Nz + 1) return x[L.iP(Nz + 1, c)] - x[L.iP(Nz, c)];
+    if (c == 0) return x[L.iP(r, 0)] - x[L.iP(r, Nx)];
+    if (c == Nx + 1) return x[L.iP(r, Nx + 1)] - x[L.iP(r, 1)];
+    double s = -L.RHOU(r - 1, c - 1) * L.invh * x[L.iU(r, c - 1)] + L.RHOU(r - 1, c) * L.invh * x[L.iU(r, c)] -
+               L.RHOW(r - 1, c - 1) * L.invh * x[L.iW(r - 1, c)] + L.RHOW(r, c - 1) * L.invh * x[L.iW(r, c)];
+    if (pin) s += x[k];
+    return s;
+}
+
+__global__ __launch_bounds__(NT) void k_ll_apply(LevD L, Flags F, const double* __restrict__ x, double* __restrict__ y,
+                                                 int64_t n) {
+    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (k < n) y[k] = ll_row(L, F, x, k);
+}
+
+// r = b - LL x
+__global__ __launch_bounds__(NT) void k_ll_resid(LevD L, Flags F, const double* __restrict__ x,
+                                                 const double* __restrict__ b, double* __restrict__ r, int64_t n) {
+    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (k < n) r[k] = b[k] - ll_row(L, F, x, k);
+}
+
+// damped Jacobi on the velocity block: xo = x + w (b - A x) / diag (BC rows with weight 1)
+__global__ __launch_bounds__(NT) void k_jacobi(LevD L, Flags F, const double* __restrict__ x,
+                                               const double* __restrict__ b, double* __restrict__ xo, double omega,
+                                               int fine, int zero_guess) {
+    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t n = L.nW() + L.nU();
+    if (k >= n) return;
+    double dg;
+    bool bc;
+    if (zero_guess) {  // x == 0: A x == 0, diag from the row formula evaluated on x (all zeros)
+        vel_row(L, F, x, k, &dg, &bc, fine != 0);
+        xo[k] = (bc ? 1.0 : omega) * b[k] / dg;
+        return;
+    }
+    const double ax = vel_row(L, F, x, k, &dg, &bc, fine != 0);
+    xo[k] = x[k] + (bc ? 1.0 : omega) * (b[k] - ax) / dg;
+}
+
+__global__ __launch_bounds__(NT) void k_vel_resid(LevD L, Flags F, const double* __restrict__ x,
+                                                  const double* __restrict__ b, double* __restrict__ r, int fine) {
+    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (k >= L.nW() + L.nU()) return;
+    double dg;
+    bool bc;
+    r[k] = b[k] - vel_row(L, F, x, k, &dg, &bc, fine != 0);
+}
+
+// staggered full-weighting restriction of a velocity residual rf (level f) into bc (level c)
+__global__ __launch_bounds__(NT) void k_restrict(LevD Lf, LevD Lc, const double* __restrict__ rf,
+                                                 double* __restrict__ bcv) {
+    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t nWc = Lc.nW(), n = nWc + Lc.nU();
+    if (k >= n) return;
+    double v = 0.0;
+    if (k < nWc) {
+        const int J = (int)(k % (Lc.Nz + 1)), I = (int)(k / (Lc.Nz + 1));
+        if (J >= 1 && J <= Lc.Nz - 1 && I >= 1 && I <= Lc.Nx) {
+            const int r = 2 * J, c0 = 2 * I - 1, c1 = 2 * I;
+            v = (2.0 * rf[Lf.iW(r, c0)] + 2.0 * rf[Lf.iW(r, c1)] + rf[Lf.iW(r - 1, c0)] + rf[Lf.iW(r - 1, c1)] +
+                 rf[Lf.iW(r + 1, c0)] + rf[Lf.iW(r + 1, c1)]) * 0.125;
+        }
+    } else {
+        const int64_t q = k - nWc;
+        const int J = (int)(q % (Lc.Nz + 2)), I = (int)(q / (Lc.Nz + 2));
+        if (J >= 1 && J <= Lc.Nz) {
+            const int r0 = 2 * J - 1, r1 = 2 * J, f = 2 * I;
+            const int fl = f == 0 ? Lf.Nx - 1 : f - 1;
+            const int fr = f == Lf.Nx ? 1 : f + 1;
+            v = (2.0 * rf[Lf.iU(r0, f)] + 2.0 * rf[Lf.iU(r1, f)] + rf[Lf.iU(r0, fl)] + rf[Lf.iU(r1, fl)] +
+                 rf[Lf.iU(r0, fr)] + rf[Lf.iU(r1, fr)]) * 0.125;
+        }
+    }
+    bcv[k] = v;
+}
+
+// bilinear prolongation of a coarse correction ec, added to the fine iterate xf (interior rows)
+__global__ __launch_bounds__(NT) void k_prolong(LevD Lf, LevD Lc, Flags F, const double* __restrict__ ec,
+                                                double* __restrict__ xf) {
+    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t nWf = Lf.nW(), n = nWf + Lf.nU();
+    if (k >= n) return;
+    if (k < nWf) {
+        const int r = (int)(k % (Lf.Nz + 1)), c = (int)(k / (Lf.Nz + 1));
+        if (r < 1 || r > Lf.Nz - 1 || c < 1 || c > Lf.Nx) return;
+        // coarse W column value with the periodic/mirror ghost map of the BC rows
+        auto E = [&](int J, int I) -> double {
+            if (J <= 0 || J >= Lc.Nz) return 0.0;
+            if (I == 0) return -F.sds * ec[Lc.iW(J, Lc.Nx)];
+            if (I == Lc.Nx + 1) return -F.sds * ec[Lc.iW(J, 1)];
+            return ec[Lc.iW(J, I)];
+        };
+        auto EX = [&](int J) -> double {  // x-interpolation at fine column c, coarse row J
+            const int I = (c + 1) / 2;
+            const int In = (c & 1) ? I - 1 : I + 1;
+            return 0.75 * E(J, I) + 0.25 * E(J, In);
+        };
+        const int J = r / 2;
+        const double v = (r & 1) ? 0.5 * (EX(J) + EX(J + 1)) : EX(J);
+        xf[k] += v;
+    } else {
+        const int64_t q = k - nWf;
+        const int r = (int)(q % (Lf.Nz + 2)), c = (int)(q / (Lf.Nz + 2));
+        if (r < 1 || r > Lf.Nz) return;
+        auto E = [&](int J, int I) -> double {
+            if (J == 0) return -F.top_cnv * ec[Lc.iU(1, I)];
+            if (J == Lc.Nz + 1) return -F.bot_cnv * ec[Lc.iU(Lc.Nz, I)];
+            return ec[Lc.iU(J, I)];
+        };
+        auto EZ = [&](int I) -> double {  // z-interpolation at fine row r, coarse face I
+            const int J = (r + 1) / 2;
+            const int Jn = (r & 1) ? J - 1 : J + 1;
+            return 0.75 * E(J, I) + 0.25 * E(Jn, I);
+        };
+        const int I = c / 2;
+        const double v = (c & 1) ? 0.5 * (EZ(I) + EZ(I + 1)) : EZ(I);
+        xf[k] += v;
+    }
+}
+
+// coarse coefficients from fine ones
+__global__ __launch_bounds__(NT) void k_coarsen(LevD Lf, double* eta, double* rho, double* rhow, double* rhou,
+                                                int Nzc, int Nxc) {
+    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t nmax = (int64_t)(Nzc + 1) * (Nxc + 1);
+    if (k >= nmax) return;
+    const int J = (int)(k % (Nzc + 1)), I = (int)(k / (Nzc + 1));
+    if (J < Nzc && I < Nxc) {
+        const double e00 = Lf.ETA(2 * J, 2 * I), e10 = Lf.ETA(2 * J + 1, 2 * I);
+        const double e01 = Lf.ETA(2 * J, 2 * I + 1), e11 = Lf.ETA(2 * J + 1, 2 * I + 1);
+        eta[J + (int64_t)I * Nzc] = sqrt(sqrt(e00 * e10) * sqrt(e01 * e11));
+        rho[J + (int64_t)I * Nzc] = 0.25 * (Lf.RHO(2 * J, 2 * I) + Lf.RHO(2 * J + 1, 2 * I) + Lf.RHO(2 * J, 2 * I + 1) +
+                                            Lf.RHO(2 * J + 1, 2 * I + 1));
+    }
+    if (I < Nxc) rhow[J + (int64_t)I * (Nzc + 1)] = 0.5 * (Lf.RHOW(2 * J, 2 * I) + Lf.RHOW(2 * J, 2 * I + 1));
+    if (J < Nzc) rhou[J + (int64_t)I * Nzc] = 0.5 * (Lf.RHOU(2 * J, 2 * I) + Lf.RHOU(2 * J + 1, 2 * I));
+}
+
+// corner viscosity of a coarse level: geometric mean, periodic x / closed z (update.jl:411-426)
+__global__ __launch_bounds__(NT) void k_corner_eta(const double* eta, double* etaco, int Nz, int Nx) {
+    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (k >= (int64_t)(Nz + 1) * (Nx + 1)) return;
+    const int j = (int)(k % (Nz + 1)), i = (int)(k / (Nz + 1));
+    const int z0 = j - 1 < 0 ? 0 : j - 1, z1 = j >= Nz ? Nz - 1 : j;
+    const int x0 = i - 1 < 0 ? Nx - 1 : i - 1, x1 = i >= Nx ? 0 : i;
+    const double p = eta[z0 + (int64_t)x0 * Nz] * eta[z1 + (int64_t)x0 * Nz] * eta[z0 + (int64_t)x1 * Nz] *
+                     eta[z1 + (int64_t)x1 * Nz];
+    etaco[k] = sqrt(sqrt(p));
+}
+
+// Schur diagonal on the P block + GG coupling into the velocity residual:
+//   zp = S^-1 rp (interior: eta/rho; pins: see below), then rv' = rv - GG zp
+__global__ __launch_bounds__(NT) void k_schur_int(LevD L, Flags F, const double* __restrict__ rP,
+                                                  double* __restrict__ zP) {
+    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int Nz = L.Nz, Nx = L.Nx;
+    if (k >= (int64_t)(Nz + 2) * (Nx + 2)) return;
+    const int r = (int)(k % (Nz + 2)), c = (int)(k / (Nz + 2));
+    if (r < 1 || r > Nz || c < 1 || c > Nx) return;
+    // S = KP - DM A^-1 GG ~ +rho/eta (DM ~ rho div, GG ~ grad, A ~ -eta lap)
+    const double s = L.RHO(r - 1, c - 1) / L.ETA(r - 1, c - 1);
+    if (is_pin(F, r, c, Nz)) zP[k] = F.bnchm ? rP[k] : rP[k] / (1.0 + s);
+    else zP[k] = rP[k] / s;
+}
+// ghost rows of KP solved exactly given the interior values (fluidmech.jl:405-417)
+__global__ __launch_bounds__(NT) void k_schur_ghost(LevD L, Flags F, const double* __restrict__ rP,
+                                                    double* __restrict__ zP) {
+    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int Nz = L.Nz, Nx = L.Nx;
+    if (k >= (int64_t)(Nz + 2) * (Nx + 2)) return;
+    const int r = (int)(k % (Nz + 2)), c = (int)(k / (Nz + 2));
+    auto P = [&](int rr, int cc) { return zP[rr + (int64_t)cc * (Nz + 2)]; };
+    const bool pin = is_pin(F, r, c, Nz);
+    double v;
+    if (r >= 1 && r <= Nz && c >= 1 && c <= Nx) return;
+    if (pin) { zP[k] = rP[k]; return; }
+    if (c == 0 && r >= 1 && r <= Nz) v = rP[k] + P(r, Nx);
+    else if (c == Nx + 1 && r >= 1 && r <= Nz) v = rP[k] + P(r, 1);
+    else {
+        // rows 0 / Nz+1 reference row 1 / Nz at the same column (which may itself be a ghost column)
+        const int rn = r == 0 ? 1 : Nz;
+        double inner;
+        if (c == 0) inner = rP[rn + (int64_t)0 * (Nz + 2)] + P(rn, Nx);
+        else if (c == Nx + 1) inner = rP[rn + (int64_t)(Nx + 1) * (Nz + 2)] + P(rn, 1);
+        else inner = P(rn, c);
+        v = rP[k] + inner;
+    }
+    zP[k] = v;
+}
+__global__ __launch_bounds__(NT) void k_sub_grad(LevD L, Flags F, const double* __restrict__ rV,
+                                                 const double* __restrict__ zP, double* __restrict__ tV) {
+    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t nW = L.nW(), n = nW + L.nU();
+    if (k >= n) return;
+    const int Nz = L.Nz, Nx = L.Nx;
+    double g = 0.0;
+    if (k != F.nu0) {
+        if (k < nW) {
+            const int r = (int)(k % (Nz + 1)), c = (int)(k / (Nz + 1));
+            if (r >= 1 && r <= Nz - 1 && c >= 1 && c <= Nx)
+                g = -L.invh * zP[r + (int64_t)c * (Nz + 2)] + L.invh * zP[r + 1 + (int64_t)c * (Nz + 2)];
+        } else {
+            const int64_t q = k - nW;
+            const int r = (int)(q % (Nz + 2)), c = (int)(q / (Nz + 2));
+            if (r >= 1 && r <= Nz)
+                g = -L.invh * zP[r + (int64_t)c * (Nz + 2)] + L.invh * zP[r + (int64_t)(c + 1) * (Nz + 2)];
+        }
+    }
+    tV[k] = rV[k] - g;
+}
+
+// ---- vector kernels for FGMRES ------------------------------------------
+// partial dot products of w against K basis vectors V[0..K) (stride n) and, as entry K, w.w
+constexpr int DOT_BLOCKS = 1024;
+__global__ __launch_bounds__(NT) void k_multidot(const double* __restrict__ V, int64_t n, int K,
+                                                 const double* __restrict__ w, double* __restrict__ part) {
+    __shared__ double red[NT];
+    for (int q = 0; q <= K; ++q) {
+        const double* v = q < K ? V + (int64_t)q * n : w;
+        double acc = 0.0;
+        for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) acc += v[i] * w[i];
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int st = NT / 2; st > 0; st >>= 1) {
+            if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) part[(int64_t)q * gridDim.x + blockIdx.x] = red[0];
+        __syncthreads();
+    }
+}
+__global__ __launch_bounds__(NT) void k_dot_final(const double* __restrict__ part, int nb, int K1, double* out) {
+    __shared__ double red[NT];
+    for (int q = 0; q < K1; ++q) {
+        double acc = 0.0;
+        for (int b = threadIdx.x; b < nb; b += NT) acc += part[(int64_t)q * nb + b];
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int st = NT / 2; st > 0; st >>= 1) {
+            if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[q] = red[0];
+        __syncthreads();
+    }
+}
+// w -= sum_q c[q] V[q]
+__global__ __launch_bounds__(NT) void k_orth(const double* __restrict__ V, int64_t n, int K, const double* __restrict__ c,
+                                             double* __restrict__ w) {
+    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (i >= n) return;
+    double s = w[i];
+    for (int q = 0; q < K; ++q) s -= c[q] * V[(int64_t)q * n + i];
+    w[i] = s;
+}
+__global__ __launch_bounds__(NT) void k_scale(const double* __restrict__ a, double s, double* __restrict__ b, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (i < n) b[i] = a[i] * s;
+}
+// x += sum_q y[q] Z[q]
+__global__ __launch_bounds__(NT) void k_update_x(const double* __restrict__ Z, int64_t n, int K,
+                                                 const double* __restrict__ y, double* __restrict__ x) {
+    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (i >= n) return;
+    double s = x[i];
+    for (int q = 0; q < K; ++q) s += y[q] * Z[(int64_t)q * n + i];
+    x[i] = s;
+}
+__global__ __launch_bounds__(NT) void k_zero(double* a, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (i < n) a[i] = 0.0;
+}
+__global__ __launch_bounds__(NT) void k_copyv(const double* __restrict__ a, double* __restrict__ b, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+// ---- state <-> stacked vector
+__global__ __launch_bounds__(NT) void k_gather(xc_mat W, xc_mat U, xc_mat P, double* x, int Nz, int Nx) {
+    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t nW = (int64_t)(Nz + 1) * (Nx + 2), nU = (int64_t)(Nz + 2) * (Nx + 1), nP = (int64_t)(Nz + 2) * (Nx + 2);
+    if (k < nW) x[k] = W.p[(k % (Nz + 1)) + (k / (Nz + 1)) * W.ld];
+    else if (k < nW + nU) { const int64_t q = k - nW; x[k] = U.p[(q % (Nz + 2)) + (q / (Nz + 2)) * U.ld]; }
+    else if (k < nW + nU + nP) { const int64_t q = k - nW - nU; x[k] = P.p[(q % (Nz + 2)) + (q / (Nz + 2)) * P.ld]; }
+}
+__global__ __launch_bounds__(NT) void k_scatter(const double* x, xc_mat W, xc_mat U, xc_mat P, int Nz, int Nx) {
+    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t nW = (int64_t)(Nz + 1) * (Nx + 2), nU = (int64_t)(Nz + 2) * (Nx + 1), nP = (int64_t)(Nz + 2) * (Nx + 2);
+    if (k < nW) W.p[(k % (Nz + 1)) + (k / (Nz + 1)) * W.ld] = x[k];
+    else if (k < nW + nU) { const int64_t q = k - nW; U.p[(q % (Nz + 2)) + (q / (Nz + 2)) * U.ld] = x[k]; }
+    else if (k < nW + nU + nP) { const int64_t q = k - nW - nU; P.p[(q % (Nz + 2)) + (q / (Nz + 2)) * P.ld] = x[k]; }
+}
+// dense copy of a strided matrix
+__global__ __launch_bounds__(NT) void k_pack(xc_mat a, double* out) {
+    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (k >= a.rows * a.cols) return;
+    out[k] = a.p[(k % a.rows) + (k / a.rows) * a.ld];
+}
+
+// ---- right-hand side (fluidmech.jl:111-127, 188-190, 236-246, 310-323, 450-476)
+struct RhsArgs {
+    xc_mat rho, rhoo, rhooo, drhodt, drhodto, drhodtoo, MFS, advn_rho;  // MFS update
+    double a1, a2, a3, b1, b2, b3, dt, alpha;
+    int do_mfs;
+};
+__global__ __launch_bounds__(NT) void k_mfs(RhsArgs a, int Nz, int Nx, double* part) {
+    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    double v = 0.0;
+    if (k < (int64_t)Nz * Nx) {
+        const int j = (int)(k % Nz), i = (int)(k / Nz);
+        auto at = [&](const xc_mat& m) -> double& { return m.p[j + (int64_t)i * m.ld]; };
+        if (a.do_mfs) {
+#pragma clang fp contract(off)
+            const double drhodt = -at(a.advn_rho);
+            at(a.drhodt) = drhodt;
+            const double res = (a.a1 * at(a.rho) - a.a2 * at(a.rhoo) - a.a3 * at(a.rhooo)) / a.dt -
+                               (a.b1 * drhodt + a.b2 * at(a.drhodto) + a.b3 * at(a.drhodtoo));
+            at(a.MFS) = at(a.MFS) - a.alpha * res;
+        }
+        v = at(a.MFS);
+    }
+    __shared__ double red[NT];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int st = NT / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+__global__ __launch_bounds__(NT) void k_sum_final(const double* part, int nb, double scale, double* out) {
+    __shared__ double red[NT];
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nb; b += NT) acc += part[b];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = NT / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0] * scale;
+}
+
+// momentum-advection inputs (fluidmech.jl:236-238, 310-312)
+__global__ __launch_bounds__(NT) void k_mom_prep(xc_mat W, xc_mat U, xc_mat rhow, xc_mat rhou, double* fz, double* uz,
+                                                 double* wz, double* fx, double* ux, double* wx, int Nz, int Nx) {
+    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    auto Wv = [&](int r, int c) { return W.p[r + (int64_t)c * W.ld]; };
+    auto Uv = [&](int r, int c) { return U.p[r + (int64_t)c * U.ld]; };
+    auto ifx = [&](int kk) { return kk == 0 ? Nx - 1 : (kk == Nx + 2 ? 1 : kk - 1); };
+    // fz (Nz-1, Nx), uz (Nz-1, Nx+1), wz (Nz, Nx), fx (Nz, Nx+1), ux (Nz, Nx+2), wx (Nz+1, Nx+1)
+    if (k < (int64_t)(Nz - 1) * Nx) {
+        const int j = (int)(k % (Nz - 1)), i = (int)(k / (Nz - 1));
+        fz[k] = rhow.p[(j + 1) + (int64_t)i * rhow.ld] * Wv(j + 1, i + 1);
+    }
+    if (k < (int64_t)(Nz - 1) * (Nx + 1)) {
+        const int j = (int)(k % (Nz - 1)), i = (int)(k / (Nz - 1));
+        uz[k] = (Uv(j + 1, i) + Uv(j + 2, i)) / 2.0;
+    }
+    if (k < (int64_t)Nz * Nx) {
+        const int j = (int)(k % Nz), i = (int)(k / Nz);
+        wz[k] = (Wv(j, i + 1) + Wv(j + 1, i + 1)) / 2.0;
+    }
+    if (k < (int64_t)Nz * (Nx + 1)) {
+        const int j = (int)(k % Nz), i = (int)(k / Nz);
+        fx[k] = rhou.p[j + (int64_t)i * rhou.ld] * Uv(j + 1, i);
+    }
+    if (k < (int64_t)Nz * (Nx + 2)) {
+        const int j = (int)(k % Nz), i = (int)(k / Nz);
+        ux[k] = (Uv(j + 1, ifx(i)) + Uv(j + 1, ifx(i + 1))) / 2.0;
+    }
+    if (k < (int64_t)(Nz + 1) * (Nx + 1)) {
+        const int j = (int)(k % (Nz + 1)), i = (int)(k / (Nz + 1));
+        wx[k] = (Wv(j, i) + Wv(j, i + 1)) / 2.0;
+    }
+}
+
+struct RRArgs {
+    xc_mat Drho, rhoWo, rhoWoo, rhoUo, rhoUoo, MFS, rhow;
+    const double *advz, *advx;   // (Nz-1, Nx), (Nz, Nx+1) dense
+    const double* mfsmean;
+    xc_mat srcW, srcU, srcP;     // bnchm only
+    double a2, a3, dt, g0, D, h, oc;
+    int bnchm;
+};
+__global__ __launch_bounds__(NT) void k_rr(RRArgs a, int Nz, int Nx, double* RR) {
+#pragma clang fp contract(off)
+    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t nW = (int64_t)(Nz + 1) * (Nx + 2), nU = (int64_t)(Nz + 2) * (Nx + 1), nP = (int64_t)(Nz + 2) * (Nx + 2);
+    if (k >= nW + nU + nP) return;
+    auto at = [](const xc_mat& m, int r, int c) { return m.p[r + (int64_t)c * m.ld]; };
+    double v = 0.0;
+    if (k < nW) {
+        const int r = (int)(k % (Nz + 1)), c = (int)(k / (Nz + 1));
+        if (c >= 1 && c <= Nx) {
+            if (r == Nz) {
+                const double mfbg = a.mfsmean[0] * a.D / a.h;
+                v = a.oc * at(a.MFS, Nz - 1, c - 1) + (1.0 - a.oc) * mfbg;
+            } else if (r >= 1) {
+                const double adv = (c - 1 >= 0 && c - 1 < Nx) ? a.advz[(r - 1) + (int64_t)(c - 1) * (Nz - 1)] : 0.0;
+                v = at(a.Drho, r, c - 1) * a.g0 + (a.a2 * at(a.rhoWo, r, c - 1) + a.a3 * at(a.rhoWoo, r, c - 1)) / a.dt - adv;
+                if (a.bnchm) v += at(a.srcW, r, c);
+            }
+        }
+    } else if (k < nW + nU) {
+        const int64_t q = k - nW;
+        const int r = (int)(q % (Nz + 2)), c = (int)(q / (Nz + 2));
+        if (r >= 1 && r <= Nz) {
+            double adv = a.advx[(r - 1) + (int64_t)c * Nz];
+            if (c == 0 || c == Nx) adv = (a.advx[(r - 1)] + a.advx[(r - 1) + (int64_t)Nx * Nz]) / 2.0;
+            v = (a.a2 * at(a.rhoUo, r - 1, c) + a.a3 * at(a.rhoUoo, r - 1, c)) / a.dt - adv;
+            if (a.bnchm) v += at(a.srcU, r, c);
+        }
+    } else {
+        const int64_t q = k - nW - nU;
+        const int r = (int)(q % (Nz + 2)), c = (int)(q / (Nz + 2));
+        if (r >= 1 && r <= Nz && c >= 1 && c <= Nx) {
+            v = at(a.MFS, r - 1, c - 1);
+            if (a.bnchm) v += at(a.srcP, r, c);
+        }
+    }
+    RR[k] = v;
+}
+// MMS pins of the right-hand side (fluidmech.jl:468, 475)
+__global__ void k_rr_pins(double* RR, int64_t kp, xc_mat Pex, int pr, int pc, int64_t ku, xc_mat Uex, int ur, int uc) {
+    RR[kp] = Pex.p[pr + (int64_t)pc * Pex.ld];
+    RR[ku] = Uex.p[ur + (int64_t)uc * Uex.ld];
+}
+
+// ---------------------------------------------------------------- host side
+struct Level {
+    int Nz, Nx;
+    double h;
+    double *eta, *etaco, *rho, *rhow, *rhou;  // device (owned for coarse levels; fine level: packed copies)
+    double *x, *b, *r, *t;                    // velocity vectors
+    int64_t nv;
+    LevD d() const {
+        LevD L;
+        L.Nz = Nz; L.Nx = Nx; L.h = h; L.invh = 1.0 / h; L.invh2 = (1.0 / h) * (1.0 / h);
+        L.eta = eta; L.etaco = etaco; L.rho = rho; L.rhow = rhow; L.rhou = rhou;
+        return L;
+    }
+};
+
+struct StokesCtx {
+    int Nz = -1, Nx = -1, m = -1;
+    std::vector<Level> lev;
+    double* pool = nullptr;   // everything below carved from one allocation
+    size_t pool_bytes = 0;
+    double *xs, *bs, *rs, *ws, *V, *Z, *part, *dots, *tmp, *adv_buf;
+    double* host = nullptr;   // pinned host scratch for reductions
+};
+
+void stokes_free(xc_ctx*, void* p) {
+    StokesCtx* s = (StokesCtx*)p;
+    if (s->pool) (void)hipFree(s->pool);
+    if (s->host) (void)hipHostFree(s->host);
+    delete s;
+}
+
+inline unsigned nblk(int64_t n) { return (unsigned)((n + NT - 1) / NT); }
+
+int setup(xc_ctx* ctx, int Nz, int Nx, int m, StokesCtx** out) {
+    StokesCtx* s = (StokesCtx*)ctx->ext[XC_EXT_STOKES];
+    if (s && s->Nz == Nz && s->Nx == Nx && s->m >= m) { *out = s; return XC_OK; }
+    if (s) { XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream)); stokes_free(ctx, s); ctx->ext[XC_EXT_STOKES] = nullptr; }
+    s = new StokesCtx();
+    s->Nz = Nz; s->Nx = Nx; s->m = m;
+    // levels: halve while both sizes stay even and >= 8 (coarsest at least 4x4)
+    int nz = Nz, nx = Nx;
+    s->lev.push_back(Level{});
+    s->lev.back().Nz = nz; s->lev.back().Nx = nx;
+    while (nz % 2 == 0 && nx % 2 == 0 && nz >= 8 && nx >= 8 && s->lev.size() < 12) {
+        nz /= 2; nx /= 2;
+        Level L{};
+        L.Nz = nz; L.Nx = nx;
+        s->lev.push_back(L);
+    }
+    const int64_t n = (int64_t)(Nz + 1) * (Nx + 2) + (int64_t)(Nz + 2) * (Nx + 1) + (int64_t)(Nz + 2) * (Nx + 2);
+    size_t tot = 0;
+    for (auto& L : s->lev) {
+        L.nv = (int64_t)(L.Nz + 1) * (L.Nx + 2) + (int64_t)(L.Nz + 2) * (L.Nx + 1);
+        tot += (size_t)L.Nz * L.Nx * 2 + (size_t)(L.Nz + 1) * (L.Nx + 1) + (size_t)(L.Nz + 1) * L.Nx +
+               (size_t)L.Nz * (L.Nx + 1) + 4 * (size_t)L.nv;
+    }
+    tot += 4 * (size_t)n + (size_t)(m + 1) * n + (size_t)m * n + (size_t)(m + 2) * DOT_BLOCKS + 64 +
+           (size_t)n /* tmp */ + 8 * (size_t)(Nz + 2) * (Nx + 3) /* advection scratch */;
+    void* pool;
+    hipError_t e = hipMalloc(&pool, tot * sizeof(double));
+    if (e != hipSuccess) { delete s; return xc_fail(ctx, XC_ENOMEM, "fluidmech!: %zu bytes: %s", tot * 8, hipGetErrorString(e)); }
+    s->pool = (double*)pool;
+    s->pool_bytes = tot * sizeof(double);
+    double* p = s->pool;
+    auto take = [&](size_t k) { double* q = p; p += k; return q; };
+    for (auto& L : s->lev) {
+        L.eta = take((size_t)L.Nz * L.Nx);
+        L.rho = take((size_t)L.Nz * L.Nx);
+        L.etaco = take((size_t)(L.Nz + 1) * (L.Nx + 1));
+        L.rhow = take((size_t)(L.Nz + 1) * L.Nx);
+        L.rhou = take((size_t)L.Nz * (L.Nx + 1));
+        L.x = take(L.nv); L.b = take(L.nv); L.r = take(L.nv); L.t = take(L.nv);
+    }
+    s->xs = take(n); s->bs = take(n); s->rs = take(n); s->ws = take(n);
+    s->V = take((size_t)(m + 1) * n);
+    s->Z = take((size_t)m * n);
+    s->part = take((size_t)(m + 2) * DOT_BLOCKS);
+    s->dots = take(128);
+    s->tmp = take(n);
+    s->adv_buf = take(8 * (size_t)(Nz + 2) * (Nx + 3));
+    e = hipHostMalloc((void**)&s->host, 256 * sizeof(double), hipHostMallocDefault);
+    if (e != hipSuccess) { stokes_free(ctx, s); return xc_fail(ctx, XC_ENOMEM, "fluidmech!: pinned scratch"); }
+    ctx->ext[XC_EXT_STOKES] = s;
+    ctx->ext_free[XC_EXT_STOKES] = stokes_free;
+    *out = s;
+    return XC_OK;
+}
+
+Flags make_flags(const xc_fm_opts* o, const xc_par* par, int Nz, int Nx, bool bnchm) {
+    Flags F{};
+    F.bnchm = bnchm;
+    F.sds = o->sds; F.top_cnv = o->top_cnv; F.bot_cnv = o->bot_cnv; F.open_cnv = o->open_cnv;
+    F.a1 = o->a1; F.gamma = par->gamma; F.dt = o->dt; F.g0 = par->g0;
+    F.nu0 = -1;
+    auto jround = [](double v) { return (int)nearbyint(v); };
+    if (bnchm) {
+        F.pin_kind = 1;
+        F.pr = jround((double)(Nz + 2) * 3.0 / 8.0) - 1;
+        F.pc = jround((double)(Nx + 2) / 2.0) - 1;
+        const int ur = jround((double)(Nz + 2) / 2.0) - 1, uc = jround((double)(Nx + 2) / 2.0) - 1;
+        F.nu0 = (int64_t)(Nz + 1) * (Nx + 2) + ur + (int64_t)uc * (Nz + 2);
+    } else if (o->open_cnv) {
+        F.pin_kind = 2;
+    } else {
+        F.pin_kind = 1;
+        F.pr = jround((double)Nz / 2.0) - 1;
+        F.pc = jround((double)Nx / 2.0) - 1;
+    }
+    return F;
+}
+
+// copy the fine-level coefficients densely and build the coarse hierarchy
+void build_levels(xc_ctx* ctx, StokesCtx* s, const xc_fluid* fl, double h) {
+    Level& L0 = s->lev[0];
+    L0.h = h;
+    hipLaunchKernelGGL(k_pack, dim3(nblk(fl->eta.rows * fl->eta.cols)), dim3(NT), 0, ctx->stream, fl->eta, L0.eta);
+    hipLaunchKernelGGL(k_pack, dim3(nblk(fl->rho.rows * fl->rho.cols)), dim3(NT), 0, ctx->stream, fl->rho, L0.rho);
+    hipLaunchKernelGGL(k_pack, dim3(nblk(fl->etaco.rows * fl->etaco.cols)), dim3(NT), 0, ctx->stream, fl->etaco, L0.etaco);
+    hipLaunchKernelGGL(k_pack, dim3(nblk(fl->rhow.rows * fl->rhow.cols)), dim3(NT), 0, ctx->stream, fl->rhow, L0.rhow);
+    hipLaunchKernelGGL(k_pack, dim3(nblk(fl->rhou.rows * fl->rhou.cols)), dim3(NT), 0, ctx->stream, fl->rhou, L0.rhou);
+    for (size_t l = 1; l < s->lev.size(); ++l) {
+        Level& Lf = s->lev[l - 1];
+        Level& Lc = s->lev[l];
+        Lc.h = 2.0 * Lf.h;
+        hipLaunchKernelGGL(k_coarsen, dim3(nblk((int64_t)(Lc.Nz + 1) * (Lc.Nx + 1))), dim3(NT), 0, ctx->stream, Lf.d(),
+                           Lc.eta, Lc.rho, Lc.rhow, Lc.rhou, Lc.Nz, Lc.Nx);
+        hipLaunchKernelGGL(k_corner_eta, dim3(nblk((int64_t)(Lc.Nz + 1) * (Lc.Nx + 1))), dim3(NT), 0, ctx->stream,
+                           Lc.eta, Lc.etaco, Lc.Nz, Lc.Nx);
+    }
+}
+
+struct MGOpts { int pre, post, coarse; double omega; };
+
+// one V-cycle on level l: solve A x = b approximately from x = 0
+void vcycle(xc_ctx* ctx, StokesCtx* s, const Flags& F, size_t l, const MGOpts& mo) {
+    Level& L = s->lev[l];
+    const LevD d = L.d();
+    const unsigned nb = nblk(L.nv);
+    const int fine = l == 0;
+    const bool coarsest = l + 1 == s->lev.size();
+    const int pre = coarsest ? mo.coarse : mo.pre;
+    // first sweep from zero
+    hipLaunchKernelGGL(k_jacobi, dim3(nb), dim3(NT), 0, ctx->stream, d, F, L.x, L.b, L.t, mo.omega, fine, 1);
+    std::swap(L.x, L.t);
+    for (int it = 1; it < pre; ++it) {
+        hipLaunchKernelGGL(k_jacobi, dim3(nb), dim3(NT), 0, ctx->stream, d, F, L.x, L.b, L.t, mo.omega, fine, 0);
+        std::swap(L.x, L.t);
+    }
+    if (coarsest) return;
+    hipLaunchKernelGGL(k_vel_resid, dim3(nb), dim3(NT), 0, ctx->stream, d, F, L.x, L.b, L.r, fine);
+    Level& C = s->lev[l + 1];
+    hipLaunchKernelGGL(k_restrict, dim3(nblk(C.nv)), dim3(NT), 0, ctx->stream, d, C.d(), L.r, C.b);
+    vcycle(ctx, s, F, l + 1, mo);
+    hipLaunchKernelGGL(k_prolong, dim3(nb), dim3(NT), 0, ctx->stream, d, C.d(), F, C.x, L.x);
+    for (int it = 0; it < mo.post; ++it) {
+        hipLaunchKernelGGL(k_jacobi, dim3(nb), dim3(NT), 0, ctx->stream, d, F, L.x, L.b, L.t, mo.omega, fine, 0);
+        std::swap(L.x, L.t);
+    }
+}
+
+// z = M^-1 v  (block-triangular preconditioner)
+void precond(xc_ctx* ctx, StokesCtx* s, const Flags& F, const double* v, double* z, const MGOpts& mo) {
+    Level& L = s->lev[0];
+    const LevD d = L.d();
+    const int64_t nv = L.nv, nP = (int64_t)(L.Nz + 2) * (L.Nx + 2);
+    const double* vP = v + nv;
+    double* zP = z + nv;
+    hipLaunchKernelGGL(k_schur_int, dim3(nblk(nP)), dim3(NT), 0, ctx->stream, d, F, vP, zP);
+    hipLaunchKernelGGL(k_schur_ghost, dim3(nblk(nP)), dim3(NT), 0, ctx->stream, d, F, vP, zP);
+    hipLaunchKernelGGL(k_sub_grad, dim3(nblk(nv)), dim3(NT), 0, ctx->stream, d, F, v, zP, L.b);
+    vcycle(ctx, s, F, 0, mo);
+    hipLaunchKernelGGL(k_copyv, dim3(nblk(nv)), dim3(NT), 0, ctx->stream, L.x, z, nv);
+}
+
+// dots[0..K] = (V[q] . w) for q < K and w.w; synchronous, results in s->host
+int multidot(xc_ctx* ctx, StokesCtx* s, const double* V, int K, const double* w, int64_t n) {
+    const int nbk = DOT_BLOCKS;
+    hipLaunchKernelGGL(k_multidot, dim3(nbk), dim3(NT), 0, ctx->stream, V, n, K, w, s->part);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(NT), 0, ctx->stream, s->part, nbk, K + 1, s->dots);
+    XC_TRY_HIP(ctx, hipMemcpyAsync(s->host, s->dots, (K + 1) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return XC_OK;
+}
+
+int build_rhs(xc_ctx* ctx, StokesCtx* s, xc_fluid* fl, const xc_phase* ph, const xc_grid* g, const xc_par* par,
+              const xc_fm_opts* o, const xc_bnchm* bn, const Flags& F, double* RR) {
+    const int Nz = (int)g->Nz, Nx = (int)g->Nx;
+    // MFS update + mean (fluidmech.jl:115-127)
+    RhsArgs ra{};
+    ra.rho = fl->rho; ra.rhoo = fl->rhoo; ra.rhooo = fl->rhooo; ra.drhodt = fl->drhodt; ra.drhodto = fl->drhodto;
+    ra.drhodtoo = fl->drhodtoo; ra.MFS = fl->MFS;
+    if (ph) ra.advn_rho = ph->advn_rho;
+    ra.a1 = o->a1; ra.a2 = o->a2; ra.a3 = o->a3; ra.b1 = o->b1; ra.b2 = o->b2; ra.b3 = o->b3; ra.dt = o->dt;
+    ra.alpha = par->alpha;
+    ra.do_mfs = bn == nullptr;
+    const unsigned nbc = nblk((int64_t)Nz * Nx);
+    hipLaunchKernelGGL(k_mfs, dim3(nbc), dim3(NT), 0, ctx->stream, ra, Nz, Nx, s->part);
+    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(NT), 0, ctx->stream, s->part, (int)nbc, 1.0 / ((double)Nz * Nx), s->dots + 100);
+    // momentum advection (fluidmech.jl:236-240, 310-318)
+    double* fz = s->adv_buf;
+    double* uz = fz + (size_t)(Nz + 2) * (Nx + 3);
+    double* wz = uz + (size_t)(Nz + 2) * (Nx + 3);
+    double* fx = wz + (size_t)(Nz + 2) * (Nx + 3);
+    double* ux = fx + (size_t)(Nz + 2) * (Nx + 3);
+    double* wx = ux + (size_t)(Nz + 2) * (Nx + 3);
+    double* az = wx + (size_t)(Nz + 2) * (Nx + 3);
+    double* ax = az + (size_t)(Nz + 2) * (Nx + 3);
+    hipLaunchKernelGGL(k_mom_prep, dim3(nblk((int64_t)(Nz + 2) * (Nx + 3))), dim3(NT), 0, ctx->stream, fl->W, fl->U,
+                       fl->rhow, fl->rhou, fz, uz, wz, fx, ux, wx, Nz, Nx);
+    const unsigned saved = ctx->flags;
+    ctx->flags &= ~XC_CTX_SYNC;
+    int st = xc_advect_centered(ctx, o->scheme, xc_mat{az, Nz - 1, Nx, Nz - 1}, nullptr, nullptr,
+                                xc_mat{fz, Nz - 1, Nx, Nz - 1}, xc_mat{uz, Nz - 1, Nx + 1, Nz - 1},
+                                xc_mat{wz, Nz, Nx, Nz}, g->h, o->xbc, o->zbc);
+    if (!st)
+        st = xc_advect_centered(ctx, o->scheme, xc_mat{ax, Nz, Nx + 1, Nz}, nullptr, nullptr,
+                                xc_mat{fx, Nz, Nx + 1, Nz}, xc_mat{ux, Nz, Nx + 2, Nz}, xc_mat{wx, Nz + 1, Nx + 1, Nz + 1},
+                                g->h, o->xbc, o->zbc);
+    ctx->flags = saved;
+    if (st) return st;
+    RRArgs a{};
+    a.Drho = fl->Drho; a.rhoWo = fl->rhoWo; a.rhoWoo = fl->rhoWoo; a.rhoUo = fl->rhoUo; a.rhoUoo = fl->rhoUoo;
+    a.MFS = fl->MFS; a.rhow = fl->rhow;
+    a.advz = az; a.advx = ax; a.mfsmean = s->dots + 100;
+    if (bn) { a.srcW = bn->src_W; a.srcU = bn->src_U; a.srcP = bn->src_P; }
+    a.a2 = o->a2; a.a3 = o->a3; a.dt = o->dt; a.g0 = par->g0; a.D = g->D; a.h = g->h;
+    a.oc = o->open_cnv ? 1.0 : 0.0;
+    a.bnchm = bn != nullptr;
+    const int64_t n = (int64_t)(Nz + 1) * (Nx + 2) + (int64_t)(Nz + 2) * (Nx + 1) + (int64_t)(Nz + 2) * (Nx + 2);
+    hipLaunchKernelGGL(k_rr, dim3(nblk(n)), dim3(NT), 0, ctx->stream, a, Nz, Nx, RR);
+    if (bn) {
+        const int64_t nW = (int64_t)(Nz + 1) * (Nx + 2), nU = (int64_t)(Nz + 2) * (Nx + 1);
+        const int64_t kp = nW + nU + F.pr + (int64_t)F.pc * (Nz + 2);
+        const int64_t q = F.nu0 - nW;
+        hipLaunchKernelGGL(k_rr_pins, dim3(1), dim3(1), 0, ctx->stream, RR, kp, bn->P_exact, F.pr, F.pc, F.nu0,
+                           bn->U_exact, (int)(q % (Nz + 2)), (int)(q / (Nz + 2)));
+    }
+    return XC_OK;
+}
+
+int check_fm(xc_ctx* ctx, const xc_fluid* fl, const xc_grid* g, const xc_fm_opts* o) {
+    if (!fl || !g || !o) return xc_fail(ctx, XC_EINVAL_ARG, "fluidmech!: NULL argument");
+    const int64_t Nz = g->Nz, Nx = g->Nx;
+    if (Nz < 4 || Nx < 4) return xc_fail(ctx, XC_EINVAL_SHAPE, "fluidmech!: grid must be at least 4x4");
+    struct { const xc_mat* m; int64_t r, c; const char* n; } t[] = {
+        {&fl->W, Nz + 1, Nx + 2, "W"}, {&fl->U, Nz + 2, Nx + 1, "U"}, {&fl->P, Nz + 2, Nx + 2, "P"},
+        {&fl->eta, Nz, Nx, "eta"}, {&fl->etaco, Nz + 1, Nx + 1, "etaco"}, {&fl->rho, Nz, Nx, "rho"},
+        {&fl->rhow, Nz + 1, Nx, "rhow"}, {&fl->rhou, Nz, Nx + 1, "rhou"}, {&fl->Drho, Nz + 1, Nx, "Drho"},
+        {&fl->MFS, Nz, Nx, "MFS"}, {&fl->rhoWo, Nz + 1, Nx, "rhoWo"}, {&fl->rhoWoo, Nz + 1, Nx, "rhoWoo"},
+        {&fl->rhoUo, Nz, Nx + 1, "rhoUo"}, {&fl->rhoUoo, Nz, Nx + 1, "rhoUoo"}};
+    for (auto& e : t)
+        if (!xc_shape(*e.m, e.r, e.c)) return xc_fail(ctx, XC_EINVAL_SHAPE, "FluidState.%s has the wrong shape", e.n);
+    if (o->sds != 1 && o->sds != -1) return xc_fail(ctx, XC_EINVAL_ARG, "fluidmech!: sds must be +-1");
+    if (o->xbc != XC_PERIODIC && o->xbc != XC_CLOSED) return xc_fail(ctx, XC_EINVAL_BC, "fluidmech!: xBC");
+    if (o->zbc != XC_PERIODIC && o->zbc != XC_CLOSED) return xc_fail(ctx, XC_EINVAL_BC, "fluidmech!: zBC");
+    if (xc_scheme_halo(o->scheme) < 0) return xc_fail(ctx, XC_EINVAL_SCHEME, "fluidmech!: scheme");
+    return XC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int xc_stokes_apply(xc_ctx* ctx, const xc_fluid* fl, const xc_grid* g, const xc_par* par, const xc_fm_opts* o,
+                    int bnchm, const double* x, double* y) {
+    XC_CHECK_CTX(ctx);
+    int st;
+    if ((st = check_fm(ctx, fl, g, o))) return st;
+    StokesCtx* s;
+    if ((st = setup(ctx, (int)g->Nz, (int)g->Nx, 1, &s))) return st;
+    build_levels(ctx, s, fl, g->h);
+    const Flags F = make_flags(o, par, (int)g->Nz, (int)g->Nx, bnchm != 0);
+    const int64_t n = s->lev[0].nv + (g->Nz + 2) * (g->Nx + 2);
+    hipLaunchKernelGGL(k_ll_apply, dim3(nblk(n)), dim3(NT), 0, ctx->stream, s->lev[0].d(), F, x, y, n);
+    return xc_finish(ctx, "stokes_apply");
+}
+
+int xc_stokes_rhs(xc_ctx* ctx, xc_fluid* fl, const xc_phase* ph, const xc_grid* g, const xc_par* par,
+                  const xc_fm_opts* o, const xc_bnchm* bn, double* rr) {
+    XC_CHECK_CTX(ctx);
+    int st;
+    if ((st = check_fm(ctx, fl, g, o))) return st;
+    if (!bn && !ph) return xc_fail(ctx, XC_EINVAL_ARG, "fluidmech!: phase required outside the benchmark mode");
+    StokesCtx* s;
+    if ((st = setup(ctx, (int)g->Nz, (int)g->Nx, 1, &s))) return st;
+    const Flags F = make_flags(o, par, (int)g->Nz, (int)g->Nx, bn != nullptr);
+    if ((st = build_rhs(ctx, s, fl, ph, g, par, o, bn, F, rr))) return st;
+    return xc_finish(ctx, "stokes_rhs");
+}
+
+int xc_fluidmech(xc_ctx* ctx, xc_fluid* fl, const xc_phase* ph, const xc_grid* g, const xc_par* par,
+                 const xc_fm_opts* o, const xc_bnchm* bn, xc_solve_stats* stats) {
+    XC_CHECK_CTX(ctx);
+    int st;
+    if ((st = check_fm(ctx, fl, g, o))) return st;
+    if (!bn && !ph) return xc_fail(ctx, XC_EINVAL_ARG, "fluidmech!: phase required outside the benchmark mode");
+    const int m = o->restart > 0 ? (o->restart < 60 ? o->restart : 60) : 30;
+    StokesCtx* s;
+    if ((st = setup(ctx, (int)g->Nz, (int)g->Nx, m, &s))) return st;
+    const int Nz = (int)g->Nz, Nx = (int)g->Nx;
+    const Flags F = make_flags(o, par, Nz, Nx, bn != nullptr);
+    const int64_t n = s->lev[0].nv + (int64_t)(Nz + 2) * (Nx + 2);
+    const unsigned nb = nblk(n);
+    const LevD d0 = s->lev[0].d();
+    (void)d0;
+    MGOpts mo{o->mg_pre > 0 ? o->mg_pre : 2, o->mg_post > 0 ? o->mg_post : 2, o->mg_coarse > 0 ? o->mg_coarse : 60,
+              o->mg_omega > 0 ? o->mg_omega : 0.7};
+    // system: coefficients, RR, initial guess
+    build_levels(ctx, s, fl, g->h);
+    const LevD dd = s->lev[0].d();
+    if ((st = build_rhs(ctx, s, fl, ph, g, par, o, bn, F, s->bs))) return st;
+    hipLaunchKernelGGL(k_gather, dim3(nb), dim3(NT), 0, ctx->stream, fl->W, fl->U, fl->P, s->xs, Nz, Nx);
+    // ||RR||
+    if ((st = multidot(ctx, s, s->bs, 0, s->bs, n))) return st;
+    const double bnorm = sqrt(s->host[0]);
+    const double target = o->tol * (bnorm > 0 ? bnorm : 1.0);
+    const int maxit = o->maxit > 0 ? o->maxit : 500;
+    int iters = 0;
+    double rel = 0.0, rel0 = -1.0;
+    bool conv = false;
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), y(m);
+    while (true) {
+        hipLaunchKernelGGL(k_ll_resid, dim3(nb), dim3(NT), 0, ctx->stream, dd, F, s->xs, s->bs, s->rs, n);
+        if ((st = multidot(ctx, s, s->rs, 0, s->rs, n))) return st;
+        const double beta = sqrt(s->host[0]);
+        rel = bnorm > 0 ? beta / bnorm : beta;
+        if (rel0 < 0) rel0 = rel;
+        if (beta <= target) { conv = true; break; }
+        if (iters >= maxit) break;
+        hipLaunchKernelGGL(k_scale, dim3(nb), dim3(NT), 0, ctx->stream, s->rs, 1.0 / beta, s->V, n);
+        std::fill(gv.begin(), gv.end(), 0.0);
+        gv[0] = beta;
+        int j = 0;
+        for (; j < m && iters < maxit; ++j) {
+            double* vj = s->V + (int64_t)j * n;
+            double* zj = s->Z + (int64_t)j * n;
+            precond(ctx, s, F, vj, zj, mo);
+            hipLaunchKernelGGL(k_ll_apply, dim3(nb), dim3(NT), 0, ctx->stream, dd, F, zj, s->ws, n);
+            // classical Gram-Schmidt, twice
+            for (int pass = 0; pass < 2; ++pass) {
+                if ((st = multidot(ctx, s, s->V, j + 1, s->ws, n))) return st;
+                for (int i = 0; i <= j; ++i) {
+                    if (pass == 0) H[i + (size_t)j * (m + 1)] = s->host[i];
+                    else H[i + (size_t)j * (m + 1)] += s->host[i];
+                }
+                XC_TRY_HIP(ctx, hipMemcpyAsync(s->dots, s->host, (j + 1) * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+                hipLaunchKernelGGL(k_orth, dim3(nb), dim3(NT), 0, ctx->stream, s->V, n, j + 1, s->dots, s->ws);
+            }
+            if ((st = multidot(ctx, s, s->V, 0, s->ws, n))) return st;
+            const double hn = sqrt(s->host[0]);
+            H[j + 1 + (size_t)j * (m + 1)] = hn;
+            ++iters;
+            // Givens rotations
+            for (int i = 0; i < j; ++i) {
+                const double a = H[i + (size_t)j * (m + 1)], b = H[i + 1 + (size_t)j * (m + 1)];
+                H[i + (size_t)j * (m + 1)] = cs[i] * a + sn[i] * b;
+                H[i + 1 + (size_t)j * (m + 1)] = -sn[i] * a + cs[i] * b;
+            }
+            const double a = H[j + (size_t)j * (m + 1)], b = H[j + 1 + (size_t)j * (m + 1)];
+            const double rr = sqrt(a * a + b * b);
+            cs[j] = rr > 0 ? a / rr : 1.0;
+            sn[j] = rr > 0 ? b / rr : 0.0;
+            H[j + (size_t)j * (m + 1)] = rr;
+            H[j + 1 + (size_t)j * (m + 1)] = 0.0;
+            gv[j + 1] = -sn[j] * gv[j];
+            gv[j] = cs[j] * gv[j];
+            if (fabs(gv[j + 1]) <= 0.5 * target || hn == 0.0) { ++j; break; }
+            if (j + 1 < m) hipLaunchKernelGGL(k_scale, dim3(nb), dim3(NT), 0, ctx->stream, s->ws, 1.0 / hn,
+                                              s->V + (int64_t)(j + 1) * n, n);
+        }
+        // y = H^-1 g (upper triangular j x j), x += Z y
+        for (int i = j - 1; i >= 0; --i) {
+            double sum = gv[i];
+            for (int k = i + 1; k < j; ++k) sum -= H[i + (size_t)k * (m + 1)] * y[k];
+            y[i] = sum / H[i + (size_t)i * (m + 1)];
+        }
+        XC_TRY_HIP(ctx, hipMemcpyAsync(s->dots, y.data(), j * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(k_update_x, dim3(nb), dim3(NT), 0, ctx->stream, s->Z, n, j, s->dots, s->xs);
+    }
+    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(NT), 0, ctx->stream, s->xs, fl->W, fl->U, fl->P, Nz, Nx);
+    if (stats) {
+        stats->iters = iters;
+        stats->converged = conv;
+        stats->rel_resid = rel;
+        stats->init_rel_resid = rel0;
+        stats->mg_levels = (int)s->lev.size();
+        stats->fallback_used = 0;
+    }
+    if ((st = xc_finish(ctx, "fluidmech!"))) return st;
+    if (!conv) return xc_fail(ctx, XC_ENOCONV, "fluidmech!: FGMRES reached %d iterations at relative residual %.3e",
+                              iters, rel);
+    return XC_OK;
+}
+
+}  // extern "C"

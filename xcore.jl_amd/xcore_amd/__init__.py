@@ -8,10 +8,13 @@ from ._lib import LIB_PATH, NoConvergence, XCoreError, declared_symbols, lib
 from .arrays import as_mat, interior, to_device, to_host, xcore_zeros
 from .kernels import (advect_, advect_centered_, advect_centered_with_flux_, advect_with_flux_, context,
                       diffus_, diffus_centered_, diffus_centered_with_flux_, diffus_with_flux_,
-                      embed_interior_, fill_ghosts_, scheme_halo)
+                      embed_interior_, exact_arithmetic, fill_ghosts_, scheme_halo)
 from .state import (FluidState, Grid, NoiseState, Parameters, PhaseState, Scales, compute_scales)
 from .fields import (compute_fft_filter, noise_, normal_fill_, phsevo_, store_noise_, store_previous_, time_coefs,
                      update_, update_dt, update_phase_velocities_)
+from .stokes import BnchmData, fluidmech_, stokes_apply, stokes_rhs, stokes_vector_size
+from .driver import (History, StepState, history_sums, initialize_, record_history_, run_, run_step_,
+                     step_opts)
 
 
 def load():

@@ -19,6 +19,7 @@ XC_ENOCONV = 5
 XC_EINVAL_ARG = 6
 XC_ENOMEM = 7
 XC_CTX_SYNC = 1
+XC_CTX_EXACT = 2
 
 SCHEMES = {"centr": 0, "upwd1": 1, "quick": 2, "fromm": 3, "weno3": 4, "weno5": 5, "tvdim": 6}
 BCS = {"periodic": 0, "closed": 1}
@@ -133,6 +134,9 @@ class Context:
 
     def bind_stream(self, stream_handle):
         lib().xc_set_stream(self.ptr, _P(stream_handle))
+
+    def set_flags(self, flags):
+        self.check(lib().xc_set_flags(self.ptr, int(flags)), "xc_set_flags")
 
     def sync(self):
         self.check(lib().xc_sync(self.ptr), "xc_sync")

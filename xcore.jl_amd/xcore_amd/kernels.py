@@ -27,6 +27,18 @@ def context(device=None):
     return c
 
 
+class exact_arithmetic:
+    """Context manager: run the stencils in the reference's exact operation order (bit-identical)."""
+
+    def __enter__(self):
+        self.c = context()
+        self.c.set_flags(_lib.XC_CTX_EXACT)
+        return self
+
+    def __exit__(self, *a):
+        self.c.set_flags(0)
+
+
 def _sym(s, table, kind):
     key = str(s).lstrip(":")
     if key not in table:
