@@ -1,21 +1,25 @@
 """Benchmark of the xcore.jl hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu] [--no-stencil]
 
-Workload (one "step"): the stencil + noise microbench at 2048^2, FP64 --
-four scalar fields (sums of 32 random Fourier modes, amplitude ~ k^-2) are
-each advected with WENO5 and with QUICK through advect_centered_with_flux!
-by a divergence-free staggered velocity (streamfunction sin(pi x) sin(pi z)
-plus 16 seeded modes), and diffused through diffus_centered_with_flux! with a
-lognormal (sigma = 1) diffusivity; x periodic, z closed.  Inputs are resident
-in HBM before timing; the four field/output sets (604 MB) rotate so the
-256 MB Infinity Cache cannot hold one step's working set.
+Workload (one "step"): one outer time step of run! at 512 x 512, FP64 --
+store_previous!, store_noise!, then the adaptive Picard loop (>= 3 sweeps of
+phsevo!, fluidmech! solved by FGMRES + block multigrid to a 1e-8 relative
+residual, noise!, update_phase_velocities!, update!, compute_resnorm) and
+update_dt.  Parameters are the reference defaults with viscosity contrast cap
+etacntr = 1e3 and crystal-melt density contrast rhox0 - rhom0 = 200 kg/m^3;
+the state comes from initialize! (seeded perturbation), x periodic, z closed.
+All state is resident in HBM before timing; value = steps / wall time.
 
-Prints one JSON line (rank 0).  `roofline` is for the dominant kernel
-(WENO5 advect_centered_with_flux!): algorithmic bytes per launch / its
-average duration measured here with HIP events on the launch stream.
-`cpu_baseline` times the CPU oracle (the C restatement of the reference,
-OpenMP) on one full step of the same workload.
+The stencil microbench (2048^2, four fields, WENO5 + QUICK advection with
+fluxes and lognormal-k diffusion with fluxes) runs after the step bench and
+gives each stencil's algorithmic GB/s from HIP events on the launch stream;
+its WENO5 kernel is the `roofline` object (the HBM-bound kernel family of the
+path: the 512^2 step's state, ~250 MB, sits in the 256 MB Infinity Cache).
+
+`cpu_baseline` times the CPU oracle (the C restatement of the reference with
+SciPy's sparse LU for the Stokes solve) on one Picard sweep of the same step
+at 256^2 and converts it to steps/s with the mean sweep count of the GPU run.
 """
 import argparse
 import json
@@ -32,6 +36,8 @@ METRIC = "steps/s at 512² learned vs plain Stokes @1e-8 resid; stencil % HBM pe
 HBM_PEAK_GBS = 8000.0
 N_STENCIL = 2048
 NFIELDS = 4
+# HBM bytes per launch of the WENO5 stencil at 2048^2 from rocprofv3 FETCH_SIZE + WRITE_SIZE (null until measured)
+TRAFFIC_WENO5_2048 = None
 
 
 def make_inputs(N, seed_base=1001):
@@ -66,14 +72,51 @@ def make_inputs(N, seed_base=1001):
     return fields, u, w, k, h
 
 
-def gpu_bench(args):
+def c3_setup(N, seed=15):
+    """The 512^2 full-step workload: reference defaults, etacntr 1e3, rhox0 - rhom0 = 200."""
+    import torch
+    import xcore_amd as xc
+    par = xc.Parameters(N=N, etacntr=1e3, rhox0=2700.0 + 200.0, seed=seed)
+    g = xc.Grid.from_par(par)
+    sc = xc.compute_scales(par, g)
+    ph, fl = xc.PhaseState(g.Nz, g.Nx), xc.FluidState(g.Nz, g.Nx)
+    ns = xc.NoiseState(g, sc, seed=seed)
+    hst = xc.History()
+    t, dt, step = xc.initialize_(ph, fl, ns, hst, g, par, sc)
+    torch.cuda.synchronize()
+    return par, g, sc, ph, fl, ns, hst, xc.StepState(t, dt, step)
+
+
+def step_bench(args):
     import torch
     import xcore_amd as xc
     xc.load()
-    N = args.n
+    torch.cuda.set_device(0)
+    par, g, sc, ph, fl, ns, hst, state = c3_setup(args.n)
+    opts = xc.step_opts(par, tol=1e-8, maxit=2000)
+    for _ in range(args.warmup):
+        xc.run_step_(ph, fl, ns, g, par, sc, state, opts)
+    torch.cuda.synchronize()
+    log = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        log.append(xc.run_step_(ph, fl, ns, g, par, sc, state, opts))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    sweeps = sum(r["sweeps"] for r in log)
+    its = sum(r["stokes_iters"] for r in log)
+    info = {"sweeps_per_step": sweeps / len(log), "fgmres_iters_per_solve": its / max(sweeps, 1),
+            "worst_stokes_rel_resid": max(r["stokes_worst_rel_resid"] for r in log),
+            "sim_time": state.time, "dt": state.dt, "step": int(state.step),
+            "x_finite": bool(torch.isfinite(ph.x).all())}
+    return 1e3 * elapsed / args.steps, info
+
+
+def stencil_bench(N, steps, warmup):
+    """C2 stencil microbench: per-kernel HIP-event timings on the launch stream."""
+    import torch
+    import xcore_amd as xc
     fields_h, u_h, w_h, k_h, h = make_inputs(N)
-    dev = torch.device("cuda", 0)
-    torch.cuda.set_device(dev)
     u = xc.to_device(u_h)
     w = xc.to_device(w_h)
     k = xc.to_device(k_h)
@@ -84,127 +127,145 @@ def gpu_bench(args):
                          dqx=xc.xcore_zeros(N + 2, N + 1), dqz=xc.xcore_zeros(N + 1, N + 2)))
     torch.cuda.synchronize()
 
-    def step():
-        for s in sets:
-            xc.advect_centered_with_flux_(s["adv"], s["qx"], s["qz"], s["f"], u, w, h, "weno5")
-            xc.advect_centered_with_flux_(s["adv"], s["qx"], s["qz"], s["f"], u, w, h, "quick")
+    def launch(kind, s):
+        if kind == "diffus":
             xc.diffus_centered_with_flux_(s["dff"], s["dqx"], s["dqz"], s["f"], k, h)
+        else:
+            xc.advect_centered_with_flux_(s["adv"], s["qx"], s["qz"], s["f"], u, w, h, kind)
 
-    for _ in range(args.warmup):
-        step()
+    for _ in range(warmup):
+        for s in sets:
+            for kind in ("weno5", "quick", "diffus"):
+                launch(kind, s)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = 1e3 * elapsed / args.steps
-
-    # per-launch duration of the dominant kernel, HIP events on its stream
     stream = torch.cuda.current_stream()
-    reps = max(8, 4 * args.steps)
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for r in range(reps):
-        s = sets[r % NFIELDS]
-        xc.advect_centered_with_flux_(s["adv"], s["qx"], s["qz"], s["f"], u, w, h, "weno5")
-    e1.record(stream)
-    e1.synchronize()
-    weno_us = 1e3 * e0.elapsed_time(e1) / reps
-    e0.record(stream)
-    for r in range(reps):
-        s = sets[r % NFIELDS]
-        xc.advect_centered_with_flux_(s["adv"], s["qx"], s["qz"], s["f"], u, w, h, "quick")
-    e1.record(stream)
-    e1.synchronize()
-    quick_us = 1e3 * e0.elapsed_time(e1) / reps
-    e0.record(stream)
-    for r in range(reps):
-        s = sets[r % NFIELDS]
-        xc.diffus_centered_with_flux_(s["dff"], s["dqx"], s["dqz"], s["f"], k, h)
-    e1.record(stream)
-    e1.synchronize()
-    diff_us = 1e3 * e0.elapsed_time(e1) / reps
-
-    # algorithmic bytes per launch: every input and output array touched once
+    reps = max(8, 4 * steps)
     adv_bytes = 8 * (N * N + N * (N + 1) + (N + 1) * N + N * N + (N + 2) * (N + 1) + (N + 1) * (N + 2))
     dif_bytes = 8 * (N * N + N * N + N * N + (N + 2) * (N + 1) + (N + 1) * (N + 2))
-    ach = adv_bytes / (weno_us * 1e-6) / 1e9
-    kernels = {
-        "weno5_advect_with_flux": {"us": weno_us, "GB/s": ach, "hbm_frac": ach / HBM_PEAK_GBS},
-        "quick_advect_with_flux": {"us": quick_us, "GB/s": adv_bytes / (quick_us * 1e-6) / 1e9,
-                                   "hbm_frac": adv_bytes / (quick_us * 1e-6) / 1e9 / HBM_PEAK_GBS},
-        "diffus_with_flux": {"us": diff_us, "GB/s": dif_bytes / (diff_us * 1e-6) / 1e9,
-                             "hbm_frac": dif_bytes / (diff_us * 1e-6) / 1e9 / HBM_PEAK_GBS},
-    }
-    roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_advect<WENO5,centered,flux>", "bytes_per_launch": adv_bytes,
-                "avg_launch_us": round(weno_us, 2)}
-    return ms_per_step, roofline, kernels
+    out = {}
+    for kind, nbytes, name in (("weno5", adv_bytes, "weno5_advect_with_flux"),
+                               ("quick", adv_bytes, "quick_advect_with_flux"),
+                               ("diffus", dif_bytes, "diffus_with_flux")):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for r in range(reps):
+            launch(kind, sets[r % NFIELDS])
+        e1.record(stream)
+        e1.synchronize()
+        us = 1e3 * e0.elapsed_time(e1) / reps
+        gbs = nbytes / (us * 1e-6) / 1e9
+        out[name] = {"avg_launch_us": round(us, 2), "bytes_per_launch": nbytes, "GB/s": round(gbs, 1),
+                     "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+    wk = out["weno5_advect_with_flux"]
+    roofline = {"bound": "hbm", "achieved": wk["GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": wk["hbm_frac"], "traffic": TRAFFIC_WENO5_2048,
+                "kernel": "k_advect<WENO5> (advect_centered_with_flux!, %d^2, %d rotating field sets)" % (N, NFIELDS),
+                "bytes_per_launch": adv_bytes, "avg_launch_us": wk["avg_launch_us"], "launches_timed": reps}
+    return roofline, out
 
 
-def cpu_baseline(N):
-    """One full step of the same workload on the CPU oracle (OpenMP)."""
+def cpu_baseline(N, sweeps_per_step):
+    """One Picard sweep of the same step on the CPU oracle (C, OpenMP; SciPy sparse LU for Stokes)."""
+    import numpy as np
     from oracle import oracle as orc
+    import xcore_amd as xc
+    import torch
     try:
         cores = len(os.sched_getaffinity(0))
     except Exception:
         cores = os.cpu_count() or 1
     cores = max(1, min(cores, int(os.environ.get("OMP_NUM_THREADS", cores))))
     orc.set_threads(cores)
-    fields, u, w, k, h = make_inputs(N)
-    adv, qx, qz = orc.zeros(N, N), orc.zeros(N + 2, N + 1), orc.zeros(N + 1, N + 2)
-    dff = orc.zeros(N, N)
+    # same initial state as the device run at this size (initialize! on the device, copied to the host)
+    par, g, sc, ph, fl, ns, hst, state = c3_setup(N)
+    hph, hfl = orc.phase_state(N, N), orc.fluid_state(N, N)
+    hns = orc.HostNoise(N, N, ns.padL0, ns.padl0, ns.fL, ns.fl)
+    for host, dev in ((hph, ph), (hfl, fl), (hns, ns)):
+        for n, a in host.arrays().items():
+            a[:] = getattr(dev, n).cpu().numpy()
+    st = dict(time=state.time, dt=state.dt, step=int(state.step), draws=ns.draws)
+    del ph, fl, ns
+    torch.cuda.empty_cache()
     t0 = time.perf_counter()
-    for f in fields:
-        orc.advect_centered(adv, f, u, w, h, "weno5", qx=qx, qz=qz)
-        orc.advect_centered(adv, f, u, w, h, "quick", qx=qx, qz=qz)
-        orc.diffus_centered(dff, f, k, h, qx=qx, qz=qz)
-    dt = time.perf_counter() - t0
-    return {"value": round(1.0 / dt, 4), "unit": "steps/s", "cores": cores, "kind": "port",
-            "sample": f"one full step ({NFIELDS} fields x WENO5+QUICK advect_centered_with_flux! + "
-                      f"diffus_centered_with_flux!) at {N}^2 on the C oracle, {dt:.2f} s"}
+    orc.run_step(hph, hfl, hns, g, par, sc, st, fixed_sweeps=1, seed=par.seed)
+    t_sweep = time.perf_counter() - t0
+    if not np.isfinite(hph.x).all():
+        raise RuntimeError("oracle sweep produced non-finite values")
+    spp = max(sweeps_per_step, 1.0)
+    return {"value": round(1.0 / (t_sweep * spp), 5), "unit": "steps/s", "cores": cores, "kind": "port",
+            "sample": f"one Picard sweep (phsevo!, fluidmech! by sparse LU, noise!, update_phase_velocities!, "
+                      f"update!) at {N}^2 on the C oracle took {t_sweep:.2f} s; steps/s = 1 / (sweep time x "
+                      f"{spp:.2f} sweeps/step of the GPU run); grid {N}^2, not 512^2"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=N_STENCIL)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--stencil-n", type=int, default=N_STENCIL)
+    ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-stencil", action="store_true")
+    ap.add_argument("--no-step", action="store_true", help="skip the full-step bench (profiling runs)")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="before the stencil bench, copy a buffer of this many MiB three times (a known-size "
+                         "kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "
-              "the hot path would shard as described in DESIGN.md", file=sys.stderr)
+              "multi-GPU runs are not part of it", file=sys.stderr)
         sys.exit(2)
-    ms, roof, kernels = gpu_bench(args)
+    if args.no_step:
+        import torch
+        import xcore_amd as xc
+        xc.load()
+        torch.cuda.set_device(0)
+        ms, info = float("nan"), {"sweeps_per_step": 1.0}
+    else:
+        print(f"bench: full step {args.n}^2, {args.warmup} warmup + {args.steps} timed", file=sys.stderr, flush=True)
+        ms, info = step_bench(args)
+        print(f"bench: {ms:.2f} ms/step {info}", file=sys.stderr, flush=True)
+    if args.calib_copy_mb > 0:
+        import torch
+        n = args.calib_copy_mb * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(f"bench: calibration copy of {args.calib_copy_mb} MiB x 3 done", file=sys.stderr, flush=True)
+        del a, b
     out = {
         "metric": METRIC,
-        "value": round(1e3 / ms, 3),
+        "value": round(1e3 / ms, 4),
         "unit": "steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms, 4),
+        "ms_per_step": round(ms, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": f"stencil microbench {args.n}^2 FP64: {NFIELDS} fields x (WENO5 + QUICK "
-                               "advect_centered_with_flux!, lognormal-k diffus_centered_with_flux!), "
+        "data": "synthetic (initialize! with a seeded perturbation; no dataset)",
+        "config": {"workload": f"full run! time step at {args.n}^2 FP64, plain FGMRES + block multigrid Stokes "
+                               "to 1e-8 relative residual, etacntr 1e3, rhox0 - rhom0 = 200, WENO5, bd2im, "
                                "x periodic / z closed",
-                   "grid": [args.n, args.n], "fields": NFIELDS},
-        "roofline": roof,
-        "kernels": kernels,
+                   "grid": [args.n, args.n], "stokes_preconditioner": "multigrid", **info},
     }
+    if not args.no_stencil:
+        print(f"bench: stencil microbench {args.stencil_n}^2", file=sys.stderr, flush=True)
+        roof, kernels = stencil_bench(args.stencil_n, 20, 3)
+        out["roofline"] = roof
+        out["stencil_kernels"] = kernels
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.n)
-    print(json.dumps(out))
+        print(f"bench: CPU oracle sweep at {args.cpu_n}^2", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_n, info["sweeps_per_step"])
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -21,6 +21,7 @@
 
 #include <vector>
 
+#include "xc_fft.h"
 #include "xc_internal.h"
 
 #pragma clang fp contract(off)
@@ -29,8 +30,6 @@ namespace {
 
 constexpr int NT = 256;
 constexpr double EPS = 2.220446049250313e-16;
-constexpr int MAXLINE = 2048;  // complex points per LDS buffer (64 KiB ping-pong)
-constexpr int MAXST = 24;
 
 // ---------------------------------------------------------------- RNG
 __device__ __forceinline__ void philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
@@ -105,128 +104,6 @@ __global__ __launch_bounds__(NT) void k_ou(OUArgs a) {
     a.psie[j + i * a.lde] = Fte * a.psieo[j + i * a.ldeo] + sqrt(ge) * sge * a.fL * re;
     a.psix[j + i * a.ldx] = Fte * a.psixo[j + i * a.ldxo] + sqrt(ge) * sgx * a.fL * re;
     a.psis[j + i * a.lds] = Fts * a.psiso[j + i * a.ldso] + sqrt(gs) * sgs * a.fl * rs;
-}
-
-// ---------------------------------------------------------------- LDS FFT
-struct Plan {
-    int n, nst;
-    int rad[MAXST];
-    const double2* tw;  // W_n^t = exp(-2 pi i t / n), t in [0, n)
-    const double* g;    // filter multiplier exp(-s^2 k_t^2) / n, t in [0, n)
-};
-
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-
-// One Stockham stage of radix R over nl lines of length n (src -> dst).
-template <int R, bool INV>
-__device__ __forceinline__ void stage(const double2* __restrict__ src, double2* __restrict__ dst, int n, int nl,
-                                      int Ns, const double2* __restrict__ tw) {
-    const int nR = n / R;
-    const int strideT = n / (Ns * R);
-    double2 wr[R];
-#pragma unroll
-    for (int m = 0; m < R; ++m) {
-        double2 w = tw[m * nR];  // W_R^m = W_n^{m n / R}
-        wr[m] = INV ? make_double2(w.x, -w.y) : w;
-    }
-    for (int k = threadIdx.x; k < nl * nR; k += NT) {
-        const int l = k / nR, j = k - l * nR;
-        const int base = l * n;
-        const int jm = j % Ns;
-        const int t0 = jm * strideT;
-        double2 v[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            double2 x = src[base + j + r * nR];
-            if (r > 0) {
-                double2 w = tw[(t0 * r) % n];
-                if (INV) w.y = -w.y;
-                x = cmul(x, w);
-            }
-            v[r] = x;
-        }
-        double2 o[R];
-        if constexpr (R == 2) {
-            o[0] = cadd(v[0], v[1]);
-            o[1] = csub(v[0], v[1]);
-        } else if constexpr (R == 4) {
-            const double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
-            const double2 s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
-            // forward: W_4 = -i
-            const double2 jd13 = INV ? make_double2(-d13.y, d13.x) : make_double2(d13.y, -d13.x);
-            o[0] = cadd(s02, s13);
-            o[2] = csub(s02, s13);
-            o[1] = cadd(d02, jd13);
-            o[3] = csub(d02, jd13);
-        } else {
-#pragma unroll
-            for (int q = 0; q < R; ++q) {
-                double2 s = v[0];
-#pragma unroll
-                for (int r = 1; r < R; ++r) s = cadd(s, cmul(v[r], wr[(r * q) % R]));
-                o[q] = s;
-            }
-        }
-        const int idxD = (j / Ns) * Ns * R + jm;
-#pragma unroll
-        for (int r = 0; r < R; ++r) dst[base + idxD + r * Ns] = o[r];
-    }
-}
-
-// Stage of any radix R (remaining prime factors): one output per lane,
-// out[q] = sum_r x[j + r n/R] W_n^{r (t0 + q n/R)}.
-template <bool INV>
-__device__ __forceinline__ void stage_generic(const double2* __restrict__ src, double2* __restrict__ dst, int n,
-                                              int nl, int Ns, int R, const double2* __restrict__ tw) {
-    const int nR = n / R;
-    const int strideT = n / (Ns * R);
-    for (int k = threadIdx.x; k < nl * n; k += NT) {
-        const int l = k / n, rem = k - l * n;
-        const int j = rem / R, q = rem - j * R;
-        const int base = l * n;
-        const int jm = j % Ns;
-        const int e = jm * strideT + q * nR;
-        double2 s = make_double2(0.0, 0.0);
-        int t = 0;
-        for (int r = 0; r < R; ++r) {
-            double2 w = tw[t];
-            if (INV) w.y = -w.y;
-            s = cadd(s, cmul(src[base + j + r * nR], w));
-            t += e;
-            if (t >= n) t -= n * (t / n);
-        }
-        dst[base + (j / Ns) * Ns * R + jm + q * Ns] = s;
-    }
-}
-
-template <bool INV>
-__device__ double2* lds_fft(double2* a, double2* b, int nl, const Plan& p) {
-    double2* src = a;
-    double2* dst = b;
-    int Ns = 1;
-    for (int s = 0; s < p.nst; ++s) {
-        const int R = p.rad[s];
-        switch (R) {
-            case 2: stage<2, INV>(src, dst, p.n, nl, Ns, p.tw); break;
-            case 3: stage<3, INV>(src, dst, p.n, nl, Ns, p.tw); break;
-            case 4: stage<4, INV>(src, dst, p.n, nl, Ns, p.tw); break;
-            case 5: stage<5, INV>(src, dst, p.n, nl, Ns, p.tw); break;
-            case 7: stage<7, INV>(src, dst, p.n, nl, Ns, p.tw); break;
-            case 11: stage<11, INV>(src, dst, p.n, nl, Ns, p.tw); break;
-            case 13: stage<13, INV>(src, dst, p.n, nl, Ns, p.tw); break;
-            default: stage_generic<INV>(src, dst, p.n, nl, Ns, R, p.tw); break;
-        }
-        __syncthreads();
-        double2* t = src;
-        src = dst;
-        dst = t;
-        Ns *= R;
-    }
-    return src;
 }
 
 struct FieldIO {
@@ -496,22 +373,6 @@ void noise_free(xc_ctx*, void* p) {
     delete nc;
 }
 
-bool factorize(int n, std::vector<int>& rad) {
-    rad.clear();
-    static const int order[] = {4, 2, 3, 5, 7, 11, 13};
-    for (int r : order)
-        while (n % r == 0) {
-            rad.push_back(r);
-            n /= r;
-        }
-    for (int p = 17; n > 1 && p <= n; p += 2)  // remaining primes: generic stages
-        while (n % p == 0) {
-            rad.push_back(p);
-            n /= p;
-        }
-    return n == 1 && (int)rad.size() <= MAXST;
-}
-
 // fftfreq(n, 1/h)[t] * 2 pi (AbstractFFTs ordering)
 double kfreq(int t, int n, double h) {
     const int nn = (n + 1) / 2;
@@ -539,14 +400,13 @@ int get_plan(xc_ctx* ctx, int n, double sigma, double h, Plan* out) {
     e.n = n;
     e.sigma = sigma;
     e.h = h;
-    if (n > MAXLINE || !factorize(n, e.rad))
+    if (n > MAXLINE || !fft_factorize(n, e.rad))
         return xc_fail(ctx, XC_EINVAL_SHAPE,
                        "noise filter: line length %d must be <= %d", n, MAXLINE);
-    std::vector<double2> tw(n);
+    std::vector<double2> tw;
+    fft_twiddles(n, tw);
     std::vector<double> g(n);
     for (int t = 0; t < n; ++t) {
-        const long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)t / (long double)n;
-        tw[t] = make_double2((double)cosl(ang), (double)sinl(ang));
         const double k = kfreq(t, n, h);
         g[t] = exp(-(sigma * sigma) * (k * k)) / (double)n;  // inverse-transform normalisation folded in
     }

@@ -7,9 +7,14 @@
 // and pins, and the system is solved by right-preconditioned FGMRES.
 //
 // Preconditioner (applied to a residual (r_v, r_p)):
-//   z_p = S^-1 r_p   with S ~ KP - DM A^-1 GG ~ +rho/eta on interior cells (the
-//                    reference's own scaling uses sqrt(eta/rho) there,
-//                    fluidmech.jl:497-500); ghost rows solved exactly from KP;
+//   z_p = S^-1 r_p   Cahouet-Chabard form of the Schur complement
+//                    S = KP - DM A^-1 GG with A = a1 rho / dt - div(eta grad):
+//                    S^-1 ~ eta/rho (viscous part, diagonal; the reference's own
+//                    scaling uses sqrt(eta/rho) there, fluidmech.jl:497-500)
+//                         + T^-1 (inertial part, T = -DM (a1 rho/dt)^-1 GG, a
+//                    constant-coefficient Laplacian solved exactly by FFT in x
+//                    and a tridiagonal sweep in z); ghost rows solved exactly
+//                    from KP;
 //   z_v = A^-1 (r_v - GG z_p) by one geometric-multigrid V-cycle on the
 //                    velocity block A = KV: coarse operators rediscretised from
 //                    geometric-mean viscosities, staggered full-weighting
@@ -21,6 +26,7 @@
 
 #include <vector>
 
+#include "xc_fft.h"
 #include "xc_internal.h"
 
 namespace {
@@ -167,12 +173,9 @@ __global__ __launch_bounds__(NT) void k_ll_resid(LevD L, Flags F, const double* 
 }
 
 // damped Jacobi on the velocity block: xo = x + w (b - A x) / diag (BC rows with weight 1)
-__global__ __launch_bounds__(NT) void k_jacobi(LevD L, Flags F, const double* __restrict__ x,
-                                               const double* __restrict__ b, double* __restrict__ xo, double omega,
-                                               int fine, int zero_guess) {
-    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
-    const int64_t n = L.nW() + L.nU();
-    if (k >= n) return;
+__device__ __forceinline__ void jacobi_at(const LevD& L, const Flags& F, const double* __restrict__ x,
+                                          const double* __restrict__ b, double* __restrict__ xo, double omega,
+                                          int fine, int zero_guess, int64_t k) {
     double dg;
     bool bc;
     if (zero_guess) {  // x == 0: A x == 0, diag from the row formula evaluated on x (all zeros)
@@ -183,22 +186,30 @@ __global__ __launch_bounds__(NT) void k_jacobi(LevD L, Flags F, const double* __
     const double ax = vel_row(L, F, x, k, &dg, &bc, fine != 0);
     xo[k] = x[k] + (bc ? 1.0 : omega) * (b[k] - ax) / dg;
 }
-
-__global__ __launch_bounds__(NT) void k_vel_resid(LevD L, Flags F, const double* __restrict__ x,
-                                                  const double* __restrict__ b, double* __restrict__ r, int fine) {
+__global__ __launch_bounds__(NT) void k_jacobi(LevD L, Flags F, const double* __restrict__ x,
+                                               const double* __restrict__ b, double* __restrict__ xo, double omega,
+                                               int fine, int zero_guess) {
     const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
-    if (k >= L.nW() + L.nU()) return;
+    if (k < L.nW() + L.nU()) jacobi_at(L, F, x, b, xo, omega, fine, zero_guess, k);
+}
+
+__device__ __forceinline__ void vel_resid_at(const LevD& L, const Flags& F, const double* __restrict__ x,
+                                             const double* __restrict__ b, double* __restrict__ r, int fine,
+                                             int64_t k) {
     double dg;
     bool bc;
     r[k] = b[k] - vel_row(L, F, x, k, &dg, &bc, fine != 0);
 }
+__global__ __launch_bounds__(NT) void k_vel_resid(LevD L, Flags F, const double* __restrict__ x,
+                                                  const double* __restrict__ b, double* __restrict__ r, int fine) {
+    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (k < L.nW() + L.nU()) vel_resid_at(L, F, x, b, r, fine, k);
+}
 
 // staggered full-weighting restriction of a velocity residual rf (level f) into bc (level c)
-__global__ __launch_bounds__(NT) void k_restrict(LevD Lf, LevD Lc, const double* __restrict__ rf,
-                                                 double* __restrict__ bcv) {
-    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
-    const int64_t nWc = Lc.nW(), n = nWc + Lc.nU();
-    if (k >= n) return;
+__device__ __forceinline__ void restrict_at(const LevD& Lf, const LevD& Lc, const double* __restrict__ rf,
+                                            double* __restrict__ bcv, int64_t k) {
+    const int64_t nWc = Lc.nW();
     double v = 0.0;
     if (k < nWc) {
         const int J = (int)(k % (Lc.Nz + 1)), I = (int)(k / (Lc.Nz + 1));
@@ -220,13 +231,16 @@ __global__ __launch_bounds__(NT) void k_restrict(LevD Lf, LevD Lc, const double*
     }
     bcv[k] = v;
 }
+__global__ __launch_bounds__(NT) void k_restrict(LevD Lf, LevD Lc, const double* __restrict__ rf,
+                                                 double* __restrict__ bcv) {
+    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (k < Lc.nW() + Lc.nU()) restrict_at(Lf, Lc, rf, bcv, k);
+}
 
 // bilinear prolongation of a coarse correction ec, added to the fine iterate xf (interior rows)
-__global__ __launch_bounds__(NT) void k_prolong(LevD Lf, LevD Lc, Flags F, const double* __restrict__ ec,
-                                                double* __restrict__ xf) {
-    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
-    const int64_t nWf = Lf.nW(), n = nWf + Lf.nU();
-    if (k >= n) return;
+__device__ __forceinline__ void prolong_at(const LevD& Lf, const LevD& Lc, const Flags& F,
+                                           const double* __restrict__ ec, double* __restrict__ xf, int64_t k) {
+    const int64_t nWf = Lf.nW();
     if (k < nWf) {
         const int r = (int)(k % (Lf.Nz + 1)), c = (int)(k / (Lf.Nz + 1));
         if (r < 1 || r > Lf.Nz - 1 || c < 1 || c > Lf.Nx) return;
@@ -262,6 +276,67 @@ __global__ __launch_bounds__(NT) void k_prolong(LevD Lf, LevD Lc, Flags F, const
         const int I = c / 2;
         const double v = (c & 1) ? 0.5 * (EZ(I) + EZ(I + 1)) : EZ(I);
         xf[k] += v;
+    }
+}
+__global__ __launch_bounds__(NT) void k_prolong(LevD Lf, LevD Lc, Flags F, const double* __restrict__ ec,
+                                                double* __restrict__ xf) {
+    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (k < Lf.nW() + Lf.nU()) prolong_at(Lf, Lc, F, ec, xf, k);
+}
+
+// The coarse end of the V-cycle (levels of at most TAIL_CELLS cells) in one
+// workgroup: the sweeps, residuals and transfers of these small levels are
+// latency-bound, so they run back to back with workgroup barriers instead of
+// one launch each.  Same arithmetic as the per-level kernels.
+constexpr int TAIL_NT = 1024;
+constexpr int TAIL_MAX = 6;
+constexpr int64_t TAIL_CELLS = 32 * 32;
+struct TailArgs {
+    LevD d[TAIL_MAX];
+    double *x[TAIL_MAX], *b[TAIL_MAX], *r[TAIL_MAX], *t[TAIL_MAX];
+    int nlev, pre, post, coarse;
+    double omega;
+    Flags F;
+};
+__global__ __launch_bounds__(TAIL_NT) void k_mg_tail(TailArgs a) {
+    double* xp[TAIL_MAX];
+    double* tp[TAIL_MAX];
+    for (int l = 0; l < a.nlev; ++l) { xp[l] = a.x[l]; tp[l] = a.t[l]; }
+    const int tid = threadIdx.x;
+    // down: smooth from zero, residual, restrict
+    for (int l = 0; l < a.nlev; ++l) {
+        const LevD& L = a.d[l];
+        const int64_t nv = L.nW() + L.nU();
+        const bool coarsest = l + 1 == a.nlev;
+        const int sweeps = coarsest ? a.coarse : a.pre;
+        for (int it = 0; it < sweeps; ++it) {
+            for (int64_t k = tid; k < nv; k += TAIL_NT) jacobi_at(L, a.F, xp[l], a.b[l], tp[l], a.omega, 0, it == 0, k);
+            __syncthreads();
+            double* q = xp[l]; xp[l] = tp[l]; tp[l] = q;
+        }
+        if (coarsest) break;
+        for (int64_t k = tid; k < nv; k += TAIL_NT) vel_resid_at(L, a.F, xp[l], a.b[l], a.r[l], 0, k);
+        __syncthreads();
+        const LevD& C = a.d[l + 1];
+        const int64_t nc = C.nW() + C.nU();
+        for (int64_t k = tid; k < nc; k += TAIL_NT) restrict_at(L, C, a.r[l], a.b[l + 1], k);
+        __syncthreads();
+    }
+    // up: prolong and post-smooth
+    for (int l = a.nlev - 2; l >= 0; --l) {
+        const LevD& L = a.d[l];
+        const int64_t nv = L.nW() + L.nU();
+        for (int64_t k = tid; k < nv; k += TAIL_NT) prolong_at(L, a.d[l + 1], a.F, xp[l + 1], xp[l], k);
+        __syncthreads();
+        for (int it = 0; it < a.post; ++it) {
+            for (int64_t k = tid; k < nv; k += TAIL_NT) jacobi_at(L, a.F, xp[l], a.b[l], tp[l], a.omega, 0, 0, k);
+            __syncthreads();
+            double* q = xp[l]; xp[l] = tp[l]; tp[l] = q;
+        }
+    }
+    if (xp[0] != a.x[0]) {  // leave the result where the caller expects the level's iterate
+        const int64_t nv = a.d[0].nW() + a.d[0].nU();
+        for (int64_t k = tid; k < nv; k += TAIL_NT) a.x[0][k] = xp[0][k];
     }
 }
 
@@ -356,6 +431,129 @@ __global__ __launch_bounds__(NT) void k_sub_grad(LevD L, Flags F, const double* 
     tV[k] = rV[k] - g;
 }
 
+// ---- inertial part of the Schur complement: a pressure Poisson solve ------
+// With the mass terms M = a1 rho_f / dt on the interior velocity rows, the
+// face densities cancel exactly in -DM M^-1 GG, leaving the constant-coefficient
+// 5-point operator T = cz (-d_zz) + cx (-d_xx), cz = dt / (a1 h^2),
+// cx = dt / ((a1 + gamma) h^2), periodic in x and zero-flux at the top and
+// bottom rows (W there is a BC row).  T is diagonalised in x by a real FFT and
+// solved in z by a tridiagonal (Thomas) sweep per wavenumber whose pivots are
+// tabulated on the host; the kx = 0 system is singular (constant pressure) and
+// is solved with the mean removed.
+
+// rows of the interior P residual -> half spectra S[r][k], k = 0..Nx/2 (two rows per complex line)
+struct PoisXArgs {
+    const double* rP;   // P block of the residual, (Nz+2) x (Nx+2)
+    double2* S;         // Nz x K, k fastest
+    Plan p;
+    int Nz, Nx, K, nl;
+};
+__global__ __launch_bounds__(FFT_NT) void k_pois_xfwd(PoisXArgs a) {
+    extern __shared__ double2 sm[];
+    const int n = a.Nx, nl = a.nl, rows = 2 * nl;
+    const int r0 = blockIdx.x * rows;
+    double2* A = sm;
+    double2* B = sm + nl * n;
+    for (int k = threadIdx.x; k < rows * n; k += FFT_NT) {
+        const int rr = k % rows, c = k / rows;
+        const int r = r0 + rr;
+        const double v = r < a.Nz ? a.rP[(r + 1) + (int64_t)(c + 1) * (a.Nz + 2)] : 0.0;
+        reinterpret_cast<double*>(&A[(rr >> 1) * n + c])[rr & 1] = v;
+    }
+    __syncthreads();
+    const double2* X = lds_fft<false>(A, B, nl, a.p);
+    for (int k = threadIdx.x; k < nl * a.K; k += FFT_NT) {
+        const int l = k / a.K, f = k - l * a.K;
+        const double2 zk = X[l * n + f], zm = X[l * n + (f == 0 ? 0 : n - f)];
+        const int ra = r0 + 2 * l, rb = ra + 1;
+        if (ra < a.Nz) a.S[(int64_t)ra * a.K + f] = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
+        if (rb < a.Nz) a.S[(int64_t)rb * a.K + f] = make_double2(0.5 * (zk.y + zm.y), -0.5 * (zk.x - zm.x));
+    }
+}
+
+// tridiagonal solve along z of every wavenumber column, in place:
+//   -p[r-1] + b_r p[r] - p[r+1] = d[r]; invw[r*K + k] = 1 / pivot (0 at the pinned row of k = 0)
+__global__ __launch_bounds__(64) void k_pois_z(double2* __restrict__ S, const double* __restrict__ invw, int Nz, int K) {
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    if (k >= K) return;
+    double2* col = S + k;
+    if (k == 0) {  // remove the mean of the right-hand side (kx = 0 is real)
+        double sum = 0.0;
+        for (int r = 0; r < Nz; ++r) sum += col[(int64_t)r * K].x;
+        const double mean = sum / (double)Nz;
+        for (int r = 0; r < Nz; ++r) col[(int64_t)r * K] = make_double2(col[(int64_t)r * K].x - mean, 0.0);
+    }
+    double2 y = col[0];
+    double iw = invw[k];
+    for (int r = 1; r < Nz; ++r) {
+        double2 d = col[(int64_t)r * K];
+        d.x += y.x * iw;
+        d.y += y.y * iw;
+        col[(int64_t)(r - 1) * K] = y;  // keep y[r-1] for the back substitution
+        y = d;
+        iw = invw[(int64_t)r * K + k];
+    }
+    double2 pn = make_double2(y.x * iw, y.y * iw);
+    col[(int64_t)(Nz - 1) * K] = pn;
+    double sum = pn.x;
+    for (int r = Nz - 2; r >= 0; --r) {
+        const double2 yr = col[(int64_t)r * K];
+        const double w = invw[(int64_t)r * K + k];
+        pn = make_double2((yr.x + pn.x) * w, (yr.y + pn.y) * w);
+        col[(int64_t)r * K] = pn;
+        sum += pn.x;
+    }
+    if (k == 0) {  // the pinned solution differs from the zero-mean one by a constant
+        const double mean = sum / (double)Nz;
+        for (int r = 0; r < Nz; ++r) col[(int64_t)r * K] = make_double2(col[(int64_t)r * K].x - mean, 0.0);
+    }
+}
+
+// inverse x transform of two rows per complex line and the whole interior Schur apply:
+//   zP = rP / s (s = rho/eta; 1 + s at a kept-DM pin; identity at an MMS pin) + scale * T^-1 rP
+struct PoisInvArgs {
+    const double2* S;
+    const double* rP;
+    double* zP;
+    Plan p;
+    LevD L;
+    Flags F;
+    double scale;   // 1 / (cz Nx)
+    int Nz, Nx, K, nl;
+};
+__global__ __launch_bounds__(FFT_NT) void k_pois_xinv(PoisInvArgs a) {
+    extern __shared__ double2 sm[];
+    const int n = a.Nx, nl = a.nl, rows = 2 * nl, K = a.K;
+    const int r0 = blockIdx.x * rows;
+    double2* A = sm;
+    double2* B = sm + nl * n;
+    for (int k = threadIdx.x; k < nl * n; k += FFT_NT) {
+        const int l = k / n, f = k - l * n;
+        const int ra = r0 + 2 * l, rb = ra + 1;
+        const bool lo = f < K;
+        const int fs = lo ? f : n - f;
+        double2 av = ra < a.Nz ? a.S[(int64_t)ra * K + fs] : make_double2(0.0, 0.0);
+        double2 bv = rb < a.Nz ? a.S[(int64_t)rb * K + fs] : make_double2(0.0, 0.0);
+        if (!lo) { av.y = -av.y; bv.y = -bv.y; }
+        A[k] = make_double2(av.x - bv.y, av.y + bv.x);
+    }
+    __syncthreads();
+    const double2* Y = lds_fft<true>(A, B, nl, a.p);
+    const int Nz = a.Nz;
+    for (int k = threadIdx.x; k < rows * n; k += FFT_NT) {
+        const int rr = k % rows, c = k / rows;
+        const int r = r0 + rr;
+        if (r >= Nz) continue;
+        const double pv = reinterpret_cast<const double*>(&Y[(rr >> 1) * n + c])[rr & 1];
+        const int64_t q = (r + 1) + (int64_t)(c + 1) * (Nz + 2);
+        const double sv = a.L.RHO(r, c) / a.L.ETA(r, c);
+        double z;
+        if (is_pin(a.F, r + 1, c + 1, Nz)) z = a.F.bnchm ? a.rP[q] : a.rP[q] / (1.0 + sv);
+        else z = a.rP[q] / sv;
+        a.zP[q] = z + a.scale * pv;
+    }
+}
+
 // ---- vector kernels for FGMRES ------------------------------------------
 // partial dot products of w against K basis vectors V[0..K) (stride n) and, as entry K, w.w
 constexpr int DOT_BLOCKS = 1024;
@@ -412,6 +610,57 @@ __global__ __launch_bounds__(NT) void k_update_x(const double* __restrict__ Z, i
     double s = x[i];
     for (int q = 0; q < K; ++q) s += y[q] * Z[(int64_t)q * n + i];
     x[i] = s;
+}
+// ---- Arnoldi bookkeeping on the device (one lane; m <= 60) -------------
+// kry layout: H (m+1) x m column-major | cs[m] | sn[m] | g[m+1] | y[m]
+struct Kry {
+    double *H, *cs, *sn, *g, *y;
+};
+__global__ void k_kry_init(Kry K, int m, double beta) {
+    for (int i = threadIdx.x; i <= m; i += blockDim.x) K.g[i] = i == 0 ? beta : 0.0;
+}
+// column j of H from the two Gram-Schmidt passes and the norm; Givens update;
+// |g[j+1]| and H[j+1,j] go to host-visible memory for the convergence test
+__global__ void k_arnoldi(Kry K, int m, int j, const double* __restrict__ d1, const double* __restrict__ d2,
+                          double* __restrict__ nrm2, double* __restrict__ host_out) {
+    if (threadIdx.x != 0) return;
+    double* Hj = K.H + (int64_t)j * (m + 1);
+    for (int i = 0; i <= j; ++i) Hj[i] = d1[i] + d2[i];
+    const double hn = sqrt(nrm2[0]);
+    nrm2[1] = hn;  // the new basis vector is w / hn (H[j+1, j] is rotated to zero below)
+    Hj[j + 1] = hn;
+    for (int i = 0; i < j; ++i) {
+        const double a = Hj[i], b = Hj[i + 1];
+        Hj[i] = K.cs[i] * a + K.sn[i] * b;
+        Hj[i + 1] = -K.sn[i] * a + K.cs[i] * b;
+    }
+    const double a = Hj[j], b = Hj[j + 1];
+    const double rr = sqrt(a * a + b * b);
+    const double c = rr > 0 ? a / rr : 1.0, sv = rr > 0 ? b / rr : 0.0;
+    K.cs[j] = c;
+    K.sn[j] = sv;
+    Hj[j] = rr;
+    Hj[j + 1] = 0.0;
+    const double gj = K.g[j];
+    K.g[j + 1] = -sv * gj;
+    K.g[j] = c * gj;
+    host_out[2 * j] = fabs(K.g[j + 1]);
+    host_out[2 * j + 1] = hn;
+}
+// v = w / hn
+__global__ __launch_bounds__(NT) void k_scale_h(const double* __restrict__ w, const double* __restrict__ hp,
+                                                double* __restrict__ v, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (i < n) v[i] = w[i] / hp[0];
+}
+// y = H(0:j, 0:j)^-1 g(0:j) (upper triangular after the rotations)
+__global__ void k_trisolve(Kry K, int m, int j) {
+    if (threadIdx.x != 0) return;
+    for (int i = j - 1; i >= 0; --i) {
+        double sum = K.g[i];
+        for (int k = i + 1; k < j; ++k) sum -= K.H[i + (int64_t)k * (m + 1)] * K.y[k];
+        K.y[i] = sum / K.H[i + (int64_t)i * (m + 1)];
+    }
 }
 __global__ __launch_bounds__(NT) void k_zero(double* a, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
@@ -595,13 +844,59 @@ struct StokesCtx {
     double* pool = nullptr;   // everything below carved from one allocation
     size_t pool_bytes = 0;
     double *xs, *bs, *rs, *ws, *V, *Z, *part, *dots, *tmp, *adv_buf;
-    double* host = nullptr;   // pinned host scratch for reductions
+    double* host = nullptr;   // pinned host scratch for reductions (mapped, coherent)
+    double* host_dev = nullptr;  // the same memory as seen by kernels
+    double* kry = nullptr;    // Arnoldi H, Givens, g, y (device)
+    double* d3 = nullptr;     // three dot-product result slots of (m + 2) each
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    // pressure Poisson (inertial Schur part)
+    int pK = 0;                 // Nx/2 + 1
+    double2* pS = nullptr;      // Nz x pK half spectra
+    double* pinvw = nullptr;    // Nz x pK inverse Thomas pivots
+    double2* ptw = nullptr;     // Nx twiddles
+    std::vector<int> prad;
+    double pratio = -1.0;       // cx / cz of the pivot table; <= 0: inertial part off for this solve
+    double ptab = -1.0;         // cx / cz the device table holds
+    double pscale = 0.0;        // a1 h^2 / (dt Nx)
 };
+
+// Decide whether the inertial Schur part applies to this solve and (re)build the Thomas pivots.
+int pois_table(xc_ctx* ctx, StokesCtx* s, const Flags& F, int Nz, int Nx, double h) {
+    s->pratio = -1.0;
+    if (F.bnchm || !(F.dt < 1e30) || !(F.a1 > 0) || s->prad.empty()) return XC_OK;
+    const double ratio = F.a1 / (F.a1 + F.gamma);
+    s->pscale = F.a1 * h * h / (F.dt * (double)Nx);
+    s->pratio = ratio;
+    if (ratio == s->ptab) return XC_OK;
+    const int K = s->pK;
+    std::vector<double> iw((size_t)Nz * K);
+    for (int k = 0; k < K; ++k) {
+        const long double ang = 2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)Nx;
+        const long double rho = (long double)ratio * (2.0L - 2.0L * cosl(ang));
+        long double w = 0.0L;
+        for (int r = 0; r < Nz; ++r) {
+            const long double b = (Nz == 1) ? rho : ((r == 0 || r == Nz - 1) ? 1.0L + rho : 2.0L + rho);
+            if (k == 0 && r == 0) {  // pinned row of the singular kx = 0 system
+                iw[(size_t)r * K + k] = 0.0;
+                w = 0.0L;
+                continue;
+            }
+            w = (r == 0 || (k == 0 && r == 1)) ? b : b - 1.0L / w;
+            iw[(size_t)r * K + k] = (double)(1.0L / w);
+        }
+    }
+    XC_TRY_HIP(ctx, hipMemcpyAsync(s->pinvw, iw.data(), iw.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    s->ptab = ratio;
+    return XC_OK;
+}
 
 void stokes_free(xc_ctx*, void* p) {
     StokesCtx* s = (StokesCtx*)p;
     if (s->pool) (void)hipFree(s->pool);
     if (s->host) (void)hipHostFree(s->host);
+    for (auto& e : s->ev)
+        if (e) (void)hipEventDestroy(e);
     delete s;
 }
 
@@ -630,8 +925,11 @@ int setup(xc_ctx* ctx, int Nz, int Nx, int m, StokesCtx** out) {
         tot += (size_t)L.Nz * L.Nx * 2 + (size_t)(L.Nz + 1) * (L.Nx + 1) + (size_t)(L.Nz + 1) * L.Nx +
                (size_t)L.Nz * (L.Nx + 1) + 4 * (size_t)L.nv;
     }
-    tot += 4 * (size_t)n + (size_t)(m + 1) * n + (size_t)m * n + (size_t)(m + 2) * DOT_BLOCKS + 64 +
-           (size_t)n /* tmp */ + 8 * (size_t)(Nz + 2) * (Nx + 3) /* advection scratch */;
+    const int K = Nx / 2 + 1;
+    tot += 4 * (size_t)n + (size_t)(m + 1) * n + (size_t)m * n + (size_t)(m + 2) * DOT_BLOCKS + 128 /* dots */ +
+           (size_t)n /* tmp */ + 8 * (size_t)(Nz + 2) * (Nx + 3) /* advection scratch */ +
+           3 * (size_t)Nz * K + 2 * (size_t)Nx + 8 /* Poisson */ +
+           (size_t)(m + 1) * m + 4 * (size_t)m + 8 /* Arnoldi */ + 3 * (size_t)(m + 2) /* dots */;
     void* pool;
     hipError_t e = hipMalloc(&pool, tot * sizeof(double));
     if (e != hipSuccess) { delete s; return xc_fail(ctx, XC_ENOMEM, "fluidmech!: %zu bytes: %s", tot * 8, hipGetErrorString(e)); }
@@ -654,8 +952,32 @@ int setup(xc_ctx* ctx, int Nz, int Nx, int m, StokesCtx** out) {
     s->dots = take(128);
     s->tmp = take(n);
     s->adv_buf = take(8 * (size_t)(Nz + 2) * (Nx + 3));
-    e = hipHostMalloc((void**)&s->host, 256 * sizeof(double), hipHostMallocDefault);
+    s->kry = take((size_t)(m + 1) * m + 4 * (size_t)m + 8);
+    s->d3 = take(3 * (size_t)(m + 2));
+    if ((p - s->pool) & 1) ++p;  // 16-byte alignment for the complex arrays
+    s->pK = K;
+    s->pS = reinterpret_cast<double2*>(take(2 * (size_t)Nz * K));
+    s->ptw = reinterpret_cast<double2*>(take(2 * (size_t)Nx));
+    s->pinvw = take((size_t)Nz * K);
+    if ((size_t)(p - s->pool) > tot) {  // carving overran the allocation: a sizing bug, never a user error
+        stokes_free(ctx, s);
+        return xc_fail(ctx, XC_EHIP, "fluidmech!: internal pool sizing (%zu > %zu)", (size_t)(p - s->pool), tot);
+    }
+    if (Nx > MAXLINE || !fft_factorize(Nx, s->prad)) s->prad.clear();
+    else {
+        std::vector<double2> tw;
+        fft_twiddles(Nx, tw);
+        e = hipMemcpy(s->ptw, tw.data(), Nx * sizeof(double2), hipMemcpyHostToDevice);
+        if (e != hipSuccess) { stokes_free(ctx, s); return xc_fail(ctx, XC_EHIP, "fluidmech!: %s", hipGetErrorString(e)); }
+    }
+    e = hipHostMalloc((void**)&s->host, 256 * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent);
     if (e != hipSuccess) { stokes_free(ctx, s); return xc_fail(ctx, XC_ENOMEM, "fluidmech!: pinned scratch"); }
+    e = hipHostGetDevicePointer((void**)&s->host_dev, s->host, 0);
+    if (e != hipSuccess) { stokes_free(ctx, s); return xc_fail(ctx, XC_EHIP, "fluidmech!: %s", hipGetErrorString(e)); }
+    for (auto& ev : s->ev) {
+        e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e != hipSuccess) { stokes_free(ctx, s); return xc_fail(ctx, XC_EHIP, "fluidmech!: event"); }
+    }
     ctx->ext[XC_EXT_STOKES] = s;
     ctx->ext_free[XC_EXT_STOKES] = stokes_free;
     *out = s;
@@ -710,6 +1032,19 @@ struct MGOpts { int pre, post, coarse; double omega; };
 // one V-cycle on level l: solve A x = b approximately from x = 0
 void vcycle(xc_ctx* ctx, StokesCtx* s, const Flags& F, size_t l, const MGOpts& mo) {
     Level& L = s->lev[l];
+    if (l > 0 && (int64_t)L.Nz * L.Nx <= TAIL_CELLS && s->lev.size() - l <= (size_t)TAIL_MAX) {
+        TailArgs ta{};
+        ta.nlev = (int)(s->lev.size() - l);
+        for (int q = 0; q < ta.nlev; ++q) {
+            Level& T = s->lev[l + q];
+            ta.d[q] = T.d();
+            ta.x[q] = T.x; ta.b[q] = T.b; ta.r[q] = T.r; ta.t[q] = T.t;
+        }
+        ta.pre = mo.pre; ta.post = mo.post; ta.coarse = mo.coarse; ta.omega = mo.omega;
+        ta.F = F;
+        hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(TAIL_NT), 0, ctx->stream, ta);
+        return;
+    }
     const LevD d = L.d();
     const unsigned nb = nblk(L.nv);
     const int fine = l == 0;
@@ -741,7 +1076,25 @@ void precond(xc_ctx* ctx, StokesCtx* s, const Flags& F, const double* v, double*
     const int64_t nv = L.nv, nP = (int64_t)(L.Nz + 2) * (L.Nx + 2);
     const double* vP = v + nv;
     double* zP = z + nv;
-    hipLaunchKernelGGL(k_schur_int, dim3(nblk(nP)), dim3(NT), 0, ctx->stream, d, F, vP, zP);
+    if (s->pratio > 0) {  // inertial Schur part active (see pois_table)
+        const int Nz = L.Nz, Nx = L.Nx, K = s->pK;
+        int nl = MAXLINE / Nx;
+        if (nl > 8) nl = 8;
+        Plan pl{};
+        pl.n = Nx;
+        pl.nst = (int)s->prad.size();
+        for (int q = 0; q < pl.nst; ++q) pl.rad[q] = s->prad[q];
+        pl.tw = s->ptw;
+        const unsigned nbx = (unsigned)((Nz + 2 * nl - 1) / (2 * nl));
+        const size_t lds = 2 * (size_t)nl * Nx * sizeof(double2);
+        PoisXArgs xa{vP, s->pS, pl, Nz, Nx, K, nl};
+        hipLaunchKernelGGL(k_pois_xfwd, dim3(nbx), dim3(FFT_NT), lds, ctx->stream, xa);
+        hipLaunchKernelGGL(k_pois_z, dim3((K + 63) / 64), dim3(64), 0, ctx->stream, s->pS, s->pinvw, Nz, K);
+        PoisInvArgs ia{s->pS, vP, zP, pl, d, F, s->pscale, Nz, Nx, K, nl};
+        hipLaunchKernelGGL(k_pois_xinv, dim3(nbx), dim3(FFT_NT), lds, ctx->stream, ia);
+    } else {
+        hipLaunchKernelGGL(k_schur_int, dim3(nblk(nP)), dim3(NT), 0, ctx->stream, d, F, vP, zP);
+    }
     hipLaunchKernelGGL(k_schur_ghost, dim3(nblk(nP)), dim3(NT), 0, ctx->stream, d, F, vP, zP);
     hipLaunchKernelGGL(k_sub_grad, dim3(nblk(nv)), dim3(NT), 0, ctx->stream, d, F, v, zP, L.b);
     vcycle(ctx, s, F, 0, mo);
@@ -756,6 +1109,12 @@ int multidot(xc_ctx* ctx, StokesCtx* s, const double* V, int K, const double* w,
     XC_TRY_HIP(ctx, hipMemcpyAsync(s->host, s->dots, (K + 1) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return XC_OK;
+}
+
+// out[0..K] = (V[q] . w) for q < K and w.w, left on the device
+void multidot_dev(xc_ctx* ctx, StokesCtx* s, const double* V, int K, const double* w, int64_t n, double* out) {
+    hipLaunchKernelGGL(k_multidot, dim3(DOT_BLOCKS), dim3(NT), 0, ctx->stream, V, n, K, w, s->part);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(NT), 0, ctx->stream, s->part, DOT_BLOCKS, K + 1, out);
 }
 
 int build_rhs(xc_ctx* ctx, StokesCtx* s, xc_fluid* fl, const xc_phase* ph, const xc_grid* g, const xc_par* par,
@@ -883,6 +1242,7 @@ int xc_fluidmech(xc_ctx* ctx, xc_fluid* fl, const xc_phase* ph, const xc_grid* g
               o->mg_omega > 0 ? o->mg_omega : 0.7};
     // system: coefficients, RR, initial guess
     build_levels(ctx, s, fl, g->h);
+    if ((st = pois_table(ctx, s, F, Nz, Nx, g->h))) return st;
     const LevD dd = s->lev[0].d();
     if ((st = build_rhs(ctx, s, fl, ph, g, par, o, bn, F, s->bs))) return st;
     hipLaunchKernelGGL(k_gather, dim3(nb), dim3(NT), 0, ctx->stream, fl->W, fl->U, fl->P, s->xs, Nz, Nx);
@@ -894,7 +1254,21 @@ int xc_fluidmech(xc_ctx* ctx, xc_fluid* fl, const xc_phase* ph, const xc_grid* g
     int iters = 0;
     double rel = 0.0, rel0 = -1.0;
     bool conv = false;
-    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), y(m);
+    // The Arnoldi steps run on the device without host round trips: the Gram-Schmidt
+    // coefficients, the Hessenberg column and the Givens rotations never leave the GPU.
+    // The host learns the residual estimate |g[j+1]| of step j only after it has
+    // enqueued step j+1, so the device never waits for the host; a converged cycle
+    // therefore computes one basis vector it does not use.
+    Kry K{};
+    K.H = s->kry;
+    K.cs = K.H + (size_t)(m + 1) * m;
+    K.sn = K.cs + m;
+    K.g = K.sn + m;
+    K.y = K.g + m + 1;
+    double* d1 = s->d3;
+    double* d2 = d1 + (m + 2);
+    double* dn = d2 + (m + 2);
+    volatile double* hout = s->host + 8;  // 2 per step: |g[j+1]|, H[j+1, j]
     while (true) {
         hipLaunchKernelGGL(k_ll_resid, dim3(nb), dim3(NT), 0, ctx->stream, dd, F, s->xs, s->bs, s->rs, n);
         if ((st = multidot(ctx, s, s->rs, 0, s->rs, n))) return st;
@@ -904,54 +1278,38 @@ int xc_fluidmech(xc_ctx* ctx, xc_fluid* fl, const xc_phase* ph, const xc_grid* g
         if (beta <= target) { conv = true; break; }
         if (iters >= maxit) break;
         hipLaunchKernelGGL(k_scale, dim3(nb), dim3(NT), 0, ctx->stream, s->rs, 1.0 / beta, s->V, n);
-        std::fill(gv.begin(), gv.end(), 0.0);
-        gv[0] = beta;
+        hipLaunchKernelGGL(k_kry_init, dim3(1), dim3(64), 0, ctx->stream, K, m, beta);
+        int used = 0;  // basis columns whose results are final
         int j = 0;
-        for (; j < m && iters < maxit; ++j) {
+        for (; j < m; ++j) {
             double* vj = s->V + (int64_t)j * n;
             double* zj = s->Z + (int64_t)j * n;
             precond(ctx, s, F, vj, zj, mo);
             hipLaunchKernelGGL(k_ll_apply, dim3(nb), dim3(NT), 0, ctx->stream, dd, F, zj, s->ws, n);
-            // classical Gram-Schmidt, twice
-            for (int pass = 0; pass < 2; ++pass) {
-                if ((st = multidot(ctx, s, s->V, j + 1, s->ws, n))) return st;
-                for (int i = 0; i <= j; ++i) {
-                    if (pass == 0) H[i + (size_t)j * (m + 1)] = s->host[i];
-                    else H[i + (size_t)j * (m + 1)] += s->host[i];
-                }
-                XC_TRY_HIP(ctx, hipMemcpyAsync(s->dots, s->host, (j + 1) * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-                hipLaunchKernelGGL(k_orth, dim3(nb), dim3(NT), 0, ctx->stream, s->V, n, j + 1, s->dots, s->ws);
-            }
-            if ((st = multidot(ctx, s, s->V, 0, s->ws, n))) return st;
-            const double hn = sqrt(s->host[0]);
-            H[j + 1 + (size_t)j * (m + 1)] = hn;
+            multidot_dev(ctx, s, s->V, j + 1, s->ws, n, d1);
+            hipLaunchKernelGGL(k_orth, dim3(nb), dim3(NT), 0, ctx->stream, s->V, n, j + 1, d1, s->ws);
+            multidot_dev(ctx, s, s->V, j + 1, s->ws, n, d2);
+            hipLaunchKernelGGL(k_orth, dim3(nb), dim3(NT), 0, ctx->stream, s->V, n, j + 1, d2, s->ws);
+            multidot_dev(ctx, s, s->V, 0, s->ws, n, dn);
+            hipLaunchKernelGGL(k_arnoldi, dim3(1), dim3(64), 0, ctx->stream, K, m, j, d1, d2, dn, s->host_dev + 8);
+            if (j + 1 < m)
+                hipLaunchKernelGGL(k_scale_h, dim3(nb), dim3(NT), 0, ctx->stream, s->ws, dn + 1,
+                                   s->V + (int64_t)(j + 1) * n, n);
+            XC_TRY_HIP(ctx, hipEventRecord(s->ev[j & 1], ctx->stream));
+            if (j == 0) continue;
+            // step j-1 is complete once its event has fired
+            XC_TRY_HIP(ctx, hipEventSynchronize(s->ev[(j - 1) & 1]));
+            used = j;
             ++iters;
-            // Givens rotations
-            for (int i = 0; i < j; ++i) {
-                const double a = H[i + (size_t)j * (m + 1)], b = H[i + 1 + (size_t)j * (m + 1)];
-                H[i + (size_t)j * (m + 1)] = cs[i] * a + sn[i] * b;
-                H[i + 1 + (size_t)j * (m + 1)] = -sn[i] * a + cs[i] * b;
-            }
-            const double a = H[j + (size_t)j * (m + 1)], b = H[j + 1 + (size_t)j * (m + 1)];
-            const double rr = sqrt(a * a + b * b);
-            cs[j] = rr > 0 ? a / rr : 1.0;
-            sn[j] = rr > 0 ? b / rr : 0.0;
-            H[j + (size_t)j * (m + 1)] = rr;
-            H[j + 1 + (size_t)j * (m + 1)] = 0.0;
-            gv[j + 1] = -sn[j] * gv[j];
-            gv[j] = cs[j] * gv[j];
-            if (fabs(gv[j + 1]) <= 0.5 * target || hn == 0.0) { ++j; break; }
-            if (j + 1 < m) hipLaunchKernelGGL(k_scale, dim3(nb), dim3(NT), 0, ctx->stream, s->ws, 1.0 / hn,
-                                              s->V + (int64_t)(j + 1) * n, n);
+            if (hout[2 * (j - 1)] <= 0.5 * target || hout[2 * (j - 1) + 1] == 0.0 || iters >= maxit) break;
         }
-        // y = H^-1 g (upper triangular j x j), x += Z y
-        for (int i = j - 1; i >= 0; --i) {
-            double sum = gv[i];
-            for (int k = i + 1; k < j; ++k) sum -= H[i + (size_t)k * (m + 1)] * y[k];
-            y[i] = sum / H[i + (size_t)i * (m + 1)];
+        if (j == m) {  // the last step of the cycle was not tested yet
+            XC_TRY_HIP(ctx, hipEventSynchronize(s->ev[(m - 1) & 1]));
+            used = m;
+            ++iters;
         }
-        XC_TRY_HIP(ctx, hipMemcpyAsync(s->dots, y.data(), j * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-        hipLaunchKernelGGL(k_update_x, dim3(nb), dim3(NT), 0, ctx->stream, s->Z, n, j, s->dots, s->xs);
+        hipLaunchKernelGGL(k_trisolve, dim3(1), dim3(64), 0, ctx->stream, K, m, used);
+        hipLaunchKernelGGL(k_update_x, dim3(nb), dim3(NT), 0, ctx->stream, s->Z, n, used, K.y, s->xs);
     }
     hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(NT), 0, ctx->stream, s->xs, fl->W, fl->U, fl->P, Nz, Nx);
     if (stats) {
