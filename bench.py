@@ -87,13 +87,18 @@ def c3_setup(N, seed=15):
     return par, g, sc, ph, fl, ns, hst, xc.StepState(t, dt, step)
 
 
-def step_bench(args):
+def step_bench(args, learned=False):
     import torch
     import xcore_amd as xc
     xc.load()
     torch.cuda.set_device(0)
     par, g, sc, ph, fl, ns, hst, state = c3_setup(args.n)
-    opts = xc.step_opts(par, tol=1e-8, maxit=2000)
+    lc_level = 2
+    if learned:
+        from xcore_amd import train_lcc
+        lc_level = train_lcc.load(args.weights)
+    opts = xc.step_opts(par, tol=1e-8, maxit=2000, mg_pre=args.mg_pre, mg_post=args.mg_post, mg_coarse=args.mg_coarse,
+                        learned=learned, lc_level=lc_level)
     for _ in range(args.warmup):
         xc.run_step_(ph, fl, ns, g, par, sc, state, opts)
     torch.cuda.synchronize()
@@ -107,6 +112,7 @@ def step_bench(args):
     its = sum(r["stokes_iters"] for r in log)
     info = {"sweeps_per_step": sweeps / len(log), "fgmres_iters_per_solve": its / max(sweeps, 1),
             "worst_stokes_rel_resid": max(r["stokes_worst_rel_resid"] for r in log),
+            "stokes_fallbacks": sum(r["stokes_fallbacks"] for r in log),
             "sim_time": state.time, "dt": state.dt, "step": int(state.step),
             "x_finite": bool(torch.isfinite(ph.x).all())}
     return 1e3 * elapsed / args.steps, info
@@ -207,6 +213,12 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--stencil-n", type=int, default=N_STENCIL)
     ap.add_argument("--cpu-n", type=int, default=256)
+    ap.add_argument("--weights", default=os.path.join(ROOT, "xcore.jl_amd", "lcc_weights.npz"),
+                    help="trained learned-correction weights; when present the learned step is timed too")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--mg-pre", type=int, default=2)
+    ap.add_argument("--mg-post", type=int, default=2)
+    ap.add_argument("--mg-coarse", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-stencil", action="store_true")
     ap.add_argument("--no-step", action="store_true", help="skip the full-step bench (profiling runs)")
@@ -228,7 +240,13 @@ def main():
     else:
         print(f"bench: full step {args.n}^2, {args.warmup} warmup + {args.steps} timed", file=sys.stderr, flush=True)
         ms, info = step_bench(args)
-        print(f"bench: {ms:.2f} ms/step {info}", file=sys.stderr, flush=True)
+        print(f"bench: plain {ms:.2f} ms/step {info}", file=sys.stderr, flush=True)
+        plain = {"steps_per_s": round(1e3 / ms, 4), "ms_per_step": round(ms, 3), **info}
+        learned = None
+        if not args.no_learned and os.path.exists(args.weights):
+            lms, linfo = step_bench(args, learned=True)
+            print(f"bench: learned {lms:.2f} ms/step {linfo}", file=sys.stderr, flush=True)
+            learned = {"steps_per_s": round(1e3 / lms, 4), "ms_per_step": round(lms, 3), **linfo}
     if args.calib_copy_mb > 0:
         import torch
         n = args.calib_copy_mb * (1 << 20) // 8
@@ -239,6 +257,14 @@ def main():
         torch.cuda.synchronize()
         print(f"bench: calibration copy of {args.calib_copy_mb} MiB x 3 done", file=sys.stderr, flush=True)
         del a, b
+    variant = "multigrid"
+    if not args.no_step:
+        info = {"plain": plain}
+        if learned is not None:
+            info["learned"] = learned
+            info["learned_over_plain"] = round(learned["steps_per_s"] / plain["steps_per_s"], 4)
+            variant = "learned coarse correction + multigrid"
+            ms = learned["ms_per_step"]
     out = {
         "metric": METRIC,
         "value": round(1e3 / ms, 4),
@@ -252,10 +278,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (initialize! with a seeded perturbation; no dataset)",
-        "config": {"workload": f"full run! time step at {args.n}^2 FP64, plain FGMRES + block multigrid Stokes "
-                               "to 1e-8 relative residual, etacntr 1e3, rhox0 - rhom0 = 200, WENO5, bd2im, "
-                               "x periodic / z closed",
-                   "grid": [args.n, args.n], "stokes_preconditioner": "multigrid", **info},
+        "config": {"workload": f"full run! time step at {args.n}^2 FP64, FGMRES Stokes to 1e-8 relative "
+                               "residual, etacntr 1e3, rhox0 - rhom0 = 200, WENO5, bd2im, x periodic / z closed",
+                   "grid": [args.n, args.n], "stokes_preconditioner": variant, **info},
     }
     if not args.no_stencil:
         print(f"bench: stencil microbench {args.stencil_n}^2", file=sys.stderr, flush=True)
@@ -264,7 +289,7 @@ def main():
         out["stencil_kernels"] = kernels
     if not args.no_cpu:
         print(f"bench: CPU oracle sweep at {args.cpu_n}^2", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(args.cpu_n, info["sweeps_per_step"])
+        out["cpu_baseline"] = cpu_baseline(args.cpu_n, info["plain"]["sweeps_per_step"] if "plain" in info else 1.0)
     print(json.dumps(out), flush=True)
 
 

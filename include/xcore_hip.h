@@ -178,6 +178,9 @@ typedef struct {  /* fluidmech! keywords (fluidmech.jl:71-80) and the iterative-
     int restart;      /* FGMRES restart length */
     int mg_pre, mg_post, mg_coarse;  /* multigrid smoothing sweeps (pre, post, coarsest level) */
     double mg_omega;  /* damped-Jacobi weight of the multigrid smoother */
+    int learned;      /* 1: add the learned coarse correction (needs xc_unet_set_params); the plain
+                         preconditioner is the fallback when the learned solve fails */
+    int lc_level;     /* grid of the learned correction: Nz / 2^lc_level x Nx / 2^lc_level cells */
 } xc_fm_opts;
 
 typedef struct {  /* state.jl:131-134 BnchmData */
@@ -208,6 +211,18 @@ int xc_stokes_apply(xc_ctx* ctx, const xc_fluid* fluid, const xc_grid* grid, con
 /* RR of fluidmech.jl (after its MFS update when phase != NULL) into a stacked device vector */
 int xc_stokes_rhs(xc_ctx* ctx, xc_fluid* fluid, const xc_phase* phase, const xc_grid* grid, const xc_par* par,
                   const xc_fm_opts* opts, const xc_bnchm* bnchm, double* rr);
+/* No reference counterpart (the solver pieces behind fluidmech!, exposed to generate the learned
+ * correction's training data and to test it): z = M^-1 v with the plain block preconditioner;
+ * FGMRES on LL x = rhs from the given x (opts->learned selects the preconditioner); and the
+ * learned correction's input map: block averages over 2^level x 2^level cells of the cell-centred
+ * (W, U, P) of vec plus log(eta / etam0) into out4 ([p][4] FP32, p = x * Hc + z), channels 1..3
+ * divided by sigma = their RMS (computed when compute_sigma, else read from *sigma; device FP32). */
+int xc_stokes_precond(xc_ctx* ctx, const xc_fluid* fluid, const xc_grid* grid, const xc_par* par,
+                      const xc_fm_opts* opts, int bnchm, const double* v, double* z);
+int xc_stokes_solve(xc_ctx* ctx, const xc_fluid* fluid, const xc_grid* grid, const xc_par* par,
+                    const xc_fm_opts* opts, int bnchm, const double* rhs, double* x, xc_solve_stats* stats);
+int xc_stokes_lc_restrict(xc_ctx* ctx, const xc_fluid* fluid, const xc_grid* grid, const xc_par* par, int level,
+                          const double* vec, float* out4, float* sigma, int compute_sigma);
 
 /* ---- time loop (src/run.jl, src/diagnose.jl, src/history.jl) ------------ */
 typedef struct {  /* run! keywords (run.jl:167-181) and Picard controls (parameters.jl:150-154) */
@@ -243,6 +258,25 @@ int xc_compute_resnorm(xc_ctx* ctx, const xc_phase* phase, const xc_fluid* fluid
 /* history.jl:122-151 raw sums: {sum rho, sum M, sum X, sum Gx, top/bottom line sums of qz_advn_X,
  * qz_advn_M, qz_dffn_X, qz_dffn_M over the interior columns} (12 values, synchronizes) */
 int xc_history_sums(xc_ctx* ctx, const xc_phase* phase, const xc_fluid* fluid, double out[12]);
+
+/* ---- learned coarse correction (no counterpart in the reference: the learned Stokes
+ * preconditioner).  U-Net, 4 levels, channels 16/32/48/64, BF16 weights and activations with
+ * FP32 accumulation.  Tensors are device FP32, pixel-major, pixel p = (b*W + x)*H + z:
+ * inputs [p][4] = (log-viscosity, r_W, r_U, r_P), outputs [p][3] = (e_W, e_U, e_P); H, W
+ * multiples of 8.  Parameters: one flat FP32 vector (layer order and layouts in xc_unet.hip). */
+int xc_unet_num_params(int64_t* n);
+int xc_unet_set_params(xc_ctx* ctx, const float* host_params, int64_t n);  /* also resets Adam */
+int xc_unet_get_params(xc_ctx* ctx, float* host_params, int64_t n);
+int xc_unet_forward(xc_ctx* ctx, const float* in, int batch, int H, int W, float* out);
+/* fixed per-channel scales of the trained network: inputs are multiplied by in_scale, outputs by out_scale */
+int xc_unet_set_io_scales(xc_ctx* ctx, const float in_scale[4], const float out_scale[3]);
+/* loss = 0.5 mean_pixels sum_channels (out - tgt)^2 (to *loss on the host if non-NULL, which
+ * synchronizes) and its gradient (to the device vector grad if non-NULL) */
+int xc_unet_loss_grad(xc_ctx* ctx, const float* in, const float* tgt, int batch, int H, int W,
+                      double* loss, float* grad);
+/* loss_grad followed by one Adam step (beta 0.9 / 0.999, eps 1e-8) with learning rate lr */
+int xc_unet_train_step(xc_ctx* ctx, const float* in, const float* tgt, int batch, int H, int W,
+                       float lr, double* loss);
 
 /* timing.jl:18-36 time_coefs(T, TINT, step); tint 0 :be1im, 1 :bd2im, 2 :cn2si, 3 :bd2si */
 int xc_time_coefs(int tint, int64_t step, double coef_out[6]);

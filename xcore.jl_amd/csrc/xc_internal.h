@@ -53,6 +53,13 @@ int xc_scratch(xc_ctx* ctx, int slot, size_t bytes, void** out);
 int xc_diffus_centered_kprod(xc_ctx* ctx, xc_mat dff, const xc_mat* qx, const xc_mat* qz, xc_mat f, xc_mat k1,
                              xc_mat k2, double h, int xbc, int zbc);
 
+// Learned coarse correction (xc_unet.hip), used by the Stokes preconditioner:
+// network forward into the context's own output buffer ([p][3] FP32, valid until the next call),
+// whether parameters were loaded, and the per-channel input/output scales of the trained network.
+int xc_unet_infer(xc_ctx* ctx, const float* in, int B, int H, int W, const float** out);
+bool xc_unet_has_params(xc_ctx* ctx);
+void xc_unet_io_scales(xc_ctx* ctx, float in[4], float out[3]);
+
 // 2-D launch domain of R x C elements, 64 (z) x 4 (x) lanes per workgroup, XCD-aware order
 struct xc_dom { int R, C, nbz, nb; };
 static inline xc_dom xc_make_dom(int64_t R, int64_t C) {

@@ -110,11 +110,11 @@ __device__ __forceinline__ double vel_row(const LevD& L, const Flags& F, const d
     if (fine && k == F.nu0) { *dg = 1.0; *bc = true; return x[k]; }
     const int64_t nW = L.nW();
     if (k < nW) {
-        const int r = (int)(k % (L.Nz + 1)), c = (int)(k / (L.Nz + 1));
+        const int r = ((int)k % (L.Nz + 1)), c = ((int)k / (L.Nz + 1));
         return w_row(L, F, x, r, c, dg, bc);
     }
     const int64_t q = k - nW;
-    const int r = (int)(q % (L.Nz + 2)), c = (int)(q / (L.Nz + 2));
+    const int r = ((int)q % (L.Nz + 2)), c = ((int)q / (L.Nz + 2));
     return u_row(L, F, x, r, c, dg, bc);
 }
 
@@ -134,18 +134,18 @@ __device__ double ll_row(const LevD& L, const Flags& F, const double* x, int64_t
         double s = vel_row(L, F, x, k, &dg, &bc, true);
         if (!bc) {  // GG (fluidmech.jl:343-356)
             if (k < nW) {
-                const int r = (int)(k % (Nz + 1)), c = (int)(k / (Nz + 1));
+                const int r = ((int)k % (Nz + 1)), c = ((int)k / (Nz + 1));
                 s += -L.invh * x[L.iP(r, c)] + L.invh * x[L.iP(r + 1, c)];
             } else {
                 const int64_t q = k - nW;
-                const int r = (int)(q % (Nz + 2)), c = (int)(q / (Nz + 2));
+                const int r = ((int)q % (Nz + 2)), c = ((int)q / (Nz + 2));
                 s += -L.invh * x[L.iP(r, c)] + L.invh * x[L.iP(r, c + 1)];
             }
         }
         return s;
     }
     const int64_t q = k - nW - nU;
-    const int r = (int)(q % (Nz + 2)), c = (int)(q / (Nz + 2));
+    const int r = ((int)q % (Nz + 2)), c = ((int)q / (Nz + 2));
     const bool pin = is_pin(F, r, c, Nz);
     const bool interior = r >= 1 && r <= Nz && c >= 1 && c <= Nx;
     if (pin && (F.bnchm || !interior)) return x[k];
@@ -212,7 +212,7 @@ __device__ __forceinline__ void restrict_at(const LevD& Lf, const LevD& Lc, cons
     const int64_t nWc = Lc.nW();
     double v = 0.0;
     if (k < nWc) {
-        const int J = (int)(k % (Lc.Nz + 1)), I = (int)(k / (Lc.Nz + 1));
+        const int J = ((int)k % (Lc.Nz + 1)), I = ((int)k / (Lc.Nz + 1));
         if (J >= 1 && J <= Lc.Nz - 1 && I >= 1 && I <= Lc.Nx) {
             const int r = 2 * J, c0 = 2 * I - 1, c1 = 2 * I;
             v = (2.0 * rf[Lf.iW(r, c0)] + 2.0 * rf[Lf.iW(r, c1)] + rf[Lf.iW(r - 1, c0)] + rf[Lf.iW(r - 1, c1)] +
@@ -220,7 +220,7 @@ __device__ __forceinline__ void restrict_at(const LevD& Lf, const LevD& Lc, cons
         }
     } else {
         const int64_t q = k - nWc;
-        const int J = (int)(q % (Lc.Nz + 2)), I = (int)(q / (Lc.Nz + 2));
+        const int J = ((int)q % (Lc.Nz + 2)), I = ((int)q / (Lc.Nz + 2));
         if (J >= 1 && J <= Lc.Nz) {
             const int r0 = 2 * J - 1, r1 = 2 * J, f = 2 * I;
             const int fl = f == 0 ? Lf.Nx - 1 : f - 1;
@@ -242,7 +242,7 @@ __device__ __forceinline__ void prolong_at(const LevD& Lf, const LevD& Lc, const
                                            const double* __restrict__ ec, double* __restrict__ xf, int64_t k) {
     const int64_t nWf = Lf.nW();
     if (k < nWf) {
-        const int r = (int)(k % (Lf.Nz + 1)), c = (int)(k / (Lf.Nz + 1));
+        const int r = ((int)k % (Lf.Nz + 1)), c = ((int)k / (Lf.Nz + 1));
         if (r < 1 || r > Lf.Nz - 1 || c < 1 || c > Lf.Nx) return;
         // coarse W column value with the periodic/mirror ghost map of the BC rows
         auto E = [&](int J, int I) -> double {
@@ -261,7 +261,7 @@ __device__ __forceinline__ void prolong_at(const LevD& Lf, const LevD& Lc, const
         xf[k] += v;
     } else {
         const int64_t q = k - nWf;
-        const int r = (int)(q % (Lf.Nz + 2)), c = (int)(q / (Lf.Nz + 2));
+        const int r = ((int)q % (Lf.Nz + 2)), c = ((int)q / (Lf.Nz + 2));
         if (r < 1 || r > Lf.Nz) return;
         auto E = [&](int J, int I) -> double {
             if (J == 0) return -F.top_cnv * ec[Lc.iU(1, I)];
@@ -287,57 +287,101 @@ __global__ __launch_bounds__(NT) void k_prolong(LevD Lf, LevD Lc, Flags F, const
 // The coarse end of the V-cycle (levels of at most TAIL_CELLS cells) in one
 // workgroup: the sweeps, residuals and transfers of these small levels are
 // latency-bound, so they run back to back with workgroup barriers instead of
-// one launch each.  Same arithmetic as the per-level kernels.
+// one launch each, on copies of the coefficients and vectors held in LDS.
+// Same arithmetic as the per-level kernels.
 constexpr int TAIL_NT = 1024;
 constexpr int TAIL_MAX = 6;
-constexpr int64_t TAIL_CELLS = 32 * 32;
+constexpr int64_t TAIL_CELLS = 16 * 16;
 struct TailArgs {
-    LevD d[TAIL_MAX];
-    double *x[TAIL_MAX], *b[TAIL_MAX], *r[TAIL_MAX], *t[TAIL_MAX];
+    LevD d[TAIL_MAX];      // the levels' coefficient arrays in global memory
+    const double* bg;      // right-hand side of the first tail level (global)
+    double* xg;            // its result (global)
+    int off[TAIL_MAX][9];  // LDS offsets (doubles): eta, etaco, rho, rhow, rhou, x, b, r, t
     int nlev, pre, post, coarse;
     double omega;
     Flags F;
 };
+__device__ __forceinline__ LevD tail_lev(const TailArgs& a, double* lds, int l) {
+    LevD L = a.d[l];
+    L.eta = lds + a.off[l][0];
+    L.etaco = lds + a.off[l][1];
+    L.rho = lds + a.off[l][2];
+    L.rhow = lds + a.off[l][3];
+    L.rhou = lds + a.off[l][4];
+    return L;
+}
 __global__ __launch_bounds__(TAIL_NT) void k_mg_tail(TailArgs a) {
-    double* xp[TAIL_MAX];
-    double* tp[TAIL_MAX];
-    for (int l = 0; l < a.nlev; ++l) { xp[l] = a.x[l]; tp[l] = a.t[l]; }
+    extern __shared__ double lds[];
     const int tid = threadIdx.x;
+    // stage coefficients (and the first level's right-hand side) in LDS
+    for (int l = 0; l < a.nlev; ++l) {
+        const LevD& G = a.d[l];
+        const int Nz = G.Nz, Nx = G.Nx;
+        const int n0 = Nz * Nx, n1 = (Nz + 1) * (Nx + 1), n3 = (Nz + 1) * Nx, n4 = Nz * (Nx + 1);
+        for (int k = tid; k < n0; k += TAIL_NT) {
+            lds[a.off[l][0] + k] = G.eta[k];
+            lds[a.off[l][2] + k] = G.rho[k];
+        }
+        for (int k = tid; k < n1; k += TAIL_NT) lds[a.off[l][1] + k] = G.etaco[k];
+        for (int k = tid; k < n3; k += TAIL_NT) lds[a.off[l][3] + k] = G.rhow[k];
+        for (int k = tid; k < n4; k += TAIL_NT) lds[a.off[l][4] + k] = G.rhou[k];
+    }
+    {
+        const int nv0 = (int)(a.d[0].nW() + a.d[0].nU());
+        for (int k = tid; k < nv0; k += TAIL_NT) lds[a.off[0][6] + k] = a.bg[k];
+    }
+    __syncthreads();
+    unsigned swp = 0;  // bit l: x and t of level l exchanged
+    auto X = [&](int l) { return lds + a.off[l][((swp >> l) & 1) ? 8 : 5]; };
+    auto T = [&](int l) { return lds + a.off[l][((swp >> l) & 1) ? 5 : 8]; };
     // down: smooth from zero, residual, restrict
     for (int l = 0; l < a.nlev; ++l) {
-        const LevD& L = a.d[l];
+        const LevD L = tail_lev(a, lds, l);
         const int64_t nv = L.nW() + L.nU();
         const bool coarsest = l + 1 == a.nlev;
         const int sweeps = coarsest ? a.coarse : a.pre;
+        double* bl = lds + a.off[l][6];
         for (int it = 0; it < sweeps; ++it) {
-            for (int64_t k = tid; k < nv; k += TAIL_NT) jacobi_at(L, a.F, xp[l], a.b[l], tp[l], a.omega, 0, it == 0, k);
+            double* xl = X(l);
+            double* tl = T(l);
+            for (int64_t k = tid; k < nv; k += TAIL_NT) jacobi_at(L, a.F, xl, bl, tl, a.omega, 0, it == 0, k);
             __syncthreads();
-            double* q = xp[l]; xp[l] = tp[l]; tp[l] = q;
+            swp ^= 1u << l;
         }
         if (coarsest) break;
-        for (int64_t k = tid; k < nv; k += TAIL_NT) vel_resid_at(L, a.F, xp[l], a.b[l], a.r[l], 0, k);
+        double* rl = lds + a.off[l][7];
+        const double* xl = X(l);
+        for (int64_t k = tid; k < nv; k += TAIL_NT) vel_resid_at(L, a.F, xl, bl, rl, 0, k);
         __syncthreads();
-        const LevD& C = a.d[l + 1];
+        const LevD C = tail_lev(a, lds, l + 1);
         const int64_t nc = C.nW() + C.nU();
-        for (int64_t k = tid; k < nc; k += TAIL_NT) restrict_at(L, C, a.r[l], a.b[l + 1], k);
+        double* bc = lds + a.off[l + 1][6];
+        for (int64_t k = tid; k < nc; k += TAIL_NT) restrict_at(L, C, rl, bc, k);
         __syncthreads();
     }
     // up: prolong and post-smooth
     for (int l = a.nlev - 2; l >= 0; --l) {
-        const LevD& L = a.d[l];
+        const LevD L = tail_lev(a, lds, l);
+        const LevD C = tail_lev(a, lds, l + 1);
         const int64_t nv = L.nW() + L.nU();
-        for (int64_t k = tid; k < nv; k += TAIL_NT) prolong_at(L, a.d[l + 1], a.F, xp[l + 1], xp[l], k);
+        double* bl = lds + a.off[l][6];
+        {
+            double* xl = X(l);
+            const double* xc = X(l + 1);
+            for (int64_t k = tid; k < nv; k += TAIL_NT) prolong_at(L, C, a.F, xc, xl, k);
+        }
         __syncthreads();
         for (int it = 0; it < a.post; ++it) {
-            for (int64_t k = tid; k < nv; k += TAIL_NT) jacobi_at(L, a.F, xp[l], a.b[l], tp[l], a.omega, 0, 0, k);
+            double* xl = X(l);
+            double* tl = T(l);
+            for (int64_t k = tid; k < nv; k += TAIL_NT) jacobi_at(L, a.F, xl, bl, tl, a.omega, 0, 0, k);
             __syncthreads();
-            double* q = xp[l]; xp[l] = tp[l]; tp[l] = q;
+            swp ^= 1u << l;
         }
     }
-    if (xp[0] != a.x[0]) {  // leave the result where the caller expects the level's iterate
-        const int64_t nv = a.d[0].nW() + a.d[0].nU();
-        for (int64_t k = tid; k < nv; k += TAIL_NT) a.x[0][k] = xp[0][k];
-    }
+    const int nv0 = (int)(a.d[0].nW() + a.d[0].nU());
+    const double* x0 = X(0);
+    for (int k = tid; k < nv0; k += TAIL_NT) a.xg[k] = x0[k];
 }
 
 // coarse coefficients from fine ones
@@ -346,7 +390,7 @@ __global__ __launch_bounds__(NT) void k_coarsen(LevD Lf, double* eta, double* rh
     const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
     const int64_t nmax = (int64_t)(Nzc + 1) * (Nxc + 1);
     if (k >= nmax) return;
-    const int J = (int)(k % (Nzc + 1)), I = (int)(k / (Nzc + 1));
+    const int J = ((int)k % (Nzc + 1)), I = ((int)k / (Nzc + 1));
     if (J < Nzc && I < Nxc) {
         const double e00 = Lf.ETA(2 * J, 2 * I), e10 = Lf.ETA(2 * J + 1, 2 * I);
         const double e01 = Lf.ETA(2 * J, 2 * I + 1), e11 = Lf.ETA(2 * J + 1, 2 * I + 1);
@@ -362,7 +406,7 @@ __global__ __launch_bounds__(NT) void k_coarsen(LevD Lf, double* eta, double* rh
 __global__ __launch_bounds__(NT) void k_corner_eta(const double* eta, double* etaco, int Nz, int Nx) {
     const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
     if (k >= (int64_t)(Nz + 1) * (Nx + 1)) return;
-    const int j = (int)(k % (Nz + 1)), i = (int)(k / (Nz + 1));
+    const int j = ((int)k % (Nz + 1)), i = ((int)k / (Nz + 1));
     const int z0 = j - 1 < 0 ? 0 : j - 1, z1 = j >= Nz ? Nz - 1 : j;
     const int x0 = i - 1 < 0 ? Nx - 1 : i - 1, x1 = i >= Nx ? 0 : i;
     const double p = eta[z0 + (int64_t)x0 * Nz] * eta[z1 + (int64_t)x0 * Nz] * eta[z0 + (int64_t)x1 * Nz] *
@@ -377,7 +421,7 @@ __global__ __launch_bounds__(NT) void k_schur_int(LevD L, Flags F, const double*
     const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
     const int Nz = L.Nz, Nx = L.Nx;
     if (k >= (int64_t)(Nz + 2) * (Nx + 2)) return;
-    const int r = (int)(k % (Nz + 2)), c = (int)(k / (Nz + 2));
+    const int r = ((int)k % (Nz + 2)), c = ((int)k / (Nz + 2));
     if (r < 1 || r > Nz || c < 1 || c > Nx) return;
     // S = KP - DM A^-1 GG ~ +rho/eta (DM ~ rho div, GG ~ grad, A ~ -eta lap)
     const double s = L.RHO(r - 1, c - 1) / L.ETA(r - 1, c - 1);
@@ -390,7 +434,7 @@ __global__ __launch_bounds__(NT) void k_schur_ghost(LevD L, Flags F, const doubl
     const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
     const int Nz = L.Nz, Nx = L.Nx;
     if (k >= (int64_t)(Nz + 2) * (Nx + 2)) return;
-    const int r = (int)(k % (Nz + 2)), c = (int)(k / (Nz + 2));
+    const int r = ((int)k % (Nz + 2)), c = ((int)k / (Nz + 2));
     auto P = [&](int rr, int cc) { return zP[rr + (int64_t)cc * (Nz + 2)]; };
     const bool pin = is_pin(F, r, c, Nz);
     double v;
@@ -418,12 +462,12 @@ __global__ __launch_bounds__(NT) void k_sub_grad(LevD L, Flags F, const double* 
     double g = 0.0;
     if (k != F.nu0) {
         if (k < nW) {
-            const int r = (int)(k % (Nz + 1)), c = (int)(k / (Nz + 1));
+            const int r = ((int)k % (Nz + 1)), c = ((int)k / (Nz + 1));
             if (r >= 1 && r <= Nz - 1 && c >= 1 && c <= Nx)
                 g = -L.invh * zP[r + (int64_t)c * (Nz + 2)] + L.invh * zP[r + 1 + (int64_t)c * (Nz + 2)];
         } else {
             const int64_t q = k - nW;
-            const int r = (int)(q % (Nz + 2)), c = (int)(q / (Nz + 2));
+            const int r = ((int)q % (Nz + 2)), c = ((int)q / (Nz + 2));
             if (r >= 1 && r <= Nz)
                 g = -L.invh * zP[r + (int64_t)c * (Nz + 2)] + L.invh * zP[r + (int64_t)(c + 1) * (Nz + 2)];
         }
@@ -471,41 +515,47 @@ __global__ __launch_bounds__(FFT_NT) void k_pois_xfwd(PoisXArgs a) {
     }
 }
 
-// tridiagonal solve along z of every wavenumber column, in place:
-//   -p[r-1] + b_r p[r] - p[r+1] = d[r]; invw[r*K + k] = 1 / pivot (0 at the pinned row of k = 0)
-__global__ __launch_bounds__(64) void k_pois_z(double2* __restrict__ S, const double* __restrict__ invw, int Nz, int K) {
-    const int k = blockIdx.x * 64 + threadIdx.x;
-    if (k >= K) return;
-    double2* col = S + k;
-    if (k == 0) {  // remove the mean of the right-hand side (kx = 0 is real)
-        double sum = 0.0;
-        for (int r = 0; r < Nz; ++r) sum += col[(int64_t)r * K].x;
-        const double mean = sum / (double)Nz;
-        for (int r = 0; r < Nz; ++r) col[(int64_t)r * K] = make_double2(col[(int64_t)r * K].x - mean, 0.0);
+// z solve of every wavenumber column, in place.  The zero-flux rows make the
+// cell-centred z operator the periodic one on the column mirrored about its end
+// faces (length 2 Nz), so a complex FFT of the mirrored column diagonalises it:
+// divide by lam_x[k] + lam_z[kz] (the kx = kz = 0 mode, the constant, is
+// dropped: zero-mean solution), transform back, keep the first Nz entries.
+struct PoisZArgs {
+    double2* S;          // Nz x K, k fastest
+    Plan p;              // length 2 Nz
+    const double* lamx;  // K: (cx/cz) (2 - 2 cos(2 pi k / Nx))
+    const double* lamz;  // 2 Nz: 2 - 2 cos(2 pi kz / (2 Nz))
+    int Nz, K, nl;
+};
+__global__ __launch_bounds__(FFT_NT) void k_pois_zfft(PoisZArgs a) {
+    extern __shared__ double2 sm[];
+    const int n = 2 * a.Nz, nl = a.nl, K = a.K;
+    const int k0 = blockIdx.x * nl;
+    double2* A = sm;
+    double2* B = sm + nl * n;
+    for (int i = threadIdx.x; i < nl * n; i += FFT_NT) {
+        const int l = i / n, zz = i - l * n;
+        const int k = k0 + l;
+        const int r = zz < a.Nz ? zz : n - 1 - zz;
+        A[i] = k < K ? a.S[(int64_t)r * K + k] : make_double2(0.0, 0.0);
     }
-    double2 y = col[0];
-    double iw = invw[k];
-    for (int r = 1; r < Nz; ++r) {
-        double2 d = col[(int64_t)r * K];
-        d.x += y.x * iw;
-        d.y += y.y * iw;
-        col[(int64_t)(r - 1) * K] = y;  // keep y[r-1] for the back substitution
-        y = d;
-        iw = invw[(int64_t)r * K + k];
+    __syncthreads();
+    double2* X = lds_fft<false>(A, B, nl, a.p);
+    const double inv_n = 1.0 / (double)n;
+    for (int i = threadIdx.x; i < nl * n; i += FFT_NT) {
+        const int l = i / n, kz = i - l * n;
+        const int k = k0 + l;
+        const double lam = (k < K ? a.lamx[k] : 1.0) + a.lamz[kz];
+        const double f = lam > 0.0 ? inv_n / lam : 0.0;
+        X[i].x *= f;
+        X[i].y *= f;
     }
-    double2 pn = make_double2(y.x * iw, y.y * iw);
-    col[(int64_t)(Nz - 1) * K] = pn;
-    double sum = pn.x;
-    for (int r = Nz - 2; r >= 0; --r) {
-        const double2 yr = col[(int64_t)r * K];
-        const double w = invw[(int64_t)r * K + k];
-        pn = make_double2((yr.x + pn.x) * w, (yr.y + pn.y) * w);
-        col[(int64_t)r * K] = pn;
-        sum += pn.x;
-    }
-    if (k == 0) {  // the pinned solution differs from the zero-mean one by a constant
-        const double mean = sum / (double)Nz;
-        for (int r = 0; r < Nz; ++r) col[(int64_t)r * K] = make_double2(col[(int64_t)r * K].x - mean, 0.0);
+    __syncthreads();
+    const double2* Y = lds_fft<true>(X, X == A ? B : A, nl, a.p);
+    for (int i = threadIdx.x; i < nl * a.Nz; i += FFT_NT) {
+        const int l = i / a.Nz, r = i - l * a.Nz;
+        const int k = k0 + l;
+        if (k < K) a.S[(int64_t)r * K + k] = Y[l * n + r];
     }
 }
 
@@ -557,37 +607,36 @@ __global__ __launch_bounds__(FFT_NT) void k_pois_xinv(PoisInvArgs a) {
 // ---- vector kernels for FGMRES ------------------------------------------
 // partial dot products of w against K basis vectors V[0..K) (stride n) and, as entry K, w.w
 constexpr int DOT_BLOCKS = 1024;
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+// partial dot products of w against K basis vectors V[0..K) (stride n) and, as entry K, w.w
 __global__ __launch_bounds__(NT) void k_multidot(const double* __restrict__ V, int64_t n, int K,
                                                  const double* __restrict__ w, double* __restrict__ part) {
-    __shared__ double red[NT];
+    __shared__ double red[NT / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int q = 0; q <= K; ++q) {
         const double* v = q < K ? V + (int64_t)q * n : w;
         double acc = 0.0;
         for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) acc += v[i] * w[i];
-        red[threadIdx.x] = acc;
+        acc = wave_sum(acc);
+        if (lane == 0) red[wave] = acc;
         __syncthreads();
-        for (int st = NT / 2; st > 0; st >>= 1) {
-            if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) part[(int64_t)q * gridDim.x + blockIdx.x] = red[0];
+        if (threadIdx.x == 0) part[(int64_t)q * gridDim.x + blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
         __syncthreads();
     }
 }
+// out[q] = sum of the nb partials of q (fixed order); one wavefront per q
 __global__ __launch_bounds__(NT) void k_dot_final(const double* __restrict__ part, int nb, int K1, double* out) {
-    __shared__ double red[NT];
-    for (int q = 0; q < K1; ++q) {
-        double acc = 0.0;
-        for (int b = threadIdx.x; b < nb; b += NT) acc += part[(int64_t)q * nb + b];
-        red[threadIdx.x] = acc;
-        __syncthreads();
-        for (int st = NT / 2; st > 0; st >>= 1) {
-            if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[q] = red[0];
-        __syncthreads();
-    }
+    const int lane = threadIdx.x & 63;
+    const int q = blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+    if (q >= K1) return;
+    double acc = 0.0;
+    for (int b = lane; b < nb; b += 64) acc += part[(int64_t)q * nb + b];
+    acc = wave_sum(acc);
+    if (lane == 0) out[q] = acc;
 }
 // w -= sum_q c[q] V[q]
 __global__ __launch_bounds__(NT) void k_orth(const double* __restrict__ V, int64_t n, int K, const double* __restrict__ c,
@@ -611,6 +660,132 @@ __global__ __launch_bounds__(NT) void k_update_x(const double* __restrict__ Z, i
     for (int q = 0; q < K; ++q) s += y[q] * Z[(int64_t)q * n + i];
     x[i] = s;
 }
+// ---- learned coarse correction: restriction to the network's grid and prolongation back ----
+// Cell-centred (W, U, P) of a stacked vector at interior cell (r, c), 0-based
+__device__ __forceinline__ void cell_vals(const LevD& L, const double* v, int r, int c, double* w, double* u,
+                                          double* p) {
+    *w = 0.5 * (v[L.iW(r, c + 1)] + v[L.iW(r + 1, c + 1)]);
+    *u = 0.5 * (v[L.iU(r + 1, c)] + v[L.iU(r + 1, c + 1)]);
+    *p = v[L.iP(r + 1, c + 1)];
+}
+struct LcArgs {
+    LevD L;
+    int f, Hc, Wc;
+    const double* v;
+    float* out;        // [p][4], p = Xc * Hc + Zc
+    double* part;      // per-block sums of squares of channels 1..3
+    double eta_ref;
+    float s[4];        // fixed channel scales
+};
+__global__ __launch_bounds__(NT) void k_lc_restrict(LcArgs a) {
+    __shared__ double red[NT];
+    const int64_t q = (int64_t)blockIdx.x * NT + threadIdx.x;
+    double ss = 0.0;
+    if (q < (int64_t)a.Hc * a.Wc) {
+        const int Zc = ((int)q % a.Hc), Xc = ((int)q / a.Hc);
+        double sw = 0.0, su = 0.0, sp = 0.0, se = 0.0;
+        for (int j = 0; j < a.f; ++j)
+            for (int i = 0; i < a.f; ++i) {
+                const int r = Zc * a.f + i, c = Xc * a.f + j;
+                double w, u, p;
+                cell_vals(a.L, a.v, r, c, &w, &u, &p);
+                sw += w;
+                su += u;
+                sp += p;
+                se += log(a.L.ETA(r, c) / a.eta_ref);
+            }
+        const double inv = 1.0 / ((double)a.f * a.f);
+        const double cw = sw * inv * a.s[1], cu = su * inv * a.s[2], cp = sp * inv * a.s[3];
+        a.out[q * 4 + 0] = (float)(se * inv * a.s[0]);
+        a.out[q * 4 + 1] = (float)cw;
+        a.out[q * 4 + 2] = (float)cu;
+        a.out[q * 4 + 3] = (float)cp;
+        ss = cw * cw + cu * cu + cp * cp;
+    }
+    red[threadIdx.x] = ss;
+    __syncthreads();
+    for (int st = NT / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.part[blockIdx.x] = red[0];
+}
+// sigma = RMS of channels 1..3 (or the given one); divide channels 1..3 by it (one workgroup)
+__global__ __launch_bounds__(1024) void k_lc_norm(float* inp, const double* part, int nb, int64_t ncell, float* sigma,
+                                                  int compute) {
+    __shared__ double red[1024];
+    __shared__ float sg;
+    if (compute) {
+        double acc = 0.0;
+        for (int b = threadIdx.x; b < nb; b += 1024) acc += part[b];
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int st = 512; st > 0; st >>= 1) {
+            if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            const double rms = sqrt(red[0] / (3.0 * (double)ncell));
+            sigma[0] = rms > 0.0 ? (float)rms : 1.0f;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sg = 1.0f / sigma[0];
+    __syncthreads();
+    const float is = sg;
+    for (int64_t k = threadIdx.x; k < ncell * 3; k += 1024) {
+        const int64_t c = k / 3;
+        inp[c * 4 + 1 + (k - c * 3)] *= is;
+    }
+}
+// z += P e: coarse cell values (times sigma and the output scales) interpolated bilinearly to the
+// fine cell centres, averaged onto the interior W and U faces; BC rows and P ghosts receive nothing
+struct LcProArgs {
+    LevD L;
+    int f, Hc, Wc;
+    const float* e;    // [p][3]
+    const float* sigma;
+    float os[3];
+    double* z;
+    int64_t n;
+};
+__global__ __launch_bounds__(NT) void k_lc_prolong(LcProArgs a) {
+    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (k >= a.n) return;
+    const LevD& L = a.L;
+    const int Nz = L.Nz, Nx = L.Nx;
+    const double sg = (double)a.sigma[0];
+    // fine cell (r, c): bilinear in the coarse cell centres, x periodic, z clamped
+    auto E = [&](int r, int c, int ch) {
+        const double zc = ((double)r + 0.5) / a.f - 0.5, xc = ((double)c + 0.5) / a.f - 0.5;
+        const int z0 = (int)floor(zc), x0 = (int)floor(xc);
+        const double tz = zc - z0, tx = xc - x0;
+        const int za = z0 < 0 ? 0 : z0, zb = z0 + 1 > a.Hc - 1 ? a.Hc - 1 : z0 + 1;
+        const int xa = x0 < 0 ? x0 + a.Wc : x0, xb = x0 + 1 >= a.Wc ? x0 + 1 - a.Wc : x0 + 1;
+        auto at = [&](int z, int x) { return (double)a.e[((int64_t)x * a.Hc + z) * 3 + ch]; };
+        return (1.0 - tz) * ((1.0 - tx) * at(za, xa) + tx * at(za, xb)) + tz * ((1.0 - tx) * at(zb, xa) + tx * at(zb, xb));
+    };
+    const int64_t nW = L.nW(), nU = L.nU();
+    double v = 0.0;
+    if (k < nW) {
+        const int r = ((int)k % (Nz + 1)), c = ((int)k / (Nz + 1));
+        if (r < 1 || r > Nz - 1 || c < 1 || c > Nx) return;
+        v = 0.5 * (E(r - 1, c - 1, 0) + E(r, c - 1, 0)) * a.os[0];
+    } else if (k < nW + nU) {
+        const int64_t q = k - nW;
+        const int r = ((int)q % (Nz + 2)), c = ((int)q / (Nz + 2));
+        if (r < 1 || r > Nz) return;
+        const int cl = c == 0 ? Nx - 1 : c - 1, cr = c == Nx ? 0 : c;
+        v = 0.5 * (E(r - 1, cl, 1) + E(r - 1, cr, 1)) * a.os[1];
+    } else {
+        const int64_t q = k - nW - nU;
+        const int r = ((int)q % (Nz + 2)), c = ((int)q / (Nz + 2));
+        if (r < 1 || r > Nz || c < 1 || c > Nx) return;
+        v = E(r - 1, c - 1, 2) * a.os[2];
+    }
+    a.z[k] += sg * v;
+}
+
 // ---- Arnoldi bookkeeping on the device (one lane; m <= 60) -------------
 // kry layout: H (m+1) x m column-major | cs[m] | sn[m] | g[m+1] | y[m]
 struct Kry {
@@ -666,6 +841,11 @@ __global__ __launch_bounds__(NT) void k_zero(double* a, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
     if (i < n) a[i] = 0.0;
 }
+// y += x
+__global__ __launch_bounds__(NT) void k_axpy1(const double* __restrict__ x, double* __restrict__ y, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (i < n) y[i] += x[i];
+}
 __global__ __launch_bounds__(NT) void k_copyv(const double* __restrict__ a, double* __restrict__ b, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
     if (i < n) b[i] = a[i];
@@ -703,7 +883,7 @@ __global__ __launch_bounds__(NT) void k_mfs(RhsArgs a, int Nz, int Nx, double* p
     const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
     double v = 0.0;
     if (k < (int64_t)Nz * Nx) {
-        const int j = (int)(k % Nz), i = (int)(k / Nz);
+        const int j = ((int)k % Nz), i = ((int)k / Nz);
         auto at = [&](const xc_mat& m) -> double& { return m.p[j + (int64_t)i * m.ld]; };
         if (a.do_mfs) {
 #pragma clang fp contract(off)
@@ -746,27 +926,27 @@ __global__ __launch_bounds__(NT) void k_mom_prep(xc_mat W, xc_mat U, xc_mat rhow
     auto ifx = [&](int kk) { return kk == 0 ? Nx - 1 : (kk == Nx + 2 ? 1 : kk - 1); };
     // fz (Nz-1, Nx), uz (Nz-1, Nx+1), wz (Nz, Nx), fx (Nz, Nx+1), ux (Nz, Nx+2), wx (Nz+1, Nx+1)
     if (k < (int64_t)(Nz - 1) * Nx) {
-        const int j = (int)(k % (Nz - 1)), i = (int)(k / (Nz - 1));
+        const int j = ((int)k % (Nz - 1)), i = ((int)k / (Nz - 1));
         fz[k] = rhow.p[(j + 1) + (int64_t)i * rhow.ld] * Wv(j + 1, i + 1);
     }
     if (k < (int64_t)(Nz - 1) * (Nx + 1)) {
-        const int j = (int)(k % (Nz - 1)), i = (int)(k / (Nz - 1));
+        const int j = ((int)k % (Nz - 1)), i = ((int)k / (Nz - 1));
         uz[k] = (Uv(j + 1, i) + Uv(j + 2, i)) / 2.0;
     }
     if (k < (int64_t)Nz * Nx) {
-        const int j = (int)(k % Nz), i = (int)(k / Nz);
+        const int j = ((int)k % Nz), i = ((int)k / Nz);
         wz[k] = (Wv(j, i + 1) + Wv(j + 1, i + 1)) / 2.0;
     }
     if (k < (int64_t)Nz * (Nx + 1)) {
-        const int j = (int)(k % Nz), i = (int)(k / Nz);
+        const int j = ((int)k % Nz), i = ((int)k / Nz);
         fx[k] = rhou.p[j + (int64_t)i * rhou.ld] * Uv(j + 1, i);
     }
     if (k < (int64_t)Nz * (Nx + 2)) {
-        const int j = (int)(k % Nz), i = (int)(k / Nz);
+        const int j = ((int)k % Nz), i = ((int)k / Nz);
         ux[k] = (Uv(j + 1, ifx(i)) + Uv(j + 1, ifx(i + 1))) / 2.0;
     }
     if (k < (int64_t)(Nz + 1) * (Nx + 1)) {
-        const int j = (int)(k % (Nz + 1)), i = (int)(k / (Nz + 1));
+        const int j = ((int)k % (Nz + 1)), i = ((int)k / (Nz + 1));
         wx[k] = (Wv(j, i) + Wv(j, i + 1)) / 2.0;
     }
 }
@@ -787,7 +967,7 @@ __global__ __launch_bounds__(NT) void k_rr(RRArgs a, int Nz, int Nx, double* RR)
     auto at = [](const xc_mat& m, int r, int c) { return m.p[r + (int64_t)c * m.ld]; };
     double v = 0.0;
     if (k < nW) {
-        const int r = (int)(k % (Nz + 1)), c = (int)(k / (Nz + 1));
+        const int r = ((int)k % (Nz + 1)), c = ((int)k / (Nz + 1));
         if (c >= 1 && c <= Nx) {
             if (r == Nz) {
                 const double mfbg = a.mfsmean[0] * a.D / a.h;
@@ -800,7 +980,7 @@ __global__ __launch_bounds__(NT) void k_rr(RRArgs a, int Nz, int Nx, double* RR)
         }
     } else if (k < nW + nU) {
         const int64_t q = k - nW;
-        const int r = (int)(q % (Nz + 2)), c = (int)(q / (Nz + 2));
+        const int r = ((int)q % (Nz + 2)), c = ((int)q / (Nz + 2));
         if (r >= 1 && r <= Nz) {
             double adv = a.advx[(r - 1) + (int64_t)c * Nz];
             if (c == 0 || c == Nx) adv = (a.advx[(r - 1)] + a.advx[(r - 1) + (int64_t)Nx * Nz]) / 2.0;
@@ -809,7 +989,7 @@ __global__ __launch_bounds__(NT) void k_rr(RRArgs a, int Nz, int Nx, double* RR)
         }
     } else {
         const int64_t q = k - nW - nU;
-        const int r = (int)(q % (Nz + 2)), c = (int)(q / (Nz + 2));
+        const int r = ((int)q % (Nz + 2)), c = ((int)q / (Nz + 2));
         if (r >= 1 && r <= Nz && c >= 1 && c <= Nx) {
             v = at(a.MFS, r - 1, c - 1);
             if (a.bnchm) v += at(a.srcP, r, c);
@@ -852,40 +1032,37 @@ struct StokesCtx {
     // pressure Poisson (inertial Schur part)
     int pK = 0;                 // Nx/2 + 1
     double2* pS = nullptr;      // Nz x pK half spectra
-    double* pinvw = nullptr;    // Nz x pK inverse Thomas pivots
+    double* plamx = nullptr;    // pK x-eigenvalues (normalised by cz)
+    double* plamz = nullptr;    // 2 Nz z-eigenvalues of the mirrored column
     double2* ptw = nullptr;     // Nx twiddles
-    std::vector<int> prad;
-    double pratio = -1.0;       // cx / cz of the pivot table; <= 0: inertial part off for this solve
-    double ptab = -1.0;         // cx / cz the device table holds
+    double2* ptwz = nullptr;    // 2 Nz twiddles
+    std::vector<int> prad, pradz;
+    double pratio = -1.0;       // cx / cz of this solve; <= 0: inertial part off
+    double ptab = -1.0;         // cx / cz the device eigenvalue table holds
     double pscale = 0.0;        // a1 h^2 / (dt Nx)
+    // learned coarse correction
+    float* lc_in = nullptr;     // [cells][4] network input
+    float* lc_sigma = nullptr;  // per-call input scale (device)
+    double* lc_part = nullptr;  // restriction partial sums
+    int lc_cap = 0;             // capacity in coarse cells
 };
 
-// Decide whether the inertial Schur part applies to this solve and (re)build the Thomas pivots.
+// Decide whether the inertial Schur part applies to this solve and (re)build its eigenvalue tables.
 int pois_table(xc_ctx* ctx, StokesCtx* s, const Flags& F, int Nz, int Nx, double h) {
     s->pratio = -1.0;
-    if (F.bnchm || !(F.dt < 1e30) || !(F.a1 > 0) || s->prad.empty()) return XC_OK;
+    if (F.bnchm || !(F.dt < 1e30) || !(F.a1 > 0) || s->prad.empty() || s->pradz.empty()) return XC_OK;
     const double ratio = F.a1 / (F.a1 + F.gamma);
     s->pscale = F.a1 * h * h / (F.dt * (double)Nx);
     s->pratio = ratio;
     if (ratio == s->ptab) return XC_OK;
     const int K = s->pK;
-    std::vector<double> iw((size_t)Nz * K);
-    for (int k = 0; k < K; ++k) {
-        const long double ang = 2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)Nx;
-        const long double rho = (long double)ratio * (2.0L - 2.0L * cosl(ang));
-        long double w = 0.0L;
-        for (int r = 0; r < Nz; ++r) {
-            const long double b = (Nz == 1) ? rho : ((r == 0 || r == Nz - 1) ? 1.0L + rho : 2.0L + rho);
-            if (k == 0 && r == 0) {  // pinned row of the singular kx = 0 system
-                iw[(size_t)r * K + k] = 0.0;
-                w = 0.0L;
-                continue;
-            }
-            w = (r == 0 || (k == 0 && r == 1)) ? b : b - 1.0L / w;
-            iw[(size_t)r * K + k] = (double)(1.0L / w);
-        }
-    }
-    XC_TRY_HIP(ctx, hipMemcpyAsync(s->pinvw, iw.data(), iw.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    const long double pi = 3.141592653589793238462643383279502884L;
+    std::vector<double> lam((size_t)K + 2 * (size_t)Nz);
+    for (int k = 0; k < K; ++k)
+        lam[k] = (double)((long double)ratio * (2.0L - 2.0L * cosl(2.0L * pi * (long double)k / (long double)Nx)));
+    for (int kz = 0; kz < 2 * Nz; ++kz)
+        lam[K + kz] = (double)(2.0L - 2.0L * cosl(2.0L * pi * (long double)kz / (long double)(2 * Nz)));
+    XC_TRY_HIP(ctx, hipMemcpyAsync(s->plamx, lam.data(), lam.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     s->ptab = ratio;
     return XC_OK;
@@ -926,9 +1103,11 @@ int setup(xc_ctx* ctx, int Nz, int Nx, int m, StokesCtx** out) {
                (size_t)L.Nz * (L.Nx + 1) + 4 * (size_t)L.nv;
     }
     const int K = Nx / 2 + 1;
+    const size_t lcc = (size_t)(Nz / 2) * (Nx / 2);  // learned correction at level >= 1
+    tot += 3 * lcc + 16 + (lcc + NT - 1) / NT + 8;
     tot += 4 * (size_t)n + (size_t)(m + 1) * n + (size_t)m * n + (size_t)(m + 2) * DOT_BLOCKS + 128 /* dots */ +
            (size_t)n /* tmp */ + 8 * (size_t)(Nz + 2) * (Nx + 3) /* advection scratch */ +
-           3 * (size_t)Nz * K + 2 * (size_t)Nx + 8 /* Poisson */ +
+           2 * (size_t)Nz * K + 2 * (size_t)Nx + 4 * (size_t)Nz + K + 2 * (size_t)Nz + 16 /* Poisson */ +
            (size_t)(m + 1) * m + 4 * (size_t)m + 8 /* Arnoldi */ + 3 * (size_t)(m + 2) /* dots */;
     void* pool;
     hipError_t e = hipMalloc(&pool, tot * sizeof(double));
@@ -952,22 +1131,34 @@ int setup(xc_ctx* ctx, int Nz, int Nx, int m, StokesCtx** out) {
     s->dots = take(128);
     s->tmp = take(n);
     s->adv_buf = take(8 * (size_t)(Nz + 2) * (Nx + 3));
+    s->lc_in = reinterpret_cast<float*>(take(2 * lcc + 8));   // 4 floats per cell
+    s->lc_sigma = reinterpret_cast<float*>(take(8));
+    s->lc_part = take(lcc / NT + 8 + (lcc + NT - 1) / NT);
+    s->lc_cap = (int)lcc;
     s->kry = take((size_t)(m + 1) * m + 4 * (size_t)m + 8);
     s->d3 = take(3 * (size_t)(m + 2));
     if ((p - s->pool) & 1) ++p;  // 16-byte alignment for the complex arrays
     s->pK = K;
     s->pS = reinterpret_cast<double2*>(take(2 * (size_t)Nz * K));
     s->ptw = reinterpret_cast<double2*>(take(2 * (size_t)Nx));
-    s->pinvw = take((size_t)Nz * K);
+    s->ptwz = reinterpret_cast<double2*>(take(4 * (size_t)Nz));
+    s->plamx = take((size_t)K + 2 * (size_t)Nz);
+    s->plamz = s->plamx + K;
     if ((size_t)(p - s->pool) > tot) {  // carving overran the allocation: a sizing bug, never a user error
         stokes_free(ctx, s);
         return xc_fail(ctx, XC_EHIP, "fluidmech!: internal pool sizing (%zu > %zu)", (size_t)(p - s->pool), tot);
     }
-    if (Nx > MAXLINE || !fft_factorize(Nx, s->prad)) s->prad.clear();
-    else {
+    if (Nx > MAXLINE || 2 * Nz > MAXLINE || !fft_factorize(Nx, s->prad) || !fft_factorize(2 * Nz, s->pradz)) {
+        s->prad.clear();  // inertial Schur part unavailable at this size: the viscous part alone is used
+        s->pradz.clear();
+    } else {
         std::vector<double2> tw;
         fft_twiddles(Nx, tw);
         e = hipMemcpy(s->ptw, tw.data(), Nx * sizeof(double2), hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            fft_twiddles(2 * Nz, tw);
+            e = hipMemcpy(s->ptwz, tw.data(), 2 * (size_t)Nz * sizeof(double2), hipMemcpyHostToDevice);
+        }
         if (e != hipSuccess) { stokes_free(ctx, s); return xc_fail(ctx, XC_EHIP, "fluidmech!: %s", hipGetErrorString(e)); }
     }
     e = hipHostMalloc((void**)&s->host, 256 * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent);
@@ -1035,16 +1226,27 @@ void vcycle(xc_ctx* ctx, StokesCtx* s, const Flags& F, size_t l, const MGOpts& m
     if (l > 0 && (int64_t)L.Nz * L.Nx <= TAIL_CELLS && s->lev.size() - l <= (size_t)TAIL_MAX) {
         TailArgs ta{};
         ta.nlev = (int)(s->lev.size() - l);
+        int o = 0;
         for (int q = 0; q < ta.nlev; ++q) {
             Level& T = s->lev[l + q];
             ta.d[q] = T.d();
-            ta.x[q] = T.x; ta.b[q] = T.b; ta.r[q] = T.r; ta.t[q] = T.t;
+            const int Nz = T.Nz, Nx = T.Nx;
+            const int sz[9] = {Nz * Nx, (Nz + 1) * (Nx + 1), Nz * Nx, (Nz + 1) * Nx, Nz * (Nx + 1),
+                               (int)T.nv, (int)T.nv, (int)T.nv, (int)T.nv};
+            for (int k = 0; k < 9; ++k) {
+                ta.off[q][k] = o;
+                o += (sz[k] + 1) & ~1;
+            }
         }
+        if ((size_t)o * sizeof(double) > 65536) goto per_level;  // tail too large for one workgroup's LDS
+        ta.bg = L.b;
+        ta.xg = L.x;
         ta.pre = mo.pre; ta.post = mo.post; ta.coarse = mo.coarse; ta.omega = mo.omega;
         ta.F = F;
-        hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(TAIL_NT), 0, ctx->stream, ta);
+        hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(TAIL_NT), (size_t)o * sizeof(double), ctx->stream, ta);
         return;
     }
+per_level:
     const LevD d = L.d();
     const unsigned nb = nblk(L.nv);
     const int fine = l == 0;
@@ -1078,8 +1280,10 @@ void precond(xc_ctx* ctx, StokesCtx* s, const Flags& F, const double* v, double*
     double* zP = z + nv;
     if (s->pratio > 0) {  // inertial Schur part active (see pois_table)
         const int Nz = L.Nz, Nx = L.Nx, K = s->pK;
-        int nl = MAXLINE / Nx;
-        if (nl > 8) nl = 8;
+        // enough workgroups to spread over the CUs: one complex line (two rows) each at 512 x 512
+        int nl = Nz / 512;
+        const int nlmax = MAXLINE / Nx < 8 ? MAXLINE / Nx : 8;
+        nl = nl < 1 ? 1 : (nl > nlmax ? nlmax : nl);
         Plan pl{};
         pl.n = Nx;
         pl.nst = (int)s->prad.size();
@@ -1089,7 +1293,17 @@ void precond(xc_ctx* ctx, StokesCtx* s, const Flags& F, const double* v, double*
         const size_t lds = 2 * (size_t)nl * Nx * sizeof(double2);
         PoisXArgs xa{vP, s->pS, pl, Nz, Nx, K, nl};
         hipLaunchKernelGGL(k_pois_xfwd, dim3(nbx), dim3(FFT_NT), lds, ctx->stream, xa);
-        hipLaunchKernelGGL(k_pois_z, dim3((K + 63) / 64), dim3(64), 0, ctx->stream, s->pS, s->pinvw, Nz, K);
+        Plan pz{};
+        pz.n = 2 * Nz;
+        pz.nst = (int)s->pradz.size();
+        for (int q = 0; q < pz.nst; ++q) pz.rad[q] = s->pradz[q];
+        pz.tw = s->ptwz;
+        int nlz = K / 256;
+        const int nlzmax = MAXLINE / (2 * Nz) < 4 ? MAXLINE / (2 * Nz) : 4;
+        nlz = nlz < 1 ? 1 : (nlz > nlzmax ? nlzmax : nlz);
+        PoisZArgs za{s->pS, pz, s->plamx, s->plamz, Nz, K, nlz};
+        hipLaunchKernelGGL(k_pois_zfft, dim3((unsigned)((K + nlz - 1) / nlz)), dim3(FFT_NT),
+                           2 * (size_t)nlz * 2 * Nz * sizeof(double2), ctx->stream, za);
         PoisInvArgs ia{s->pS, vP, zP, pl, d, F, s->pscale, Nz, Nx, K, nl};
         hipLaunchKernelGGL(k_pois_xinv, dim3(nbx), dim3(FFT_NT), lds, ctx->stream, ia);
     } else {
@@ -1105,7 +1319,7 @@ void precond(xc_ctx* ctx, StokesCtx* s, const Flags& F, const double* v, double*
 int multidot(xc_ctx* ctx, StokesCtx* s, const double* V, int K, const double* w, int64_t n) {
     const int nbk = DOT_BLOCKS;
     hipLaunchKernelGGL(k_multidot, dim3(nbk), dim3(NT), 0, ctx->stream, V, n, K, w, s->part);
-    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(NT), 0, ctx->stream, s->part, nbk, K + 1, s->dots);
+    hipLaunchKernelGGL(k_dot_final, dim3((K + 4) / 4), dim3(NT), 0, ctx->stream, s->part, nbk, K + 1, s->dots);
     XC_TRY_HIP(ctx, hipMemcpyAsync(s->host, s->dots, (K + 1) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return XC_OK;
@@ -1114,7 +1328,7 @@ int multidot(xc_ctx* ctx, StokesCtx* s, const double* V, int K, const double* w,
 // out[0..K] = (V[q] . w) for q < K and w.w, left on the device
 void multidot_dev(xc_ctx* ctx, StokesCtx* s, const double* V, int K, const double* w, int64_t n, double* out) {
     hipLaunchKernelGGL(k_multidot, dim3(DOT_BLOCKS), dim3(NT), 0, ctx->stream, V, n, K, w, s->part);
-    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(NT), 0, ctx->stream, s->part, DOT_BLOCKS, K + 1, out);
+    hipLaunchKernelGGL(k_dot_final, dim3((K + 4) / 4), dim3(NT), 0, ctx->stream, s->part, DOT_BLOCKS, K + 1, out);
 }
 
 int build_rhs(xc_ctx* ctx, StokesCtx* s, xc_fluid* fl, const xc_phase* ph, const xc_grid* g, const xc_par* par,
@@ -1168,8 +1382,163 @@ int build_rhs(xc_ctx* ctx, StokesCtx* s, xc_fluid* fl, const xc_phase* ph, const
         const int64_t kp = nW + nU + F.pr + (int64_t)F.pc * (Nz + 2);
         const int64_t q = F.nu0 - nW;
         hipLaunchKernelGGL(k_rr_pins, dim3(1), dim3(1), 0, ctx->stream, RR, kp, bn->P_exact, F.pr, F.pc, F.nu0,
-                           bn->U_exact, (int)(q % (Nz + 2)), (int)(q / (Nz + 2)));
+                           bn->U_exact, ((int)q % (Nz + 2)), ((int)q / (Nz + 2)));
     }
+    return XC_OK;
+}
+
+struct SolveOut { int iters; bool conv; double rel, rel0; };
+
+struct LcCfg {
+    bool on;
+    int level, f, Hc, Wc;
+    float is[4], os[3];
+    double eta_ref;
+};
+
+// learned correction settings for this solve (off unless requested, loaded and the grid divides)
+int lc_config(xc_ctx* ctx, StokesCtx* s, const xc_fm_opts* o, const xc_grid* g, const xc_par* par, LcCfg* lc) {
+    *lc = LcCfg{};
+    if (!o->learned || !xc_unet_has_params(ctx)) return XC_OK;
+    const int lev = o->lc_level > 0 ? o->lc_level : 2;
+    const int f = 1 << lev;
+    if (g->Nz % f || g->Nx % f) return xc_fail(ctx, XC_EINVAL_SHAPE, "fluidmech!: grid not divisible by 2^%d", lev);
+    const int Hc = (int)g->Nz / f, Wc = (int)g->Nx / f;
+    if (Hc % 8 || Wc % 8 || Hc < 8 || Wc < 8 || (int64_t)Hc * Wc > s->lc_cap)
+        return xc_fail(ctx, XC_EINVAL_SHAPE, "fluidmech!: learned correction grid %d x %d (multiples of 8 needed)", Hc, Wc);
+    lc->on = true;
+    lc->level = lev;
+    lc->f = f;
+    lc->Hc = Hc;
+    lc->Wc = Wc;
+    xc_unet_io_scales(ctx, lc->is, lc->os);
+    lc->eta_ref = par->etam0;
+    return XC_OK;
+}
+
+void lc_restrict(xc_ctx* ctx, StokesCtx* s, const LevD& d, const LcCfg& lc, const double* v, float* out, float* sigma,
+                 int compute) {
+    LcArgs a{};
+    a.L = d;
+    a.f = lc.f;
+    a.Hc = lc.Hc;
+    a.Wc = lc.Wc;
+    a.v = v;
+    a.out = out;
+    a.part = s->lc_part;
+    a.eta_ref = lc.eta_ref;
+    for (int k = 0; k < 4; ++k) a.s[k] = lc.is[k];
+    const int64_t nc = (int64_t)lc.Hc * lc.Wc;
+    const unsigned nbk = nblk(nc);
+    hipLaunchKernelGGL(k_lc_restrict, dim3(nbk), dim3(NT), 0, ctx->stream, a);
+    hipLaunchKernelGGL(k_lc_norm, dim3(1), dim3(1024), 0, ctx->stream, out, s->lc_part, (int)nbk, nc, sigma, compute);
+}
+
+// z = M^-1 v.  Plain: the block preconditioner.  Learned: the network's coarse estimate
+// e = P N(R v) of LL^-1 v first, then the block preconditioner on what it leaves,
+// z = e + M_plain (v - LL e); the multigrid smoothing removes the block-wise roughness
+// of the prolonged estimate.
+int precond_full(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int64_t n, const LcCfg& lc,
+                 const double* v, double* z) {
+    if (!lc.on) {
+        precond(ctx, s, F, v, z, mo);
+        return XC_OK;
+    }
+    const LevD dd = s->lev[0].d();
+    const unsigned nb = nblk(n);
+    lc_restrict(ctx, s, dd, lc, v, s->lc_in, s->lc_sigma, 1);
+    const float* e;
+    int st;
+    if ((st = xc_unet_infer(ctx, s->lc_in, 1, lc.Hc, lc.Wc, &e))) return st;
+    hipLaunchKernelGGL(k_zero, dim3(nb), dim3(NT), 0, ctx->stream, z, n);
+    LcProArgs pa{};
+    pa.L = dd;
+    pa.f = lc.f;
+    pa.Hc = lc.Hc;
+    pa.Wc = lc.Wc;
+    pa.e = e;
+    pa.sigma = s->lc_sigma;
+    for (int k = 0; k < 3; ++k) pa.os[k] = lc.os[k];
+    pa.z = z;
+    pa.n = n;
+    hipLaunchKernelGGL(k_lc_prolong, dim3(nb), dim3(NT), 0, ctx->stream, pa);
+    hipLaunchKernelGGL(k_ll_resid, dim3(nb), dim3(NT), 0, ctx->stream, dd, F, z, v, s->tmp, n);
+    precond(ctx, s, F, s->tmp, s->rs, mo);  // rs is free inside an Arnoldi cycle
+    hipLaunchKernelGGL(k_axpy1, dim3(nb), dim3(NT), 0, ctx->stream, s->rs, z, n);
+    return XC_OK;
+}
+
+// right-preconditioned FGMRES(m) on LL x = b from the given x; stops at ||b - LL x|| <= target
+int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int m, int64_t n, const double* b,
+               double* x, double target, double bnorm, int maxit, const LcCfg& lc, SolveOut* out) {
+    int st;
+    const unsigned nb = nblk(n);
+    const LevD dd = s->lev[0].d();
+    int iters = 0;
+    double rel = 0.0, rel0 = -1.0;
+    bool conv = false;
+    // The Arnoldi steps run on the device without host round trips: the Gram-Schmidt
+    // coefficients, the Hessenberg column and the Givens rotations never leave the GPU.
+    // The host learns the residual estimate |g[j+1]| of step j only after it has
+    // enqueued step j+1, so the device never waits for the host; a converged cycle
+    // therefore computes one basis vector it does not use.
+    Kry K{};
+    K.H = s->kry;
+    K.cs = K.H + (size_t)(m + 1) * m;
+    K.sn = K.cs + m;
+    K.g = K.sn + m;
+    K.y = K.g + m + 1;
+    double* d1 = s->d3;
+    double* d2 = d1 + (m + 2);
+    double* dn = d2 + (m + 2);
+    volatile double* hout = s->host + 8;  // 2 per step: |g[j+1]|, H[j+1, j]
+    while (true) {
+        hipLaunchKernelGGL(k_ll_resid, dim3(nb), dim3(NT), 0, ctx->stream, dd, F, x, b, s->rs, n);
+        if ((st = multidot(ctx, s, s->rs, 0, s->rs, n))) return st;
+        const double beta = sqrt(s->host[0]);
+        rel = bnorm > 0 ? beta / bnorm : beta;
+        if (rel0 < 0) rel0 = rel;
+        if (!isfinite(beta)) break;  // not converged; a learned solve falls back to the plain one
+        if (beta <= target) { conv = true; break; }
+        if (iters >= maxit) break;
+        hipLaunchKernelGGL(k_scale, dim3(nb), dim3(NT), 0, ctx->stream, s->rs, 1.0 / beta, s->V, n);
+        hipLaunchKernelGGL(k_kry_init, dim3(1), dim3(64), 0, ctx->stream, K, m, beta);
+        int used = 0;  // basis columns whose results are final
+        int j = 0;
+        for (; j < m; ++j) {
+            double* vj = s->V + (int64_t)j * n;
+            double* zj = s->Z + (int64_t)j * n;
+            if ((st = precond_full(ctx, s, F, mo, n, lc, vj, zj))) return st;
+            hipLaunchKernelGGL(k_ll_apply, dim3(nb), dim3(NT), 0, ctx->stream, dd, F, zj, s->ws, n);
+            multidot_dev(ctx, s, s->V, j + 1, s->ws, n, d1);
+            hipLaunchKernelGGL(k_orth, dim3(nb), dim3(NT), 0, ctx->stream, s->V, n, j + 1, d1, s->ws);
+            multidot_dev(ctx, s, s->V, j + 1, s->ws, n, d2);
+            hipLaunchKernelGGL(k_orth, dim3(nb), dim3(NT), 0, ctx->stream, s->V, n, j + 1, d2, s->ws);
+            multidot_dev(ctx, s, s->V, 0, s->ws, n, dn);
+            hipLaunchKernelGGL(k_arnoldi, dim3(1), dim3(64), 0, ctx->stream, K, m, j, d1, d2, dn, s->host_dev + 8);
+            if (j + 1 < m)
+                hipLaunchKernelGGL(k_scale_h, dim3(nb), dim3(NT), 0, ctx->stream, s->ws, dn + 1,
+                                   s->V + (int64_t)(j + 1) * n, n);
+            XC_TRY_HIP(ctx, hipEventRecord(s->ev[j & 1], ctx->stream));
+            if (j == 0) continue;
+            // step j-1 is complete once its event has fired
+            XC_TRY_HIP(ctx, hipEventSynchronize(s->ev[(j - 1) & 1]));
+            used = j;
+            ++iters;
+            if (hout[2 * (j - 1)] <= 0.5 * target || hout[2 * (j - 1) + 1] == 0.0 || iters >= maxit) break;
+        }
+        if (j == m) {  // the last step of the cycle was not tested yet
+            XC_TRY_HIP(ctx, hipEventSynchronize(s->ev[(m - 1) & 1]));
+            used = m;
+            ++iters;
+        }
+        hipLaunchKernelGGL(k_trisolve, dim3(1), dim3(64), 0, ctx->stream, K, m, used);
+        hipLaunchKernelGGL(k_update_x, dim3(nb), dim3(NT), 0, ctx->stream, s->Z, n, used, K.y, x);
+    }
+    out->iters = iters;
+    out->conv = conv;
+    out->rel = rel;
+    out->rel0 = rel0;
     return XC_OK;
 }
 
@@ -1236,14 +1605,11 @@ int xc_fluidmech(xc_ctx* ctx, xc_fluid* fl, const xc_phase* ph, const xc_grid* g
     const Flags F = make_flags(o, par, Nz, Nx, bn != nullptr);
     const int64_t n = s->lev[0].nv + (int64_t)(Nz + 2) * (Nx + 2);
     const unsigned nb = nblk(n);
-    const LevD d0 = s->lev[0].d();
-    (void)d0;
-    MGOpts mo{o->mg_pre > 0 ? o->mg_pre : 2, o->mg_post > 0 ? o->mg_post : 2, o->mg_coarse > 0 ? o->mg_coarse : 60,
+    MGOpts mo{o->mg_pre > 0 ? o->mg_pre : 2, o->mg_post > 0 ? o->mg_post : 2, o->mg_coarse > 0 ? o->mg_coarse : 20,
               o->mg_omega > 0 ? o->mg_omega : 0.7};
     // system: coefficients, RR, initial guess
     build_levels(ctx, s, fl, g->h);
     if ((st = pois_table(ctx, s, F, Nz, Nx, g->h))) return st;
-    const LevD dd = s->lev[0].d();
     if ((st = build_rhs(ctx, s, fl, ph, g, par, o, bn, F, s->bs))) return st;
     hipLaunchKernelGGL(k_gather, dim3(nb), dim3(NT), 0, ctx->stream, fl->W, fl->U, fl->P, s->xs, Nz, Nx);
     // ||RR||
@@ -1254,63 +1620,25 @@ int xc_fluidmech(xc_ctx* ctx, xc_fluid* fl, const xc_phase* ph, const xc_grid* g
     int iters = 0;
     double rel = 0.0, rel0 = -1.0;
     bool conv = false;
-    // The Arnoldi steps run on the device without host round trips: the Gram-Schmidt
-    // coefficients, the Hessenberg column and the Givens rotations never leave the GPU.
-    // The host learns the residual estimate |g[j+1]| of step j only after it has
-    // enqueued step j+1, so the device never waits for the host; a converged cycle
-    // therefore computes one basis vector it does not use.
-    Kry K{};
-    K.H = s->kry;
-    K.cs = K.H + (size_t)(m + 1) * m;
-    K.sn = K.cs + m;
-    K.g = K.sn + m;
-    K.y = K.g + m + 1;
-    double* d1 = s->d3;
-    double* d2 = d1 + (m + 2);
-    double* dn = d2 + (m + 2);
-    volatile double* hout = s->host + 8;  // 2 per step: |g[j+1]|, H[j+1, j]
-    while (true) {
-        hipLaunchKernelGGL(k_ll_resid, dim3(nb), dim3(NT), 0, ctx->stream, dd, F, s->xs, s->bs, s->rs, n);
-        if ((st = multidot(ctx, s, s->rs, 0, s->rs, n))) return st;
-        const double beta = sqrt(s->host[0]);
-        rel = bnorm > 0 ? beta / bnorm : beta;
-        if (rel0 < 0) rel0 = rel;
-        if (beta <= target) { conv = true; break; }
-        if (iters >= maxit) break;
-        hipLaunchKernelGGL(k_scale, dim3(nb), dim3(NT), 0, ctx->stream, s->rs, 1.0 / beta, s->V, n);
-        hipLaunchKernelGGL(k_kry_init, dim3(1), dim3(64), 0, ctx->stream, K, m, beta);
-        int used = 0;  // basis columns whose results are final
-        int j = 0;
-        for (; j < m; ++j) {
-            double* vj = s->V + (int64_t)j * n;
-            double* zj = s->Z + (int64_t)j * n;
-            precond(ctx, s, F, vj, zj, mo);
-            hipLaunchKernelGGL(k_ll_apply, dim3(nb), dim3(NT), 0, ctx->stream, dd, F, zj, s->ws, n);
-            multidot_dev(ctx, s, s->V, j + 1, s->ws, n, d1);
-            hipLaunchKernelGGL(k_orth, dim3(nb), dim3(NT), 0, ctx->stream, s->V, n, j + 1, d1, s->ws);
-            multidot_dev(ctx, s, s->V, j + 1, s->ws, n, d2);
-            hipLaunchKernelGGL(k_orth, dim3(nb), dim3(NT), 0, ctx->stream, s->V, n, j + 1, d2, s->ws);
-            multidot_dev(ctx, s, s->V, 0, s->ws, n, dn);
-            hipLaunchKernelGGL(k_arnoldi, dim3(1), dim3(64), 0, ctx->stream, K, m, j, d1, d2, dn, s->host_dev + 8);
-            if (j + 1 < m)
-                hipLaunchKernelGGL(k_scale_h, dim3(nb), dim3(NT), 0, ctx->stream, s->ws, dn + 1,
-                                   s->V + (int64_t)(j + 1) * n, n);
-            XC_TRY_HIP(ctx, hipEventRecord(s->ev[j & 1], ctx->stream));
-            if (j == 0) continue;
-            // step j-1 is complete once its event has fired
-            XC_TRY_HIP(ctx, hipEventSynchronize(s->ev[(j - 1) & 1]));
-            used = j;
-            ++iters;
-            if (hout[2 * (j - 1)] <= 0.5 * target || hout[2 * (j - 1) + 1] == 0.0 || iters >= maxit) break;
-        }
-        if (j == m) {  // the last step of the cycle was not tested yet
-            XC_TRY_HIP(ctx, hipEventSynchronize(s->ev[(m - 1) & 1]));
-            used = m;
-            ++iters;
-        }
-        hipLaunchKernelGGL(k_trisolve, dim3(1), dim3(64), 0, ctx->stream, K, m, used);
-        hipLaunchKernelGGL(k_update_x, dim3(nb), dim3(NT), 0, ctx->stream, s->Z, n, used, K.y, s->xs);
+    LcCfg lc{};
+    if ((st = lc_config(ctx, s, o, g, par, &lc))) return st;
+    SolveOut so{};
+    // a learned attempt gets a bounded budget before the plain preconditioner takes over
+    const int lmax = lc.on ? (maxit < 40 ? maxit : 40) : maxit;
+    if ((st = fgmres_run(ctx, s, F, mo, m, n, s->bs, s->xs, target, bnorm, lmax, lc, &so))) return st;
+    int fallback = 0;
+    if (lc.on && !so.conv) {  // the learned correction failed this system: plain multigrid from the start
+        fallback = 1;
+        hipLaunchKernelGGL(k_gather, dim3(nb), dim3(NT), 0, ctx->stream, fl->W, fl->U, fl->P, s->xs, Nz, Nx);
+        lc.on = false;
+        const int used = so.iters;
+        if ((st = fgmres_run(ctx, s, F, mo, m, n, s->bs, s->xs, target, bnorm, maxit, lc, &so))) return st;
+        so.iters += used;
     }
+    iters = so.iters;
+    conv = so.conv;
+    rel = so.rel;
+    rel0 = so.rel0;
     hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(NT), 0, ctx->stream, s->xs, fl->W, fl->U, fl->P, Nz, Nx);
     if (stats) {
         stats->iters = iters;
@@ -1318,7 +1646,7 @@ int xc_fluidmech(xc_ctx* ctx, xc_fluid* fl, const xc_phase* ph, const xc_grid* g
         stats->rel_resid = rel;
         stats->init_rel_resid = rel0;
         stats->mg_levels = (int)s->lev.size();
-        stats->fallback_used = 0;
+        stats->fallback_used = fallback;
     }
     if ((st = xc_finish(ctx, "fluidmech!"))) return st;
     if (!conv) return xc_fail(ctx, XC_ENOCONV, "fluidmech!: FGMRES reached %d iterations at relative residual %.3e",
@@ -1326,4 +1654,96 @@ int xc_fluidmech(xc_ctx* ctx, xc_fluid* fl, const xc_phase* ph, const xc_grid* g
     return XC_OK;
 }
 
+
+// shared set-up of the solver-piece entry points: coefficients, flags, Poisson table
+static int piece_setup(xc_ctx* ctx, const xc_fluid* fl, const xc_grid* g, const xc_par* par, const xc_fm_opts* o,
+                       int bnchm, int m, StokesCtx** sp, Flags* F, MGOpts* mo, int64_t* n) {
+    int st;
+    if ((st = check_fm(ctx, fl, g, o))) return st;
+    if ((st = setup(ctx, (int)g->Nz, (int)g->Nx, m, sp))) return st;
+    StokesCtx* s = *sp;
+    build_levels(ctx, s, fl, g->h);
+    *F = make_flags(o, par, (int)g->Nz, (int)g->Nx, bnchm != 0);
+    if ((st = pois_table(ctx, s, *F, (int)g->Nz, (int)g->Nx, g->h))) return st;
+    *mo = MGOpts{o->mg_pre > 0 ? o->mg_pre : 2, o->mg_post > 0 ? o->mg_post : 2, o->mg_coarse > 0 ? o->mg_coarse : 20,
+                 o->mg_omega > 0 ? o->mg_omega : 0.7};
+    *n = s->lev[0].nv + (g->Nz + 2) * (g->Nx + 2);
+    return XC_OK;
+}
+
+int xc_stokes_precond(xc_ctx* ctx, const xc_fluid* fl, const xc_grid* g, const xc_par* par, const xc_fm_opts* o,
+                      int bnchm, const double* v, double* z) {
+    XC_CHECK_CTX(ctx);
+    if (!v || !z) return xc_fail(ctx, XC_EINVAL_ARG, "stokes_precond: NULL vector");
+    StokesCtx* s;
+    Flags F;
+    MGOpts mo;
+    int64_t n;
+    int st;
+    if ((st = piece_setup(ctx, fl, g, par, o, bnchm, 1, &s, &F, &mo, &n))) return st;
+    precond(ctx, s, F, v, z, mo);
+    return xc_finish(ctx, "stokes_precond");
+}
+
+int xc_stokes_solve(xc_ctx* ctx, const xc_fluid* fl, const xc_grid* g, const xc_par* par, const xc_fm_opts* o,
+                    int bnchm, const double* rhs, double* x, xc_solve_stats* stats) {
+    XC_CHECK_CTX(ctx);
+    if (!rhs || !x) return xc_fail(ctx, XC_EINVAL_ARG, "stokes_solve: NULL vector");
+    const int m = o && o->restart > 0 ? (o->restart < 60 ? o->restart : 60) : 30;
+    StokesCtx* s;
+    Flags F;
+    MGOpts mo;
+    int64_t n;
+    int st;
+    if ((st = piece_setup(ctx, fl, g, par, o, bnchm, m, &s, &F, &mo, &n))) return st;
+    if ((st = multidot(ctx, s, rhs, 0, rhs, n))) return st;
+    const double bnorm = sqrt(s->host[0]);
+    const double target = o->tol * (bnorm > 0 ? bnorm : 1.0);
+    const int maxit = o->maxit > 0 ? o->maxit : 500;
+    LcCfg lc{};
+    if ((st = lc_config(ctx, s, o, g, par, &lc))) return st;
+    SolveOut so{};
+    if ((st = fgmres_run(ctx, s, F, mo, m, n, rhs, x, target, bnorm, maxit, lc, &so))) return st;
+    if (stats) {
+        stats->iters = so.iters;
+        stats->converged = so.conv;
+        stats->rel_resid = so.rel;
+        stats->init_rel_resid = so.rel0;
+        stats->mg_levels = (int)s->lev.size();
+        stats->fallback_used = 0;
+    }
+    if ((st = xc_finish(ctx, "stokes_solve"))) return st;
+    if (!so.conv) return xc_fail(ctx, XC_ENOCONV, "stokes_solve: FGMRES reached %d iterations at relative residual %.3e",
+                                 so.iters, so.rel);
+    return XC_OK;
+}
+
+int xc_stokes_lc_restrict(xc_ctx* ctx, const xc_fluid* fl, const xc_grid* g, const xc_par* par, int level,
+                          const double* vec, float* out4, float* sigma, int compute_sigma) {
+    XC_CHECK_CTX(ctx);
+    if (!vec || !out4 || !sigma) return xc_fail(ctx, XC_EINVAL_ARG, "stokes_lc_restrict: NULL buffer");
+    xc_fm_opts o{};
+    o.sds = -1;
+    o.top_cnv = 1;
+    o.bot_cnv = 1;
+    o.learned = 1;
+    o.lc_level = level;
+    int st;
+    if ((st = check_fm(ctx, fl, g, &o))) return st;
+    StokesCtx* s;
+    if ((st = setup(ctx, (int)g->Nz, (int)g->Nx, 1, &s))) return st;
+    build_levels(ctx, s, fl, g->h);
+    const int f = 1 << (level > 0 ? level : 2);
+    if (level < 1 || g->Nz % f || g->Nx % f) return xc_fail(ctx, XC_EINVAL_SHAPE, "stokes_lc_restrict: level %d", level);
+    LcCfg lc{};
+    lc.on = true;
+    lc.level = level;
+    lc.f = f;
+    lc.Hc = (int)g->Nz / f;
+    lc.Wc = (int)g->Nx / f;
+    xc_unet_io_scales(ctx, lc.is, lc.os);
+    lc.eta_ref = par->etam0;
+    lc_restrict(ctx, s, s->lev[0].d(), lc, vec, out4, sigma, compute_sigma);
+    return xc_finish(ctx, "stokes_lc_restrict");
+}
 }  // extern "C"

@@ -400,6 +400,9 @@ struct Net {
     float* out = nullptr;
     double* lpart = nullptr;
     double* lsum = nullptr;
+    bool has_params = false;
+    float in_scale[4] = {1.f, 1.f, 1.f, 1.f};
+    float out_scale[3] = {1.f, 1.f, 1.f};
 };
 
 void net_free(xc_ctx*, void* q) {
@@ -679,6 +682,7 @@ int xc_unet_set_params(xc_ctx* ctx, const float* host, int64_t n) {
     XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     net->adam_t = 0;
     net->wb_fresh = false;
+    net->has_params = true;
     return xc_finish(ctx, "unet_set_params");
 }
 
@@ -690,6 +694,17 @@ int xc_unet_get_params(xc_ctx* ctx, float* host, int64_t n) {
     if (!host || n != net->Ls.n) return xc_fail(ctx, XC_EINVAL_ARG, "unet: expected %lld parameters", (long long)net->Ls.n);
     XC_TRY_HIP(ctx, hipMemcpyAsync(host, net->th, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return XC_OK;
+}
+
+int xc_unet_set_io_scales(xc_ctx* ctx, const float in_scale[4], const float out_scale[3]) {
+    XC_CHECK_CTX(ctx);
+    Net* net;
+    int st;
+    if ((st = get_net(ctx, &net))) return st;
+    if (!in_scale || !out_scale) return xc_fail(ctx, XC_EINVAL_ARG, "unet_set_io_scales: NULL");
+    for (int k = 0; k < 4; ++k) net->in_scale[k] = in_scale[k];
+    for (int k = 0; k < 3; ++k) net->out_scale[k] = out_scale[k];
     return XC_OK;
 }
 
@@ -741,3 +756,24 @@ int xc_unet_train_step(xc_ctx* ctx, const float* in, const float* tgt, int B, in
 }
 
 }  // extern "C"
+
+int xc_unet_infer(xc_ctx* ctx, const float* in, int B, int H, int W, const float** out) {
+    Net* net;
+    int st;
+    if ((st = check_geom(ctx, B, H, W))) return st;
+    if ((st = get_net(ctx, &net))) return st;
+    if ((st = forward(ctx, net, in, B, H, W, nullptr))) return st;
+    *out = net->out;
+    return XC_OK;
+}
+
+bool xc_unet_has_params(xc_ctx* ctx) {
+    const Net* n = (const Net*)ctx->ext[XC_EXT_UNET];
+    return n && n->has_params;
+}
+
+void xc_unet_io_scales(xc_ctx* ctx, float in[4], float out[3]) {
+    const Net* n = (const Net*)ctx->ext[XC_EXT_UNET];
+    for (int k = 0; k < 4; ++k) in[k] = n ? n->in_scale[k] : 1.f;
+    for (int k = 0; k < 3; ++k) out[k] = n ? n->out_scale[k] : 1.f;
+}

@@ -12,9 +12,11 @@ from .kernels import (advect_, advect_centered_, advect_centered_with_flux_, adv
 from .state import (FluidState, Grid, NoiseState, Parameters, PhaseState, Scales, compute_scales)
 from .fields import (compute_fft_filter, noise_, normal_fill_, phsevo_, store_noise_, store_previous_, time_coefs,
                      update_, update_dt, update_phase_velocities_)
-from .stokes import BnchmData, fluidmech_, stokes_apply, stokes_rhs, stokes_vector_size
+from .stokes import (BnchmData, fluidmech_, lc_restrict, stokes_apply, stokes_precond, stokes_rhs, stokes_solve,
+                     stokes_vector_size)
 from .driver import (History, StepState, history_sums, initialize_, record_history_, run_, run_step_,
                      step_opts)
+from . import learned
 
 
 def load():

@@ -23,7 +23,8 @@ _D = ctypes.c_double
 class FmOpts(ctypes.Structure):
     _fields_ = [(n, _I) for n in ("sds", "top_cnv", "bot_cnv", "open_cnv", "xbc", "zbc", "scheme")] + \
                [(n, _D) for n in ("dt", "a1", "a2", "a3", "b1", "b2", "b3", "tol")] + \
-               [(n, _I) for n in ("maxit", "restart", "mg_pre", "mg_post", "mg_coarse")] + [("mg_omega", _D)]
+               [(n, _I) for n in ("maxit", "restart", "mg_pre", "mg_post", "mg_coarse")] + [("mg_omega", _D)] + \
+               [("learned", _I), ("lc_level", _I)]
 
 
 class BnchmStruct(ctypes.Structure):
@@ -38,6 +39,9 @@ class SolveStats(ctypes.Structure):
 _lib.register("xc_fluidmech", [_P, _P, _P, _P, _P, _P, _P, _P])
 _lib.register("xc_stokes_apply", [_P, _P, _P, _P, _P, _I, _P, _P])
 _lib.register("xc_stokes_rhs", [_P, _P, _P, _P, _P, _P, _P, _P])
+_lib.register("xc_stokes_precond", [_P, _P, _P, _P, _P, _I, _P, _P])
+_lib.register("xc_stokes_solve", [_P, _P, _P, _P, _P, _I, _P, _P, _P])
+_lib.register("xc_stokes_lc_restrict", [_P, _P, _P, _P, _I, _P, _P, _P, _I])
 
 
 class BnchmData:
@@ -54,11 +58,11 @@ class BnchmData:
 
 def fm_opts(par, sds=-1, top_cnv=1, bot_cnv=1, open_cnv=False, xBC="periodic", zBC="closed", dt=1e32,
             a1=1.0, a2=1.0, a3=0.0, b1=1.0, b2=0.0, b3=0.0, tol=1e-8, maxit=1000, restart=40,
-            mg_pre=2, mg_post=2, mg_coarse=60, mg_omega=0.7):
+            mg_pre=2, mg_post=2, mg_coarse=20, mg_omega=0.7, learned=False, lc_level=2):
     return FmOpts(int(sds), int(top_cnv), int(bot_cnv), int(bool(open_cnv)), _sym(xBC, BCS, "xBC"),
                   _sym(zBC, BCS, "zBC"), _sym(par.ADVN, SCHEMES, "scheme"), float(dt), float(a1), float(a2),
                   float(a3), float(b1), float(b2), float(b3), float(tol), int(maxit), int(restart), int(mg_pre),
-                  int(mg_post), int(mg_coarse), float(mg_omega))
+                  int(mg_post), int(mg_coarse), float(mg_omega), int(bool(learned)), int(lc_level))
 
 
 def fluidmech_(state, grid, par, phase=None, bnchm_data=None, raise_on_noconv=True, **kw):
@@ -108,3 +112,46 @@ def stokes_rhs(state, grid, par, phase=None, bnchm_data=None, **kw):
                                 ctypes.byref(grid_struct(grid)), ctypes.byref(par_struct(par)), ctypes.byref(o),
                                 ctypes.byref(bs) if bs is not None else None, _P(rr.data_ptr())), "stokes_rhs")
     return rr
+
+
+def stokes_precond(state, grid, par, v, bnchm=False, **kw):
+    """z = M^-1 v with the plain block preconditioner (no reference counterpart)."""
+    c = context()
+    o = fm_opts(par, **kw)
+    z = torch.empty_like(v)
+    fs = state.pack()
+    c.check(lib().xc_stokes_precond(c.ptr, ctypes.byref(fs), ctypes.byref(grid_struct(grid)),
+                                    ctypes.byref(par_struct(par)), ctypes.byref(o), int(bool(bnchm)),
+                                    _P(v.data_ptr()), _P(z.data_ptr())), "stokes_precond")
+    return z
+
+
+def stokes_solve(state, grid, par, rhs, x=None, bnchm=False, raise_on_noconv=True, **kw):
+    """FGMRES on LL x = rhs for any right-hand side (no reference counterpart); returns (x, stats)."""
+    c = context()
+    o = fm_opts(par, **kw)
+    x = torch.zeros_like(rhs) if x is None else x
+    fs = state.pack()
+    st = SolveStats()
+    code = lib().xc_stokes_solve(c.ptr, ctypes.byref(fs), ctypes.byref(grid_struct(grid)),
+                                 ctypes.byref(par_struct(par)), ctypes.byref(o), int(bool(bnchm)),
+                                 _P(rhs.data_ptr()), _P(x.data_ptr()), ctypes.byref(st))
+    stats = {k: getattr(st, k) for k, _ in SolveStats._fields_}
+    if not (code == _lib.XC_ENOCONV and not raise_on_noconv):
+        c.check(code, "stokes_solve")
+    return x, stats
+
+
+def lc_restrict(state, grid, par, level, vec, sigma=None):
+    """The learned correction's input map of a stacked vector: (out (1, Wc, Hc, 4) float32, sigma (1,))."""
+    c = context()
+    f = 1 << level
+    Hc, Wc = grid.Nz // f, grid.Nx // f
+    out = torch.empty((1, Wc, Hc, 4), dtype=torch.float32, device=vec.device)
+    compute = sigma is None
+    sg = torch.empty(1, dtype=torch.float32, device=vec.device) if compute else sigma
+    fs = state.pack()
+    c.check(lib().xc_stokes_lc_restrict(c.ptr, ctypes.byref(fs), ctypes.byref(grid_struct(grid)),
+                                        ctypes.byref(par_struct(par)), int(level), _P(vec.data_ptr()),
+                                        _P(out.data_ptr()), _P(sg.data_ptr()), int(compute)), "stokes_lc_restrict")
+    return out, sg
