@@ -97,6 +97,10 @@ def step_bench(args, learned=False):
     if learned:
         from xcore_amd import train_lcc
         lc_level = train_lcc.load(args.weights)
+    with xc.autotune():  # tile shapes for the step's advection calls, chosen once before timing
+        xc.run_step_(ph, fl, ns, g, par, sc, xc.StepState(state.time, state.dt, state.step),
+                     xc.step_opts(par, tol=1e-8, maxit=2000, fixed_sweeps=1))
+    par, g, sc, ph, fl, ns, hst, state = c3_setup(args.n)
     opts = xc.step_opts(par, tol=1e-8, maxit=2000, mg_pre=args.mg_pre, mg_post=args.mg_post, mg_coarse=args.mg_coarse,
                         learned=learned, lc_level=lc_level)
     for _ in range(args.warmup):
@@ -139,6 +143,10 @@ def stencil_bench(N, steps, warmup):
         else:
             xc.advect_centered_with_flux_(s["adv"], s["qx"], s["qz"], s["f"], u, w, h, kind)
 
+    with xc.autotune():  # the first call per scheme times the tile shapes and keeps the fastest
+        launch("weno5", sets[0])
+        launch("quick", sets[0])
+    tiles = {k: xc.advect_tile(k, N, N)[:3] for k in ("weno5", "quick")}
     for _ in range(warmup):
         for s in sets:
             for kind in ("weno5", "quick", "diffus"):
@@ -163,6 +171,10 @@ def stencil_bench(N, steps, warmup):
         gbs = nbytes / (us * 1e-6) / 1e9
         out[name] = {"avg_launch_us": round(us, 2), "bytes_per_launch": nbytes, "GB/s": round(gbs, 1),
                      "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+    for k in ("weno5", "quick"):
+        tz, tx, cand = tiles[k]
+        out[f"{k}_advect_with_flux"]["tile"] = f"{tz}x{tx}"
+        out[f"{k}_advect_with_flux"]["tile_candidates_us"] = cand
     wk = out["weno5_advect_with_flux"]
     roofline = {"bound": "hbm", "achieved": wk["GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": wk["hbm_frac"], "traffic": TRAFFIC_WENO5_2048,

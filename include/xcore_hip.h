@@ -49,6 +49,8 @@ enum { XC_PERIODIC = 0, XC_CLOSED = 1 };
 #define XC_CTX_SYNC 1u   /* synchronize the stream before every entry point returns */
 #define XC_CTX_EXACT 2u  /* reference operation order everywhere (bit-identical to a scalar
                             restatement); default: WENO5 weights with one division + FMA (<= 1e-12) */
+#define XC_CTX_AUTOTUNE 4u  /* the first advection call at a new (scheme, fluxes, grid) times every tile
+                               shape and keeps the fastest (synchronizes once); later calls reuse it */
 
 /* ---- context (no reference counterpart: replaces the KA backend object,
  *      backend.jl:12-13, as the owner of device state) ------------------- */
@@ -137,6 +139,13 @@ typedef struct {  /* parameters.jl:87-164: the knobs the hot path reads */
 
 typedef struct { double L0, L0h, l0, l0h, D0, G0, txi0; } xc_scales;  /* scales.jl:290-334 subset */
 
+/* Tile shape the advection kernels use at this (scheme, fluxes, grid): height tz (z) x width tx, and the
+ * measured time per launch of every candidate in us[4] (if non-NULL).  No counterpart in the
+ * reference (the autotuner of the tile shapes).  XC_ENOCONV: not tuned yet (the default shape). */
+int xc_advect_tile(xc_ctx* ctx, int scheme, int with_flux, int64_t Nz, int64_t Nx, int* tz, int* tx, float* us);
+/* pin the tile shape for (scheme, fluxes, grid), e.g. a choice tuned in an earlier run: 64x16, 128x8, 64x8, 32x32 */
+int xc_advect_set_tile(xc_ctx* ctx, int scheme, int with_flux, int64_t Nz, int64_t Nx, int tz, int tx);
+
 /* ---- field updates -------------------------------------------------------- */
 /* phsevo.jl:164-214 phsevo!(phase, fluid, grid, par, scales; ADVN, xBC, zBC, a1..b3, dt, alpha);
  * coef = {a1, a2, a3, b1, b2, b3} from time_coefs (timing.jl:18-36) */
@@ -223,6 +232,9 @@ int xc_stokes_solve(xc_ctx* ctx, const xc_fluid* fluid, const xc_grid* grid, con
                     const xc_fm_opts* opts, int bnchm, const double* rhs, double* x, xc_solve_stats* stats);
 int xc_stokes_lc_restrict(xc_ctx* ctx, const xc_fluid* fluid, const xc_grid* grid, const xc_par* par, int level,
                           const double* vec, float* out4, float* sigma, int compute_sigma);
+/* basis vector j of the last FGMRES cycle (unit norm) into out (the training data of the learned
+ * correction are the directions FGMRES actually feeds its preconditioner) */
+int xc_stokes_krylov_vector(xc_ctx* ctx, int j, double* out);
 
 /* ---- time loop (src/run.jl, src/diagnose.jl, src/history.jl) ------------ */
 typedef struct {  /* run! keywords (run.jl:167-181) and Picard controls (parameters.jl:150-154) */

@@ -186,3 +186,34 @@ def test_weno5_order_on_device(xc):
         errs.append(math.sqrt(np.sum((xc.to_host(adv) - exact) ** 2) / (N * N)))
     assert math.log2(errs[0] / errs[1]) > 4.0
     assert math.log2(errs[1] / errs[2]) > 3.5
+
+
+@pytest.mark.parametrize("scheme", ["weno5", "quick", "tvdim"])
+def test_every_tile_shape_bitwise(xc, orc, scheme):
+    # the autotuner's candidates compute the same faces in different tilings: all bit-identical
+    Nz, Nx = 200, 136
+    h = 1.0 / Nx
+    f, u, w = _inputs(Nz, Nx, 77)
+    adv_o, qx_o, qz_o = orc.zeros(Nz, Nx), orc.zeros(Nz + 2, Nx + 1), orc.zeros(Nz + 1, Nx + 2)
+    orc.advect_centered(adv_o, f, u, w, h, scheme, xbc="periodic", zbc="closed", qx=qx_o, qz=qz_o)
+    fd, ud, wd = xc.to_device(f), xc.to_device(u), xc.to_device(w)
+    with xc.exact_arithmetic():
+        for tz, tx in ((64, 16), (128, 8), (64, 8), (32, 32)):
+            print(f"tile {tz}x{tx} {scheme}", flush=True)
+            xc.set_advect_tile(scheme, Nz, Nx, tz, tx)
+            adv, qx, qz = xc.xcore_zeros(Nz, Nx), xc.xcore_zeros(Nz + 2, Nx + 1), xc.xcore_zeros(Nz + 1, Nx + 2)
+            xc.advect_centered_with_flux_(adv, qx, qz, fd, ud, wd, h, scheme)
+            torch.cuda.synchronize()
+            for got, want in ((adv, adv_o), (qx, qx_o), (qz, qz_o)):
+                assert np.array_equal(xc.to_host(got), want), (tz, tx)
+        with xc.autotune():
+            Nz2 = Nz + 8  # a grid not tuned yet
+            f2, u2, w2 = _inputs(Nz2, Nx, 78)
+            ref = orc.zeros(Nz2, Nx)
+            orc.advect_centered(ref, f2, u2, w2, h, scheme, xbc="periodic", zbc="closed")
+            adv2 = xc.xcore_zeros(Nz2, Nx)
+            xc.advect_centered_(adv2, xc.to_device(f2), xc.to_device(u2), xc.to_device(w2), h, scheme)
+            assert np.array_equal(xc.to_host(adv2), ref)
+        tz, tx, cand, tuned = xc.advect_tile(scheme, Nz + 8, Nx, with_flux=False)
+    print("tuned", scheme, tz, tx, cand, flush=True)
+    assert tuned and (tz, tx) in ((64, 16), (128, 8), (64, 8), (32, 32))

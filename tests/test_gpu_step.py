@@ -76,3 +76,43 @@ def test_run_conservation(xc):
     assert max(abs(e) for e in hst.EX) < 1e-6
     for k in ("X", "x"):
         assert bool(torch.isfinite(getattr(ph, k)).all())
+
+
+def test_config1_box_free_slip_vs_oracle(xc, orc):
+    # BASELINE config 1 mapped onto the reference: 10 km square box, 100 x 100 cells, free-slip walls
+    # (top_cnv = bot_cnv = -1), crystal fraction uniform random in [0.04, 0.06], WENO5, CFL 0.5.
+    # Three time steps on the device (FGMRES) against the oracle (sparse LU): fields and the
+    # conservation diagnostics agree.
+    N = 100
+    par = xc.Parameters(D=1e4, N=N, L=1e4, CFL=0.5, Da=0.05, x0=0.05)
+    g = xc.Grid.from_par(par)
+    sc = xc.compute_scales(par, g)
+    ph, fl = xc.PhaseState(N, N), xc.FluidState(N, N)
+    ns = xc.NoiseState(g, sc, seed=5)
+    rng = np.random.default_rng(2024)
+    x_init = xc.to_device(np.asfortranarray(rng.uniform(0.04, 0.06, (N, N))))
+    hst = xc.History()
+    t, dt, step = xc.initialize_(ph, fl, ns, hst, g, par, sc, x_init=x_init)
+    hph, hfl, hns = (_host_copy(orc, xc, ph, "phase"), _host_copy(orc, xc, fl, "fluid"),
+                     _host_copy(orc, xc, ns, "noise"))
+    state_h = dict(time=t, dt=dt, step=step, draws=ns.draws)
+    state_d = xc.StepState(t, dt, step)
+    opts = xc.step_opts(par, fixed_sweeps=3, top_cnv=-1, bot_cnv=-1, tol=1e-12, maxit=3000)
+    hst_h = xc.History()
+    for k in range(3):
+        print(f"config-1 step {k + 1}", flush=True)
+        stats = xc.run_step_(ph, fl, ns, g, par, sc, state_d, opts)
+        orc.run_step(hph, hfl, hns, g, par, sc, state_h, fixed_sweeps=3, top_cnv=-1, bot_cnv=-1, seed=5)
+        print(f"  {stats}", flush=True)
+        assert abs(state_d.dt - state_h["dt"]) <= 1e-6 * state_h["dt"]
+    torch.cuda.synchronize()
+    worst = {}
+    for n in ("X", "x", "W", "U", "P", "rho"):
+        src = hph if hasattr(hph, n) else hfl
+        worst[n] = _rel(getattr(ph if hasattr(ph, n) else fl, n).cpu().numpy(), getattr(src, n))
+    print("config-1 relative differences after 3 steps:", worst, flush=True)
+    assert max(worst.values()) <= 1e-6, worst
+    # mass sums of the device state equal the oracle's
+    s_dev = xc.history_sums(ph, fl)
+    assert abs(s_dev[0] - float(hfl.rho.sum())) <= 1e-12 * abs(float(hfl.rho.sum()))
+    assert abs(s_dev[2] - float(hph.X.sum())) <= 1e-10 * abs(float(hph.X.sum()))

@@ -25,9 +25,13 @@
 
 namespace {
 
-constexpr int TZ = 64;   // tile height (z, contiguous)
-constexpr int TX = 16;   // tile width (x)
+constexpr int TZ = 64;   // default tile height (z, contiguous)
+constexpr int TX = 16;   // default tile width (x)
 constexpr int NT = 256;  // lanes per workgroup
+// advection tile shapes the autotuner chooses from (variant 0 is the default)
+constexpr int NVAR = 4;
+constexpr int VAR_TZ[NVAR] = {64, 128, 64, 32};
+constexpr int VAR_TX[NVAR] = {16, 8, 8, 32};
 
 __host__ __device__ constexpr int halo_of(int s) {
     return s == XC_CENTR ? 1 : s == XC_UPWD1 ? 1 : s == XC_WENO5 ? 3 : 2;
@@ -120,30 +124,30 @@ __device__ __forceinline__ double face_flux(const double* c, int st, double vl, 
         double fpneg = tvd_flux(F(1), F(0), F(-1), vneg(vr), vneg(v));
         return vpos(v) * fppos + vneg(v) * fpneg;
     } else {
-        double fppos, fpneg;
+        // Only the upwind reconstruction can contribute: for v > 0, vpos(v) = v and vneg(v) = 0
+        // exactly (and the reverse for v < 0), so v * f_upwind has the bits of the reference's
+        // vpos(v) * fppos + vneg(v) * fpneg (advect.jl:613-617) for finite values.  The stencil is
+        // mirrored by selecting its points, not by branching.
+        const bool p = v > 0.0;
+        double fp;
         if constexpr (S == XC_UPWD1) {
-            fppos = F(-1);
-            fpneg = F(0);
+            fp = p ? F(-1) : F(0);
         } else if constexpr (S == XC_QUICK) {
             const double sixth = 1.0 / 6.0;
-            fppos = sixth * (2 * F(0) + 5 * F(-1) - F(-2));
-            fpneg = sixth * (2 * F(-1) + 5 * F(0) - F(1));
+            const double s0 = p ? F(-2) : F(1), s1 = p ? F(-1) : F(0), s2 = p ? F(0) : F(-1);
+            fp = sixth * (2 * s2 + 5 * s1 - s0);
         } else if constexpr (S == XC_FROMM) {
-            fppos = F(-1) + 0.25 * (F(0) - F(-2));
-            fpneg = F(0) + 0.25 * (F(-1) - F(1));
+            const double s0 = p ? F(-2) : F(1), s1 = p ? F(-1) : F(0), s2 = p ? F(0) : F(-1);
+            fp = s1 + 0.25 * (s2 - s0);
         } else if constexpr (S == XC_WENO3) {
-            fppos = weno3_poly(F(-2), F(-1), F(0));
-            fpneg = weno3_poly(F(1), F(0), F(-1));
+            fp = p ? weno3_poly(F(-2), F(-1), F(0)) : weno3_poly(F(1), F(0), F(-1));
         } else {
-            if constexpr (FAST) {
-                fppos = weno5_poly_fast(F(-3), F(-2), F(-1), F(0), F(1));
-                fpneg = weno5_poly_fast(F(2), F(1), F(0), F(-1), F(-2));
-            } else {
-                fppos = weno5_poly(F(-3), F(-2), F(-1), F(0), F(1));
-                fpneg = weno5_poly(F(2), F(1), F(0), F(-1), F(-2));
-            }
+            const double a0 = p ? F(-3) : F(2), a1 = p ? F(-2) : F(1), a2 = p ? F(-1) : F(0);
+            const double a3 = p ? F(0) : F(-1), a4 = p ? F(1) : F(-2);
+            if constexpr (FAST) fp = weno5_poly_fast(a0, a1, a2, a3, a4);
+            else fp = weno5_poly(a0, a1, a2, a3, a4);
         }
-        return vpos(v) * fppos + vneg(v) * fpneg;
+        return v * fp;
     }
 #undef F
 }
@@ -174,22 +178,23 @@ struct AdvArgs {
 
 // HIN: f is a halo'd buffer (advect!); otherwise f is (Nz, Nx) and the BCs are
 // an index map (advect_centered!).  FLUX: also write qx/qz with padding.
-template <int S, bool HIN, bool FLUX, bool FAST>
+template <int S, bool HIN, bool FLUX, bool FAST, int BZ, int BX>
 __global__ __launch_bounds__(NT) void k_advect(AdvArgs a) {
     constexpr int G = halo_of(S);
-    constexpr int PZ = TZ + 2 * G;
-    constexpr int PX = TX + 2 * G;
+    constexpr int PZ = BZ + 2 * G;
+    constexpr int PX = BX + 2 * G;
     __shared__ double fs[PX * PZ];
-    __shared__ double qxs[(TX + 1) * TZ];
-    __shared__ double qzs[TX * (TZ + 1)];
+    __shared__ double qxs[(BX + 1) * BZ];
+    __shared__ double qzs[BX * (BZ + 1)];
 
     const int t = xcd_tile(blockIdx.x, a.nb);
-    const int z0 = (t % a.nbz) * TZ;
-    const int x0 = (t / a.nbz) * TX;
+    const int z0 = (t % a.nbz) * BZ;
+    const int x0 = (t / a.nbz) * BX;
     const int tid = threadIdx.x;
     const int Nz = a.Nz, Nx = a.Nx;
 
     // ---- stage the tile + halo (cross-shaped use; corners loaded for simplicity)
+#pragma unroll
     for (int k = tid; k < PX * PZ; k += NT) {
         const int lz = k % PZ, lx = k / PZ;
         int gz = z0 + lz - G, gx = x0 + lx - G;
@@ -212,9 +217,10 @@ __global__ __launch_bounds__(NT) void k_advect(AdvArgs a) {
     const int qz_lft = (a.xbc == XC_PERIODIC && Nx > 1) ? Nx - 1 : 0;
     const int qz_rgt = (a.xbc == XC_PERIODIC && Nx > 1) ? 0 : Nx - 1;
 
-    // ---- x-faces: TZ rows x (TX+1) faces
-    for (int k = tid; k < TZ * (TX + 1); k += NT) {
-        const int lz = k % TZ, fx = k / TZ;
+    // ---- x-faces: BZ rows x (BX+1) faces
+#pragma unroll
+    for (int k = tid; k < BZ * (BX + 1); k += NT) {
+        const int lz = k % BZ, fx = k / BZ;
         const int iz = z0 + lz, F = x0 + fx;
         double q = 0.0;
         if (iz < Nz && F <= Nx) {
@@ -234,7 +240,7 @@ __global__ __launch_bounds__(NT) void k_advect(AdvArgs a) {
             }
             q = face_flux<S, FAST>(&fs[(fx + G) * PZ + lz + G], PZ, vl, v, vr);
             if constexpr (FLUX) {
-                if (fx < TX || F == Nx) {
+                if (fx < BX || F == Nx) {
                     double* col = a.qx + (int64_t)F * a.ld_qx;
                     col[iz + 1] = q;
                     if (iz == qx_top) col[0] = q;
@@ -242,11 +248,12 @@ __global__ __launch_bounds__(NT) void k_advect(AdvArgs a) {
                 }
             }
         }
-        qxs[fx * TZ + lz] = q;
+        qxs[fx * BZ + lz] = q;
     }
-    // ---- z-faces: (TZ+1) faces x TX columns
-    for (int k = tid; k < (TZ + 1) * TX; k += NT) {
-        const int fz = k % (TZ + 1), lx = k / (TZ + 1);
+    // ---- z-faces: (BZ+1) faces x BX columns
+#pragma unroll
+    for (int k = tid; k < (BZ + 1) * BX; k += NT) {
+        const int fz = k % (BZ + 1), lx = k / (BZ + 1);
         const int F = z0 + fz, ix = x0 + lx;
         double q = 0.0;
         if (ix < Nx && F <= Nz) {
@@ -266,49 +273,121 @@ __global__ __launch_bounds__(NT) void k_advect(AdvArgs a) {
             }
             q = face_flux<S, FAST>(&fs[(lx + G) * PZ + fz + G], 1, vl, v, vr);
             if constexpr (FLUX) {
-                if (fz < TZ || F == Nz) {
+                if (fz < BZ || F == Nz) {
                     a.qz[F + (int64_t)(ix + 1) * a.ld_qz] = q;
                     if (ix == qz_lft) a.qz[F] = q;
                     if (ix == qz_rgt) a.qz[F + (int64_t)(Nx + 1) * a.ld_qz] = q;
                 }
             }
         }
-        qzs[lx * (TZ + 1) + fz] = q;
+        qzs[lx * (BZ + 1) + fz] = q;
     }
     __syncthreads();
 
     // ---- divergence, advect.jl:94: (qxr - qxl + qzb - qzt) * invh
-    for (int k = tid; k < TZ * TX; k += NT) {
-        const int lz = k % TZ, lx = k / TZ;
+#pragma unroll
+    for (int k = tid; k < BZ * BX; k += NT) {
+        const int lz = k % BZ, lx = k / BZ;
         const int iz = z0 + lz, ix = x0 + lx;
         if (iz < Nz && ix < Nx) {
-            const double qxr = qxs[(lx + 1) * TZ + lz], qxl = qxs[lx * TZ + lz];
-            const double qzb = qzs[lx * (TZ + 1) + lz + 1], qzt = qzs[lx * (TZ + 1) + lz];
+            const double qxr = qxs[(lx + 1) * BZ + lz], qxl = qxs[lx * BZ + lz];
+            const double qzb = qzs[lx * (BZ + 1) + lz + 1], qzt = qzs[lx * (BZ + 1) + lz];
             a.adv[iz + (int64_t)ix * a.ld_adv] = (qxr - qxl + qzb - qzt) * a.invh;
         }
     }
 }
 
-template <int S, bool HIN, bool FLUX>
-hipError_t launch_advect_t(const AdvArgs& a0, hipStream_t s, bool exact) {
+template <int S, bool HIN, bool FLUX, bool FAST, int V>
+hipError_t launch_advect_v(const AdvArgs& a0, hipStream_t s) {
+    constexpr int BZ = VAR_TZ[V], BX = VAR_TX[V];
     AdvArgs a = a0;
-    // only WENO5 has a distinct fast form; every other scheme is the reference arithmetic
-    if (S == XC_WENO5 && !exact) hipLaunchKernelGGL((k_advect<S, HIN, FLUX, true>), dim3(a.nb), dim3(NT), 0, s, a);
-    else hipLaunchKernelGGL((k_advect<S, HIN, FLUX, false>), dim3(a.nb), dim3(NT), 0, s, a);
+    a.nbz = (a.Nz + BZ - 1) / BZ;
+    a.nb = a.nbz * ((a.Nx + BX - 1) / BX);
+    hipLaunchKernelGGL((k_advect<S, HIN, FLUX, FAST, BZ, BX>), dim3(a.nb), dim3(NT), 0, s, a);
     return hipGetLastError();
 }
 
-template <bool HIN, bool FLUX>
-hipError_t launch_advect_s(int scheme, const AdvArgs& a, hipStream_t s, bool exact) {
-    switch (scheme) {
-        case XC_CENTR: return launch_advect_t<XC_CENTR, HIN, FLUX>(a, s, exact);
-        case XC_UPWD1: return launch_advect_t<XC_UPWD1, HIN, FLUX>(a, s, exact);
-        case XC_QUICK: return launch_advect_t<XC_QUICK, HIN, FLUX>(a, s, exact);
-        case XC_FROMM: return launch_advect_t<XC_FROMM, HIN, FLUX>(a, s, exact);
-        case XC_WENO3: return launch_advect_t<XC_WENO3, HIN, FLUX>(a, s, exact);
-        case XC_WENO5: return launch_advect_t<XC_WENO5, HIN, FLUX>(a, s, exact);
-        default: return launch_advect_t<XC_TVDIM, HIN, FLUX>(a, s, exact);
+template <int S, bool HIN, bool FLUX, bool FAST>
+hipError_t launch_advect_f(const AdvArgs& a, hipStream_t s, int v) {
+    if constexpr (HIN) {  // halo'd-input entry points keep the default tile
+        return launch_advect_v<S, HIN, FLUX, FAST, 0>(a, s);
+    } else {
+        switch (v) {
+            case 1: return launch_advect_v<S, HIN, FLUX, FAST, 1>(a, s);
+            case 2: return launch_advect_v<S, HIN, FLUX, FAST, 2>(a, s);
+            case 3: return launch_advect_v<S, HIN, FLUX, FAST, 3>(a, s);
+            default: return launch_advect_v<S, HIN, FLUX, FAST, 0>(a, s);
+        }
     }
+}
+
+template <int S, bool HIN, bool FLUX>
+hipError_t launch_advect_t(const AdvArgs& a, hipStream_t s, bool exact, int v) {
+    // only WENO5 has a distinct fast form; every other scheme is the reference arithmetic
+    if constexpr (S == XC_WENO5) {
+        if (!exact) return launch_advect_f<S, HIN, FLUX, true>(a, s, v);
+    }
+    return launch_advect_f<S, HIN, FLUX, false>(a, s, v);
+}
+
+template <bool HIN, bool FLUX>
+hipError_t launch_advect_s(int scheme, const AdvArgs& a, hipStream_t s, bool exact, int v) {
+    switch (scheme) {
+        case XC_CENTR: return launch_advect_t<XC_CENTR, HIN, FLUX>(a, s, exact, v);
+        case XC_UPWD1: return launch_advect_t<XC_UPWD1, HIN, FLUX>(a, s, exact, v);
+        case XC_QUICK: return launch_advect_t<XC_QUICK, HIN, FLUX>(a, s, exact, v);
+        case XC_FROMM: return launch_advect_t<XC_FROMM, HIN, FLUX>(a, s, exact, v);
+        case XC_WENO3: return launch_advect_t<XC_WENO3, HIN, FLUX>(a, s, exact, v);
+        case XC_WENO5: return launch_advect_t<XC_WENO5, HIN, FLUX>(a, s, exact, v);
+        default: return launch_advect_t<XC_TVDIM, HIN, FLUX>(a, s, exact, v);
+    }
+}
+
+// ---- tile autotuner: per (scheme, fluxes, arithmetic, grid) the fastest tile shape, measured once
+struct TuneEntry { int scheme, flux, exact, Nz, Nx, v; float us[NVAR]; };
+struct TuneCtx { std::vector<TuneEntry> e; };
+void tune_free(xc_ctx*, void* p) { delete (TuneCtx*)p; }
+TuneCtx* tune_ctx(xc_ctx* ctx) {
+    TuneCtx* t = (TuneCtx*)ctx->ext[XC_EXT_TUNE];
+    if (!t) {
+        t = new TuneCtx();
+        ctx->ext[XC_EXT_TUNE] = t;
+        ctx->ext_free[XC_EXT_TUNE] = tune_free;
+    }
+    return t;
+}
+TuneEntry* tune_find(xc_ctx* ctx, int scheme, int flux, int exact, int Nz, int Nx) {
+    for (auto& e : tune_ctx(ctx)->e)
+        if (e.scheme == scheme && e.flux == flux && e.exact == exact && e.Nz == Nz && e.Nx == Nx) return &e;
+    return nullptr;
+}
+// time every tile variant (3 launches each, HIP events on the context's stream) and keep the fastest;
+// every variant computes the same values, so the outputs are those of the last launch either way
+int tune_advect(xc_ctx* ctx, int scheme, bool flux, bool ex, const AdvArgs& a, int* out) {
+    TuneEntry t{scheme, flux ? 1 : 0, ex ? 1 : 0, a.Nz, a.Nx, 0, {}};
+    hipEvent_t e0, e1;
+    XC_TRY_HIP(ctx, hipEventCreate(&e0));
+    XC_TRY_HIP(ctx, hipEventCreate(&e1));
+    float best = 1e30f;
+    for (int v = 0; v < NVAR; ++v) {
+        hipError_t e = hipSuccess;
+        for (int r = 0; r < 4 && e == hipSuccess; ++r) {  // first launch warms up
+            if (r == 1) (void)hipEventRecord(e0, ctx->stream);
+            e = flux ? launch_advect_s<false, true>(scheme, a, ctx->stream, ex, v)
+                     : launch_advect_s<false, false>(scheme, a, ctx->stream, ex, v);
+        }
+        (void)hipEventRecord(e1, ctx->stream);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        t.us[v] = e == hipSuccess ? 1e3f * ms / 3.f : 1e30f;
+        if (t.us[v] < best) { best = t.us[v]; t.v = v; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    tune_ctx(ctx)->e.push_back(t);
+    *out = t.v;
+    return XC_OK;
 }
 
 int advect_common(xc_ctx* ctx, int scheme, bool hin, xc_mat adv, const xc_mat* qx, const xc_mat* qz,
@@ -352,15 +431,21 @@ int advect_common(xc_ctx* ctx, int scheme, bool hin, xc_mat adv, const xc_mat* q
     a.u = u.p; a.ld_u = u.ld;
     a.w = w.p; a.ld_w = w.ld;
     a.Nz = (int)Nz; a.Nx = (int)Nx;
-    a.nbz = (int)((Nz + TZ - 1) / TZ);
-    const int nbx = (int)((Nx + TX - 1) / TX);
-    a.nb = a.nbz * nbx;
     a.invh = 1.0 / h;
     a.xbc = xbc; a.zbc = zbc;
     hipError_t e;
     const bool ex = (ctx->flags & XC_CTX_EXACT) != 0;
-    if (hin) e = flux ? launch_advect_s<true, true>(scheme, a, ctx->stream, ex) : launch_advect_s<true, false>(scheme, a, ctx->stream, ex);
-    else     e = flux ? launch_advect_s<false, true>(scheme, a, ctx->stream, ex) : launch_advect_s<false, false>(scheme, a, ctx->stream, ex);
+    int v = 0;
+    if (!hin) {
+        const TuneEntry* te = tune_find(ctx, scheme, flux, ex, a.Nz, a.Nx);
+        if (te) v = te->v;
+        else if (ctx->flags & XC_CTX_AUTOTUNE) {
+            int st;
+            if ((st = tune_advect(ctx, scheme, flux, ex, a, &v))) return st;
+        }
+    }
+    if (hin) e = flux ? launch_advect_s<true, true>(scheme, a, ctx->stream, ex, 0) : launch_advect_s<true, false>(scheme, a, ctx->stream, ex, 0);
+    else     e = flux ? launch_advect_s<false, true>(scheme, a, ctx->stream, ex, v) : launch_advect_s<false, false>(scheme, a, ctx->stream, ex, v);
     if (e != hipSuccess) return xc_fail(ctx, XC_EHIP, "%s: launch: %s", name, hipGetErrorString(e));
     return xc_finish(ctx, name);
 }
@@ -437,6 +522,67 @@ __global__ __launch_bounds__(NT) void k_diffus(DifArgs a) {
     }
 }
 
+// LDS-tiled form of k_diffus for the centred entry points (f, k are (Nz, Nx) and the BCs are an index
+// map): the 64 x 16 tile of f and of the diffusivity (k, or k .* k2 formed once per cell) plus a
+// one-cell halo is staged once, so each cell costs two global loads instead of ten.  Same arithmetic
+// and operation order as k_diffus.
+template <bool FLUX>
+__global__ __launch_bounds__(NT) void k_diffus_lds(DifArgs a) {
+    constexpr int PZ = TZ + 2, PX = TX + 2;
+    __shared__ double fs[PX * PZ];
+    __shared__ double ks[PX * PZ];
+    const int t = xcd_tile(blockIdx.x, a.nb);
+    const int z0 = (t % a.nbz) * TZ;
+    const int x0 = (t / a.nbz) * TX;
+    const int Nz = a.Nz, Nx = a.Nx;
+#pragma unroll
+    for (int k = threadIdx.x; k < PX * PZ; k += NT) {
+        const int lz = k % PZ, lx = k / PZ;
+        int gz = z0 + lz - 1, gx = x0 + lx - 1;
+        gz = gz < Nz ? gz : Nz;  // rows/columns past the grid end map like the one after the last
+        gx = gx < Nx ? gx : Nx;
+        const int64_t hz = map_cell(gz, Nz, a.zbc), hx = map_cell(gx, Nx, a.xbc);
+        fs[k] = a.f[hz + hx * a.ld_f];
+        ks[k] = a.k2 ? a.k[hz + hx * a.ld_k] * a.k2[hz + hx * a.ld_k2] : a.k[hz + hx * a.ld_k];
+    }
+    __syncthreads();
+    const int qx_top = (a.zbc == XC_PERIODIC && Nz > 1) ? Nz - 1 : 0;
+    const int qx_bot = (a.zbc == XC_PERIODIC && Nz > 1) ? 0 : Nz - 1;
+    const int qz_lft = (a.xbc == XC_PERIODIC && Nx > 1) ? Nx - 1 : 0;
+    const int qz_rgt = (a.xbc == XC_PERIODIC && Nx > 1) ? 0 : Nx - 1;
+#pragma unroll
+    for (int k = threadIdx.x; k < TZ * TX; k += NT) {
+        const int lz = k % TZ, lx = k / TZ;
+        const int iz = z0 + lz, ix = x0 + lx;
+        if (iz >= Nz || ix >= Nx) continue;
+        const int c = (lx + 1) * PZ + lz + 1;
+        const double fc = fs[c], fmx = fs[c - PZ], fpx = fs[c + PZ], fmz = fs[c - 1], fpz = fs[c + 1];
+        const double kc = ks[c], kmx = ks[c - PZ], kpx = ks[c + PZ], kmz = ks[c - 1], kpz = ks[c + 1];
+        const double flux_xr = (kpx + kc) * (fpx - fc);
+        const double flux_xl = (kc + kmx) * (fc - fmx);
+        const double flux_zb = (kpz + kc) * (fpz - fc);
+        const double flux_zt = (kc + kmz) * (fc - fmz);
+        a.dff[iz + (int64_t)ix * a.ld_d] = 0.5 * a.invh2 * (flux_xr - flux_xl + flux_zb - flux_zt);
+        if constexpr (FLUX) {
+            auto put_qx = [&](int F, double q) {
+                double* col = a.qx + (int64_t)F * a.ld_qx;
+                col[iz + 1] = q;
+                if (iz == qx_top) col[0] = q;
+                if (iz == qx_bot) col[Nz + 1] = q;
+            };
+            auto put_qz = [&](int F, double q) {
+                a.qz[F + (int64_t)(ix + 1) * a.ld_qz] = q;
+                if (ix == qz_lft) a.qz[F] = q;
+                if (ix == qz_rgt) a.qz[F + (int64_t)(Nx + 1) * a.ld_qz] = q;
+            };
+            put_qx(ix, -(kmx + kc) * 0.5 * (fc - fmx) * a.invh);
+            if (ix == Nx - 1) put_qx(Nx, -(kc + kpx) * 0.5 * (fpx - fc) * a.invh);
+            put_qz(iz, -(kmz + kc) * 0.5 * (fc - fmz) * a.invh);
+            if (iz == Nz - 1) put_qz(Nz, -(kc + kpz) * 0.5 * (fpz - fc) * a.invh);
+        }
+    }
+}
+
 int diffus_common(xc_ctx* ctx, bool hin, int halo, xc_mat dff, const xc_mat* qx, const xc_mat* qz,
                   xc_mat f, xc_mat k, double h, int xbc, int zbc, const char* name, const xc_mat* k2 = nullptr) {
     XC_CHECK_CTX(ctx);
@@ -476,8 +622,10 @@ int diffus_common(xc_ctx* ctx, bool hin, int halo, xc_mat dff, const xc_mat* qx,
         if (flux) hipLaunchKernelGGL((k_diffus<true, true>), dim3(a.nb), dim3(NT), 0, ctx->stream, a);
         else hipLaunchKernelGGL((k_diffus<true, false>), dim3(a.nb), dim3(NT), 0, ctx->stream, a);
     } else {
-        if (flux) hipLaunchKernelGGL((k_diffus<false, true>), dim3(a.nb), dim3(NT), 0, ctx->stream, a);
-        else hipLaunchKernelGGL((k_diffus<false, false>), dim3(a.nb), dim3(NT), 0, ctx->stream, a);
+        a.nbz = (int)((Nz + TZ - 1) / TZ);
+        a.nb = a.nbz * (int)((Nx + TX - 1) / TX);
+        if (flux) hipLaunchKernelGGL((k_diffus_lds<true>), dim3(a.nb), dim3(NT), 0, ctx->stream, a);
+        else hipLaunchKernelGGL((k_diffus_lds<false>), dim3(a.nb), dim3(NT), 0, ctx->stream, a);
     }
     return xc_finish(ctx, name);
 }
@@ -585,6 +733,31 @@ int xc_diffus_centered(xc_ctx* ctx, xc_mat dff, const xc_mat* qx, const xc_mat* 
                        int xbc, int zbc) {
     return diffus_common(ctx, false, 1, dff, qx, qz, f, k, h, xbc, zbc,
                          qx ? "diffus_centered_with_flux!" : "diffus_centered!");
+}
+
+int xc_advect_tile(xc_ctx* ctx, int scheme, int with_flux, int64_t Nz, int64_t Nx, int* tz, int* tx, float* us) {
+    XC_CHECK_CTX(ctx);
+    if (!tz || !tx) return xc_fail(ctx, XC_EINVAL_ARG, "advect_tile: NULL output");
+    const TuneEntry* te = tune_find(ctx, scheme, with_flux ? 1 : 0, (ctx->flags & XC_CTX_EXACT) ? 1 : 0, (int)Nz, (int)Nx);
+    const int v = te ? te->v : 0;
+    *tz = VAR_TZ[v];
+    *tx = VAR_TX[v];
+    if (us)
+        for (int k = 0; k < NVAR; ++k) us[k] = te ? te->us[k] : 0.f;
+    return te ? XC_OK : XC_ENOCONV;
+}
+
+int xc_advect_set_tile(xc_ctx* ctx, int scheme, int with_flux, int64_t Nz, int64_t Nx, int tz, int tx) {
+    XC_CHECK_CTX(ctx);
+    int v = -1;
+    for (int k = 0; k < NVAR; ++k)
+        if (VAR_TZ[k] == tz && VAR_TX[k] == tx) v = k;
+    if (v < 0) return xc_fail(ctx, XC_EINVAL_ARG, "advect_set_tile: no %d x %d tile", tz, tx);
+    const int ex = (ctx->flags & XC_CTX_EXACT) ? 1 : 0;
+    TuneEntry* te = tune_find(ctx, scheme, with_flux ? 1 : 0, ex, (int)Nz, (int)Nx);
+    if (te) te->v = v;
+    else tune_ctx(ctx)->e.push_back(TuneEntry{scheme, with_flux ? 1 : 0, ex, (int)Nz, (int)Nx, v, {}});
+    return XC_OK;
 }
 
 }  // extern "C"

@@ -1746,4 +1746,13 @@ int xc_stokes_lc_restrict(xc_ctx* ctx, const xc_fluid* fl, const xc_grid* g, con
     lc_restrict(ctx, s, s->lev[0].d(), lc, vec, out4, sigma, compute_sigma);
     return xc_finish(ctx, "stokes_lc_restrict");
 }
+
+int xc_stokes_krylov_vector(xc_ctx* ctx, int j, double* out) {
+    XC_CHECK_CTX(ctx);
+    StokesCtx* s = (StokesCtx*)ctx->ext[XC_EXT_STOKES];
+    if (!s || !out || j < 0 || j > s->m) return xc_fail(ctx, XC_EINVAL_ARG, "stokes_krylov_vector: no basis vector %d", j);
+    const int64_t n = s->lev[0].nv + (int64_t)(s->Nz + 2) * (s->Nx + 2);
+    XC_TRY_HIP(ctx, hipMemcpyAsync(out, s->V + (int64_t)j * n, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    return xc_finish(ctx, "stokes_krylov_vector");
+}
 }  // extern "C"

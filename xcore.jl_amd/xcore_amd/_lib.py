@@ -20,6 +20,7 @@ XC_EINVAL_ARG = 6
 XC_ENOMEM = 7
 XC_CTX_SYNC = 1
 XC_CTX_EXACT = 2
+XC_CTX_AUTOTUNE = 4
 
 SCHEMES = {"centr": 0, "upwd1": 1, "quick": 2, "fromm": 3, "weno3": 4, "weno5": 5, "tvdim": 6}
 BCS = {"periodic": 0, "closed": 1}
@@ -70,6 +71,9 @@ _PROTOS = {
     "xc_diffus": [_P, _M, _M, _M, _D, _I],
     "xc_diffus_with_flux": [_P, _M, _M, _M, _M, _M, _D, _I, _I, _I],
     "xc_diffus_centered": [_P, _M, _PM, _PM, _M, _M, _D, _I, _I],
+    "xc_advect_set_tile": [_P, _I, _I, ctypes.c_int64, ctypes.c_int64, _I, _I],
+    "xc_advect_tile": [_P, _I, _I, ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(_I), ctypes.POINTER(_I),
+                       ctypes.POINTER(ctypes.c_float)],
 }
 _RESTYPE = {"xc_last_error": ctypes.c_char_p, "xc_get_stream": _P}
 _EXTRA = []  # (name, argtypes, restype) registered by other modules
@@ -135,8 +139,11 @@ class Context:
     def bind_stream(self, stream_handle):
         lib().xc_set_stream(self.ptr, _P(stream_handle))
 
+    flags = 0
+
     def set_flags(self, flags):
         self.check(lib().xc_set_flags(self.ptr, int(flags)), "xc_set_flags")
+        self.flags = int(flags)
 
     def sync(self):
         self.check(lib().xc_sync(self.ptr), "xc_sync")

@@ -27,16 +27,39 @@ def context(device=None):
     return c
 
 
-class exact_arithmetic:
-    """Context manager: run the stencils in the reference's exact operation order (bit-identical)."""
+class _flag:
+    bit = 0
 
     def __enter__(self):
         self.c = context()
-        self.c.set_flags(_lib.XC_CTX_EXACT)
+        self.old = self.c.flags
+        self.c.set_flags(self.old | self.bit)
         return self
 
     def __exit__(self, *a):
-        self.c.set_flags(0)
+        self.c.set_flags(self.old)
+
+
+class exact_arithmetic(_flag):
+    """Context manager: run the stencils in the reference's exact operation order (bit-identical)."""
+    bit = _lib.XC_CTX_EXACT
+
+
+class autotune(_flag):
+    """Context manager: advection calls at a new (scheme, fluxes, grid) time every tile shape once and keep
+    the fastest (the choice persists in the device context)."""
+    bit = _lib.XC_CTX_AUTOTUNE
+
+
+def advect_tile(scheme, Nz, Nx, with_flux=True):
+    """(tz, tx, {tile: us}) the advection kernels use at this size; tuned=False if not measured yet."""
+    c = context()
+    tz, tx = ctypes.c_int(0), ctypes.c_int(0)
+    us = (ctypes.c_float * 4)()
+    code = lib().xc_advect_tile(c.ptr, _sym(scheme, SCHEMES, "scheme"), int(bool(with_flux)), int(Nz), int(Nx),
+                                ctypes.byref(tz), ctypes.byref(tx), us)
+    shapes = [(64, 16), (128, 8), (64, 8), (32, 32)]
+    return tz.value, tx.value, {f"{a}x{b}": round(float(u), 2) for (a, b), u in zip(shapes, us)}, code == 0
 
 
 def _sym(s, table, kind):
@@ -143,3 +166,10 @@ def diffus_centered_with_flux_(dff, qx, qz, f, k, h, xBC="periodic", zBC="closed
                                      _sym(xBC, BCS, "xBC"), _sym(zBC, BCS, "zBC")),
             "diffus_centered_with_flux!")
     return dff
+
+
+def set_advect_tile(scheme, Nz, Nx, tz, tx, with_flux=True):
+    """Pin the advection tile shape for this (scheme, fluxes, grid): one of 64x16, 128x8, 64x8, 32x32."""
+    c = context()
+    c.check(lib().xc_advect_set_tile(c.ptr, _sym(scheme, SCHEMES, "scheme"), int(bool(with_flux)), int(Nz), int(Nx),
+                                     int(tz), int(tx)), "advect_set_tile")

@@ -42,6 +42,7 @@ _lib.register("xc_stokes_rhs", [_P, _P, _P, _P, _P, _P, _P, _P])
 _lib.register("xc_stokes_precond", [_P, _P, _P, _P, _P, _I, _P, _P])
 _lib.register("xc_stokes_solve", [_P, _P, _P, _P, _P, _I, _P, _P, _P])
 _lib.register("xc_stokes_lc_restrict", [_P, _P, _P, _P, _I, _P, _P, _P, _I])
+_lib.register("xc_stokes_krylov_vector", [_P, _I, _P])
 
 
 class BnchmData:
@@ -155,3 +156,11 @@ def lc_restrict(state, grid, par, level, vec, sigma=None):
                                         ctypes.byref(par_struct(par)), int(level), _P(vec.data_ptr()),
                                         _P(out.data_ptr()), _P(sg.data_ptr()), int(compute)), "stokes_lc_restrict")
     return out, sg
+
+
+def krylov_vector(grid, j):
+    """Basis vector j of the last FGMRES cycle (no reference counterpart)."""
+    c = context()
+    out = torch.empty(stokes_vector_size(grid), dtype=torch.float64, device="cuda")
+    c.check(lib().xc_stokes_krylov_vector(c.ptr, int(j), _P(out.data_ptr())), "stokes_krylov_vector")
+    return out

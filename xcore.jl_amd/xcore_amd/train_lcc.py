@@ -1,8 +1,9 @@
 """Training of the learned coarse correction (no reference counterpart).
 
 For a Stokes system LL (random lognormal viscosity, the reference's operator
-with inertia) and a residual v, the target is the exact correction
-e = LL^-1 v solved by the plain FGMRES to 1e-8; inside FGMRES the network's
+with inertia, buoyancy forcing) the inputs are the directions v a plain FGMRES
+solve feeds its preconditioner (its Krylov basis) and the targets their exact
+corrections e = LL^-1 v (plain FGMRES); inside FGMRES the network's
 coarse estimate of e is applied first and the plain block preconditioner
 cleans up what it leaves.  Inputs and targets are block averages
 on a grid 2^level times coarser (xc_stokes_lc_restrict), normalised per
@@ -23,7 +24,7 @@ import torch
 from . import learned as L
 from .fields import compute_fft_filter, normal_fill_
 from .state import FluidState, Grid, Parameters, compute_scales
-from .stokes import lc_restrict, stokes_solve, stokes_vector_size
+from .stokes import krylov_vector, lc_restrict, stokes_solve, stokes_vector_size
 
 WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "lcc_weights.npz")
 
@@ -81,8 +82,22 @@ def random_residual(g, seed):
     return v / v.norm()
 
 
-def generate(samples, N=256, level=2, seed=0, tol=1e-8, log_every=500, maxit=1000):
-    """Raw training pairs: (log-eta, residual averages, error averages) on the coarse grid."""
+def buoyancy_rhs(g, fl, par, seed):
+    """A buoyancy-driven right-hand side: W rows carry a smooth density anomaly times g0, the rest 0."""
+    N = g.Nz
+    n = stokes_vector_size(g)
+    rhs = torch.zeros(n, dtype=torch.float64, device="cuda")
+    drho = 0.2 * (par.rhox0 - par.rhom0) * _smooth_field(N, seed, 13, N / 12)
+    w = torch.zeros((N + 2, N + 1), dtype=torch.float64, device="cuda").t()   # W-type (N+1, N+2)
+    w[1:N, 1:N + 1] = 0.5 * (drho[:-1] + drho[1:]) * par.g0
+    rhs[:(N + 1) * (N + 2)] = w.t().reshape(-1)
+    return rhs
+
+
+def generate(samples, N=256, level=2, seed=0, per_system=12, tol=1e-8, log_every=500, maxit=1000):
+    """Training pairs from the library's own solves: for random lognormal-viscosity systems driven by
+    buoyancy, the directions FGMRES feeds its preconditioner (the Krylov basis of a plain solve) and
+    their exact corrections e = LL^-1 v (plain FGMRES); returned as coarse-grid block averages."""
     L.set_io_scales([1, 1, 1, 1], [1, 1, 1])
     f = 1 << level
     Hc, Wc = N // f, N // f
@@ -92,24 +107,33 @@ def generate(samples, N=256, level=2, seed=0, tol=1e-8, log_every=500, maxit=100
     its = []
     keep = torch.ones(samples, dtype=torch.bool)
     t0 = time.time()
-    for s in range(samples):
-        par, g, fl, kw = random_problem(N, seed * 100003 + s)
-        rp = random_residual(g, seed * 100003 + s)
-        e, st = stokes_solve(fl, g, par, rp, tol=tol, maxit=maxit, raise_on_noconv=False, **kw)
+    s = 0
+    system = 0
+    while s < samples:
+        par, g, fl, kw = random_problem(N, seed * 100003 + system)
+        rhs = buoyancy_rhs(g, fl, par, seed * 100003 + system)
+        system += 1
+        _, st = stokes_solve(fl, g, par, rhs, tol=tol, maxit=maxit, restart=40, raise_on_noconv=False, **kw)
         its.append(st["iters"])
-        if not st["rel_resid"] <= 1e-5:  # keep only solved systems (the cap is rarely reached)
-            e = None
-        if e is None:
-            keep[s] = False
-            continue
-        a, sg = lc_restrict(fl, g, par, level, rp)
-        b, _ = lc_restrict(fl, g, par, level, e, sigma=sg)
-        le[s] = a[0, ..., 0]
-        rr[s] = a[0, ..., 1:] * sg
-        ee[s] = b[0, ..., 1:] * sg
-        if log_every and (s + 1) % log_every == 0:
-            print(f"  generated {s + 1}/{samples} solves ({time.time() - t0:.1f} s, mean FGMRES "
-                  f"iterations {np.mean(its):.1f})", flush=True)
+        nv = min(per_system, st["iters"], 40)
+        basis = [krylov_vector(g, j) for j in range(nv)]
+        for v in basis:
+            if s >= samples:
+                break
+            e, st2 = stokes_solve(fl, g, par, v, tol=1e-6, maxit=maxit, raise_on_noconv=False, **kw)
+            if not st2["rel_resid"] <= 1e-5:
+                keep[s] = False
+                s += 1
+                continue
+            a, sg = lc_restrict(fl, g, par, level, v)
+            b, _ = lc_restrict(fl, g, par, level, e, sigma=sg)
+            le[s] = a[0, ..., 0]
+            rr[s] = a[0, ..., 1:] * sg
+            ee[s] = b[0, ..., 1:] * sg
+            s += 1
+            if log_every and s % log_every == 0:
+                print(f"  generated {s}/{samples} pairs from {system} systems ({time.time() - t0:.1f} s, mean "
+                      f"FGMRES iterations of the buoyancy solves {np.mean(its):.1f})", flush=True)
     k = keep.to(le.device)
     return le[k], rr[k], ee[k], its
 
