@@ -139,6 +139,23 @@ typedef struct {  /* parameters.jl:87-164: the knobs the hot path reads */
 
 typedef struct { double L0, L0h, l0, l0h, D0, G0, txi0; } xc_scales;  /* scales.jl:290-334 subset */
 
+/* ---- ensembles: many grids of any sizes advance in one launch (no counterpart in the reference;
+ * each member is exactly the single-grid advect_centered[_with_flux]! / diffus_centered[_with_flux]!
+ * of phsevo.jl:21-149 on its own arrays).  qx.p == NULL on every member: no fluxes. */
+typedef struct { xc_mat adv, qx, qz, f, u, w; double h; } xc_adv_member;
+typedef struct { xc_mat dff, qx, qz, f, k; double h; } xc_dif_member;
+int xc_advect_centered_batch(xc_ctx* ctx, int scheme, int n, const xc_adv_member* members, int xbc, int zbc);
+int xc_diffus_centered_batch(xc_ctx* ctx, int n, const xc_dif_member* members, int xbc, int zbc);
+/* Reusable form: the plan keeps the members' argument table on the device, so xc_batch_run only
+ * launches one kernel -- the form to record in a hipGraph and replay every step (the members'
+ * arrays must stay where they are).  The arithmetic mode (XC_CTX_EXACT) is read at run time. */
+typedef struct xc_batch_plan xc_batch_plan;
+int xc_advect_batch_plan(xc_ctx* ctx, int scheme, int n, const xc_adv_member* members, int xbc, int zbc,
+                         xc_batch_plan** plan);
+int xc_diffus_batch_plan(xc_ctx* ctx, int n, const xc_dif_member* members, int xbc, int zbc, xc_batch_plan** plan);
+int xc_batch_run(xc_ctx* ctx, const xc_batch_plan* plan);
+int xc_batch_destroy(xc_ctx* ctx, xc_batch_plan* plan);  /* synchronizes the context's stream */
+
 /* Tile shape the advection kernels use at this (scheme, fluxes, grid): height tz (z) x width tx, and the
  * measured time per launch of every candidate in us[4] (if non-NULL).  No counterpart in the
  * reference (the autotuner of the tile shapes).  XC_ENOCONV: not tuned yet (the default shape). */
@@ -235,6 +252,10 @@ int xc_stokes_lc_restrict(xc_ctx* ctx, const xc_fluid* fluid, const xc_grid* gri
 /* basis vector j of the last FGMRES cycle (unit norm) into out (the training data of the learned
  * correction are the directions FGMRES actually feeds its preconditioner) */
 int xc_stokes_krylov_vector(xc_ctx* ctx, int j, double* out);
+/* z += P e: the learned correction's prolongation of coarse cell values e3 ([p][3], times *sigma and
+ * the output scales) to the stacked vector z (bilinear to fine cells, averaged onto interior faces) */
+int xc_stokes_lc_prolong(xc_ctx* ctx, const xc_fluid* fluid, const xc_grid* grid, int level, const float* e3,
+                         const float* sigma, double* z);
 
 /* ---- time loop (src/run.jl, src/diagnose.jl, src/history.jl) ------------ */
 typedef struct {  /* run! keywords (run.jl:167-181) and Picard controls (parameters.jl:150-154) */

@@ -21,7 +21,7 @@ struct xc_ctx {
     void (*ext_free[8])(xc_ctx*, void*);  // releases ext[i] in xc_destroy
 };
 
-enum { XC_EXT_NOISE = 0, XC_EXT_STOKES = 1, XC_EXT_UNET = 2, XC_EXT_STEP = 3, XC_EXT_TUNE = 4 };
+enum { XC_EXT_NOISE = 0, XC_EXT_STOKES = 1, XC_EXT_UNET = 2, XC_EXT_STEP = 3, XC_EXT_TUNE = 4, XC_EXT_BATCH = 5 };
 
 // Scratch slots: one growing device buffer per slot, owned by the context.
 enum {

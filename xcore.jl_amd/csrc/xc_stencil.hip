@@ -178,8 +178,9 @@ struct AdvArgs {
 
 // HIN: f is a halo'd buffer (advect!); otherwise f is (Nz, Nx) and the BCs are
 // an index map (advect_centered!).  FLUX: also write qx/qz with padding.
+// one tile t of one grid; the body of k_advect and of the ensemble kernel k_advect_batch
 template <int S, bool HIN, bool FLUX, bool FAST, int BZ, int BX>
-__global__ __launch_bounds__(NT) void k_advect(AdvArgs a) {
+__device__ __forceinline__ void advect_tile(const AdvArgs& a, int t) {
     constexpr int G = halo_of(S);
     constexpr int PZ = BZ + 2 * G;
     constexpr int PX = BX + 2 * G;
@@ -187,7 +188,6 @@ __global__ __launch_bounds__(NT) void k_advect(AdvArgs a) {
     __shared__ double qxs[(BX + 1) * BZ];
     __shared__ double qzs[BX * (BZ + 1)];
 
-    const int t = xcd_tile(blockIdx.x, a.nb);
     const int z0 = (t % a.nbz) * BZ;
     const int x0 = (t / a.nbz) * BX;
     const int tid = threadIdx.x;
@@ -297,6 +297,27 @@ __global__ __launch_bounds__(NT) void k_advect(AdvArgs a) {
     }
 }
 
+template <int S, bool HIN, bool FLUX, bool FAST, int BZ, int BX>
+__global__ __launch_bounds__(NT) void k_advect(AdvArgs a) {
+    advect_tile<S, HIN, FLUX, FAST, BZ, BX>(a, xcd_tile(blockIdx.x, a.nb));
+}
+
+// Ensemble form: the tiles of n grids of any sizes in one launch.  Workgroup b (XCD-remapped)
+// belongs to the grid whose tile range [first[i], first[i+1]) holds it.
+template <int S, bool FLUX, bool FAST, int BZ, int BX>
+__global__ __launch_bounds__(NT) void k_advect_batch(const AdvArgs* __restrict__ items,
+                                                     const int* __restrict__ first, int n, int total) {
+    const int b = xcd_tile(blockIdx.x, total);
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {  // last i with first[i] <= b
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const AdvArgs a = items[lo];
+    advect_tile<S, false, FLUX, FAST, BZ, BX>(a, b - first[lo]);
+}
+
 template <int S, bool HIN, bool FLUX, bool FAST, int V>
 hipError_t launch_advect_v(const AdvArgs& a0, hipStream_t s) {
     constexpr int BZ = VAR_TZ[V], BX = VAR_TX[V];
@@ -388,6 +409,29 @@ int tune_advect(xc_ctx* ctx, int scheme, bool flux, bool ex, const AdvArgs& a, i
     tune_ctx(ctx)->e.push_back(t);
     *out = t.v;
     return XC_OK;
+}
+
+// ensemble launches use the 64 x 8 tile (variant 2)
+template <int S, bool FLUX, bool FAST>
+void launch_advect_batch_t(const AdvArgs* items, const int* first, int n, int total, hipStream_t s) {
+    hipLaunchKernelGGL((k_advect_batch<S, FLUX, FAST, VAR_TZ[2], VAR_TX[2]>), dim3(total), dim3(NT), 0, s, items,
+                       first, n, total);
+}
+template <bool FLUX>
+void launch_advect_batch_s(int scheme, bool exact, const AdvArgs* items, const int* first, int n, int total,
+                           hipStream_t s) {
+    switch (scheme) {
+        case XC_CENTR: launch_advect_batch_t<XC_CENTR, FLUX, false>(items, first, n, total, s); break;
+        case XC_UPWD1: launch_advect_batch_t<XC_UPWD1, FLUX, false>(items, first, n, total, s); break;
+        case XC_QUICK: launch_advect_batch_t<XC_QUICK, FLUX, false>(items, first, n, total, s); break;
+        case XC_FROMM: launch_advect_batch_t<XC_FROMM, FLUX, false>(items, first, n, total, s); break;
+        case XC_WENO3: launch_advect_batch_t<XC_WENO3, FLUX, false>(items, first, n, total, s); break;
+        case XC_WENO5:
+            if (exact) launch_advect_batch_t<XC_WENO5, FLUX, false>(items, first, n, total, s);
+            else launch_advect_batch_t<XC_WENO5, FLUX, true>(items, first, n, total, s);
+            break;
+        default: launch_advect_batch_t<XC_TVDIM, FLUX, false>(items, first, n, total, s); break;
+    }
 }
 
 int advect_common(xc_ctx* ctx, int scheme, bool hin, xc_mat adv, const xc_mat* qx, const xc_mat* qz,
@@ -527,11 +571,10 @@ __global__ __launch_bounds__(NT) void k_diffus(DifArgs a) {
 // one-cell halo is staged once, so each cell costs two global loads instead of ten.  Same arithmetic
 // and operation order as k_diffus.
 template <bool FLUX>
-__global__ __launch_bounds__(NT) void k_diffus_lds(DifArgs a) {
+__device__ __forceinline__ void diffus_tile(const DifArgs& a, int t) {
     constexpr int PZ = TZ + 2, PX = TX + 2;
     __shared__ double fs[PX * PZ];
     __shared__ double ks[PX * PZ];
-    const int t = xcd_tile(blockIdx.x, a.nb);
     const int z0 = (t % a.nbz) * TZ;
     const int x0 = (t / a.nbz) * TX;
     const int Nz = a.Nz, Nx = a.Nx;
@@ -581,6 +624,68 @@ __global__ __launch_bounds__(NT) void k_diffus_lds(DifArgs a) {
             if (iz == Nz - 1) put_qz(Nz, -(kc + kpz) * 0.5 * (fpz - fc) * a.invh);
         }
     }
+}
+template <bool FLUX>
+__global__ __launch_bounds__(NT) void k_diffus_lds(DifArgs a) {
+    diffus_tile<FLUX>(a, xcd_tile(blockIdx.x, a.nb));
+}
+template <bool FLUX>
+__global__ __launch_bounds__(NT) void k_diffus_batch(const DifArgs* __restrict__ items, const int* __restrict__ first,
+                                                     int n, int total) {
+    const int b = xcd_tile(blockIdx.x, total);
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const DifArgs a = items[lo];
+    diffus_tile<FLUX>(a, b - first[lo]);
+}
+
+// ---- ensemble tables: per-grid argument blocks staged in pinned memory and copied to the device
+struct BatchCtx {
+    char* host = nullptr;   // pinned
+    char* dev = nullptr;
+    size_t cap = 0;
+    hipEvent_t done = nullptr;  // the last copy out of `host` has completed
+};
+void batch_free(xc_ctx*, void* p) {
+    BatchCtx* b = (BatchCtx*)p;
+    if (b->done) (void)hipEventSynchronize(b->done), (void)hipEventDestroy(b->done);
+    if (b->host) (void)hipHostFree(b->host);
+    if (b->dev) (void)hipFree(b->dev);
+    delete b;
+}
+// room for `bytes` of table; returns host staging and device pointers (the previous copy has finished)
+int batch_table(xc_ctx* ctx, size_t bytes, char** host, char** dev) {
+    BatchCtx* b = (BatchCtx*)ctx->ext[XC_EXT_BATCH];
+    if (!b) {
+        b = new BatchCtx();
+        ctx->ext[XC_EXT_BATCH] = b;
+        ctx->ext_free[XC_EXT_BATCH] = batch_free;
+        XC_TRY_HIP(ctx, hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
+    }
+    XC_TRY_HIP(ctx, hipEventSynchronize(b->done));
+    if (bytes > b->cap) {
+        if (b->host) (void)hipHostFree(b->host);
+        if (b->dev) (void)hipFree(b->dev);
+        b->host = nullptr;
+        b->dev = nullptr;
+        b->cap = 0;
+        XC_TRY_HIP(ctx, hipHostMalloc((void**)&b->host, bytes, hipHostMallocDefault));
+        XC_TRY_HIP(ctx, hipMalloc((void**)&b->dev, bytes));
+        b->cap = bytes;
+    }
+    *host = b->host;
+    *dev = b->dev;
+    return XC_OK;
+}
+int batch_upload(xc_ctx* ctx, size_t bytes) {
+    BatchCtx* b = (BatchCtx*)ctx->ext[XC_EXT_BATCH];
+    XC_TRY_HIP(ctx, hipMemcpyAsync(b->dev, b->host, bytes, hipMemcpyHostToDevice, ctx->stream));
+    XC_TRY_HIP(ctx, hipEventRecord(b->done, ctx->stream));
+    return XC_OK;
 }
 
 int diffus_common(xc_ctx* ctx, bool hin, int halo, xc_mat dff, const xc_mat* qx, const xc_mat* qz,
@@ -757,6 +862,235 @@ int xc_advect_set_tile(xc_ctx* ctx, int scheme, int with_flux, int64_t Nz, int64
     TuneEntry* te = tune_find(ctx, scheme, with_flux ? 1 : 0, ex, (int)Nz, (int)Nx);
     if (te) te->v = v;
     else tune_ctx(ctx)->e.push_back(TuneEntry{scheme, with_flux ? 1 : 0, ex, (int)Nz, (int)Nx, v, {}});
+    return XC_OK;
+}
+
+int xc_advect_centered_batch(xc_ctx* ctx, int scheme, int n, const xc_adv_member* m, int xbc, int zbc) {
+    XC_CHECK_CTX(ctx);
+    const int g = xc_scheme_halo(scheme);
+    if (g < 0) return xc_fail(ctx, XC_EINVAL_SCHEME, "advect_centered_batch: unsupported scheme = %d", scheme);
+    if (xbc != XC_PERIODIC && xbc != XC_CLOSED) return xc_fail(ctx, XC_EINVAL_BC, "advect_centered_batch: xBC = %d", xbc);
+    if (zbc != XC_PERIODIC && zbc != XC_CLOSED) return xc_fail(ctx, XC_EINVAL_BC, "advect_centered_batch: zBC = %d", zbc);
+    if (n < 1 || !m) return xc_fail(ctx, XC_EINVAL_ARG, "advect_centered_batch: no members");
+    const bool flux = m[0].qx.p != nullptr;
+    const size_t bytes = (size_t)n * sizeof(AdvArgs) + (size_t)(n + 1) * sizeof(int);
+    char *host, *dev;
+    int st;
+    if ((st = batch_table(ctx, bytes, &host, &dev))) return st;
+    AdvArgs* A = (AdvArgs*)host;
+    int* first = (int*)(host + (size_t)n * sizeof(AdvArgs));
+    int64_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        const xc_adv_member& q = m[i];
+        const int64_t Nz = q.adv.rows, Nx = q.adv.cols;
+        if (!xc_mat_ok(q.adv) || Nz < g || Nx < g || !xc_shape(q.f, Nz, Nx) || !xc_shape(q.u, Nz, Nx + 1) ||
+            !xc_shape(q.w, Nz + 1, Nx))
+            return xc_fail(ctx, XC_EINVAL_SHAPE, "advect_centered_batch: member %d has inconsistent shapes", i);
+        if ((q.qx.p != nullptr) != flux || (q.qz.p != nullptr) != flux)
+            return xc_fail(ctx, XC_EINVAL_ARG, "advect_centered_batch: fluxes for every member or for none");
+        if (flux && (!xc_shape(q.qx, Nz + 2, Nx + 1) || !xc_shape(q.qz, Nz + 1, Nx + 2)))
+            return xc_fail(ctx, XC_EINVAL_SHAPE, "advect_centered_batch: member %d flux shapes", i);
+        AdvArgs a{};
+        a.adv = q.adv.p; a.ld_adv = q.adv.ld;
+        if (flux) { a.qx = q.qx.p; a.ld_qx = q.qx.ld; a.qz = q.qz.p; a.ld_qz = q.qz.ld; }
+        a.f = q.f.p; a.ld_f = q.f.ld;
+        a.u = q.u.p; a.ld_u = q.u.ld;
+        a.w = q.w.p; a.ld_w = q.w.ld;
+        a.Nz = (int)Nz; a.Nx = (int)Nx;
+        a.nbz = (a.Nz + VAR_TZ[2] - 1) / VAR_TZ[2];
+        a.nb = a.nbz * ((a.Nx + VAR_TX[2] - 1) / VAR_TX[2]);
+        a.invh = 1.0 / q.h;
+        a.xbc = xbc; a.zbc = zbc;
+        A[i] = a;
+        first[i] = (int)total;
+        total += a.nb;
+        if (total > (1 << 30)) return xc_fail(ctx, XC_EINVAL_SHAPE, "advect_centered_batch: too many tiles");
+    }
+    first[n] = (int)total;
+    if ((st = batch_upload(ctx, bytes))) return st;
+    const AdvArgs* dA = (const AdvArgs*)dev;
+    const int* dfirst = (const int*)(dev + (size_t)n * sizeof(AdvArgs));
+    const bool ex = (ctx->flags & XC_CTX_EXACT) != 0;
+    if (flux) launch_advect_batch_s<true>(scheme, ex, dA, dfirst, n, (int)total, ctx->stream);
+    else launch_advect_batch_s<false>(scheme, ex, dA, dfirst, n, (int)total, ctx->stream);
+    return xc_finish(ctx, "advect_centered_batch");
+}
+
+int xc_diffus_centered_batch(xc_ctx* ctx, int n, const xc_dif_member* m, int xbc, int zbc) {
+    XC_CHECK_CTX(ctx);
+    if (xbc != XC_PERIODIC && xbc != XC_CLOSED) return xc_fail(ctx, XC_EINVAL_BC, "diffus_centered_batch: xBC = %d", xbc);
+    if (zbc != XC_PERIODIC && zbc != XC_CLOSED) return xc_fail(ctx, XC_EINVAL_BC, "diffus_centered_batch: zBC = %d", zbc);
+    if (n < 1 || !m) return xc_fail(ctx, XC_EINVAL_ARG, "diffus_centered_batch: no members");
+    const bool flux = m[0].qx.p != nullptr;
+    const size_t bytes = (size_t)n * sizeof(DifArgs) + (size_t)(n + 1) * sizeof(int);
+    char *host, *dev;
+    int st;
+    if ((st = batch_table(ctx, bytes, &host, &dev))) return st;
+    DifArgs* A = (DifArgs*)host;
+    int* first = (int*)(host + (size_t)n * sizeof(DifArgs));
+    int64_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        const xc_dif_member& q = m[i];
+        const int64_t Nz = q.dff.rows, Nx = q.dff.cols;
+        if (!xc_mat_ok(q.dff) || Nz < 1 || Nx < 1 || !xc_shape(q.f, Nz, Nx) || !xc_shape(q.k, Nz, Nx))
+            return xc_fail(ctx, XC_EINVAL_SHAPE, "diffus_centered_batch: member %d has inconsistent shapes", i);
+        if ((q.qx.p != nullptr) != flux || (q.qz.p != nullptr) != flux)
+            return xc_fail(ctx, XC_EINVAL_ARG, "diffus_centered_batch: fluxes for every member or for none");
+        if (flux && (!xc_shape(q.qx, Nz + 2, Nx + 1) || !xc_shape(q.qz, Nz + 1, Nx + 2)))
+            return xc_fail(ctx, XC_EINVAL_SHAPE, "diffus_centered_batch: member %d flux shapes", i);
+        DifArgs a{};
+        a.dff = q.dff.p; a.ld_d = q.dff.ld;
+        if (flux) { a.qx = q.qx.p; a.ld_qx = q.qx.ld; a.qz = q.qz.p; a.ld_qz = q.qz.ld; }
+        a.f = q.f.p; a.ld_f = q.f.ld; a.k = q.k.p; a.ld_k = q.k.ld;
+        a.Nz = (int)Nz; a.Nx = (int)Nx; a.halo = 1;
+        a.nbz = (int)((Nz + TZ - 1) / TZ);
+        a.nb = a.nbz * (int)((Nx + TX - 1) / TX);
+        a.invh = 1.0 / q.h;
+        a.invh2 = a.invh * a.invh;
+        a.xbc = xbc; a.zbc = zbc;
+        A[i] = a;
+        first[i] = (int)total;
+        total += a.nb;
+    }
+    first[n] = (int)total;
+    if ((st = batch_upload(ctx, bytes))) return st;
+    const DifArgs* dA = (const DifArgs*)dev;
+    const int* dfirst = (const int*)(dev + (size_t)n * sizeof(DifArgs));
+    if (flux) hipLaunchKernelGGL((k_diffus_batch<true>), dim3((unsigned)total), dim3(NT), 0, ctx->stream, dA, dfirst, n, (int)total);
+    else hipLaunchKernelGGL((k_diffus_batch<false>), dim3((unsigned)total), dim3(NT), 0, ctx->stream, dA, dfirst, n, (int)total);
+    return xc_finish(ctx, "diffus_centered_batch");
+}
+
+struct xc_batch_plan {
+    int kind;      // 0: advection, 1: diffusion
+    int scheme;
+    int flux;
+    int n, total;
+    char* dev;     // argument blocks + first-tile table
+};
+
+static int plan_make(xc_ctx* ctx, int kind, int scheme, int n, bool flux, int64_t total, const char* host,
+                     size_t bytes, xc_batch_plan** out) {
+    xc_batch_plan* p = new xc_batch_plan{kind, scheme, flux ? 1 : 0, n, (int)total, nullptr};
+    hipError_t e = hipMalloc((void**)&p->dev, bytes);
+    if (e == hipSuccess) e = hipMemcpy(p->dev, host, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (p->dev) (void)hipFree(p->dev);
+        delete p;
+        return xc_fail(ctx, XC_EHIP, "batch plan: %s", hipGetErrorString(e));
+    }
+    *out = p;
+    return XC_OK;
+}
+
+int xc_advect_batch_plan(xc_ctx* ctx, int scheme, int n, const xc_adv_member* m, int xbc, int zbc,
+                         xc_batch_plan** out) {
+    XC_CHECK_CTX(ctx);
+    if (!out) return xc_fail(ctx, XC_EINVAL_ARG, "advect_batch_plan: NULL output");
+    // validate and lay out exactly as the immediate form, then keep the table in its own buffer
+    const int g = xc_scheme_halo(scheme);
+    if (g < 0) return xc_fail(ctx, XC_EINVAL_SCHEME, "advect_batch_plan: unsupported scheme = %d", scheme);
+    if (xbc != XC_PERIODIC && xbc != XC_CLOSED) return xc_fail(ctx, XC_EINVAL_BC, "advect_batch_plan: xBC = %d", xbc);
+    if (zbc != XC_PERIODIC && zbc != XC_CLOSED) return xc_fail(ctx, XC_EINVAL_BC, "advect_batch_plan: zBC = %d", zbc);
+    if (n < 1 || !m) return xc_fail(ctx, XC_EINVAL_ARG, "advect_batch_plan: no members");
+    const bool flux = m[0].qx.p != nullptr;
+    const size_t bytes = (size_t)n * sizeof(AdvArgs) + (size_t)(n + 1) * sizeof(int);
+    std::vector<char> host(bytes);
+    AdvArgs* A = (AdvArgs*)host.data();
+    int* first = (int*)(host.data() + (size_t)n * sizeof(AdvArgs));
+    int64_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        const xc_adv_member& q = m[i];
+        const int64_t Nz = q.adv.rows, Nx = q.adv.cols;
+        if (!xc_mat_ok(q.adv) || Nz < g || Nx < g || !xc_shape(q.f, Nz, Nx) || !xc_shape(q.u, Nz, Nx + 1) ||
+            !xc_shape(q.w, Nz + 1, Nx))
+            return xc_fail(ctx, XC_EINVAL_SHAPE, "advect_batch_plan: member %d has inconsistent shapes", i);
+        if ((q.qx.p != nullptr) != flux || (q.qz.p != nullptr) != flux)
+            return xc_fail(ctx, XC_EINVAL_ARG, "advect_batch_plan: fluxes for every member or for none");
+        if (flux && (!xc_shape(q.qx, Nz + 2, Nx + 1) || !xc_shape(q.qz, Nz + 1, Nx + 2)))
+            return xc_fail(ctx, XC_EINVAL_SHAPE, "advect_batch_plan: member %d flux shapes", i);
+        AdvArgs a{};
+        a.adv = q.adv.p; a.ld_adv = q.adv.ld;
+        if (flux) { a.qx = q.qx.p; a.ld_qx = q.qx.ld; a.qz = q.qz.p; a.ld_qz = q.qz.ld; }
+        a.f = q.f.p; a.ld_f = q.f.ld;
+        a.u = q.u.p; a.ld_u = q.u.ld;
+        a.w = q.w.p; a.ld_w = q.w.ld;
+        a.Nz = (int)Nz; a.Nx = (int)Nx;
+        a.nbz = (a.Nz + VAR_TZ[2] - 1) / VAR_TZ[2];
+        a.nb = a.nbz * ((a.Nx + VAR_TX[2] - 1) / VAR_TX[2]);
+        a.invh = 1.0 / q.h;
+        a.xbc = xbc; a.zbc = zbc;
+        A[i] = a;
+        first[i] = (int)total;
+        total += a.nb;
+    }
+    first[n] = (int)total;
+    return plan_make(ctx, 0, scheme, n, flux, total, host.data(), bytes, out);
+}
+
+int xc_diffus_batch_plan(xc_ctx* ctx, int n, const xc_dif_member* m, int xbc, int zbc, xc_batch_plan** out) {
+    XC_CHECK_CTX(ctx);
+    if (!out) return xc_fail(ctx, XC_EINVAL_ARG, "diffus_batch_plan: NULL output");
+    if (xbc != XC_PERIODIC && xbc != XC_CLOSED) return xc_fail(ctx, XC_EINVAL_BC, "diffus_batch_plan: xBC = %d", xbc);
+    if (zbc != XC_PERIODIC && zbc != XC_CLOSED) return xc_fail(ctx, XC_EINVAL_BC, "diffus_batch_plan: zBC = %d", zbc);
+    if (n < 1 || !m) return xc_fail(ctx, XC_EINVAL_ARG, "diffus_batch_plan: no members");
+    const bool flux = m[0].qx.p != nullptr;
+    const size_t bytes = (size_t)n * sizeof(DifArgs) + (size_t)(n + 1) * sizeof(int);
+    std::vector<char> host(bytes);
+    DifArgs* A = (DifArgs*)host.data();
+    int* first = (int*)(host.data() + (size_t)n * sizeof(DifArgs));
+    int64_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        const xc_dif_member& q = m[i];
+        const int64_t Nz = q.dff.rows, Nx = q.dff.cols;
+        if (!xc_mat_ok(q.dff) || Nz < 1 || Nx < 1 || !xc_shape(q.f, Nz, Nx) || !xc_shape(q.k, Nz, Nx))
+            return xc_fail(ctx, XC_EINVAL_SHAPE, "diffus_batch_plan: member %d has inconsistent shapes", i);
+        if ((q.qx.p != nullptr) != flux || (q.qz.p != nullptr) != flux)
+            return xc_fail(ctx, XC_EINVAL_ARG, "diffus_batch_plan: fluxes for every member or for none");
+        if (flux && (!xc_shape(q.qx, Nz + 2, Nx + 1) || !xc_shape(q.qz, Nz + 1, Nx + 2)))
+            return xc_fail(ctx, XC_EINVAL_SHAPE, "diffus_batch_plan: member %d flux shapes", i);
+        DifArgs a{};
+        a.dff = q.dff.p; a.ld_d = q.dff.ld;
+        if (flux) { a.qx = q.qx.p; a.ld_qx = q.qx.ld; a.qz = q.qz.p; a.ld_qz = q.qz.ld; }
+        a.f = q.f.p; a.ld_f = q.f.ld; a.k = q.k.p; a.ld_k = q.k.ld;
+        a.Nz = (int)Nz; a.Nx = (int)Nx; a.halo = 1;
+        a.nbz = (int)((Nz + TZ - 1) / TZ);
+        a.nb = a.nbz * (int)((Nx + TX - 1) / TX);
+        a.invh = 1.0 / q.h;
+        a.invh2 = a.invh * a.invh;
+        a.xbc = xbc; a.zbc = zbc;
+        A[i] = a;
+        first[i] = (int)total;
+        total += a.nb;
+    }
+    first[n] = (int)total;
+    return plan_make(ctx, 1, 0, n, flux, total, host.data(), bytes, out);
+}
+
+int xc_batch_run(xc_ctx* ctx, const xc_batch_plan* p) {
+    XC_CHECK_CTX(ctx);
+    if (!p) return xc_fail(ctx, XC_EINVAL_ARG, "batch_run: NULL plan");
+    const bool ex = (ctx->flags & XC_CTX_EXACT) != 0;
+    if (p->kind == 0) {
+        const AdvArgs* dA = (const AdvArgs*)p->dev;
+        const int* dfirst = (const int*)(p->dev + (size_t)p->n * sizeof(AdvArgs));
+        if (p->flux) launch_advect_batch_s<true>(p->scheme, ex, dA, dfirst, p->n, p->total, ctx->stream);
+        else launch_advect_batch_s<false>(p->scheme, ex, dA, dfirst, p->n, p->total, ctx->stream);
+    } else {
+        const DifArgs* dA = (const DifArgs*)p->dev;
+        const int* dfirst = (const int*)(p->dev + (size_t)p->n * sizeof(DifArgs));
+        if (p->flux) hipLaunchKernelGGL((k_diffus_batch<true>), dim3((unsigned)p->total), dim3(NT), 0, ctx->stream, dA, dfirst, p->n, p->total);
+        else hipLaunchKernelGGL((k_diffus_batch<false>), dim3((unsigned)p->total), dim3(NT), 0, ctx->stream, dA, dfirst, p->n, p->total);
+    }
+    return xc_finish(ctx, "batch_run");
+}
+
+int xc_batch_destroy(xc_ctx* ctx, xc_batch_plan* p) {
+    XC_CHECK_CTX(ctx);
+    if (!p) return XC_OK;
+    XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));  // no launch may still read the table
+    (void)hipFree(p->dev);
+    delete p;
     return XC_OK;
 }
 

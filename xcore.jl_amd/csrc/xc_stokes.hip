@@ -1755,4 +1755,33 @@ int xc_stokes_krylov_vector(xc_ctx* ctx, int j, double* out) {
     XC_TRY_HIP(ctx, hipMemcpyAsync(out, s->V + (int64_t)j * n, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     return xc_finish(ctx, "stokes_krylov_vector");
 }
+
+int xc_stokes_lc_prolong(xc_ctx* ctx, const xc_fluid* fl, const xc_grid* g, int level, const float* e3,
+                         const float* sigma, double* z) {
+    XC_CHECK_CTX(ctx);
+    if (!e3 || !sigma || !z) return xc_fail(ctx, XC_EINVAL_ARG, "stokes_lc_prolong: NULL buffer");
+    xc_fm_opts o{};
+    o.sds = -1;
+    o.top_cnv = 1;
+    o.bot_cnv = 1;
+    int st;
+    if ((st = check_fm(ctx, fl, g, &o))) return st;
+    StokesCtx* s;
+    if ((st = setup(ctx, (int)g->Nz, (int)g->Nx, 1, &s))) return st;
+    build_levels(ctx, s, fl, g->h);
+    const int f = 1 << level;
+    if (level < 1 || g->Nz % f || g->Nx % f) return xc_fail(ctx, XC_EINVAL_SHAPE, "stokes_lc_prolong: level %d", level);
+    LcProArgs pa{};
+    pa.L = s->lev[0].d();
+    pa.f = f;
+    pa.Hc = (int)g->Nz / f;
+    pa.Wc = (int)g->Nx / f;
+    pa.e = e3;
+    pa.sigma = sigma;
+    xc_unet_io_scales(ctx, nullptr, pa.os);
+    pa.z = z;
+    pa.n = s->lev[0].nv + (g->Nz + 2) * (g->Nx + 2);
+    hipLaunchKernelGGL(k_lc_prolong, dim3(nblk(pa.n)), dim3(NT), 0, ctx->stream, pa);
+    return xc_finish(ctx, "stokes_lc_prolong");
+}
 }  // extern "C"

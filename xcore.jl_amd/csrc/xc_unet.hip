@@ -774,6 +774,8 @@ bool xc_unet_has_params(xc_ctx* ctx) {
 
 void xc_unet_io_scales(xc_ctx* ctx, float in[4], float out[3]) {
     const Net* n = (const Net*)ctx->ext[XC_EXT_UNET];
-    for (int k = 0; k < 4; ++k) in[k] = n ? n->in_scale[k] : 1.f;
-    for (int k = 0; k < 3; ++k) out[k] = n ? n->out_scale[k] : 1.f;
+    if (in)
+        for (int k = 0; k < 4; ++k) in[k] = n ? n->in_scale[k] : 1.f;
+    if (out)
+        for (int k = 0; k < 3; ++k) out[k] = n ? n->out_scale[k] : 1.f;
 }

@@ -43,6 +43,7 @@ _lib.register("xc_stokes_precond", [_P, _P, _P, _P, _P, _I, _P, _P])
 _lib.register("xc_stokes_solve", [_P, _P, _P, _P, _P, _I, _P, _P, _P])
 _lib.register("xc_stokes_lc_restrict", [_P, _P, _P, _P, _I, _P, _P, _P, _I])
 _lib.register("xc_stokes_krylov_vector", [_P, _I, _P])
+_lib.register("xc_stokes_lc_prolong", [_P, _P, _P, _I, _P, _P, _P])
 
 
 class BnchmData:
@@ -164,3 +165,12 @@ def krylov_vector(grid, j):
     out = torch.empty(stokes_vector_size(grid), dtype=torch.float64, device="cuda")
     c.check(lib().xc_stokes_krylov_vector(c.ptr, int(j), _P(out.data_ptr())), "stokes_krylov_vector")
     return out
+
+
+def lc_prolong_(z, state, grid, level, e3, sigma):
+    """z += P e3 (the learned correction's prolongation; no reference counterpart)."""
+    c = context()
+    fs = state.pack()
+    c.check(lib().xc_stokes_lc_prolong(c.ptr, ctypes.byref(fs), ctypes.byref(grid_struct(grid)), int(level),
+                                       _P(e3.data_ptr()), _P(sigma.data_ptr()), _P(z.data_ptr())), "stokes_lc_prolong")
+    return z
