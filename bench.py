@@ -17,6 +17,9 @@ gives each stencil's algorithmic GB/s from HIP events on the launch stream;
 its WENO5 kernel is the `roofline` object (the HBM-bound kernel family of the
 path: the 512^2 step's state, ~250 MB, sits in the 256 MB Infinity Cache).
 
+The ensemble section (64 members of 128^2 / 256^2 / 384^2) times one batched launch per
+operator -- replayed from a HIP graph -- against one launch per member and operator.
+
 `cpu_baseline` times the CPU oracle (the C restatement of the reference with
 SciPy's sparse LU for the Stokes solve) on one Picard sweep of the same step
 at 256^2 and converts it to steps/s with the mean sweep count of the GPU run.
@@ -102,7 +105,7 @@ def step_bench(args, learned=False):
                      xc.step_opts(par, tol=1e-8, maxit=2000, fixed_sweeps=1))
     par, g, sc, ph, fl, ns, hst, state = c3_setup(args.n)
     opts = xc.step_opts(par, tol=1e-8, maxit=2000, mg_pre=args.mg_pre, mg_post=args.mg_post, mg_coarse=args.mg_coarse,
-                        learned=learned, lc_level=lc_level)
+                        mg_sub=args.mg_sub, learned=learned, lc_level=lc_level)
     for _ in range(args.warmup):
         xc.run_step_(ph, fl, ns, g, par, sc, state, opts)
     torch.cuda.synchronize()
@@ -183,6 +186,71 @@ def stencil_bench(N, steps, warmup):
     return roofline, out
 
 
+def ensemble_bench(reps=20, seed=4001):
+    """Config-4 style ensemble: 64 members of 128^2, 256^2 and 384^2 (cycled), each with its own
+    fields, advanced by WENO5 and QUICK advection and diffusion (all with fluxes) -- one launch per
+    operator for the whole ensemble (a device-resident batch plan, replayed from a HIP graph) against
+    one launch per member and operator."""
+    import numpy as np
+    import torch
+    import xcore_amd as xc
+    sizes = [128, 256, 384]
+    rng = np.random.default_rng(seed)
+    mem = []
+    for i in range(64):
+        N = sizes[i % 3]
+        f = xc.to_device(np.asfortranarray(rng.standard_normal((N, N)) + 2.0))
+        u = xc.to_device(np.asfortranarray(rng.standard_normal((N, N + 1))))
+        w = xc.to_device(np.asfortranarray(rng.standard_normal((N + 1, N))))
+        k = xc.to_device(np.asfortranarray(np.exp(rng.standard_normal((N, N)))))
+        mem.append(dict(f=f, u=u, w=w, k=k, h=1.0 / N, adv=xc.xcore_zeros(N, N), qx=xc.xcore_zeros(N + 2, N + 1),
+                        qz=xc.xcore_zeros(N + 1, N + 2), dff=xc.xcore_zeros(N, N), dqx=xc.xcore_zeros(N + 2, N + 1),
+                        dqz=xc.xcore_zeros(N + 1, N + 2)))
+    cells = sum(m["f"].shape[0] * m["f"].shape[1] for m in mem)
+    dmem = [dict(dff=m["dff"], qx=m["dqx"], qz=m["dqz"], f=m["f"], k=m["k"], h=m["h"]) for m in mem]
+    plans = [xc.BatchPlan(mem, scheme="weno5"), xc.BatchPlan(mem, scheme="quick"), xc.BatchPlan(dmem)]
+
+    def batched():
+        for p in plans:
+            p.run()
+
+    def single():
+        for m in mem:
+            xc.advect_centered_with_flux_(m["adv"], m["qx"], m["qz"], m["f"], m["u"], m["w"], m["h"], "weno5")
+            xc.advect_centered_with_flux_(m["adv"], m["qx"], m["qz"], m["f"], m["u"], m["w"], m["h"], "quick")
+            xc.diffus_centered_with_flux_(m["dff"], m["dqx"], m["dqz"], m["f"], m["k"], m["h"])
+
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        batched()
+    torch.cuda.current_stream().wait_stream(s)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        batched()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    t_graph = timed(g.replay)
+    t_batch = timed(batched)
+    t_single = timed(single)
+    for p in plans:
+        p.close()
+    return {"members": 64, "sizes": "128^2, 256^2, 384^2 (cycled)", "cells": cells,
+            "operators": "WENO5 + QUICK advection with fluxes, diffusion with fluxes",
+            "ms_per_ensemble_step_graph": round(1e3 * t_graph, 3), "ms_per_ensemble_step_batched": round(1e3 * t_batch, 3),
+            "ms_per_ensemble_step_per_member_launches": round(1e3 * t_single, 3),
+            "cell_steps_per_s_graph": round(cells / t_graph, 1), "speedup_vs_per_member": round(t_single / t_graph, 3),
+            "launches_per_step": {"batched": 3, "per_member": 192}}
+
+
 def cpu_baseline(N, sweeps_per_step):
     """One Picard sweep of the same step on the CPU oracle (C, OpenMP; SciPy sparse LU for Stokes)."""
     import numpy as np
@@ -228,11 +296,13 @@ def main():
     ap.add_argument("--weights", default=os.path.join(ROOT, "xcore.jl_amd", "lcc_weights.npz"),
                     help="trained learned-correction weights; when present the learned step is timed too")
     ap.add_argument("--no-learned", action="store_true")
-    ap.add_argument("--mg-pre", type=int, default=2)
-    ap.add_argument("--mg-post", type=int, default=2)
+    ap.add_argument("--mg-pre", type=int, default=3)
+    ap.add_argument("--mg-post", type=int, default=3)
     ap.add_argument("--mg-coarse", type=int, default=20)
+    ap.add_argument("--mg-sub", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-stencil", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-step", action="store_true", help="skip the full-step bench (profiling runs)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="before the stencil bench, copy a buffer of this many MiB three times (a known-size "
@@ -299,6 +369,9 @@ def main():
         roof, kernels = stencil_bench(args.stencil_n, 20, 3)
         out["roofline"] = roof
         out["stencil_kernels"] = kernels
+    if not args.no_ensemble:
+        print("bench: ensemble of 64 members", file=sys.stderr, flush=True)
+        out["ensemble"] = ensemble_bench()
     if not args.no_cpu:
         print(f"bench: CPU oracle sweep at {args.cpu_n}^2", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, info["plain"]["sweeps_per_step"] if "plain" in info else 1.0)

@@ -202,11 +202,15 @@ typedef struct {  /* fluidmech! keywords (fluidmech.jl:71-80) and the iterative-
     double tol;       /* stop at ||RR - LL x||_2 <= tol ||RR||_2 (1e-8) */
     int maxit;        /* FGMRES iteration cap (all restarts) */
     int restart;      /* FGMRES restart length */
-    int mg_pre, mg_post, mg_coarse;  /* multigrid smoothing sweeps (pre, post, coarsest level) */
+    int mg_pre, mg_post, mg_coarse;  /* multigrid smoothing sweeps on the finest level (pre, post) and on
+                                        the coarsest; 0 selects the defaults 3, 3, 20 */
     double mg_omega;  /* damped-Jacobi weight of the multigrid smoother */
-    int learned;      /* 1: add the learned coarse correction (needs xc_unet_set_params); the plain
-                         preconditioner is the fallback when the learned solve fails */
+    int learned;      /* 1: add the learned coarse correction (needs xc_unet_set_params) for a first
+                         FGMRES cycle of 6 iterations; if that has not converged the plain
+                         preconditioner continues from the iterate reached (fallback_used = 1) */
     int lc_level;     /* grid of the learned correction: Nz / 2^lc_level x Nx / 2^lc_level cells */
+    int mg_sub;       /* pre- and post-smoothing sweeps on the levels between (0: default 1; < 0: as the
+                         finest level) */
 } xc_fm_opts;
 
 typedef struct {  /* state.jl:131-134 BnchmData */

@@ -200,6 +200,34 @@ __device__ __forceinline__ void vel_resid_at(const LevD& L, const Flags& F, cons
     bool bc;
     r[k] = b[k] - vel_row(L, F, x, k, &dg, &bc, fine != 0);
 }
+// level-0 right-hand side b = r_v - GG z_p (stored) and the first damped-Jacobi sweep from zero
+__global__ __launch_bounds__(NT) void k_jacobi0_grad(LevD L, Flags F, const double* __restrict__ rV,
+                                                     const double* __restrict__ zP, double* __restrict__ b,
+                                                     double* __restrict__ xo, double omega) {
+    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t nW = L.nW(), n = nW + L.nU();
+    if (k >= n) return;
+    const int Nz = L.Nz, Nx = L.Nx;
+    double g = 0.0;
+    if (k != F.nu0) {
+        if (k < nW) {
+            const int r = ((int)k % (Nz + 1)), c = ((int)k / (Nz + 1));
+            if (r >= 1 && r <= Nz - 1 && c >= 1 && c <= Nx)
+                g = -L.invh * zP[r + (int64_t)c * (Nz + 2)] + L.invh * zP[r + 1 + (int64_t)c * (Nz + 2)];
+        } else {
+            const int64_t q = k - nW;
+            const int r = ((int)q % (Nz + 2)), c = ((int)q / (Nz + 2));
+            if (r >= 1 && r <= Nz)
+                g = -L.invh * zP[r + (int64_t)c * (Nz + 2)] + L.invh * zP[r + (int64_t)(c + 1) * (Nz + 2)];
+        }
+    }
+    const double bk = rV[k] - g;
+    b[k] = bk;
+    double dg;
+    bool bc;
+    vel_row(L, F, rV, k, &dg, &bc, true);  // diagonal only (the row value is unused)
+    xo[k] = (bc ? 1.0 : omega) * bk / dg;
+}
 __global__ __launch_bounds__(NT) void k_vel_resid(LevD L, Flags F, const double* __restrict__ x,
                                                   const double* __restrict__ b, double* __restrict__ r, int fine) {
     const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
@@ -796,13 +824,20 @@ __global__ void k_kry_init(Kry K, int m, double beta) {
 }
 // column j of H from the two Gram-Schmidt passes and the norm; Givens update;
 // |g[j+1]| and H[j+1,j] go to host-visible memory for the convergence test
+// d1, d2: the two Gram-Schmidt passes' coefficients; d2[j+1] = |w'|^2 after the first pass, so the
+// norm after the second is |w''|^2 = |w'|^2 - sum d2^2 (the basis is orthonormal); no third pass.
 __global__ void k_arnoldi(Kry K, int m, int j, const double* __restrict__ d1, const double* __restrict__ d2,
                           double* __restrict__ nrm2, double* __restrict__ host_out) {
     if (threadIdx.x != 0) return;
     double* Hj = K.H + (int64_t)j * (m + 1);
-    for (int i = 0; i <= j; ++i) Hj[i] = d1[i] + d2[i];
-    const double hn = sqrt(nrm2[0]);
-    nrm2[1] = hn;  // the new basis vector is w / hn (H[j+1, j] is rotated to zero below)
+    double ss = 0.0;
+    for (int i = 0; i <= j; ++i) {
+        Hj[i] = d1[i] + d2[i];
+        ss += d2[i] * d2[i];
+    }
+    const double h2 = d2[j + 1] - ss;
+    const double hn = h2 > 0.0 ? sqrt(h2) : 0.0;
+    nrm2[1] = hn;  // the new basis vector is w'' / hn (H[j+1, j] is rotated to zero below)
     Hj[j + 1] = hn;
     for (int i = 0; i < j; ++i) {
         const double a = Hj[i], b = Hj[i + 1];
@@ -822,11 +857,15 @@ __global__ void k_arnoldi(Kry K, int m, int j, const double* __restrict__ d1, co
     host_out[2 * j] = fabs(K.g[j + 1]);
     host_out[2 * j + 1] = hn;
 }
-// v = w / hn
-__global__ __launch_bounds__(NT) void k_scale_h(const double* __restrict__ w, const double* __restrict__ hp,
-                                                double* __restrict__ v, int64_t n) {
+// second Gram-Schmidt pass and normalisation in one sweep: v = (w - sum_q c[q] V[q]) / hn
+__global__ __launch_bounds__(NT) void k_orth_scale(const double* __restrict__ V, int64_t n, int K,
+                                                   const double* __restrict__ c, const double* __restrict__ w,
+                                                   const double* __restrict__ hp, double* __restrict__ v) {
     const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
-    if (i < n) v[i] = w[i] / hp[0];
+    if (i >= n) return;
+    double s = w[i];
+    for (int q = 0; q < K; ++q) s -= c[q] * V[(int64_t)q * n + i];
+    v[i] = s / hp[0];
 }
 // y = H(0:j, 0:j)^-1 g(0:j) (upper triangular after the rotations)
 __global__ void k_trisolve(Kry K, int m, int j) {
@@ -1218,10 +1257,21 @@ void build_levels(xc_ctx* ctx, StokesCtx* s, const xc_fluid* fl, double h) {
     }
 }
 
-struct MGOpts { int pre, post, coarse; double omega; };
+struct MGOpts { int pre, post, coarse; double omega; int sub; };  // sub: pre = post sweeps on levels >= 1 (0: as level 0)
+
+// solver controls with the defaults measured best on the 512^2 production step (zero fields: default)
+MGOpts mg_opts(const xc_fm_opts* o) {
+    MGOpts m;
+    m.pre = o->mg_pre > 0 ? o->mg_pre : 3;
+    m.post = o->mg_post > 0 ? o->mg_post : 3;
+    m.coarse = o->mg_coarse > 0 ? o->mg_coarse : 20;
+    m.omega = o->mg_omega > 0 ? o->mg_omega : 0.7;
+    m.sub = o->mg_sub > 0 ? o->mg_sub : (o->mg_sub == 0 ? 1 : 0);
+    return m;
+}
 
 // one V-cycle on level l: solve A x = b approximately from x = 0
-void vcycle(xc_ctx* ctx, StokesCtx* s, const Flags& F, size_t l, const MGOpts& mo) {
+void vcycle(xc_ctx* ctx, StokesCtx* s, const Flags& F, size_t l, const MGOpts& mo, bool first_done = false) {
     Level& L = s->lev[l];
     if (l > 0 && (int64_t)L.Nz * L.Nx <= TAIL_CELLS && s->lev.size() - l <= (size_t)TAIL_MAX) {
         TailArgs ta{};
@@ -1241,7 +1291,9 @@ void vcycle(xc_ctx* ctx, StokesCtx* s, const Flags& F, size_t l, const MGOpts& m
         if ((size_t)o * sizeof(double) > 65536) goto per_level;  // tail too large for one workgroup's LDS
         ta.bg = L.b;
         ta.xg = L.x;
-        ta.pre = mo.pre; ta.post = mo.post; ta.coarse = mo.coarse; ta.omega = mo.omega;
+        ta.pre = mo.sub > 0 ? mo.sub : mo.pre;
+        ta.post = mo.sub > 0 ? mo.sub : mo.post;
+        ta.coarse = mo.coarse; ta.omega = mo.omega;
         ta.F = F;
         hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(TAIL_NT), (size_t)o * sizeof(double), ctx->stream, ta);
         return;
@@ -1251,10 +1303,13 @@ per_level:
     const unsigned nb = nblk(L.nv);
     const int fine = l == 0;
     const bool coarsest = l + 1 == s->lev.size();
-    const int pre = coarsest ? mo.coarse : mo.pre;
-    // first sweep from zero
-    hipLaunchKernelGGL(k_jacobi, dim3(nb), dim3(NT), 0, ctx->stream, d, F, L.x, L.b, L.t, mo.omega, fine, 1);
-    std::swap(L.x, L.t);
+    const int pre = coarsest ? mo.coarse : (l > 0 && mo.sub > 0 ? mo.sub : mo.pre);
+    const int post = l > 0 && mo.sub > 0 ? mo.sub : mo.post;
+    // first sweep from zero (already done by the caller when first_done)
+    if (!first_done) {
+        hipLaunchKernelGGL(k_jacobi, dim3(nb), dim3(NT), 0, ctx->stream, d, F, L.x, L.b, L.t, mo.omega, fine, 1);
+        std::swap(L.x, L.t);
+    }
     for (int it = 1; it < pre; ++it) {
         hipLaunchKernelGGL(k_jacobi, dim3(nb), dim3(NT), 0, ctx->stream, d, F, L.x, L.b, L.t, mo.omega, fine, 0);
         std::swap(L.x, L.t);
@@ -1265,7 +1320,7 @@ per_level:
     hipLaunchKernelGGL(k_restrict, dim3(nblk(C.nv)), dim3(NT), 0, ctx->stream, d, C.d(), L.r, C.b);
     vcycle(ctx, s, F, l + 1, mo);
     hipLaunchKernelGGL(k_prolong, dim3(nb), dim3(NT), 0, ctx->stream, d, C.d(), F, C.x, L.x);
-    for (int it = 0; it < mo.post; ++it) {
+    for (int it = 0; it < post; ++it) {
         hipLaunchKernelGGL(k_jacobi, dim3(nb), dim3(NT), 0, ctx->stream, d, F, L.x, L.b, L.t, mo.omega, fine, 0);
         std::swap(L.x, L.t);
     }
@@ -1310,9 +1365,16 @@ void precond(xc_ctx* ctx, StokesCtx* s, const Flags& F, const double* v, double*
         hipLaunchKernelGGL(k_schur_int, dim3(nblk(nP)), dim3(NT), 0, ctx->stream, d, F, vP, zP);
     }
     hipLaunchKernelGGL(k_schur_ghost, dim3(nblk(nP)), dim3(NT), 0, ctx->stream, d, F, vP, zP);
-    hipLaunchKernelGGL(k_sub_grad, dim3(nblk(nv)), dim3(NT), 0, ctx->stream, d, F, v, zP, L.b);
-    vcycle(ctx, s, F, 0, mo);
-    hipLaunchKernelGGL(k_copyv, dim3(nblk(nv)), dim3(NT), 0, ctx->stream, L.x, z, nv);
+    // velocity: b = r_v - GG z_p and the first sweep in one kernel, then the V-cycle.  With an even
+    // number of level-0 sweeps the iterates alternate between z and the level's scratch and end in z.
+    double* own_x = L.x;
+    const bool direct = s->lev.size() > 1 && (mo.pre + mo.post) % 2 == 0;
+    if (direct) L.x = z;
+    hipLaunchKernelGGL(k_jacobi0_grad, dim3(nblk(nv)), dim3(NT), 0, ctx->stream, d, F, v, zP, L.b, L.t, mo.omega);
+    std::swap(L.x, L.t);
+    vcycle(ctx, s, F, 0, mo, true);
+    if (!direct) hipLaunchKernelGGL(k_copyv, dim3(nblk(nv)), dim3(NT), 0, ctx->stream, L.x, z, nv);
+    else L.x = own_x;  // L.x is z here; hand the level its own buffer back
 }
 
 // dots[0..K] = (V[q] . w) for q < K and w.w; synchronous, results in s->host
@@ -1513,12 +1575,10 @@ int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int 
             multidot_dev(ctx, s, s->V, j + 1, s->ws, n, d1);
             hipLaunchKernelGGL(k_orth, dim3(nb), dim3(NT), 0, ctx->stream, s->V, n, j + 1, d1, s->ws);
             multidot_dev(ctx, s, s->V, j + 1, s->ws, n, d2);
-            hipLaunchKernelGGL(k_orth, dim3(nb), dim3(NT), 0, ctx->stream, s->V, n, j + 1, d2, s->ws);
-            multidot_dev(ctx, s, s->V, 0, s->ws, n, dn);
             hipLaunchKernelGGL(k_arnoldi, dim3(1), dim3(64), 0, ctx->stream, K, m, j, d1, d2, dn, s->host_dev + 8);
             if (j + 1 < m)
-                hipLaunchKernelGGL(k_scale_h, dim3(nb), dim3(NT), 0, ctx->stream, s->ws, dn + 1,
-                                   s->V + (int64_t)(j + 1) * n, n);
+                hipLaunchKernelGGL(k_orth_scale, dim3(nb), dim3(NT), 0, ctx->stream, s->V, n, j + 1, d2, s->ws,
+                                   dn + 1, s->V + (int64_t)(j + 1) * n);
             XC_TRY_HIP(ctx, hipEventRecord(s->ev[j & 1], ctx->stream));
             if (j == 0) continue;
             // step j-1 is complete once its event has fired
@@ -1605,8 +1665,7 @@ int xc_fluidmech(xc_ctx* ctx, xc_fluid* fl, const xc_phase* ph, const xc_grid* g
     const Flags F = make_flags(o, par, Nz, Nx, bn != nullptr);
     const int64_t n = s->lev[0].nv + (int64_t)(Nz + 2) * (Nx + 2);
     const unsigned nb = nblk(n);
-    MGOpts mo{o->mg_pre > 0 ? o->mg_pre : 2, o->mg_post > 0 ? o->mg_post : 2, o->mg_coarse > 0 ? o->mg_coarse : 20,
-              o->mg_omega > 0 ? o->mg_omega : 0.7};
+    const MGOpts mo = mg_opts(o);
     // system: coefficients, RR, initial guess
     build_levels(ctx, s, fl, g->h);
     if ((st = pois_table(ctx, s, F, Nz, Nx, g->h))) return st;
@@ -1623,17 +1682,23 @@ int xc_fluidmech(xc_ctx* ctx, xc_fluid* fl, const xc_phase* ph, const xc_grid* g
     LcCfg lc{};
     if ((st = lc_config(ctx, s, o, g, par, &lc))) return st;
     SolveOut so{};
-    // a learned attempt gets a bounded budget before the plain preconditioner takes over
-    const int lmax = lc.on ? (maxit < 40 ? maxit : 40) : maxit;
+    // A learned attempt gets one short cycle (LC_PROBE iterations); if that has not converged the
+    // plain preconditioner continues from the iterate reached (FGMRES minimises the residual, so the
+    // learned directions cannot leave it worse than the start).
+    constexpr int LC_PROBE = 6;
+    const int lmax = lc.on ? (maxit < LC_PROBE ? maxit : LC_PROBE) : maxit;
     if ((st = fgmres_run(ctx, s, F, mo, m, n, s->bs, s->xs, target, bnorm, lmax, lc, &so))) return st;
     int fallback = 0;
-    if (lc.on && !so.conv) {  // the learned correction failed this system: plain multigrid from the start
+    if (lc.on && !so.conv) {
         fallback = 1;
-        hipLaunchKernelGGL(k_gather, dim3(nb), dim3(NT), 0, ctx->stream, fl->W, fl->U, fl->P, s->xs, Nz, Nx);
         lc.on = false;
         const int used = so.iters;
-        if ((st = fgmres_run(ctx, s, F, mo, m, n, s->bs, s->xs, target, bnorm, maxit, lc, &so))) return st;
+        const double rel0_l = so.rel0;
+        if (!(so.rel < 1e30))  // non-finite iterate: restart from the initial guess
+            hipLaunchKernelGGL(k_gather, dim3(nb), dim3(NT), 0, ctx->stream, fl->W, fl->U, fl->P, s->xs, Nz, Nx);
+        if ((st = fgmres_run(ctx, s, F, mo, m, n, s->bs, s->xs, target, bnorm, maxit - used, lc, &so))) return st;
         so.iters += used;
+        so.rel0 = rel0_l;
     }
     iters = so.iters;
     conv = so.conv;
@@ -1665,8 +1730,7 @@ static int piece_setup(xc_ctx* ctx, const xc_fluid* fl, const xc_grid* g, const 
     build_levels(ctx, s, fl, g->h);
     *F = make_flags(o, par, (int)g->Nz, (int)g->Nx, bnchm != 0);
     if ((st = pois_table(ctx, s, *F, (int)g->Nz, (int)g->Nx, g->h))) return st;
-    *mo = MGOpts{o->mg_pre > 0 ? o->mg_pre : 2, o->mg_post > 0 ? o->mg_post : 2, o->mg_coarse > 0 ? o->mg_coarse : 20,
-                 o->mg_omega > 0 ? o->mg_omega : 0.7};
+    *mo = mg_opts(o);
     *n = s->lev[0].nv + (g->Nz + 2) * (g->Nx + 2);
     return XC_OK;
 }

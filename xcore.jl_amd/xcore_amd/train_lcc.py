@@ -94,6 +94,19 @@ def buoyancy_rhs(g, fl, par, seed):
     return rhs
 
 
+def zero_mean_pressure(x, g):
+    """x with the interior mean of its pressure block removed.  The pressure pin (a +1 on one
+    continuity row) makes the constant pressure mode a near-null direction of LL, so LL^-1 v carries
+    a large constant pressure set by v at the pinned cell alone; that mode is left to FGMRES and
+    removed from the targets."""
+    Nz, Nx = g.Nz, g.Nx
+    nv = (Nz + 1) * (Nx + 2) + (Nz + 2) * (Nx + 1)
+    P = x[nv:].view(Nx + 2, Nz + 2)
+    y = x.clone()
+    y[nv:] -= P[1:-1, 1:-1].mean()
+    return y
+
+
 def generate(samples, N=256, level=2, seed=0, per_system=12, tol=1e-8, log_every=500, maxit=1000):
     """Training pairs from the library's own solves: for random lognormal-viscosity systems driven by
     buoyancy, the directions FGMRES feeds its preconditioner (the Krylov basis of a plain solve) and
@@ -125,6 +138,7 @@ def generate(samples, N=256, level=2, seed=0, per_system=12, tol=1e-8, log_every
                 keep[s] = False
                 s += 1
                 continue
+            e = zero_mean_pressure(e, g)
             a, sg = lc_restrict(fl, g, par, level, v)
             b, _ = lc_restrict(fl, g, par, level, e, sigma=sg)
             le[s] = a[0, ..., 0]
