@@ -153,6 +153,18 @@ __device__ __forceinline__ double face_flux(const double* c, int st, double vl, 
 }
 
 // cell index map for the BCs of fill_ghosts! (ghosts.jl:18-51), valid for |i| within one halo
+// Workgroup barrier for LDS traffic only: waits for this wave's LDS operations (lgkmcnt(0)) and
+// meets the other waves, but leaves global stores in flight (a full __syncthreads() would also wait
+// for every outstanding global memory operation).  Used where no thread reads global memory
+// another thread of the workgroup wrote.  The empty asm keeps the compiler from moving memory
+// accesses across it.
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // vmcnt(63) expcnt(7) lgkmcnt(0)
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
 __device__ __forceinline__ int map_cell(int i, int n, int bc) {
     if (bc == XC_PERIODIC) return i < 0 ? i + n : (i >= n ? i - n : i);
     return i < 0 ? 0 : (i >= n ? n - 1 : i);
@@ -210,7 +222,7 @@ __device__ __forceinline__ void advect_tile(const AdvArgs& a, int t) {
         }
         fs[k] = a.f[hz + (int64_t)hx * a.ld_f];
     }
-    __syncthreads();
+    lds_barrier();
 
     const int qx_top = (a.zbc == XC_PERIODIC && Nz > 1) ? Nz - 1 : 0;  // source row of qx[0, :]
     const int qx_bot = (a.zbc == XC_PERIODIC && Nz > 1) ? 0 : Nz - 1;  // source row of qx[Nz+1, :]
@@ -282,7 +294,7 @@ __device__ __forceinline__ void advect_tile(const AdvArgs& a, int t) {
         }
         qzs[lx * (BZ + 1) + fz] = q;
     }
-    __syncthreads();
+    lds_barrier();
 
     // ---- divergence, advect.jl:94: (qxr - qxl + qzb - qzt) * invh
 #pragma unroll
@@ -588,7 +600,7 @@ __device__ __forceinline__ void diffus_tile(const DifArgs& a, int t) {
         fs[k] = a.f[hz + hx * a.ld_f];
         ks[k] = a.k2 ? a.k[hz + hx * a.ld_k] * a.k2[hz + hx * a.ld_k2] : a.k[hz + hx * a.ld_k];
     }
-    __syncthreads();
+    lds_barrier();
     const int qx_top = (a.zbc == XC_PERIODIC && Nz > 1) ? Nz - 1 : 0;
     const int qx_bot = (a.zbc == XC_PERIODIC && Nz > 1) ? 0 : Nz - 1;
     const int qz_lft = (a.xbc == XC_PERIODIC && Nx > 1) ? Nx - 1 : 0;
