@@ -181,7 +181,7 @@ def stencil_bench(N, steps, warmup):
                      "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
     for k in ("weno5", "quick"):
         tz, tx, cand = tiles[k]
-        out[f"{k}_advect_with_flux"]["tile"] = f"{tz}x{tx}"
+        out[f"{k}_advect_with_flux"]["tile"] = f"{tz}x{abs(tx)}" + (" pipelined" if tx < 0 else "")
         out[f"{k}_advect_with_flux"]["tile_candidates_us"] = cand
     wk = out["weno5_advect_with_flux"]
     roofline = {"bound": "hbm", "achieved": wk["GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",

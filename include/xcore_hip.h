@@ -156,11 +156,13 @@ int xc_diffus_batch_plan(xc_ctx* ctx, int n, const xc_dif_member* members, int x
 int xc_batch_run(xc_ctx* ctx, const xc_batch_plan* plan);
 int xc_batch_destroy(xc_ctx* ctx, xc_batch_plan* plan);  /* synchronizes the context's stream */
 
-/* Tile shape the advection kernels use at this (scheme, fluxes, grid): height tz (z) x width tx, and the
- * measured time per launch of every candidate in us[4] (if non-NULL).  No counterpart in the
+/* Tile shape the advection kernels use at this (scheme, fluxes, grid): height tz (z) x width tx (tx < 0:
+ * the persistent, software-pipelined 64 x |tx| kernel), and the measured time per launch of every
+ * candidate -- 64x16, 128x8, 64x8, 32x32, 64x8 pipelined -- in us[5] (if non-NULL).  No counterpart in the
  * reference (the autotuner of the tile shapes).  XC_ENOCONV: not tuned yet (the default shape). */
 int xc_advect_tile(xc_ctx* ctx, int scheme, int with_flux, int64_t Nz, int64_t Nx, int* tz, int* tx, float* us);
-/* pin the tile shape for (scheme, fluxes, grid), e.g. a choice tuned in an earlier run: 64x16, 128x8, 64x8, 32x32 */
+/* pin the tile shape for (scheme, fluxes, grid), e.g. a choice tuned in an earlier run: 64x16, 128x8, 64x8,
+ * 32x32, or 64x-8 for the pipelined kernel (not for tvdim) */
 int xc_advect_set_tile(xc_ctx* ctx, int scheme, int with_flux, int64_t Nz, int64_t Nx, int tz, int tx);
 
 /* ---- field updates -------------------------------------------------------- */

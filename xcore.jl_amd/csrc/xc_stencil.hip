@@ -29,9 +29,10 @@ constexpr int TZ = 64;   // default tile height (z, contiguous)
 constexpr int TX = 16;   // default tile width (x)
 constexpr int NT = 256;  // lanes per workgroup
 // advection tile shapes the autotuner chooses from (variant 0 is the default)
-constexpr int NVAR = 4;
-constexpr int VAR_TZ[NVAR] = {64, 128, 64, 32};
-constexpr int VAR_TX[NVAR] = {16, 8, 8, 32};
+constexpr int NVAR = 5;
+constexpr int VAR_TZ[NVAR] = {64, 128, 64, 32, 64};
+constexpr int VAR_TX[NVAR] = {16, 8, 8, 32, 8};   // variant 4: 64 x 8, persistent and pipelined
+constexpr int VAR_PIPE = 4;
 
 __host__ __device__ constexpr int halo_of(int s) {
     return s == XC_CENTR ? 1 : s == XC_UPWD1 ? 1 : s == XC_WENO5 ? 3 : 2;
@@ -309,6 +310,125 @@ __device__ __forceinline__ void advect_tile(const AdvArgs& a, int t) {
     }
 }
 
+// Persistent, software-pipelined form (centred input, schemes without padded velocities): each
+// workgroup walks its XCD's tiles; while tile t is reconstructed from LDS, the f halo tile and the
+// face velocities of its next tile are already in flight into registers, so the memory system is
+// not idle during the reconstruction arithmetic.  Same face arithmetic as advect_tile.
+template <int S, bool FLUX, bool FAST, int BZ, int BX>
+__global__ __launch_bounds__(NT) void k_advect_pipe(AdvArgs a) {
+    static_assert(S != XC_TVDIM, "tvdim needs its padded velocities");
+    constexpr int G = halo_of(S);
+    constexpr int PZ = BZ + 2 * G;
+    constexpr int PX = BX + 2 * G;
+    constexpr int NF = (PX * PZ + NT - 1) / NT;
+    constexpr int NU = (BZ * (BX + 1) + NT - 1) / NT;
+    constexpr int NW = ((BZ + 1) * BX + NT - 1) / NT;
+    __shared__ double fs[PX * PZ];
+    __shared__ double us[BZ * (BX + 1)];
+    __shared__ double ws[(BZ + 1) * BX];
+    __shared__ double qxs[(BX + 1) * BZ];
+    __shared__ double qzs[BX * (BZ + 1)];
+    const int tid = threadIdx.x;
+    const int Nz = a.Nz, Nx = a.Nx;
+    double rf[NF], ru[NU], rw[NW];
+    auto fetch = [&](int t) {
+        const int z0 = (t % a.nbz) * BZ, x0 = (t / a.nbz) * BX;
+#pragma unroll
+        for (int i = 0; i < NF; ++i) {
+            const int k = tid + i * NT;
+            if (k < PX * PZ) {
+                const int lz = k % PZ, lx = k / PZ;
+                int gz = z0 + lz - G, gx = x0 + lx - G;
+                gz = gz < Nz + G - 1 ? gz : Nz + G - 1;
+                gx = gx < Nx + G - 1 ? gx : Nx + G - 1;
+                rf[i] = a.f[map_cell(gz, Nz, a.zbc) + (int64_t)map_cell(gx, Nx, a.xbc) * a.ld_f];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < NU; ++i) {
+            const int k = tid + i * NT;
+            const int lz = k % BZ, fx = k / BZ;
+            const int iz = z0 + lz, F = x0 + fx;
+            ru[i] = (k < BZ * (BX + 1) && iz < Nz && F <= Nx) ? a.u[iz + (int64_t)F * a.ld_u] : 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < NW; ++i) {
+            const int k = tid + i * NT;
+            const int fz = k % (BZ + 1), lx = k / (BZ + 1);
+            const int F = z0 + fz, ix = x0 + lx;
+            rw[i] = (k < (BZ + 1) * BX && ix < Nx && F <= Nz) ? a.w[F + (int64_t)ix * a.ld_w] : 0.0;
+        }
+    };
+    const int qx_top = (a.zbc == XC_PERIODIC && Nz > 1) ? Nz - 1 : 0;
+    const int qx_bot = (a.zbc == XC_PERIODIC && Nz > 1) ? 0 : Nz - 1;
+    const int qz_lft = (a.xbc == XC_PERIODIC && Nx > 1) ? Nx - 1 : 0;
+    const int qz_rgt = (a.xbc == XC_PERIODIC && Nx > 1) ? 0 : Nx - 1;
+    int it = blockIdx.x;
+    if (it < a.nb) fetch(xcd_tile(it, a.nb));
+    for (; it < a.nb; it += gridDim.x) {
+        const int t = xcd_tile(it, a.nb);
+        const int z0 = (t % a.nbz) * BZ, x0 = (t / a.nbz) * BX;
+#pragma unroll
+        for (int i = 0; i < NF; ++i)
+            if (tid + i * NT < PX * PZ) fs[tid + i * NT] = rf[i];
+#pragma unroll
+        for (int i = 0; i < NU; ++i)
+            if (tid + i * NT < BZ * (BX + 1)) us[tid + i * NT] = ru[i];
+#pragma unroll
+        for (int i = 0; i < NW; ++i)
+            if (tid + i * NT < (BZ + 1) * BX) ws[tid + i * NT] = rw[i];
+        lds_barrier();
+        if (it + (int)gridDim.x < a.nb) fetch(xcd_tile(it + gridDim.x, a.nb));  // next tile, in flight
+#pragma unroll
+        for (int k = tid; k < BZ * (BX + 1); k += NT) {
+            const int lz = k % BZ, fx = k / BZ;
+            const int iz = z0 + lz, F = x0 + fx;
+            double q = 0.0;
+            if (iz < Nz && F <= Nx) {
+                q = face_flux<S, FAST>(&fs[(fx + G) * PZ + lz + G], PZ, 0.0, us[k], 0.0);
+                if constexpr (FLUX) {
+                    if (fx < BX || F == Nx) {
+                        double* col = a.qx + (int64_t)F * a.ld_qx;
+                        col[iz + 1] = q;
+                        if (iz == qx_top) col[0] = q;
+                        if (iz == qx_bot) col[Nz + 1] = q;
+                    }
+                }
+            }
+            qxs[fx * BZ + lz] = q;
+        }
+#pragma unroll
+        for (int k = tid; k < (BZ + 1) * BX; k += NT) {
+            const int fz = k % (BZ + 1), lx = k / (BZ + 1);
+            const int F = z0 + fz, ix = x0 + lx;
+            double q = 0.0;
+            if (ix < Nx && F <= Nz) {
+                q = face_flux<S, FAST>(&fs[(lx + G) * PZ + fz + G], 1, 0.0, ws[k], 0.0);
+                if constexpr (FLUX) {
+                    if (fz < BZ || F == Nz) {
+                        a.qz[F + (int64_t)(ix + 1) * a.ld_qz] = q;
+                        if (ix == qz_lft) a.qz[F] = q;
+                        if (ix == qz_rgt) a.qz[F + (int64_t)(Nx + 1) * a.ld_qz] = q;
+                    }
+                }
+            }
+            qzs[lx * (BZ + 1) + fz] = q;
+        }
+        lds_barrier();
+#pragma unroll
+        for (int k = tid; k < BZ * BX; k += NT) {
+            const int lz = k % BZ, lx = k / BZ;
+            const int iz = z0 + lz, ix = x0 + lx;
+            if (iz < Nz && ix < Nx) {
+                const double qxr = qxs[(lx + 1) * BZ + lz], qxl = qxs[lx * BZ + lz];
+                const double qzb = qzs[lx * (BZ + 1) + lz + 1], qzt = qzs[lx * (BZ + 1) + lz];
+                a.adv[iz + (int64_t)ix * a.ld_adv] = (qxr - qxl + qzb - qzt) * a.invh;
+            }
+        }
+        lds_barrier();  // fs/us/ws/qxs/qzs are rewritten for the next tile
+    }
+}
+
 template <int S, bool HIN, bool FLUX, bool FAST, int BZ, int BX>
 __global__ __launch_bounds__(NT) void k_advect(AdvArgs a) {
     advect_tile<S, HIN, FLUX, FAST, BZ, BX>(a, xcd_tile(blockIdx.x, a.nb));
@@ -340,11 +460,32 @@ hipError_t launch_advect_v(const AdvArgs& a0, hipStream_t s) {
     return hipGetLastError();
 }
 
+// persistent grid: every resident workgroup slot of the device once (a multiple of the 8 XCDs)
+template <typename K>
+int pipe_grid(K kernel, int nb) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 8)
+        cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, NT, 0) != hipSuccess || per < 1) per = 4;
+    const int g = ((per * cus) / 8) * 8;
+    return nb < g ? nb : (g > 0 ? g : nb);
+}
+
 template <int S, bool HIN, bool FLUX, bool FAST>
 hipError_t launch_advect_f(const AdvArgs& a, hipStream_t s, int v) {
     if constexpr (HIN) {  // halo'd-input entry points keep the default tile
         return launch_advect_v<S, HIN, FLUX, FAST, 0>(a, s);
     } else {
+        if constexpr (S != XC_TVDIM) {
+            if (v == VAR_PIPE) {
+                AdvArgs b = a;
+                b.nbz = (b.Nz + VAR_TZ[VAR_PIPE] - 1) / VAR_TZ[VAR_PIPE];
+                b.nb = b.nbz * ((b.Nx + VAR_TX[VAR_PIPE] - 1) / VAR_TX[VAR_PIPE]);
+                auto kern = k_advect_pipe<S, FLUX, FAST, VAR_TZ[VAR_PIPE], VAR_TX[VAR_PIPE]>;
+                hipLaunchKernelGGL(kern, dim3(pipe_grid(kern, b.nb)), dim3(NT), 0, s, b);
+                return hipGetLastError();
+            }
+        }
         switch (v) {
             case 1: return launch_advect_v<S, HIN, FLUX, FAST, 1>(a, s);
             case 2: return launch_advect_v<S, HIN, FLUX, FAST, 2>(a, s);
@@ -403,6 +544,7 @@ int tune_advect(xc_ctx* ctx, int scheme, bool flux, bool ex, const AdvArgs& a, i
     XC_TRY_HIP(ctx, hipEventCreate(&e1));
     float best = 1e30f;
     for (int v = 0; v < NVAR; ++v) {
+        if (v == VAR_PIPE && scheme == XC_TVDIM) { t.us[v] = 1e30f; continue; }
         hipError_t e = hipSuccess;
         for (int r = 0; r < 4 && e == hipSuccess; ++r) {  // first launch warms up
             if (r == 1) (void)hipEventRecord(e0, ctx->stream);
@@ -858,7 +1000,7 @@ int xc_advect_tile(xc_ctx* ctx, int scheme, int with_flux, int64_t Nz, int64_t N
     const TuneEntry* te = tune_find(ctx, scheme, with_flux ? 1 : 0, (ctx->flags & XC_CTX_EXACT) ? 1 : 0, (int)Nz, (int)Nx);
     const int v = te ? te->v : 0;
     *tz = VAR_TZ[v];
-    *tx = VAR_TX[v];
+    *tx = v == VAR_PIPE ? -VAR_TX[v] : VAR_TX[v];
     if (us)
         for (int k = 0; k < NVAR; ++k) us[k] = te ? te->us[k] : 0.f;
     return te ? XC_OK : XC_ENOCONV;
@@ -867,8 +1009,9 @@ int xc_advect_tile(xc_ctx* ctx, int scheme, int with_flux, int64_t Nz, int64_t N
 int xc_advect_set_tile(xc_ctx* ctx, int scheme, int with_flux, int64_t Nz, int64_t Nx, int tz, int tx) {
     XC_CHECK_CTX(ctx);
     int v = -1;
-    for (int k = 0; k < NVAR; ++k)
+    for (int k = 0; k < VAR_PIPE; ++k)
         if (VAR_TZ[k] == tz && VAR_TX[k] == tx) v = k;
+    if (tz == VAR_TZ[VAR_PIPE] && tx == -VAR_TX[VAR_PIPE] && scheme != XC_TVDIM) v = VAR_PIPE;
     if (v < 0) return xc_fail(ctx, XC_EINVAL_ARG, "advect_set_tile: no %d x %d tile", tz, tx);
     const int ex = (ctx->flags & XC_CTX_EXACT) ? 1 : 0;
     TuneEntry* te = tune_find(ctx, scheme, with_flux ? 1 : 0, ex, (int)Nz, (int)Nx);

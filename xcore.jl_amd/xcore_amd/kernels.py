@@ -55,11 +55,12 @@ def advect_tile(scheme, Nz, Nx, with_flux=True):
     """(tz, tx, {tile: us}) the advection kernels use at this size; tuned=False if not measured yet."""
     c = context()
     tz, tx = ctypes.c_int(0), ctypes.c_int(0)
-    us = (ctypes.c_float * 4)()
+    us = (ctypes.c_float * 5)()
     code = lib().xc_advect_tile(c.ptr, _sym(scheme, SCHEMES, "scheme"), int(bool(with_flux)), int(Nz), int(Nx),
                                 ctypes.byref(tz), ctypes.byref(tx), us)
-    shapes = [(64, 16), (128, 8), (64, 8), (32, 32)]
-    return tz.value, tx.value, {f"{a}x{b}": round(float(u), 2) for (a, b), u in zip(shapes, us)}, code == 0
+    names = ["64x16", "128x8", "64x8", "32x32", "64x8p"]
+    cand = {n: round(float(u), 2) for n, u in zip(names, us) if float(u) < 1e29}
+    return tz.value, tx.value, cand, code == 0
 
 
 def _sym(s, table, kind):
@@ -169,7 +170,8 @@ def diffus_centered_with_flux_(dff, qx, qz, f, k, h, xBC="periodic", zBC="closed
 
 
 def set_advect_tile(scheme, Nz, Nx, tz, tx, with_flux=True):
-    """Pin the advection tile shape for this (scheme, fluxes, grid): one of 64x16, 128x8, 64x8, 32x32."""
+    """Pin the advection tile shape for this (scheme, fluxes, grid): one of 64x16, 128x8, 64x8, 32x32, or
+    (64, -8) for the persistent pipelined kernel."""
     c = context()
     c.check(lib().xc_advect_set_tile(c.ptr, _sym(scheme, SCHEMES, "scheme"), int(bool(with_flux)), int(Nz), int(Nx),
                                      int(tz), int(tx)), "advect_set_tile")
