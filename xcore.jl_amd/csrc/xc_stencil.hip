@@ -68,25 +68,34 @@ __device__ __forceinline__ double weno5_poly(double fmm, double fm, double fc, d
     return (w1 * p1 + w2 * p2 + w3 * p3) / (w1 + w2 + w3);
 }
 
-// Same reconstruction with one division: the weights d_k/(b_k+eps)^2 are scaled
-// by prod_k (b_k+eps)^2 before normalising (mathematically identical; differs
-// from the reference's rounding in the last bits).  FMA contraction allowed.
+// Same reconstruction, fewer operations (the fast path; differs from the reference's rounding in the
+// last bits): the 1/6 of the candidate polynomials and the linear weights d_k = (0.1, 0.6, 0.3) are
+// folded into the polynomial coefficients, the nonlinear weights d_k/(b_k+eps)^2 are scaled by
+// prod_k (b_k+eps)^2 so that one quotient remains, and that quotient is a reciprocal refined by two
+// Newton-Raphson steps plus one residual correction of the product (the denominator is a positive
+// normal number: each b_k + eps >= eps).  FMA contraction allowed.
 __device__ __forceinline__ double weno5_poly_fast(double fmm, double fm, double fc, double fp, double fpp) {
 #pragma clang fp contract(fast)
     const double e = 2.220446049250313e-16;
-    const double sixth = 1.0 / 6.0, quarter = 0.25, tt = 13.0 / 12.0;
-    const double p1 = sixth * (2.0 * fmm - 7.0 * fm + 11.0 * fc);
-    const double p2 = sixth * (-fm + 5.0 * fc + 2.0 * fp);
-    const double p3 = sixth * (2.0 * fc + 5.0 * fp - fpp);
+    const double tt = 13.0 / 12.0;
+    const double P1 = fmm * (0.1 * 2.0 / 6.0) + fm * (-0.1 * 7.0 / 6.0) + fc * (0.1 * 11.0 / 6.0);
+    const double P2 = fm * (-0.6 / 6.0) + fc * (0.6 * 5.0 / 6.0) + fp * (0.6 * 2.0 / 6.0);
+    const double P3 = fc * (0.3 * 2.0 / 6.0) + fp * (0.3 * 5.0 / 6.0) + fpp * (-0.3 / 6.0);
     const double s1 = fmm - 2.0 * fm + fc, t1 = fmm - 4.0 * fm + 3.0 * fc;
     const double s2 = fm - 2.0 * fc + fp, t2 = fm - fp;
     const double s3 = fc - 2.0 * fp + fpp, t3 = 3.0 * fc - 4.0 * fp + fpp;
-    const double a1 = tt * (s1 * s1) + quarter * (t1 * t1) + e;
-    const double a2 = tt * (s2 * s2) + quarter * (t2 * t2) + e;
-    const double a3 = tt * (s3 * s3) + quarter * (t3 * t3) + e;
+    const double a1 = (tt * s1) * s1 + ((0.25 * t1) * t1 + e);
+    const double a2 = (tt * s2) * s2 + ((0.25 * t2) * t2 + e);
+    const double a3 = (tt * s3) * s3 + ((0.25 * t3) * t3 + e);
     const double q1 = a1 * a1, q2 = a2 * a2, q3 = a3 * a3;
-    const double w1 = 0.1 * (q2 * q3), w2 = 0.6 * (q1 * q3), w3 = 0.3 * (q1 * q2);
-    return (w1 * p1 + w2 * p2 + w3 * p3) / (w1 + w2 + w3);
+    const double q23 = q2 * q3, q13 = q1 * q3, q12 = q1 * q2;
+    const double num = q23 * P1 + q13 * P2 + q12 * P3;
+    const double den = q23 * 0.1 + q13 * 0.6 + q12 * 0.3;
+    double r = __builtin_amdgcn_rcp(den);
+    r = fma(r, fma(-den, r, 1.0), r);
+    r = fma(r, fma(-den, r, 1.0), r);
+    const double y = num * r;
+    return fma(r, fma(-den, y, num), y);
 }
 
 // advect.jl:216-229 (eps fixed at 1e-6)
@@ -170,6 +179,18 @@ __device__ __forceinline__ int map_cell(int i, int n, int bc) {
     if (bc == XC_PERIODIC) return i < 0 ? i + n : (i >= n ? i - n : i);
     return i < 0 ? 0 : (i >= n ? n - 1 : i);
 }
+// x of lane l + 1 in lane l (lane 63 receives 0): DPP wavefront shift by one lane.  Every lane of
+// the wave must execute it.
+__device__ __forceinline__ double lane_next(double x) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x130, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x130, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+// map_cell for a wave-uniform periodic flag, as selects (no branch between a tile's loads)
+__device__ __forceinline__ int map_cell_sel(int i, int n, bool per) {
+    const int lo = per ? i + n : 0, hi = per ? i - n : n - 1;
+    return i < 0 ? lo : (i >= n ? hi : i);
+}
 // face index map for the padded TVD velocities (phsevo.jl:34-53): faces -1 and n+1
 __device__ __forceinline__ int map_face(int f, int n, int bc) {
     if (f < 0) return bc == XC_PERIODIC ? n - 1 : 0;
@@ -191,12 +212,19 @@ struct AdvArgs {
 
 // HIN: f is a halo'd buffer (advect!); otherwise f is (Nz, Nx) and the BCs are
 // an index map (advect_centered!).  FLUX: also write qx/qz with padding.
-// one tile t of one grid; the body of k_advect and of the ensemble kernel k_advect_batch
+// One tile t of one grid; the body of k_advect and of the ensemble kernel k_advect_batch.
+// Every global load of the tile (the f halo tile and the face velocities) is issued back to back
+// into registers before the first LDS store, so a wave keeps all of them in flight at once; indexes
+// past the grid are clamped to a valid element whose value is never used.
 template <int S, bool HIN, bool FLUX, bool FAST, int BZ, int BX>
 __device__ __forceinline__ void advect_tile(const AdvArgs& a, int t) {
     constexpr int G = halo_of(S);
     constexpr int PZ = BZ + 2 * G;
     constexpr int PX = BX + 2 * G;
+    constexpr int NF = (PX * PZ + NT - 1) / NT;        // f halo-tile elements per lane
+    constexpr int NU = (BZ * (BX + 1) + NT - 1) / NT;  // x-faces per lane
+    constexpr int NW = ((BZ + 1) * BX + NT - 1) / NT;  // z-faces per lane
+    constexpr int NV = S == XC_TVDIM ? 3 : 1;          // velocities per face (TVD: left, own, right)
     __shared__ double fs[PX * PZ];
     __shared__ double qxs[(BX + 1) * BZ];
     __shared__ double qzs[BX * (BZ + 1)];
@@ -205,95 +233,115 @@ __device__ __forceinline__ void advect_tile(const AdvArgs& a, int t) {
     const int x0 = (t / a.nbz) * BX;
     const int tid = threadIdx.x;
     const int Nz = a.Nz, Nx = a.Nx;
+    const bool zper = a.zbc == XC_PERIODIC, xper = a.xbc == XC_PERIODIC;
 
-    // ---- stage the tile + halo (cross-shaped use; corners loaded for simplicity)
+    // ---- issue every load: f tile + halo (cross-shaped use; corners loaded for simplicity)
+    double rf[NF];
 #pragma unroll
-    for (int k = tid; k < PX * PZ; k += NT) {
+    for (int i = 0; i < NF; ++i) {
+        const int k = min(tid + i * NT, PX * PZ - 1);
         const int lz = k % PZ, lx = k / PZ;
-        int gz = z0 + lz - G, gx = x0 + lx - G;
-        gz = gz < Nz + G - 1 ? gz : Nz + G - 1;
-        gx = gx < Nx + G - 1 ? gx : Nx + G - 1;
+        const int gz = min(z0 + lz - G, Nz + G - 1), gx = min(x0 + lx - G, Nx + G - 1);
         int hz, hx;
         if constexpr (HIN) {
             hz = gz + G;
             hx = gx + G;
         } else {
-            hz = map_cell(gz, Nz, a.zbc);
-            hx = map_cell(gx, Nx, a.xbc);
+            hz = map_cell_sel(gz, Nz, zper);
+            hx = map_cell_sel(gx, Nx, xper);
         }
-        fs[k] = a.f[hz + (int64_t)hx * a.ld_f];
+        rf[i] = a.f[hz + (int64_t)hx * a.ld_f];
     }
+    double ru[NU][NV], rw[NW][NV];
+#pragma unroll
+    for (int i = 0; i < NU; ++i) {
+        const int k = min(tid + i * NT, BZ * (BX + 1) - 1);
+        const int iz = min(z0 + k % BZ, Nz - 1), F = min(x0 + k / BZ, Nx);
+        if constexpr (S == XC_TVDIM) {
+            if constexpr (HIN) {
+                ru[i][0] = a.u[iz + (int64_t)F * a.ld_u];
+                ru[i][1] = a.u[iz + (int64_t)(F + 1) * a.ld_u];
+                ru[i][2] = a.u[iz + (int64_t)(F + 2) * a.ld_u];
+            } else {
+                ru[i][0] = a.u[iz + (int64_t)map_face(F - 1, Nx, a.xbc) * a.ld_u];
+                ru[i][1] = a.u[iz + (int64_t)F * a.ld_u];
+                ru[i][2] = a.u[iz + (int64_t)map_face(F + 1, Nx, a.xbc) * a.ld_u];
+            }
+        } else {
+            ru[i][0] = a.u[iz + (int64_t)F * a.ld_u];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+        const int k = min(tid + i * NT, (BZ + 1) * BX - 1);
+        const int F = min(z0 + k % (BZ + 1), Nz), ix = min(x0 + k / (BZ + 1), Nx - 1);
+        if constexpr (S == XC_TVDIM) {
+            if constexpr (HIN) {
+                rw[i][0] = a.w[F + (int64_t)ix * a.ld_w];
+                rw[i][1] = a.w[F + 1 + (int64_t)ix * a.ld_w];
+                rw[i][2] = a.w[F + 2 + (int64_t)ix * a.ld_w];
+            } else {
+                rw[i][0] = a.w[map_face(F - 1, Nz, a.zbc) + (int64_t)ix * a.ld_w];
+                rw[i][1] = a.w[F + (int64_t)ix * a.ld_w];
+                rw[i][2] = a.w[map_face(F + 1, Nz, a.zbc) + (int64_t)ix * a.ld_w];
+            }
+        } else {
+            rw[i][0] = a.w[F + (int64_t)ix * a.ld_w];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NF; ++i)
+        if (tid + i * NT < PX * PZ) fs[tid + i * NT] = rf[i];
     lds_barrier();
 
-    const int qx_top = (a.zbc == XC_PERIODIC && Nz > 1) ? Nz - 1 : 0;  // source row of qx[0, :]
-    const int qx_bot = (a.zbc == XC_PERIODIC && Nz > 1) ? 0 : Nz - 1;  // source row of qx[Nz+1, :]
-    const int qz_lft = (a.xbc == XC_PERIODIC && Nx > 1) ? Nx - 1 : 0;
-    const int qz_rgt = (a.xbc == XC_PERIODIC && Nx > 1) ? 0 : Nx - 1;
+    const int qx_top = (zper && Nz > 1) ? Nz - 1 : 0;  // source row of qx[0, :]
+    const int qx_bot = (zper && Nz > 1) ? 0 : Nz - 1;  // source row of qx[Nz+1, :]
+    const int qz_lft = (xper && Nx > 1) ? Nx - 1 : 0;
+    const int qz_rgt = (xper && Nx > 1) ? 0 : Nx - 1;
+    constexpr int VI = S == XC_TVDIM ? 1 : 0;  // index of the face's own velocity
 
     // ---- x-faces: BZ rows x (BX+1) faces
 #pragma unroll
-    for (int k = tid; k < BZ * (BX + 1); k += NT) {
-        const int lz = k % BZ, fx = k / BZ;
-        const int iz = z0 + lz, F = x0 + fx;
-        double q = 0.0;
-        if (iz < Nz && F <= Nx) {
-            double vl = 0.0, v, vr = 0.0;
-            if constexpr (S == XC_TVDIM) {
-                if constexpr (HIN) {
-                    vl = a.u[iz + (int64_t)F * a.ld_u];
-                    v = a.u[iz + (int64_t)(F + 1) * a.ld_u];
-                    vr = a.u[iz + (int64_t)(F + 2) * a.ld_u];
-                } else {
-                    vl = a.u[iz + (int64_t)map_face(F - 1, Nx, a.xbc) * a.ld_u];
-                    v = a.u[iz + (int64_t)F * a.ld_u];
-                    vr = a.u[iz + (int64_t)map_face(F + 1, Nx, a.xbc) * a.ld_u];
-                }
-            } else {
-                v = a.u[iz + (int64_t)F * a.ld_u];
-            }
-            q = face_flux<S, FAST>(&fs[(fx + G) * PZ + lz + G], PZ, vl, v, vr);
-            if constexpr (FLUX) {
-                if (fx < BX || F == Nx) {
-                    double* col = a.qx + (int64_t)F * a.ld_qx;
-                    col[iz + 1] = q;
-                    if (iz == qx_top) col[0] = q;
-                    if (iz == qx_bot) col[Nz + 1] = q;
+    for (int i = 0; i < NU; ++i) {
+        const int k = tid + i * NT;
+        if (k < BZ * (BX + 1)) {
+            const int lz = k % BZ, fx = k / BZ;
+            const int iz = z0 + lz, F = x0 + fx;
+            double q = 0.0;
+            if (iz < Nz && F <= Nx) {
+                q = face_flux<S, FAST>(&fs[(fx + G) * PZ + lz + G], PZ, ru[i][0], ru[i][VI], ru[i][NV - 1]);
+                if constexpr (FLUX) {
+                    if (fx < BX || F == Nx) {
+                        double* col = a.qx + (int64_t)F * a.ld_qx;
+                        col[iz + 1] = q;
+                        if (iz == qx_top) col[0] = q;
+                        if (iz == qx_bot) col[Nz + 1] = q;
+                    }
                 }
             }
+            qxs[fx * BZ + lz] = q;
         }
-        qxs[fx * BZ + lz] = q;
     }
     // ---- z-faces: (BZ+1) faces x BX columns
 #pragma unroll
-    for (int k = tid; k < (BZ + 1) * BX; k += NT) {
-        const int fz = k % (BZ + 1), lx = k / (BZ + 1);
-        const int F = z0 + fz, ix = x0 + lx;
-        double q = 0.0;
-        if (ix < Nx && F <= Nz) {
-            double vl = 0.0, v, vr = 0.0;
-            if constexpr (S == XC_TVDIM) {
-                if constexpr (HIN) {
-                    vl = a.w[F + (int64_t)ix * a.ld_w];
-                    v = a.w[F + 1 + (int64_t)ix * a.ld_w];
-                    vr = a.w[F + 2 + (int64_t)ix * a.ld_w];
-                } else {
-                    vl = a.w[map_face(F - 1, Nz, a.zbc) + (int64_t)ix * a.ld_w];
-                    v = a.w[F + (int64_t)ix * a.ld_w];
-                    vr = a.w[map_face(F + 1, Nz, a.zbc) + (int64_t)ix * a.ld_w];
-                }
-            } else {
-                v = a.w[F + (int64_t)ix * a.ld_w];
-            }
-            q = face_flux<S, FAST>(&fs[(lx + G) * PZ + fz + G], 1, vl, v, vr);
-            if constexpr (FLUX) {
-                if (fz < BZ || F == Nz) {
-                    a.qz[F + (int64_t)(ix + 1) * a.ld_qz] = q;
-                    if (ix == qz_lft) a.qz[F] = q;
-                    if (ix == qz_rgt) a.qz[F + (int64_t)(Nx + 1) * a.ld_qz] = q;
+    for (int i = 0; i < NW; ++i) {
+        const int k = tid + i * NT;
+        if (k < (BZ + 1) * BX) {
+            const int fz = k % (BZ + 1), lx = k / (BZ + 1);
+            const int F = z0 + fz, ix = x0 + lx;
+            double q = 0.0;
+            if (ix < Nx && F <= Nz) {
+                q = face_flux<S, FAST>(&fs[(lx + G) * PZ + fz + G], 1, rw[i][0], rw[i][VI], rw[i][NV - 1]);
+                if constexpr (FLUX) {
+                    if (fz < BZ || F == Nz) {
+                        a.qz[F + (int64_t)(ix + 1) * a.ld_qz] = q;
+                        if (ix == qz_lft) a.qz[F] = q;
+                        if (ix == qz_rgt) a.qz[F + (int64_t)(Nx + 1) * a.ld_qz] = q;
+                    }
                 }
             }
+            qzs[lx * (BZ + 1) + fz] = q;
         }
-        qzs[lx * (BZ + 1) + fz] = q;
     }
     lds_barrier();
 
@@ -308,6 +356,185 @@ __device__ __forceinline__ void advect_tile(const AdvArgs& a, int t) {
             a.adv[iz + (int64_t)ix * a.ld_adv] = (qxr - qxl + qzb - qzt) * a.invh;
         }
     }
+}
+
+// Lane-row form of advect_tile for 64-row tiles: lane l of every wave owns tile row l and each wave
+// takes whole columns (of the f halo tile, of x-faces, of z-faces, of cells), so column indexes and
+// column addresses are wave-uniform (scalar registers), only the row offset is per lane, and every
+// LDS access of a wave touches 64 consecutive doubles.  Same loads, face arithmetic and stores as
+// advect_tile, hence bitwise the same results.
+template <int S, bool HIN, bool FLUX, bool FAST, int BX>
+__device__ __forceinline__ void advect_tile_lr(const AdvArgs& a, int t) {
+    constexpr int BZ = 64;
+    constexpr int G = halo_of(S);
+    constexpr int PZ = BZ + 2 * G;
+    constexpr int PX = BX + 2 * G;
+    constexpr int NWV = NT / 64;                  // waves per workgroup
+    constexpr int CF = (PX + NWV - 1) / NWV;      // halo-tile columns per wave
+    constexpr int CU = (BX + 1 + NWV - 1) / NWV;  // x-face columns per wave
+    constexpr int CW = (BX + NWV - 1) / NWV;      // z-face (and cell) columns per wave
+    constexpr int NV = S == XC_TVDIM ? 3 : 1;     // velocities per face (TVD: left, own, right)
+    constexpr int VI = S == XC_TVDIM ? 1 : 0;
+    constexpr int WC = 1 % NWV;                   // the wave that also takes the closing z-faces
+    static_assert(BX <= 64 && NT % 64 == 0, "lane-row tiles");
+    __shared__ double fs[PX * PZ];
+    __shared__ double qxs[(BX + 1) * BZ];
+    __shared__ double qzc[BX];  // closing z-faces; the other z-faces stay in registers
+
+    const int l = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int z0 = (t % a.nbz) * BZ, x0 = (t / a.nbz) * BX;
+    const int Nz = a.Nz, Nx = a.Nx;
+    const bool zper = a.zbc == XC_PERIODIC, xper = a.xbc == XC_PERIODIC;
+
+    // ---- issue every load of the tile before the first LDS store
+    // halo-tile rows of this lane: l - G, and (lanes l < 2G) BZ - G + l
+    const int gzA = min(z0 + l - G, Nz + G - 1);
+    const int gzB = min(z0 + BZ - G + min(l, 2 * G - 1), Nz + G - 1);
+    int hzA, hzB;
+    if constexpr (HIN) {
+        hzA = gzA + G;
+        hzB = gzB + G;
+    } else {
+        hzA = map_cell_sel(gzA, Nz, zper);
+        hzB = map_cell_sel(gzB, Nz, zper);
+    }
+    double rf[CF][2];
+#pragma unroll
+    for (int j = 0; j < CF; ++j) {
+        const int c = min(wv + j * NWV, PX - 1);  // past PX: a duplicate, never stored
+        const int gx = min(x0 + c - G, Nx + G - 1);
+        int hx;
+        if constexpr (HIN) hx = gx + G;
+        else hx = map_cell_sel(gx, Nx, xper);
+        const double* col = a.f + (int64_t)hx * a.ld_f;
+        rf[j][0] = col[hzA];
+        rf[j][1] = col[hzB];
+    }
+    const int izc = min(z0 + l, Nz - 1);
+    double ru[CU][NV];
+#pragma unroll
+    for (int j = 0; j < CU; ++j) {
+        const int F = min(x0 + min(wv + j * NWV, BX), Nx);
+        if constexpr (S == XC_TVDIM) {
+            int c0 = F, c2 = F + 2, c1 = F + 1;
+            if constexpr (!HIN) {
+                c0 = map_face(F - 1, Nx, a.xbc);
+                c1 = F;
+                c2 = map_face(F + 1, Nx, a.xbc);
+            }
+            ru[j][0] = a.u[izc + (int64_t)c0 * a.ld_u];
+            ru[j][1] = a.u[izc + (int64_t)c1 * a.ld_u];
+            ru[j][2] = a.u[izc + (int64_t)c2 * a.ld_u];
+        } else {
+            ru[j][0] = a.u[izc + (int64_t)F * a.ld_u];
+        }
+    }
+    auto wload = [&](double* r, int F, int ix) {  // F: face row, 0..Nz; ix: cell column, 0..Nx-1
+        const double* col = a.w + (int64_t)ix * a.ld_w;
+        if constexpr (S == XC_TVDIM) {
+            if constexpr (HIN) {
+                r[0] = col[F];
+                r[1] = col[F + 1];
+                r[2] = col[F + 2];
+            } else {
+                r[0] = col[map_face(F - 1, Nz, a.zbc)];
+                r[1] = col[F];
+                r[2] = col[map_face(F + 1, Nz, a.zbc)];
+            }
+        } else {
+            r[0] = col[F];
+        }
+    };
+    double rw[CW][NV];
+#pragma unroll
+    for (int j = 0; j < CW; ++j)
+        wload(rw[j], min(z0 + l, Nz), min(x0 + min(wv + j * NWV, BX - 1), Nx - 1));
+    double rc[NV];  // closing z-faces (tile row BZ) of the columns x0 + l, lanes l < BX
+    if (wv == WC) wload(rc, min(z0 + BZ, Nz), min(x0 + min(l, BX - 1), Nx - 1));
+
+#pragma unroll
+    for (int j = 0; j < CF; ++j) {
+        const int c = wv + j * NWV;
+        if (c < PX) {
+            fs[c * PZ + l] = rf[j][0];
+            if (l < 2 * G) fs[c * PZ + BZ + l] = rf[j][1];
+        }
+    }
+    lds_barrier();
+
+    const int qx_top = (zper && Nz > 1) ? Nz - 1 : 0;  // source row of qx[0, :]
+    const int qx_bot = (zper && Nz > 1) ? 0 : Nz - 1;  // source row of qx[Nz+1, :]
+    const int qz_lft = (xper && Nx > 1) ? Nx - 1 : 0;
+    const int qz_rgt = (xper && Nx > 1) ? 0 : Nx - 1;
+    const int iz = z0 + l;
+
+    // ---- x-faces: column F = x0 + fx of the wave, row iz of the lane
+#pragma unroll
+    for (int j = 0; j < CU; ++j) {
+        const int fx = wv + j * NWV;
+        if (fx <= BX) {
+            const int F = x0 + fx;
+            double q = 0.0;
+            if (F <= Nx && iz < Nz) {
+                q = face_flux<S, FAST>(&fs[(fx + G) * PZ + l + G], PZ, ru[j][0], ru[j][VI], ru[j][NV - 1]);
+                if constexpr (FLUX) {
+                    if (fx < BX || F == Nx) {
+                        double* col = a.qx + (int64_t)F * a.ld_qx;
+                        col[iz + 1] = q;
+                        if (iz == qx_top) col[0] = q;
+                        if (iz == qx_bot) col[Nz + 1] = q;
+                    }
+                }
+            }
+            qxs[fx * BZ + l] = q;
+        }
+    }
+    // ---- z-faces: face row F = z0 + l of the wave's column ix
+    auto zface = [&](const double* r, int lx, int fz) -> double {
+        const int F = z0 + fz, ix = x0 + lx;
+        double q = 0.0;
+        if (ix < Nx && F <= Nz) {
+            q = face_flux<S, FAST>(&fs[(lx + G) * PZ + fz + G], 1, r[0], r[VI], r[NV - 1]);
+            if constexpr (FLUX) {
+                if (fz < BZ || F == Nz) {
+                    a.qz[F + (int64_t)(ix + 1) * a.ld_qz] = q;
+                    if (ix == qz_lft) a.qz[F] = q;
+                    if (ix == qz_rgt) a.qz[F + (int64_t)(Nx + 1) * a.ld_qz] = q;
+                }
+            }
+        }
+        return q;
+    };
+    double zq[CW];
+#pragma unroll
+    for (int j = 0; j < CW; ++j) {
+        const int lx = wv + j * NWV;
+        zq[j] = lx < BX ? zface(rw[j], lx, l) : 0.0;
+    }
+    if (wv == WC && l < BX) qzc[l] = zface(rc, l, BZ);
+    lds_barrier();
+
+    // ---- divergence, advect.jl:94: (qxr - qxl + qzb - qzt) * invh; the face below row l is lane
+    // l + 1's z-face (a wave shift), below row 63 the closing face
+#pragma unroll
+    for (int j = 0; j < CW; ++j) {
+        const int lx = wv + j * NWV;
+        const int ix = x0 + lx;
+        const double below = lane_next(zq[j]);
+        if (lx < BX && ix < Nx && iz < Nz) {
+            const double qxr = qxs[(lx + 1) * BZ + l], qxl = qxs[lx * BZ + l];
+            const double qzb = l == 63 ? qzc[lx] : below, qzt = zq[j];
+            a.adv[iz + (int64_t)ix * a.ld_adv] = (qxr - qxl + qzb - qzt) * a.invh;
+        }
+    }
+}
+
+// the tile body a kernel uses: lane-row form for 64-row tiles, the generic form otherwise
+template <int S, bool HIN, bool FLUX, bool FAST, int BZ, int BX>
+__device__ __forceinline__ void advect_tile_any(const AdvArgs& a, int t) {
+    if constexpr (BZ == 64) advect_tile_lr<S, HIN, FLUX, FAST, BX>(a, t);
+    else advect_tile<S, HIN, FLUX, FAST, BZ, BX>(a, t);
 }
 
 // Persistent, software-pipelined form (centred input, schemes without padded velocities): each
@@ -431,7 +658,7 @@ __global__ __launch_bounds__(NT) void k_advect_pipe(AdvArgs a) {
 
 template <int S, bool HIN, bool FLUX, bool FAST, int BZ, int BX>
 __global__ __launch_bounds__(NT) void k_advect(AdvArgs a) {
-    advect_tile<S, HIN, FLUX, FAST, BZ, BX>(a, xcd_tile(blockIdx.x, a.nb));
+    advect_tile_any<S, HIN, FLUX, FAST, BZ, BX>(a, xcd_tile(blockIdx.x, a.nb));
 }
 
 // Ensemble form: the tiles of n grids of any sizes in one launch.  Workgroup b (XCD-remapped)
@@ -447,7 +674,7 @@ __global__ __launch_bounds__(NT) void k_advect_batch(const AdvArgs* __restrict__
         else hi = mid - 1;
     }
     const AdvArgs a = items[lo];
-    advect_tile<S, false, FLUX, FAST, BZ, BX>(a, b - first[lo]);
+    advect_tile_any<S, false, FLUX, FAST, BZ, BX>(a, b - first[lo]);
 }
 
 template <int S, bool HIN, bool FLUX, bool FAST, int V>
@@ -729,18 +956,32 @@ __device__ __forceinline__ void diffus_tile(const DifArgs& a, int t) {
     constexpr int PZ = TZ + 2, PX = TX + 2;
     __shared__ double fs[PX * PZ];
     __shared__ double ks[PX * PZ];
+    constexpr int NF = (PX * PZ + NT - 1) / NT;
     const int z0 = (t % a.nbz) * TZ;
     const int x0 = (t / a.nbz) * TX;
     const int Nz = a.Nz, Nx = a.Nx;
+    const bool zper = a.zbc == XC_PERIODIC, xper = a.xbc == XC_PERIODIC;
+    // every load of the tile is issued before the first LDS store (see advect_tile)
+    double rf[NF], rk[NF], rk2[NF];
+    const bool has_k2 = a.k2 != nullptr;
 #pragma unroll
-    for (int k = threadIdx.x; k < PX * PZ; k += NT) {
+    for (int i = 0; i < NF; ++i) {
+        const int k = min((int)threadIdx.x + i * NT, PX * PZ - 1);
         const int lz = k % PZ, lx = k / PZ;
-        int gz = z0 + lz - 1, gx = x0 + lx - 1;
-        gz = gz < Nz ? gz : Nz;  // rows/columns past the grid end map like the one after the last
-        gx = gx < Nx ? gx : Nx;
-        const int64_t hz = map_cell(gz, Nz, a.zbc), hx = map_cell(gx, Nx, a.xbc);
-        fs[k] = a.f[hz + hx * a.ld_f];
-        ks[k] = a.k2 ? a.k[hz + hx * a.ld_k] * a.k2[hz + hx * a.ld_k2] : a.k[hz + hx * a.ld_k];
+        // rows/columns past the grid end map like the one after the last
+        const int gz = min(z0 + lz - 1, Nz), gx = min(x0 + lx - 1, Nx);
+        const int64_t hz = map_cell_sel(gz, Nz, zper), hx = map_cell_sel(gx, Nx, xper);
+        rf[i] = a.f[hz + hx * a.ld_f];
+        rk[i] = a.k[hz + hx * a.ld_k];
+        if (has_k2) rk2[i] = a.k2[hz + hx * a.ld_k2];
+    }
+#pragma unroll
+    for (int i = 0; i < NF; ++i) {
+        const int k = threadIdx.x + i * NT;
+        if (k < PX * PZ) {
+            fs[k] = rf[i];
+            ks[k] = has_k2 ? rk[i] * rk2[i] : rk[i];
+        }
     }
     lds_barrier();
     const int qx_top = (a.zbc == XC_PERIODIC && Nz > 1) ? Nz - 1 : 0;
