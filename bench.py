@@ -44,6 +44,10 @@ METRIC = "steps/s at 512² learned vs plain Stokes @1e-8 resid; stencil % HBM pe
 HBM_PEAK_GBS = 8000.0
 N_STENCIL = 2048
 NFIELDS = 4
+# advection tile shapes (rows x columns); (64, -8) is the persistent software-pipelined kernel
+# the library writes a launch's outputs with non-temporal stores above this size
+NT_STORE_BYTES = 64 << 20
+TILE_SHAPES = {"64x16": (64, 16), "128x8": (128, 8), "64x8": (64, 8), "32x32": (32, 32), "64x8p": (64, -8)}
 # HBM bytes per launch of the WENO5 stencil at 2048^2 from rocprofv3 FETCH_SIZE + WRITE_SIZE (null until measured)
 TRAFFIC_WENO5_2048 = None
 
@@ -131,43 +135,31 @@ def step_bench(args, learned=False):
 
 
 def stencil_bench(N, steps, warmup):
-    """C2 stencil microbench: per-kernel HIP-event timings on the launch stream."""
+    """C2 stencil microbench: per-kernel HIP-event timings on the launch stream.  Each of the
+    NFIELDS rotating sets holds its own field, velocities, diffusivity and outputs (about 300 MB a
+    set at 2048^2), so no launch finds its inputs in the 256 MB last-level cache left by the one
+    before: the timings are HBM-bound.  The advection tile shape is chosen the same way: each
+    candidate is timed over the rotating sets and the fastest is pinned."""
     import torch
     import xcore_amd as xc
     fields_h, u_h, w_h, k_h, h = make_inputs(N)
-    u = xc.to_device(u_h)
-    w = xc.to_device(w_h)
-    k = xc.to_device(k_h)
     sets = []
     for f in fields_h:
-        sets.append(dict(f=xc.to_device(f), adv=xc.xcore_zeros(N, N), qx=xc.xcore_zeros(N + 2, N + 1),
-                         qz=xc.xcore_zeros(N + 1, N + 2), dff=xc.xcore_zeros(N, N),
-                         dqx=xc.xcore_zeros(N + 2, N + 1), dqz=xc.xcore_zeros(N + 1, N + 2)))
+        sets.append(dict(f=xc.to_device(f), u=xc.to_device(u_h), w=xc.to_device(w_h), k=xc.to_device(k_h),
+                         adv=xc.xcore_zeros(N, N), qx=xc.xcore_zeros(N + 2, N + 1), qz=xc.xcore_zeros(N + 1, N + 2),
+                         dff=xc.xcore_zeros(N, N), dqx=xc.xcore_zeros(N + 2, N + 1),
+                         dqz=xc.xcore_zeros(N + 1, N + 2)))
     torch.cuda.synchronize()
 
     def launch(kind, s):
         if kind == "diffus":
-            xc.diffus_centered_with_flux_(s["dff"], s["dqx"], s["dqz"], s["f"], k, h)
+            xc.diffus_centered_with_flux_(s["dff"], s["dqx"], s["dqz"], s["f"], s["k"], h)
         else:
-            xc.advect_centered_with_flux_(s["adv"], s["qx"], s["qz"], s["f"], u, w, h, kind)
+            xc.advect_centered_with_flux_(s["adv"], s["qx"], s["qz"], s["f"], s["u"], s["w"], h, kind)
 
-    with xc.autotune():  # the first call per scheme times the tile shapes and keeps the fastest
-        launch("weno5", sets[0])
-        launch("quick", sets[0])
-    tiles = {k: xc.advect_tile(k, N, N)[:3] for k in ("weno5", "quick")}
-    for _ in range(warmup):
-        for s in sets:
-            for kind in ("weno5", "quick", "diffus"):
-                launch(kind, s)
-    torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
-    reps = max(8, 4 * steps)
-    adv_bytes = 8 * (N * N + N * (N + 1) + (N + 1) * N + N * N + (N + 2) * (N + 1) + (N + 1) * (N + 2))
-    dif_bytes = 8 * (N * N + N * N + N * N + (N + 2) * (N + 1) + (N + 1) * (N + 2))
-    out = {}
-    for kind, nbytes, name in (("weno5", adv_bytes, "weno5_advect_with_flux"),
-                               ("quick", adv_bytes, "quick_advect_with_flux"),
-                               ("diffus", dif_bytes, "diffus_with_flux")):
+
+    def timed(kind, reps):
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record(stream)
@@ -175,19 +167,70 @@ def stencil_bench(N, steps, warmup):
             launch(kind, sets[r % NFIELDS])
         e1.record(stream)
         e1.synchronize()
-        us = 1e3 * e0.elapsed_time(e1) / reps
+        return 1e3 * e0.elapsed_time(e1) / reps
+
+    tiles = {}
+    for kind in ("weno5", "quick"):
+        cand = {}
+        for name, (tz, tx) in TILE_SHAPES.items():
+            xc.set_advect_tile(kind, N, N, tz, tx)
+            for s in sets:
+                launch(kind, s)
+            cand[name] = round(timed(kind, 2 * NFIELDS), 2)
+        best = min(cand, key=cand.get)
+        xc.set_advect_tile(kind, N, N, *TILE_SHAPES[best])
+        tiles[kind] = (best, cand)
+    for _ in range(warmup):
+        for s in sets:
+            for kind in ("weno5", "quick", "diffus"):
+                launch(kind, s)
+    torch.cuda.synchronize()
+    reps = max(8, 4 * steps)
+    adv_bytes = 8 * (N * N + N * (N + 1) + (N + 1) * N + N * N + (N + 2) * (N + 1) + (N + 1) * (N + 2))
+    dif_bytes = 8 * (N * N + N * N + N * N + (N + 2) * (N + 1) + (N + 1) * (N + 2))
+    out = {}
+    for kind, nbytes, name in (("weno5", adv_bytes, "weno5_advect_with_flux"),
+                               ("quick", adv_bytes, "quick_advect_with_flux"),
+                               ("diffus", dif_bytes, "diffus_with_flux")):
+        us = timed(kind, reps)
         gbs = nbytes / (us * 1e-6) / 1e9
         out[name] = {"avg_launch_us": round(us, 2), "bytes_per_launch": nbytes, "GB/s": round(gbs, 1),
                      "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
     for k in ("weno5", "quick"):
-        tz, tx, cand = tiles[k]
-        out[f"{k}_advect_with_flux"]["tile"] = f"{tz}x{abs(tx)}" + (" pipelined" if tx < 0 else "")
+        best, cand = tiles[k]
+        out[f"{k}_advect_with_flux"]["tile"] = best
         out[f"{k}_advect_with_flux"]["tile_candidates_us"] = cand
+    # the same traffic without the arithmetic (xc.diag_stream_, same arrays, same store kind): the
+    # bandwidth HBM sustains for this mix of streams, which bounds what the stencil can reach
+    floors = {}
+    for kind, ins, outs, nbytes in (("advect", ("f", "u", "w"), ("adv", "qx", "qz"), adv_bytes),
+                                    ("diffus", ("f", "k"), ("dff", "dqx", "dqz"), dif_bytes)):
+        nts = 8 * (N * N + 2 * (N + 2) * (N + 1)) > NT_STORE_BYTES  # the outputs of one launch
+        for s in sets:
+            xc.diag_stream_([s[a] for a in ins], [s[a] for a in outs], nts)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for r in range(reps):
+            s = sets[r % NFIELDS]
+            xc.diag_stream_([s[a] for a in ins], [s[a] for a in outs], nts)
+        e1.record(stream)
+        e1.synchronize()
+        floors[kind] = 1e3 * e0.elapsed_time(e1) / reps
+    for name, kind in (("weno5_advect_with_flux", "advect"), ("quick_advect_with_flux", "advect"),
+                       ("diffus_with_flux", "diffus")):
+        o = out[name]
+        o["stream_floor_us"] = round(floors[kind], 2)
+        o["frac_of_stream_floor"] = round(floors[kind] / o["avg_launch_us"], 4)
     wk = out["weno5_advect_with_flux"]
     roofline = {"bound": "hbm", "achieved": wk["GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": wk["hbm_frac"], "traffic": TRAFFIC_WENO5_2048,
-                "kernel": "k_advect<WENO5> (advect_centered_with_flux!, %d^2, %d rotating field sets)" % (N, NFIELDS),
-                "bytes_per_launch": adv_bytes, "avg_launch_us": wk["avg_launch_us"], "launches_timed": reps}
+                "kernel": "k_advect<WENO5> (advect_centered_with_flux!, %d^2, %d rotating sets of field, velocities "
+                          "and outputs)" % (N, NFIELDS),
+                "bytes_per_launch": adv_bytes, "avg_launch_us": wk["avg_launch_us"], "launches_timed": reps,
+                "same_traffic_stream_us": wk["stream_floor_us"],
+                "same_traffic_stream_GBs": round(adv_bytes / (wk["stream_floor_us"] * 1e-6) / 1e9, 1),
+                "frac_of_same_traffic_stream": wk["frac_of_stream_floor"]}
     return roofline, out
 
 

@@ -165,6 +165,13 @@ int xc_advect_tile(xc_ctx* ctx, int scheme, int with_flux, int64_t Nz, int64_t N
  * 32x32, or 64x-8 for the pipelined kernel (not for tvdim) */
 int xc_advect_set_tile(xc_ctx* ctx, int scheme, int with_flux, int64_t Nz, int64_t Nx, int tz, int tx);
 
+/* Diagnostic, no counterpart in the reference: the memory traffic of a stencil launch without its
+ * arithmetic.  out[k](i + (k > 0), j) = sum_r in[r](i, j) for every cell (i, j) of out[0]'s shape
+ * (0..3 inputs, 1..3 outputs; outputs k >= 1 one row down, as the padded face fluxes), written with
+ * non-temporal stores when nts != 0 -- the bandwidth HBM sustains for that mix of streams, the
+ * achievable bound a stencil kernel with the same traffic is measured against. */
+int xc_diag_stream(xc_ctx* ctx, int nr, const xc_mat* in, int nw, const xc_mat* out, int nts);
+
 /* ---- field updates -------------------------------------------------------- */
 /* phsevo.jl:164-214 phsevo!(phase, fluid, grid, par, scales; ADVN, xBC, zBC, a1..b3, dt, alpha);
  * coef = {a1, a2, a3, b1, b2, b3} from time_coefs (timing.jl:18-36) */

@@ -197,8 +197,20 @@ def test_every_tile_shape_bitwise(xc, orc, scheme):
     adv_o, qx_o, qz_o = orc.zeros(Nz, Nx), orc.zeros(Nz + 2, Nx + 1), orc.zeros(Nz + 1, Nx + 2)
     orc.advect_centered(adv_o, f, u, w, h, scheme, xbc="periodic", zbc="closed", qx=qx_o, qz=qz_o)
     fd, ud, wd = xc.to_device(f), xc.to_device(u), xc.to_device(w)
+    tiles = [(64, 16), (128, 8), (64, 8), (32, 32)] + ([(64, -8)] if scheme != "tvdim" else [])
+    if scheme == "weno5":  # the fast form: the same bits in every tiling too
+        first = None
+        for tz, tx in tiles:
+            xc.set_advect_tile(scheme, Nz, Nx, tz, tx)
+            adv, qx, qz = xc.xcore_zeros(Nz, Nx), xc.xcore_zeros(Nz + 2, Nx + 1), xc.xcore_zeros(Nz + 1, Nx + 2)
+            xc.advect_centered_with_flux_(adv, qx, qz, fd, ud, wd, h, scheme)
+            got = [xc.to_host(t) for t in (adv, qx, qz)]
+            assert _rel(got[0], adv_o) <= 1e-12
+            if first is None:
+                first = got
+            for g, g0 in zip(got, first):
+                assert np.array_equal(g, g0), ("fast", tz, tx)
     with xc.exact_arithmetic():
-        tiles = [(64, 16), (128, 8), (64, 8), (32, 32)] + ([(64, -8)] if scheme != "tvdim" else [])
         for tz, tx in tiles:
             print(f"tile {tz}x{tx} {scheme}", flush=True)
             xc.set_advect_tile(scheme, Nz, Nx, tz, tx)
@@ -218,3 +230,42 @@ def test_every_tile_shape_bitwise(xc, orc, scheme):
         tz, tx, cand, tuned = xc.advect_tile(scheme, Nz + 8, Nx, with_flux=False)
     print("tuned", scheme, tz, tx, cand, flush=True)
     assert tuned and (tz, tx) in ((64, 16), (128, 8), (64, 8), (32, 32), (64, -8))
+
+
+@pytest.mark.parametrize("scheme", ["weno5", "quick", "tvdim"])
+def test_streaming_store_path(xc, scheme):
+    # outputs above 64 MiB per launch are written with non-temporal stores; a one-member batch launch
+    # (plain stores, 64 x 8 tiles) gives the reference bits for every tile shape
+    Nz, Nx = 2048, 1500
+    h = 1.0 / Nx
+    print(f"streaming stores {scheme} {Nz}x{Nx}", flush=True)
+    f, u, w = _inputs(Nz, Nx, 91)
+    fd, ud, wd = xc.to_device(f), xc.to_device(u), xc.to_device(w)
+    ref = dict(adv=xc.xcore_zeros(Nz, Nx), qx=xc.xcore_zeros(Nz + 2, Nx + 1), qz=xc.xcore_zeros(Nz + 1, Nx + 2),
+               f=fd, u=ud, w=wd, h=h)
+    xc.advect_centered_batch_([ref], scheme)
+    want = [xc.to_host(ref[k]) for k in ("adv", "qx", "qz")]
+    tiles = [(64, 16), (128, 8), (64, 8), (32, 32)]
+    for tz, tx in tiles:
+        xc.set_advect_tile(scheme, Nz, Nx, tz, tx)
+        adv, qx, qz = xc.xcore_zeros(Nz, Nx), xc.xcore_zeros(Nz + 2, Nx + 1), xc.xcore_zeros(Nz + 1, Nx + 2)
+        xc.advect_centered_with_flux_(adv, qx, qz, fd, ud, wd, h, scheme)
+        torch.cuda.synchronize()
+        for got, wnt in zip((adv, qx, qz), want):
+            assert np.array_equal(xc.to_host(got), wnt), (scheme, tz, tx)
+
+
+def test_streaming_store_path_diffus(xc):
+    Nz, Nx = 2048, 1500
+    h = 1.0 / Nx
+    rng = np.random.default_rng(92)
+    f = xc.to_device(np.asfortranarray(rng.standard_normal((Nz, Nx))))
+    k = xc.to_device(np.asfortranarray(np.exp(rng.standard_normal((Nz, Nx)))))
+    ref = dict(dff=xc.xcore_zeros(Nz, Nx), qx=xc.xcore_zeros(Nz + 2, Nx + 1), qz=xc.xcore_zeros(Nz + 1, Nx + 2),
+               f=f, k=k, h=h)
+    xc.diffus_centered_batch_([ref])
+    dff, qx, qz = xc.xcore_zeros(Nz, Nx), xc.xcore_zeros(Nz + 2, Nx + 1), xc.xcore_zeros(Nz + 1, Nx + 2)
+    xc.diffus_centered_with_flux_(dff, qx, qz, f, k, h)
+    torch.cuda.synchronize()
+    for got, key in ((dff, "dff"), (qx, "qx"), (qz, "qz")):
+        assert np.array_equal(xc.to_host(got), xc.to_host(ref[key])), key

@@ -1,0 +1,79 @@
+// Diagnostic streaming kernel (no reference counterpart): the memory traffic of a stencil launch
+// without its arithmetic, to measure the bandwidth HBM sustains for that mix of streams.  Each of
+// nw outputs receives, at every cell (i, j) of out[0]'s shape, the sum of the nr inputs there;
+// outputs k >= 1 are written one row down (the padded flux arrays' layout).  Plain or
+// non-temporal stores.
+#include "xc_internal.h"
+
+namespace {
+
+constexpr int MAXS = 3;
+struct StreamArgs {
+    const double* in[MAXS]; int64_t ld_in[MAXS];
+    double* out[MAXS]; int64_t ld_out[MAXS];
+    int Nz, Nx, nbz;
+};
+
+template <int NR, int NW, bool NTS>
+__global__ __launch_bounds__(256) void k_stream(StreamArgs a) {
+    const int t = blockIdx.x;
+    const int iz = (t % a.nbz) * 64 + (threadIdx.x & 63);
+    const int ix = (t / a.nbz) * 4 + (threadIdx.x >> 6);
+    if (iz >= a.Nz || ix >= a.Nx) return;
+    double s = 0.0;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) s += a.in[r][iz + (int64_t)ix * a.ld_in[r]];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        double* p = &a.out[w][iz + (w ? 1 : 0) + (int64_t)ix * a.ld_out[w]];
+        if constexpr (NTS) __builtin_nontemporal_store(s, p);
+        else *p = s;
+    }
+}
+
+template <int NR, int NW>
+void launch_nr_nw(const StreamArgs& a, int nts, hipStream_t st) {
+    const dim3 grid((unsigned)(a.nbz * ((a.Nx + 3) / 4)));
+    if (nts) hipLaunchKernelGGL((k_stream<NR, NW, true>), grid, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_stream<NR, NW, false>), grid, dim3(256), 0, st, a);
+}
+
+}  // namespace
+
+extern "C" int xc_diag_stream(xc_ctx* ctx, int nr, const xc_mat* in, int nw, const xc_mat* out, int nts) {
+    XC_CHECK_CTX(ctx);
+    if (nr < 0 || nr > MAXS || nw < 1 || nw > MAXS || (nr && !in) || !out)
+        return xc_fail(ctx, XC_EINVAL_ARG, "diag_stream: 0..3 inputs and 1..3 outputs");
+    const int64_t Nz = out[0].rows, Nx = out[0].cols;
+    if (!xc_mat_ok(out[0]) || Nz < 1 || Nx < 1 || Nz > (1 << 30) || Nx > (1 << 30))
+        return xc_fail(ctx, XC_EINVAL_SHAPE, "diag_stream: out[0] must be a non-empty matrix");
+    StreamArgs a{};
+    for (int r = 0; r < nr; ++r) {
+        if (!xc_mat_ok(in[r]) || in[r].rows < Nz || in[r].cols < Nx)
+            return xc_fail(ctx, XC_EINVAL_SHAPE, "diag_stream: input %d smaller than out[0]", r);
+        a.in[r] = in[r].p; a.ld_in[r] = in[r].ld;
+    }
+    for (int w = 0; w < nw; ++w) {
+        if (!xc_mat_ok(out[w]) || out[w].rows < Nz + (w ? 1 : 0) || out[w].cols < Nx)
+            return xc_fail(ctx, XC_EINVAL_SHAPE, "diag_stream: output %d too small (outputs 1.. need a row more)", w);
+        a.out[w] = out[w].p; a.ld_out[w] = out[w].ld;
+    }
+    a.Nz = (int)Nz; a.Nx = (int)Nx; a.nbz = (int)((Nz + 63) / 64);
+    if ((int64_t)a.nbz * ((Nx + 3) / 4) > (1ll << 31) - 1) return xc_fail(ctx, XC_EINVAL_SHAPE, "diag_stream: grid too large");
+    hipStream_t st = ctx->stream;
+    switch (nr * 4 + nw) {
+        case 1: launch_nr_nw<0, 1>(a, nts, st); break;
+        case 2: launch_nr_nw<0, 2>(a, nts, st); break;
+        case 3: launch_nr_nw<0, 3>(a, nts, st); break;
+        case 5: launch_nr_nw<1, 1>(a, nts, st); break;
+        case 6: launch_nr_nw<1, 2>(a, nts, st); break;
+        case 7: launch_nr_nw<1, 3>(a, nts, st); break;
+        case 9: launch_nr_nw<2, 1>(a, nts, st); break;
+        case 10: launch_nr_nw<2, 2>(a, nts, st); break;
+        case 11: launch_nr_nw<2, 3>(a, nts, st); break;
+        case 13: launch_nr_nw<3, 1>(a, nts, st); break;
+        case 14: launch_nr_nw<3, 2>(a, nts, st); break;
+        default: launch_nr_nw<3, 3>(a, nts, st); break;
+    }
+    return xc_finish(ctx, "diag_stream");
+}

@@ -186,6 +186,20 @@ __device__ __forceinline__ double lane_next(double x) {
     const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x130, 0xf, 0xf, false);
     return __hiloint2double(hi, lo);
 }
+// Output store; NTS: non-temporal (the line is streamed past the caches instead of being kept for a
+// later reader).  Used when a launch's outputs are much larger than the last-level cache.
+template <bool NTS>
+__device__ __forceinline__ void put(double* p, double v) {
+    if constexpr (NTS) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+// Outputs of one launch above this many bytes are written non-temporally: a quarter of the 256 MB
+// last-level cache, beyond which a later kernel would find little of them there anyway.
+constexpr int64_t NT_STORE_BYTES = 64ll << 20;
+// bytes a stencil launch writes: the cell output, and the padded face fluxes when requested
+inline int64_t out_bytes(int64_t Nz, int64_t Nx, bool flux) {
+    return 8 * (Nz * Nx + (flux ? (Nz + 2) * (Nx + 1) + (Nz + 1) * (Nx + 2) : 0));
+}
 // map_cell for a wave-uniform periodic flag, as selects (no branch between a tile's loads)
 __device__ __forceinline__ int map_cell_sel(int i, int n, bool per) {
     const int lo = per ? i + n : 0, hi = per ? i - n : n - 1;
@@ -208,6 +222,7 @@ struct AdvArgs {
     int Nz, Nx, nbz, nb;
     double invh;
     int xbc, zbc;
+    int nts;  // host side: launch the non-temporal-store form (outputs larger than NT_STORE_BYTES)
 };
 
 // HIN: f is a halo'd buffer (advect!); otherwise f is (Nz, Nx) and the BCs are
@@ -216,7 +231,7 @@ struct AdvArgs {
 // Every global load of the tile (the f halo tile and the face velocities) is issued back to back
 // into registers before the first LDS store, so a wave keeps all of them in flight at once; indexes
 // past the grid are clamped to a valid element whose value is never used.
-template <int S, bool HIN, bool FLUX, bool FAST, int BZ, int BX>
+template <int S, bool HIN, bool FLUX, bool FAST, int BZ, int BX, bool NTS = false>
 __device__ __forceinline__ void advect_tile(const AdvArgs& a, int t) {
     constexpr int G = halo_of(S);
     constexpr int PZ = BZ + 2 * G;
@@ -313,9 +328,9 @@ __device__ __forceinline__ void advect_tile(const AdvArgs& a, int t) {
                 if constexpr (FLUX) {
                     if (fx < BX || F == Nx) {
                         double* col = a.qx + (int64_t)F * a.ld_qx;
-                        col[iz + 1] = q;
-                        if (iz == qx_top) col[0] = q;
-                        if (iz == qx_bot) col[Nz + 1] = q;
+                        put<NTS>(&col[iz + 1], q);
+                        if (iz == qx_top) put<NTS>(&col[0], q);
+                        if (iz == qx_bot) put<NTS>(&col[Nz + 1], q);
                     }
                 }
             }
@@ -334,9 +349,9 @@ __device__ __forceinline__ void advect_tile(const AdvArgs& a, int t) {
                 q = face_flux<S, FAST>(&fs[(lx + G) * PZ + fz + G], 1, rw[i][0], rw[i][VI], rw[i][NV - 1]);
                 if constexpr (FLUX) {
                     if (fz < BZ || F == Nz) {
-                        a.qz[F + (int64_t)(ix + 1) * a.ld_qz] = q;
-                        if (ix == qz_lft) a.qz[F] = q;
-                        if (ix == qz_rgt) a.qz[F + (int64_t)(Nx + 1) * a.ld_qz] = q;
+                        put<NTS>(&a.qz[F + (int64_t)(ix + 1) * a.ld_qz], q);
+                        if (ix == qz_lft) put<NTS>(&a.qz[F], q);
+                        if (ix == qz_rgt) put<NTS>(&a.qz[F + (int64_t)(Nx + 1) * a.ld_qz], q);
                     }
                 }
             }
@@ -353,7 +368,7 @@ __device__ __forceinline__ void advect_tile(const AdvArgs& a, int t) {
         if (iz < Nz && ix < Nx) {
             const double qxr = qxs[(lx + 1) * BZ + lz], qxl = qxs[lx * BZ + lz];
             const double qzb = qzs[lx * (BZ + 1) + lz + 1], qzt = qzs[lx * (BZ + 1) + lz];
-            a.adv[iz + (int64_t)ix * a.ld_adv] = (qxr - qxl + qzb - qzt) * a.invh;
+            put<NTS>(&a.adv[iz + (int64_t)ix * a.ld_adv], (qxr - qxl + qzb - qzt) * a.invh);
         }
     }
 }
@@ -363,7 +378,7 @@ __device__ __forceinline__ void advect_tile(const AdvArgs& a, int t) {
 // column addresses are wave-uniform (scalar registers), only the row offset is per lane, and every
 // LDS access of a wave touches 64 consecutive doubles.  Same loads, face arithmetic and stores as
 // advect_tile, hence bitwise the same results.
-template <int S, bool HIN, bool FLUX, bool FAST, int BX>
+template <int S, bool HIN, bool FLUX, bool FAST, int BX, bool NTS = false>
 __device__ __forceinline__ void advect_tile_lr(const AdvArgs& a, int t) {
     constexpr int BZ = 64;
     constexpr int G = halo_of(S);
@@ -481,9 +496,9 @@ __device__ __forceinline__ void advect_tile_lr(const AdvArgs& a, int t) {
                 if constexpr (FLUX) {
                     if (fx < BX || F == Nx) {
                         double* col = a.qx + (int64_t)F * a.ld_qx;
-                        col[iz + 1] = q;
-                        if (iz == qx_top) col[0] = q;
-                        if (iz == qx_bot) col[Nz + 1] = q;
+                        put<NTS>(&col[iz + 1], q);
+                        if (iz == qx_top) put<NTS>(&col[0], q);
+                        if (iz == qx_bot) put<NTS>(&col[Nz + 1], q);
                     }
                 }
             }
@@ -498,9 +513,9 @@ __device__ __forceinline__ void advect_tile_lr(const AdvArgs& a, int t) {
             q = face_flux<S, FAST>(&fs[(lx + G) * PZ + fz + G], 1, r[0], r[VI], r[NV - 1]);
             if constexpr (FLUX) {
                 if (fz < BZ || F == Nz) {
-                    a.qz[F + (int64_t)(ix + 1) * a.ld_qz] = q;
-                    if (ix == qz_lft) a.qz[F] = q;
-                    if (ix == qz_rgt) a.qz[F + (int64_t)(Nx + 1) * a.ld_qz] = q;
+                    put<NTS>(&a.qz[F + (int64_t)(ix + 1) * a.ld_qz], q);
+                    if (ix == qz_lft) put<NTS>(&a.qz[F], q);
+                    if (ix == qz_rgt) put<NTS>(&a.qz[F + (int64_t)(Nx + 1) * a.ld_qz], q);
                 }
             }
         }
@@ -525,16 +540,16 @@ __device__ __forceinline__ void advect_tile_lr(const AdvArgs& a, int t) {
         if (lx < BX && ix < Nx && iz < Nz) {
             const double qxr = qxs[(lx + 1) * BZ + l], qxl = qxs[lx * BZ + l];
             const double qzb = l == 63 ? qzc[lx] : below, qzt = zq[j];
-            a.adv[iz + (int64_t)ix * a.ld_adv] = (qxr - qxl + qzb - qzt) * a.invh;
+            put<NTS>(&a.adv[iz + (int64_t)ix * a.ld_adv], (qxr - qxl + qzb - qzt) * a.invh);
         }
     }
 }
 
 // the tile body a kernel uses: lane-row form for 64-row tiles, the generic form otherwise
-template <int S, bool HIN, bool FLUX, bool FAST, int BZ, int BX>
+template <int S, bool HIN, bool FLUX, bool FAST, int BZ, int BX, bool NTS = false>
 __device__ __forceinline__ void advect_tile_any(const AdvArgs& a, int t) {
-    if constexpr (BZ == 64) advect_tile_lr<S, HIN, FLUX, FAST, BX>(a, t);
-    else advect_tile<S, HIN, FLUX, FAST, BZ, BX>(a, t);
+    if constexpr (BZ == 64) advect_tile_lr<S, HIN, FLUX, FAST, BX, NTS>(a, t);
+    else advect_tile<S, HIN, FLUX, FAST, BZ, BX, NTS>(a, t);
 }
 
 // Persistent, software-pipelined form (centred input, schemes without padded velocities): each
@@ -544,6 +559,7 @@ __device__ __forceinline__ void advect_tile_any(const AdvArgs& a, int t) {
 template <int S, bool FLUX, bool FAST, int BZ, int BX>
 __global__ __launch_bounds__(NT) void k_advect_pipe(AdvArgs a) {
     static_assert(S != XC_TVDIM, "tvdim needs its padded velocities");
+    constexpr bool NTS = false;
     constexpr int G = halo_of(S);
     constexpr int PZ = BZ + 2 * G;
     constexpr int PX = BX + 2 * G;
@@ -616,9 +632,9 @@ __global__ __launch_bounds__(NT) void k_advect_pipe(AdvArgs a) {
                 if constexpr (FLUX) {
                     if (fx < BX || F == Nx) {
                         double* col = a.qx + (int64_t)F * a.ld_qx;
-                        col[iz + 1] = q;
-                        if (iz == qx_top) col[0] = q;
-                        if (iz == qx_bot) col[Nz + 1] = q;
+                        put<NTS>(&col[iz + 1], q);
+                        if (iz == qx_top) put<NTS>(&col[0], q);
+                        if (iz == qx_bot) put<NTS>(&col[Nz + 1], q);
                     }
                 }
             }
@@ -633,9 +649,9 @@ __global__ __launch_bounds__(NT) void k_advect_pipe(AdvArgs a) {
                 q = face_flux<S, FAST>(&fs[(lx + G) * PZ + fz + G], 1, 0.0, ws[k], 0.0);
                 if constexpr (FLUX) {
                     if (fz < BZ || F == Nz) {
-                        a.qz[F + (int64_t)(ix + 1) * a.ld_qz] = q;
-                        if (ix == qz_lft) a.qz[F] = q;
-                        if (ix == qz_rgt) a.qz[F + (int64_t)(Nx + 1) * a.ld_qz] = q;
+                        put<NTS>(&a.qz[F + (int64_t)(ix + 1) * a.ld_qz], q);
+                        if (ix == qz_lft) put<NTS>(&a.qz[F], q);
+                        if (ix == qz_rgt) put<NTS>(&a.qz[F + (int64_t)(Nx + 1) * a.ld_qz], q);
                     }
                 }
             }
@@ -649,16 +665,16 @@ __global__ __launch_bounds__(NT) void k_advect_pipe(AdvArgs a) {
             if (iz < Nz && ix < Nx) {
                 const double qxr = qxs[(lx + 1) * BZ + lz], qxl = qxs[lx * BZ + lz];
                 const double qzb = qzs[lx * (BZ + 1) + lz + 1], qzt = qzs[lx * (BZ + 1) + lz];
-                a.adv[iz + (int64_t)ix * a.ld_adv] = (qxr - qxl + qzb - qzt) * a.invh;
+                put<NTS>(&a.adv[iz + (int64_t)ix * a.ld_adv], (qxr - qxl + qzb - qzt) * a.invh);
             }
         }
         lds_barrier();  // fs/us/ws/qxs/qzs are rewritten for the next tile
     }
 }
 
-template <int S, bool HIN, bool FLUX, bool FAST, int BZ, int BX>
+template <int S, bool HIN, bool FLUX, bool FAST, int BZ, int BX, bool NTS>
 __global__ __launch_bounds__(NT) void k_advect(AdvArgs a) {
-    advect_tile_any<S, HIN, FLUX, FAST, BZ, BX>(a, xcd_tile(blockIdx.x, a.nb));
+    advect_tile_any<S, HIN, FLUX, FAST, BZ, BX, NTS>(a, xcd_tile(blockIdx.x, a.nb));
 }
 
 // Ensemble form: the tiles of n grids of any sizes in one launch.  Workgroup b (XCD-remapped)
@@ -683,7 +699,8 @@ hipError_t launch_advect_v(const AdvArgs& a0, hipStream_t s) {
     AdvArgs a = a0;
     a.nbz = (a.Nz + BZ - 1) / BZ;
     a.nb = a.nbz * ((a.Nx + BX - 1) / BX);
-    hipLaunchKernelGGL((k_advect<S, HIN, FLUX, FAST, BZ, BX>), dim3(a.nb), dim3(NT), 0, s, a);
+    if (!HIN && a.nts) hipLaunchKernelGGL((k_advect<S, HIN, FLUX, FAST, BZ, BX, !HIN>), dim3(a.nb), dim3(NT), 0, s, a);
+    else hipLaunchKernelGGL((k_advect<S, HIN, FLUX, FAST, BZ, BX, false>), dim3(a.nb), dim3(NT), 0, s, a);
     return hipGetLastError();
 }
 
@@ -858,6 +875,7 @@ int advect_common(xc_ctx* ctx, int scheme, bool hin, xc_mat adv, const xc_mat* q
     a.Nz = (int)Nz; a.Nx = (int)Nx;
     a.invh = 1.0 / h;
     a.xbc = xbc; a.zbc = zbc;
+    a.nts = out_bytes(a.Nz, a.Nx, flux) > NT_STORE_BYTES;
     hipError_t e;
     const bool ex = (ctx->flags & XC_CTX_EXACT) != 0;
     int v = 0;
@@ -890,10 +908,12 @@ struct DifArgs {
     int Nz, Nx, halo, nbz, nb;
     double invh, invh2;
     int xbc, zbc;
+    int nts;  // host side: launch the non-temporal-store form (outputs larger than NT_STORE_BYTES)
 };
 
 template <bool HIN, bool FLUX>
 __global__ __launch_bounds__(NT) void k_diffus(DifArgs a) {
+    constexpr bool NTS = false;
     const int t = xcd_tile(blockIdx.x, a.nb);
     const int iz = (t % a.nbz) * 64 + (threadIdx.x & 63);
     const int ix = (t / a.nbz) * 4 + (threadIdx.x >> 6);
@@ -922,7 +942,7 @@ __global__ __launch_bounds__(NT) void k_diffus(DifArgs a) {
     const double flux_xl = (kc + kmx) * (fc - fmx);
     const double flux_zb = (kpz + kc) * (fpz - fc);
     const double flux_zt = (kc + kmz) * (fc - fmz);
-    a.dff[iz + (int64_t)ix * a.ld_d] = 0.5 * a.invh2 * (flux_xr - flux_xl + flux_zb - flux_zt);
+    put<NTS>(&a.dff[iz + (int64_t)ix * a.ld_d], 0.5 * a.invh2 * (flux_xr - flux_xl + flux_zb - flux_zt));
     if constexpr (FLUX) {
         // diffus.jl:77 / 89: q = -(kL + kR) * 1/2 * (fR - fL) / h
         const int qx_top = (a.zbc == XC_PERIODIC && Nz > 1) ? Nz - 1 : 0;
@@ -931,14 +951,14 @@ __global__ __launch_bounds__(NT) void k_diffus(DifArgs a) {
         const int qz_rgt = (a.xbc == XC_PERIODIC && Nx > 1) ? 0 : Nx - 1;
         auto put_qx = [&](int F, double q) {
             double* col = a.qx + (int64_t)F * a.ld_qx;
-            col[iz + 1] = q;
-            if (iz == qx_top) col[0] = q;
-            if (iz == qx_bot) col[Nz + 1] = q;
+            put<NTS>(&col[iz + 1], q);
+            if (iz == qx_top) put<NTS>(&col[0], q);
+            if (iz == qx_bot) put<NTS>(&col[Nz + 1], q);
         };
         auto put_qz = [&](int F, double q) {
-            a.qz[F + (int64_t)(ix + 1) * a.ld_qz] = q;
-            if (ix == qz_lft) a.qz[F] = q;
-            if (ix == qz_rgt) a.qz[F + (int64_t)(Nx + 1) * a.ld_qz] = q;
+            put<NTS>(&a.qz[F + (int64_t)(ix + 1) * a.ld_qz], q);
+            if (ix == qz_lft) put<NTS>(&a.qz[F], q);
+            if (ix == qz_rgt) put<NTS>(&a.qz[F + (int64_t)(Nx + 1) * a.ld_qz], q);
         };
         put_qx(ix, -(kmx + kc) * 0.5 * (fc - fmx) * a.invh);
         if (ix == Nx - 1) put_qx(Nx, -(kc + kpx) * 0.5 * (fpx - fc) * a.invh);
@@ -951,7 +971,7 @@ __global__ __launch_bounds__(NT) void k_diffus(DifArgs a) {
 // map): the 64 x 16 tile of f and of the diffusivity (k, or k .* k2 formed once per cell) plus a
 // one-cell halo is staged once, so each cell costs two global loads instead of ten.  Same arithmetic
 // and operation order as k_diffus.
-template <bool FLUX>
+template <bool FLUX, bool NTS = false>
 __device__ __forceinline__ void diffus_tile(const DifArgs& a, int t) {
     constexpr int PZ = TZ + 2, PX = TX + 2;
     __shared__ double fs[PX * PZ];
@@ -1000,18 +1020,18 @@ __device__ __forceinline__ void diffus_tile(const DifArgs& a, int t) {
         const double flux_xl = (kc + kmx) * (fc - fmx);
         const double flux_zb = (kpz + kc) * (fpz - fc);
         const double flux_zt = (kc + kmz) * (fc - fmz);
-        a.dff[iz + (int64_t)ix * a.ld_d] = 0.5 * a.invh2 * (flux_xr - flux_xl + flux_zb - flux_zt);
+        put<NTS>(&a.dff[iz + (int64_t)ix * a.ld_d], 0.5 * a.invh2 * (flux_xr - flux_xl + flux_zb - flux_zt));
         if constexpr (FLUX) {
             auto put_qx = [&](int F, double q) {
                 double* col = a.qx + (int64_t)F * a.ld_qx;
-                col[iz + 1] = q;
-                if (iz == qx_top) col[0] = q;
-                if (iz == qx_bot) col[Nz + 1] = q;
+                put<NTS>(&col[iz + 1], q);
+                if (iz == qx_top) put<NTS>(&col[0], q);
+                if (iz == qx_bot) put<NTS>(&col[Nz + 1], q);
             };
             auto put_qz = [&](int F, double q) {
-                a.qz[F + (int64_t)(ix + 1) * a.ld_qz] = q;
-                if (ix == qz_lft) a.qz[F] = q;
-                if (ix == qz_rgt) a.qz[F + (int64_t)(Nx + 1) * a.ld_qz] = q;
+                put<NTS>(&a.qz[F + (int64_t)(ix + 1) * a.ld_qz], q);
+                if (ix == qz_lft) put<NTS>(&a.qz[F], q);
+                if (ix == qz_rgt) put<NTS>(&a.qz[F + (int64_t)(Nx + 1) * a.ld_qz], q);
             };
             put_qx(ix, -(kmx + kc) * 0.5 * (fc - fmx) * a.invh);
             if (ix == Nx - 1) put_qx(Nx, -(kc + kpx) * 0.5 * (fpx - fc) * a.invh);
@@ -1020,9 +1040,9 @@ __device__ __forceinline__ void diffus_tile(const DifArgs& a, int t) {
         }
     }
 }
-template <bool FLUX>
+template <bool FLUX, bool NTS>
 __global__ __launch_bounds__(NT) void k_diffus_lds(DifArgs a) {
-    diffus_tile<FLUX>(a, xcd_tile(blockIdx.x, a.nb));
+    diffus_tile<FLUX, NTS>(a, xcd_tile(blockIdx.x, a.nb));
 }
 template <bool FLUX>
 __global__ __launch_bounds__(NT) void k_diffus_batch(const DifArgs* __restrict__ items, const int* __restrict__ first,
@@ -1118,14 +1138,17 @@ int diffus_common(xc_ctx* ctx, bool hin, int halo, xc_mat dff, const xc_mat* qx,
     a.invh = 1.0 / h;
     a.invh2 = a.invh * a.invh;  // diffus.jl:53 T(inv(h)^2)
     a.xbc = xbc; a.zbc = zbc;
+    a.nts = out_bytes(a.Nz, a.Nx, flux) > NT_STORE_BYTES;
     if (hin) {
         if (flux) hipLaunchKernelGGL((k_diffus<true, true>), dim3(a.nb), dim3(NT), 0, ctx->stream, a);
         else hipLaunchKernelGGL((k_diffus<true, false>), dim3(a.nb), dim3(NT), 0, ctx->stream, a);
     } else {
         a.nbz = (int)((Nz + TZ - 1) / TZ);
         a.nb = a.nbz * (int)((Nx + TX - 1) / TX);
-        if (flux) hipLaunchKernelGGL((k_diffus_lds<true>), dim3(a.nb), dim3(NT), 0, ctx->stream, a);
-        else hipLaunchKernelGGL((k_diffus_lds<false>), dim3(a.nb), dim3(NT), 0, ctx->stream, a);
+        if (flux && a.nts) hipLaunchKernelGGL((k_diffus_lds<true, true>), dim3(a.nb), dim3(NT), 0, ctx->stream, a);
+        else if (flux) hipLaunchKernelGGL((k_diffus_lds<true, false>), dim3(a.nb), dim3(NT), 0, ctx->stream, a);
+        else if (a.nts) hipLaunchKernelGGL((k_diffus_lds<false, true>), dim3(a.nb), dim3(NT), 0, ctx->stream, a);
+        else hipLaunchKernelGGL((k_diffus_lds<false, false>), dim3(a.nb), dim3(NT), 0, ctx->stream, a);
     }
     return xc_finish(ctx, name);
 }

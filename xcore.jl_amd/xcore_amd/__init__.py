@@ -9,7 +9,7 @@ from .arrays import as_mat, interior, to_device, to_host, xcore_zeros
 from .kernels import (advect_, advect_centered_, advect_centered_with_flux_, advect_with_flux_, context,
                       diffus_, diffus_centered_, diffus_centered_with_flux_, diffus_with_flux_,
                       embed_interior_, exact_arithmetic, autotune, advect_tile, set_advect_tile, fill_ghosts_,
-                      scheme_halo)
+                      scheme_halo, diag_stream_)
 from .state import (FluidState, Grid, NoiseState, Parameters, PhaseState, Scales, compute_scales)
 from .fields import (compute_fft_filter, noise_, normal_fill_, phsevo_, store_noise_, store_previous_, time_coefs,
                      update_, update_dt, update_phase_velocities_)

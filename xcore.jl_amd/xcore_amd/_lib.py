@@ -72,6 +72,7 @@ _PROTOS = {
     "xc_diffus_with_flux": [_P, _M, _M, _M, _M, _M, _D, _I, _I, _I],
     "xc_diffus_centered": [_P, _M, _PM, _PM, _M, _M, _D, _I, _I],
     "xc_advect_set_tile": [_P, _I, _I, ctypes.c_int64, ctypes.c_int64, _I, _I],
+    "xc_diag_stream": [_P, _I, _P, _I, _P, _I],
     "xc_advect_tile": [_P, _I, _I, ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(_I), ctypes.POINTER(_I),
                        ctypes.POINTER(ctypes.c_float)],
 }
