@@ -114,7 +114,7 @@ def step_bench(args, learned=False):
                      xc.step_opts(par, tol=1e-8, maxit=2000, fixed_sweeps=1))
     par, g, sc, ph, fl, ns, hst, state = c3_setup(args.n)
     opts = xc.step_opts(par, tol=1e-8, maxit=2000, mg_pre=args.mg_pre, mg_post=args.mg_post, mg_coarse=args.mg_coarse,
-                        mg_sub=args.mg_sub, learned=learned, lc_level=lc_level)
+                        mg_sub=args.mg_sub, mg_levels=args.mg_levels, learned=learned, lc_level=lc_level)
     for _ in range(args.warmup):
         xc.run_step_(ph, fl, ns, g, par, sc, state, opts)
     torch.cuda.synchronize()
@@ -129,6 +129,8 @@ def step_bench(args, learned=False):
     info = {"sweeps_per_step": sweeps / len(log), "fgmres_iters_per_solve": its / max(sweeps, 1),
             "worst_stokes_rel_resid": max(r["stokes_worst_rel_resid"] for r in log),
             "stokes_fallbacks": sum(r["stokes_fallbacks"] for r in log),
+            "vcycle_levels": [min(r["stokes_mg_levels"] for r in log), max(r["stokes_mg_levels"] for r in log)],
+            "inertia_ratio": [min(r["stokes_inertia_ratio"] for r in log), max(r["stokes_inertia_ratio"] for r in log)],
             "sim_time": state.time, "dt": state.dt, "step": int(state.step),
             "x_finite": bool(torch.isfinite(ph.x).all())}
     return 1e3 * elapsed / args.steps, info
@@ -346,8 +348,9 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--mg-pre", type=int, default=3)
     ap.add_argument("--mg-post", type=int, default=3)
-    ap.add_argument("--mg-coarse", type=int, default=20)
+    ap.add_argument("--mg-coarse", type=int, default=0, help="coarsest-level sweeps (0: the library's choice)")
     ap.add_argument("--mg-sub", type=int, default=1)
+    ap.add_argument("--mg-levels", type=int, default=0, help="most multigrid levels (0: down to 4 x 4)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-stencil", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")

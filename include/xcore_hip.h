@@ -212,7 +212,8 @@ typedef struct {  /* fluidmech! keywords (fluidmech.jl:71-80) and the iterative-
     int maxit;        /* FGMRES iteration cap (all restarts) */
     int restart;      /* FGMRES restart length */
     int mg_pre, mg_post, mg_coarse;  /* multigrid smoothing sweeps on the finest level (pre, post) and on
-                                        the coarsest; 0 selects the defaults 3, 3, 20 */
+                                        the coarsest; 0 selects the defaults 3, 3 and, on the coarsest,
+                                        20 at the 4 x 4 bottom or 2 where the cycle stopped early */
     double mg_omega;  /* damped-Jacobi weight of the multigrid smoother */
     int learned;      /* 1: add the learned coarse correction (needs xc_unet_set_params) for a first
                          FGMRES cycle of 6 iterations; if that has not converged the plain
@@ -220,6 +221,8 @@ typedef struct {  /* fluidmech! keywords (fluidmech.jl:71-80) and the iterative-
     int lc_level;     /* grid of the learned correction: Nz / 2^lc_level x Nx / 2^lc_level cells */
     int mg_sub;       /* pre- and post-smoothing sweeps on the levels between (0: default 1; < 0: as the
                          finest level) */
+    int mg_levels;    /* most multigrid levels, the finest included (0: chosen per solve from the inertia ratio,
+                         see xc_solve_stats) */
 } xc_fm_opts;
 
 typedef struct {  /* state.jl:131-134 BnchmData */
@@ -231,8 +234,11 @@ typedef struct {
     int converged;         /* 1 when the residual target was met */
     double rel_resid;      /* final ||RR - LL x|| / ||RR|| */
     double init_rel_resid; /* the same for the initial guess (previous W, U, P) */
-    int mg_levels;         /* multigrid levels of the velocity preconditioner */
+    int mg_levels;         /* multigrid levels the velocity preconditioner's V-cycle used */
     int fallback_used;     /* 1 if the learned coarse correction was rejected (plain multigrid used) */
+    double inertia_ratio;  /* a1 min(rho) h^2 / (dt max(eta)) on the finest level (0 without inertia): the
+                              V-cycle stops at the first level l with 4^l times it >= 6 (mg_levels == 0),
+                              where two damped-Jacobi sweeps cut the smoothest error mode 4x */
 } xc_solve_stats;
 
 /* fluidmech.jl:71-534 fluidmech!(state, grid, par; phase, bnchm_data, sds, top_cnv, bot_cnv, open_cnv,
@@ -289,6 +295,8 @@ typedef struct {
     int stokes_iters;               /* FGMRES iterations over all sweeps */
     double stokes_worst_rel_resid;  /* largest final Stokes relative residual */
     int stokes_fallbacks;           /* sweeps where the learned correction was rejected */
+    int stokes_mg_levels;           /* V-cycle depth of the last Stokes solve */
+    double stokes_inertia_ratio;    /* its inertia ratio (xc_solve_stats.inertia_ratio) */
 } xc_step_stats;
 
 /* run.jl:204-275: one outer step of run! -- time_coefs, store_previous!, store_noise!, then the

@@ -215,8 +215,8 @@ int xc_run_step(xc_ctx* ctx, xc_phase* ph, xc_fluid* fl, xc_noise* ns, const xc_
     if ((st = xc_store_previous(ctx, fl, ph))) return st;
     if ((st = xc_store_noise(ctx, ns))) return st;
     double resnorm = 1.0, resnorm0 = 1.0;
-    int iter = 0, fm_iters = 0, fm_fallbacks = 0;
-    double fm_worst = 0.0;
+    int iter = 0, fm_iters = 0, fm_fallbacks = 0, fm_levels = 0;
+    double fm_worst = 0.0, fm_inertia = 0.0;
     const int fixed = so->fixed_sweeps;
     xc_fm_opts fo = so->fm;
     fo.dt = dt;
@@ -231,6 +231,8 @@ int xc_run_step(xc_ctx* ctx, xc_phase* ph, xc_fluid* fl, xc_noise* ns, const xc_
         st = xc_fluidmech(ctx, fl, ph, g, par, &fo, nullptr, &fs);
         fm_iters += fs.iters;
         fm_fallbacks += fs.fallback_used;
+        fm_levels = fs.mg_levels;
+        fm_inertia = fs.inertia_ratio;
         if (fs.rel_resid > fm_worst) fm_worst = fs.rel_resid;
         if (st) return st;
         if ((st = xc_noise_step(ctx, ns, ph, g, par, sc, dt, iter <= 1))) return st;
@@ -253,6 +255,8 @@ int xc_run_step(xc_ctx* ctx, xc_phase* ph, xc_fluid* fl, xc_noise* ns, const xc_
         stats->stokes_iters = fm_iters;
         stats->stokes_worst_rel_resid = fm_worst;
         stats->stokes_fallbacks = fm_fallbacks;
+        stats->stokes_mg_levels = fm_levels;
+        stats->stokes_inertia_ratio = fm_inertia;
     }
     return XC_OK;
 }

@@ -314,18 +314,27 @@ __global__ __launch_bounds__(NT) void k_prolong(LevD Lf, LevD Lc, Flags F, const
 
 // The coarse end of the V-cycle (levels of at most TAIL_CELLS cells) in one
 // workgroup: the sweeps, residuals and transfers of these small levels are
-// latency-bound, so they run back to back with workgroup barriers instead of
-// one launch each, on copies of the coefficients and vectors held in LDS.
-// Same arithmetic as the per-level kernels.
+// latency-bound, so they run back to back in one launch instead of one launch
+// each, on copies of the coefficients and vectors held in LDS.  Levels of more
+// than TAIL_WAVE_PTS unknowns use the whole workgroup with a barrier after each
+// phase; smaller ones are run by wave 0 alone, wave-synchronously (no workgroup
+// barrier), while the other waves wait at the next barrier.  The coefficients
+// of the tail levels are one contiguous block of the solver's pool (carved in
+// the LDS layout), staged with every load in flight.  Same arithmetic as the
+// per-level kernels.
 constexpr int TAIL_NT = 1024;
 constexpr int TAIL_MAX = 6;
 constexpr int64_t TAIL_CELLS = 16 * 16;
+constexpr int TAIL_WAVE_PTS = 256;  // at most 4 unknowns per lane of one wave
+constexpr int TAIL_IMG_PER_THREAD = 8;
 struct TailArgs {
-    LevD d[TAIL_MAX];      // the levels' coefficient arrays in global memory
+    LevD d[TAIL_MAX];      // the levels' sizes (coefficient pointers are replaced by LDS ones)
+    const double* img;     // the tail levels' coefficients, contiguous, in the LDS layout
+    int T;                 // its length (doubles) <= TAIL_IMG_PER_THREAD * TAIL_NT
     const double* bg;      // right-hand side of the first tail level (global)
     double* xg;            // its result (global)
     int off[TAIL_MAX][9];  // LDS offsets (doubles): eta, etaco, rho, rhow, rhou, x, b, r, t
-    int nlev, pre, post, coarse;
+    int nlev, nbig, pre, post, coarse;
     double omega;
     Flags F;
 };
@@ -338,32 +347,37 @@ __device__ __forceinline__ LevD tail_lev(const TailArgs& a, double* lds, int l) 
     L.rhou = lds + a.off[l][4];
     return L;
 }
+// completes this wave's LDS accesses: after it, every lane of the wave sees the others' LDS writes
+// (a wave's LDS operations execute in order)
+__device__ __forceinline__ void wave_sync() {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+}
 __global__ __launch_bounds__(TAIL_NT) void k_mg_tail(TailArgs a) {
     extern __shared__ double lds[];
+    __shared__ unsigned swp_shared;
     const int tid = threadIdx.x;
-    // stage coefficients (and the first level's right-hand side) in LDS
-    for (int l = 0; l < a.nlev; ++l) {
-        const LevD& G = a.d[l];
-        const int Nz = G.Nz, Nx = G.Nx;
-        const int n0 = Nz * Nx, n1 = (Nz + 1) * (Nx + 1), n3 = (Nz + 1) * Nx, n4 = Nz * (Nx + 1);
-        for (int k = tid; k < n0; k += TAIL_NT) {
-            lds[a.off[l][0] + k] = G.eta[k];
-            lds[a.off[l][2] + k] = G.rho[k];
-        }
-        for (int k = tid; k < n1; k += TAIL_NT) lds[a.off[l][1] + k] = G.etaco[k];
-        for (int k = tid; k < n3; k += TAIL_NT) lds[a.off[l][3] + k] = G.rhow[k];
-        for (int k = tid; k < n4; k += TAIL_NT) lds[a.off[l][4] + k] = G.rhou[k];
-    }
+    // stage the coefficient image and the first level's right-hand side, every load in flight
     {
-        const int nv0 = (int)(a.d[0].nW() + a.d[0].nU());
-        for (int k = tid; k < nv0; k += TAIL_NT) lds[a.off[0][6] + k] = a.bg[k];
+        double v[TAIL_IMG_PER_THREAD];
+#pragma unroll
+        for (int i = 0; i < TAIL_IMG_PER_THREAD; ++i) v[i] = a.img[min(tid + i * TAIL_NT, a.T - 1)];
+        const int nv0 = (int)(a.d[0].nW() + a.d[0].nU());  // <= TAIL_NT (host check)
+        const double bv = a.bg[min(tid, nv0 - 1)];
+#pragma unroll
+        for (int i = 0; i < TAIL_IMG_PER_THREAD; ++i)
+            if (tid + i * TAIL_NT < a.T) lds[tid + i * TAIL_NT] = v[i];
+        if (tid < nv0) lds[a.off[0][6] + tid] = bv;
     }
     __syncthreads();
     unsigned swp = 0;  // bit l: x and t of level l exchanged
     auto X = [&](int l) { return lds + a.off[l][((swp >> l) & 1) ? 8 : 5]; };
     auto T = [&](int l) { return lds + a.off[l][((swp >> l) & 1) ? 5 : 8]; };
-    // down: smooth from zero, residual, restrict
-    for (int l = 0; l < a.nlev; ++l) {
+    // level l down: sweeps from zero, then (not coarsest) residual and restriction into l + 1.
+    // k0 / dk: this thread's first point and stride; sync: the barrier of the phase.
+    auto down = [&](int l, int k0, int dk, auto sync) -> bool {
         const LevD L = tail_lev(a, lds, l);
         const int64_t nv = L.nW() + L.nU();
         const bool coarsest = l + 1 == a.nlev;
@@ -372,23 +386,24 @@ __global__ __launch_bounds__(TAIL_NT) void k_mg_tail(TailArgs a) {
         for (int it = 0; it < sweeps; ++it) {
             double* xl = X(l);
             double* tl = T(l);
-            for (int64_t k = tid; k < nv; k += TAIL_NT) jacobi_at(L, a.F, xl, bl, tl, a.omega, 0, it == 0, k);
-            __syncthreads();
+            for (int64_t k = k0; k < nv; k += dk) jacobi_at(L, a.F, xl, bl, tl, a.omega, 0, it == 0, k);
+            sync();
             swp ^= 1u << l;
         }
-        if (coarsest) break;
+        if (coarsest) return false;
         double* rl = lds + a.off[l][7];
         const double* xl = X(l);
-        for (int64_t k = tid; k < nv; k += TAIL_NT) vel_resid_at(L, a.F, xl, bl, rl, 0, k);
-        __syncthreads();
+        for (int64_t k = k0; k < nv; k += dk) vel_resid_at(L, a.F, xl, bl, rl, 0, k);
+        sync();
         const LevD C = tail_lev(a, lds, l + 1);
         const int64_t nc = C.nW() + C.nU();
         double* bc = lds + a.off[l + 1][6];
-        for (int64_t k = tid; k < nc; k += TAIL_NT) restrict_at(L, C, rl, bc, k);
-        __syncthreads();
-    }
-    // up: prolong and post-smooth
-    for (int l = a.nlev - 2; l >= 0; --l) {
+        for (int64_t k = k0; k < nc; k += dk) restrict_at(L, C, rl, bc, k);
+        sync();
+        return true;
+    };
+    // level l up: prolongation of level l + 1's correction, then post-sweeps
+    auto up = [&](int l, int k0, int dk, auto sync) {
         const LevD L = tail_lev(a, lds, l);
         const LevD C = tail_lev(a, lds, l + 1);
         const int64_t nv = L.nW() + L.nU();
@@ -396,17 +411,31 @@ __global__ __launch_bounds__(TAIL_NT) void k_mg_tail(TailArgs a) {
         {
             double* xl = X(l);
             const double* xc = X(l + 1);
-            for (int64_t k = tid; k < nv; k += TAIL_NT) prolong_at(L, C, a.F, xc, xl, k);
+            for (int64_t k = k0; k < nv; k += dk) prolong_at(L, C, a.F, xc, xl, k);
         }
-        __syncthreads();
+        sync();
         for (int it = 0; it < a.post; ++it) {
             double* xl = X(l);
             double* tl = T(l);
-            for (int64_t k = tid; k < nv; k += TAIL_NT) jacobi_at(L, a.F, xl, bl, tl, a.omega, 0, 0, k);
-            __syncthreads();
+            for (int64_t k = k0; k < nv; k += dk) jacobi_at(L, a.F, xl, bl, tl, a.omega, 0, 0, k);
+            sync();
             swp ^= 1u << l;
         }
+    };
+    auto block_sync = [] { __syncthreads(); };
+    for (int l = 0; l < a.nbig; ++l)
+        if (!down(l, tid, TAIL_NT, block_sync)) break;
+    if (a.nbig < a.nlev) {
+        if (tid < 64) {  // the small levels: wave 0 alone
+            for (int l = a.nbig; l < a.nlev; ++l)
+                if (!down(l, tid, 64, wave_sync)) break;
+            for (int l = a.nlev - 2; l >= a.nbig; --l) up(l, tid, 64, wave_sync);
+            if (tid == 0) swp_shared = swp;
+        }
+        __syncthreads();
+        swp = swp_shared;  // the other waves did not follow the small levels' exchanges
     }
+    for (int l = min(a.nbig, a.nlev - 1) - 1; l >= 0; --l) up(l, tid, TAIL_NT, block_sync);
     const int nv0 = (int)(a.d[0].nW() + a.d[0].nU());
     const double* x0 = X(0);
     for (int k = tid; k < nv0; k += TAIL_NT) a.xg[k] = x0[k];
@@ -665,6 +694,36 @@ __global__ __launch_bounds__(NT) void k_dot_final(const double* __restrict__ par
     for (int b = lane; b < nb; b += 64) acc += part[(int64_t)q * nb + b];
     acc = wave_sum(acc);
     if (lane == 0) out[q] = acc;
+}
+// largest viscosity and smallest density of a level: per-block partials, then one wave combines them
+__global__ __launch_bounds__(NT) void k_eta_rho_part(LevD L, double* __restrict__ part) {
+    const int64_t n = (int64_t)L.Nz * L.Nx;
+    double emax = 0.0, rmin = 1e308;
+    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+        emax = fmax(emax, L.eta[i]);
+        rmin = fmin(rmin, L.rho[i]);
+    }
+    __shared__ double se[NT / 64], sr[NT / 64];
+    for (int o = 32; o > 0; o >>= 1) {
+        emax = fmax(emax, __shfl_xor(emax, o));
+        rmin = fmin(rmin, __shfl_xor(rmin, o));
+    }
+    if ((threadIdx.x & 63) == 0) { se[threadIdx.x >> 6] = emax; sr[threadIdx.x >> 6] = rmin; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < NT / 64; ++w) { emax = fmax(emax, se[w]); rmin = fmin(rmin, sr[w]); }
+        part[2 * blockIdx.x] = emax;
+        part[2 * blockIdx.x + 1] = rmin;
+    }
+}
+__global__ __launch_bounds__(64) void k_eta_rho_final(const double* __restrict__ part, int nb, double* out) {
+    double emax = 0.0, rmin = 1e308;
+    for (int b = threadIdx.x; b < nb; b += 64) { emax = fmax(emax, part[2 * b]); rmin = fmin(rmin, part[2 * b + 1]); }
+    for (int o = 32; o > 0; o >>= 1) {
+        emax = fmax(emax, __shfl_xor(emax, o));
+        rmin = fmin(rmin, __shfl_xor(rmin, o));
+    }
+    if (threadIdx.x == 0) { out[0] = emax; out[1] = rmin; }
 }
 // w -= sum_q c[q] V[q]
 __global__ __launch_bounds__(NT) void k_orth(const double* __restrict__ V, int64_t n, int K, const double* __restrict__ c,
@@ -1058,7 +1117,7 @@ struct Level {
 };
 
 struct StokesCtx {
-    int Nz = -1, Nx = -1, m = -1;
+    int Nz = -1, Nx = -1, m = -1, maxlev = 0;
     std::vector<Level> lev;
     double* pool = nullptr;   // everything below carved from one allocation
     size_t pool_bytes = 0;
@@ -1084,6 +1143,13 @@ struct StokesCtx {
     float* lc_sigma = nullptr;  // per-call input scale (device)
     double* lc_part = nullptr;  // restriction partial sums
     int lc_cap = 0;             // capacity in coarse cells
+    // the V-cycle's one-workgroup tail: from level tail_l on (-1: none), its launch arguments and LDS bytes
+    int tail_l = -1;
+    TailArgs tail{};
+    size_t tail_lds = 0;
+    // depth of the V-cycle for the current solve (levels 0 .. leff-1) and the inertia ratio it came from
+    int leff = 0;
+    double inertia = 0.0;
 };
 
 // Decide whether the inertial Schur part applies to this solve and (re)build its eigenvalue tables.
@@ -1118,17 +1184,61 @@ void stokes_free(xc_ctx*, void* p) {
 
 inline unsigned nblk(int64_t n) { return (unsigned)((n + NT - 1) / NT); }
 
-int setup(xc_ctx* ctx, int Nz, int Nx, int m, StokesCtx** out) {
+// the tail of the V-cycle: the first level l >= 1 of at most TAIL_CELLS cells with at most TAIL_MAX
+// levels from it to the coarsest, if its coefficient image, vectors and right-hand side fit
+void tail_setup(StokesCtx* s) {
+    s->tail_l = -1;
+    const int nl = (int)s->lev.size();
+    int l0 = -1;
+    for (int l = 1; l < nl; ++l)
+        if ((int64_t)s->lev[l].Nz * s->lev[l].Nx <= TAIL_CELLS && nl - l <= TAIL_MAX) { l0 = l; break; }
+    if (l0 < 0) return;
+    TailArgs& ta = s->tail;
+    ta = TailArgs{};
+    ta.nlev = nl - l0;
+    const double* img = s->lev[l0].eta;
+    const Level& last = s->lev[nl - 1];
+    ta.T = (int)((last.rhou + (((size_t)last.Nz * (last.Nx + 1) + 1) & ~(size_t)1)) - img);
+    int o = (ta.T + 1) & ~1;
+    ta.nbig = 0;
+    for (int q = 0; q < ta.nlev; ++q) {
+        const Level& L = s->lev[l0 + q];
+        ta.d[q] = L.d();
+        ta.off[q][0] = (int)(L.eta - img);
+        ta.off[q][1] = (int)(L.etaco - img);
+        ta.off[q][2] = (int)(L.rho - img);
+        ta.off[q][3] = (int)(L.rhow - img);
+        ta.off[q][4] = (int)(L.rhou - img);
+        for (int k = 5; k < 9; ++k) {
+            ta.off[q][k] = o;
+            o += ((int)L.nv + 1) & ~1;
+        }
+        if (L.nv > TAIL_WAVE_PTS && ta.nbig == q) ta.nbig = q + 1;
+    }
+    ta.img = img;
+    const size_t bytes = (size_t)o * sizeof(double);
+    if (bytes > 65536 || ta.T > TAIL_IMG_PER_THREAD * TAIL_NT || s->lev[l0].nv > TAIL_NT) return;
+    s->tail_lds = bytes;
+    s->tail_l = l0;
+}
+
+// maxlev: most multigrid levels (0: all; < 0: whatever the existing hierarchy has)
+int setup(xc_ctx* ctx, int Nz, int Nx, int m, StokesCtx** out, int maxlev = -1) {
     StokesCtx* s = (StokesCtx*)ctx->ext[XC_EXT_STOKES];
-    if (s && s->Nz == Nz && s->Nx == Nx && s->m >= m) { *out = s; return XC_OK; }
+    if (s && s->Nz == Nz && s->Nx == Nx && s->m >= m && (maxlev < 0 || s->maxlev == maxlev)) {
+        *out = s;
+        return XC_OK;
+    }
+    if (maxlev < 0) maxlev = s ? s->maxlev : 0;
     if (s) { XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream)); stokes_free(ctx, s); ctx->ext[XC_EXT_STOKES] = nullptr; }
     s = new StokesCtx();
-    s->Nz = Nz; s->Nx = Nx; s->m = m;
-    // levels: halve while both sizes stay even and >= 8 (coarsest at least 4x4)
+    s->Nz = Nz; s->Nx = Nx; s->m = m; s->maxlev = maxlev;
+    // levels: halve while both sizes stay even and >= 8 (coarsest at least 4x4), at most maxlev of them
     int nz = Nz, nx = Nx;
     s->lev.push_back(Level{});
     s->lev.back().Nz = nz; s->lev.back().Nx = nx;
-    while (nz % 2 == 0 && nx % 2 == 0 && nz >= 8 && nx >= 8 && s->lev.size() < 12) {
+    const size_t cap = maxlev > 0 && maxlev < 12 ? (size_t)maxlev : 12;
+    while (nz % 2 == 0 && nx % 2 == 0 && nz >= 8 && nx >= 8 && s->lev.size() < cap) {
         nz /= 2; nx /= 2;
         Level L{};
         L.Nz = nz; L.Nx = nx;
@@ -1139,7 +1249,7 @@ int setup(xc_ctx* ctx, int Nz, int Nx, int m, StokesCtx** out) {
     for (auto& L : s->lev) {
         L.nv = (int64_t)(L.Nz + 1) * (L.Nx + 2) + (int64_t)(L.Nz + 2) * (L.Nx + 1);
         tot += (size_t)L.Nz * L.Nx * 2 + (size_t)(L.Nz + 1) * (L.Nx + 1) + (size_t)(L.Nz + 1) * L.Nx +
-               (size_t)L.Nz * (L.Nx + 1) + 4 * (size_t)L.nv;
+               (size_t)L.Nz * (L.Nx + 1) + 5 /* even rounding */ + 4 * (size_t)L.nv;
     }
     const int K = Nx / 2 + 1;
     const size_t lcc = (size_t)(Nz / 2) * (Nx / 2);  // learned correction at level >= 1
@@ -1155,12 +1265,17 @@ int setup(xc_ctx* ctx, int Nz, int Nx, int m, StokesCtx** out) {
     s->pool_bytes = tot * sizeof(double);
     double* p = s->pool;
     auto take = [&](size_t k) { double* q = p; p += k; return q; };
+    // coefficients of every level first, in the order and even padding of the tail's LDS layout, so
+    // that the tail levels' coefficients are one contiguous block
+    auto take2 = [&](size_t k) { return take((k + 1) & ~(size_t)1); };
     for (auto& L : s->lev) {
-        L.eta = take((size_t)L.Nz * L.Nx);
-        L.rho = take((size_t)L.Nz * L.Nx);
-        L.etaco = take((size_t)(L.Nz + 1) * (L.Nx + 1));
-        L.rhow = take((size_t)(L.Nz + 1) * L.Nx);
-        L.rhou = take((size_t)L.Nz * (L.Nx + 1));
+        L.eta = take2((size_t)L.Nz * L.Nx);
+        L.etaco = take2((size_t)(L.Nz + 1) * (L.Nx + 1));
+        L.rho = take2((size_t)L.Nz * L.Nx);
+        L.rhow = take2((size_t)(L.Nz + 1) * L.Nx);
+        L.rhou = take2((size_t)L.Nz * (L.Nx + 1));
+    }
+    for (auto& L : s->lev) {
         L.x = take(L.nv); L.b = take(L.nv); L.r = take(L.nv); L.t = take(L.nv);
     }
     s->xs = take(n); s->bs = take(n); s->rs = take(n); s->ws = take(n);
@@ -1187,6 +1302,7 @@ int setup(xc_ctx* ctx, int Nz, int Nx, int m, StokesCtx** out) {
         stokes_free(ctx, s);
         return xc_fail(ctx, XC_EHIP, "fluidmech!: internal pool sizing (%zu > %zu)", (size_t)(p - s->pool), tot);
     }
+    tail_setup(s);
     if (Nx > MAXLINE || 2 * Nz > MAXLINE || !fft_factorize(Nx, s->prad) || !fft_factorize(2 * Nz, s->pradz)) {
         s->prad.clear();  // inertial Schur part unavailable at this size: the viscous part alone is used
         s->pradz.clear();
@@ -1239,6 +1355,8 @@ Flags make_flags(const xc_fm_opts* o, const xc_par* par, int Nz, int Nx, bool bn
 
 // copy the fine-level coefficients densely and build the coarse hierarchy
 void build_levels(xc_ctx* ctx, StokesCtx* s, const xc_fluid* fl, double h) {
+    s->leff = (int)s->lev.size();  // the full V-cycle unless choose_depth says otherwise
+    s->inertia = 0.0;
     Level& L0 = s->lev[0];
     L0.h = h;
     hipLaunchKernelGGL(k_pack, dim3(nblk(fl->eta.rows * fl->eta.cols)), dim3(NT), 0, ctx->stream, fl->eta, L0.eta);
@@ -1257,6 +1375,47 @@ void build_levels(xc_ctx* ctx, StokesCtx* s, const xc_fluid* fl, double h) {
     }
 }
 
+// V-cycle depth for this solve.  The velocity block is a1 rho / dt (inertia) + eta k^2 (viscous).  With
+// R = a1 rho h^2 / (dt eta) a W or U row's diagonal is (R + 7/3) eta / h^2 and the smoothest error mode's
+// eigenvalue R eta / h^2, so a damped-Jacobi sweep scales that mode by 1 - omega R / (R + 7/3): where R is
+// large the smoother alone removes the smooth error and coarser levels add work but not convergence.  On
+// level l (spacing 2^l h) R_l = 4^l R_0 with R_0 = a1 min(rho) h^2 / (dt max(eta)) (the least inertial
+// cell); the cycle stops at the first level where two sweeps reduce the smoothest mode at least 4x
+// (omega R / (R + 7/3) >= 1/2, R >= 2.5 * 7/3, MG_INERTIA_STOP).  Without inertia (dt = Inf, the
+// benchmark mode) every level is kept.  mg_levels > 0 in the options caps the depth instead.
+constexpr double MG_INERTIA_STOP = 6.0;
+// the reduction choose_depth needs, enqueued (results in s->host[200..201] once the stream has reached it);
+// returns whether it was needed
+bool depth_launch(xc_ctx* ctx, StokesCtx* s, const Flags& F, const xc_fm_opts* o) {
+    if (o->mg_levels > 0 || F.bnchm || !(F.dt < 1e30) || !(F.a1 > 0)) return false;
+    const LevD d = s->lev[0].d();
+    const int nb = 256;
+    hipLaunchKernelGGL(k_eta_rho_part, dim3(nb), dim3(NT), 0, ctx->stream, d, s->part);
+    hipLaunchKernelGGL(k_eta_rho_final, dim3(1), dim3(64), 0, ctx->stream, s->part, nb, s->host_dev + 200);
+    return true;
+}
+// sets s->leff; launched: depth_launch enqueued the reduction; synced: the stream has been synchronized since
+int choose_depth(xc_ctx* ctx, StokesCtx* s, const Flags& F, const xc_fm_opts* o, double h, bool launched,
+                 bool synced) {
+    const int nl = (int)s->lev.size();
+    s->leff = nl;
+    s->inertia = 0.0;
+    if (o->mg_levels > 0) {
+        s->leff = o->mg_levels < nl ? o->mg_levels : nl;
+        return XC_OK;
+    }
+    if (!launched) return XC_OK;
+    if (!synced) XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const double emax = s->host[200], rmin = s->host[201];
+    if (!(emax > 0) || !(rmin > 0)) return XC_OK;
+    const double R0 = F.a1 * rmin * h * h / (F.dt * emax);
+    s->inertia = R0;
+    int L = 1;
+    for (double R = R0; L < nl && R < MG_INERTIA_STOP; R *= 4.0) ++L;
+    s->leff = L;
+    return XC_OK;
+}
+
 struct MGOpts { int pre, post, coarse; double omega; int sub; };  // sub: pre = post sweeps on levels >= 1 (0: as level 0)
 
 // solver controls with the defaults measured best on the 512^2 production step (zero fields: default)
@@ -1264,46 +1423,42 @@ MGOpts mg_opts(const xc_fm_opts* o) {
     MGOpts m;
     m.pre = o->mg_pre > 0 ? o->mg_pre : 3;
     m.post = o->mg_post > 0 ? o->mg_post : 3;
-    m.coarse = o->mg_coarse > 0 ? o->mg_coarse : 20;
+    m.coarse = o->mg_coarse > 0 ? o->mg_coarse : 0;  // 0: coarse_sweeps decides per solve
     m.omega = o->mg_omega > 0 ? o->mg_omega : 0.7;
     m.sub = o->mg_sub > 0 ? o->mg_sub : (o->mg_sub == 0 ? 1 : 0);
     return m;
 }
 
+// sweeps on the coarsest level of the cycle: as set, else 20 on a 4 x 4 bottom level and 2 on a level where
+// the cycle stopped because inertia dominates (the two sweeps choose_depth's criterion is stated for; on the
+// 512^2 step 1, 2 and 4 sweeps there gave 10.4, 10.3 and 9.6 FGMRES iterations per solve)
+int coarse_sweeps(const StokesCtx* s, const MGOpts& mo) {
+    if (mo.coarse > 0) return mo.coarse;
+    return s->leff < (int)s->lev.size() ? 2 : 20;
+}
+
 // one V-cycle on level l: solve A x = b approximately from x = 0
 void vcycle(xc_ctx* ctx, StokesCtx* s, const Flags& F, size_t l, const MGOpts& mo, bool first_done = false) {
     Level& L = s->lev[l];
-    if (l > 0 && (int64_t)L.Nz * L.Nx <= TAIL_CELLS && s->lev.size() - l <= (size_t)TAIL_MAX) {
-        TailArgs ta{};
-        ta.nlev = (int)(s->lev.size() - l);
-        int o = 0;
-        for (int q = 0; q < ta.nlev; ++q) {
-            Level& T = s->lev[l + q];
-            ta.d[q] = T.d();
-            const int Nz = T.Nz, Nx = T.Nx;
-            const int sz[9] = {Nz * Nx, (Nz + 1) * (Nx + 1), Nz * Nx, (Nz + 1) * Nx, Nz * (Nx + 1),
-                               (int)T.nv, (int)T.nv, (int)T.nv, (int)T.nv};
-            for (int k = 0; k < 9; ++k) {
-                ta.off[q][k] = o;
-                o += (sz[k] + 1) & ~1;
-            }
-        }
-        if ((size_t)o * sizeof(double) > 65536) goto per_level;  // tail too large for one workgroup's LDS
+    if ((int)l == s->tail_l && s->leff > s->tail_l) {
+        TailArgs ta = s->tail;
+        ta.nlev = s->leff - s->tail_l;
+        if (ta.nbig > ta.nlev) ta.nbig = ta.nlev;
+        for (int q = 0; q < ta.nlev; ++q) ta.d[q] = s->lev[l + q].d();  // h is set per solve
         ta.bg = L.b;
         ta.xg = L.x;
         ta.pre = mo.sub > 0 ? mo.sub : mo.pre;
         ta.post = mo.sub > 0 ? mo.sub : mo.post;
-        ta.coarse = mo.coarse; ta.omega = mo.omega;
+        ta.coarse = coarse_sweeps(s, mo); ta.omega = mo.omega;
         ta.F = F;
-        hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(TAIL_NT), (size_t)o * sizeof(double), ctx->stream, ta);
+        hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(TAIL_NT), s->tail_lds, ctx->stream, ta);
         return;
     }
-per_level:
     const LevD d = L.d();
     const unsigned nb = nblk(L.nv);
     const int fine = l == 0;
-    const bool coarsest = l + 1 == s->lev.size();
-    const int pre = coarsest ? mo.coarse : (l > 0 && mo.sub > 0 ? mo.sub : mo.pre);
+    const bool coarsest = (int)l + 1 == s->leff;
+    const int pre = coarsest ? coarse_sweeps(s, mo) : (l > 0 && mo.sub > 0 ? mo.sub : mo.pre);
     const int post = l > 0 && mo.sub > 0 ? mo.sub : mo.post;
     // first sweep from zero (already done by the caller when first_done)
     if (!first_done) {
@@ -1368,7 +1523,7 @@ void precond(xc_ctx* ctx, StokesCtx* s, const Flags& F, const double* v, double*
     // velocity: b = r_v - GG z_p and the first sweep in one kernel, then the V-cycle.  With an even
     // number of level-0 sweeps the iterates alternate between z and the level's scratch and end in z.
     double* own_x = L.x;
-    const bool direct = s->lev.size() > 1 && (mo.pre + mo.post) % 2 == 0;
+    const bool direct = s->leff > 1 && (mo.pre + mo.post) % 2 == 0;
     if (direct) L.x = z;
     hipLaunchKernelGGL(k_jacobi0_grad, dim3(nblk(nv)), dim3(NT), 0, ctx->stream, d, F, v, zP, L.b, L.t, mo.omega);
     std::swap(L.x, L.t);
@@ -1618,6 +1773,7 @@ int check_fm(xc_ctx* ctx, const xc_fluid* fl, const xc_grid* g, const xc_fm_opts
     if (o->xbc != XC_PERIODIC && o->xbc != XC_CLOSED) return xc_fail(ctx, XC_EINVAL_BC, "fluidmech!: xBC");
     if (o->zbc != XC_PERIODIC && o->zbc != XC_CLOSED) return xc_fail(ctx, XC_EINVAL_BC, "fluidmech!: zBC");
     if (xc_scheme_halo(o->scheme) < 0) return xc_fail(ctx, XC_EINVAL_SCHEME, "fluidmech!: scheme");
+    if (o->mg_levels < 0) return xc_fail(ctx, XC_EINVAL_ARG, "fluidmech!: mg_levels must be >= 0");
     return XC_OK;
 }
 
@@ -1631,7 +1787,7 @@ int xc_stokes_apply(xc_ctx* ctx, const xc_fluid* fl, const xc_grid* g, const xc_
     int st;
     if ((st = check_fm(ctx, fl, g, o))) return st;
     StokesCtx* s;
-    if ((st = setup(ctx, (int)g->Nz, (int)g->Nx, 1, &s))) return st;
+    if ((st = setup(ctx, (int)g->Nz, (int)g->Nx, 1, &s, o->mg_levels))) return st;
     build_levels(ctx, s, fl, g->h);
     const Flags F = make_flags(o, par, (int)g->Nz, (int)g->Nx, bnchm != 0);
     const int64_t n = s->lev[0].nv + (g->Nz + 2) * (g->Nx + 2);
@@ -1646,7 +1802,7 @@ int xc_stokes_rhs(xc_ctx* ctx, xc_fluid* fl, const xc_phase* ph, const xc_grid* 
     if ((st = check_fm(ctx, fl, g, o))) return st;
     if (!bn && !ph) return xc_fail(ctx, XC_EINVAL_ARG, "fluidmech!: phase required outside the benchmark mode");
     StokesCtx* s;
-    if ((st = setup(ctx, (int)g->Nz, (int)g->Nx, 1, &s))) return st;
+    if ((st = setup(ctx, (int)g->Nz, (int)g->Nx, 1, &s, o->mg_levels))) return st;
     const Flags F = make_flags(o, par, (int)g->Nz, (int)g->Nx, bn != nullptr);
     if ((st = build_rhs(ctx, s, fl, ph, g, par, o, bn, F, rr))) return st;
     return xc_finish(ctx, "stokes_rhs");
@@ -1660,7 +1816,7 @@ int xc_fluidmech(xc_ctx* ctx, xc_fluid* fl, const xc_phase* ph, const xc_grid* g
     if (!bn && !ph) return xc_fail(ctx, XC_EINVAL_ARG, "fluidmech!: phase required outside the benchmark mode");
     const int m = o->restart > 0 ? (o->restart < 60 ? o->restart : 60) : 30;
     StokesCtx* s;
-    if ((st = setup(ctx, (int)g->Nz, (int)g->Nx, m, &s))) return st;
+    if ((st = setup(ctx, (int)g->Nz, (int)g->Nx, m, &s, o->mg_levels))) return st;
     const int Nz = (int)g->Nz, Nx = (int)g->Nx;
     const Flags F = make_flags(o, par, Nz, Nx, bn != nullptr);
     const int64_t n = s->lev[0].nv + (int64_t)(Nz + 2) * (Nx + 2);
@@ -1668,12 +1824,14 @@ int xc_fluidmech(xc_ctx* ctx, xc_fluid* fl, const xc_phase* ph, const xc_grid* g
     const MGOpts mo = mg_opts(o);
     // system: coefficients, RR, initial guess
     build_levels(ctx, s, fl, g->h);
+    const bool depth_pending = depth_launch(ctx, s, F, o);
     if ((st = pois_table(ctx, s, F, Nz, Nx, g->h))) return st;
     if ((st = build_rhs(ctx, s, fl, ph, g, par, o, bn, F, s->bs))) return st;
     hipLaunchKernelGGL(k_gather, dim3(nb), dim3(NT), 0, ctx->stream, fl->W, fl->U, fl->P, s->xs, Nz, Nx);
     // ||RR||
     if ((st = multidot(ctx, s, s->bs, 0, s->bs, n))) return st;
     const double bnorm = sqrt(s->host[0]);
+    if ((st = choose_depth(ctx, s, F, o, g->h, depth_pending, true))) return st;  // multidot synchronized
     const double target = o->tol * (bnorm > 0 ? bnorm : 1.0);
     const int maxit = o->maxit > 0 ? o->maxit : 500;
     int iters = 0;
@@ -1710,7 +1868,8 @@ int xc_fluidmech(xc_ctx* ctx, xc_fluid* fl, const xc_phase* ph, const xc_grid* g
         stats->converged = conv;
         stats->rel_resid = rel;
         stats->init_rel_resid = rel0;
-        stats->mg_levels = (int)s->lev.size();
+        stats->mg_levels = s->leff;
+        stats->inertia_ratio = s->inertia;
         stats->fallback_used = fallback;
     }
     if ((st = xc_finish(ctx, "fluidmech!"))) return st;
@@ -1725,10 +1884,11 @@ static int piece_setup(xc_ctx* ctx, const xc_fluid* fl, const xc_grid* g, const 
                        int bnchm, int m, StokesCtx** sp, Flags* F, MGOpts* mo, int64_t* n) {
     int st;
     if ((st = check_fm(ctx, fl, g, o))) return st;
-    if ((st = setup(ctx, (int)g->Nz, (int)g->Nx, m, sp))) return st;
+    if ((st = setup(ctx, (int)g->Nz, (int)g->Nx, m, sp, o->mg_levels))) return st;
     StokesCtx* s = *sp;
     build_levels(ctx, s, fl, g->h);
     *F = make_flags(o, par, (int)g->Nz, (int)g->Nx, bnchm != 0);
+    if ((st = choose_depth(ctx, s, *F, o, g->h, depth_launch(ctx, s, *F, o), false))) return st;
     if ((st = pois_table(ctx, s, *F, (int)g->Nz, (int)g->Nx, g->h))) return st;
     *mo = mg_opts(o);
     *n = s->lev[0].nv + (g->Nz + 2) * (g->Nx + 2);
@@ -1773,7 +1933,8 @@ int xc_stokes_solve(xc_ctx* ctx, const xc_fluid* fl, const xc_grid* g, const xc_
         stats->converged = so.conv;
         stats->rel_resid = so.rel;
         stats->init_rel_resid = so.rel0;
-        stats->mg_levels = (int)s->lev.size();
+        stats->mg_levels = s->leff;
+        stats->inertia_ratio = s->inertia;
         stats->fallback_used = 0;
     }
     if ((st = xc_finish(ctx, "stokes_solve"))) return st;
