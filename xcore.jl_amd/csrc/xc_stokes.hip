@@ -36,6 +36,7 @@ constexpr int NT = 256;
 struct Flags {
     int bnchm, sds, top_cnv, bot_cnv, open_cnv;
     double a1, gamma, dt, g0;
+    double a1_dt, a1g_dt;  // a1 / dt and (a1 + gamma) / dt, for the preconditioner's rows
     int64_t nu0;          // MMS U-pin velocity index (fine level) or -1
     int pin_kind;         // 1: single P pin (pr, pc); 2: open_cnv pins on P row Nz
     int pr, pc;
@@ -60,6 +61,10 @@ struct LevD {  // coefficients of one level, dense column-major (ld = rows)
 };
 
 // ---- rows of the velocity block KV (fluidmech.jl:145-324).  Return sum_j KV_ij x_j, diag in *dg.
+// PRE: the multigrid smoother's form of the row (precomputed a1 / dt, products with 1/3 instead of
+// quotients): the same values to rounding, without the divisions.  The operator itself (ll_row) keeps
+// the reference's operations.
+template <bool PRE = false>
 __device__ double w_row(const LevD& L, const Flags& F, const double* x, int r, int c, double* dg, bool* bc) {
     const int Nz = L.Nz, Nx = L.Nx;
     *bc = true;
@@ -76,15 +81,18 @@ __device__ double w_row(const LevD& L, const Flags& F, const double* x, int r, i
     *bc = false;
     const double i2 = L.invh2;
     const double C1 = L.ETACO(r, c - 1), C2 = L.ETACO(r, c), P1 = L.ETA(r - 1, c - 1), P2 = L.ETA(r, c - 1);
-    double d = F.a1 * L.RHOW(r, c - 1) / F.dt + ((2.0 / 3.0) * (P1 + P2) * i2 + 0.5 * (C1 + C2) * i2);
+    double d = (PRE ? L.RHOW(r, c - 1) * F.a1_dt : F.a1 * L.RHOW(r, c - 1) / F.dt) +
+               ((2.0 / 3.0) * (P1 + P2) * i2 + 0.5 * (C1 + C2) * i2);
     if (!F.bnchm) d += (L.RHO(r, c - 1) - L.RHO(r - 1, c - 1)) * L.invh * F.g0 * F.dt;
     *dg = d;
+    const double T1 = PRE ? P1 * (1.0 / 3.0) : P1 / 3.0, T2 = PRE ? P2 * (1.0 / 3.0) : P2 / 3.0;
     return d * x[L.iW(r, c)] - (2.0 / 3.0) * P1 * i2 * x[L.iW(r - 1, c)] - (2.0 / 3.0) * P2 * i2 * x[L.iW(r + 1, c)] -
            0.5 * C1 * i2 * x[L.iW(r, c - 1)] - 0.5 * C2 * i2 * x[L.iW(r, c + 1)] -
-           (0.5 * C1 - P1 / 3.0) * i2 * x[L.iU(r, c - 1)] + (0.5 * C1 - P2 / 3.0) * i2 * x[L.iU(r + 1, c - 1)] +
-           (0.5 * C2 - P1 / 3.0) * i2 * x[L.iU(r, c)] - (0.5 * C2 - P2 / 3.0) * i2 * x[L.iU(r + 1, c)];
+           (0.5 * C1 - T1) * i2 * x[L.iU(r, c - 1)] + (0.5 * C1 - T2) * i2 * x[L.iU(r + 1, c - 1)] +
+           (0.5 * C2 - T1) * i2 * x[L.iU(r, c)] - (0.5 * C2 - T2) * i2 * x[L.iU(r + 1, c)];
 }
 
+template <bool PRE = false>
 __device__ double u_row(const LevD& L, const Flags& F, const double* x, int r, int c, double* dg, bool* bc) {
     const int Nz = L.Nz;
     *bc = true;
@@ -94,28 +102,31 @@ __device__ double u_row(const LevD& L, const Flags& F, const double* x, int r, i
     const double i2 = L.invh2;
     const double C1 = L.ETACO(r - 1, c), C2 = L.ETACO(r, c);
     const double P1 = L.ETA(r - 1, L.icx(c)), P2 = L.ETA(r - 1, L.icx(c + 1));
-    double d = (F.a1 + F.gamma) * L.RHOU(r - 1, c) / F.dt + ((2.0 / 3.0) * (P1 + P2) * i2 + 0.5 * (C1 + C2) * i2);
+    double d = (PRE ? L.RHOU(r - 1, c) * F.a1g_dt : (F.a1 + F.gamma) * L.RHOU(r - 1, c) / F.dt) +
+               ((2.0 / 3.0) * (P1 + P2) * i2 + 0.5 * (C1 + C2) * i2);
     if (!F.bnchm) d += (L.RHO(r - 1, L.icx(c + 1)) - L.RHO(r - 1, L.icx(c))) * L.invh * F.g0 * F.dt;
     *dg = d;
+    const double T1 = PRE ? P1 * (1.0 / 3.0) : P1 / 3.0, T2 = PRE ? P2 * (1.0 / 3.0) : P2 / 3.0;
     return d * x[L.iU(r, c)] - (2.0 / 3.0) * P1 * i2 * x[L.iU(r, L.ifx(c))] -
            (2.0 / 3.0) * P2 * i2 * x[L.iU(r, L.ifx(c + 2))] - 0.5 * C1 * i2 * x[L.iU(r - 1, c)] -
-           0.5 * C2 * i2 * x[L.iU(r + 1, c)] - (0.5 * C1 - P1 / 3.0) * i2 * x[L.iW(r - 1, c)] +
-           (0.5 * C1 - P2 / 3.0) * i2 * x[L.iW(r - 1, c + 1)] + (0.5 * C2 - P1 / 3.0) * i2 * x[L.iW(r, c)] -
-           (0.5 * C2 - P2 / 3.0) * i2 * x[L.iW(r, c + 1)];
+           0.5 * C2 * i2 * x[L.iU(r + 1, c)] - (0.5 * C1 - T1) * i2 * x[L.iW(r - 1, c)] +
+           (0.5 * C1 - T2) * i2 * x[L.iW(r - 1, c + 1)] + (0.5 * C2 - T1) * i2 * x[L.iW(r, c)] -
+           (0.5 * C2 - T2) * i2 * x[L.iW(r, c + 1)];
 }
 
 // velocity row k of KV (index into the W|U vector): value and diagonal
+template <bool PRE = false>
 __device__ __forceinline__ double vel_row(const LevD& L, const Flags& F, const double* x, int64_t k, double* dg,
                                           bool* bc, bool fine) {
     if (fine && k == F.nu0) { *dg = 1.0; *bc = true; return x[k]; }
     const int64_t nW = L.nW();
     if (k < nW) {
         const int r = ((int)k % (L.Nz + 1)), c = ((int)k / (L.Nz + 1));
-        return w_row(L, F, x, r, c, dg, bc);
+        return w_row<PRE>(L, F, x, r, c, dg, bc);
     }
     const int64_t q = k - nW;
     const int r = ((int)q % (L.Nz + 2)), c = ((int)q / (L.Nz + 2));
-    return u_row(L, F, x, r, c, dg, bc);
+    return u_row<PRE>(L, F, x, r, c, dg, bc);
 }
 
 __device__ __forceinline__ bool is_pin(const Flags& F, int r, int c, int Nz) {
@@ -173,18 +184,25 @@ __global__ __launch_bounds__(NT) void k_ll_resid(LevD L, Flags F, const double* 
 }
 
 // damped Jacobi on the velocity block: xo = x + w (b - A x) / diag (BC rows with weight 1)
+// 1 / d for the smoother: hardware reciprocal refined by two Newton-Raphson steps (d: a nonzero normal
+// diagonal entry)
+__device__ __forceinline__ double recip(double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(r, fma(-d, r, 1.0), r);
+    return fma(r, fma(-d, r, 1.0), r);
+}
 __device__ __forceinline__ void jacobi_at(const LevD& L, const Flags& F, const double* __restrict__ x,
                                           const double* __restrict__ b, double* __restrict__ xo, double omega,
                                           int fine, int zero_guess, int64_t k) {
     double dg;
     bool bc;
     if (zero_guess) {  // x == 0: A x == 0, diag from the row formula evaluated on x (all zeros)
-        vel_row(L, F, x, k, &dg, &bc, fine != 0);
-        xo[k] = (bc ? 1.0 : omega) * b[k] / dg;
+        vel_row<true>(L, F, x, k, &dg, &bc, fine != 0);
+        xo[k] = (bc ? 1.0 : omega) * b[k] * recip(dg);
         return;
     }
-    const double ax = vel_row(L, F, x, k, &dg, &bc, fine != 0);
-    xo[k] = x[k] + (bc ? 1.0 : omega) * (b[k] - ax) / dg;
+    const double ax = vel_row<true>(L, F, x, k, &dg, &bc, fine != 0);
+    xo[k] = x[k] + (bc ? 1.0 : omega) * (b[k] - ax) * recip(dg);
 }
 __global__ __launch_bounds__(NT) void k_jacobi(LevD L, Flags F, const double* __restrict__ x,
                                                const double* __restrict__ b, double* __restrict__ xo, double omega,
@@ -198,7 +216,7 @@ __device__ __forceinline__ void vel_resid_at(const LevD& L, const Flags& F, cons
                                              int64_t k) {
     double dg;
     bool bc;
-    r[k] = b[k] - vel_row(L, F, x, k, &dg, &bc, fine != 0);
+    r[k] = b[k] - vel_row<true>(L, F, x, k, &dg, &bc, fine != 0);
 }
 // level-0 right-hand side b = r_v - GG z_p (stored) and the first damped-Jacobi sweep from zero
 __global__ __launch_bounds__(NT) void k_jacobi0_grad(LevD L, Flags F, const double* __restrict__ rV,
@@ -225,8 +243,8 @@ __global__ __launch_bounds__(NT) void k_jacobi0_grad(LevD L, Flags F, const doub
     b[k] = bk;
     double dg;
     bool bc;
-    vel_row(L, F, rV, k, &dg, &bc, true);  // diagonal only (the row value is unused)
-    xo[k] = (bc ? 1.0 : omega) * bk / dg;
+    vel_row<true>(L, F, rV, k, &dg, &bc, true);  // diagonal only (the row value is unused)
+    xo[k] = (bc ? 1.0 : omega) * bk * recip(dg);
 }
 __global__ __launch_bounds__(NT) void k_vel_resid(LevD L, Flags F, const double* __restrict__ x,
                                                   const double* __restrict__ b, double* __restrict__ r, int fine) {
@@ -1335,6 +1353,8 @@ Flags make_flags(const xc_fm_opts* o, const xc_par* par, int Nz, int Nx, bool bn
     F.bnchm = bnchm;
     F.sds = o->sds; F.top_cnv = o->top_cnv; F.bot_cnv = o->bot_cnv; F.open_cnv = o->open_cnv;
     F.a1 = o->a1; F.gamma = par->gamma; F.dt = o->dt; F.g0 = par->g0;
+    F.a1_dt = F.a1 / F.dt;
+    F.a1g_dt = (F.a1 + F.gamma) / F.dt;
     F.nu0 = -1;
     auto jround = [](double v) { return (int)nearbyint(v); };
     if (bnchm) {
