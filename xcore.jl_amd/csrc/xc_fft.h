@@ -24,7 +24,19 @@ struct Plan {
     int rad[MAXST];
     const double2* tw;  // W_n^t = exp(-2 pi i t / n), t in [0, n)
     const double* g;    // filter multiplier exp(-s^2 k_t^2) / n, t in [0, n)
+    int lg1;            // log2(n) + 1 when n is a power of two factored into radices 4 and 2, else 0
 };
+
+// host: mark a completed plan whose length is a power of two (index arithmetic by shifts and masks)
+inline void plan_mark_pow2(Plan& p) {
+    p.lg1 = 0;
+    if (p.n < 1 || (p.n & (p.n - 1))) return;
+    for (int s = 0; s < p.nst; ++s)
+        if (p.rad[s] != 2 && p.rad[s] != 4) return;
+    int lg = 0;
+    while ((1 << lg) < p.n) ++lg;
+    p.lg1 = lg + 1;
+}
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -32,12 +44,15 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 
-// One Stockham stage of radix R over nl lines of length n (src -> dst).
-template <int R, bool INV>
+// One Stockham stage of radix R over nl lines of length n (src -> dst).  P2: n and Ns are powers of two
+// (lg = log2 n, lgNs = log2 Ns) and the index arithmetic is shifts and masks instead of divisions.
+template <int R, bool INV, bool P2 = false>
 __device__ __forceinline__ void stage(const double2* __restrict__ src, double2* __restrict__ dst, int n, int nl,
-                                      int Ns, const double2* __restrict__ tw) {
-    const int nR = n / R;
-    const int strideT = n / (Ns * R);
+                                      int Ns, const double2* __restrict__ tw, int lg = 0, int lgNs = 0) {
+    constexpr int LR = R == 2 ? 1 : (R == 4 ? 2 : 0);
+    static_assert(!P2 || LR > 0, "power-of-two stages are radix 2 or 4");
+    const int nR = P2 ? n >> LR : n / R;
+    const int strideT = P2 ? 1 << (lg - lgNs - LR) : n / (Ns * R);
     double2 wr[R];
 #pragma unroll
     for (int m = 0; m < R; ++m) {
@@ -45,16 +60,16 @@ __device__ __forceinline__ void stage(const double2* __restrict__ src, double2* 
         wr[m] = INV ? make_double2(w.x, -w.y) : w;
     }
     for (int k = threadIdx.x; k < nl * nR; k += FFT_NT) {
-        const int l = k / nR, j = k - l * nR;
+        const int l = P2 ? k >> (lg - LR) : k / nR, j = k - l * nR;
         const int base = l * n;
-        const int jm = j % Ns;
+        const int jm = P2 ? j & (Ns - 1) : j % Ns;
         const int t0 = jm * strideT;
         double2 v[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             double2 x = src[base + j + r * nR];
             if (r > 0) {
-                double2 w = tw[(t0 * r) % n];
+                double2 w = tw[P2 ? (t0 * r) & (n - 1) : (t0 * r) % n];
                 if (INV) w.y = -w.y;
                 x = cmul(x, w);
             }
@@ -82,7 +97,7 @@ __device__ __forceinline__ void stage(const double2* __restrict__ src, double2* 
                 o[q] = s;
             }
         }
-        const int idxD = (j / Ns) * Ns * R + jm;
+        const int idxD = P2 ? ((j >> lgNs) << (lgNs + LR)) + jm : (j / Ns) * Ns * R + jm;
 #pragma unroll
         for (int r = 0; r < R; ++r) dst[base + idxD + r * Ns] = o[r];
     }
@@ -119,6 +134,22 @@ __device__ double2* lds_fft(double2* a, double2* b, int nl, const Plan& p) {
     double2* src = a;
     double2* dst = b;
     int Ns = 1;
+    if (p.lg1 > 0) {  // power of two: radix-4 and radix-2 stages with shift/mask indexing
+        const int lg = p.lg1 - 1;
+        int lgNs = 0;
+        for (int s = 0; s < p.nst; ++s) {
+            if (p.rad[s] == 4) stage<4, INV, true>(src, dst, p.n, nl, Ns, p.tw, lg, lgNs);
+            else stage<2, INV, true>(src, dst, p.n, nl, Ns, p.tw, lg, lgNs);
+            __syncthreads();
+            double2* t = src;
+            src = dst;
+            dst = t;
+            const int lr = p.rad[s] == 4 ? 2 : 1;
+            Ns <<= lr;
+            lgNs += lr;
+        }
+        return src;
+    }
     for (int s = 0; s < p.nst; ++s) {
         const int R = p.rad[s];
         switch (R) {

@@ -394,6 +394,7 @@ int get_plan(xc_ctx* ctx, int n, double sigma, double h, Plan* out) {
             for (int s = 0; s < out->nst; ++s) out->rad[s] = e.rad[s];
             out->tw = e.tw;
             out->g = e.g;
+            plan_mark_pow2(*out);
             return XC_OK;
         }
     PlanEntry e;

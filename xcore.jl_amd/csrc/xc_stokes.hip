@@ -1519,6 +1519,7 @@ void precond(xc_ctx* ctx, StokesCtx* s, const Flags& F, const double* v, double*
         pl.nst = (int)s->prad.size();
         for (int q = 0; q < pl.nst; ++q) pl.rad[q] = s->prad[q];
         pl.tw = s->ptw;
+        plan_mark_pow2(pl);
         const unsigned nbx = (unsigned)((Nz + 2 * nl - 1) / (2 * nl));
         const size_t lds = 2 * (size_t)nl * Nx * sizeof(double2);
         PoisXArgs xa{vP, s->pS, pl, Nz, Nx, K, nl};
@@ -1528,6 +1529,7 @@ void precond(xc_ctx* ctx, StokesCtx* s, const Flags& F, const double* v, double*
         pz.nst = (int)s->pradz.size();
         for (int q = 0; q < pz.nst; ++q) pz.rad[q] = s->pradz[q];
         pz.tw = s->ptwz;
+        plan_mark_pow2(pz);
         int nlz = K / 256;
         const int nlzmax = MAXLINE / (2 * Nz) < 4 ? MAXLINE / (2 * Nz) : 4;
         nlz = nlz < 1 ? 1 : (nlz > nlzmax ? nlzmax : nlz);
