@@ -42,6 +42,14 @@ struct Flags {
     int pr, pc;
 };
 
+// Global element index of a 1-D launch of NT-lane workgroups, with the workgroups remapped so that each
+// XCD (which receives workgroup b when b % 8 is its number) walks a contiguous run of elements: the
+// neighbours a row reads, and the same elements in the next kernel (the next sweep of a level, the next
+// pass over a vector), are then in that XCD's L2.
+__device__ __forceinline__ int64_t gidx() {
+    return (int64_t)xcd_tile(blockIdx.x, gridDim.x) * NT + threadIdx.x;
+}
+
 struct LevD {  // coefficients of one level, dense column-major (ld = rows)
     int Nz, Nx;
     double h, invh, invh2;
@@ -172,14 +180,14 @@ __device__ double ll_row(const LevD& L, const Flags& F, const double* x, int64_t
 
 __global__ __launch_bounds__(NT) void k_ll_apply(LevD L, Flags F, const double* __restrict__ x, double* __restrict__ y,
                                                  int64_t n) {
-    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t k = gidx();
     if (k < n) y[k] = ll_row(L, F, x, k);
 }
 
 // r = b - LL x
 __global__ __launch_bounds__(NT) void k_ll_resid(LevD L, Flags F, const double* __restrict__ x,
                                                  const double* __restrict__ b, double* __restrict__ r, int64_t n) {
-    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t k = gidx();
     if (k < n) r[k] = b[k] - ll_row(L, F, x, k);
 }
 
@@ -207,7 +215,7 @@ __device__ __forceinline__ void jacobi_at(const LevD& L, const Flags& F, const d
 __global__ __launch_bounds__(NT) void k_jacobi(LevD L, Flags F, const double* __restrict__ x,
                                                const double* __restrict__ b, double* __restrict__ xo, double omega,
                                                int fine, int zero_guess) {
-    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t k = gidx();
     if (k < L.nW() + L.nU()) jacobi_at(L, F, x, b, xo, omega, fine, zero_guess, k);
 }
 
@@ -222,7 +230,7 @@ __device__ __forceinline__ void vel_resid_at(const LevD& L, const Flags& F, cons
 __global__ __launch_bounds__(NT) void k_jacobi0_grad(LevD L, Flags F, const double* __restrict__ rV,
                                                      const double* __restrict__ zP, double* __restrict__ b,
                                                      double* __restrict__ xo, double omega) {
-    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t k = gidx();
     const int64_t nW = L.nW(), n = nW + L.nU();
     if (k >= n) return;
     const int Nz = L.Nz, Nx = L.Nx;
@@ -248,7 +256,7 @@ __global__ __launch_bounds__(NT) void k_jacobi0_grad(LevD L, Flags F, const doub
 }
 __global__ __launch_bounds__(NT) void k_vel_resid(LevD L, Flags F, const double* __restrict__ x,
                                                   const double* __restrict__ b, double* __restrict__ r, int fine) {
-    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t k = gidx();
     if (k < L.nW() + L.nU()) vel_resid_at(L, F, x, b, r, fine, k);
 }
 
@@ -279,7 +287,7 @@ __device__ __forceinline__ void restrict_at(const LevD& Lf, const LevD& Lc, cons
 }
 __global__ __launch_bounds__(NT) void k_restrict(LevD Lf, LevD Lc, const double* __restrict__ rf,
                                                  double* __restrict__ bcv) {
-    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t k = gidx();
     if (k < Lc.nW() + Lc.nU()) restrict_at(Lf, Lc, rf, bcv, k);
 }
 
@@ -326,7 +334,7 @@ __device__ __forceinline__ void prolong_at(const LevD& Lf, const LevD& Lc, const
 }
 __global__ __launch_bounds__(NT) void k_prolong(LevD Lf, LevD Lc, Flags F, const double* __restrict__ ec,
                                                 double* __restrict__ xf) {
-    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t k = gidx();
     if (k < Lf.nW() + Lf.nU()) prolong_at(Lf, Lc, F, ec, xf, k);
 }
 
@@ -462,7 +470,7 @@ __global__ __launch_bounds__(TAIL_NT) void k_mg_tail(TailArgs a) {
 // coarse coefficients from fine ones
 __global__ __launch_bounds__(NT) void k_coarsen(LevD Lf, double* eta, double* rho, double* rhow, double* rhou,
                                                 int Nzc, int Nxc) {
-    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t k = gidx();
     const int64_t nmax = (int64_t)(Nzc + 1) * (Nxc + 1);
     if (k >= nmax) return;
     const int J = ((int)k % (Nzc + 1)), I = ((int)k / (Nzc + 1));
@@ -479,7 +487,7 @@ __global__ __launch_bounds__(NT) void k_coarsen(LevD Lf, double* eta, double* rh
 
 // corner viscosity of a coarse level: geometric mean, periodic x / closed z (update.jl:411-426)
 __global__ __launch_bounds__(NT) void k_corner_eta(const double* eta, double* etaco, int Nz, int Nx) {
-    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t k = gidx();
     if (k >= (int64_t)(Nz + 1) * (Nx + 1)) return;
     const int j = ((int)k % (Nz + 1)), i = ((int)k / (Nz + 1));
     const int z0 = j - 1 < 0 ? 0 : j - 1, z1 = j >= Nz ? Nz - 1 : j;
@@ -493,7 +501,7 @@ __global__ __launch_bounds__(NT) void k_corner_eta(const double* eta, double* et
 //   zp = S^-1 rp (interior: eta/rho; pins: see below), then rv' = rv - GG zp
 __global__ __launch_bounds__(NT) void k_schur_int(LevD L, Flags F, const double* __restrict__ rP,
                                                   double* __restrict__ zP) {
-    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t k = gidx();
     const int Nz = L.Nz, Nx = L.Nx;
     if (k >= (int64_t)(Nz + 2) * (Nx + 2)) return;
     const int r = ((int)k % (Nz + 2)), c = ((int)k / (Nz + 2));
@@ -506,7 +514,7 @@ __global__ __launch_bounds__(NT) void k_schur_int(LevD L, Flags F, const double*
 // ghost rows of KP solved exactly given the interior values (fluidmech.jl:405-417)
 __global__ __launch_bounds__(NT) void k_schur_ghost(LevD L, Flags F, const double* __restrict__ rP,
                                                     double* __restrict__ zP) {
-    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t k = gidx();
     const int Nz = L.Nz, Nx = L.Nx;
     if (k >= (int64_t)(Nz + 2) * (Nx + 2)) return;
     const int r = ((int)k % (Nz + 2)), c = ((int)k / (Nz + 2));
@@ -530,7 +538,7 @@ __global__ __launch_bounds__(NT) void k_schur_ghost(LevD L, Flags F, const doubl
 }
 __global__ __launch_bounds__(NT) void k_sub_grad(LevD L, Flags F, const double* __restrict__ rV,
                                                  const double* __restrict__ zP, double* __restrict__ tV) {
-    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t k = gidx();
     const int64_t nW = L.nW(), n = nW + L.nU();
     if (k >= n) return;
     const int Nz = L.Nz, Nx = L.Nx;
@@ -746,20 +754,20 @@ __global__ __launch_bounds__(64) void k_eta_rho_final(const double* __restrict__
 // w -= sum_q c[q] V[q]
 __global__ __launch_bounds__(NT) void k_orth(const double* __restrict__ V, int64_t n, int K, const double* __restrict__ c,
                                              double* __restrict__ w) {
-    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t i = gidx();
     if (i >= n) return;
     double s = w[i];
     for (int q = 0; q < K; ++q) s -= c[q] * V[(int64_t)q * n + i];
     w[i] = s;
 }
 __global__ __launch_bounds__(NT) void k_scale(const double* __restrict__ a, double s, double* __restrict__ b, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t i = gidx();
     if (i < n) b[i] = a[i] * s;
 }
 // x += sum_q y[q] Z[q]
 __global__ __launch_bounds__(NT) void k_update_x(const double* __restrict__ Z, int64_t n, int K,
                                                  const double* __restrict__ y, double* __restrict__ x) {
-    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t i = gidx();
     if (i >= n) return;
     double s = x[i];
     for (int q = 0; q < K; ++q) s += y[q] * Z[(int64_t)q * n + i];
@@ -855,7 +863,7 @@ struct LcProArgs {
     int64_t n;
 };
 __global__ __launch_bounds__(NT) void k_lc_prolong(LcProArgs a) {
-    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t k = gidx();
     if (k >= a.n) return;
     const LevD& L = a.L;
     const int Nz = L.Nz, Nx = L.Nx;
@@ -938,7 +946,7 @@ __global__ void k_arnoldi(Kry K, int m, int j, const double* __restrict__ d1, co
 __global__ __launch_bounds__(NT) void k_orth_scale(const double* __restrict__ V, int64_t n, int K,
                                                    const double* __restrict__ c, const double* __restrict__ w,
                                                    const double* __restrict__ hp, double* __restrict__ v) {
-    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t i = gidx();
     if (i >= n) return;
     double s = w[i];
     for (int q = 0; q < K; ++q) s -= c[q] * V[(int64_t)q * n + i];
@@ -954,29 +962,29 @@ __global__ void k_trisolve(Kry K, int m, int j) {
     }
 }
 __global__ __launch_bounds__(NT) void k_zero(double* a, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t i = gidx();
     if (i < n) a[i] = 0.0;
 }
 // y += x
 __global__ __launch_bounds__(NT) void k_axpy1(const double* __restrict__ x, double* __restrict__ y, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t i = gidx();
     if (i < n) y[i] += x[i];
 }
 __global__ __launch_bounds__(NT) void k_copyv(const double* __restrict__ a, double* __restrict__ b, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t i = gidx();
     if (i < n) b[i] = a[i];
 }
 
 // ---- state <-> stacked vector
 __global__ __launch_bounds__(NT) void k_gather(xc_mat W, xc_mat U, xc_mat P, double* x, int Nz, int Nx) {
-    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t k = gidx();
     const int64_t nW = (int64_t)(Nz + 1) * (Nx + 2), nU = (int64_t)(Nz + 2) * (Nx + 1), nP = (int64_t)(Nz + 2) * (Nx + 2);
     if (k < nW) x[k] = W.p[(k % (Nz + 1)) + (k / (Nz + 1)) * W.ld];
     else if (k < nW + nU) { const int64_t q = k - nW; x[k] = U.p[(q % (Nz + 2)) + (q / (Nz + 2)) * U.ld]; }
     else if (k < nW + nU + nP) { const int64_t q = k - nW - nU; x[k] = P.p[(q % (Nz + 2)) + (q / (Nz + 2)) * P.ld]; }
 }
 __global__ __launch_bounds__(NT) void k_scatter(const double* x, xc_mat W, xc_mat U, xc_mat P, int Nz, int Nx) {
-    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t k = gidx();
     const int64_t nW = (int64_t)(Nz + 1) * (Nx + 2), nU = (int64_t)(Nz + 2) * (Nx + 1), nP = (int64_t)(Nz + 2) * (Nx + 2);
     if (k < nW) W.p[(k % (Nz + 1)) + (k / (Nz + 1)) * W.ld] = x[k];
     else if (k < nW + nU) { const int64_t q = k - nW; U.p[(q % (Nz + 2)) + (q / (Nz + 2)) * U.ld] = x[k]; }
@@ -984,7 +992,7 @@ __global__ __launch_bounds__(NT) void k_scatter(const double* x, xc_mat W, xc_ma
 }
 // dense copy of a strided matrix
 __global__ __launch_bounds__(NT) void k_pack(xc_mat a, double* out) {
-    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t k = gidx();
     if (k >= a.rows * a.cols) return;
     out[k] = a.p[(k % a.rows) + (k / a.rows) * a.ld];
 }
@@ -996,7 +1004,7 @@ struct RhsArgs {
     int do_mfs;
 };
 __global__ __launch_bounds__(NT) void k_mfs(RhsArgs a, int Nz, int Nx, double* part) {
-    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t k = gidx();
     double v = 0.0;
     if (k < (int64_t)Nz * Nx) {
         const int j = ((int)k % Nz), i = ((int)k / Nz);
@@ -1036,7 +1044,7 @@ __global__ __launch_bounds__(NT) void k_sum_final(const double* part, int nb, do
 // momentum-advection inputs (fluidmech.jl:236-238, 310-312)
 __global__ __launch_bounds__(NT) void k_mom_prep(xc_mat W, xc_mat U, xc_mat rhow, xc_mat rhou, double* fz, double* uz,
                                                  double* wz, double* fx, double* ux, double* wx, int Nz, int Nx) {
-    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t k = gidx();
     auto Wv = [&](int r, int c) { return W.p[r + (int64_t)c * W.ld]; };
     auto Uv = [&](int r, int c) { return U.p[r + (int64_t)c * U.ld]; };
     auto ifx = [&](int kk) { return kk == 0 ? Nx - 1 : (kk == Nx + 2 ? 1 : kk - 1); };
@@ -1077,7 +1085,7 @@ struct RRArgs {
 };
 __global__ __launch_bounds__(NT) void k_rr(RRArgs a, int Nz, int Nx, double* RR) {
 #pragma clang fp contract(off)
-    const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t k = gidx();
     const int64_t nW = (int64_t)(Nz + 1) * (Nx + 2), nU = (int64_t)(Nz + 2) * (Nx + 1), nP = (int64_t)(Nz + 2) * (Nx + 2);
     if (k >= nW + nU + nP) return;
     auto at = [](const xc_mat& m, int r, int c) { return m.p[r + (int64_t)c * m.ld]; };
