@@ -236,8 +236,9 @@ typedef struct {
     double init_rel_resid; /* the same for the initial guess (previous W, U, P) */
     int mg_levels;         /* multigrid levels the velocity preconditioner's V-cycle used */
     int fallback_used;     /* 1 if the learned coarse correction was rejected (plain multigrid used) */
-    double inertia_ratio;  /* a1 min(rho) h^2 / (dt max(eta)) on the finest level (0 without inertia): the
-                              V-cycle stops at the first level l with 4^l times it >= 6 (mg_levels == 0),
+    double inertia_ratio;  /* a1 min(rho) h^2 / (dt max(eta)) on the finest level (0 without inertia or when
+                              mg_levels is set): the
+                              V-cycle stops at the first level l >= 1 with 4^l times it >= 6 (mg_levels == 0),
                               where two damped-Jacobi sweeps cut the smoothest error mode 4x */
 } xc_solve_stats;
 

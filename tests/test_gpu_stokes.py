@@ -5,6 +5,7 @@ LL x and RR to 1e-12 (normwise); the FGMRES solution has relative residual
 ||RR - LL x|| / ||RR|| <= 1e-8 on the oracle's own assembled system, and the
 MMS convergence orders of test_mms.jl (> 1.8) hold on the device path.
 """
+import copy
 import math
 
 import numpy as np
@@ -120,3 +121,41 @@ def test_noconv_reports(xc, orc):
     assert not stats["converged"] and stats["iters"] == 3
     with pytest.raises(xc.NoConvergence):
         xc.fluidmech_(dfl, g, par, bnchm_data=_dev_bnchm(xc, m), tol=1e-14, maxit=2, **kw)
+
+
+def test_vcycle_depth_rule(xc, orc):
+    # with inertia the V-cycle stops at the first level l >= 1 with 4^l R0 >= 6 (R0 = a1 min(rho) h^2 /
+    # (dt max(eta))); the solution meets the same bar against the oracle's assembled system whatever the depth
+    Nz, Nx = 64, 64
+    par, g, sc, ph, fl, ns = plausible_state(orc, xc, Nz, Nx, seed=9)
+    orc.update(ph, fl, g, par, sc)
+    orc.phsevo(ph, fl, g, par, sc, coef=orc.time_coefs("bd2im", 3), dt=sc.dt0)
+    kw = dict(dt=sc.dt0, a1=1.5, a2=2.0, a3=-0.5, b1=1, b2=0, b3=0)
+    # the assembly updates MFS in the state it is given (fluidmech.jl:115-127): assemble on a copy
+    LL, RR = orc.fluidmech_assemble(copy.deepcopy(fl), copy.deepcopy(ph), g, orc.fm_opts(par, **kw))
+    R0 = 1.5 * fl.rho.min() * g.h ** 2 / (sc.dt0 * fl.eta.max())
+    full = 5  # 64 -> 32 -> 16 -> 8 -> 4
+    want = 2  # at least two levels
+    while want < full and R0 * 4.0 ** (want - 1) < 6.0:
+        want += 1
+    for levels in (0, 1, 3, full):
+        dph, dfl = to_device_state(xc, ph, "phase"), to_device_state(xc, fl, "fluid")
+        stats = xc.fluidmech_(dfl, g, par, phase=dph, tol=1e-8, mg_levels=levels, **kw)
+        print(f"mg_levels={levels}: {stats} (R0 {R0:.3g}, rule {want})", flush=True)
+        assert stats["converged"]
+        if levels == 0:
+            assert stats["mg_levels"] == want
+            assert abs(stats["inertia_ratio"] - R0) <= 1e-12 * R0
+        else:
+            assert stats["mg_levels"] == levels
+        sol = np.concatenate([xc.to_host(getattr(dfl, k)).ravel(order="F") for k in ("W", "U", "P")])
+        assert np.linalg.norm(LL @ sol - RR) / np.linalg.norm(RR) <= 1.5e-8
+
+
+def test_vcycle_full_depth_without_inertia(xc, orc):
+    m, par, g, fl, kw = _mms(orc, xc, 32)
+    dfl = to_device_state(xc, fl, "fluid")
+    stats = xc.fluidmech_(dfl, g, par, bnchm_data=_dev_bnchm(xc, m), tol=1e-8, **kw)
+    assert stats["converged"] and stats["mg_levels"] == 4 and stats["inertia_ratio"] == 0.0
+    with pytest.raises(xc.XCoreError):
+        xc.fluidmech_(dfl, g, par, bnchm_data=_dev_bnchm(xc, m), tol=1e-8, mg_levels=-1, **kw)

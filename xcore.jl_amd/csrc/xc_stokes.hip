@@ -1408,9 +1408,12 @@ void build_levels(xc_ctx* ctx, StokesCtx* s, const xc_fluid* fl, double h) {
 // eigenvalue R eta / h^2, so a damped-Jacobi sweep scales that mode by 1 - omega R / (R + 7/3): where R is
 // large the smoother alone removes the smooth error and coarser levels add work but not convergence.  On
 // level l (spacing 2^l h) R_l = 4^l R_0 with R_0 = a1 min(rho) h^2 / (dt max(eta)) (the least inertial
-// cell); the cycle stops at the first level where two sweeps reduce the smoothest mode at least 4x
-// (omega R / (R + 7/3) >= 1/2, R >= 2.5 * 7/3, MG_INERTIA_STOP).  Without inertia (dt = Inf, the
-// benchmark mode) every level is kept.  mg_levels > 0 in the options caps the depth instead.
+// cell); the cycle stops at the first level below the finest where two sweeps reduce the smoothest
+// mode at least 4x (omega R / (R + 7/3) >= 1/2, R >= 2.5 * 7/3, MG_INERTIA_STOP).  It keeps at least
+// two levels: with the finest alone that level would be the coarsest and lose its pre- and post-sweeps
+// (22-26 FGMRES iterations instead of 15-16 on the 32 x 40 and 64^2 production systems of the tests).
+// Without inertia (dt = Inf, the benchmark mode) every level is kept.  mg_levels > 0 in the options caps
+// the depth instead.
 constexpr double MG_INERTIA_STOP = 6.0;
 // the reduction choose_depth needs, enqueued (results in s->host[200..201] once the stream has reached it);
 // returns whether it was needed
@@ -1438,8 +1441,8 @@ int choose_depth(xc_ctx* ctx, StokesCtx* s, const Flags& F, const xc_fm_opts* o,
     if (!(emax > 0) || !(rmin > 0)) return XC_OK;
     const double R0 = F.a1 * rmin * h * h / (F.dt * emax);
     s->inertia = R0;
-    int L = 1;
-    for (double R = R0; L < nl && R < MG_INERTIA_STOP; R *= 4.0) ++L;
+    int L = nl < 2 ? nl : 2;  // level 0 keeps its pre- and post-smoothing, level 1 costs a quarter of it
+    for (double R = 4.0 * R0; L < nl && R < MG_INERTIA_STOP; R *= 4.0) ++L;
     s->leff = L;
     return XC_OK;
 }
