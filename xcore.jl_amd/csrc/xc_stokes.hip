@@ -751,15 +751,6 @@ __global__ __launch_bounds__(64) void k_eta_rho_final(const double* __restrict__
     }
     if (threadIdx.x == 0) { out[0] = emax; out[1] = rmin; }
 }
-// w -= sum_q c[q] V[q]
-__global__ __launch_bounds__(NT) void k_orth(const double* __restrict__ V, int64_t n, int K, const double* __restrict__ c,
-                                             double* __restrict__ w) {
-    const int64_t i = gidx();
-    if (i >= n) return;
-    double s = w[i];
-    for (int q = 0; q < K; ++q) s -= c[q] * V[(int64_t)q * n + i];
-    w[i] = s;
-}
 __global__ __launch_bounds__(NT) void k_scale(const double* __restrict__ a, double s, double* __restrict__ b, int64_t n) {
     const int64_t i = gidx();
     if (i < n) b[i] = a[i] * s;
@@ -907,50 +898,140 @@ struct Kry {
 __global__ void k_kry_init(Kry K, int m, double beta) {
     for (int i = threadIdx.x; i <= m; i += blockDim.x) K.g[i] = i == 0 ? beta : 0.0;
 }
-// column j of H from the two Gram-Schmidt passes and the norm; Givens update;
-// |g[j+1]| and H[j+1,j] go to host-visible memory for the convergence test
-// d1, d2: the two Gram-Schmidt passes' coefficients; d2[j+1] = |w'|^2 after the first pass, so the
-// norm after the second is |w''|^2 = |w'|^2 - sum d2^2 (the basis is orthonormal); no third pass.
-__global__ void k_arnoldi(Kry K, int m, int j, const double* __restrict__ d1, const double* __restrict__ d2,
-                          double* __restrict__ nrm2, double* __restrict__ host_out) {
-    if (threadIdx.x != 0) return;
-    double* Hj = K.H + (int64_t)j * (m + 1);
-    double ss = 0.0;
-    for (int i = 0; i <= j; ++i) {
-        Hj[i] = d1[i] + d2[i];
-        ss += d2[i] * d2[i];
+// ---- Arnoldi with delayed reorthogonalisation (DCGS2, classical Gram-Schmidt twice with the second
+// pass of v_j folded into step j+1: Bielich, Langou, Thomas, Swirydowicz, Yamazaki, Boman, "Low-synch
+// Gram-Schmidt with delayed reorthogonalization for Krylov solvers", Parallel Computing 2022).
+// Step j holds the provisional vector p_j (one Gram-Schmidt pass, ~unit norm) in basis slot j and
+// w = LL M^-1 p_j.  One reduction gives a = V_j^T p_j, b = V_j^T w, p.p, p.w, w.w (V_j = v_0..v_{j-1},
+// orthonormal); then v_j = (p_j - V_j a) / rho with rho^2 = p.p - |a|^2 (p_j's second pass), column j-1
+// of H is final (its last vector is now v_j), and w's projections give the provisional column j and
+// p_{j+1} = (w - V_j b - h_jj v_j) / nu_{j+1}.  FGMRES needs only LL z_j in span(V): z_j = M^-1 p_j is a
+// direction like any other.
+//
+// partials: q < K: V_q.p (row q) and V_q.w (row K + q); then p.p, p.w, w.w (rows 2K, 2K+1, 2K+2)
+__global__ __launch_bounds__(NT) void k_multidot2(const double* __restrict__ V, int64_t n, int K,
+                                                  const double* __restrict__ p, const double* __restrict__ w,
+                                                  double* __restrict__ part) {
+    __shared__ double red[NT / 64][3];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int q = 0; q <= K; ++q) {
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+            const double pi = p[i], wi = w[i];
+            if (q < K) {
+                const double v = V[(int64_t)q * n + i];
+                s0 += v * pi;
+                s1 += v * wi;
+            } else {
+                s0 += pi * pi;
+                s1 += pi * wi;
+                s2 += wi * wi;
+            }
+        }
+        s0 = wave_sum(s0);
+        s1 = wave_sum(s1);
+        if (q == K) s2 = wave_sum(s2);
+        if (lane == 0) { red[wave][0] = s0; red[wave][1] = s1; red[wave][2] = s2; }
+        __syncthreads();
+        if (threadIdx.x < 3 && (q == K || threadIdx.x < 2)) {
+            const int t = threadIdx.x;
+            const double v = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+            const int row = q < K ? (t == 0 ? q : K + q) : 2 * K + t;
+            part[(int64_t)row * gridDim.x + blockIdx.x] = v;
+        }
+        __syncthreads();
     }
-    const double h2 = d2[j + 1] - ss;
-    const double hn = h2 > 0.0 ? sqrt(h2) : 0.0;
-    nrm2[1] = hn;  // the new basis vector is w'' / hn (H[j+1, j] is rotated to zero below)
-    Hj[j + 1] = hn;
+}
+// rotate column Hc (length j + 2, entry j + 1 = the subdiagonal) by the rotations 0..j-1, then form
+// rotation j and apply it to g; Hc[j + 1] becomes 0
+__device__ void givens_column(Kry K, int j, double* Hc) {
     for (int i = 0; i < j; ++i) {
-        const double a = Hj[i], b = Hj[i + 1];
-        Hj[i] = K.cs[i] * a + K.sn[i] * b;
-        Hj[i + 1] = -K.sn[i] * a + K.cs[i] * b;
+        const double a = Hc[i], b = Hc[i + 1];
+        Hc[i] = K.cs[i] * a + K.sn[i] * b;
+        Hc[i + 1] = -K.sn[i] * a + K.cs[i] * b;
     }
-    const double a = Hj[j], b = Hj[j + 1];
+    const double a = Hc[j], b = Hc[j + 1];
     const double rr = sqrt(a * a + b * b);
     const double c = rr > 0 ? a / rr : 1.0, sv = rr > 0 ? b / rr : 0.0;
     K.cs[j] = c;
     K.sn[j] = sv;
-    Hj[j] = rr;
-    Hj[j + 1] = 0.0;
+    Hc[j] = rr;
+    Hc[j + 1] = 0.0;
     const double gj = K.g[j];
     K.g[j + 1] = -sv * gj;
     K.g[j] = c * gj;
-    host_out[2 * j] = fabs(K.g[j + 1]);
-    host_out[2 * j + 1] = hn;
 }
-// second Gram-Schmidt pass and normalisation in one sweep: v = (w - sum_q c[q] V[q]) / hn
-__global__ __launch_bounds__(NT) void k_orth_scale(const double* __restrict__ V, int64_t n, int K,
-                                                   const double* __restrict__ c, const double* __restrict__ w,
-                                                   const double* __restrict__ hp, double* __restrict__ v) {
+// d: the reduction of step j (a[0..j), b[0..j), p.p, p.w, w.w).  cf: 1/rho, h_jj, 1/nu_{j+1} for the
+// update kernel; cf[3] carries nu from step to step.  host_out[2j], [2j+1]: the provisional |g[j+1]| and
+// nu_{j+1} (the convergence and breakdown tests read them).
+__global__ void k_arnoldi2(Kry K, int m, int j, const double* __restrict__ d, double* __restrict__ cf,
+                           double* __restrict__ host_out) {
+    if (threadIdx.x != 0) return;
+    const double* a = d;
+    const double* b = d + j;
+    const double pp = d[2 * j], pw = d[2 * j + 1], ww = d[2 * j + 2];
+    double rho = 1.0;  // v_0 = r / beta is final as it is
+    if (j > 0) {
+        double sa = 0.0;
+        for (int q = 0; q < j; ++q) sa += a[q] * a[q];
+        const double r2 = pp - sa;
+        rho = r2 > 0.0 ? sqrt(r2) : 0.0;
+        // column j-1: LL z_{j-1} = V_{j-1} h + nu_j p_j = V_j (h + nu_j a) + nu_j rho v_j
+        const double nu = cf[3];
+        double* Hc = K.H + (int64_t)(j - 1) * (m + 1);
+        for (int q = 0; q < j; ++q) Hc[q] += nu * a[q];
+        Hc[j] = nu * rho;
+        givens_column(K, j - 1, Hc);
+    }
+    double ab = 0.0, bb = 0.0;
+    for (int q = 0; q < j; ++q) {
+        ab += a[q] * b[q];
+        bb += b[q] * b[q];
+    }
+    const double hjj = rho > 0.0 ? (pw - ab) / rho : 0.0;
+    double* Hj = K.H + (int64_t)j * (m + 1);
+    for (int q = 0; q < j; ++q) Hj[q] = b[q];
+    Hj[j] = hjj;
+    const double n2 = ww - bb - hjj * hjj;
+    const double nu = n2 > 0.0 ? sqrt(n2) : 0.0;
+    Hj[j + 1] = nu;  // provisional (corrected by step j + 1)
+    // provisional residual estimate: column j rotated on a copy (K.y is free until the cycle's solve)
+    double* t = K.y;
+    for (int q = 0; q <= j + 1; ++q) t[q] = Hj[q];
+    for (int i = 0; i < j; ++i) {
+        const double x = t[i], y = t[i + 1];
+        t[i] = K.cs[i] * x + K.sn[i] * y;
+        t[i + 1] = -K.sn[i] * x + K.cs[i] * y;
+    }
+    const double rr = sqrt(t[j] * t[j] + nu * nu);
+    host_out[2 * j] = rr > 0 ? fabs(nu / rr * K.g[j]) : 0.0;
+    host_out[2 * j + 1] = nu;
+    cf[0] = rho > 0.0 ? 1.0 / rho : 0.0;
+    cf[1] = hjj;
+    cf[2] = nu > 0.0 ? 1.0 / nu : 0.0;
+    cf[3] = nu;
+}
+// the last column of a full cycle, with its provisional subdiagonal as final
+__global__ void k_arnoldi_close(Kry K, int m, int j) {
+    if (threadIdx.x != 0) return;
+    givens_column(K, j, K.H + (int64_t)j * (m + 1));
+}
+// v_j = (p_j - sum_q a_q V_q) / rho in place, p_{j+1} = (w - sum_q b_q V_q - h_jj v_j) / nu_{j+1}
+__global__ __launch_bounds__(NT) void k_dcgs_update(const double* __restrict__ V, int64_t n, int K,
+                                                    const double* __restrict__ d, const double* __restrict__ cf,
+                                                    double* vj, const double* __restrict__ w,
+                                                    double* __restrict__ pnext) {
     const int64_t i = gidx();
     if (i >= n) return;
-    double s = w[i];
-    for (int q = 0; q < K; ++q) s -= c[q] * V[(int64_t)q * n + i];
-    v[i] = s / hp[0];
+    double sa = 0.0, sb = 0.0;
+    for (int q = 0; q < K; ++q) {
+        const double v = V[(int64_t)q * n + i];
+        sa += d[q] * v;
+        sb += d[K + q] * v;
+    }
+    const double v = (vj[i] - sa) * cf[0];
+    vj[i] = v;
+    pnext[i] = (w[i] - sb - cf[1] * v) * cf[2];
 }
 // y = H(0:j, 0:j)^-1 g(0:j) (upper triangular after the rotations)
 __global__ void k_trisolve(Kry K, int m, int j) {
@@ -1151,7 +1232,7 @@ struct StokesCtx {
     double* host = nullptr;   // pinned host scratch for reductions (mapped, coherent)
     double* host_dev = nullptr;  // the same memory as seen by kernels
     double* kry = nullptr;    // Arnoldi H, Givens, g, y (device)
-    double* d3 = nullptr;     // three dot-product result slots of (m + 2) each
+    double* d3 = nullptr;     // dot-product results: 2m + 3 of one Arnoldi step, then 4 coefficients
     hipEvent_t ev[2] = {nullptr, nullptr};
     // pressure Poisson (inertial Schur part)
     int pK = 0;                 // Nx/2 + 1
@@ -1280,10 +1361,10 @@ int setup(xc_ctx* ctx, int Nz, int Nx, int m, StokesCtx** out, int maxlev = -1) 
     const int K = Nx / 2 + 1;
     const size_t lcc = (size_t)(Nz / 2) * (Nx / 2);  // learned correction at level >= 1
     tot += 3 * lcc + 16 + (lcc + NT - 1) / NT + 8;
-    tot += 4 * (size_t)n + (size_t)(m + 1) * n + (size_t)m * n + (size_t)(m + 2) * DOT_BLOCKS + 128 /* dots */ +
+    tot += 4 * (size_t)n + (size_t)(m + 1) * n + (size_t)m * n + (size_t)(2 * m + 4) * DOT_BLOCKS + 128 /* dots */ +
            (size_t)n /* tmp */ + 8 * (size_t)(Nz + 2) * (Nx + 3) /* advection scratch */ +
            2 * (size_t)Nz * K + 2 * (size_t)Nx + 4 * (size_t)Nz + K + 2 * (size_t)Nz + 16 /* Poisson */ +
-           (size_t)(m + 1) * m + 4 * (size_t)m + 8 /* Arnoldi */ + 3 * (size_t)(m + 2) /* dots */;
+           (size_t)(m + 1) * m + 4 * (size_t)m + 8 /* Arnoldi */ + 4 * (size_t)(m + 2) /* dots */;
     void* pool;
     hipError_t e = hipMalloc(&pool, tot * sizeof(double));
     if (e != hipSuccess) { delete s; return xc_fail(ctx, XC_ENOMEM, "fluidmech!: %zu bytes: %s", tot * 8, hipGetErrorString(e)); }
@@ -1307,7 +1388,7 @@ int setup(xc_ctx* ctx, int Nz, int Nx, int m, StokesCtx** out, int maxlev = -1) 
     s->xs = take(n); s->bs = take(n); s->rs = take(n); s->ws = take(n);
     s->V = take((size_t)(m + 1) * n);
     s->Z = take((size_t)m * n);
-    s->part = take((size_t)(m + 2) * DOT_BLOCKS);
+    s->part = take((size_t)(2 * m + 4) * DOT_BLOCKS);
     s->dots = take(128);
     s->tmp = take(n);
     s->adv_buf = take(8 * (size_t)(Nz + 2) * (Nx + 3));
@@ -1316,7 +1397,7 @@ int setup(xc_ctx* ctx, int Nz, int Nx, int m, StokesCtx** out, int maxlev = -1) 
     s->lc_part = take(lcc / NT + 8 + (lcc + NT - 1) / NT);
     s->lc_cap = (int)lcc;
     s->kry = take((size_t)(m + 1) * m + 4 * (size_t)m + 8);
-    s->d3 = take(3 * (size_t)(m + 2));
+    s->d3 = take(4 * (size_t)(m + 2));
     if ((p - s->pool) & 1) ++p;  // 16-byte alignment for the complex arrays
     s->pK = K;
     s->pS = reinterpret_cast<double2*>(take(2 * (size_t)Nz * K));
@@ -1575,11 +1656,6 @@ int multidot(xc_ctx* ctx, StokesCtx* s, const double* V, int K, const double* w,
     return XC_OK;
 }
 
-// out[0..K] = (V[q] . w) for q < K and w.w, left on the device
-void multidot_dev(xc_ctx* ctx, StokesCtx* s, const double* V, int K, const double* w, int64_t n, double* out) {
-    hipLaunchKernelGGL(k_multidot, dim3(DOT_BLOCKS), dim3(NT), 0, ctx->stream, V, n, K, w, s->part);
-    hipLaunchKernelGGL(k_dot_final, dim3((K + 4) / 4), dim3(NT), 0, ctx->stream, s->part, DOT_BLOCKS, K + 1, out);
-}
 
 int build_rhs(xc_ctx* ctx, StokesCtx* s, xc_fluid* fl, const xc_phase* ph, const xc_grid* g, const xc_par* par,
               const xc_fm_opts* o, const xc_bnchm* bn, const Flags& F, double* RR) {
@@ -1738,10 +1814,9 @@ int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int 
     K.sn = K.cs + m;
     K.g = K.sn + m;
     K.y = K.g + m + 1;
-    double* d1 = s->d3;
-    double* d2 = d1 + (m + 2);
-    double* dn = d2 + (m + 2);
-    volatile double* hout = s->host + 8;  // 2 per step: |g[j+1]|, H[j+1, j]
+    double* dd1 = s->d3;                  // step reductions (2m + 3)
+    double* cf = s->d3 + 3 * (m + 2);     // update coefficients and nu
+    volatile double* hout = s->host + 8;  // 2 per step: provisional |g[j+1]|, nu_{j+1}
     while (true) {
         hipLaunchKernelGGL(k_ll_resid, dim3(nb), dim3(NT), 0, ctx->stream, dd, F, x, b, s->rs, n);
         if ((st = multidot(ctx, s, s->rs, 0, s->rs, n))) return st;
@@ -1756,29 +1831,30 @@ int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int 
         int used = 0;  // basis columns whose results are final
         int j = 0;
         for (; j < m; ++j) {
-            double* vj = s->V + (int64_t)j * n;
+            double* vj = s->V + (int64_t)j * n;  // p_j, provisional until this step's update
             double* zj = s->Z + (int64_t)j * n;
             if ((st = precond_full(ctx, s, F, mo, n, lc, vj, zj))) return st;
             hipLaunchKernelGGL(k_ll_apply, dim3(nb), dim3(NT), 0, ctx->stream, dd, F, zj, s->ws, n);
-            multidot_dev(ctx, s, s->V, j + 1, s->ws, n, d1);
-            hipLaunchKernelGGL(k_orth, dim3(nb), dim3(NT), 0, ctx->stream, s->V, n, j + 1, d1, s->ws);
-            multidot_dev(ctx, s, s->V, j + 1, s->ws, n, d2);
-            hipLaunchKernelGGL(k_arnoldi, dim3(1), dim3(64), 0, ctx->stream, K, m, j, d1, d2, dn, s->host_dev + 8);
-            if (j + 1 < m)
-                hipLaunchKernelGGL(k_orth_scale, dim3(nb), dim3(NT), 0, ctx->stream, s->V, n, j + 1, d2, s->ws,
-                                   dn + 1, s->V + (int64_t)(j + 1) * n);
+            hipLaunchKernelGGL(k_multidot2, dim3(DOT_BLOCKS), dim3(NT), 0, ctx->stream, s->V, n, j, vj, s->ws,
+                               s->part);
+            hipLaunchKernelGGL(k_dot_final, dim3((2 * j + 6) / 4), dim3(NT), 0, ctx->stream, s->part, DOT_BLOCKS,
+                               2 * j + 3, dd1);
+            hipLaunchKernelGGL(k_arnoldi2, dim3(1), dim3(64), 0, ctx->stream, K, m, j, dd1, cf, s->host_dev + 8);
+            hipLaunchKernelGGL(k_dcgs_update, dim3(nb), dim3(NT), 0, ctx->stream, s->V, n, j, dd1, cf, vj, s->ws,
+                               s->V + (int64_t)(j + 1) * n);
             XC_TRY_HIP(ctx, hipEventRecord(s->ev[j & 1], ctx->stream));
             if (j == 0) continue;
-            // step j-1 is complete once its event has fired
+            // step j-1 is complete once its event has fired; step j (enqueued) finalises its column
             XC_TRY_HIP(ctx, hipEventSynchronize(s->ev[(j - 1) & 1]));
             used = j;
             ++iters;
             if (hout[2 * (j - 1)] <= 0.5 * target || hout[2 * (j - 1) + 1] == 0.0 || iters >= maxit) break;
         }
-        if (j == m) {  // the last step of the cycle was not tested yet
+        if (j == m) {  // the last step of the cycle was not tested yet; its column keeps the provisional values
             XC_TRY_HIP(ctx, hipEventSynchronize(s->ev[(m - 1) & 1]));
             used = m;
             ++iters;
+            hipLaunchKernelGGL(k_arnoldi_close, dim3(1), dim3(64), 0, ctx->stream, K, m, m - 1);
         }
         hipLaunchKernelGGL(k_trisolve, dim3(1), dim3(64), 0, ctx->stream, K, m, used);
         hipLaunchKernelGGL(k_update_x, dim3(nb), dim3(NT), 0, ctx->stream, s->Z, n, used, K.y, x);
