@@ -963,25 +963,62 @@ __device__ void givens_column(Kry K, int j, double* Hc) {
 }
 // d: the reduction of step j (a[0..j), b[0..j), p.p, p.w, w.w).  cf: 1/rho, h_jj, 1/nu_{j+1} for the
 // update kernel; cf[3] carries nu from step to step.  host_out[2j], [2j+1]: the provisional |g[j+1]| and
-// nu_{j+1} (the convergence and breakdown tests read them).
-__global__ void k_arnoldi2(Kry K, int m, int j, const double* __restrict__ d, double* __restrict__ cf,
-                           double* __restrict__ host_out) {
-    if (threadIdx.x != 0) return;
-    const double* a = d;
-    const double* b = d + j;
-    const double pp = d[2 * j], pw = d[2 * j + 1], ww = d[2 * j + 2];
+// nu_{j+1} (the convergence and breakdown tests read them).  One wave: the lanes stage every input in
+// LDS with their loads in flight together, then lane 0 does the arithmetic (m <= ARN_MMAX).
+constexpr int ARN_MMAX = 64;
+__global__ __launch_bounds__(64) void k_arnoldi2(Kry K, int m, int j, const double* __restrict__ d,
+                                                 double* __restrict__ cf, double* __restrict__ host_out) {
+    __shared__ double sd[2 * ARN_MMAX + 3], sc[ARN_MMAX + 2], scs[ARN_MMAX], ssn[ARN_MMAX], sg[2], snu;
+    const int lane = threadIdx.x;
+    double* Hc = K.H + (int64_t)(j > 0 ? j - 1 : 0) * (m + 1);  // column j - 1
+    for (int i = lane; i < 2 * j + 3; i += 64) sd[i] = d[i];
+    for (int i = lane; i < j; i += 64) {
+        scs[i] = K.cs[i];
+        ssn[i] = K.sn[i];
+    }
+    if (j > 0)
+        for (int i = lane; i < j; i += 64) sc[i] = Hc[i];
+    if (lane == 0) {
+        sg[0] = j > 0 ? K.g[j - 1] : 0.0;
+        sg[1] = K.g[j];
+        snu = cf[3];
+    }
+    __syncthreads();
+    if (lane != 0) return;
+    const double* a = sd;
+    const double* b = sd + j;
+    const double pp = sd[2 * j], pw = sd[2 * j + 1], ww = sd[2 * j + 2];
     double rho = 1.0;  // v_0 = r / beta is final as it is
+    double gj = sg[1];
     if (j > 0) {
         double sa = 0.0;
         for (int q = 0; q < j; ++q) sa += a[q] * a[q];
         const double r2 = pp - sa;
         rho = r2 > 0.0 ? sqrt(r2) : 0.0;
-        // column j-1: LL z_{j-1} = V_{j-1} h + nu_j p_j = V_j (h + nu_j a) + nu_j rho v_j
-        const double nu = cf[3];
-        double* Hc = K.H + (int64_t)(j - 1) * (m + 1);
-        for (int q = 0; q < j; ++q) Hc[q] += nu * a[q];
-        Hc[j] = nu * rho;
-        givens_column(K, j - 1, Hc);
+        // column j-1: LL z_{j-1} = V_{j-1} h + nu_j p_j = V_j (h + nu_j a) + nu_j rho v_j; then its Givens
+        // rotations (0..j-2 applied, j-1 formed)
+        const double nu = snu;
+        for (int q = 0; q < j; ++q) sc[q] += nu * a[q];
+        sc[j] = nu * rho;
+        for (int i = 0; i < j - 1; ++i) {
+            const double x = sc[i], y = sc[i + 1];
+            sc[i] = scs[i] * x + ssn[i] * y;
+            sc[i + 1] = -ssn[i] * x + scs[i] * y;
+        }
+        const double x = sc[j - 1], y = sc[j];
+        const double rr = sqrt(x * x + y * y);
+        const double c = rr > 0 ? x / rr : 1.0, sv = rr > 0 ? y / rr : 0.0;
+        scs[j - 1] = c;
+        ssn[j - 1] = sv;
+        sc[j - 1] = rr;
+        sc[j] = 0.0;
+        for (int q = 0; q <= j; ++q) Hc[q] = sc[q];
+        K.cs[j - 1] = c;
+        K.sn[j - 1] = sv;
+        const double gp = sg[0];
+        gj = -sv * gp;
+        K.g[j] = gj;
+        K.g[j - 1] = c * gp;
     }
     double ab = 0.0, bb = 0.0;
     for (int q = 0; q < j; ++q) {
@@ -989,22 +1026,24 @@ __global__ void k_arnoldi2(Kry K, int m, int j, const double* __restrict__ d, do
         bb += b[q] * b[q];
     }
     const double hjj = rho > 0.0 ? (pw - ab) / rho : 0.0;
-    double* Hj = K.H + (int64_t)j * (m + 1);
-    for (int q = 0; q < j; ++q) Hj[q] = b[q];
-    Hj[j] = hjj;
     const double n2 = ww - bb - hjj * hjj;
     const double nu = n2 > 0.0 ? sqrt(n2) : 0.0;
-    Hj[j + 1] = nu;  // provisional (corrected by step j + 1)
-    // provisional residual estimate: column j rotated on a copy (K.y is free until the cycle's solve)
-    double* t = K.y;
-    for (int q = 0; q <= j + 1; ++q) t[q] = Hj[q];
-    for (int i = 0; i < j; ++i) {
-        const double x = t[i], y = t[i + 1];
-        t[i] = K.cs[i] * x + K.sn[i] * y;
-        t[i + 1] = -K.sn[i] * x + K.cs[i] * y;
+    // the provisional column j (corrected by step j + 1) and its residual estimate, rotated on a copy
+    double* Hj = K.H + (int64_t)j * (m + 1);
+    for (int q = 0; q < j; ++q) {
+        Hj[q] = b[q];
+        sc[q] = b[q];
     }
-    const double rr = sqrt(t[j] * t[j] + nu * nu);
-    host_out[2 * j] = rr > 0 ? fabs(nu / rr * K.g[j]) : 0.0;
+    Hj[j] = hjj;
+    Hj[j + 1] = nu;
+    sc[j] = hjj;
+    for (int i = 0; i < j; ++i) {
+        const double x = sc[i], y = sc[i + 1];
+        sc[i] = scs[i] * x + ssn[i] * y;
+        sc[i + 1] = -ssn[i] * x + scs[i] * y;
+    }
+    const double rr = sqrt(sc[j] * sc[j] + nu * nu);
+    host_out[2 * j] = rr > 0 ? fabs(nu / rr * gj) : 0.0;
     host_out[2 * j + 1] = nu;
     cf[0] = rho > 0.0 ? 1.0 / rho : 0.0;
     cf[1] = hjj;
