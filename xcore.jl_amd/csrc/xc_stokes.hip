@@ -672,6 +672,7 @@ __global__ __launch_bounds__(FFT_NT) void k_pois_xinv(PoisInvArgs a) {
     }
     __syncthreads();
     const double2* Y = lds_fft<true>(A, B, nl, a.p);
+    double* Zs = reinterpret_cast<double*>(Y == A ? B : A);  // rows x n interior results for the ghost ring
     const int Nz = a.Nz;
     for (int k = threadIdx.x; k < rows * n; k += FFT_NT) {
         const int rr = k % rows, c = k / rows;
@@ -683,7 +684,37 @@ __global__ __launch_bounds__(FFT_NT) void k_pois_xinv(PoisInvArgs a) {
         double z;
         if (is_pin(a.F, r + 1, c + 1, Nz)) z = a.F.bnchm ? a.rP[q] : a.rP[q] / (1.0 + sv);
         else z = a.rP[q] / sv;
-        a.zP[q] = z + a.scale * pv;
+        z += a.scale * pv;
+        a.zP[q] = z;
+        Zs[rr * n + c] = z;
+    }
+    __syncthreads();
+    // the ghost ring of zP, as k_schur_ghost computes it (fluidmech.jl:405-417): columns 0 and Nx+1 of
+    // this workgroup's rows; row 0 / Nz+1 when this workgroup holds row 1 / Nz
+    const int Nx = n;
+    auto P = [&](int rr, int c) { return Zs[rr * n + c]; };  // interior (r0 + rr + 1, c + 1)
+    auto rPv = [&](int r, int c) { return a.rP[r + (int64_t)c * (Nz + 2)]; };
+    for (int k = threadIdx.x; k < 2 * rows; k += FFT_NT) {
+        const int rr = k >> 1, r = r0 + rr;
+        if (r >= Nz) continue;
+        const int zr = r + 1, c = (k & 1) ? Nx + 1 : 0;
+        const int64_t q = zr + (int64_t)c * (Nz + 2);
+        a.zP[q] = is_pin(a.F, zr, c, Nz) ? a.rP[q] : a.rP[q] + ((k & 1) ? P(rr, 0) : P(rr, Nx - 1));
+    }
+    for (int side = 0; side < 2; ++side) {
+        const int zr = side ? Nz + 1 : 0;              // ghost row
+        const int rn = side ? Nz : 1;                  // the interior row it reads
+        const int rr = rn - 1 - r0;                    // that row in this workgroup
+        if (rr < 0 || rr >= rows) continue;
+        for (int c = threadIdx.x; c < Nx + 2; c += FFT_NT) {
+            const int64_t q = zr + (int64_t)c * (Nz + 2);
+            if (is_pin(a.F, zr, c, Nz)) { a.zP[q] = a.rP[q]; continue; }
+            double inner;
+            if (c == 0) inner = rPv(rn, 0) + P(rr, Nx - 1);
+            else if (c == Nx + 1) inner = rPv(rn, Nx + 1) + P(rr, 0);
+            else inner = P(rr, c - 1);
+            a.zP[q] = a.rP[q] + inner;
+        }
     }
 }
 
@@ -1668,11 +1699,11 @@ void precond(xc_ctx* ctx, StokesCtx* s, const Flags& F, const double* v, double*
         hipLaunchKernelGGL(k_pois_zfft, dim3((unsigned)((K + nlz - 1) / nlz)), dim3(FFT_NT),
                            2 * (size_t)nlz * 2 * Nz * sizeof(double2), ctx->stream, za);
         PoisInvArgs ia{s->pS, vP, zP, pl, d, F, s->pscale, Nz, Nx, K, nl};
-        hipLaunchKernelGGL(k_pois_xinv, dim3(nbx), dim3(FFT_NT), lds, ctx->stream, ia);
+        hipLaunchKernelGGL(k_pois_xinv, dim3(nbx), dim3(FFT_NT), lds, ctx->stream, ia);  // with the ghost ring
     } else {
         hipLaunchKernelGGL(k_schur_int, dim3(nblk(nP)), dim3(NT), 0, ctx->stream, d, F, vP, zP);
+        hipLaunchKernelGGL(k_schur_ghost, dim3(nblk(nP)), dim3(NT), 0, ctx->stream, d, F, vP, zP);
     }
-    hipLaunchKernelGGL(k_schur_ghost, dim3(nblk(nP)), dim3(NT), 0, ctx->stream, d, F, vP, zP);
     // velocity: b = r_v - GG z_p and the first sweep in one kernel, then the V-cycle.  With an even
     // number of level-0 sweeps the iterates alternate between z and the level's scratch and end in z.
     double* own_x = L.x;
