@@ -96,13 +96,14 @@ def test_mms_orders_on_device(xc, orc):
         assert math.log2(e[0] / e[1]) > 1.8 and math.log2(e[1] / e[2]) > 1.8, (k, e)
 
 
-@pytest.mark.parametrize("shape", [(32, 40), (64, 64)])
-def test_production_solve(xc, orc, shape):
+@pytest.mark.parametrize("shape,open_cnv", [((32, 40), False), ((64, 64), False), ((48, 64), True)])
+def test_production_solve(xc, orc, shape, open_cnv):
+    # open_cnv pins the pressure on row Nz, ghost columns included (the Schur solve's ghost ring)
     Nz, Nx = shape
     par, g, sc, ph, fl, ns = plausible_state(orc, xc, Nz, Nx, seed=7)
     orc.update(ph, fl, g, par, sc)
     orc.phsevo(ph, fl, g, par, sc, coef=orc.time_coefs("bd2im", 3), dt=sc.dt0)
-    kw = dict(dt=sc.dt0, a1=1.5, a2=2.0, a3=-0.5, b1=1, b2=0, b3=0)
+    kw = dict(dt=sc.dt0, a1=1.5, a2=2.0, a3=-0.5, b1=1, b2=0, b3=0, open_cnv=open_cnv)
     dph, dfl = to_device_state(xc, ph, "phase"), to_device_state(xc, fl, "fluid")
     stats = xc.fluidmech_(dfl, g, par, phase=dph, tol=1e-8, **kw)
     print(f"production {shape}: {stats}", flush=True)
