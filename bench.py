@@ -134,6 +134,9 @@ def step_bench(args, learned=False):
             "stokes_fallbacks": sum(r["stokes_fallbacks"] for r in log),
             "vcycle_levels": [min(r["stokes_mg_levels"] for r in log), max(r["stokes_mg_levels"] for r in log)],
             "inertia_ratio": [min(r["stokes_inertia_ratio"] for r in log), max(r["stokes_inertia_ratio"] for r in log)],
+            "fgmres_iters_first_sweep": sum(r["stokes_iters_first"] for r in log) / len(log),
+            "init_rel_resid_first_sweep": max(r["stokes_init_rel_first"] for r in log),
+            "init_rel_resid_later_sweeps": max(r["stokes_init_rel_later"] for r in log),
             "sim_time": state.time, "dt": state.dt, "step": int(state.step),
             "x_finite": bool(torch.isfinite(ph.x).all())}
     return 1e3 * elapsed / args.steps, info

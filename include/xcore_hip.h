@@ -298,6 +298,9 @@ typedef struct {
     int stokes_fallbacks;           /* sweeps where the learned correction was rejected */
     int stokes_mg_levels;           /* V-cycle depth of the last Stokes solve */
     double stokes_inertia_ratio;    /* its inertia ratio (xc_solve_stats.inertia_ratio) */
+    int stokes_iters_first;         /* FGMRES iterations of the first sweep's solve */
+    double stokes_init_rel_first;   /* its initial relative residual (the previous step's solution) */
+    double stokes_init_rel_later;   /* the largest initial relative residual of the later sweeps */
 } xc_step_stats;
 
 /* run.jl:204-275: one outer step of run! -- time_coefs, store_previous!, store_noise!, then the

@@ -215,8 +215,8 @@ int xc_run_step(xc_ctx* ctx, xc_phase* ph, xc_fluid* fl, xc_noise* ns, const xc_
     if ((st = xc_store_previous(ctx, fl, ph))) return st;
     if ((st = xc_store_noise(ctx, ns))) return st;
     double resnorm = 1.0, resnorm0 = 1.0;
-    int iter = 0, fm_iters = 0, fm_fallbacks = 0, fm_levels = 0;
-    double fm_worst = 0.0, fm_inertia = 0.0;
+    int iter = 0, fm_iters = 0, fm_fallbacks = 0, fm_levels = 0, fm_first = 0;
+    double fm_worst = 0.0, fm_inertia = 0.0, fm_init_first = 0.0, fm_init_later = 0.0;
     const int fixed = so->fixed_sweeps;
     xc_fm_opts fo = so->fm;
     fo.dt = dt;
@@ -233,6 +233,12 @@ int xc_run_step(xc_ctx* ctx, xc_phase* ph, xc_fluid* fl, xc_noise* ns, const xc_
         fm_fallbacks += fs.fallback_used;
         fm_levels = fs.mg_levels;
         fm_inertia = fs.inertia_ratio;
+        if (iter == 1) {
+            fm_first = fs.iters;
+            fm_init_first = fs.init_rel_resid;
+        } else if (fs.init_rel_resid > fm_init_later) {
+            fm_init_later = fs.init_rel_resid;
+        }
         if (fs.rel_resid > fm_worst) fm_worst = fs.rel_resid;
         if (st) return st;
         if ((st = xc_noise_step(ctx, ns, ph, g, par, sc, dt, iter <= 1))) return st;
@@ -257,6 +263,9 @@ int xc_run_step(xc_ctx* ctx, xc_phase* ph, xc_fluid* fl, xc_noise* ns, const xc_
         stats->stokes_fallbacks = fm_fallbacks;
         stats->stokes_mg_levels = fm_levels;
         stats->stokes_inertia_ratio = fm_inertia;
+        stats->stokes_iters_first = fm_first;
+        stats->stokes_init_rel_first = fm_init_first;
+        stats->stokes_init_rel_later = fm_init_later;
     }
     return XC_OK;
 }

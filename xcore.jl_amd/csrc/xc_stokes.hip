@@ -939,39 +939,99 @@ __global__ void k_kry_init(Kry K, int m, double beta) {
 // p_{j+1} = (w - V_j b - h_jj v_j) / nu_{j+1}.  FGMRES needs only LL z_j in span(V): z_j = M^-1 p_j is a
 // direction like any other.
 //
-// partials: q < K: V_q.p (row q) and V_q.w (row K + q); then p.p, p.w, w.w (rows 2K, 2K+1, 2K+2)
+// Full-wavefront sum by DPP (row shifts within rows of 16 lanes, then the row broadcasts), returned to
+// every lane; lanes whose source is outside the row read 0.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_shift(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)__double2loint(v), CTRL, ROWS, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)__double2hiint(v), CTRL, ROWS, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double wave_total(double v) {
+    v += dpp_shift<0x111, 0xf>(v);  // row_shr:1
+    v += dpp_shift<0x112, 0xf>(v);  // row_shr:2
+    v += dpp_shift<0x114, 0xf>(v);  // row_shr:4
+    v += dpp_shift<0x118, 0xf>(v);  // row_shr:8: lane 15 of each row holds the row's sum
+    v += dpp_shift<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_shift<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3: lane 63 holds the total
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 63),
+                            __builtin_amdgcn_readlane(__double2loint(v), 63));
+}
+
+// The reduction of one DCGS2 step: per block, partials of V_q.p (row q) and V_q.w (row K + q) for q < K,
+// then p.p, p.w, w.w (rows 2K .. 2K+2); k_arnoldi2 adds the nb = gridDim.x partials of each row in a
+// fixed order.  Each thread keeps its EPT elements of p and w in registers and streams the basis vectors
+// MD_QC at a time, their loads in flight together (p and w are read once, not once per basis vector).
+constexpr int MD_QC = 4;
+constexpr int MD_BLOCKS = 512;  // target number of blocks (two per CU); more when n needs them
+constexpr int MD_ROWS = 2 * 64 + 3;  // rows of the reduction for a restart length up to 64
+template <int EPT>
 __global__ __launch_bounds__(NT) void k_multidot2(const double* __restrict__ V, int64_t n, int K,
                                                   const double* __restrict__ p, const double* __restrict__ w,
                                                   double* __restrict__ part) {
-    __shared__ double red[NT / 64][3];
+    __shared__ double red[MD_ROWS][NT / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int q = 0; q <= K; ++q) {
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-        for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
-            const double pi = p[i], wi = w[i];
-            if (q < K) {
-                const double v = V[(int64_t)q * n + i];
-                s0 += v * pi;
-                s1 += v * wi;
-            } else {
-                s0 += pi * pi;
-                s1 += pi * wi;
-                s2 += wi * wi;
-            }
-        }
-        s0 = wave_sum(s0);
-        s1 = wave_sum(s1);
-        if (q == K) s2 = wave_sum(s2);
-        if (lane == 0) { red[wave][0] = s0; red[wave][1] = s1; red[wave][2] = s2; }
-        __syncthreads();
-        if (threadIdx.x < 3 && (q == K || threadIdx.x < 2)) {
-            const int t = threadIdx.x;
-            const double v = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
-            const int row = q < K ? (t == 0 ? q : K + q) : 2 * K + t;
-            part[(int64_t)row * gridDim.x + blockIdx.x] = v;
-        }
-        __syncthreads();
+    const int blk = xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t i0 = (int64_t)blk * (EPT * NT) + threadIdx.x;
+    // indices past the end are clamped (in bounds) and their p, w entries zeroed, so that every load is
+    // unconditional and the V entries read there contribute nothing
+    int64_t ix[EPT];
+    double pr[EPT], wr[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+        const int64_t i = i0 + (int64_t)e * NT;
+        ix[e] = i < n ? i : n - 1;
+        const double pv = p[ix[e]], wv = w[ix[e]];
+        pr[e] = i < n ? pv : 0.0;
+        wr[e] = i < n ? wv : 0.0;
     }
+    {
+        double a = 0.0, b = 0.0, c = 0.0;
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+            a = fma(pr[e], pr[e], a);
+            b = fma(pr[e], wr[e], b);
+            c = fma(wr[e], wr[e], c);
+        }
+        a = wave_total(a);
+        b = wave_total(b);
+        c = wave_total(c);
+        if (lane == 0) { red[2 * K][wave] = a; red[2 * K + 1][wave] = b; red[2 * K + 2][wave] = c; }
+    }
+    for (int q0 = 0; q0 < K; q0 += MD_QC) {
+        double v[MD_QC][EPT];
+#pragma unroll
+        for (int c = 0; c < MD_QC; ++c) {
+            const double* Vq = V + (int64_t)(q0 + c < K ? q0 + c : q0) * n;
+#pragma unroll
+            for (int e = 0; e < EPT; ++e) v[c][e] = Vq[ix[e]];
+        }
+#pragma unroll
+        for (int c = 0; c < MD_QC; ++c) {
+            double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+            for (int e = 0; e < EPT; ++e) {
+                s0 = fma(v[c][e], pr[e], s0);
+                s1 = fma(v[c][e], wr[e], s1);
+            }
+            s0 = wave_total(s0);
+            s1 = wave_total(s1);
+            if (lane == 0 && q0 + c < K) { red[q0 + c][wave] = s0; red[K + q0 + c][wave] = s1; }
+        }
+    }
+    __syncthreads();
+    const int R = 2 * K + 3, nb = gridDim.x;
+    for (int r = threadIdx.x; r < R; r += NT)
+        part[(int64_t)r * nb + blk] = (red[r][0] + red[r][1]) + (red[r][2] + red[r][3]);
+}
+// elements per thread and blocks of k_multidot2 for a vector of n elements
+int md_ept(int64_t n) {
+    const int64_t need = (n + (int64_t)MD_BLOCKS * NT - 1) / ((int64_t)MD_BLOCKS * NT);
+    return need <= 1 ? 1 : (need <= 2 ? 2 : (need <= 4 ? 4 : 8));
+}
+int64_t md_blocks(int64_t n) {
+    const int64_t c = (int64_t)md_ept(n) * NT;
+    return (n + c - 1) / c;
 }
 // rotate column Hc (length j + 2, entry j + 1 = the subdiagonal) by the rotations 0..j-1, then form
 // rotation j and apply it to g; Hc[j + 1] becomes 0
@@ -992,23 +1052,53 @@ __device__ void givens_column(Kry K, int j, double* Hc) {
     K.g[j + 1] = -sv * gj;
     K.g[j] = c * gj;
 }
-// d: the reduction of step j (a[0..j), b[0..j), p.p, p.w, w.w).  cf: 1/rho, h_jj, 1/nu_{j+1} for the
-// update kernel; cf[3] carries nu from step to step.  host_out[2j], [2j+1]: the provisional |g[j+1]| and
-// nu_{j+1} (the convergence and breakdown tests read them).  One wave: the lanes stage every input in
-// LDS with their loads in flight together, then lane 0 does the arithmetic (m <= ARN_MMAX).
+// part: k_multidot2's nb partials of each row of step j's reduction; their sums d (a[0..j), b[0..j),
+// p.p, p.w, w.w) are formed here (in a fixed order) and stored for the update kernel.  cf: 1/rho, h_jj,
+// 1/nu_{j+1} for the update kernel; cf[3] carries nu from step to step.  host_out[2j], [2j+1]: the
+// provisional |g[j+1]| and nu_{j+1} (the convergence and breakdown tests read them).  The block sums the
+// partials, sixteen lanes a row (one DPP row), with every lane's loads in flight together; then lane 0
+// does the arithmetic (m <= ARN_MMAX).
 constexpr int ARN_MMAX = 64;
-__global__ __launch_bounds__(64) void k_arnoldi2(Kry K, int m, int j, const double* __restrict__ d,
-                                                 double* __restrict__ cf, double* __restrict__ host_out) {
+constexpr int ARN_NT = 256;
+__global__ __launch_bounds__(ARN_NT) void k_arnoldi2(Kry K, int m, int j, const double* __restrict__ part, int nb,
+                                                     double* __restrict__ d, double* __restrict__ cf,
+                                                     double* __restrict__ host_out) {
     __shared__ double sd[2 * ARN_MMAX + 3], sc[ARN_MMAX + 2], scs[ARN_MMAX], ssn[ARN_MMAX], sg[2], snu;
     const int lane = threadIdx.x;
     double* Hc = K.H + (int64_t)(j > 0 ? j - 1 : 0) * (m + 1);  // column j - 1
-    for (int i = lane; i < 2 * j + 3; i += 64) sd[i] = d[i];
-    for (int i = lane; i < j; i += 64) {
+    {
+        const int R = 2 * j + 3, l16 = lane & 15;
+        constexpr int U = 16;
+        for (int r0 = 0; r0 < R; r0 += ARN_NT / 16) {
+            const int r = r0 + (lane >> 4);
+            const double* pr = part + (int64_t)(r < R ? r : R - 1) * nb;
+            double acc = 0.0;
+            for (int b0 = l16; b0 < nb; b0 += 16 * U) {
+                double v[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int b = b0 + 16 * u;
+                    v[u] = pr[b < nb ? b : nb - 1];
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) acc += b0 + 16 * u < nb ? v[u] : 0.0;
+            }
+            acc += dpp_shift<0x111, 0xf>(acc);  // row_shr 1, 2, 4, 8: lane 15 of the row holds its sum
+            acc += dpp_shift<0x112, 0xf>(acc);
+            acc += dpp_shift<0x114, 0xf>(acc);
+            acc += dpp_shift<0x118, 0xf>(acc);
+            if (l16 == 15 && r < R) {
+                sd[r] = acc;
+                d[r] = acc;
+            }
+        }
+    }
+    for (int i = lane; i < j; i += ARN_NT) {
         scs[i] = K.cs[i];
         ssn[i] = K.sn[i];
     }
     if (j > 0)
-        for (int i = lane; i < j; i += 64) sc[i] = Hc[i];
+        for (int i = lane; i < j; i += ARN_NT) sc[i] = Hc[i];
     if (lane == 0) {
         sg[0] = j > 0 ? K.g[j - 1] : 0.0;
         sg[1] = K.g[j];
@@ -1431,7 +1521,8 @@ int setup(xc_ctx* ctx, int Nz, int Nx, int m, StokesCtx** out, int maxlev = -1) 
     const int K = Nx / 2 + 1;
     const size_t lcc = (size_t)(Nz / 2) * (Nx / 2);  // learned correction at level >= 1
     tot += 3 * lcc + 16 + (lcc + NT - 1) / NT + 8;
-    tot += 4 * (size_t)n + (size_t)(m + 1) * n + (size_t)m * n + (size_t)(2 * m + 4) * DOT_BLOCKS + 128 /* dots */ +
+    const size_t part_cols = md_blocks(n) > DOT_BLOCKS ? (size_t)md_blocks(n) : (size_t)DOT_BLOCKS;
+    tot += 4 * (size_t)n + (size_t)(m + 1) * n + (size_t)m * n + (size_t)(2 * m + 4) * part_cols + 128 /* dots */ +
            (size_t)n /* tmp */ + 8 * (size_t)(Nz + 2) * (Nx + 3) /* advection scratch */ +
            2 * (size_t)Nz * K + 2 * (size_t)Nx + 4 * (size_t)Nz + K + 2 * (size_t)Nz + 16 /* Poisson */ +
            (size_t)(m + 1) * m + 4 * (size_t)m + 8 /* Arnoldi */ + 4 * (size_t)(m + 2) /* dots */;
@@ -1458,7 +1549,7 @@ int setup(xc_ctx* ctx, int Nz, int Nx, int m, StokesCtx** out, int maxlev = -1) 
     s->xs = take(n); s->bs = take(n); s->rs = take(n); s->ws = take(n);
     s->V = take((size_t)(m + 1) * n);
     s->Z = take((size_t)m * n);
-    s->part = take((size_t)(2 * m + 4) * DOT_BLOCKS);
+    s->part = take((size_t)(2 * m + 4) * part_cols);
     s->dots = take(128);
     s->tmp = take(n);
     s->adv_buf = take(8 * (size_t)(Nz + 2) * (Nx + 3));
@@ -1905,11 +1996,17 @@ int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int 
             double* zj = s->Z + (int64_t)j * n;
             if ((st = precond_full(ctx, s, F, mo, n, lc, vj, zj))) return st;
             hipLaunchKernelGGL(k_ll_apply, dim3(nb), dim3(NT), 0, ctx->stream, dd, F, zj, s->ws, n);
-            hipLaunchKernelGGL(k_multidot2, dim3(DOT_BLOCKS), dim3(NT), 0, ctx->stream, s->V, n, j, vj, s->ws,
-                               s->part);
-            hipLaunchKernelGGL(k_dot_final, dim3((2 * j + 6) / 4), dim3(NT), 0, ctx->stream, s->part, DOT_BLOCKS,
-                               2 * j + 3, dd1);
-            hipLaunchKernelGGL(k_arnoldi2, dim3(1), dim3(64), 0, ctx->stream, K, m, j, dd1, cf, s->host_dev + 8);
+            {
+                const dim3 g((unsigned)md_blocks(n)), b(NT), g1(1);
+                switch (md_ept(n)) {
+                    case 1: hipLaunchKernelGGL(k_multidot2<1>, g, b, 0, ctx->stream, s->V, n, j, vj, s->ws, s->part); break;
+                    case 2: hipLaunchKernelGGL(k_multidot2<2>, g, b, 0, ctx->stream, s->V, n, j, vj, s->ws, s->part); break;
+                    case 4: hipLaunchKernelGGL(k_multidot2<4>, g, b, 0, ctx->stream, s->V, n, j, vj, s->ws, s->part); break;
+                    default: hipLaunchKernelGGL(k_multidot2<8>, g, b, 0, ctx->stream, s->V, n, j, vj, s->ws, s->part); break;
+                }
+                hipLaunchKernelGGL(k_arnoldi2, g1, dim3(ARN_NT), 0, ctx->stream, K, m, j, s->part, (int)g.x, dd1, cf,
+                                   s->host_dev + 8);
+            }
             hipLaunchKernelGGL(k_dcgs_update, dim3(nb), dim3(NT), 0, ctx->stream, s->V, n, j, dd1, cf, vj, s->ws,
                                s->V + (int64_t)(j + 1) * n);
             XC_TRY_HIP(ctx, hipEventRecord(s->ev[j & 1], ctx->stream));

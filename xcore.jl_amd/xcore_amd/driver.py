@@ -35,7 +35,8 @@ class StepState(ctypes.Structure):
 
 class StepStats(ctypes.Structure):
     _fields_ = [("sweeps", _I), ("resnorm", _D), ("stokes_iters", _I), ("stokes_worst_rel_resid", _D),
-                ("stokes_fallbacks", _I), ("stokes_mg_levels", _I), ("stokes_inertia_ratio", _D)]
+                ("stokes_fallbacks", _I), ("stokes_mg_levels", _I), ("stokes_inertia_ratio", _D),
+                ("stokes_iters_first", _I), ("stokes_init_rel_first", _D), ("stokes_init_rel_later", _D)]
 
 
 _lib.register("xc_run_step", [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P])
