@@ -223,6 +223,9 @@ typedef struct {  /* fluidmech! keywords (fluidmech.jl:71-80) and the iterative-
                          finest level) */
     int mg_levels;    /* most multigrid levels, the finest included (0: chosen per solve from the inertia ratio,
                          see xc_solve_stats) */
+    int mg_fp64;      /* 1: the finest level's smoother in double precision; 0 (default): in single precision
+                         (its vectors and a copy of its coefficients; FGMRES, the operator and the coarser
+                         levels stay in double precision, and so does the convergence test) */
 } xc_fm_opts;
 
 typedef struct {  /* state.jl:131-134 BnchmData */

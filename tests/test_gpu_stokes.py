@@ -114,6 +114,28 @@ def test_production_solve(xc, orc, shape, open_cnv):
     print(f"  |x - x_LU| / |x_LU| = {_rel(sol, orc.stokes_vector(fl)):.3e}", flush=True)
 
 
+@pytest.mark.parametrize("open_cnv", [False, True])
+def test_fine_smoother_precision(xc, orc, open_cnv):
+    # the fine level's smoother in single precision (default) or double precision (mg_fp64): the same
+    # residual bar against the oracle's assembled system, and about the same iteration count
+    Nz, Nx = 64, 64
+    par, g, sc, ph, fl, ns = plausible_state(orc, xc, Nz, Nx, seed=11)
+    orc.update(ph, fl, g, par, sc)
+    orc.phsevo(ph, fl, g, par, sc, coef=orc.time_coefs("bd2im", 3), dt=sc.dt0)
+    kw = dict(dt=sc.dt0, a1=1.5, a2=2.0, a3=-0.5, b1=1, b2=0, b3=0, open_cnv=open_cnv)
+    LL, RR = orc.fluidmech_assemble(copy.deepcopy(fl), copy.deepcopy(ph), g, orc.fm_opts(par, **kw))
+    its = {}
+    for fp64 in (False, True):
+        dph, dfl = to_device_state(xc, ph, "phase"), to_device_state(xc, fl, "fluid")
+        stats = xc.fluidmech_(dfl, g, par, phase=dph, tol=1e-8, mg_fp64=fp64, **kw)
+        print(f"mg_fp64={fp64} open_cnv={open_cnv}: {stats}", flush=True)
+        assert stats["converged"]
+        sol = np.concatenate([xc.to_host(getattr(dfl, k)).ravel(order="F") for k in ("W", "U", "P")])
+        assert np.linalg.norm(LL @ sol - RR) / np.linalg.norm(RR) <= 1.5e-8
+        its[fp64] = stats["iters"]
+    assert abs(its[False] - its[True]) <= max(2, its[True] // 10), its
+
+
 def test_noconv_reports(xc, orc):
     m, par, g, fl, kw = _mms(orc, xc, 32)
     dfl = to_device_state(xc, fl, "fluid")

@@ -51,6 +51,7 @@ __device__ __forceinline__ int64_t gidx() {
 }
 
 struct LevD {  // coefficients of one level, dense column-major (ld = rows)
+    typedef double real;
     int Nz, Nx;
     double h, invh, invh2;
     const double *eta, *etaco, *rho, *rhow, *rhou;
@@ -66,6 +67,16 @@ struct LevD {  // coefficients of one level, dense column-major (ld = rows)
     __device__ __forceinline__ double RHOU(int r, int c) const { return rhou[r + (int64_t)c * Nz]; }
     __device__ __forceinline__ int icx(int k) const { return k == 0 ? Nx - 1 : (k == Nx + 1 ? 0 : k - 1); }
     __device__ __forceinline__ int ifx(int k) const { return k == 0 ? Nx - 1 : (k == Nx + 2 ? 1 : k - 1); }
+};
+
+// the fine level's coefficients in single precision: the multigrid smoother's copy (the operator and the
+// coarse levels keep double precision)
+struct LevF {
+    typedef float real;
+    int Nz, Nx;
+    double h, invh, invh2;
+    const float *eta, *etaco, *rho, *rhow, *rhou;
+    __device__ __forceinline__ float RHOU(int r, int c) const { return rhou[r + (int64_t)c * Nz]; }
 };
 
 // ---- rows of the velocity block KV (fluidmech.jl:145-324).  Return sum_j KV_ij x_j, diag in *dg.
@@ -260,8 +271,178 @@ __global__ __launch_bounds__(NT) void k_vel_resid(LevD L, Flags F, const double*
     if (k < L.nW() + L.nU()) vel_resid_at(L, F, x, b, r, fine, k);
 }
 
+// ---- the smoother and the level residual on the grid of cells: one lane per (r, c), r in [0, Nz+1],
+// c in [0, Nx+1], computing the W row (r, c) (r <= Nz) and the U row (r, c) (c <= Nx) together.  A wave
+// holds 64 consecutive rows of one column, so every column offset is a scalar and every operand is a
+// scalar base plus the lane's row offset (clamped into the array); the rows are those of vel_row.
+// LV: the level's coefficients (LevD, or LevF: single precision, arithmetic in single precision).
+// MODE 0: damped Jacobi sweep out = x + w (b - A x) / diag (BC rows weight 1); 1: the same from x = 0
+// (x is not read); 2: residual out = b - A x; 3: the first sweep of the fine level from x = 0 with the
+// right-hand side b = r_v - GG z_p formed here from the Krylov vector's velocity part (b) and pressure
+// part (zP), and stored to bout.
+template <typename T>
+__device__ __forceinline__ T ldr(const T* __restrict__ col, int row) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(col) + (size_t)((unsigned)row * (unsigned)sizeof(T)));
+}
+__device__ __forceinline__ float recip(float d) {
+    const float r = __builtin_amdgcn_rcpf(d);
+    return fmaf(r, fmaf(-d, r, 1.0f), r);
+}
+template <int MODE, class LV, typename TX, typename TB, typename TO>
+__global__ __launch_bounds__(NT) void k_smooth_cells(LV L, Flags F, const TX* __restrict__ x,
+                                                     const TB* __restrict__ b, TO* __restrict__ out, double omega,
+                                                     int fine, const double* __restrict__ zP, TO* __restrict__ bout) {
+    using R = typename LV::real;
+    const int Nz = L.Nz, Nx = L.Nx;
+    const int segs = (Nz + 2 + 63) >> 6;
+    const int task = __builtin_amdgcn_readfirstlane(xcd_tile(blockIdx.x, gridDim.x) * (NT / 64) + (threadIdx.x >> 6));
+    const int c = task / segs;
+    if (c > Nx + 1) return;
+    const int lane = threadIdx.x & 63;
+    const int r = ((task - c * segs) << 6) + lane;
+    const bool live = r <= Nz + 1;
+    const int ru = r <= Nz + 1 ? r : Nz + 1;   // U row
+    const int rw = ru <= Nz ? ru : Nz;         // W row
+    const int um = ru > 0 ? ru - 1 : 0, up = ru <= Nz ? ru + 1 : Nz + 1;
+    const int wm = rw > 0 ? rw - 1 : 0, wp = rw < Nz ? rw + 1 : Nz;
+    const int em = rw > 0 ? (rw - 1 < Nz - 1 ? rw - 1 : Nz - 1) : 0;  // cell row r - 1
+    const int e0 = rw < Nz - 1 ? rw : Nz - 1;                          // cell row r
+    const int ldW = Nz + 1, ldU = Nz + 2;
+    // columns (wave-uniform)
+    const int cw = c;                                        // W column, [0, Nx+1]
+    const int cwm = c > 0 ? c - 1 : 0, cwp = c <= Nx ? c + 1 : Nx + 1;
+    const int cu = c <= Nx ? c : Nx;                         // U column, [0, Nx]
+    const int cul = c == 0 ? Nx - 1 : (c - 1 <= Nx ? c - 1 : Nx);   // periodic left U column / W row's c - 1
+    const int cur = c + 1 >= Nx + 1 ? (c + 1 == Nx + 1 ? 1 : Nx) : c + 1;  // periodic right U column (c + 2 -> 1)
+    const int cl = c == 0 ? Nx - 1 : (c - 1 < Nx ? c - 1 : Nx - 1);  // cell column left of the face
+    const int cr = c < Nx ? c : 0;                                    // cell column right of the U face
+    const int ceco = c <= Nx ? c : Nx, cecm = c > 0 ? (c - 1 <= Nx ? c - 1 : Nx) : 0;
+    const int64_t offU = (int64_t)ldW * (Nx + 2);
+    // operands
+    const TX* Wc = x + (int64_t)cw * ldW;
+    const TX* Uc = x + offU + (int64_t)cu * ldU;
+    R Wr = 0, Wn = 0, Ws = 0, Ww = 0, We = 0, Wen = 0, Wg = 0;
+    R Ur = 0, Un = 0, Us = 0, Ul = 0, Uls = 0, Urt = 0;
+    if (MODE == 0 || MODE == 2) {
+        Wr = (R)ldr(Wc, rw); Wn = (R)ldr(Wc, wm); Ws = (R)ldr(Wc, wp);
+        Ww = (R)ldr(x + (int64_t)cwm * ldW, rw);
+        const TX* Wpc = x + (int64_t)cwp * ldW;
+        We = (R)ldr(Wpc, rw); Wen = (R)ldr(Wpc, wm);
+        Ur = (R)ldr(Uc, ru); Un = (R)ldr(Uc, um); Us = (R)ldr(Uc, up);
+        const TX* Ulc = x + offU + (int64_t)cul * ldU;
+        Ul = (R)ldr(Ulc, ru); Uls = (R)ldr(Ulc, up);
+        Urt = (R)ldr(x + offU + (int64_t)cur * ldU, ru);
+        if (c == 0 || c == Nx + 1) Wg = (R)ldr(x + (int64_t)(c == 0 ? Nx : 1) * ldW, rw);
+    }
+    double bw = (double)ldr(b + (int64_t)cw * ldW, rw), bu = (double)ldr(b + offU + (int64_t)cu * ldU, ru);
+    if (MODE == 3) {  // b = r_v - GG z_p (fluidmech.jl:343-356): interior W and U rows, not at the MMS pin
+        const int ldP = Nz + 2;
+        const double* Pc = zP + (int64_t)cu * ldP;
+        const double p0 = ldr(Pc, ru), p1 = ldr(Pc, ru <= Nz ? ru + 1 : Nz + 1);
+        const double pe = ldr(zP + (int64_t)(cu + 1) * ldP, ru);
+        const double ih = L.invh;
+        const int64_t kw = rw + (int64_t)cw * ldW, ku = offU + ru + (int64_t)cu * ldU;
+        if (r >= 1 && r <= Nz - 1 && c >= 1 && c <= Nx && kw != F.nu0) bw -= -ih * p0 + ih * p1;
+        if (r >= 1 && r <= Nz && c <= Nx && ku != F.nu0) bu -= -ih * p0 + ih * pe;
+    }
+    const auto* ECm = L.etaco + (int64_t)cecm * (Nz + 1);
+    const auto* EC0 = L.etaco + (int64_t)ceco * (Nz + 1);
+    const auto* El = L.eta + (int64_t)cl * Nz;
+    const auto* Er = L.eta + (int64_t)cr * Nz;
+    const auto* Rl = L.rho + (int64_t)cl * Nz;
+    const auto* Rr = L.rho + (int64_t)cr * Nz;
+    const auto* RWc = L.rhow + (int64_t)cl * (Nz + 1);
+    const auto* RUc = L.rhou + (int64_t)cu * Nz;
+    const R eC1w = ldr(ECm, rw), eC2w = ldr(EC0, rw), eC1u = ldr(EC0, wm);  // etaco(r, c-1), (r, c), (r-1, c)
+    const R eP1w = ldr(El, em), eP2w = ldr(El, e0), eP2u = ldr(Er, em);    // eta(r-1, cl), (r, cl), (r-1, cr)
+    const R rl0 = ldr(Rl, e0), rlm = ldr(Rl, em), rrm = ldr(Rr, em);      // rho(r, cl), (r-1, cl), (r-1, cr)
+    const R rw0 = ldr(RWc, rw), rwm = ldr(RWc, wm);                       // rhow(r, c-1), (r-1, c-1)
+    const R ru0 = ldr(RUc, em);                                           // rhou(r-1, c)
+    const R i2 = (R)L.invh2, ih = (R)L.invh;
+    const R gz = F.bnchm ? (R)0 : (R)(L.invh * F.g0 * F.dt);
+    const R a1w = (R)F.a1_dt, a1u = (R)F.a1g_dt, two3 = (R)(2.0 / 3.0), third = (R)(1.0 / 3.0), half = (R)0.5;
+    // W row (r, c)
+    R dw, vw;
+    {
+        const R C1 = eC1w, C2 = eC2w, P1 = eP1w, P2 = eP2w;
+        dw = rw0 * a1w + (two3 * (P1 + P2) * i2 + half * (C1 + C2) * i2) + (rl0 - rlm) * gz;
+        const R T1 = P1 * third, T2 = P2 * third;
+        vw = dw * Wr - two3 * P1 * i2 * Wn - two3 * P2 * i2 * Ws - half * C1 * i2 * Ww - half * C2 * i2 * We -
+             (half * C1 - T1) * i2 * Ul + (half * C1 - T2) * i2 * Uls + (half * C2 - T1) * i2 * Ur -
+             (half * C2 - T2) * i2 * Us;
+    }
+    const bool wghost = c == 0 || c == Nx + 1;
+    const bool wtop = !wghost && r == 0, wbot = !wghost && r == Nz;
+    if (wbot) {
+        dw = rw0 * ih;
+        if (F.open_cnv) {
+            const R RU0 = L.RHOU(Nz - 1, cul), RU1 = L.RHOU(Nz - 1, cu);
+            vw = dw * Wr - rwm * ih * Wn + RU1 * ih * Ur - RU0 * ih * Ul;
+        } else {
+            vw = dw * Wr;
+        }
+    }
+    if (wtop) { dw = 1; vw = Wr; }
+    if (wghost) { dw = 1; vw = Wr + (R)F.sds * Wg; }
+    bool bcw = wghost || wtop || wbot;
+    // U row (r, c)
+    R du, vu;
+    {
+        const R C1 = eC1u, C2 = eC2w, P1 = eP1w, P2 = eP2u;
+        du = ru0 * a1u + (two3 * (P1 + P2) * i2 + half * (C1 + C2) * i2) + (rrm - rlm) * gz;
+        const R T1 = P1 * third, T2 = P2 * third;
+        vu = du * Ur - two3 * P1 * i2 * Ul - two3 * P2 * i2 * Urt - half * C1 * i2 * Un - half * C2 * i2 * Us -
+             (half * C1 - T1) * i2 * Wn + (half * C1 - T2) * i2 * Wen + (half * C2 - T1) * i2 * Wr -
+             (half * C2 - T2) * i2 * We;
+    }
+    const bool utop = r == 0, ubot = r == Nz + 1;
+    if (utop) { du = 1; vu = Ur + (R)F.top_cnv * Us; }
+    if (ubot) { du = 1; vu = Ur + (R)F.bot_cnv * Un; }
+    bool bcu = utop || ubot;
+    const int64_t kw = rw + (int64_t)cw * ldW, ku = offU + ru + (int64_t)cu * ldU;
+    if (fine && kw == F.nu0) { dw = 1; vw = Wr; bcw = true; }
+    if (fine && ku == F.nu0) { du = 1; vu = Ur; bcu = true; }
+    if (!live) return;
+    const R om = (R)omega;
+    R ow, ou;
+    if (MODE == 2) {
+        ow = (R)bw - vw;
+        ou = (R)bu - vu;
+    } else if (MODE == 1 || MODE == 3) {
+        ow = (bcw ? (R)1 : om) * (R)bw * recip(dw);
+        ou = (bcu ? (R)1 : om) * (R)bu * recip(du);
+    } else {
+        ow = Wr + (bcw ? (R)1 : om) * ((R)bw - vw) * recip(dw);
+        ou = Ur + (bcu ? (R)1 : om) * ((R)bu - vu) * recip(du);
+    }
+    if (r <= Nz) {
+        out[kw] = (TO)ow;
+        if (MODE == 3) bout[kw] = (TO)bw;
+    }
+    if (c <= Nx) {
+        out[ku] = (TO)ou;
+        if (MODE == 3) bout[ku] = (TO)bu;
+    }
+}
+#define SMOOTH_D(M) (k_smooth_cells<M, LevD, double, double, double>)
+#define SMOOTH_F(M, TB, TO) (k_smooth_cells<M, LevF, float, TB, TO>)
+// single <-> double precision copies
+__global__ __launch_bounds__(NT) void k_d2f(const double* __restrict__ a, float* __restrict__ b, int64_t n) {
+    const int64_t i = gidx();
+    if (i < n) b[i] = (float)a[i];
+}
+__global__ __launch_bounds__(NT) void k_f2d(const float* __restrict__ a, double* __restrict__ b, int64_t n) {
+    const int64_t i = gidx();
+    if (i < n) b[i] = (double)a[i];
+}
+inline unsigned smooth_cells_blocks(int Nz, int Nx) {
+    const int64_t tasks = (int64_t)(Nx + 2) * ((Nz + 2 + 63) / 64);
+    return (unsigned)((tasks + NT / 64 - 1) / (NT / 64));
+}
+
 // staggered full-weighting restriction of a velocity residual rf (level f) into bc (level c)
-__device__ __forceinline__ void restrict_at(const LevD& Lf, const LevD& Lc, const double* __restrict__ rf,
+template <typename TF>
+__device__ __forceinline__ void restrict_at(const LevD& Lf, const LevD& Lc, const TF* __restrict__ rf,
                                             double* __restrict__ bcv, int64_t k) {
     const int64_t nWc = Lc.nW();
     double v = 0.0;
@@ -269,8 +450,8 @@ __device__ __forceinline__ void restrict_at(const LevD& Lf, const LevD& Lc, cons
         const int J = ((int)k % (Lc.Nz + 1)), I = ((int)k / (Lc.Nz + 1));
         if (J >= 1 && J <= Lc.Nz - 1 && I >= 1 && I <= Lc.Nx) {
             const int r = 2 * J, c0 = 2 * I - 1, c1 = 2 * I;
-            v = (2.0 * rf[Lf.iW(r, c0)] + 2.0 * rf[Lf.iW(r, c1)] + rf[Lf.iW(r - 1, c0)] + rf[Lf.iW(r - 1, c1)] +
-                 rf[Lf.iW(r + 1, c0)] + rf[Lf.iW(r + 1, c1)]) * 0.125;
+            v = (2.0 * (double)rf[Lf.iW(r, c0)] + 2.0 * (double)rf[Lf.iW(r, c1)] + (double)rf[Lf.iW(r - 1, c0)] +
+                 (double)rf[Lf.iW(r - 1, c1)] + (double)rf[Lf.iW(r + 1, c0)] + (double)rf[Lf.iW(r + 1, c1)]) * 0.125;
         }
     } else {
         const int64_t q = k - nWc;
@@ -279,21 +460,23 @@ __device__ __forceinline__ void restrict_at(const LevD& Lf, const LevD& Lc, cons
             const int r0 = 2 * J - 1, r1 = 2 * J, f = 2 * I;
             const int fl = f == 0 ? Lf.Nx - 1 : f - 1;
             const int fr = f == Lf.Nx ? 1 : f + 1;
-            v = (2.0 * rf[Lf.iU(r0, f)] + 2.0 * rf[Lf.iU(r1, f)] + rf[Lf.iU(r0, fl)] + rf[Lf.iU(r1, fl)] +
-                 rf[Lf.iU(r0, fr)] + rf[Lf.iU(r1, fr)]) * 0.125;
+            v = (2.0 * (double)rf[Lf.iU(r0, f)] + 2.0 * (double)rf[Lf.iU(r1, f)] + (double)rf[Lf.iU(r0, fl)] +
+                 (double)rf[Lf.iU(r1, fl)] + (double)rf[Lf.iU(r0, fr)] + (double)rf[Lf.iU(r1, fr)]) * 0.125;
         }
     }
     bcv[k] = v;
 }
-__global__ __launch_bounds__(NT) void k_restrict(LevD Lf, LevD Lc, const double* __restrict__ rf,
+template <typename TF>
+__global__ __launch_bounds__(NT) void k_restrict(LevD Lf, LevD Lc, const TF* __restrict__ rf,
                                                  double* __restrict__ bcv) {
     const int64_t k = gidx();
     if (k < Lc.nW() + Lc.nU()) restrict_at(Lf, Lc, rf, bcv, k);
 }
 
 // bilinear prolongation of a coarse correction ec, added to the fine iterate xf (interior rows)
+template <typename TF>
 __device__ __forceinline__ void prolong_at(const LevD& Lf, const LevD& Lc, const Flags& F,
-                                           const double* __restrict__ ec, double* __restrict__ xf, int64_t k) {
+                                           const double* __restrict__ ec, TF* __restrict__ xf, int64_t k) {
     const int64_t nWf = Lf.nW();
     if (k < nWf) {
         const int r = ((int)k % (Lf.Nz + 1)), c = ((int)k / (Lf.Nz + 1));
@@ -332,8 +515,9 @@ __device__ __forceinline__ void prolong_at(const LevD& Lf, const LevD& Lc, const
         xf[k] += v;
     }
 }
+template <typename TF>
 __global__ __launch_bounds__(NT) void k_prolong(LevD Lf, LevD Lc, Flags F, const double* __restrict__ ec,
-                                                double* __restrict__ xf) {
+                                                TF* __restrict__ xf) {
     const int64_t k = gidx();
     if (k < Lf.nW() + Lf.nU()) prolong_at(Lf, Lc, F, ec, xf, k);
 }
@@ -1386,6 +1570,18 @@ struct Level {
 struct StokesCtx {
     int Nz = -1, Nx = -1, m = -1, maxlev = 0;
     std::vector<Level> lev;
+    // the fine level's smoother in single precision: its coefficients (the level's five arrays at the same
+    // offsets from eta) and vectors x, b, t (ping-pong), r
+    float *f_coef = nullptr, *fx = nullptr, *fb = nullptr, *ft = nullptr, *fr = nullptr;
+    int64_t f_ncoef = 0;
+    LevF levf() const {
+        const Level& L = lev[0];
+        LevF f;
+        f.Nz = L.Nz; f.Nx = L.Nx; f.h = L.h; f.invh = 1.0 / L.h; f.invh2 = (1.0 / L.h) * (1.0 / L.h);
+        f.eta = f_coef; f.etaco = f_coef + (L.etaco - L.eta); f.rho = f_coef + (L.rho - L.eta);
+        f.rhow = f_coef + (L.rhow - L.eta); f.rhou = f_coef + (L.rhou - L.eta);
+        return f;
+    }
     double* pool = nullptr;   // everything below carved from one allocation
     size_t pool_bytes = 0;
     double *xs, *bs, *rs, *ws, *V, *Z, *part, *dots, *tmp, *adv_buf;
@@ -1526,6 +1722,10 @@ int setup(xc_ctx* ctx, int Nz, int Nx, int m, StokesCtx** out, int maxlev = -1) 
            (size_t)n /* tmp */ + 8 * (size_t)(Nz + 2) * (Nx + 3) /* advection scratch */ +
            2 * (size_t)Nz * K + 2 * (size_t)Nx + 4 * (size_t)Nz + K + 2 * (size_t)Nz + 16 /* Poisson */ +
            (size_t)(m + 1) * m + 4 * (size_t)m + 8 /* Arnoldi */ + 4 * (size_t)(m + 2) /* dots */;
+    const Level& F0 = s->lev[0];
+    const size_t ncoef0 = 2 * (size_t)F0.Nz * F0.Nx + (size_t)(F0.Nz + 1) * (F0.Nx + 1) + (size_t)(F0.Nz + 1) * F0.Nx +
+                          (size_t)F0.Nz * (F0.Nx + 1) + 10 /* even rounding */;
+    tot += (ncoef0 + 4 * (size_t)F0.nv) / 2 + 16;  /* single-precision fine-level smoother */
     void* pool;
     hipError_t e = hipMalloc(&pool, tot * sizeof(double));
     if (e != hipSuccess) { delete s; return xc_fail(ctx, XC_ENOMEM, "fluidmech!: %zu bytes: %s", tot * 8, hipGetErrorString(e)); }
@@ -1559,6 +1759,17 @@ int setup(xc_ctx* ctx, int Nz, int Nx, int m, StokesCtx** out, int maxlev = -1) 
     s->lc_cap = (int)lcc;
     s->kry = take((size_t)(m + 1) * m + 4 * (size_t)m + 8);
     s->d3 = take(4 * (size_t)(m + 2));
+    {
+        const Level& L0 = s->lev[0];
+        s->f_ncoef = (L0.rhou + (size_t)L0.Nz * (L0.Nx + 1)) - L0.eta;
+        const size_t fl = ((size_t)s->f_ncoef + 3) & ~(size_t)3, fv = ((size_t)L0.nv + 3) & ~(size_t)3;
+        float* f = reinterpret_cast<float*>(take((fl + 4 * fv) / 2));
+        s->f_coef = f;
+        s->fx = f + fl;
+        s->fb = s->fx + fv;
+        s->ft = s->fb + fv;
+        s->fr = s->ft + fv;
+    }
     if ((p - s->pool) & 1) ++p;  // 16-byte alignment for the complex arrays
     s->pK = K;
     s->pS = reinterpret_cast<double2*>(take(2 * (size_t)Nz * K));
@@ -1634,6 +1845,7 @@ void build_levels(xc_ctx* ctx, StokesCtx* s, const xc_fluid* fl, double h) {
     hipLaunchKernelGGL(k_pack, dim3(nblk(fl->etaco.rows * fl->etaco.cols)), dim3(NT), 0, ctx->stream, fl->etaco, L0.etaco);
     hipLaunchKernelGGL(k_pack, dim3(nblk(fl->rhow.rows * fl->rhow.cols)), dim3(NT), 0, ctx->stream, fl->rhow, L0.rhow);
     hipLaunchKernelGGL(k_pack, dim3(nblk(fl->rhou.rows * fl->rhou.cols)), dim3(NT), 0, ctx->stream, fl->rhou, L0.rhou);
+    hipLaunchKernelGGL(k_d2f, dim3(nblk(s->f_ncoef)), dim3(NT), 0, ctx->stream, L0.eta, s->f_coef, s->f_ncoef);
     for (size_t l = 1; l < s->lev.size(); ++l) {
         Level& Lf = s->lev[l - 1];
         Level& Lc = s->lev[l];
@@ -1689,7 +1901,8 @@ int choose_depth(xc_ctx* ctx, StokesCtx* s, const Flags& F, const xc_fm_opts* o,
     return XC_OK;
 }
 
-struct MGOpts { int pre, post, coarse; double omega; int sub; };  // sub: pre = post sweeps on levels >= 1 (0: as level 0)
+// sub: pre = post sweeps on levels >= 1 (0: as level 0); f32: the fine level's smoother in single precision
+struct MGOpts { int pre, post, coarse; double omega; int sub; bool f32; };
 
 // solver controls with the defaults measured best on the 512^2 production step (zero fields: default)
 MGOpts mg_opts(const xc_fm_opts* o) {
@@ -1699,6 +1912,7 @@ MGOpts mg_opts(const xc_fm_opts* o) {
     m.coarse = o->mg_coarse > 0 ? o->mg_coarse : 0;  // 0: coarse_sweeps decides per solve
     m.omega = o->mg_omega > 0 ? o->mg_omega : 0.7;
     m.sub = o->mg_sub > 0 ? o->mg_sub : (o->mg_sub == 0 ? 1 : 0);
+    m.f32 = o->mg_fp64 == 0;
     return m;
 }
 
@@ -1728,30 +1942,71 @@ void vcycle(xc_ctx* ctx, StokesCtx* s, const Flags& F, size_t l, const MGOpts& m
         return;
     }
     const LevD d = L.d();
-    const unsigned nb = nblk(L.nv);
+    const unsigned nb = nblk(L.nv), nbc = smooth_cells_blocks(L.Nz, L.Nx);
     const int fine = l == 0;
     const bool coarsest = (int)l + 1 == s->leff;
     const int pre = coarsest ? coarse_sweeps(s, mo) : (l > 0 && mo.sub > 0 ? mo.sub : mo.pre);
     const int post = l > 0 && mo.sub > 0 ? mo.sub : mo.post;
     // first sweep from zero (already done by the caller when first_done)
     if (!first_done) {
-        hipLaunchKernelGGL(k_jacobi, dim3(nb), dim3(NT), 0, ctx->stream, d, F, L.x, L.b, L.t, mo.omega, fine, 1);
+        hipLaunchKernelGGL(SMOOTH_D(1), dim3(nbc), dim3(NT), 0, ctx->stream, d, F, L.x, L.b, L.t, mo.omega, fine, nullptr, nullptr);
         std::swap(L.x, L.t);
     }
     for (int it = 1; it < pre; ++it) {
-        hipLaunchKernelGGL(k_jacobi, dim3(nb), dim3(NT), 0, ctx->stream, d, F, L.x, L.b, L.t, mo.omega, fine, 0);
+        hipLaunchKernelGGL(SMOOTH_D(0), dim3(nbc), dim3(NT), 0, ctx->stream, d, F, L.x, L.b, L.t, mo.omega, fine, nullptr, nullptr);
         std::swap(L.x, L.t);
     }
     if (coarsest) return;
-    hipLaunchKernelGGL(k_vel_resid, dim3(nb), dim3(NT), 0, ctx->stream, d, F, L.x, L.b, L.r, fine);
+    hipLaunchKernelGGL(SMOOTH_D(2), dim3(nbc), dim3(NT), 0, ctx->stream, d, F, L.x, L.b, L.r, 0.0, fine, nullptr, nullptr);
     Level& C = s->lev[l + 1];
-    hipLaunchKernelGGL(k_restrict, dim3(nblk(C.nv)), dim3(NT), 0, ctx->stream, d, C.d(), L.r, C.b);
+    hipLaunchKernelGGL(k_restrict<double>, dim3(nblk(C.nv)), dim3(NT), 0, ctx->stream, d, C.d(), L.r, C.b);
     vcycle(ctx, s, F, l + 1, mo);
-    hipLaunchKernelGGL(k_prolong, dim3(nb), dim3(NT), 0, ctx->stream, d, C.d(), F, C.x, L.x);
+    hipLaunchKernelGGL(k_prolong<double>, dim3(nb), dim3(NT), 0, ctx->stream, d, C.d(), F, C.x, L.x);
     for (int it = 0; it < post; ++it) {
-        hipLaunchKernelGGL(k_jacobi, dim3(nb), dim3(NT), 0, ctx->stream, d, F, L.x, L.b, L.t, mo.omega, fine, 0);
+        hipLaunchKernelGGL(SMOOTH_D(0), dim3(nbc), dim3(NT), 0, ctx->stream, d, F, L.x, L.b, L.t, mo.omega, fine, nullptr, nullptr);
         std::swap(L.x, L.t);
     }
+}
+
+// The V-cycle with the fine level in single precision: the first sweep from zero with the right-hand side
+// b = r_v - GG z_p formed from v, the cycle (levels >= 1 in double precision), and the last fine-level
+// sweep written to z's velocity part in double precision.  The preconditioner's precision does not bound
+// the solution's: FGMRES forms its residual and basis in double precision from LL, and the smoother only
+// needs to reduce the error by a constant factor per sweep.
+void vcycle_f32(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, const double* v, const double* zP,
+                double* z) {
+    Level& L = s->lev[0];
+    const LevD d = L.d();
+    const LevF f = s->levf();
+    const unsigned nbc = smooth_cells_blocks(L.Nz, L.Nx);
+    float *x = s->fx, *t = s->ft;
+    const bool coarsest = s->leff <= 1;
+    const int pre = coarsest ? coarse_sweeps(s, mo) : mo.pre, post = mo.post;
+    hipLaunchKernelGGL(SMOOTH_F(3, double, float), dim3(nbc), dim3(NT), 0, ctx->stream, f, F, x, v, x, mo.omega, 1, zP,
+                       s->fb);
+    bool in_z = false;
+    auto sweep = [&](bool to_z) {
+        if (to_z) {
+            hipLaunchKernelGGL(SMOOTH_F(0, float, double), dim3(nbc), dim3(NT), 0, ctx->stream, f, F, x, s->fb, z,
+                               mo.omega, 1, nullptr, nullptr);
+            in_z = true;
+        } else {
+            hipLaunchKernelGGL(SMOOTH_F(0, float, float), dim3(nbc), dim3(NT), 0, ctx->stream, f, F, x, s->fb, t,
+                               mo.omega, 1, nullptr, nullptr);
+            std::swap(x, t);
+        }
+    };
+    for (int it = 1; it < pre; ++it) sweep(coarsest && it == pre - 1);
+    if (!coarsest) {
+        hipLaunchKernelGGL(SMOOTH_F(2, float, float), dim3(nbc), dim3(NT), 0, ctx->stream, f, F, x, s->fb, s->fr, 0.0,
+                           1, nullptr, nullptr);
+        Level& C = s->lev[1];
+        hipLaunchKernelGGL(k_restrict<float>, dim3(nblk(C.nv)), dim3(NT), 0, ctx->stream, d, C.d(), s->fr, C.b);
+        vcycle(ctx, s, F, 1, mo);
+        hipLaunchKernelGGL(k_prolong<float>, dim3(nblk(L.nv)), dim3(NT), 0, ctx->stream, d, C.d(), F, C.x, x);
+        for (int it = 0; it < post; ++it) sweep(it == post - 1);
+    }
+    if (!in_z) hipLaunchKernelGGL(k_f2d, dim3(nblk(L.nv)), dim3(NT), 0, ctx->stream, x, z, L.nv);
 }
 
 // z = M^-1 v  (block-triangular preconditioner)
@@ -1797,6 +2052,10 @@ void precond(xc_ctx* ctx, StokesCtx* s, const Flags& F, const double* v, double*
     }
     // velocity: b = r_v - GG z_p and the first sweep in one kernel, then the V-cycle.  With an even
     // number of level-0 sweeps the iterates alternate between z and the level's scratch and end in z.
+    if (mo.f32) {
+        vcycle_f32(ctx, s, F, mo, v, zP, z);
+        return;
+    }
     double* own_x = L.x;
     const bool direct = s->leff > 1 && (mo.pre + mo.post) % 2 == 0;
     if (direct) L.x = z;
