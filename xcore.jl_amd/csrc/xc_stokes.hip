@@ -762,7 +762,8 @@ struct PoisXArgs {
 __global__ __launch_bounds__(FFT_NT) void k_pois_xfwd(PoisXArgs a) {
     extern __shared__ double2 sm[];
     const int n = a.Nx, nl = a.nl, rows = 2 * nl;
-    const int r0 = blockIdx.x * rows;
+    // neighbouring row blocks share the cache lines of the column-major P arrays: keep them on one XCD
+    const int r0 = xcd_tile(blockIdx.x, gridDim.x) * rows;
     double2* A = sm;
     double2* B = sm + nl * n;
     for (int k = threadIdx.x; k < rows * n; k += FFT_NT) {
@@ -797,7 +798,7 @@ struct PoisZArgs {
 __global__ __launch_bounds__(FFT_NT) void k_pois_zfft(PoisZArgs a) {
     extern __shared__ double2 sm[];
     const int n = 2 * a.Nz, nl = a.nl, K = a.K;
-    const int k0 = blockIdx.x * nl;
+    const int k0 = xcd_tile(blockIdx.x, gridDim.x) * nl;  // neighbouring wavenumbers share S's cache lines
     double2* A = sm;
     double2* B = sm + nl * n;
     for (int i = threadIdx.x; i < nl * n; i += FFT_NT) {
@@ -841,7 +842,7 @@ struct PoisInvArgs {
 __global__ __launch_bounds__(FFT_NT) void k_pois_xinv(PoisInvArgs a) {
     extern __shared__ double2 sm[];
     const int n = a.Nx, nl = a.nl, rows = 2 * nl, K = a.K;
-    const int r0 = blockIdx.x * rows;
+    const int r0 = xcd_tile(blockIdx.x, gridDim.x) * rows;  // row blocks of one XCD share cache lines
     double2* A = sm;
     double2* B = sm + nl * n;
     for (int k = threadIdx.x; k < nl * n; k += FFT_NT) {
