@@ -154,54 +154,6 @@ __device__ __forceinline__ bool is_pin(const Flags& F, int r, int c, int Nz) {
     return false;
 }
 
-// full row k of LL x (fine level), fluidmech.jl:145-488
-__device__ double ll_row(const LevD& L, const Flags& F, const double* x, int64_t k) {
-    const int64_t nW = L.nW(), nU = L.nU();
-    const int Nz = L.Nz, Nx = L.Nx;
-    double dg;
-    bool bc;
-    if (k < nW + nU) {
-        double s = vel_row(L, F, x, k, &dg, &bc, true);
-        if (!bc) {  // GG (fluidmech.jl:343-356)
-            if (k < nW) {
-                const int r = ((int)k % (Nz + 1)), c = ((int)k / (Nz + 1));
-                s += -L.invh * x[L.iP(r, c)] + L.invh * x[L.iP(r + 1, c)];
-            } else {
-                const int64_t q = k - nW;
-                const int r = ((int)q % (Nz + 2)), c = ((int)q / (Nz + 2));
-                s += -L.invh * x[L.iP(r, c)] + L.invh * x[L.iP(r, c + 1)];
-            }
-        }
-        return s;
-    }
-    const int64_t q = k - nW - nU;
-    const int r = ((int)q % (Nz + 2)), c = ((int)q / (Nz + 2));
-    const bool pin = is_pin(F, r, c, Nz);
-    const bool interior = r >= 1 && r <= Nz && c >= 1 && c <= Nx;
-    if (pin && (F.bnchm || !interior)) return x[k];
-    if (r == 0) return x[L.iP(0, c)] - x[L.iP(1, c)];
-    if (r == Nz + 1) return x[L.iP(Nz + 1, c)] - x[L.iP(Nz, c)];
-    if (c == 0) return x[L.iP(r, 0)] - x[L.iP(r, Nx)];
-    if (c == Nx + 1) return x[L.iP(r, Nx + 1)] - x[L.iP(r, 1)];
-    double s = -L.RHOU(r - 1, c - 1) * L.invh * x[L.iU(r, c - 1)] + L.RHOU(r - 1, c) * L.invh * x[L.iU(r, c)] -
-               L.RHOW(r - 1, c - 1) * L.invh * x[L.iW(r - 1, c)] + L.RHOW(r, c - 1) * L.invh * x[L.iW(r, c)];
-    if (pin) s += x[k];
-    return s;
-}
-
-__global__ __launch_bounds__(NT) void k_ll_apply(LevD L, Flags F, const double* __restrict__ x, double* __restrict__ y,
-                                                 int64_t n) {
-    const int64_t k = gidx();
-    if (k < n) y[k] = ll_row(L, F, x, k);
-}
-
-// r = b - LL x
-__global__ __launch_bounds__(NT) void k_ll_resid(LevD L, Flags F, const double* __restrict__ x,
-                                                 const double* __restrict__ b, double* __restrict__ r, int64_t n) {
-    const int64_t k = gidx();
-    if (k < n) r[k] = b[k] - ll_row(L, F, x, k);
-}
-
 // damped Jacobi on the velocity block: xo = x + w (b - A x) / diag (BC rows with weight 1)
 // 1 / d for the smoother: hardware reciprocal refined by two Newton-Raphson steps (d: a nonzero normal
 // diagonal entry)
@@ -223,13 +175,6 @@ __device__ __forceinline__ void jacobi_at(const LevD& L, const Flags& F, const d
     const double ax = vel_row<true>(L, F, x, k, &dg, &bc, fine != 0);
     xo[k] = x[k] + (bc ? 1.0 : omega) * (b[k] - ax) * recip(dg);
 }
-__global__ __launch_bounds__(NT) void k_jacobi(LevD L, Flags F, const double* __restrict__ x,
-                                               const double* __restrict__ b, double* __restrict__ xo, double omega,
-                                               int fine, int zero_guess) {
-    const int64_t k = gidx();
-    if (k < L.nW() + L.nU()) jacobi_at(L, F, x, b, xo, omega, fine, zero_guess, k);
-}
-
 __device__ __forceinline__ void vel_resid_at(const LevD& L, const Flags& F, const double* __restrict__ x,
                                              const double* __restrict__ b, double* __restrict__ r, int fine,
                                              int64_t k) {
@@ -237,40 +182,6 @@ __device__ __forceinline__ void vel_resid_at(const LevD& L, const Flags& F, cons
     bool bc;
     r[k] = b[k] - vel_row<true>(L, F, x, k, &dg, &bc, fine != 0);
 }
-// level-0 right-hand side b = r_v - GG z_p (stored) and the first damped-Jacobi sweep from zero
-__global__ __launch_bounds__(NT) void k_jacobi0_grad(LevD L, Flags F, const double* __restrict__ rV,
-                                                     const double* __restrict__ zP, double* __restrict__ b,
-                                                     double* __restrict__ xo, double omega) {
-    const int64_t k = gidx();
-    const int64_t nW = L.nW(), n = nW + L.nU();
-    if (k >= n) return;
-    const int Nz = L.Nz, Nx = L.Nx;
-    double g = 0.0;
-    if (k != F.nu0) {
-        if (k < nW) {
-            const int r = ((int)k % (Nz + 1)), c = ((int)k / (Nz + 1));
-            if (r >= 1 && r <= Nz - 1 && c >= 1 && c <= Nx)
-                g = -L.invh * zP[r + (int64_t)c * (Nz + 2)] + L.invh * zP[r + 1 + (int64_t)c * (Nz + 2)];
-        } else {
-            const int64_t q = k - nW;
-            const int r = ((int)q % (Nz + 2)), c = ((int)q / (Nz + 2));
-            if (r >= 1 && r <= Nz)
-                g = -L.invh * zP[r + (int64_t)c * (Nz + 2)] + L.invh * zP[r + (int64_t)(c + 1) * (Nz + 2)];
-        }
-    }
-    const double bk = rV[k] - g;
-    b[k] = bk;
-    double dg;
-    bool bc;
-    vel_row<true>(L, F, rV, k, &dg, &bc, true);  // diagonal only (the row value is unused)
-    xo[k] = (bc ? 1.0 : omega) * bk * recip(dg);
-}
-__global__ __launch_bounds__(NT) void k_vel_resid(LevD L, Flags F, const double* __restrict__ x,
-                                                  const double* __restrict__ b, double* __restrict__ r, int fine) {
-    const int64_t k = gidx();
-    if (k < L.nW() + L.nU()) vel_resid_at(L, F, x, b, r, fine, k);
-}
-
 // ---- the smoother and the level residual on the grid of cells: one lane per (r, c), r in [0, Nz+1],
 // c in [0, Nx+1], computing the W row (r, c) (r <= Nz) and the U row (r, c) (c <= Nx) together.  A wave
 // holds 64 consecutive rows of one column, so every column offset is a scalar and every operand is a
@@ -424,6 +335,133 @@ __global__ __launch_bounds__(NT) void k_smooth_cells(LV L, Flags F, const TX* __
         if (MODE == 3) bout[ku] = (TO)bu;
     }
 }
+// ---- LL x (RESID: b - LL x) on the grid of cells: one lane per (r, c), r in [0, Nz+1], c in [0, Nx+1],
+// computing the W row (r <= Nz), the U row (c <= Nx) and the P row of that position with the operations
+// of the reference's rows (fluidmech.jl:145-488) in the order vel_row and the pressure rows use, from
+// scalar column bases plus the lane's row offset.
+template <bool RESID>
+__global__ __launch_bounds__(NT) void k_ll_cells(LevD L, Flags F, const double* __restrict__ x,
+                                                 const double* __restrict__ b, double* __restrict__ y) {
+    const int Nz = L.Nz, Nx = L.Nx;
+    const int segs = (Nz + 2 + 63) >> 6;
+    const int task = __builtin_amdgcn_readfirstlane(xcd_tile(blockIdx.x, gridDim.x) * (NT / 64) + (threadIdx.x >> 6));
+    const int c = task / segs;
+    if (c > Nx + 1) return;
+    const int lane = threadIdx.x & 63;
+    const int r = ((task - c * segs) << 6) + lane;
+    if (r > Nz + 1) return;
+    const int ru = r;                            // U and P row
+    const int rw = r <= Nz ? r : Nz;             // W row
+    const int um = ru > 0 ? ru - 1 : 0, up = ru <= Nz ? ru + 1 : Nz + 1;
+    const int wm = rw > 0 ? rw - 1 : 0, wp = rw < Nz ? rw + 1 : Nz;
+    const int em = rw > 0 ? (rw - 1 < Nz - 1 ? rw - 1 : Nz - 1) : 0;
+    const int e0 = rw < Nz - 1 ? rw : Nz - 1;
+    const int ldW = Nz + 1, ldU = Nz + 2, ldP = Nz + 2;
+    const int cw = c;
+    const int cwm = c > 0 ? c - 1 : 0, cwp = c <= Nx ? c + 1 : Nx + 1;
+    const int cu = c <= Nx ? c : Nx;
+    const int cul = c == 0 ? Nx - 1 : (c - 1 <= Nx ? c - 1 : Nx);
+    const int cur = c + 1 >= Nx + 1 ? (c + 1 == Nx + 1 ? 1 : Nx) : c + 1;
+    const int cl = c == 0 ? Nx - 1 : (c - 1 < Nx ? c - 1 : Nx - 1);
+    const int cr = c < Nx ? c : 0;
+    const int ceco = c <= Nx ? c : Nx, cecm = c > 0 ? (c - 1 <= Nx ? c - 1 : Nx) : 0;
+    const int64_t offU = (int64_t)ldW * (Nx + 2), offP = offU + (int64_t)ldU * (Nx + 1);
+    const double* xU = x + offU;
+    const double* xP = x + offP;
+    const double* Wc = x + (int64_t)cw * ldW;
+    const double* Uc = xU + (int64_t)cu * ldU;
+    const double* Pc = xP + (int64_t)c * ldP;
+    const double Wr = ldr(Wc, rw), Wn = ldr(Wc, wm), Ws = ldr(Wc, wp);
+    const double Ww = ldr(x + (int64_t)cwm * ldW, rw);
+    const double* Wpc = x + (int64_t)cwp * ldW;
+    const double We = ldr(Wpc, rw), Wen = ldr(Wpc, wm);
+    const double Ur = ldr(Uc, ru), Un = ldr(Uc, um), Us = ldr(Uc, up);
+    const double* Ulc = xU + (int64_t)cul * ldU;
+    const double Ul = ldr(Ulc, ru), Uls = ldr(Ulc, up);
+    const double Urt = ldr(xU + (int64_t)cur * ldU, ru);
+    const double Pr = ldr(Pc, ru), Pn = ldr(Pc, um), Ps = ldr(Pc, up);
+    const double Pe = ldr(xP + (int64_t)(cu + 1) * ldP, ru);
+    double Wg = 0.0, Pg = 0.0;
+    if (c == 0 || c == Nx + 1) {
+        Wg = ldr(x + (int64_t)(c == 0 ? Nx : 1) * ldW, rw);
+        Pg = ldr(xP + (int64_t)(c == 0 ? Nx : 1) * ldP, ru);
+    }
+    const double* EC0 = L.etaco + (int64_t)ceco * (Nz + 1);
+    const double eC1w = ldr(L.etaco + (int64_t)cecm * (Nz + 1), rw), eC2w = ldr(EC0, rw), eC1u = ldr(EC0, wm);
+    const double* El = L.eta + (int64_t)cl * Nz;
+    const double eP1w = ldr(El, em), eP2w = ldr(El, e0), eP2u = ldr(L.eta + (int64_t)cr * Nz, em);
+    const double* Rl = L.rho + (int64_t)cl * Nz;
+    const double rl0 = ldr(Rl, e0), rlm = ldr(Rl, em), rrm = ldr(L.rho + (int64_t)cr * Nz, em);
+    const double* RWc = L.rhow + (int64_t)cl * (Nz + 1);
+    const double rw0 = ldr(RWc, rw), rwm = ldr(RWc, wm);
+    const double ru0 = ldr(L.rhou + (int64_t)cu * Nz, em), rul = ldr(L.rhou + (int64_t)cul * Nz, em);
+    const double i2 = L.invh2, ih = L.invh;
+    const int64_t kw = rw + (int64_t)cw * ldW, ku = offU + ru + (int64_t)cu * ldU, kp = offP + ru + (int64_t)c * ldP;
+    // W row (r, c) (w_row), then GG on the interior rows (fluidmech.jl:343-356)
+    if (r <= Nz) {
+        double v;
+        bool bc = true;
+        if (kw == F.nu0) v = Wr;
+        else if (c == 0 || c == Nx + 1) v = Wr + F.sds * Wg;
+        else if (r == 0) v = Wr;
+        else if (r == Nz) {
+            const double oc = F.open_cnv ? 1.0 : 0.0;
+            const double d = rw0 * ih;
+            v = d * Wr - oc * rwm * ih * Wn + oc * ru0 * ih * Ur - oc * rul * ih * Ul;
+        } else {
+            bc = false;
+            const double C1 = eC1w, C2 = eC2w, P1 = eP1w, P2 = eP2w;
+            double d = F.a1 * rw0 / F.dt + ((2.0 / 3.0) * (P1 + P2) * i2 + 0.5 * (C1 + C2) * i2);
+            if (!F.bnchm) d += (rl0 - rlm) * ih * F.g0 * F.dt;
+            const double T1 = P1 / 3.0, T2 = P2 / 3.0;
+            v = d * Wr - (2.0 / 3.0) * P1 * i2 * Wn - (2.0 / 3.0) * P2 * i2 * Ws - 0.5 * C1 * i2 * Ww -
+                0.5 * C2 * i2 * We - (0.5 * C1 - T1) * i2 * Ul + (0.5 * C1 - T2) * i2 * Uls +
+                (0.5 * C2 - T1) * i2 * Ur - (0.5 * C2 - T2) * i2 * Us;
+        }
+        if (!bc) v += -ih * Pr + ih * Ps;
+        y[kw] = RESID ? ldr(b + (int64_t)cw * ldW, rw) - v : v;
+    }
+    // U row (r, c) (u_row), then GG on the interior rows
+    if (c <= Nx) {
+        double v;
+        bool bc = true;
+        if (ku == F.nu0) v = Ur;
+        else if (r == 0) v = Ur + F.top_cnv * Us;
+        else if (r == Nz + 1) v = Ur + F.bot_cnv * Un;
+        else {
+            bc = false;
+            const double C1 = eC1u, C2 = eC2w, P1 = eP1w, P2 = eP2u;
+            double d = (F.a1 + F.gamma) * ru0 / F.dt + ((2.0 / 3.0) * (P1 + P2) * i2 + 0.5 * (C1 + C2) * i2);
+            if (!F.bnchm) d += (rrm - rlm) * ih * F.g0 * F.dt;
+            const double T1 = P1 / 3.0, T2 = P2 / 3.0;
+            v = d * Ur - (2.0 / 3.0) * P1 * i2 * Ul - (2.0 / 3.0) * P2 * i2 * Urt - 0.5 * C1 * i2 * Un -
+                0.5 * C2 * i2 * Us - (0.5 * C1 - T1) * i2 * Wn + (0.5 * C1 - T2) * i2 * Wen +
+                (0.5 * C2 - T1) * i2 * Wr - (0.5 * C2 - T2) * i2 * We;
+        }
+        if (!bc) v += -ih * Pr + ih * Pe;
+        y[ku] = RESID ? ldr(b + offU + (int64_t)cu * ldU, ru) - v : v;
+    }
+    // P row (r, c): DM, the BC rows and the pins (fluidmech.jl:357-488)
+    {
+        double v;
+        const bool pin = is_pin(F, r, c, Nz);
+        const bool interior = r >= 1 && r <= Nz && c >= 1 && c <= Nx;
+        if (pin && (F.bnchm || !interior)) v = Pr;
+        else if (r == 0) v = Pr - Ps;
+        else if (r == Nz + 1) v = Pr - Pn;
+        else if (c == 0 || c == Nx + 1) v = Pr - Pg;
+        else {
+            v = -rul * ih * Ul + ru0 * ih * Ur - rwm * ih * Wn + rw0 * ih * Wr;
+            if (pin) v += Pr;
+        }
+        y[kp] = RESID ? ldr(b + offP + (int64_t)c * ldP, ru) - v : v;
+    }
+}
+inline unsigned ll_cells_blocks(int Nz, int Nx) {
+    const int64_t tasks = (int64_t)(Nx + 2) * ((Nz + 2 + 63) / 64);
+    return (unsigned)((tasks + NT / 64 - 1) / (NT / 64));
+}
+
 #define SMOOTH_D(M) (k_smooth_cells<M, LevD, double, double, double>)
 #define SMOOTH_F(M, TB, TO) (k_smooth_cells<M, LevF, float, TB, TO>)
 // single <-> double precision copies
@@ -2060,7 +2098,8 @@ void precond(xc_ctx* ctx, StokesCtx* s, const Flags& F, const double* v, double*
     double* own_x = L.x;
     const bool direct = s->leff > 1 && (mo.pre + mo.post) % 2 == 0;
     if (direct) L.x = z;
-    hipLaunchKernelGGL(k_jacobi0_grad, dim3(nblk(nv)), dim3(NT), 0, ctx->stream, d, F, v, zP, L.b, L.t, mo.omega);
+    hipLaunchKernelGGL(SMOOTH_D(3), dim3(smooth_cells_blocks(L.Nz, L.Nx)), dim3(NT), 0, ctx->stream, d, F, L.x, v, L.t,
+                       mo.omega, 1, zP, L.b);
     std::swap(L.x, L.t);
     vcycle(ctx, s, F, 0, mo, true);
     if (!direct) hipLaunchKernelGGL(k_copyv, dim3(nblk(nv)), dim3(NT), 0, ctx->stream, L.x, z, nv);
@@ -2209,7 +2248,8 @@ int precond_full(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, in
     pa.z = z;
     pa.n = n;
     hipLaunchKernelGGL(k_lc_prolong, dim3(nb), dim3(NT), 0, ctx->stream, pa);
-    hipLaunchKernelGGL(k_ll_resid, dim3(nb), dim3(NT), 0, ctx->stream, dd, F, z, v, s->tmp, n);
+    hipLaunchKernelGGL(k_ll_cells<true>, dim3(ll_cells_blocks(dd.Nz, dd.Nx)), dim3(NT), 0, ctx->stream, dd, F, z, v,
+                       s->tmp);
     precond(ctx, s, F, s->tmp, s->rs, mo);  // rs is free inside an Arnoldi cycle
     hipLaunchKernelGGL(k_axpy1, dim3(nb), dim3(NT), 0, ctx->stream, s->rs, z, n);
     return XC_OK;
@@ -2239,7 +2279,8 @@ int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int 
     double* cf = s->d3 + 3 * (m + 2);     // update coefficients and nu
     volatile double* hout = s->host + 8;  // 2 per step: provisional |g[j+1]|, nu_{j+1}
     while (true) {
-        hipLaunchKernelGGL(k_ll_resid, dim3(nb), dim3(NT), 0, ctx->stream, dd, F, x, b, s->rs, n);
+        hipLaunchKernelGGL(k_ll_cells<true>, dim3(ll_cells_blocks(dd.Nz, dd.Nx)), dim3(NT), 0, ctx->stream, dd, F, x, b,
+                           s->rs);
         if ((st = multidot(ctx, s, s->rs, 0, s->rs, n))) return st;
         const double beta = sqrt(s->host[0]);
         rel = bnorm > 0 ? beta / bnorm : beta;
@@ -2255,7 +2296,8 @@ int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int 
             double* vj = s->V + (int64_t)j * n;  // p_j, provisional until this step's update
             double* zj = s->Z + (int64_t)j * n;
             if ((st = precond_full(ctx, s, F, mo, n, lc, vj, zj))) return st;
-            hipLaunchKernelGGL(k_ll_apply, dim3(nb), dim3(NT), 0, ctx->stream, dd, F, zj, s->ws, n);
+            hipLaunchKernelGGL(k_ll_cells<false>, dim3(ll_cells_blocks(dd.Nz, dd.Nx)), dim3(NT), 0, ctx->stream, dd, F,
+                               zj, zj, s->ws);
             {
                 const dim3 g((unsigned)md_blocks(n)), b(NT), g1(1);
                 switch (md_ept(n)) {
@@ -2327,7 +2369,8 @@ int xc_stokes_apply(xc_ctx* ctx, const xc_fluid* fl, const xc_grid* g, const xc_
     build_levels(ctx, s, fl, g->h);
     const Flags F = make_flags(o, par, (int)g->Nz, (int)g->Nx, bnchm != 0);
     const int64_t n = s->lev[0].nv + (g->Nz + 2) * (g->Nx + 2);
-    hipLaunchKernelGGL(k_ll_apply, dim3(nblk(n)), dim3(NT), 0, ctx->stream, s->lev[0].d(), F, x, y, n);
+    hipLaunchKernelGGL(k_ll_cells<false>, dim3(ll_cells_blocks(g->Nz, g->Nx)), dim3(NT), 0, ctx->stream, s->lev[0].d(), F,
+                       x, x, y);
     return xc_finish(ctx, "stokes_apply");
 }
 
