@@ -190,7 +190,8 @@ __device__ __forceinline__ void vel_resid_at(const LevD& L, const Flags& F, cons
 // MODE 0: damped Jacobi sweep out = x + w (b - A x) / diag (BC rows weight 1); 1: the same from x = 0
 // (x is not read); 2: residual out = b - A x; 3: the first sweep of the fine level from x = 0 with the
 // right-hand side b = r_v - GG z_p formed here from the Krylov vector's velocity part (b) and pressure
-// part (zP), and stored to bout.
+// part (zP), and stored to bout; 4: the first sweep of a coarse level from x = 0 with its right-hand side
+// restricted here from the finer level's residual (b, of that level's size), and stored to bout.
 template <typename T>
 __device__ __forceinline__ T ldr(const T* __restrict__ col, int row) {
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(col) + (size_t)((unsigned)row * (unsigned)sizeof(T)));
@@ -245,7 +246,30 @@ __global__ __launch_bounds__(NT) void k_smooth_cells(LV L, Flags F, const TX* __
         Urt = (R)ldr(x + offU + (int64_t)cur * ldU, ru);
         if (c == 0 || c == Nx + 1) Wg = (R)ldr(x + (int64_t)(c == 0 ? Nx : 1) * ldW, rw);
     }
-    double bw = (double)ldr(b + (int64_t)cw * ldW, rw), bu = (double)ldr(b + offU + (int64_t)cu * ldU, ru);
+    double bw, bu;
+    if (MODE == 4) {  // b = the full-weighting restriction of the finer level's residual (restrict_at)
+        const int fz = 2 * Nz, fx = 2 * Nx, ldWf = fz + 1, ldUf = fz + 2;
+        const int64_t offUf = (int64_t)ldWf * (fx + 2);
+        const int rr = rw < 1 ? 1 : (rw > Nz - 1 ? Nz - 1 : rw), cc = c < 1 ? 1 : (c > Nx ? Nx : c);
+        const TB* F0 = b + (int64_t)(2 * cc - 1) * ldWf;
+        const TB* F1 = b + (int64_t)(2 * cc) * ldWf;
+        const double vw = (2.0 * (double)ldr(F0, 2 * rr) + 2.0 * (double)ldr(F1, 2 * rr) + (double)ldr(F0, 2 * rr - 1) +
+                           (double)ldr(F1, 2 * rr - 1) + (double)ldr(F0, 2 * rr + 1) + (double)ldr(F1, 2 * rr + 1)) *
+                          0.125;
+        bw = r >= 1 && r <= Nz - 1 && c >= 1 && c <= Nx ? vw : 0.0;
+        const int ur = ru < 1 ? 1 : (ru > Nz ? Nz : ru);
+        const int f = 2 * cu, fl = f == 0 ? fx - 1 : f - 1, fr = f == fx ? 1 : f + 1;
+        const TB* G0 = b + offUf + (int64_t)f * ldUf;
+        const TB* Gl = b + offUf + (int64_t)fl * ldUf;
+        const TB* Gr = b + offUf + (int64_t)fr * ldUf;
+        const double vu = (2.0 * (double)ldr(G0, 2 * ur - 1) + 2.0 * (double)ldr(G0, 2 * ur) + (double)ldr(Gl, 2 * ur - 1) +
+                           (double)ldr(Gl, 2 * ur) + (double)ldr(Gr, 2 * ur - 1) + (double)ldr(Gr, 2 * ur)) *
+                          0.125;
+        bu = r >= 1 && r <= Nz ? vu : 0.0;
+    } else {
+        bw = (double)ldr(b + (int64_t)cw * ldW, rw);
+        bu = (double)ldr(b + offU + (int64_t)cu * ldU, ru);
+    }
     if (MODE == 3) {  // b = r_v - GG z_p (fluidmech.jl:343-356): interior W and U rows, not at the MMS pin
         const int ldP = Nz + 2;
         const double* Pc = zP + (int64_t)cu * ldP;
@@ -319,7 +343,7 @@ __global__ __launch_bounds__(NT) void k_smooth_cells(LV L, Flags F, const TX* __
     if (MODE == 2) {
         ow = (R)bw - vw;
         ou = (R)bu - vu;
-    } else if (MODE == 1 || MODE == 3) {
+    } else if (MODE == 1 || MODE == 3 || MODE == 4) {
         ow = (bcw ? (R)1 : om) * (R)bw * recip(dw);
         ou = (bcu ? (R)1 : om) * (R)bu * recip(du);
     } else {
@@ -328,11 +352,11 @@ __global__ __launch_bounds__(NT) void k_smooth_cells(LV L, Flags F, const TX* __
     }
     if (r <= Nz) {
         out[kw] = (TO)ow;
-        if (MODE == 3) bout[kw] = (TO)bw;
+        if (MODE >= 3) bout[kw] = (TO)bw;
     }
     if (c <= Nx) {
         out[ku] = (TO)ou;
-        if (MODE == 3) bout[ku] = (TO)bu;
+        if (MODE >= 3) bout[ku] = (TO)bu;
     }
 }
 // ---- LL x (RESID: b - LL x) on the grid of cells: one lane per (r, c), r in [0, Nz+1], c in [0, Nx+1],
@@ -462,6 +486,53 @@ inline unsigned ll_cells_blocks(int Nz, int Nx) {
     return (unsigned)((tasks + NT / 64 - 1) / (NT / 64));
 }
 
+// bilinear prolongation of the coarse correction ec added to the fine iterate xf (prolong_at), on the fine
+// level's grid of cells: the coarse columns are wave-uniform, the coarse rows per lane
+template <typename TF>
+__global__ __launch_bounds__(NT) void k_prolong_cells(int Nz, int Nx, Flags F, const double* __restrict__ ec,
+                                                      TF* __restrict__ xf) {
+    const int segs = (Nz + 2 + 63) >> 6;
+    const int task = __builtin_amdgcn_readfirstlane(xcd_tile(blockIdx.x, gridDim.x) * (NT / 64) + (threadIdx.x >> 6));
+    const int c = task / segs;
+    if (c > Nx + 1) return;
+    const int r = ((task - c * segs) << 6) + (threadIdx.x & 63);
+    if (r > Nz + 1) return;
+    const int Zc = Nz / 2, Xc = Nx / 2, ldWc = Zc + 1, ldUc = Zc + 2;
+    const int64_t offUc = (int64_t)ldWc * (Xc + 2), offUf = (int64_t)(Nz + 1) * (Nx + 2);
+    if (r >= 1 && r <= Nz - 1 && c >= 1 && c <= Nx) {  // W (r, c)
+        // coarse W column value with the periodic/mirror ghost map of the BC rows (wave-uniform columns)
+        const int I = (c + 1) / 2, In = (c & 1) ? I - 1 : I + 1;
+        const int cI = I == 0 ? Xc : (I == Xc + 1 ? 1 : I), cN = In == 0 ? Xc : (In == Xc + 1 ? 1 : In);
+        const double fI = (I == 0 || I == Xc + 1) ? -F.sds : 1.0, fN = (In == 0 || In == Xc + 1) ? -F.sds : 1.0;
+        const double* eI = ec + (int64_t)cI * ldWc;
+        const double* eN = ec + (int64_t)cN * ldWc;
+        auto EX = [&](int J) -> double {  // x-interpolation at fine column c, coarse row J (0 outside 1..Zc-1)
+            const int Jc = J < 0 ? 0 : (J > Zc ? Zc : J);
+            const double a = fI * ldr(eI, Jc), b = fN * ldr(eN, Jc);
+            return J <= 0 || J >= Zc ? 0.0 : 0.75 * a + 0.25 * b;
+        };
+        const int J = r / 2;
+        const double v = (r & 1) ? 0.5 * (EX(J) + EX(J + 1)) : EX(J);
+        const int64_t k = r + (int64_t)c * (Nz + 1);
+        xf[k] += v;
+    }
+    if (r >= 1 && r <= Nz && c <= Nx) {  // U (r, c)
+        const int I = c / 2;
+        const double* u0 = ec + offUc + (int64_t)I * ldUc;
+        const double* u1 = ec + offUc + (int64_t)(I + 1 <= Xc ? I + 1 : Xc) * ldUc;
+        auto E = [&](const double* col, int J) -> double {
+            if (J == 0) return -F.top_cnv * ldr(col, 1);
+            if (J == Zc + 1) return -F.bot_cnv * ldr(col, Zc);
+            return ldr(col, J);
+        };
+        const int J = (r + 1) / 2, Jn = (r & 1) ? J - 1 : J + 1;
+        auto EZ = [&](const double* col) -> double { return 0.75 * E(col, J) + 0.25 * E(col, Jn); };
+        const double v = (c & 1) ? 0.5 * (EZ(u0) + EZ(u1)) : EZ(u0);
+        const int64_t k = offUf + r + (int64_t)c * (Nz + 2);
+        xf[k] += v;
+    }
+}
+
 #define SMOOTH_D(M) (k_smooth_cells<M, LevD, double, double, double>)
 #define SMOOTH_F(M, TB, TO) (k_smooth_cells<M, LevF, float, TB, TO>)
 // single <-> double precision copies
@@ -552,12 +623,6 @@ __device__ __forceinline__ void prolong_at(const LevD& Lf, const LevD& Lc, const
         const double v = (c & 1) ? 0.5 * (EZ(I) + EZ(I + 1)) : EZ(I);
         xf[k] += v;
     }
-}
-template <typename TF>
-__global__ __launch_bounds__(NT) void k_prolong(LevD Lf, LevD Lc, Flags F, const double* __restrict__ ec,
-                                                TF* __restrict__ xf) {
-    const int64_t k = gidx();
-    if (k < Lf.nW() + Lf.nU()) prolong_at(Lf, Lc, F, ec, xf, k);
 }
 
 // The coarse end of the V-cycle (levels of at most TAIL_CELLS cells) in one
@@ -1963,8 +2028,26 @@ int coarse_sweeps(const StokesCtx* s, const MGOpts& mo) {
     return s->leff < (int)s->lev.size() ? 2 : 20;
 }
 
+void vcycle(xc_ctx* ctx, StokesCtx* s, const Flags& F, size_t l, const MGOpts& mo, bool first_done = false);
+
+// level l's residual rf restricted to level l + 1 and the cycle from there; the restriction is fused into
+// level l + 1's first sweep (from zero) unless that level starts the one-workgroup tail
+template <typename TF>
+void descend(xc_ctx* ctx, StokesCtx* s, const Flags& F, size_t l, const MGOpts& mo, const TF* rf) {
+    Level& C = s->lev[l + 1];
+    if ((int)(l + 1) == s->tail_l && s->leff > s->tail_l) {
+        hipLaunchKernelGGL(k_restrict<TF>, dim3(nblk(C.nv)), dim3(NT), 0, ctx->stream, s->lev[l].d(), C.d(), rf, C.b);
+        vcycle(ctx, s, F, l + 1, mo);
+        return;
+    }
+    hipLaunchKernelGGL((k_smooth_cells<4, LevD, double, TF, double>), dim3(smooth_cells_blocks(C.Nz, C.Nx)), dim3(NT), 0,
+                       ctx->stream, C.d(), F, C.x, rf, C.t, mo.omega, 0, nullptr, C.b);
+    std::swap(C.x, C.t);
+    vcycle(ctx, s, F, l + 1, mo, true);
+}
+
 // one V-cycle on level l: solve A x = b approximately from x = 0
-void vcycle(xc_ctx* ctx, StokesCtx* s, const Flags& F, size_t l, const MGOpts& mo, bool first_done = false) {
+void vcycle(xc_ctx* ctx, StokesCtx* s, const Flags& F, size_t l, const MGOpts& mo, bool first_done) {
     Level& L = s->lev[l];
     if ((int)l == s->tail_l && s->leff > s->tail_l) {
         TailArgs ta = s->tail;
@@ -1981,7 +2064,7 @@ void vcycle(xc_ctx* ctx, StokesCtx* s, const Flags& F, size_t l, const MGOpts& m
         return;
     }
     const LevD d = L.d();
-    const unsigned nb = nblk(L.nv), nbc = smooth_cells_blocks(L.Nz, L.Nx);
+    const unsigned nbc = smooth_cells_blocks(L.Nz, L.Nx);
     const int fine = l == 0;
     const bool coarsest = (int)l + 1 == s->leff;
     const int pre = coarsest ? coarse_sweeps(s, mo) : (l > 0 && mo.sub > 0 ? mo.sub : mo.pre);
@@ -1997,10 +2080,8 @@ void vcycle(xc_ctx* ctx, StokesCtx* s, const Flags& F, size_t l, const MGOpts& m
     }
     if (coarsest) return;
     hipLaunchKernelGGL(SMOOTH_D(2), dim3(nbc), dim3(NT), 0, ctx->stream, d, F, L.x, L.b, L.r, 0.0, fine, nullptr, nullptr);
-    Level& C = s->lev[l + 1];
-    hipLaunchKernelGGL(k_restrict<double>, dim3(nblk(C.nv)), dim3(NT), 0, ctx->stream, d, C.d(), L.r, C.b);
-    vcycle(ctx, s, F, l + 1, mo);
-    hipLaunchKernelGGL(k_prolong<double>, dim3(nb), dim3(NT), 0, ctx->stream, d, C.d(), F, C.x, L.x);
+    descend(ctx, s, F, l, mo, (const double*)L.r);
+    hipLaunchKernelGGL(k_prolong_cells<double>, dim3(nbc), dim3(NT), 0, ctx->stream, L.Nz, L.Nx, F, s->lev[l + 1].x, L.x);
     for (int it = 0; it < post; ++it) {
         hipLaunchKernelGGL(SMOOTH_D(0), dim3(nbc), dim3(NT), 0, ctx->stream, d, F, L.x, L.b, L.t, mo.omega, fine, nullptr, nullptr);
         std::swap(L.x, L.t);
@@ -2039,10 +2120,8 @@ void vcycle_f32(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, con
     if (!coarsest) {
         hipLaunchKernelGGL(SMOOTH_F(2, float, float), dim3(nbc), dim3(NT), 0, ctx->stream, f, F, x, s->fb, s->fr, 0.0,
                            1, nullptr, nullptr);
-        Level& C = s->lev[1];
-        hipLaunchKernelGGL(k_restrict<float>, dim3(nblk(C.nv)), dim3(NT), 0, ctx->stream, d, C.d(), s->fr, C.b);
-        vcycle(ctx, s, F, 1, mo);
-        hipLaunchKernelGGL(k_prolong<float>, dim3(nblk(L.nv)), dim3(NT), 0, ctx->stream, d, C.d(), F, C.x, x);
+        descend(ctx, s, F, 0, mo, (const float*)s->fr);
+        hipLaunchKernelGGL(k_prolong_cells<float>, dim3(nbc), dim3(NT), 0, ctx->stream, L.Nz, L.Nx, F, s->lev[1].x, x);
         for (int it = 0; it < post; ++it) sweep(it == post - 1);
     }
     if (!in_z) hipLaunchKernelGGL(k_f2d, dim3(nblk(L.nv)), dim3(NT), 0, ctx->stream, x, z, L.nv);
