@@ -171,6 +171,104 @@ __device__ double2* lds_fft(double2* a, double2* b, int nl, const Plan& p) {
     return src;
 }
 
+// Power-of-two transforms of length 2^LG in which every stage has at most FFT_NT butterflies over the nl
+// lines (one per lane): the lane's twiddles of every stage are fetched once, up front (fft_p2_twiddles,
+// issued with the caller's data loads), instead of at each stage.  The stages are those fft_factorize
+// gives (LG/2 of radix 4, then one of radix 2 when LG is odd), with stage<R, INV, true>'s arithmetic, so
+// the results are the same.  LG = 0: not this path.
+template <int LG>
+struct P2Fast {
+    static constexpr int N4 = LG / 2, N2 = LG & 1, NST = N4 + N2 > 0 ? N4 + N2 : 1;
+};
+// host: LG for a plan and nl lines if the fast path applies (8 <= LG <= 10), else 0
+inline int p2_fast_lg(const Plan& p, int nl) {
+    const int lg = p.lg1 - 1;
+    if (p.lg1 <= 0 || lg < 8 || lg > 10) return 0;
+    for (int s = 0; s < p.nst; ++s)
+        if (p.rad[s] != (s < lg / 2 ? 4 : 2)) return 0;
+    if ((int64_t)nl * ((1 << lg) >> ((lg & 1) ? 1 : 2)) > FFT_NT) return 0;
+    return lg;
+}
+template <int LG>
+__device__ __forceinline__ void fft_p2_twiddles(int nl, const double2* __restrict__ tw,
+                                                double2 (&w)[P2Fast<LG>::NST][3]) {
+    constexpr int n = 1 << LG;
+    const int k = threadIdx.x;
+    int lgNs = 0;
+#pragma unroll
+    for (int s = 0; s < P2Fast<LG>::NST; ++s) {
+        const int LR = s < P2Fast<LG>::N4 ? 2 : 1;
+        const int nR = n >> LR;
+        const int kk = k < nl * nR ? k : 0;
+        const int l = kk >> (LG - LR), j = kk - l * nR;
+        const int jm = j & ((1 << lgNs) - 1);
+        const int t0 = jm << (LG - lgNs - LR);
+#pragma unroll
+        for (int r = 1; r < 4; ++r)
+            if (r < (1 << LR)) w[s][r - 1] = tw[(t0 * r) & (n - 1)];
+        lgNs += LR;
+    }
+}
+template <int R, bool INV, int LG>
+__device__ __forceinline__ void stage_p2_one(const double2* __restrict__ src, double2* __restrict__ dst, int nl,
+                                             int lgNs, const double2 (&wr)[3]) {
+    constexpr int LR = R == 2 ? 1 : 2;
+    constexpr int n = 1 << LG, nR = n >> LR;
+    const int k = threadIdx.x;
+    if (k >= nl * nR) return;
+    const int l = k >> (LG - LR), j = k - l * nR;
+    const int base = l * n;
+    const int jm = j & ((1 << lgNs) - 1);
+    double2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        double2 x = src[base + j + r * nR];
+        if (r > 0) {
+            double2 w = wr[r - 1];
+            if (INV) w.y = -w.y;
+            x = cmul(x, w);
+        }
+        v[r] = x;
+    }
+    double2 o[R];
+    if constexpr (R == 2) {
+        o[0] = cadd(v[0], v[1]);
+        o[1] = csub(v[0], v[1]);
+    } else {
+        const double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+        const double2 s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
+        const double2 jd13 = INV ? make_double2(-d13.y, d13.x) : make_double2(d13.y, -d13.x);
+        o[0] = cadd(s02, s13);
+        o[2] = csub(s02, s13);
+        o[1] = cadd(d02, jd13);
+        o[3] = csub(d02, jd13);
+    }
+    const int idxD = ((j >> lgNs) << (lgNs + LR)) + jm;
+#pragma unroll
+    for (int r = 0; r < R; ++r) dst[base + idxD + r * (1 << lgNs)] = o[r];
+}
+template <bool INV, int LG>
+__device__ double2* lds_fft_p2(double2* a, double2* b, int nl, const double2 (&w)[P2Fast<LG>::NST][3]) {
+    double2* src = a;
+    double2* dst = b;
+    int lgNs = 0;
+#pragma unroll
+    for (int s = 0; s < P2Fast<LG>::NST; ++s) {
+        if (s < P2Fast<LG>::N4) {
+            stage_p2_one<4, INV, LG>(src, dst, nl, lgNs, w[s]);
+            lgNs += 2;
+        } else {
+            stage_p2_one<2, INV, LG>(src, dst, nl, lgNs, w[s]);
+            lgNs += 1;
+        }
+        __syncthreads();
+        double2* t = src;
+        src = dst;
+        dst = t;
+    }
+    return src;
+}
+
 // host: radix sequence of n (false if it needs more than MAXST stages)
 inline bool fft_factorize(int n, std::vector<int>& rad) {
     rad.clear();

@@ -862,8 +862,11 @@ struct PoisXArgs {
     Plan p;
     int Nz, Nx, K, nl;
 };
+template <int LG>  // 8..10: the power-of-two path with prefetched twiddles (P2Fast); 0: lds_fft
 __global__ __launch_bounds__(FFT_NT) void k_pois_xfwd(PoisXArgs a) {
     extern __shared__ double2 sm[];
+    double2 w[P2Fast<LG>::NST][3];
+    if constexpr (LG > 0) fft_p2_twiddles<LG>(a.nl, a.p.tw, w);
     const int n = a.Nx, nl = a.nl, rows = 2 * nl;
     // neighbouring row blocks share the cache lines of the column-major P arrays: keep them on one XCD
     const int r0 = xcd_tile(blockIdx.x, gridDim.x) * rows;
@@ -876,7 +879,9 @@ __global__ __launch_bounds__(FFT_NT) void k_pois_xfwd(PoisXArgs a) {
         reinterpret_cast<double*>(&A[(rr >> 1) * n + c])[rr & 1] = v;
     }
     __syncthreads();
-    const double2* X = lds_fft<false>(A, B, nl, a.p);
+    const double2* X;
+    if constexpr (LG > 0) X = lds_fft_p2<false, LG>(A, B, nl, w);
+    else X = lds_fft<false>(A, B, nl, a.p);
     for (int k = threadIdx.x; k < nl * a.K; k += FFT_NT) {
         const int l = k / a.K, f = k - l * a.K;
         const double2 zk = X[l * n + f], zm = X[l * n + (f == 0 ? 0 : n - f)];
@@ -898,8 +903,11 @@ struct PoisZArgs {
     const double* lamz;  // 2 Nz: 2 - 2 cos(2 pi kz / (2 Nz))
     int Nz, K, nl;
 };
+template <int LG>
 __global__ __launch_bounds__(FFT_NT) void k_pois_zfft(PoisZArgs a) {
     extern __shared__ double2 sm[];
+    double2 w[P2Fast<LG>::NST][3];
+    if constexpr (LG > 0) fft_p2_twiddles<LG>(a.nl, a.p.tw, w);
     const int n = 2 * a.Nz, nl = a.nl, K = a.K;
     const int k0 = xcd_tile(blockIdx.x, gridDim.x) * nl;  // neighbouring wavenumbers share S's cache lines
     double2* A = sm;
@@ -910,19 +918,35 @@ __global__ __launch_bounds__(FFT_NT) void k_pois_zfft(PoisZArgs a) {
         const int r = zz < a.Nz ? zz : n - 1 - zz;
         A[i] = k < K ? a.S[(int64_t)r * K + k] : make_double2(0.0, 0.0);
     }
-    __syncthreads();
-    double2* X = lds_fft<false>(A, B, nl, a.p);
-    const double inv_n = 1.0 / (double)n;
-    for (int i = threadIdx.x; i < nl * n; i += FFT_NT) {
-        const int l = i / n, kz = i - l * n;
+    // the eigenvalues of this lane's entries, fetched before the transform (at most ZF_PER entries a lane)
+    constexpr int ZF_PER = 2 * MAXLINE / FFT_NT;
+    double lam[ZF_PER];
+#pragma unroll
+    for (int q = 0; q < ZF_PER; ++q) {
+        const int i = threadIdx.x + q * FFT_NT;
+        const int ic = i < nl * n ? i : 0;
+        const int l = ic / n, kz = ic - l * n;
         const int k = k0 + l;
-        const double lam = (k < K ? a.lamx[k] : 1.0) + a.lamz[kz];
-        const double f = lam > 0.0 ? inv_n / lam : 0.0;
-        X[i].x *= f;
-        X[i].y *= f;
+        lam[q] = (k < K ? a.lamx[k] : 1.0) + a.lamz[kz];
     }
     __syncthreads();
-    const double2* Y = lds_fft<true>(X, X == A ? B : A, nl, a.p);
+    double2* X;
+    if constexpr (LG > 0) X = lds_fft_p2<false, LG>(A, B, nl, w);
+    else X = lds_fft<false>(A, B, nl, a.p);
+    const double inv_n = 1.0 / (double)n;
+#pragma unroll
+    for (int q = 0; q < ZF_PER; ++q) {
+        const int i = threadIdx.x + q * FFT_NT;
+        if (i < nl * n) {
+            const double f = lam[q] > 0.0 ? inv_n / lam[q] : 0.0;
+            X[i].x *= f;
+            X[i].y *= f;
+        }
+    }
+    __syncthreads();
+    const double2* Y;
+    if constexpr (LG > 0) Y = lds_fft_p2<true, LG>(X, X == A ? B : A, nl, w);
+    else Y = lds_fft<true>(X, X == A ? B : A, nl, a.p);
     for (int i = threadIdx.x; i < nl * a.Nz; i += FFT_NT) {
         const int l = i / a.Nz, r = i - l * a.Nz;
         const int k = k0 + l;
@@ -942,8 +966,11 @@ struct PoisInvArgs {
     double scale;   // 1 / (cz Nx)
     int Nz, Nx, K, nl;
 };
+template <int LG>
 __global__ __launch_bounds__(FFT_NT) void k_pois_xinv(PoisInvArgs a) {
     extern __shared__ double2 sm[];
+    double2 w[P2Fast<LG>::NST][3];
+    if constexpr (LG > 0) fft_p2_twiddles<LG>(a.nl, a.p.tw, w);
     const int n = a.Nx, nl = a.nl, rows = 2 * nl, K = a.K;
     const int r0 = xcd_tile(blockIdx.x, gridDim.x) * rows;  // row blocks of one XCD share cache lines
     double2* A = sm;
@@ -959,7 +986,9 @@ __global__ __launch_bounds__(FFT_NT) void k_pois_xinv(PoisInvArgs a) {
         A[k] = make_double2(av.x - bv.y, av.y + bv.x);
     }
     __syncthreads();
-    const double2* Y = lds_fft<true>(A, B, nl, a.p);
+    const double2* Y;
+    if constexpr (LG > 0) Y = lds_fft_p2<true, LG>(A, B, nl, w);
+    else Y = lds_fft<true>(A, B, nl, a.p);
     double* Zs = reinterpret_cast<double*>(Y == A ? B : A);  // rows x n interior results for the ghost ring
     const int Nz = a.Nz;
     for (int k = threadIdx.x; k < rows * n; k += FFT_NT) {
@@ -2149,7 +2178,13 @@ void precond(xc_ctx* ctx, StokesCtx* s, const Flags& F, const double* v, double*
         const unsigned nbx = (unsigned)((Nz + 2 * nl - 1) / (2 * nl));
         const size_t lds = 2 * (size_t)nl * Nx * sizeof(double2);
         PoisXArgs xa{vP, s->pS, pl, Nz, Nx, K, nl};
-        hipLaunchKernelGGL(k_pois_xfwd, dim3(nbx), dim3(FFT_NT), lds, ctx->stream, xa);
+        const int lgx = p2_fast_lg(pl, nl);
+        switch (lgx) {
+            case 8: hipLaunchKernelGGL(k_pois_xfwd<8>, dim3(nbx), dim3(FFT_NT), lds, ctx->stream, xa); break;
+            case 9: hipLaunchKernelGGL(k_pois_xfwd<9>, dim3(nbx), dim3(FFT_NT), lds, ctx->stream, xa); break;
+            case 10: hipLaunchKernelGGL(k_pois_xfwd<10>, dim3(nbx), dim3(FFT_NT), lds, ctx->stream, xa); break;
+            default: hipLaunchKernelGGL(k_pois_xfwd<0>, dim3(nbx), dim3(FFT_NT), lds, ctx->stream, xa); break;
+        }
         Plan pz{};
         pz.n = 2 * Nz;
         pz.nst = (int)s->pradz.size();
@@ -2160,10 +2195,23 @@ void precond(xc_ctx* ctx, StokesCtx* s, const Flags& F, const double* v, double*
         const int nlzmax = MAXLINE / (2 * Nz) < 4 ? MAXLINE / (2 * Nz) : 4;
         nlz = nlz < 1 ? 1 : (nlz > nlzmax ? nlzmax : nlz);
         PoisZArgs za{s->pS, pz, s->plamx, s->plamz, Nz, K, nlz};
-        hipLaunchKernelGGL(k_pois_zfft, dim3((unsigned)((K + nlz - 1) / nlz)), dim3(FFT_NT),
-                           2 * (size_t)nlz * 2 * Nz * sizeof(double2), ctx->stream, za);
+        {
+            const dim3 gz((unsigned)((K + nlz - 1) / nlz));
+            const size_t ldz = 2 * (size_t)nlz * 2 * Nz * sizeof(double2);
+            switch (p2_fast_lg(pz, nlz)) {
+                case 8: hipLaunchKernelGGL(k_pois_zfft<8>, gz, dim3(FFT_NT), ldz, ctx->stream, za); break;
+                case 9: hipLaunchKernelGGL(k_pois_zfft<9>, gz, dim3(FFT_NT), ldz, ctx->stream, za); break;
+                case 10: hipLaunchKernelGGL(k_pois_zfft<10>, gz, dim3(FFT_NT), ldz, ctx->stream, za); break;
+                default: hipLaunchKernelGGL(k_pois_zfft<0>, gz, dim3(FFT_NT), ldz, ctx->stream, za); break;
+            }
+        }
         PoisInvArgs ia{s->pS, vP, zP, pl, d, F, s->pscale, Nz, Nx, K, nl};
-        hipLaunchKernelGGL(k_pois_xinv, dim3(nbx), dim3(FFT_NT), lds, ctx->stream, ia);  // with the ghost ring
+        switch (lgx) {  // with the ghost ring
+            case 8: hipLaunchKernelGGL(k_pois_xinv<8>, dim3(nbx), dim3(FFT_NT), lds, ctx->stream, ia); break;
+            case 9: hipLaunchKernelGGL(k_pois_xinv<9>, dim3(nbx), dim3(FFT_NT), lds, ctx->stream, ia); break;
+            case 10: hipLaunchKernelGGL(k_pois_xinv<10>, dim3(nbx), dim3(FFT_NT), lds, ctx->stream, ia); break;
+            default: hipLaunchKernelGGL(k_pois_xinv<0>, dim3(nbx), dim3(FFT_NT), lds, ctx->stream, ia); break;
+        }
     } else {
         hipLaunchKernelGGL(k_schur_int, dim3(nblk(nP)), dim3(NT), 0, ctx->stream, d, F, vP, zP);
         hipLaunchKernelGGL(k_schur_ghost, dim3(nblk(nP)), dim3(NT), 0, ctx->stream, d, F, vP, zP);
