@@ -120,13 +120,16 @@ struct XPassArgs {
     Plan plan[3];
     int Nz, Nx, nl, nblk;
 };
+template <int LG>  // 8..10: the power-of-two path with prefetched twiddles (P2Fast); 0: lds_fft
 __global__ __launch_bounds__(NT) void k_filter_x(XPassArgs a) {
     extern __shared__ double2 sm[];
     const int f = blockIdx.y;
     const Plan& p = a.plan[f];
     const FieldIO io = a.io[f];
     const int n = a.Nx, nl = a.nl, rows = 2 * nl;
-    const int r0 = blockIdx.x * rows;
+    double2 w[P2Fast<LG>::NST][3];
+    if constexpr (LG > 0) fft_p2_twiddles<LG>(nl, p.tw, w);
+    const int r0 = xcd_tile(blockIdx.x, gridDim.x) * rows;  // row blocks sharing cache lines on one XCD
     double2* A = sm;
     double2* B = sm + nl * n;
     for (int k = threadIdx.x; k < rows * n; k += NT) {
@@ -137,14 +140,18 @@ __global__ __launch_bounds__(NT) void k_filter_x(XPassArgs a) {
         slot[rr & 1] = v;
     }
     __syncthreads();
-    double2* X = lds_fft<false>(A, B, nl, p);
+    double2* X;
+    if constexpr (LG > 0) X = lds_fft_p2<false, LG>(A, B, nl, w);
+    else X = lds_fft<false>(A, B, nl, p);
     for (int k = threadIdx.x; k < nl * n; k += NT) {
         const double g = p.g[k % n];
         X[k].x *= g;
         X[k].y *= g;
     }
     __syncthreads();
-    double2* Y = lds_fft<true>(X, X == A ? B : A, nl, p);
+    double2* Y;
+    if constexpr (LG > 0) Y = lds_fft_p2<true, LG>(X, X == A ? B : A, nl, w);
+    else Y = lds_fft<true>(X, X == A ? B : A, nl, p);
     for (int k = threadIdx.x; k < rows * n; k += NT) {
         const int rr = k % rows, c = k / rows;
         const int r = r0 + rr;
@@ -169,7 +176,7 @@ __global__ __launch_bounds__(NT) void k_filter_z(ZPassArgs a) {
     const Plan& p = a.plan[f];
     const FieldIO io = a.io[f];
     const int n = p.n, nl = a.nl[f], cols = 2 * nl, hp = a.hp[f];
-    const int c0 = blockIdx.x * cols;
+    const int c0 = xcd_tile(blockIdx.x, gridDim.x) * cols;
     if (c0 >= a.Nx) return;  // whole workgroup (uniform)
     double2* A = sm;
     double2* B = sm + nl * n;
@@ -432,15 +439,26 @@ int filter_fields(xc_ctx* ctx, int nf, const xc_mat* in, xc_mat* out, xc_mat* tm
         xa.io[f] = FieldIO{in[f].p, in[f].ld, tmp[f].p, tmp[f].ld};
         za.io[f] = FieldIO{tmp[f].p, tmp[f].ld, out[f].p, out[f].ld};
         za.hp[f] = (int)(pad[f] / 2);
-        za.nl[f] = MAXLINE / za.plan[f].n;
-        if (za.nl[f] > 4) za.nl[f] = 4;
+        // one complex line (two columns) per workgroup: these transforms are latency-bound, and more
+        // workgroups keep more CUs busy (the per-line arithmetic does not depend on the grouping)
+        za.nl[f] = 1;
     }
     xa.Nz = za.Nz = Nz;
     xa.Nx = za.Nx = Nx;
-    xa.nl = MAXLINE / Nx;
-    if (xa.nl > 8) xa.nl = 8;
+    // the x plans of the fields are the same length; the power-of-two path when every field's plan has it
+    int lgx = p2_fast_lg(xa.plan[0], 1);
+    for (int f = 1; f < nf; ++f)
+        if (p2_fast_lg(xa.plan[f], 1) != lgx) lgx = 0;
+    xa.nl = 1;
     const int xblocks = (Nz + 2 * xa.nl - 1) / (2 * xa.nl);
-    hipLaunchKernelGGL(k_filter_x, dim3(xblocks, nf), dim3(NT), 2 * xa.nl * Nx * sizeof(double2), ctx->stream, xa);
+    const dim3 gx(xblocks, nf);
+    const size_t ldx = 2 * xa.nl * Nx * sizeof(double2);
+    switch (lgx) {
+        case 8: hipLaunchKernelGGL(k_filter_x<8>, gx, dim3(NT), ldx, ctx->stream, xa); break;
+        case 9: hipLaunchKernelGGL(k_filter_x<9>, gx, dim3(NT), ldx, ctx->stream, xa); break;
+        case 10: hipLaunchKernelGGL(k_filter_x<10>, gx, dim3(NT), ldx, ctx->stream, xa); break;
+        default: hipLaunchKernelGGL(k_filter_x<0>, gx, dim3(NT), ldx, ctx->stream, xa); break;
+    }
     int nlmin = za.nl[0], nmax = za.plan[0].n * za.nl[0];
     for (int f = 1; f < nf; ++f) {
         nlmin = za.nl[f] < nlmin ? za.nl[f] : nlmin;

@@ -1968,7 +1968,9 @@ Flags make_flags(const xc_fm_opts* o, const xc_par* par, int Nz, int Nx, bool bn
 }
 
 // copy the fine-level coefficients densely and build the coarse hierarchy
-void build_levels(xc_ctx* ctx, StokesCtx* s, const xc_fluid* fl, double h) {
+void build_coarse(xc_ctx* ctx, StokesCtx* s, int upto);
+// the finest level's coefficients (and its single-precision copy); all: the coarser levels too
+void build_levels(xc_ctx* ctx, StokesCtx* s, const xc_fluid* fl, double h, bool all = true) {
     s->leff = (int)s->lev.size();  // the full V-cycle unless choose_depth says otherwise
     s->inertia = 0.0;
     Level& L0 = s->lev[0];
@@ -1979,7 +1981,12 @@ void build_levels(xc_ctx* ctx, StokesCtx* s, const xc_fluid* fl, double h) {
     hipLaunchKernelGGL(k_pack, dim3(nblk(fl->rhow.rows * fl->rhow.cols)), dim3(NT), 0, ctx->stream, fl->rhow, L0.rhow);
     hipLaunchKernelGGL(k_pack, dim3(nblk(fl->rhou.rows * fl->rhou.cols)), dim3(NT), 0, ctx->stream, fl->rhou, L0.rhou);
     hipLaunchKernelGGL(k_d2f, dim3(nblk(s->f_ncoef)), dim3(NT), 0, ctx->stream, L0.eta, s->f_coef, s->f_ncoef);
-    for (size_t l = 1; l < s->lev.size(); ++l) {
+    if (all) build_coarse(ctx, s, (int)s->lev.size());
+}
+
+// coefficients of levels 1 .. upto-1 from the finer ones (the levels a cycle of depth upto reads)
+void build_coarse(xc_ctx* ctx, StokesCtx* s, int upto) {
+    for (size_t l = 1; l < (size_t)upto && l < s->lev.size(); ++l) {
         Level& Lf = s->lev[l - 1];
         Level& Lc = s->lev[l];
         Lc.h = 2.0 * Lf.h;
@@ -2529,7 +2536,7 @@ int xc_fluidmech(xc_ctx* ctx, xc_fluid* fl, const xc_phase* ph, const xc_grid* g
     const unsigned nb = nblk(n);
     const MGOpts mo = mg_opts(o);
     // system: coefficients, RR, initial guess
-    build_levels(ctx, s, fl, g->h);
+    build_levels(ctx, s, fl, g->h, false);
     const bool depth_pending = depth_launch(ctx, s, F, o);
     if ((st = pois_table(ctx, s, F, Nz, Nx, g->h))) return st;
     if ((st = build_rhs(ctx, s, fl, ph, g, par, o, bn, F, s->bs))) return st;
@@ -2538,6 +2545,7 @@ int xc_fluidmech(xc_ctx* ctx, xc_fluid* fl, const xc_phase* ph, const xc_grid* g
     if ((st = multidot(ctx, s, s->bs, 0, s->bs, n))) return st;
     const double bnorm = sqrt(s->host[0]);
     if ((st = choose_depth(ctx, s, F, o, g->h, depth_pending, true))) return st;  // multidot synchronized
+    build_coarse(ctx, s, s->leff);
     const double target = o->tol * (bnorm > 0 ? bnorm : 1.0);
     const int maxit = o->maxit > 0 ? o->maxit : 500;
     int iters = 0;
@@ -2592,9 +2600,10 @@ static int piece_setup(xc_ctx* ctx, const xc_fluid* fl, const xc_grid* g, const 
     if ((st = check_fm(ctx, fl, g, o))) return st;
     if ((st = setup(ctx, (int)g->Nz, (int)g->Nx, m, sp, o->mg_levels))) return st;
     StokesCtx* s = *sp;
-    build_levels(ctx, s, fl, g->h);
+    build_levels(ctx, s, fl, g->h, false);
     *F = make_flags(o, par, (int)g->Nz, (int)g->Nx, bnchm != 0);
     if ((st = choose_depth(ctx, s, *F, o, g->h, depth_launch(ctx, s, *F, o), false))) return st;
+    build_coarse(ctx, s, s->leff);
     if ((st = pois_table(ctx, s, *F, (int)g->Nz, (int)g->Nx, g->h))) return st;
     *mo = mg_opts(o);
     *n = s->lev[0].nv + (g->Nz + 2) * (g->Nx + 2);
