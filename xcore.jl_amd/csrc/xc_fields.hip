@@ -124,34 +124,60 @@ __global__ __launch_bounds__(64) void k_rowsum(DM rhow, int rows, int cols, doub
     const int j = blockIdx.x * 64 + threadIdx.x;
     const int c = blockIdx.y;
     if (j >= rows) return;
-    const int i0 = c * ROW_CHUNK, i1 = min(cols, i0 + ROW_CHUNK);
+    const int i0 = c * ROW_CHUNK;
+    double v[ROW_CHUNK];
+#pragma unroll
+    for (int u = 0; u < ROW_CHUNK; ++u) v[u] = i0 + u < cols ? rhow(j, i0 + u) : 0.0;
     double acc = 0.0;
-    for (int i = i0; i < i1; ++i) acc += rhow(j, i);
+#pragma unroll
+    for (int u = 0; u < ROW_CHUNK; ++u) acc += v[u];
     partial[(int64_t)c * rows + j] = acc;
 }
 
-// K3: row means + sequential lithostatic scan (update.jl:122-131) in one workgroup
-__global__ __launch_bounds__(NT) void k_rowmean_scan(const double* partial, int rows, int nchunk, int cols,
-                                                      double g0, double h, double Ptop, double* ref, double* plcol) {
-    extern __shared__ double sref[];
+// K3: row means + sequential lithostatic scan (update.jl:122-131) in one workgroup of RM_NT lanes: the
+// means in one pass, then one lane adds them in row order (operands read ahead from LDS in blocks of 16,
+// the row count padded so that the blocks need no bounds tests), the results stored by all lanes
+constexpr int RM_NT = 1024;
+__global__ __launch_bounds__(RM_NT) void k_rowmean_scan(const double* partial, int rows, int nchunk, int cols,
+                                                         double g0, double h, double Ptop, double* ref, double* plcol) {
+    extern __shared__ double sref[];  // rows + 16 means, then rows + 16 scan results
+    const int pad = ((rows + 15) & ~15) + 16;
+    double* spl = sref + pad;
     const double s = 1.0 / (double)cols;
-    for (int j = threadIdx.x; j < rows; j += NT) {
+    for (int j = threadIdx.x; j < pad; j += RM_NT) {
         double acc = 0.0;
-        for (int c = 0; c < nchunk; ++c) acc += partial[(int64_t)c * rows + j];
-        const double r = acc * s;
-        sref[j] = r;
-        ref[j] = r;
+        if (j < rows) {
+            for (int c0 = 0; c0 < nchunk; c0 += 16) {
+                double v[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) v[u] = c0 + u < nchunk ? partial[(int64_t)(c0 + u) * rows + j] : 0.0;
+#pragma unroll
+                for (int u = 0; u < 16; ++u) acc += v[u];
+            }
+            const double r = acc * s;
+            ref[j] = r;
+            acc = r;
+        }
+        sref[j] = acc;  // 0 in the padding
     }
     __syncthreads();
+    const int Nz = rows - 1;
     if (threadIdx.x == 0) {
-        const int Nz = rows - 1;
         double pl = sref[0] * g0 * h / 2.0 + Ptop;
-        plcol[0] = pl;
-        for (int j = 1; j < Nz; ++j) {
-            pl = pl + sref[j] * g0 * h;
-            plcol[j] = pl;
+        spl[0] = pl;
+        for (int j0 = 1; j0 < Nz; j0 += 16) {
+            double v[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) v[u] = sref[j0 + u];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                pl = pl + v[u] * g0 * h;
+                spl[j0 + u] = pl;
+            }
         }
     }
+    __syncthreads();
+    for (int j = threadIdx.x; j < Nz; j += RM_NT) plcol[j] = spl[j];
 }
 
 // K4: Drho, Pl, Pt (update.jl:87, 132-134).  Domain (Nz+1) x Nx.
@@ -289,27 +315,11 @@ __global__ __launch_bounds__(NT) void k_cells(CellArgs a) {
     }
 }
 
-__global__ __launch_bounds__(NT) void k_min_final(const double* bmin, int nb, double etacntr, double* emax) {
-    __shared__ double s1[NT], s2[NT];
-    double m1 = INFINITY, m2 = INFINITY;
-    for (int b = threadIdx.x; b < nb; b += NT) {
-        m1 = fmin(m1, bmin[b]);
-        m2 = fmin(m2, bmin[nb + b]);
-    }
-    s1[threadIdx.x] = m1;
-    s2[threadIdx.x] = m2;
-    __syncthreads();
-    for (int st = NT / 2; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st) {
-            s1[threadIdx.x] = fmin(s1[threadIdx.x], s1[threadIdx.x + st]);
-            s2[threadIdx.x] = fmin(s2[threadIdx.x], s2[threadIdx.x + st]);
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        emax[0] = s1[0] * etacntr;  // update.jl:331
-        emax[1] = s2[0] * etacntr;  // update.jl:396
-    }
+// the minima of both viscosity kernels over the workgroup minima, one wave each, then the caps
+__global__ __launch_bounds__(128) void k_min_final(const double* bmin, int nb, double etacntr, double* emax) {
+    const int q = threadIdx.x >> 6;
+    const double m = xc_rows_reduce<true>(bmin, nb, q);
+    if ((threadIdx.x & 63) == 0) emax[q] = m * etacntr;  // update.jl:331, 396
 }
 
 // K6: harmonic cap + 50/50 Picard blend (update.jl:332-337, 397-402).  Cells.
@@ -639,7 +649,8 @@ int xc_update(xc_ctx* ctx, xc_phase* ph, xc_fluid* fl, const xc_grid* g, const x
     LAUNCH(k_interp, ia.d, ia);
     hipLaunchKernelGGL(k_rowsum, dim3((rows + 63) / 64, nchunk), dim3(64), 0, ctx->stream, dm(fl->rhow), rows,
                        (int)Nx, part);
-    hipLaunchKernelGGL(k_rowmean_scan, dim3(1), dim3(NT), rows * sizeof(double), ctx->stream, part, rows, nchunk,
+    hipLaunchKernelGGL(k_rowmean_scan, dim3(1), dim3(RM_NT), 2 * (((rows + 15) & ~15) + 16) * sizeof(double), ctx->stream,
+                       part, rows, nchunk,
                        (int)Nx, par->g0, g->h, par->Ptop, ref, plcol);
     xc_dom dZ = xc_make_dom(Nz + 1, Nx);
     LAUNCH(k_drho_pt, dZ, dm(fl->rhow), dm(fl->Drho), dm(ph->Pl), dm(ph->Pt), dm(fl->P), ref, plcol,
@@ -649,7 +660,7 @@ int xc_update(xc_ctx* ctx, xc_phase* ph, xc_fluid* fl, const xc_grid* g, const x
                 dm(ph->vm), dm(fl->rho), dm(ph->etae), dm(ph->ke), dm(ph->kx), dm(ph->ReL), dm(ph->fReL),
                 dm(ph->Rel), dm(ph->fRel), dm(ph->ks), dm(ph->etat), dm(ph->Rc), etai, etasi, bmin, p, dK};
     LAUNCH(k_cells, dK, ca);
-    hipLaunchKernelGGL(k_min_final, dim3(1), dim3(NT), 0, ctx->stream, bmin, dK.nb, par->etacntr, emax);
+    hipLaunchKernelGGL(k_min_final, dim3(1), dim3(128), 0, ctx->stream, bmin, dK.nb, par->etacntr, emax);
     xc_dom dC = xc_make_dom(Nz, Nx);
     LAUNCH(k_blend, dC, dm(fl->eta), dm(ph->etas), dm(ph->ReD), dm(ph->Red), dm(ph->V), dm(ph->vx), dm(fl->rho),
            etai, etasi, emax, sc->D0, par->d0, (int)Nz, (int)Nx, dC);

@@ -85,6 +85,60 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
     const int per = nb >> 3, rem = nb & 7;
     return xcd * per + (xcd < rem ? xcd : rem) + slot;
 }
+// Wavefront reductions by DPP: row shifts within the rows of 16 lanes (row_shr 1, 2, 4, 8: lane 15 of
+// each row then holds the row's result), then the row broadcasts (row_bcast 15 into rows 1 and 3,
+// row_bcast 31 into rows 2 and 3: lane 63 holds the wave's), read back to every lane.  A lane whose source
+// lies outside its row, or whose row the mask excludes, takes OLD (the operation's identity).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_shift(double v, double old = 0.0) {
+    const int lo = __builtin_amdgcn_update_dpp((int)__double2loint(old), (int)__double2loint(v), CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)__double2hiint(old), (int)__double2hiint(v), CTRL, ROWS, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane63(double v) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 63),
+                            __builtin_amdgcn_readlane(__double2loint(v), 63));
+}
+__device__ __forceinline__ double wave_total(double v) {
+    v += dpp_shift<0x111, 0xf>(v);
+    v += dpp_shift<0x112, 0xf>(v);
+    v += dpp_shift<0x114, 0xf>(v);
+    v += dpp_shift<0x118, 0xf>(v);
+    v += dpp_shift<0x142, 0xa>(v);
+    v += dpp_shift<0x143, 0xc>(v);
+    return lane63(v);
+}
+__device__ __forceinline__ double wave_min(double v) {
+    const double I = INFINITY;
+    v = fmin(v, dpp_shift<0x111, 0xf>(v, I));
+    v = fmin(v, dpp_shift<0x112, 0xf>(v, I));
+    v = fmin(v, dpp_shift<0x114, 0xf>(v, I));
+    v = fmin(v, dpp_shift<0x118, 0xf>(v, I));
+    v = fmin(v, dpp_shift<0x142, 0xa>(v, I));
+    v = fmin(v, dpp_shift<0x143, 0xc>(v, I));
+    return lane63(v);
+}
+// out[q] = the sum (or, MIN, the minimum) of the nb partials part[q * nb + b] for q < nq: one wave per q
+// (blockDim.x = 64 * waves), every lane's loads in flight together, a fixed order of operations
+template <bool MIN>
+__device__ __forceinline__ double xc_rows_reduce(const double* __restrict__ part, int nb, int q) {
+    const int lane = threadIdx.x & 63;
+    constexpr int U = 8;
+    double acc = MIN ? INFINITY : 0.0;
+    for (int b0 = lane; b0 < nb; b0 += 64 * U) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int b = b0 + 64 * u;
+            v[u] = part[(int64_t)q * nb + (b < nb ? b : nb - 1)];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (b0 + 64 * u < nb) acc = MIN ? fmin(acc, v[u]) : acc + v[u];
+    }
+    return MIN ? wave_min(acc) : wave_total(acc);
+}
+
 __device__ __forceinline__ void xc_dom_ij(const xc_dom& d, int* j, int* i) {
     const int t = xcd_tile(blockIdx.x, d.nb);
     *j = (t % d.nbz) * 64 + (threadIdx.x & 63);

@@ -246,25 +246,14 @@ __global__ __launch_bounds__(NT) void k_stat_partial(StatArgs s) {
 #pragma unroll
         for (int q = 0; q < 6; ++q) s.part[(int64_t)q * s.d.nb + blockIdx.x] = red[q][0];
 }
-// sums partials; mode 0: stats[q] = mean; mode 1: stats[6+q] = std (n-1)
-__global__ __launch_bounds__(NT) void k_stat_final(const double* part, int nb, double n, int mode, double* stats) {
-    __shared__ double red[6][NT];
-    for (int q = 0; q < 6; ++q) {
-        double acc = 0.0;
-        for (int b = threadIdx.x; b < nb; b += NT) acc += part[(int64_t)q * nb + b];
-        red[q][threadIdx.x] = acc;
+// sums partials; mode 0: stats[q] = mean; mode 1: stats[6+q] = std (n-1).  One wave per statistic.
+__global__ __launch_bounds__(6 * 64) void k_stat_final(const double* part, int nb, double n, int mode, double* stats) {
+    const int q = threadIdx.x >> 6;
+    const double t = xc_rows_reduce<false>(part, nb, q);
+    if ((threadIdx.x & 63) == 0) {
+        if (mode == 0) stats[q] = t / n;
+        else stats[6 + q] = sqrt(t / (n - 1.0));
     }
-    __syncthreads();
-    for (int st = NT / 2; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st)
-            for (int q = 0; q < 6; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + st];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0)
-        for (int q = 0; q < 6; ++q) {
-            if (mode == 0) stats[q] = red[q][0] / n;
-            else stats[6 + q] = sqrt(red[q][0] / (n - 1.0));
-        }
 }
 
 // rescaled filtered value (noise.jl:265-267); st: [0..5] means, [6..11] stds
@@ -508,9 +497,9 @@ int xc_fft_filter(xc_ctx* ctx, xc_mat psi, xc_mat out, int64_t pad, double h, do
     sa.part = sc + ncell;
     sa.stats = sa.part + 6 * d.nb;
     hipLaunchKernelGGL(k_stat_partial<false>, dim3(d.nb), dim3(NT), 0, ctx->stream, sa);
-    hipLaunchKernelGGL(k_stat_final, dim3(1), dim3(NT), 0, ctx->stream, sa.part, d.nb, (double)ncell, 0, sa.stats);
+    hipLaunchKernelGGL(k_stat_final, dim3(1), dim3(6 * 64), 0, ctx->stream, sa.part, d.nb, (double)ncell, 0, sa.stats);
     hipLaunchKernelGGL(k_stat_partial<true>, dim3(d.nb), dim3(NT), 0, ctx->stream, sa);
-    hipLaunchKernelGGL(k_stat_final, dim3(1), dim3(NT), 0, ctx->stream, sa.part, d.nb, (double)ncell, 1, sa.stats);
+    hipLaunchKernelGGL(k_stat_final, dim3(1), dim3(6 * 64), 0, ctx->stream, sa.part, d.nb, (double)ncell, 1, sa.stats);
     // rescale to the raw mean/std (field 0's statistics), elementwise in place
     hipLaunchKernelGGL(k_resc_inplace, dim3(d.nb), dim3(NT), 0, ctx->stream, out.p, out.ld, sa.stats, Nz, Nx, d);
     return xc_finish(ctx, "compute_fft_filter");
@@ -599,9 +588,9 @@ int xc_noise_step(xc_ctx* ctx, xc_noise* ns, const xc_phase* ph, const xc_grid* 
     sa.part = sb + 6 * ncell;
     sa.stats = sa.part + 6 * d.nb;
     hipLaunchKernelGGL(k_stat_partial<false>, dim3(d.nb), dim3(NT), 0, ctx->stream, sa);
-    hipLaunchKernelGGL(k_stat_final, dim3(1), dim3(NT), 0, ctx->stream, sa.part, d.nb, (double)ncell, 0, sa.stats);
+    hipLaunchKernelGGL(k_stat_final, dim3(1), dim3(6 * 64), 0, ctx->stream, sa.part, d.nb, (double)ncell, 0, sa.stats);
     hipLaunchKernelGGL(k_stat_partial<true>, dim3(d.nb), dim3(NT), 0, ctx->stream, sa);
-    hipLaunchKernelGGL(k_stat_final, dim3(1), dim3(NT), 0, ctx->stream, sa.part, d.nb, (double)ncell, 1, sa.stats);
+    hipLaunchKernelGGL(k_stat_final, dim3(1), dim3(6 * 64), 0, ctx->stream, sa.part, d.nb, (double)ncell, 1, sa.stats);
     // 4. fluxes and magnitudes (noise.jl:190-236)
     FluxArgs fa{};
     for (int f = 0; f < 3; ++f) { fa.flt[f] = flt[f].p; fa.ldf[f] = flt[f].ld; }

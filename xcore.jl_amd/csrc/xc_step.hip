@@ -64,20 +64,12 @@ __global__ __launch_bounds__(NT) void k_resnorm_part(xc_mat W, xc_mat U, xc_mat 
         for (int q = 0; q < 9; ++q) part[(int64_t)q * gridDim.x + blockIdx.x] = red[q][0];
 }
 
-__global__ __launch_bounds__(NT) void k_sum_rows(const double* part, int nb, int nq, double* out) {
-    __shared__ double red[NT];
-    for (int q = 0; q < nq; ++q) {
-        double acc = 0.0;
-        for (int b = threadIdx.x; b < nb; b += NT) acc += part[(int64_t)q * nb + b];
-        red[threadIdx.x] = acc;
-        __syncthreads();
-        for (int st = NT / 2; st > 0; st >>= 1) {
-            if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[q] = red[0];
-        __syncthreads();
-    }
+// out[q] = sum of the nb partials of row q, one wave per row (blockDim.x = 64 nq)
+__global__ __launch_bounds__(1024) void k_sum_rows(const double* part, int nb, int nq, double* out) {
+    const int q = threadIdx.x >> 6;
+    if (q >= nq) return;
+    const double t = xc_rows_reduce<false>(part, nb, q);
+    if ((threadIdx.x & 63) == 0) out[q] = t;
 }
 
 // history.jl:122-151 sums: [0] sum rho [1] sum M [2] sum X [3] sum Gx
@@ -174,7 +166,7 @@ int xc_compute_resnorm(xc_ctx* ctx, const xc_phase* ph, const xc_fluid* fl, doub
     if (!s || !s->snap) return xc_fail(ctx, XC_EINVAL_ARG, "compute_resnorm: no snapshot taken");
     hipLaunchKernelGGL(k_resnorm_part, dim3(RED_BLOCKS), dim3(NT), 0, ctx->stream, fl->W, fl->U, fl->P, ph->X,
                        fl->MFS, fl->rho, s->snap, Nz, Nx, s->part);
-    hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(NT), 0, ctx->stream, s->part, RED_BLOCKS, 9, s->out);
+    hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(9 * 64), 0, ctx->stream, s->part, RED_BLOCKS, 9, s->out);
     XC_TRY_HIP(ctx, hipMemcpyAsync(s->host, s->out, 9 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const double* a = s->host;
@@ -196,7 +188,7 @@ int xc_history_sums(xc_ctx* ctx, const xc_phase* ph, const xc_fluid* fl, double 
     if ((st = step_ctx(ctx, 1, &s))) return st;
     hipLaunchKernelGGL(k_hist_part, dim3(RED_BLOCKS), dim3(NT), 0, ctx->stream, fl->rho, ph->M, ph->X, ph->Gx,
                        ph->qz_advn_X, ph->qz_advn_M, ph->qz_dffn_X, ph->qz_dffn_M, Nz, Nx, s->part);
-    hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(NT), 0, ctx->stream, s->part, RED_BLOCKS, 12, s->out);
+    hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(12 * 64), 0, ctx->stream, s->part, RED_BLOCKS, 12, s->out);
     XC_TRY_HIP(ctx, hipMemcpyAsync(s->host, s->out, 12 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     for (int q = 0; q < 12; ++q) out[q] = s->host[q];

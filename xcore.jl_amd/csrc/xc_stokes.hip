@@ -1061,13 +1061,10 @@ __global__ __launch_bounds__(NT) void k_multidot(const double* __restrict__ V, i
 }
 // out[q] = sum of the nb partials of q (fixed order); one wavefront per q
 __global__ __launch_bounds__(NT) void k_dot_final(const double* __restrict__ part, int nb, int K1, double* out) {
-    const int lane = threadIdx.x & 63;
     const int q = blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
     if (q >= K1) return;
-    double acc = 0.0;
-    for (int b = lane; b < nb; b += 64) acc += part[(int64_t)q * nb + b];
-    acc = wave_sum(acc);
-    if (lane == 0) out[q] = acc;
+    const double t = xc_rows_reduce<false>(part, nb, q);
+    if ((threadIdx.x & 63) == 0) out[q] = t;
 }
 // largest viscosity and smallest density of a level: per-block partials, then one wave combines them
 __global__ __launch_bounds__(NT) void k_eta_rho_part(LevD L, double* __restrict__ part) {
@@ -1256,25 +1253,6 @@ __global__ void k_kry_init(Kry K, int m, double beta) {
 // p_{j+1} = (w - V_j b - h_jj v_j) / nu_{j+1}.  FGMRES needs only LL z_j in span(V): z_j = M^-1 p_j is a
 // direction like any other.
 //
-// Full-wavefront sum by DPP (row shifts within rows of 16 lanes, then the row broadcasts), returned to
-// every lane; lanes whose source is outside the row read 0.
-template <int CTRL, int ROWS>
-__device__ __forceinline__ double dpp_shift(double v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)__double2loint(v), CTRL, ROWS, 0xf, true);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)__double2hiint(v), CTRL, ROWS, 0xf, true);
-    return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ double wave_total(double v) {
-    v += dpp_shift<0x111, 0xf>(v);  // row_shr:1
-    v += dpp_shift<0x112, 0xf>(v);  // row_shr:2
-    v += dpp_shift<0x114, 0xf>(v);  // row_shr:4
-    v += dpp_shift<0x118, 0xf>(v);  // row_shr:8: lane 15 of each row holds the row's sum
-    v += dpp_shift<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
-    v += dpp_shift<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3: lane 63 holds the total
-    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 63),
-                            __builtin_amdgcn_readlane(__double2loint(v), 63));
-}
-
 // The reduction of one DCGS2 step: per block, partials of V_q.p (row q) and V_q.w (row K + q) for q < K,
 // then p.p, p.w, w.w (rows 2K .. 2K+2); k_arnoldi2 adds the nb = gridDim.x partials of each row in a
 // fixed order.  Each thread keeps its EPT elements of p and w in registers and streams the basis vectors
