@@ -207,8 +207,9 @@ __device__ double etamix_cell(double f1, double f2, const Par& p) {
     double Xf21 = a2 * f1 + (1.0 - a2) * Sf21;
     double Xf22 = a2 * f2 + (1.0 - a2) * Sf22;
     const double ex = p.etax0, em = p.etam0;
-    double thtv1 = pow(ex / ex, Xf11) * pow(em / ex, Xf12);
-    double thtv2 = pow(ex / em, Xf21) * pow(em / em, Xf22);
+    // pow(ex / ex, Xf11) and pow(em / em, Xf22) of the reference are pow(1, y) = 1 exactly (IEEE 754 pow)
+    double thtv1 = 1.0 * pow(em / ex, Xf12);
+    double thtv2 = pow(ex / em, Xf21) * 1.0;
     return f1 * (ex * thtv1) + f2 * (em * thtv2);
 }
 

@@ -1038,11 +1038,6 @@ __global__ __launch_bounds__(FFT_NT) void k_pois_xinv(PoisInvArgs a) {
 // ---- vector kernels for FGMRES ------------------------------------------
 // partial dot products of w against K basis vectors V[0..K) (stride n) and, as entry K, w.w
 constexpr int DOT_BLOCKS = 1024;
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    return v;
-}
 // partial dot products of w against K basis vectors V[0..K) (stride n) and, as entry K, w.w
 __global__ __launch_bounds__(NT) void k_multidot(const double* __restrict__ V, int64_t n, int K,
                                                  const double* __restrict__ w, double* __restrict__ part) {
@@ -1052,7 +1047,7 @@ __global__ __launch_bounds__(NT) void k_multidot(const double* __restrict__ V, i
         const double* v = q < K ? V + (int64_t)q * n : w;
         double acc = 0.0;
         for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) acc += v[i] * w[i];
-        acc = wave_sum(acc);
+        acc = wave_total(acc);
         if (lane == 0) red[wave] = acc;
         __syncthreads();
         if (threadIdx.x == 0) part[(int64_t)q * gridDim.x + blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
@@ -1489,12 +1484,22 @@ __global__ __launch_bounds__(NT) void k_dcgs_update(const double* __restrict__ V
     pnext[i] = (w[i] - sb - cf[1] * v) * cf[2];
 }
 // y = H(0:j, 0:j)^-1 g(0:j) (upper triangular after the rotations)
-__global__ void k_trisolve(Kry K, int m, int j) {
+// y = R^-1 g for the j x j upper triangle R of H: the wave stages R and g in LDS with every load in
+// flight, then lane 0 back-substitutes (j <= ARN_MMAX)
+__global__ __launch_bounds__(64) void k_trisolve(Kry K, int m, int j) {
+    __shared__ double R[ARN_MMAX * ARN_MMAX], gy[ARN_MMAX];
+    for (int t = threadIdx.x; t < j * j; t += 64) {
+        const int i = t % j, k = t / j;
+        R[t] = i <= k ? K.H[i + (int64_t)k * (m + 1)] : 0.0;
+    }
+    for (int i = threadIdx.x; i < j; i += 64) gy[i] = K.g[i];
+    __syncthreads();
     if (threadIdx.x != 0) return;
     for (int i = j - 1; i >= 0; --i) {
-        double sum = K.g[i];
-        for (int k = i + 1; k < j; ++k) sum -= K.H[i + (int64_t)k * (m + 1)] * K.y[k];
-        K.y[i] = sum / K.H[i + (int64_t)i * (m + 1)];
+        double sum = gy[i];
+        for (int k = i + 1; k < j; ++k) sum -= R[i + k * j] * gy[k];
+        gy[i] = sum / R[i + i * j];
+        K.y[i] = gy[i];
     }
 }
 __global__ __launch_bounds__(NT) void k_zero(double* a, int64_t n) {
@@ -2222,8 +2227,8 @@ void precond(xc_ctx* ctx, StokesCtx* s, const Flags& F, const double* v, double*
 int multidot(xc_ctx* ctx, StokesCtx* s, const double* V, int K, const double* w, int64_t n) {
     const int nbk = DOT_BLOCKS;
     hipLaunchKernelGGL(k_multidot, dim3(nbk), dim3(NT), 0, ctx->stream, V, n, K, w, s->part);
-    hipLaunchKernelGGL(k_dot_final, dim3((K + 4) / 4), dim3(NT), 0, ctx->stream, s->part, nbk, K + 1, s->dots);
-    XC_TRY_HIP(ctx, hipMemcpyAsync(s->host, s->dots, (K + 1) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    // the sums are written straight into the mapped pinned host buffer (no copy launch)
+    hipLaunchKernelGGL(k_dot_final, dim3((K + 4) / 4), dim3(NT), 0, ctx->stream, s->part, nbk, K + 1, s->host_dev);
     XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return XC_OK;
 }
