@@ -117,7 +117,8 @@ def step_bench(args, learned=False):
                      xc.step_opts(par, tol=1e-8, maxit=2000, fixed_sweeps=1))
     par, g, sc, ph, fl, ns, hst, state = c3_setup(args.n)
     opts = xc.step_opts(par, tol=1e-8, maxit=2000, mg_pre=args.mg_pre, mg_post=args.mg_post, mg_coarse=args.mg_coarse,
-                        mg_sub=args.mg_sub, mg_levels=args.mg_levels, learned=learned, lc_level=lc_level)
+                        mg_sub=args.mg_sub, mg_levels=args.mg_levels, mg_omega=args.mg_omega, learned=learned,
+                        lc_level=lc_level)
     for _ in range(args.warmup):
         xc.run_step_(ph, fl, ns, g, par, sc, state, opts)
     torch.cuda.synchronize()
@@ -356,6 +357,7 @@ def main():
     ap.add_argument("--mg-post", type=int, default=3)
     ap.add_argument("--mg-coarse", type=int, default=0, help="coarsest-level sweeps (0: the library's choice)")
     ap.add_argument("--mg-sub", type=int, default=1)
+    ap.add_argument("--mg-omega", type=float, default=0.8, help="damped-Jacobi weight of the multigrid smoother")
     ap.add_argument("--mg-levels", type=int, default=0, help="most multigrid levels (0: down to 4 x 4)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-stencil", action="store_true")

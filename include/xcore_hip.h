@@ -214,7 +214,7 @@ typedef struct {  /* fluidmech! keywords (fluidmech.jl:71-80) and the iterative-
     int mg_pre, mg_post, mg_coarse;  /* multigrid smoothing sweeps on the finest level (pre, post) and on
                                         the coarsest; 0 selects the defaults 3, 3 and, on the coarsest,
                                         20 at the 4 x 4 bottom or 2 where the cycle stopped early */
-    double mg_omega;  /* damped-Jacobi weight of the multigrid smoother */
+    double mg_omega;  /* damped-Jacobi weight of the multigrid smoother (0: 0.8) */
     int learned;      /* 1: add the learned coarse correction (needs xc_unet_set_params) for a first
                          FGMRES cycle of 6 iterations; if that has not converged the plain
                          preconditioner continues from the iterate reached (fallback_used = 1) */

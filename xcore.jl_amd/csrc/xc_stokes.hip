@@ -2033,7 +2033,7 @@ MGOpts mg_opts(const xc_fm_opts* o) {
     m.pre = o->mg_pre > 0 ? o->mg_pre : 3;
     m.post = o->mg_post > 0 ? o->mg_post : 3;
     m.coarse = o->mg_coarse > 0 ? o->mg_coarse : 0;  // 0: coarse_sweeps decides per solve
-    m.omega = o->mg_omega > 0 ? o->mg_omega : 0.7;
+    m.omega = o->mg_omega > 0 ? o->mg_omega : 0.8;
     m.sub = o->mg_sub > 0 ? o->mg_sub : (o->mg_sub == 0 ? 1 : 0);
     m.f32 = o->mg_fp64 == 0;
     return m;
