@@ -363,17 +363,24 @@ __global__ __launch_bounds__(NT) void k_smooth_cells(LV L, Flags F, const TX* __
 // computing the W row (r <= Nz), the U row (c <= Nx) and the P row of that position with the operations
 // of the reference's rows (fluidmech.jl:145-488) in the order vel_row and the pressure rows use, from
 // scalar column bases plus the lane's row offset.
+// sq: when given, wave w (of 4 gridDim.x) writes the sum of squares of the values its lanes stored to sq[w]
 template <bool RESID>
 __global__ __launch_bounds__(NT) void k_ll_cells(LevD L, Flags F, const double* __restrict__ x,
-                                                 const double* __restrict__ b, double* __restrict__ y) {
+                                                 const double* __restrict__ b, double* __restrict__ y,
+                                                 double* __restrict__ sq = nullptr) {
     const int Nz = L.Nz, Nx = L.Nx;
     const int segs = (Nz + 2 + 63) >> 6;
     const int task = __builtin_amdgcn_readfirstlane(xcd_tile(blockIdx.x, gridDim.x) * (NT / 64) + (threadIdx.x >> 6));
     const int c = task / segs;
-    if (c > Nx + 1) return;
     const int lane = threadIdx.x & 63;
-    const int r = ((task - c * segs) << 6) + lane;
-    if (r > Nz + 1) return;
+    if (c > Nx + 1) {
+        if (sq && lane == 0) sq[task] = 0.0;
+        return;
+    }
+    const int r0 = ((task - c * segs) << 6) + lane;
+    const bool live = r0 <= Nz + 1;
+    const int r = live ? r0 : Nz + 1;  // lanes past the last row repeat it and store nothing
+    double ss = 0.0;
     const int ru = r;                            // U and P row
     const int rw = r <= Nz ? r : Nz;             // W row
     const int um = ru > 0 ? ru - 1 : 0, up = ru <= Nz ? ru + 1 : Nz + 1;
@@ -443,7 +450,11 @@ __global__ __launch_bounds__(NT) void k_ll_cells(LevD L, Flags F, const double* 
                 (0.5 * C2 - T1) * i2 * Ur - (0.5 * C2 - T2) * i2 * Us;
         }
         if (!bc) v += -ih * Pr + ih * Ps;
-        y[kw] = RESID ? ldr(b + (int64_t)cw * ldW, rw) - v : v;
+        const double o = RESID ? ldr(b + (int64_t)cw * ldW, rw) - v : v;
+        if (live) {
+            y[kw] = o;
+            ss = fma(o, o, ss);
+        }
     }
     // U row (r, c) (u_row), then GG on the interior rows
     if (c <= Nx) {
@@ -463,7 +474,11 @@ __global__ __launch_bounds__(NT) void k_ll_cells(LevD L, Flags F, const double* 
                 (0.5 * C2 - T1) * i2 * Wr - (0.5 * C2 - T2) * i2 * We;
         }
         if (!bc) v += -ih * Pr + ih * Pe;
-        y[ku] = RESID ? ldr(b + offU + (int64_t)cu * ldU, ru) - v : v;
+        const double o = RESID ? ldr(b + offU + (int64_t)cu * ldU, ru) - v : v;
+        if (live) {
+            y[ku] = o;
+            ss = fma(o, o, ss);
+        }
     }
     // P row (r, c): DM, the BC rows and the pins (fluidmech.jl:357-488)
     {
@@ -478,7 +493,15 @@ __global__ __launch_bounds__(NT) void k_ll_cells(LevD L, Flags F, const double* 
             v = -rul * ih * Ul + ru0 * ih * Ur - rwm * ih * Wn + rw0 * ih * Wr;
             if (pin) v += Pr;
         }
-        y[kp] = RESID ? ldr(b + offP + (int64_t)c * ldP, ru) - v : v;
+        const double o = RESID ? ldr(b + offP + (int64_t)c * ldP, ru) - v : v;
+        if (live) {
+            y[kp] = o;
+            ss = fma(o, o, ss);
+        }
+    }
+    if (sq) {
+        const double t = wave_total(ss);
+        if (lane == 0) sq[task] = t;
     }
 }
 inline unsigned ll_cells_blocks(int Nz, int Nx) {
@@ -1833,7 +1856,12 @@ int setup(xc_ctx* ctx, int Nz, int Nx, int m, StokesCtx** out, int maxlev = -1) 
     const int K = Nx / 2 + 1;
     const size_t lcc = (size_t)(Nz / 2) * (Nx / 2);  // learned correction at level >= 1
     tot += 3 * lcc + 16 + (lcc + NT - 1) / NT + 8;
-    const size_t part_cols = md_blocks(n) > DOT_BLOCKS ? (size_t)md_blocks(n) : (size_t)DOT_BLOCKS;
+    size_t part_cols = md_blocks(n) > DOT_BLOCKS ? (size_t)md_blocks(n) : (size_t)DOT_BLOCKS;
+    {  // also room for k_ll_cells' per-wave partials
+        const size_t waves = (size_t)ll_cells_blocks(Nz, Nx) * (NT / 64);
+        const size_t rows = 2 * (size_t)m + 4;
+        if (rows * part_cols < waves) part_cols = (waves + rows - 1) / rows;
+    }
     tot += 4 * (size_t)n + (size_t)(m + 1) * n + (size_t)m * n + (size_t)(2 * m + 4) * part_cols + 128 /* dots */ +
            (size_t)n /* tmp */ + 8 * (size_t)(Nz + 2) * (Nx + 3) /* advection scratch */ +
            2 * (size_t)Nz * K + 2 * (size_t)Nx + 4 * (size_t)Nz + K + 2 * (size_t)Nz + 16 /* Poisson */ +
@@ -2396,9 +2424,13 @@ int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int 
     double* cf = s->d3 + 3 * (m + 2);     // update coefficients and nu
     volatile double* hout = s->host + 8;  // 2 per step: provisional |g[j+1]|, nu_{j+1}
     while (true) {
-        hipLaunchKernelGGL(k_ll_cells<true>, dim3(ll_cells_blocks(dd.Nz, dd.Nx)), dim3(NT), 0, ctx->stream, dd, F, x, b,
-                           s->rs);
-        if ((st = multidot(ctx, s, s->rs, 0, s->rs, n))) return st;
+        {  // r = b - LL x and ||r||^2 (per-wave partials, their sum written to the mapped host buffer)
+            const unsigned nbl = ll_cells_blocks(dd.Nz, dd.Nx);
+            hipLaunchKernelGGL(k_ll_cells<true>, dim3(nbl), dim3(NT), 0, ctx->stream, dd, F, x, b, s->rs, s->part);
+            hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(NT), 0, ctx->stream, s->part, (int)(nbl * (NT / 64)), 1,
+                               s->host_dev);
+            XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        }
         const double beta = sqrt(s->host[0]);
         rel = bnorm > 0 ? beta / bnorm : beta;
         if (rel0 < 0) rel0 = rel;
