@@ -558,11 +558,7 @@ __global__ __launch_bounds__(NT) void k_prolong_cells(int Nz, int Nx, Flags F, c
 
 #define SMOOTH_D(M) (k_smooth_cells<M, LevD, double, double, double>)
 #define SMOOTH_F(M, TB, TO) (k_smooth_cells<M, LevF, float, TB, TO>)
-// single <-> double precision copies
-__global__ __launch_bounds__(NT) void k_d2f(const double* __restrict__ a, float* __restrict__ b, int64_t n) {
-    const int64_t i = gidx();
-    if (i < n) b[i] = (float)a[i];
-}
+// single -> double precision copy
 __global__ __launch_bounds__(NT) void k_f2d(const float* __restrict__ a, double* __restrict__ b, int64_t n) {
     const int64_t i = gidx();
     if (i < n) b[i] = (double)a[i];
@@ -1084,34 +1080,25 @@ __global__ __launch_bounds__(NT) void k_dot_final(const double* __restrict__ par
     const double t = xc_rows_reduce<false>(part, nb, q);
     if ((threadIdx.x & 63) == 0) out[q] = t;
 }
-// largest viscosity and smallest density of a level: per-block partials, then one wave combines them
-__global__ __launch_bounds__(NT) void k_eta_rho_part(LevD L, double* __restrict__ part) {
-    const int64_t n = (int64_t)L.Nz * L.Nx;
-    double emax = 0.0, rmin = 1e308;
-    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
-        emax = fmax(emax, L.eta[i]);
-        rmin = fmin(rmin, L.rho[i]);
-    }
-    __shared__ double se[NT / 64], sr[NT / 64];
-    for (int o = 32; o > 0; o >>= 1) {
-        emax = fmax(emax, __shfl_xor(emax, o));
-        rmin = fmin(rmin, __shfl_xor(rmin, o));
-    }
-    if ((threadIdx.x & 63) == 0) { se[threadIdx.x >> 6] = emax; sr[threadIdx.x >> 6] = rmin; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < NT / 64; ++w) { emax = fmax(emax, se[w]); rmin = fmin(rmin, sr[w]); }
-        part[2 * blockIdx.x] = emax;
-        part[2 * blockIdx.x + 1] = rmin;
-    }
-}
+// largest viscosity and smallest density of the fine level from k_pack5's per-block partials (one wave)
 __global__ __launch_bounds__(64) void k_eta_rho_final(const double* __restrict__ part, int nb, double* out) {
     double emax = 0.0, rmin = 1e308;
-    for (int b = threadIdx.x; b < nb; b += 64) { emax = fmax(emax, part[2 * b]); rmin = fmin(rmin, part[2 * b + 1]); }
-    for (int o = 32; o > 0; o >>= 1) {
-        emax = fmax(emax, __shfl_xor(emax, o));
-        rmin = fmin(rmin, __shfl_xor(rmin, o));
+    for (int b0 = threadIdx.x; b0 < nb; b0 += 64 * 8) {
+        double e[8], r[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int b = b0 + 64 * u < nb ? b0 + 64 * u : nb - 1;
+            e[u] = part[2 * b];
+            r[u] = part[2 * b + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            emax = fmax(emax, e[u]);
+            rmin = fmin(rmin, r[u]);
+        }
     }
+    emax = -wave_min(-emax);
+    rmin = wave_min(rmin);
     if (threadIdx.x == 0) { out[0] = emax; out[1] = rmin; }
 }
 __global__ __launch_bounds__(NT) void k_scale(const double* __restrict__ a, double s, double* __restrict__ b, int64_t n) {
@@ -1555,10 +1542,41 @@ __global__ __launch_bounds__(NT) void k_scatter(const double* x, xc_mat W, xc_ma
     else if (k < nW + nU + nP) { const int64_t q = k - nW - nU; P.p[(q % (Nz + 2)) + (q / (Nz + 2)) * P.ld] = x[k]; }
 }
 // dense copy of a strided matrix
-__global__ __launch_bounds__(NT) void k_pack(xc_mat a, double* out) {
+
+// the fine level's five coefficient fields packed dense (eta, rho, etaco, rhow, rhou), their single-
+// precision copies at the same offsets from the first (fo = f_coef - dst[0]), and per-block max(eta),
+// min(rho) partials (part[2b], part[2b+1]) for the V-cycle depth
+struct Pack5 {
+    xc_mat a[5];
+    double* dst[5];
+    float* f;        // single-precision copy of dst[0]'s region
+    const double* base;
+    double* part;
+};
+__global__ __launch_bounds__(NT) void k_pack5(Pack5 p) {
     const int64_t k = gidx();
-    if (k >= a.rows * a.cols) return;
-    out[k] = a.p[(k % a.rows) + (k / a.rows) * a.ld];
+    double emax = 0.0, rmin = 1e308;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        const xc_mat& a = p.a[q];
+        if (k < a.rows * a.cols) {
+            const double v = a.p[(k % a.rows) + (k / a.rows) * a.ld];
+            p.dst[q][k] = v;
+            p.f[(p.dst[q] - p.base) + k] = (float)v;
+            if (q == 0) emax = v;
+            if (q == 1) rmin = v;
+        }
+    }
+    emax = -wave_min(-emax);
+    rmin = wave_min(rmin);
+    __shared__ double se[NT / 64], sr[NT / 64];
+    if ((threadIdx.x & 63) == 0) { se[threadIdx.x >> 6] = emax; sr[threadIdx.x >> 6] = rmin; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < NT / 64; ++w) { emax = fmax(emax, se[w]); rmin = fmin(rmin, sr[w]); }
+        p.part[2 * blockIdx.x] = emax;
+        p.part[2 * blockIdx.x + 1] = rmin;
+    }
 }
 
 // ---- right-hand side (fluidmech.jl:111-127, 188-190, 236-246, 310-323, 450-476)
@@ -1713,6 +1731,7 @@ struct StokesCtx {
     // offsets from eta) and vectors x, b, t (ping-pong), r
     float *f_coef = nullptr, *fx = nullptr, *fb = nullptr, *ft = nullptr, *fr = nullptr;
     int64_t f_ncoef = 0;
+    unsigned pack_blocks = 0;  // blocks of the last k_pack5 launch (its max(eta) / min(rho) partials)
     LevF levf() const {
         const Level& L = lev[0];
         LevF f;
@@ -1986,12 +2005,12 @@ void build_levels(xc_ctx* ctx, StokesCtx* s, const xc_fluid* fl, double h, bool 
     s->inertia = 0.0;
     Level& L0 = s->lev[0];
     L0.h = h;
-    hipLaunchKernelGGL(k_pack, dim3(nblk(fl->eta.rows * fl->eta.cols)), dim3(NT), 0, ctx->stream, fl->eta, L0.eta);
-    hipLaunchKernelGGL(k_pack, dim3(nblk(fl->rho.rows * fl->rho.cols)), dim3(NT), 0, ctx->stream, fl->rho, L0.rho);
-    hipLaunchKernelGGL(k_pack, dim3(nblk(fl->etaco.rows * fl->etaco.cols)), dim3(NT), 0, ctx->stream, fl->etaco, L0.etaco);
-    hipLaunchKernelGGL(k_pack, dim3(nblk(fl->rhow.rows * fl->rhow.cols)), dim3(NT), 0, ctx->stream, fl->rhow, L0.rhow);
-    hipLaunchKernelGGL(k_pack, dim3(nblk(fl->rhou.rows * fl->rhou.cols)), dim3(NT), 0, ctx->stream, fl->rhou, L0.rhou);
-    hipLaunchKernelGGL(k_d2f, dim3(nblk(s->f_ncoef)), dim3(NT), 0, ctx->stream, L0.eta, s->f_coef, s->f_ncoef);
+    Pack5 pk{{fl->eta, fl->rho, fl->etaco, fl->rhow, fl->rhou}, {L0.eta, L0.rho, L0.etaco, L0.rhow, L0.rhou},
+             s->f_coef, L0.eta, s->part};
+    int64_t most = 0;
+    for (int q = 0; q < 5; ++q) most = pk.a[q].rows * pk.a[q].cols > most ? pk.a[q].rows * pk.a[q].cols : most;
+    s->pack_blocks = nblk(most);
+    hipLaunchKernelGGL(k_pack5, dim3(s->pack_blocks), dim3(NT), 0, ctx->stream, pk);
     if (all) build_coarse(ctx, s, (int)s->lev.size());
 }
 
@@ -2024,10 +2043,9 @@ constexpr double MG_INERTIA_STOP = 6.0;
 // returns whether it was needed
 bool depth_launch(xc_ctx* ctx, StokesCtx* s, const Flags& F, const xc_fm_opts* o) {
     if (o->mg_levels > 0 || F.bnchm || !(F.dt < 1e30) || !(F.a1 > 0)) return false;
-    const LevD d = s->lev[0].d();
-    const int nb = 256;
-    hipLaunchKernelGGL(k_eta_rho_part, dim3(nb), dim3(NT), 0, ctx->stream, d, s->part);
-    hipLaunchKernelGGL(k_eta_rho_final, dim3(1), dim3(64), 0, ctx->stream, s->part, nb, s->host_dev + 200);
+    // the per-block partials were written by the fine level's packing kernel
+    hipLaunchKernelGGL(k_eta_rho_final, dim3(1), dim3(64), 0, ctx->stream, s->part, (int)s->pack_blocks,
+                       s->host_dev + 200);
     return true;
 }
 // sets s->leff; launched: depth_launch enqueued the reduction; synced: the stream has been synchronized since
