@@ -1366,6 +1366,18 @@ __global__ __launch_bounds__(ARN_NT) void k_arnoldi2(Kry K, int m, int j, const 
     __shared__ double sd[2 * ARN_MMAX + 3], sc[ARN_MMAX + 2], scs[ARN_MMAX], ssn[ARN_MMAX], sg[2], snu;
     const int lane = threadIdx.x;
     double* Hc = K.H + (int64_t)(j > 0 ? j - 1 : 0) * (m + 1);  // column j - 1
+    // the rotations, column j - 1, g and nu staged first: their loads overlap the partial sums'
+    for (int i = lane; i < j; i += ARN_NT) {
+        scs[i] = K.cs[i];
+        ssn[i] = K.sn[i];
+    }
+    if (j > 0)
+        for (int i = lane; i < j; i += ARN_NT) sc[i] = Hc[i];
+    if (lane == 0) {
+        sg[0] = j > 0 ? K.g[j - 1] : 0.0;
+        sg[1] = K.g[j];
+        snu = cf[3];
+    }
     {
         const int R = 2 * j + 3, l16 = lane & 15;
         constexpr int U = 16;
@@ -1392,17 +1404,6 @@ __global__ __launch_bounds__(ARN_NT) void k_arnoldi2(Kry K, int m, int j, const 
                 d[r] = acc;
             }
         }
-    }
-    for (int i = lane; i < j; i += ARN_NT) {
-        scs[i] = K.cs[i];
-        ssn[i] = K.sn[i];
-    }
-    if (j > 0)
-        for (int i = lane; i < j; i += ARN_NT) sc[i] = Hc[i];
-    if (lane == 0) {
-        sg[0] = j > 0 ? K.g[j - 1] : 0.0;
-        sg[1] = K.g[j];
-        snu = cf[3];
     }
     __syncthreads();
     if (lane != 0) return;
@@ -1493,9 +1494,8 @@ __global__ __launch_bounds__(NT) void k_dcgs_update(const double* __restrict__ V
     vj[i] = v;
     pnext[i] = (w[i] - sb - cf[1] * v) * cf[2];
 }
-// y = H(0:j, 0:j)^-1 g(0:j) (upper triangular after the rotations)
-// y = R^-1 g for the j x j upper triangle R of H: the wave stages R and g in LDS with every load in
-// flight, then lane 0 back-substitutes (j <= ARN_MMAX)
+// y = R^-1 g for the j x j upper triangle R of H (upper triangular after the rotations): the wave
+// stages R and g in LDS with every load in flight, then lane 0 back-substitutes (j <= ARN_MMAX)
 __global__ __launch_bounds__(64) void k_trisolve(Kry K, int m, int j) {
     __shared__ double R[ARN_MMAX * ARN_MMAX], gy[ARN_MMAX];
     for (int t = threadIdx.x; t < j * j; t += 64) {
