@@ -136,6 +136,41 @@ def test_fine_smoother_precision(xc, orc, open_cnv):
     assert abs(its[False] - its[True]) <= max(2, its[True] // 10), its
 
 
+@pytest.mark.parametrize("shape", [(48, 64), (256, 32)])
+def test_schur_poisson_solve(xc, orc, shape):
+    # the inertial part of the pressure block of z = M^-1 v: with inertia minus without is
+    # (a1 h^2 / dt) T^-1 rP on the interior cells, T = -d_zz + a1 / (a1 + gamma) (-d_xx) in grid units
+    # (zero flux at the top and bottom rows, periodic in x) on the zero-mean part of rP; checked against a
+    # sparse direct solve of T bordered by the constant mode.  (48, 64): mixed-radix transforms; (256, 32):
+    # the power-of-two z path
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    Nz, Nx = shape
+    par, g, sc, ph, fl, ns = plausible_state(orc, xc, Nz, Nx, seed=5)
+    orc.update(ph, fl, g, par, sc)
+    dfl = to_device_state(xc, fl, "fluid")
+    n = xc.stokes_vector_size(g)
+    v = torch.randn(n, generator=torch.Generator().manual_seed(3), dtype=torch.float64).cuda()
+    kw = dict(a1=1.5, a2=2.0, a3=-0.5, b1=1, b2=0, b3=0)
+    z_in = xc.stokes_precond(dfl, g, par, v, dt=sc.dt0, **kw)
+    z_vis = xc.stokes_precond(dfl, g, par, v, dt=1e32, **kw)
+    nv = (Nz + 1) * (Nx + 2) + (Nz + 2) * (Nx + 1)
+    blk = lambda t: t[nv:].cpu().numpy().reshape(Nx + 2, Nz + 2).T[1:-1, 1:-1]
+    d = (blk(z_in) - blk(z_vis)).ravel(order="F")
+    rP = blk(v).ravel(order="F")
+    ratio = 1.5 / (1.5 + par.gamma)
+    Dz = sp.diags([-np.ones(Nz - 1), 2 * np.ones(Nz), -np.ones(Nz - 1)], [-1, 0, 1]).tolil()
+    Dz[0, 0] = Dz[Nz - 1, Nz - 1] = 1
+    Dx = sp.diags([-np.ones(Nx - 1), 2 * np.ones(Nx), -np.ones(Nx - 1)], [-1, 0, 1]).tolil()
+    Dx[0, Nx - 1] = Dx[Nx - 1, 0] = -1
+    T = sp.kron(sp.eye(Nx), Dz) + ratio * sp.kron(Dx, sp.eye(Nz))
+    one = sp.csr_matrix(np.ones((Nz * Nx, 1)))
+    y = spl.spsolve(sp.bmat([[T, one], [one.T, None]]).tocsc(), np.concatenate([rP, [0.0]]))[:-1]
+    want = 1.5 * g.h ** 2 / sc.dt0 * y
+    print(f"schur poisson {shape}: rel err {_rel(d, want):.3e}", flush=True)
+    assert _rel(d, want) <= 1e-12
+
+
 def test_noconv_reports(xc, orc):
     m, par, g, fl, kw = _mms(orc, xc, 32)
     dfl = to_device_state(xc, fl, "fluid")

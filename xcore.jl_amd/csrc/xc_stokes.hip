@@ -912,64 +912,93 @@ __global__ __launch_bounds__(FFT_NT) void k_pois_xfwd(PoisXArgs a) {
 
 // z solve of every wavenumber column, in place.  The zero-flux rows make the
 // cell-centred z operator the periodic one on the column mirrored about its end
-// faces (length 2 Nz), so a complex FFT of the mirrored column diagonalises it:
-// divide by lam_x[k] + lam_z[kz] (the kx = kz = 0 mode, the constant, is
-// dropped: zero-mean solution), transform back, keep the first Nz entries.
+// faces, so the cosine transform (DCT-II) of the column diagonalises it with
+// eigenvalues lam_z[kz] = 2 - 2 cos(pi kz / Nz): transform, divide by
+// lam_x[k] + lam_z[kz] (the kx = kz = 0 mode, the constant, is dropped:
+// zero-mean solution), transform back (DCT-III).  Both cosine transforms are
+// Nz-point complex FFTs (Makhoul's reordering, J. Makhoul, "A fast cosine
+// transform in one and two dimensions", IEEE Trans. ASSP 28 (1980) 27-34):
+//   v[n] = s[2n], v[Nz-1-n] = s[2n+1];  V = FFT(v);
+//   C[k] = (w_k V[k] + conj(w_k) V[Nz-k]) / 2,  w_k = exp(-i pi k / (2 Nz)),
+// which holds for complex columns because the transform is real-linear; the
+// inverse is V'[k] = conj(w_k) (E[k] - i E[Nz-k]) (E[Nz] = 0), v' = IFFT(V'),
+// y[src(n)] = v'[n] / Nz.  A lane takes the pair (k, Nz-k) through the
+// forward post-twiddle, the division and the inverse pre-twiddle in registers.
+// This is the mirrored 2 Nz-point FFT solve at half the transform length.
 struct PoisZArgs {
     double2* S;          // Nz x K, k fastest
-    Plan p;              // length 2 Nz
+    Plan p;              // length Nz
+    const double2* wk;   // Nz: exp(-i pi k / (2 Nz))
     const double* lamx;  // K: (cx/cz) (2 - 2 cos(2 pi k / Nx))
-    const double* lamz;  // 2 Nz: 2 - 2 cos(2 pi kz / (2 Nz))
+    const double* lamz;  // Nz: 2 - 2 cos(pi kz / Nz)
     int Nz, K, nl;
 };
+constexpr int ZD_PER = 4;  // (k, Nz-k) pairs a lane handles: nl (Nz/2 + 1) <= ZD_PER FFT_NT
 template <int LG>
 __global__ __launch_bounds__(FFT_NT) void k_pois_zfft(PoisZArgs a) {
     extern __shared__ double2 sm[];
     double2 w[P2Fast<LG>::NST][3];
     if constexpr (LG > 0) fft_p2_twiddles<LG>(a.nl, a.p.tw, w);
-    const int n = 2 * a.Nz, nl = a.nl, K = a.K;
+    const int n = a.Nz, nl = a.nl, K = a.K, H = n / 2 + 1;
     const int k0 = xcd_tile(blockIdx.x, gridDim.x) * nl;  // neighbouring wavenumbers share S's cache lines
     double2* A = sm;
     double2* B = sm + nl * n;
+    auto src = [&](int m) { return m < (n + 1) / 2 ? 2 * m : 2 * n - 2 * m - 1; };
     for (int i = threadIdx.x; i < nl * n; i += FFT_NT) {
-        const int l = i / n, zz = i - l * n;
+        const int l = i / n, m = i - l * n;
         const int k = k0 + l;
-        const int r = zz < a.Nz ? zz : n - 1 - zz;
-        A[i] = k < K ? a.S[(int64_t)r * K + k] : make_double2(0.0, 0.0);
+        A[i] = k < K ? a.S[(int64_t)src(m) * K + k] : make_double2(0.0, 0.0);
     }
-    // the eigenvalues of this lane's entries, fetched before the transform (at most ZF_PER entries a lane)
-    constexpr int ZF_PER = 2 * MAXLINE / FFT_NT;
-    double lam[ZF_PER];
+    // this lane's pairs: the twiddle and both scaled inverse eigenvalues, fetched before the transform
+    double2 wq[ZD_PER];
+    double fa[ZD_PER], fb[ZD_PER];
+    const double inv_n = 1.0 / (double)n;
 #pragma unroll
-    for (int q = 0; q < ZF_PER; ++q) {
-        const int i = threadIdx.x + q * FFT_NT;
-        const int ic = i < nl * n ? i : 0;
-        const int l = ic / n, kz = ic - l * n;
-        const int k = k0 + l;
-        lam[q] = (k < K ? a.lamx[k] : 1.0) + a.lamz[kz];
+    for (int q = 0; q < ZD_PER; ++q) {
+        const int t = threadIdx.x + q * FFT_NT;
+        const int tc = t < nl * H ? t : 0;
+        const int l = tc / H, kz = tc - l * H;
+        const double lx = k0 + l < K ? a.lamx[k0 + l] : 1.0;
+        const double la = lx + a.lamz[kz], lb = lx + a.lamz[kz == 0 ? 0 : n - kz];
+        wq[q] = a.wk[kz];
+        fa[q] = la > 0.0 ? inv_n / la : 0.0;
+        fb[q] = lb > 0.0 ? inv_n / lb : 0.0;
     }
     __syncthreads();
     double2* X;
     if constexpr (LG > 0) X = lds_fft_p2<false, LG>(A, B, nl, w);
     else X = lds_fft<false>(A, B, nl, a.p);
-    const double inv_n = 1.0 / (double)n;
 #pragma unroll
-    for (int q = 0; q < ZF_PER; ++q) {
-        const int i = threadIdx.x + q * FFT_NT;
-        if (i < nl * n) {
-            const double f = lam[q] > 0.0 ? inv_n / lam[q] : 0.0;
-            X[i].x *= f;
-            X[i].y *= f;
+    for (int q = 0; q < ZD_PER; ++q) {
+        const int t = threadIdx.x + q * FFT_NT;
+        if (t < nl * H) {
+            const int l = t / H, kz = t - l * H, kb = kz == 0 ? 0 : n - kz;
+            double2* Xl = X + l * n;
+            const double2 va = Xl[kz], vb = Xl[kb], wa = wq[q];
+            // w_{Nz-k} = -i conj(w_k)
+            const double2 wb = make_double2(-wa.y, -wa.x);
+            // C[k] = (w_k V[k] + conj(w_k) V[Nz-k]) / 2, C[Nz-k] = (w_{Nz-k} V[Nz-k] + conj(w_{Nz-k}) V[k]) / 2
+            const double2 ca = make_double2(0.5 * ((wa.x * va.x - wa.y * va.y) + (wa.x * vb.x + wa.y * vb.y)),
+                                            0.5 * ((wa.x * va.y + wa.y * va.x) + (wa.x * vb.y - wa.y * vb.x)));
+            const double2 cb = make_double2(0.5 * ((wb.x * vb.x - wb.y * vb.y) + (wb.x * va.x + wb.y * va.y)),
+                                            0.5 * ((wb.x * vb.y + wb.y * vb.x) + (wb.x * va.y - wb.y * va.x)));
+            const double2 ea = make_double2(ca.x * fa[q], ca.y * fa[q]);
+            const double2 eb = kz == 0 ? make_double2(0.0, 0.0) : make_double2(cb.x * fb[q], cb.y * fb[q]);
+            // V'[k] = conj(w_k) (E[k] - i E[Nz-k]), V'[Nz-k] = conj(w_{Nz-k}) (E[Nz-k] - i E[k])
+            const double2 da = make_double2(ea.x + eb.y, ea.y - eb.x);
+            const double2 db = make_double2(eb.x + ea.y, eb.y - ea.x);
+            Xl[kz] = make_double2(wa.x * da.x + wa.y * da.y, wa.x * da.y - wa.y * da.x);
+            if (kb != kz && kz != 0) Xl[kb] = make_double2(wb.x * db.x + wb.y * db.y, wb.x * db.y - wb.y * db.x);
         }
     }
     __syncthreads();
     const double2* Y;
     if constexpr (LG > 0) Y = lds_fft_p2<true, LG>(X, X == A ? B : A, nl, w);
     else Y = lds_fft<true>(X, X == A ? B : A, nl, a.p);
-    for (int i = threadIdx.x; i < nl * a.Nz; i += FFT_NT) {
-        const int l = i / a.Nz, r = i - l * a.Nz;
+    for (int i = threadIdx.x; i < nl * n; i += FFT_NT) {
+        const int l = i / n, m = i - l * n;
         const int k = k0 + l;
-        if (k < K) a.S[(int64_t)r * K + k] = Y[l * n + r];
+        if (k < K) a.S[(int64_t)src(m) * K + k] = Y[i];
     }
 }
 
@@ -1752,9 +1781,9 @@ struct StokesCtx {
     int pK = 0;                 // Nx/2 + 1
     double2* pS = nullptr;      // Nz x pK half spectra
     double* plamx = nullptr;    // pK x-eigenvalues (normalised by cz)
-    double* plamz = nullptr;    // 2 Nz z-eigenvalues of the mirrored column
+    double* plamz = nullptr;    // Nz z-eigenvalues of the cosine transform
     double2* ptw = nullptr;     // Nx twiddles
-    double2* ptwz = nullptr;    // 2 Nz twiddles
+    double2* ptwz = nullptr;    // Nz twiddles of the z plan, then Nz cosine-transform twiddles
     std::vector<int> prad, pradz;
     double pratio = -1.0;       // cx / cz of this solve; <= 0: inertial part off
     double ptab = -1.0;         // cx / cz the device eigenvalue table holds
@@ -1783,11 +1812,11 @@ int pois_table(xc_ctx* ctx, StokesCtx* s, const Flags& F, int Nz, int Nx, double
     if (ratio == s->ptab) return XC_OK;
     const int K = s->pK;
     const long double pi = 3.141592653589793238462643383279502884L;
-    std::vector<double> lam((size_t)K + 2 * (size_t)Nz);
+    std::vector<double> lam((size_t)K + (size_t)Nz);
     for (int k = 0; k < K; ++k)
         lam[k] = (double)((long double)ratio * (2.0L - 2.0L * cosl(2.0L * pi * (long double)k / (long double)Nx)));
-    for (int kz = 0; kz < 2 * Nz; ++kz)
-        lam[K + kz] = (double)(2.0L - 2.0L * cosl(2.0L * pi * (long double)kz / (long double)(2 * Nz)));
+    for (int kz = 0; kz < Nz; ++kz)
+        lam[K + kz] = (double)(2.0L - 2.0L * cosl(pi * (long double)kz / (long double)Nz));
     XC_TRY_HIP(ctx, hipMemcpyAsync(s->plamx, lam.data(), lam.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     s->ptab = ratio;
@@ -1945,15 +1974,22 @@ int setup(xc_ctx* ctx, int Nz, int Nx, int m, StokesCtx** out, int maxlev = -1) 
         return xc_fail(ctx, XC_EHIP, "fluidmech!: internal pool sizing (%zu > %zu)", (size_t)(p - s->pool), tot);
     }
     tail_setup(s);
-    if (Nx > MAXLINE || 2 * Nz > MAXLINE || !fft_factorize(Nx, s->prad) || !fft_factorize(2 * Nz, s->pradz)) {
+    if (Nx > MAXLINE || Nz > MAXLINE || (Nz / 2 + 1) > ZD_PER * FFT_NT || !fft_factorize(Nx, s->prad) ||
+        !fft_factorize(Nz, s->pradz)) {
         s->prad.clear();  // inertial Schur part unavailable at this size: the viscous part alone is used
         s->pradz.clear();
     } else {
         std::vector<double2> tw;
         fft_twiddles(Nx, tw);
         e = hipMemcpy(s->ptw, tw.data(), Nx * sizeof(double2), hipMemcpyHostToDevice);
-        if (e == hipSuccess) {
-            fft_twiddles(2 * Nz, tw);
+        if (e == hipSuccess) {  // the Nz-point plan's twiddles, then the cosine transform's w_k
+            fft_twiddles(Nz, tw);
+            tw.resize(2 * (size_t)Nz);
+            const long double pi = 3.141592653589793238462643383279502884L;
+            for (int k = 0; k < Nz; ++k) {
+                const long double ang = -pi * (long double)k / (2.0L * (long double)Nz);
+                tw[Nz + k] = make_double2((double)cosl(ang), (double)sinl(ang));
+            }
             e = hipMemcpy(s->ptwz, tw.data(), 2 * (size_t)Nz * sizeof(double2), hipMemcpyHostToDevice);
         }
         if (e != hipSuccess) { stokes_free(ctx, s); return xc_fail(ctx, XC_EHIP, "fluidmech!: %s", hipGetErrorString(e)); }
@@ -2222,18 +2258,20 @@ void precond(xc_ctx* ctx, StokesCtx* s, const Flags& F, const double* v, double*
             default: hipLaunchKernelGGL(k_pois_xfwd<0>, dim3(nbx), dim3(FFT_NT), lds, ctx->stream, xa); break;
         }
         Plan pz{};
-        pz.n = 2 * Nz;
+        pz.n = Nz;
         pz.nst = (int)s->pradz.size();
         for (int q = 0; q < pz.nst; ++q) pz.rad[q] = s->pradz[q];
         pz.tw = s->ptwz;
         plan_mark_pow2(pz);
-        int nlz = K / 256;
-        const int nlzmax = MAXLINE / (2 * Nz) < 4 ? MAXLINE / (2 * Nz) : 4;
+        // one line per workgroup up to K = 512 lines; at most ZD_PER (k, Nz-k) pairs a lane
+        int nlz = K / 512;
+        int nlzmax = MAXLINE / Nz < 4 ? MAXLINE / Nz : 4;
+        while (nlzmax > 1 && nlzmax * (Nz / 2 + 1) > ZD_PER * FFT_NT) --nlzmax;
         nlz = nlz < 1 ? 1 : (nlz > nlzmax ? nlzmax : nlz);
-        PoisZArgs za{s->pS, pz, s->plamx, s->plamz, Nz, K, nlz};
+        PoisZArgs za{s->pS, pz, s->ptwz + Nz, s->plamx, s->plamz, Nz, K, nlz};
         {
             const dim3 gz((unsigned)((K + nlz - 1) / nlz));
-            const size_t ldz = 2 * (size_t)nlz * 2 * Nz * sizeof(double2);
+            const size_t ldz = 2 * (size_t)nlz * Nz * sizeof(double2);
             switch (p2_fast_lg(pz, nlz)) {
                 case 8: hipLaunchKernelGGL(k_pois_zfft<8>, gz, dim3(FFT_NT), ldz, ctx->stream, za); break;
                 case 9: hipLaunchKernelGGL(k_pois_zfft<9>, gz, dim3(FFT_NT), ldz, ctx->stream, za); break;
