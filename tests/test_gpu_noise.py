@@ -27,8 +27,11 @@ def test_normal_fill_parity(xc, orc):
         assert _rel(xc.to_host(got), want) < 1e-14
 
 
+# Filters several cells wide run as direct convolutions (both passes at (100, 100) and (512, 512); x only
+# at (40, 200); z only at (200, 40); z over two 512-row segments at (600, 24)); the rest as FFT passes.
 @pytest.mark.parametrize("Nz,Nx,pad,sig", [(24, 20, 8, 1.5), (100, 100, 16, 3.1), (31, 26, 12, 2.0),
-                                           (104, 100, 4, 0.7), (512, 512, 16, 3.06)])
+                                           (104, 100, 4, 0.7), (512, 512, 16, 3.06), (40, 200, 8, 3.0),
+                                           (200, 40, 16, 3.0), (600, 24, 16, 3.0)])
 def test_fft_filter_parity(xc, orc, Nz, Nx, pad, sig):
     print(f"fft_filter {Nz}x{Nx} pad {pad}", flush=True)
     h = 1.0 / Nz

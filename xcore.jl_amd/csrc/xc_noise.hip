@@ -11,6 +11,10 @@
 //   2/3/4/5/7/11/13, twiddles from a table computed on the host in long
 //   double).  Two real lines ride in one complex transform: the multiplier is
 //   real and even, so real and imaginary parts are filtered independently.
+//   Filters several cells wide, whose taps beyond a half-width R carry at most
+//   2^-54 of the kernel's mass, run instead as direct convolutions with the
+//   2R+1 taps of the same circular convolution (k_gconv_x for R <= 43,
+//   k_gconv_z for R <= 128; lines of at least 64 points).
 // * Mean/std rescale (noise.jl:261-267): two-pass reductions (mean, then the
 //   centred sum of squares) over all six raw/filtered fields.
 // * Flux differencing, tapers and magnitudes (noise.jl:195-236) as two
@@ -207,6 +211,182 @@ __global__ __launch_bounds__(NT) void k_filter_z(ZPassArgs a) {
     }
 }
 
+// Narrow filters as a direct convolution.  The FFT filter of a line of period n is the circular
+// convolution with c[m] = sum_t g_t cos(2 pi t m / n) (g real and even in t).  When the taps beyond
+// |m| = R carry at most 2^-54 of the total mass (a filter several cells wide), the 2R+1 taps are applied
+// directly: each lane computes CV_P consecutive outputs from a window of CV_P + 2R (rounded up) inputs,
+// the taps CV_P at a time from two register blocks of the window (ping-pong, no copies), every tap a
+// uniform (broadcast) LDS read.  taps[u] = c[|R - u|] for u <= 2R and 0 up to nblk CV_P, so output j of a lane
+// is sum_u taps[u] window[j + u], window[i] = input (first output - R + i).  The work is FMA-bound: CV_P
+// is small enough that a 512^2 field gives every SIMD several waves.
+constexpr int CV_P = 4;
+constexpr int CV_MAXTAP = 257;  // 2R+1 at most: beyond it the FFT passes are used
+constexpr int CV_MINLINE = 64;  // shorter lines: the FFT passes (the window indexing wraps once per step)
+struct ConvField {
+    const double* in;
+    int64_t ld_in;
+    double* out;
+    int64_t ld_out;
+    const double* taps;
+    int R, nblk;
+};
+// acc[s & 1][j] += sum_s tp[s] win[j + s], win = lo | hi (even and odd taps in separate sums: twice the
+// independent FMA chains)
+__device__ __forceinline__ void conv_block(const double* __restrict__ tp, const double (&lo)[CV_P],
+                                           const double (&hi)[CV_P], double (&acc)[2][CV_P]) {
+#pragma unroll
+    for (int s = 0; s < CV_P; ++s) {
+        const double w = tp[s];
+#pragma unroll
+        for (int j = 0; j < CV_P; ++j)
+            acc[s & 1][j] = fma(w, j + s < CV_P ? lo[j + s] : hi[j + s - CV_P], acc[s & 1][j]);
+    }
+}
+// taps to LDS (every wait in the convolution loop is then an in-order LDS wait, so reads of the next block
+// overlap the FMAs of this one; scalar loads would force waits for every outstanding LDS read)
+__device__ __forceinline__ void stage_taps(const ConvField& F, double* stap) {
+    for (int u = threadIdx.x; u < F.nblk * CV_P; u += blockDim.x) stap[u] = F.taps[u];
+}
+// the blocked convolution of a lane: get(i, v) fills v with window elements i .. i + CV_P - 1
+template <class Get>
+__device__ __forceinline__ void conv_lane(const double* __restrict__ taps, int nblk, Get get, double (&out)[CV_P]) {
+    double a[CV_P], b[CV_P], acc[2][CV_P];
+#pragma unroll
+    for (int j = 0; j < CV_P; ++j) acc[0][j] = acc[1][j] = 0.0;
+    get(0, a);
+    int k = 0;
+    for (; k + 1 < nblk; k += 2) {
+        get((k + 1) * CV_P, b);
+        conv_block(taps + k * CV_P, a, b, acc);
+        get((k + 2) * CV_P, a);
+        conv_block(taps + (k + 1) * CV_P, b, a, acc);
+    }
+    if (k < nblk) {
+        get((k + 1) * CV_P, b);
+        conv_block(taps + k * CV_P, a, b, acc);
+    }
+#pragma unroll
+    for (int j = 0; j < CV_P; ++j) out[j] = acc[0][j] + acc[1][j];
+}
+// x pass, periodic over Nx.  A workgroup of CVX_W waves takes 64 rows and CVX_C = CVX_W CV_P output
+// columns: the CVX_C + 2R (rounded up) window columns go to LDS ([column][row], loads in flight U at a
+// time, each a coalesced 64-row column), then lane = row and wave w computes columns w CV_P ..
+// w CV_P + CV_P - 1.  blockIdx.y = field.
+constexpr int CVX_W = 8, CVX_C = CVX_W * CV_P;
+// window tile and taps within 64 KiB of LDS (R <= 43)
+constexpr int CVX_MAXBLK = ((8192 - CV_MAXTAP - CV_P) / 64 - CVX_C) / CV_P;
+struct ConvXArgs {
+    ConvField f[3];
+    int Nz, Nx, nrc, ncc;  // row chunks of 64, column chunks of CVX_C
+};
+__global__ __launch_bounds__(64 * CVX_W) void k_gconv_x(ConvXArgs a) {
+    extern __shared__ double tile[];
+    __shared__ double stap[CV_MAXTAP + CV_P];
+    const ConvField F = a.f[blockIdx.y];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);  // neighbouring column chunks (shared halos) on one XCD
+    const int rc = t % a.nrc, cc = t / a.nrc;
+    const int r = rc * 64 + lane, Nx = a.Nx;
+    const bool ok = r < a.Nz;
+    const double* x = F.in + (ok ? r : a.Nz - 1);
+    const int c0 = cc * CVX_C;
+    const int wc = CVX_C + F.nblk * CV_P;  // window columns: column (c0 - R + i) mod Nx
+    // this wave's window columns i = wave + CVX_W k, their columns advanced by CVX_W < Nx with one wrap
+    int col = c0 - F.R + wave;
+    col += col < 0 ? Nx : 0;
+    col -= col >= Nx ? Nx : 0;
+    constexpr int U = 8;
+    for (int i0 = wave; i0 < wc; i0 += CVX_W * U) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {  // in bounds past the window too (the column wraps), so unconditional
+            v[u] = x[(int64_t)col * F.ld_in];
+            col += CVX_W;
+            col -= col >= Nx ? Nx : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + u * CVX_W;
+            if (i < wc) tile[i * 64 + lane] = v[u];
+        }
+    }
+    stage_taps(F, stap);
+    __syncthreads();
+    const double* tw = tile + wave * CV_P * 64 + lane;
+    double acc[CV_P];
+    conv_lane(stap, F.nblk, [&](int i, double (&v)[CV_P]) {
+#pragma unroll
+        for (int s = 0; s < CV_P; ++s) v[s] = tw[(i + s) * 64];
+    }, acc);
+    // stores without per-store branches, so that none waits on the one before
+    const int cw = c0 + wave * CV_P;
+    double* y = F.out + r + (int64_t)cw * F.ld_out;
+    if (ok && cw + CV_P <= Nx) {
+#pragma unroll
+        for (int j = 0; j < CV_P; ++j) y[(int64_t)j * F.ld_out] = acc[j];
+    } else if (ok) {
+        for (int j = 0; j < CV_P && cw + j < Nx; ++j) y[(int64_t)j * F.ld_out] = acc[j];
+    }
+}
+// z pass over the column zero-padded to period n = Nz + pad (the circular convolution does not depend on
+// where the zeros sit, so the column is taken as rows 0..Nz-1 then n - Nz zeros, repeated).  One wave a
+// segment of 64 CV_P rows of one column: the window goes to LDS with element q at (q % CV_P) S + q / CV_P,
+// so that lane l's element CV_P l + i sits at (i % CV_P) S + l + i / CV_P: consecutive lanes read
+// consecutive words.  blockIdx.y = field.
+struct ConvZArgs {
+    ConvField f[3];
+    int n[3];
+    int Nz, Nx, nseg;
+};
+__global__ __launch_bounds__(64) void k_gconv_z(ConvZArgs a) {
+    extern __shared__ double win[];
+    __shared__ double stap[CV_MAXTAP + CV_P];
+    const ConvField F = a.f[blockIdx.y];
+    const int n = a.n[blockIdx.y], Nz = a.Nz;
+    const int lane = threadIdx.x;
+    const int c = blockIdx.x / a.nseg, seg = blockIdx.x - c * a.nseg;
+    const int r0 = seg * 64 * CV_P;
+    const int S = 64 + F.nblk;  // words per residue class
+    const int len = CV_P * S;   // window elements: row (r0 - R + q) mod n
+    const double* x = F.in + (int64_t)c * F.ld_in;
+    // lane's elements q = lane + 64 k, their rows advanced by 64 <= n with one wrap
+    int j = r0 - F.R + lane;
+    j += j < 0 ? n : 0;
+    j -= j >= n ? n : 0;
+    constexpr int U = 8;
+    for (int q0 = lane; q0 < len; q0 += 64 * U) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {  // unconditional loads (clamped row): all in flight at once
+            const double t = x[j < Nz ? j : Nz - 1];
+            v[u] = j < Nz ? t : 0.0;
+            j += 64;
+            j -= j >= n ? n : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int q = q0 + 64 * u;
+            if (q < len) win[(q % CV_P) * S + q / CV_P] = v[u];
+        }
+    }
+    stage_taps(F, stap);
+    __syncthreads();
+    double acc[CV_P];
+    conv_lane(stap, F.nblk, [&](int i, double (&v)[CV_P]) {  // i is a multiple of CV_P
+#pragma unroll
+        for (int s = 0; s < CV_P; ++s) v[s] = win[s * S + lane + i / CV_P];
+    }, acc);
+    // stores without per-store branches, so that none waits on the one before
+    double* y = F.out + (int64_t)c * F.ld_out + r0 + CV_P * lane;
+    const int rem = Nz - (r0 + CV_P * lane);
+    if (rem >= CV_P) {
+#pragma unroll
+        for (int q = 0; q < CV_P; ++q) y[q] = acc[q];
+    } else {
+        for (int q = 0; q < rem; ++q) y[q] = acc[q];
+    }
+}
+
 // ---------------------------------------------------------------- statistics
 // six (Nz x Nx) arrays: raw e, x, s and filtered e, x, s
 struct StatArgs {
@@ -355,6 +535,8 @@ struct PlanEntry {
     std::vector<int> rad;
     double2* tw;  // device
     double* g;    // device
+    double* taps = nullptr;  // device, direct-convolution taps (ConvField layout) when R >= 0
+    int R = -1, nblk = 0;
 };
 struct NoiseCtx {
     std::vector<PlanEntry> plans;
@@ -365,6 +547,7 @@ void noise_free(xc_ctx*, void* p) {
     for (auto& e : nc->plans) {
         (void)hipFree(e.tw);
         (void)hipFree(e.g);
+        if (e.taps) (void)hipFree(e.taps);
     }
     delete nc;
 }
@@ -376,7 +559,42 @@ double kfreq(int t, int n, double h) {
     return 2.0 * 3.14159265358979323846 * ((double)(t < nn ? t : t - n) * mult);
 }
 
-int get_plan(xc_ctx* ctx, int n, double sigma, double h, Plan* out) {
+// The taps of the circular convolution the FFT filter of period n applies, in long double: c[m] =
+// sum_t g_t cos(2 pi ((t m) mod n) / n), g_t = exp(-sigma^2 k_t^2) / n.  R = the least half-width whose
+// dropped taps (R < m < n - R) carry at most 2^-54 of sum |c|; -1 when 2R+1 would reach n or CV_MAXTAP.
+int conv_taps(int n, double sigma, double h, std::vector<double>& taps, int& nblk) {
+    const long double pi = 3.141592653589793238462643383279502884L;
+    std::vector<long double> cs(n), g(n), c(n);
+    for (int j = 0; j < n; ++j) cs[j] = cosl(2.0L * pi * (long double)j / (long double)n);
+    const int nn = (n + 1) / 2;
+    for (int t = 0; t < n; ++t) {
+        const long double k = 2.0L * pi * (long double)(t < nn ? t : t - n) / ((long double)n * (long double)h);
+        g[t] = expl(-(long double)sigma * (long double)sigma * k * k) / (long double)n;
+    }
+    for (int m = 0; m <= n / 2; ++m) {
+        long double acc = 0.0L;
+        for (int t = 0; t < n; ++t) acc += g[t] * cs[(int)(((int64_t)t * m) % n)];
+        c[m] = acc;
+        c[(n - m) % n] = acc;
+    }
+    long double tot = 0.0L;
+    for (int m = 0; m < n; ++m) tot += fabsl(c[m]);
+    const long double bar = ldexpl(tot, -54);
+    // tail[m] = sum of |c| over R < m' < n - R for R = m, from the middle outwards
+    for (int R = 0; 2 * R + 1 < n && 2 * R + 1 <= CV_MAXTAP; ++R) {
+        long double drop = 0.0L;
+        for (int m = R + 1; m < n - R; ++m) drop += fabsl(c[m]);
+        if (drop <= bar) {
+            nblk = (2 * R + 1 + CV_P - 1) / CV_P;
+            taps.assign((size_t)nblk * CV_P, 0.0);
+            for (int u = 0; u <= 2 * R; ++u) taps[u] = (double)c[R > u ? R - u : u - R];
+            return R;
+        }
+    }
+    return -1;
+}
+
+int get_plan(xc_ctx* ctx, int n, double sigma, double h, Plan* out, ConvField* cf = nullptr) {
     NoiseCtx* nc = (NoiseCtx*)ctx->ext[XC_EXT_NOISE];
     if (!nc) {
         nc = new NoiseCtx();
@@ -391,6 +609,11 @@ int get_plan(xc_ctx* ctx, int n, double sigma, double h, Plan* out) {
             out->tw = e.tw;
             out->g = e.g;
             plan_mark_pow2(*out);
+            if (cf) {
+                cf->taps = e.taps;
+                cf->R = e.R;
+                cf->nblk = e.nblk;
+            }
             return XC_OK;
         }
     PlanEntry e;
@@ -411,8 +634,14 @@ int get_plan(xc_ctx* ctx, int n, double sigma, double h, Plan* out) {
     XC_TRY_HIP(ctx, hipMalloc(&e.g, n * sizeof(double)));
     XC_TRY_HIP(ctx, hipMemcpy(e.tw, tw.data(), n * sizeof(double2), hipMemcpyHostToDevice));
     XC_TRY_HIP(ctx, hipMemcpy(e.g, g.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    std::vector<double> taps;
+    e.R = conv_taps(n, sigma, h, taps, e.nblk);
+    if (e.R >= 0) {
+        XC_TRY_HIP(ctx, hipMalloc(&e.taps, taps.size() * sizeof(double)));
+        XC_TRY_HIP(ctx, hipMemcpy(e.taps, taps.data(), taps.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     nc->plans.push_back(e);
-    return get_plan(ctx, n, sigma, h, out);
+    return get_plan(ctx, n, sigma, h, out, cf);
 }
 
 // Filter three (or fewer) fields: in[f] -> out[f] using scratch tmp[f] between the passes.
@@ -422,9 +651,17 @@ int filter_fields(xc_ctx* ctx, int nf, const xc_mat* in, xc_mat* out, xc_mat* tm
     int st;
     XPassArgs xa{};
     ZPassArgs za{};
+    ConvXArgs cx{};
+    ConvZArgs cz{};
+    bool dx = true, dz = true;  // every field's pass has direct-convolution taps
     for (int f = 0; f < nf; ++f) {
-        if ((st = get_plan(ctx, Nx, sigma[f], h, &xa.plan[f]))) return st;
-        if ((st = get_plan(ctx, Nz + (int)pad[f], sigma[f], h, &za.plan[f]))) return st;
+        if ((st = get_plan(ctx, Nx, sigma[f], h, &xa.plan[f], &cx.f[f]))) return st;
+        if ((st = get_plan(ctx, Nz + (int)pad[f], sigma[f], h, &za.plan[f], &cz.f[f]))) return st;
+        dx = dx && cx.f[f].R >= 0 && cx.f[f].nblk <= CVX_MAXBLK && Nx >= CV_MINLINE;
+        dz = dz && cz.f[f].R >= 0 && Nz + pad[f] >= CV_MINLINE;
+        cx.f[f].in = in[f].p; cx.f[f].ld_in = in[f].ld; cx.f[f].out = tmp[f].p; cx.f[f].ld_out = tmp[f].ld;
+        cz.f[f].in = tmp[f].p; cz.f[f].ld_in = tmp[f].ld; cz.f[f].out = out[f].p; cz.f[f].ld_out = out[f].ld;
+        cz.n[f] = Nz + (int)pad[f];
         xa.io[f] = FieldIO{in[f].p, in[f].ld, tmp[f].p, tmp[f].ld};
         za.io[f] = FieldIO{tmp[f].p, tmp[f].ld, out[f].p, out[f].ld};
         za.hp[f] = (int)(pad[f] / 2);
@@ -442,7 +679,16 @@ int filter_fields(xc_ctx* ctx, int nf, const xc_mat* in, xc_mat* out, xc_mat* tm
     const int xblocks = (Nz + 2 * xa.nl - 1) / (2 * xa.nl);
     const dim3 gx(xblocks, nf);
     const size_t ldx = 2 * xa.nl * Nx * sizeof(double2);
-    switch (lgx) {
+    if (dx) {
+        cx.Nz = Nz;
+        cx.Nx = Nx;
+        cx.nrc = (Nz + 63) / 64;
+        cx.ncc = (Nx + CVX_C - 1) / CVX_C;
+        int nbmax = 0;
+        for (int f = 0; f < nf; ++f) nbmax = cx.f[f].nblk > nbmax ? cx.f[f].nblk : nbmax;
+        const size_t lx = (size_t)(CVX_C + nbmax * CV_P) * 64 * sizeof(double);
+        hipLaunchKernelGGL(k_gconv_x, dim3(cx.nrc * cx.ncc, nf), dim3(64 * CVX_W), lx, ctx->stream, cx);
+    } else switch (lgx) {
         case 8: hipLaunchKernelGGL(k_filter_x<8>, gx, dim3(NT), ldx, ctx->stream, xa); break;
         case 9: hipLaunchKernelGGL(k_filter_x<9>, gx, dim3(NT), ldx, ctx->stream, xa); break;
         case 10: hipLaunchKernelGGL(k_filter_x<10>, gx, dim3(NT), ldx, ctx->stream, xa); break;
@@ -452,6 +698,16 @@ int filter_fields(xc_ctx* ctx, int nf, const xc_mat* in, xc_mat* out, xc_mat* tm
     for (int f = 1; f < nf; ++f) {
         nlmin = za.nl[f] < nlmin ? za.nl[f] : nlmin;
         nmax = za.plan[f].n * za.nl[f] > nmax ? za.plan[f].n * za.nl[f] : nmax;
+    }
+    if (dz) {
+        cz.Nz = Nz;
+        cz.Nx = Nx;
+        cz.nseg = (Nz + 64 * CV_P - 1) / (64 * CV_P);
+        int nbmax = 0;
+        for (int f = 0; f < nf; ++f) nbmax = cz.f[f].nblk > nbmax ? cz.f[f].nblk : nbmax;
+        const size_t lz = (size_t)CV_P * (64 + nbmax) * sizeof(double);
+        hipLaunchKernelGGL(k_gconv_z, dim3(Nx * cz.nseg, nf), dim3(64), lz, ctx->stream, cz);
+        return XC_OK;
     }
     const int zblocks = (Nx + 2 * nlmin - 1) / (2 * nlmin);
     hipLaunchKernelGGL(k_filter_z, dim3(zblocks, nf), dim3(NT), 2 * nmax * sizeof(double2), ctx->stream, za);
