@@ -298,21 +298,23 @@ __global__ __launch_bounds__(NT) void k_cells(CellArgs a) {
         mn_eta = etai;
         mn_etas = etasi;
     }
-    // workgroup minima (exact: min is order independent)
-    __shared__ double s1[NT], s2[NT];
-    s1[threadIdx.x] = mn_eta;
-    s2[threadIdx.x] = mn_etas;
-    __syncthreads();
-    for (int st = NT / 2; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st) {
-            s1[threadIdx.x] = fmin(s1[threadIdx.x], s1[threadIdx.x + st]);
-            s2[threadIdx.x] = fmin(s2[threadIdx.x], s2[threadIdx.x + st]);
-        }
-        __syncthreads();
+    // workgroup minima (exact: min is order independent): wave minima by DPP, then the NT / 64 of them
+    __shared__ double s1[NT / 64], s2[NT / 64];
+    mn_eta = wave_min(mn_eta);
+    mn_etas = wave_min(mn_etas);
+    if ((threadIdx.x & 63) == 0) {
+        s1[threadIdx.x >> 6] = mn_eta;
+        s2[threadIdx.x >> 6] = mn_etas;
     }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        a.bmin[blockIdx.x] = s1[0];
-        a.bmin[a.d.nb + blockIdx.x] = s2[0];
+        double m1 = s1[0], m2 = s2[0];
+        for (int w = 1; w < NT / 64; ++w) {
+            m1 = fmin(m1, s1[w]);
+            m2 = fmin(m2, s2[w]);
+        }
+        a.bmin[blockIdx.x] = m1;
+        a.bmin[a.d.nb + blockIdx.x] = m2;
     }
 }
 

@@ -77,6 +77,13 @@ static inline bool xc_shape(const xc_mat& m, int64_t r, int64_t c) {
     return m.rows == r && m.cols == c && xc_mat_ok(m);
 }
 
+// element q (column-major order) of a matrix: a contiguous matrix (ld == rows) is indexed directly, any
+// other through one 32-bit division (the matrices here hold fewer than 2^31 elements)
+__device__ __forceinline__ double xc_mat_at(const xc_mat& m, int64_t q) {
+    if (m.ld == m.rows) return m.p[q];
+    const unsigned uq = (unsigned)q, ur = (unsigned)m.rows, c = uq / ur;
+    return m.p[(uq - c * ur) + (int64_t)c * m.ld];
+}
 // Hardware dispatches workgroup b of a 1-D grid to XCD (b % 8).  Remap so each
 // XCD walks a contiguous run of logical tiles: neighbouring tiles then share an
 // L2 and their halos are re-read from cache rather than from HBM.

@@ -402,7 +402,7 @@ __global__ __launch_bounds__(NT) void k_stat_partial(StatArgs s) {
     int j, i;
     xc_dom_ij(s.d, &j, &i);
     const bool in = j < s.Nz && i < s.Nx;
-    __shared__ double red[6][NT];
+    __shared__ double red[6][NT / 64];
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
         double v = 0.0;
@@ -413,18 +413,16 @@ __global__ __launch_bounds__(NT) void k_stat_partial(StatArgs s) {
                 v = v * v;
             }
         }
-        red[q][threadIdx.x] = v;
+        v = wave_total(v);  // DPP wave sum, then the NT / 64 wave sums
+        if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = v;
     }
     __syncthreads();
-    for (int st = NT / 2; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st)
+    if (threadIdx.x < 6) {
+        double t = 0.0;
 #pragma unroll
-            for (int q = 0; q < 6; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + st];
-        __syncthreads();
+        for (int w = 0; w < NT / 64; ++w) t += red[threadIdx.x][w];
+        s.part[(int64_t)threadIdx.x * s.d.nb + blockIdx.x] = t;
     }
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int q = 0; q < 6; ++q) s.part[(int64_t)q * s.d.nb + blockIdx.x] = red[q][0];
 }
 // sums partials; mode 0: stats[q] = mean; mode 1: stats[6+q] = std (n-1).  One wave per statistic.
 __global__ __launch_bounds__(6 * 64) void k_stat_final(const double* part, int nb, double n, int mode, double* stats) {

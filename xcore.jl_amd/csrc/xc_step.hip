@@ -21,7 +21,7 @@ __global__ __launch_bounds__(NT) void k_snap(xc_mat W, xc_mat U, xc_mat P, xc_ma
     const int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x;
     const int64_t nW = (int64_t)(Nz + 1) * (Nx + 2), nU = (int64_t)(Nz + 2) * (Nx + 1), nP = (int64_t)(Nz + 2) * (Nx + 2),
                   nC = (int64_t)Nz * Nx;
-    auto g = [](const xc_mat& m, int64_t q) { return m.p[(q % m.rows) + (q / m.rows) * m.ld]; };
+    auto g = [](const xc_mat& m, int64_t q) { return xc_mat_at(m, q); };
     if (k < nW) dst[k] = g(W, k);
     else if (k < nW + nU) dst[k] = g(U, k - nW);
     else if (k < nW + nU + nP) dst[k] = g(P, k - nW - nU);
@@ -35,33 +35,41 @@ __global__ __launch_bounds__(NT) void k_resnorm_part(xc_mat W, xc_mat U, xc_mat 
                                                      const double* snap, int Nz, int Nx, double* part) {
     const int64_t nW = (int64_t)(Nz + 1) * (Nx + 2), nU = (int64_t)(Nz + 2) * (Nx + 1), nP = (int64_t)(Nz + 2) * (Nx + 2),
                   nC = (int64_t)Nz * Nx;
-    const int64_t n = nW + nU + nP + 2 * nC;
     double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    auto g = [](const xc_mat& m, int64_t q) { return m.p[(q % m.rows) + (q / m.rows) * m.ld]; };
-    for (int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x; k < n; k += (int64_t)gridDim.x * NT) {
-        if (k < nW) { const double v = g(W, k), d = v - snap[k]; a[3] += d * d; a[4] += v * v; }
-        else if (k < nW + nU) { const double v = g(U, k - nW), d = v - snap[k]; a[5] += d * d; a[6] += v * v; }
-        else if (k < nW + nU + nP) { const double v = g(P, k - nW - nU), d = v - snap[k]; a[7] += d * d; a[8] += v * v; }
-        else if (k < nW + nU + nP + nC) {
-            const int64_t q = k - nW - nU - nP;
-            const double d = g(X, q) - snap[k], r = g(rho, q);
+    auto g = [](const xc_mat& m, int64_t q) { return xc_mat_at(m, q); };
+    // element q of every block at once: the loads (clamped indices) are all in flight together
+    const int64_t nmax = nW > nU ? (nW > nP ? nW : nP) : (nU > nP ? nU : nP);
+    const int64_t oU = nW, oP = nW + nU, oX = nW + nU + nP, oM = oX + nC;
+    for (int64_t q = (int64_t)blockIdx.x * NT + threadIdx.x; q < nmax; q += (int64_t)gridDim.x * NT) {
+        const int64_t qw = q < nW ? q : nW - 1, qu = q < nU ? q : nU - 1, qp = q < nP ? q : nP - 1,
+                      qc = q < nC ? q : nC - 1;
+        const double w = g(W, qw), sw = snap[qw], u = g(U, qu), su = snap[oU + qu], pp = g(P, qp),
+                     sp = snap[oP + qp], x = g(X, qc), sx = snap[oX + qc], rh = g(rho, qc), mf = g(MFS, qc),
+                     sm = snap[oM + qc];
+        if (q < nW) { const double d = w - sw; a[3] += d * d; a[4] += w * w; }
+        if (q < nU) { const double d = u - su; a[5] += d * d; a[6] += u * u; }
+        if (q < nP) { const double d = pp - sp; a[7] += d * d; a[8] += pp * pp; }
+        if (q < nC) {
+            const double d = x - sx, e = mf - sm;
             a[0] += d * d;
-            a[1] += r * r;
-        } else {
-            const double d = g(MFS, k - nW - nU - nP - nC) - snap[k];
-            a[2] += d * d;
+            a[1] += rh * rh;
+            a[2] += e * e;
         }
     }
-    __shared__ double red[9][NT];
-    for (int q = 0; q < 9; ++q) red[q][threadIdx.x] = a[q];
-    __syncthreads();
-    for (int st = NT / 2; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st)
-            for (int q = 0; q < 9; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + st];
-        __syncthreads();
+    // wave sums (DPP), then the NT / 64 wave sums of each quantity
+    __shared__ double red[9][NT / 64];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const double t = wave_total(a[q]);
+        if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = t;
     }
-    if (threadIdx.x == 0)
-        for (int q = 0; q < 9; ++q) part[(int64_t)q * gridDim.x + blockIdx.x] = red[q][0];
+    __syncthreads();
+    if (threadIdx.x < 9) {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < NT / 64; ++w) t += red[threadIdx.x][w];
+        part[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = t;
+    }
 }
 
 // out[q] = sum of the nb partials of row q, one wave per row (blockDim.x = 64 nq)

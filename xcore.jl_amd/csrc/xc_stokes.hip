@@ -1589,7 +1589,7 @@ __global__ __launch_bounds__(NT) void k_pack5(Pack5 p) {
     for (int q = 0; q < 5; ++q) {
         const xc_mat& a = p.a[q];
         if (k < a.rows * a.cols) {
-            const double v = a.p[(k % a.rows) + (k / a.rows) * a.ld];
+            const double v = xc_mat_at(a, k);
             p.dst[q][k] = v;
             p.f[(p.dst[q] - p.base) + k] = (float)v;
             if (q == 0) emax = v;
