@@ -116,3 +116,32 @@ def test_config1_box_free_slip_vs_oracle(xc, orc):
     s_dev = xc.history_sums(ph, fl)
     assert abs(s_dev[0] - float(hfl.rho.sum())) <= 1e-12 * abs(float(hfl.rho.sum()))
     assert abs(s_dev[2] - float(hph.X.sum())) <= 1e-10 * abs(float(hph.X.sum()))
+
+
+def test_checkpoint_resume_bitwise(xc, tmp_path):
+    # restart.jl: a run saved after two steps and resumed in fresh states takes the third step exactly as
+    # the uninterrupted run does (every fluid, phase and noise array and the history, bit for bit)
+    Nz = Nx = 32
+    par = xc.Parameters(D=10.0, N=Nz, L=10.0, etam0=10.0, x0=0.1)
+    g = xc.Grid.from_par(par)
+    sc = xc.compute_scales(par, g)
+
+    def fresh():
+        return xc.PhaseState(Nz, Nx), xc.FluidState(Nz, Nx), xc.NoiseState(g, sc, seed=3), xc.History()
+
+    ph, fl, ns, hst = fresh()
+    t, dt, step = xc.initialize_(ph, fl, ns, hst, g, par, sc)
+    t, dt, _ = xc.run_(ph, fl, ns, hst, g, par, sc, nsteps=2, time=t, dt=dt, step=step)
+    p = xc.restart_path(str(tmp_path), "r")
+    xc.save_checkpoint(p, ph, fl, ns, hst, time=t, dt=dt, step=2)
+    ta, dta, _ = xc.run_(ph, fl, ns, hst, g, par, sc, nsteps=3, time=t, dt=dt, step=2)
+    ph2, fl2, ns2, hst2 = fresh()
+    t2, dt2, step2 = xc.load_checkpoint_(p, ph2, fl2, ns2, hst2)
+    assert (t2, dt2, step2) == (t, dt, 2)
+    tb, dtb, _ = xc.run_(ph2, fl2, ns2, hst2, g, par, sc, nsteps=3, time=t2, dt=dt2, step=step2)
+    torch.cuda.synchronize()
+    assert (ta, dta) == (tb, dtb)
+    for a, b in ((ph, ph2), (fl, fl2), (ns, ns2)):
+        for n, x in a.arrays().items():
+            assert torch.equal(x, b.arrays()[n]), n
+    assert ns.draws == ns2.draws and hst.EB == hst2.EB and hst.time == hst2.time

@@ -18,6 +18,7 @@ from .stokes import (BnchmData, fluidmech_, lc_restrict, stokes_apply, stokes_pr
 from .driver import (History, StepState, history_sums, initialize_, record_history_, run_, run_step_,
                      step_opts)
 from . import learned
+from .restart import load_checkpoint_, resolve_restart, restart_path, save_checkpoint
 from .ensemble import BatchPlan, advect_centered_batch_, diffus_centered_batch_
 
 

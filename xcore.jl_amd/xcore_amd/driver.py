@@ -126,7 +126,7 @@ def record_history_(hst, time, dt, phase, fluid, ns, grid, a1=1.0, a2=1.0, a3=0.
 
 
 def initialize_(phase, fluid, ns, hst, grid, par, scales, xBC="periodic", zBC="closed", x_init=None):
-    """run.jl:71-140 ``initialize!`` (fresh start; restart files are out of scope).
+    """run.jl:71-140 ``initialize!`` (fresh start; a restart loads a checkpoint with ``load_checkpoint_``).
 
     The random perturbation ``rp`` comes from the device generator (seed par.seed, stream 2) in place of
     Julia's MersenneTwister.  ``x_init`` (a device matrix) replaces the perturbed profile when given.
@@ -185,7 +185,8 @@ def run_step_(phase, fluid, ns, grid, par, scales, state, opts):
 
 def run_(phase, fluid, ns, hst, grid, par, scales, nsteps=None, time=0.0, dt=None, step=0, ADVN=None,
          xBC="periodic", zBC="closed", sds=-1, top_cnv=1, bot_cnv=1, open_cnv=False, verbose=False, **fmkw):
-    """run.jl:167-277 ``run!`` (history every par.nrh steps; figures/checkpoints are out of scope)."""
+    """run.jl:167-277 ``run!`` (history every par.nrh steps; figures are out of scope, checkpoints are
+    ``save_checkpoint`` / ``load_checkpoint_`` between calls)."""
     nsteps = nsteps if nsteps is not None else 10 ** 9
     state = StepState(float(time), float(scales.dt0 if dt is None else dt), int(step))
     opts = step_opts(par, ADVN=ADVN, xBC=xBC, zBC=zBC, sds=sds, top_cnv=top_cnv, bot_cnv=bot_cnv,
