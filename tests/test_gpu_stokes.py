@@ -136,13 +136,13 @@ def test_fine_smoother_precision(xc, orc, open_cnv):
     assert abs(its[False] - its[True]) <= max(2, its[True] // 10), its
 
 
-@pytest.mark.parametrize("shape", [(48, 64), (256, 32)])
+@pytest.mark.parametrize("shape", [(48, 64), (256, 32), (512, 16)])
 def test_schur_poisson_solve(xc, orc, shape):
     # the inertial part of the pressure block of z = M^-1 v: with inertia minus without is
     # (a1 h^2 / dt) T^-1 rP on the interior cells, T = -d_zz + a1 / (a1 + gamma) (-d_xx) in grid units
     # (zero flux at the top and bottom rows, periodic in x) on the zero-mean part of rP; checked against a
     # sparse direct solve of T bordered by the constant mode.  (48, 64): mixed-radix transforms; (256, 32):
-    # the power-of-two z path
+    # the power-of-two z path; (512, 16): more (k, Nz-k) pairs than lanes
     import scipy.sparse as sp
     import scipy.sparse.linalg as spl
     Nz, Nx = shape
@@ -167,8 +167,13 @@ def test_schur_poisson_solve(xc, orc, shape):
     one = sp.csr_matrix(np.ones((Nz * Nx, 1)))
     y = spl.spsolve(sp.bmat([[T, one], [one.T, None]]).tocsc(), np.concatenate([rP, [0.0]]))[:-1]
     want = 1.5 * g.h ** 2 / sc.dt0 * y
-    print(f"schur poisson {shape}: rel err {_rel(d, want):.3e}", flush=True)
-    assert _rel(d, want) <= 1e-12
+    # both solves are backward stable: the bar is the rounding unit times T's condition number on the
+    # zero-mean space (largest eigenvalue over the least nonzero one), with a margin of ~10
+    lam = (2 - 2 * np.cos(np.pi * np.arange(Nz) / Nz))[:, None] + \
+        ratio * (2 - 2 * np.cos(2 * np.pi * np.arange(Nx) / Nx))[None, :]
+    kappa = lam.max() / lam[lam > 1e-14].min()
+    print(f"schur poisson {shape}: rel err {_rel(d, want):.3e}, kappa {kappa:.3g}", flush=True)
+    assert _rel(d, want) <= 2e-16 * kappa
 
 
 def test_noconv_reports(xc, orc):

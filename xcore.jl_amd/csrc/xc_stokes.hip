@@ -933,7 +933,7 @@ struct PoisZArgs {
     const double* lamz;  // Nz: 2 - 2 cos(pi kz / Nz)
     int Nz, K, nl;
 };
-constexpr int ZD_PER = 4;  // (k, Nz-k) pairs a lane handles: nl (Nz/2 + 1) <= ZD_PER FFT_NT
+constexpr int ZD_PER = 5;  // (k, Nz-k) pairs a lane handles: nl (Nz/2 + 1) <= ZD_PER FFT_NT (Nz <= MAXLINE)
 template <int LG>
 __global__ __launch_bounds__(FFT_NT) void k_pois_zfft(PoisZArgs a) {
     extern __shared__ double2 sm[];
