@@ -79,21 +79,23 @@ __global__ __launch_bounds__(NT) void k_interp(InterpArgs a) {
     const int Nz = p.Nz, Nx = p.Nx;
     if (j >= Nz + 2 || i >= Nx + 2) return;
     auto vf = [&](int z, int x) { return volfrac(a.x(z, x), a.m(z, x), p); };
+    const int z0 = gidx(j - 1, Nz, p.zbc), z1 = gidx(j, Nz, p.zbc);
+    const int x0 = gidx(i - 1, Nx, p.xbc), x1 = gidx(i, Nx, p.xbc);
+    // the four cells every output of this lane reads, each evaluated once (gidx is the identity inside
+    // the grid, so the cell (j, i) is (z1, x1) where it exists, (z0, i) is (z0, x1) for i < Nx, and
+    // (j, x0) is (z1, x0) for j < Nz)
+    const VF A = vf(z0, x0), B = vf(z0, x1), C = vf(z1, x0), D = vf(z1, x1);
     if (j < Nz && i < Nx) {
         const double x = a.x(j, i), m = a.m(j, i);
-        VF c = volfrac(x, m, p);
-        a.rho(j, i) = c.rho;
-        a.chi(j, i) = c.chi;
-        a.mu(j, i) = c.mu;
+        a.rho(j, i) = D.rho;
+        a.chi(j, i) = D.chi;
+        a.mu(j, i) = D.mu;
         const double sq = sqrt(EPS);
         a.hasx(j, i) = x >= sq;
         a.hasm(j, i) = m >= sq;
     }
-    const int z0 = gidx(j - 1, Nz, p.zbc), z1 = gidx(j, Nz, p.zbc);
-    const int x0 = gidx(i - 1, Nx, p.xbc), x1 = gidx(i, Nx, p.xbc);
     if (j <= Nz && i < Nx) {  // z-faces (Nz+1, Nx)
-        VF t = vf(z0, i), b = vf(z1, i);
-        const double rw = (t.rho + b.rho) / 2.0;
+        const double rw = (B.rho + D.rho) / 2.0;
         a.rhow(j, i) = rw;
         a.Drhom(j, i) = p.rhom0 - rw;
         a.Drhox(j, i) = p.rhox0 - rw;
@@ -101,20 +103,17 @@ __global__ __launch_bounds__(NT) void k_interp(InterpArgs a) {
         a.m_w(j, i) = (a.m(z0, i) + a.m(z1, i)) / 2.0;
     }
     if (j < Nz && i <= Nx) {  // x-faces (Nz, Nx+1)
-        VF l = vf(j, x0), r = vf(j, x1);
-        a.rhou(j, i) = (l.rho + r.rho) / 2.0;
+        a.rhou(j, i) = (C.rho + D.rho) / 2.0;
         a.x_u(j, i) = (a.x(j, x0) + a.x(j, x1)) / 2.0;
         a.m_u(j, i) = (a.m(j, x0) + a.m(j, x1)) / 2.0;
     }
     if (j <= Nz) {  // W-type (Nz+1, Nx+2): icx column i
-        VF t = vf(z0, x0), b = vf(z1, x0);
-        a.chiw(j, i) = (t.chi + b.chi) / 2.0;
-        a.muw(j, i) = (t.mu + b.mu) / 2.0;
+        a.chiw(j, i) = (A.chi + C.chi) / 2.0;
+        a.muw(j, i) = (A.mu + C.mu) / 2.0;
     }
     if (i <= Nx) {  // U-type (Nz+2, Nx+1): icz row j
-        VF l = vf(z0, x0), r = vf(z0, x1);
-        a.chiu(j, i) = (l.chi + r.chi) / 2.0;
-        a.muu(j, i) = (l.mu + r.mu) / 2.0;
+        a.chiu(j, i) = (A.chi + B.chi) / 2.0;
+        a.muu(j, i) = (A.mu + B.mu) / 2.0;
     }
 }
 
