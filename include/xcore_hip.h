@@ -135,9 +135,16 @@ typedef struct {  /* parameters.jl:87-164: the knobs the hot path reads */
     double rhom0, rhox0, d0, g0, etam0, etax0;
     double AA[4], BB[4], CC[4];  /* row-major 2x2: [11, 12, 21, 22] */
     double Xi, Ptop, alpha, gamma, CFL, dtmax, etacntr;
+    /* read by initialize! only (run.jl:71-140) */
+    double x0, Da, dxr, dxg;
+    uint64_t seed;
+    int open_sgr;
 } xc_par;
 
-typedef struct { double L0, L0h, l0, l0h, D0, G0, txi0; } xc_scales;  /* scales.jl:290-334 subset */
+typedef struct {  /* scales.jl:290-334 subset */
+    double L0, L0h, l0, l0h, D0, G0, txi0;
+    double bnd_w, dt0;  /* read by initialize! only */
+} xc_scales;
 
 /* ---- ensembles: many grids of any sizes advance in one launch (no counterpart in the reference;
  * each member is exactly the single-grid advect_centered[_with_flux]! / diffus_centered[_with_flux]!
@@ -308,17 +315,42 @@ typedef struct {
 
 /* run.jl:204-275: one outer step of run! -- time_coefs, store_previous!, store_noise!, then the
  * Picard loop (>= 3 sweeps of snapshot!, phsevo!, fluidmech!, noise!, update_phase_velocities!,
- * update!, compute_resnorm), time += dt, dt = update_dt(...).  state->step/time/dt advance. */
+ * update!, compute_resnorm), time += dt, dt = update_dt(...).  state->step/time/dt advance only when
+ * the step succeeds.  A failed step (XC_ENOCONV from a Stokes solve or a NaN Picard residual, or any
+ * other error) returns with state unchanged but the fields part-way through the step: they are not a
+ * state of the run, which has to resume from a checkpoint (the reference raises at the same point,
+ * diagnose.jl:80). */
 int xc_run_step(xc_ctx* ctx, xc_phase* phase, xc_fluid* fluid, xc_noise* ns, const xc_grid* grid,
                 const xc_par* par, const xc_scales* scales, const xc_step_opts* opts, xc_step_state* state,
                 xc_step_stats* stats);
-/* diagnose.jl:42-49 snapshot!(res, phase, fluid) into the context */
+/* diagnose.jl:42-49 snapshot!(res, phase, fluid): the context holds one snapshot (of the grid last
+ * passed; the reference's StepResidual, diagnose.jl:13-28) */
 int xc_snapshot(xc_ctx* ctx, const xc_phase* phase, const xc_fluid* fluid);
-/* diagnose.jl:57-71 compute_resnorm(res, phase, fluid, dt) -> out = {resnorm, res_mass, res_mmnt} */
+/* diagnose.jl:57-71 compute_resnorm(res, phase, fluid, dt) -> out = {resnorm, res_mass, res_mmnt}
+ * against the context's snapshot; XC_EINVAL_ARG when no snapshot was taken or it is of another grid
+ * size (synchronizes) */
 int xc_compute_resnorm(xc_ctx* ctx, const xc_phase* phase, const xc_fluid* fluid, double dt, double out[3]);
 /* history.jl:122-151 raw sums: {sum rho, sum M, sum X, sum Gx, top/bottom line sums of qz_advn_X,
  * qz_advn_M, qz_dffn_X, qz_dffn_M over the interior columns} (12 values, synchronizes) */
 int xc_history_sums(xc_ctx* ctx, const xc_phase* phase, const xc_fluid* fluid, double out[12]);
+/* history.jl:101-222 record_history!'s reductions (synchronizes): out[0..11] as xc_history_sums, then
+ * for each field f of {x, V, vx, vm, ns.xie, ns.xix, ns.xis, Ra, Rc, ReD, Red, ks, ke, kx, eta, etamix,
+ * etae, etas, etat} (history.jl:184-221) the 7 values out[12 + 7 f + q]: q = 0 min, 1 max, 2 sum,
+ * 3 sum of squares, 4 sum of (a - a[0]), 5 sum of (a - a[0])^2 (mean and n-1 std without
+ * cancellation), 6 sum of log(max(a, eps)) (the geometric mean).  The host turns them into the
+ * History record (min/mean/max/std, min/rms/max, min/geomean/max). */
+#define XC_HIST_NSTATS 145
+int xc_history_stats(xc_ctx* ctx, const xc_phase* phase, const xc_fluid* fluid, const xc_noise* ns,
+                     double out[XC_HIST_NSTATS]);
+/* run.jl:71-140 initialize!(phase, fluid, ns, hst, grid, par, scales; xBC, zBC), fresh start: the
+ * boundary taper and shape (run.jl:24-55), x = xin (1 + dxr rp + dxg gp) with rp the normalised white
+ * noise (the library's Philox4x32-10 generator, seed par->seed, stream 2, in place of Julia's
+ * MersenneTwister) -- or x = *x_init when x_init != NULL --, m = 1 - x, eta = etaco = etas = etam0,
+ * update!, X = rho x, M = rho m, noise!(dt0, first_iter), update_phase_velocities!, Xo = Xoo = X,
+ * Mo = Moo = M, rhoo = rhooo = rho, store_noise!.  The t = 0 history record is the caller's
+ * (xc_history_stats), as is a restart (load the checkpoint instead). */
+int xc_initialize(xc_ctx* ctx, xc_phase* phase, xc_fluid* fluid, xc_noise* ns, const xc_grid* grid,
+                  const xc_par* par, const xc_scales* scales, int xbc, int zbc, const xc_mat* x_init);
 
 /* ---- learned coarse correction (no counterpart in the reference: the learned Stokes
  * preconditioner).  U-Net, 4 levels, channels 16/32/48/64, BF16 weights and activations with

@@ -483,3 +483,120 @@ def run_step(ph, fl, ns, g, par, sc, state, tint="bd2im", scheme="weno5", xbc="p
     state["time"] += dt
     state["dt"] = update_dt(ph, fl, g, par, dt)
     return it
+
+
+# ---- initialize! (run.jl:71-140) and record_history! (history.jl:101-222) ------
+
+def initialize(ph, fl, ns, g, par, sc, xbc="periodic", zbc="closed", x_init=None, seed=0):
+    """run.jl:71-140 fresh-start ``initialize!`` (without the t = 0 history record, see record_history).
+
+    rp is drawn from the counter-based generator (seed par.seed, stream 2, draw 0) in place of Julia's
+    MersenneTwister.  Returns (time, dt, step, draws)."""
+    Nz, Nx, h, D, L = g.Nz, g.Nx, g.h, g.D, g.L
+    osgr = 1.0 if par.open_sgr else 0.0
+    # run.jl:24-34 compute_bndtaperw!
+    for j in range(Nz + 1):
+        z = float(j) * h
+        ph.bndtaperw[j, :] = 1.0 - (np.exp(-z / sc.l0h) + np.exp(-(D - z) / sc.l0h)) * (1.0 - osgr)
+    # run.jl:47-55 compute_bndshape!
+    for j in range(Nz):
+        zc = (float(j + 1) - 0.5) * h
+        ph.bndshape[j, :] = np.exp((-zc + h / 2.0) / sc.bnd_w)
+    if x_init is None:
+        rp = normal_fill(zeros(Nz, Nx), par.seed, 2, 0)
+        rp = (rp - np.sum(rp) / rp.size) / np.std(rp, ddof=1)   # run.jl:99
+        for j in range(Nz):                                      # run.jl:100-107
+            for i in range(Nx):
+                xc = (float(i + 1) - 0.5) * h / L
+                zc = (float(j + 1) - 0.5) * h / D
+                gp = np.exp(-((xc - 0.5) / 0.125) ** 2) * np.exp(-((zc - 0.5) / 0.125) ** 2)
+                xin = par.x0 + (par.Da - par.x0) * ph.bndshape[j, i]
+                ph.x[j, i] = xin * (1.0 + par.dxr * rp[j, i] + par.dxg * gp)
+    else:
+        ph.x[:] = x_init
+    ph.m[:] = 1.0 - ph.x
+    fl.eta[:] = par.etam0
+    fl.etaco[:] = par.etam0
+    ph.etas[:] = par.etam0
+    update(ph, fl, g, par, sc, xbc, zbc)
+    ph.X[:] = fl.rho * ph.x
+    ph.M[:] = fl.rho * ph.m
+    noise(ns, ph, g, par, sc, sc.dt0, first_iter=True, seed=seed, draw=0)
+    update_phase_velocities(ph, fl, ns, g, par, xbc, zbc)
+    ph.Xo[:] = ph.X; ph.Xoo[:] = ph.X
+    ph.Mo[:] = ph.M; ph.Moo[:] = ph.M
+    fl.rhoo[:] = fl.rho; fl.rhooo[:] = fl.rho
+    store_noise(ns)
+    return 0.0, sc.dt0, 0, 1
+
+
+HISTORY_STATS = (("x", "std"), ("V", "rms"), ("vx", "rms"), ("vm", "rms"), ("xie", "rms"), ("xix", "rms"),
+                 ("xis", "rms"), ("Ra", "gm"), ("Rc", "gm"), ("ReD", "gm"), ("Red", "gm"), ("ks", "gm"),
+                 ("ke", "gm"), ("kx", "gm"), ("eta", "gm"), ("etamix", "gm"), ("etae", "gm"), ("etas", "gm"),
+                 ("etat", "gm"))
+
+
+def new_history():
+    """history.jl:74-93 ``History(T)`` as a dict of lists plus the nine cached rates."""
+    hst = {k: [] for k in ("time", "dt", "sumB", "sumM", "sumX", "EB", "EM", "EX", "DB", "DM", "DX")}
+    for f, kind in HISTORY_STATS:
+        for s in ("min", "mean" if kind == "std" else kind, "max") + (("std",) if kind == "std" else ()):
+            hst[f"{f}_{s}"] = []
+    for q in "BMX":
+        for sfx in ("", "o", "oo"):
+            hst[f"dsum{q}dt{sfx}"] = 0.0
+    return hst
+
+
+def record_history(hst, time, dt, ph, fl, ns, g, a1=1.0, a2=1.0, a3=0.0, b1=1.0, b2=0.0, b3=0.0):
+    """history.jl:101-222 ``record_history!`` on the host arrays (numpy reductions)."""
+    ep = 2.220446049250313e-16
+    h = g.h
+    h2 = h ** 2
+    for q in "BMX":
+        hst[f"dsum{q}dtoo"] = hst[f"dsum{q}dto"]
+        hst[f"dsum{q}dto"] = hst[f"dsum{q}dt"]
+    hst["time"].append(float(time))
+    hst["dt"].append(float(dt))
+    sumB = np.sum(fl.rho) * h2 + ep
+    sumM = np.sum(ph.M) * h2 + ep
+    sumX = np.sum(ph.X) * h2 + ep
+    hst["sumB"].append(sumB); hst["sumM"].append(sumM); hst["sumX"].append(sumX)
+    Nz1 = ph.qz_advn_X.shape[0]
+
+    def line(q, r):
+        return np.sum(q[r, 1:-1]) * h
+
+    top_aX, bot_aX = line(ph.qz_advn_X, 0), line(ph.qz_advn_X, Nz1 - 1)
+    top_aM, bot_aM = line(ph.qz_advn_M, 0), line(ph.qz_advn_M, Nz1 - 1)
+    top_dX, bot_dX = line(ph.qz_dffn_X, 0), line(ph.qz_dffn_X, Nz1 - 1)
+    top_dM, bot_dM = line(ph.qz_dffn_M, 0), line(ph.qz_dffn_M, Nz1 - 1)
+    sumGx = np.sum(ph.Gx) * h2
+    dB = (top_aX - bot_aX) + (top_aM - bot_aM) + (top_dX - bot_dX) + (top_dM - bot_dM)
+    dM = -sumGx + (top_aM - bot_aM) + (top_dM - bot_dM)
+    dX = +sumGx + (top_aX - bot_aX) + (top_dX - bot_dX)
+    hst["dsumBdt"], hst["dsumMdt"], hst["dsumXdt"] = dB, dM, dX
+    stp = len(hst["time"])
+    for q, new in (("B", dB), ("M", dM), ("X", dX)):
+        Dv = hst[f"D{q}"]
+        if stp >= 2:
+            Dv.append((a2 * Dv[stp - 2] + a3 * Dv[max(1, stp - 2) - 1] +
+                       (b1 * new + b2 * hst[f"dsum{q}dto"] + b3 * hst[f"dsum{q}dtoo"]) * dt) / a1)
+        else:
+            Dv.append(0.0)
+    sumB0 = hst["sumB"][0]
+    hst["EB"].append((sumB - hst["DB"][stp - 1] - sumB0) / sumB0)
+    hst["EM"].append((sumM - hst["DM"][stp - 1] - hst["sumM"][0]) / sumB0)
+    hst["EX"].append((sumX - hst["DX"][stp - 1] - hst["sumX"][0]) / sumB0)
+    for f, kind in HISTORY_STATS:
+        a = getattr(ns, f) if f in ("xie", "xix", "xis") else (getattr(ph, f) if hasattr(ph, f) else getattr(fl, f))
+        hst[f"{f}_min"].append(float(np.min(a)))
+        hst[f"{f}_max"].append(float(np.max(a)))
+        if kind == "std":
+            hst[f"{f}_mean"].append(float(np.mean(a)))
+            hst[f"{f}_std"].append(float(np.std(a, ddof=1)))
+        elif kind == "rms":
+            hst[f"{f}_rms"].append(float(np.sqrt(np.mean(a ** 2))))
+        else:
+            hst[f"{f}_gm"].append(float(np.exp(np.mean(np.log(np.maximum(a, ep))))))
+    return hst

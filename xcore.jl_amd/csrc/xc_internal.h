@@ -53,6 +53,12 @@ int xc_scratch(xc_ctx* ctx, int slot, size_t bytes, void** out);
 int xc_diffus_centered_kprod(xc_ctx* ctx, xc_mat dff, const xc_mat* qx, const xc_mat* qz, xc_mat f, xc_mat k1,
                              xc_mat k2, double h, int xbc, int zbc);
 
+// record_history!'s reductions (xc_step.hip) enqueued on stream st: part holds XC_HIST_PART_N doubles of
+// partial sums (12 x 512 for the sums, then 19 x 7 x 64 for the field statistics), out XC_HIST_NSTATS results
+#define XC_HIST_PART_N (12 * 512 + 19 * 7 * 64)
+int xc_history_enqueue(xc_ctx* ctx, hipStream_t st, const xc_phase* ph, const xc_fluid* fl, const xc_noise* ns,
+                       double* part, double* out, const char* who);
+
 // Learned coarse correction (xc_unet.hip), used by the Stokes preconditioner:
 // network forward into the context's own output buffer ([p][3] FP32, valid until the next call),
 // whether parameters were loaded, and the per-channel input/output scales of the trained network.

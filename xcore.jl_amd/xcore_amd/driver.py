@@ -1,5 +1,5 @@
 """Time loop with the reference's signatures: initialize! (run.jl:71-140), run!
-(run.jl:167-277) and the history sums of record_history! (history.jl:101-181).
+(run.jl:167-277) and record_history! (history.jl:101-222).
 
 One outer step runs inside the library (xc_run_step); this module only keeps
 the host-side bookkeeping the reference keeps in Julia (time, dt, step,
@@ -7,14 +7,12 @@ History vectors).
 """
 import ctypes
 import math
-from dataclasses import dataclass, field
-
-import torch
 
 from . import _lib
 from ._lib import BCS, SCHEMES, lib
-from .fields import TINTS, noise_, normal_fill_, store_noise_, update_, update_phase_velocities_
+from .fields import TINTS
 from .kernels import _sym, context
+from .arrays import as_mat
 from .state import grid_struct, par_struct, scales_struct
 from .stokes import FmOpts, fm_opts
 
@@ -39,10 +37,50 @@ class StepStats(ctypes.Structure):
                 ("stokes_iters_first", _I), ("stokes_init_rel_first", _D), ("stokes_init_rel_later", _D)]
 
 
+# history.jl:184-221: the statistics record_history! keeps, in the order of xc_history_stats
+HIST_FIELDS = ("x", "V", "vx", "vm", "xie", "xix", "xis", "Ra", "Rc", "ReD", "Red", "ks", "ke", "kx", "eta",
+               "etamix", "etae", "etas", "etat")
+# what each field records: x min/mean/max/std; speeds min/rms/max; the rest min/geometric mean/max
+HIST_KIND = {"x": "std", "V": "rms", "vx": "rms", "vm": "rms", "xie": "rms", "xix": "rms", "xis": "rms"}
+XC_HIST_NSTATS = 12 + 7 * len(HIST_FIELDS)
+
+
+def _hist_vectors():
+    names = ["time", "dt", "sumB", "sumM", "sumX", "EB", "EM", "EX", "DB", "DM", "DX"]
+    for f in HIST_FIELDS:
+        kind = HIST_KIND.get(f, "gm")
+        names += [f"{f}_min", f"{f}_mean" if kind == "std" else f"{f}_{kind}", f"{f}_max"]
+        if kind == "std":
+            names.append(f"{f}_std")
+    return names
+
+
+HIST_VECTORS = _hist_vectors()
+
+
+class History:
+    """history.jl:21-93 ``History(T)``: every vector the reference appends to, plus the cached rates."""
+
+    def __init__(self):
+        for n in HIST_VECTORS:
+            setattr(self, n, [])
+        for q in ("B", "M", "X"):
+            for sfx in ("", "o", "oo"):
+                setattr(self, f"dsum{q}dt{sfx}", 0.0)
+
+    def vectors(self):
+        return {n: getattr(self, n) for n in HIST_VECTORS}
+
+    def rates(self):
+        return {f"dsum{q}dt{sfx}": getattr(self, f"dsum{q}dt{sfx}") for q in ("B", "M", "X") for sfx in ("", "o", "oo")}
+
+
 _lib.register("xc_run_step", [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P])
 _lib.register("xc_history_sums", [_P, _P, _P, ctypes.POINTER(_D)])
+_lib.register("xc_history_stats", [_P, _P, _P, _P, ctypes.POINTER(_D)])
 _lib.register("xc_snapshot", [_P, _P, _P])
 _lib.register("xc_compute_resnorm", [_P, _P, _P, _D, ctypes.POINTER(_D)])
+_lib.register("xc_initialize", [_P, _P, _P, _P, _P, _P, _P, _I, _I, _P])
 
 
 def step_opts(par, ADVN=None, xBC="periodic", zBC="closed", fixed_sweeps=0, sds=-1, top_cnv=1, bot_cnv=1,
@@ -57,32 +95,6 @@ def step_opts(par, ADVN=None, xBC="periodic", zBC="closed", fixed_sweeps=0, sds=
     return o
 
 
-@dataclass
-class History:
-    """history.jl:21-93 (the conservation part of it)."""
-    time: list = field(default_factory=list)
-    dt: list = field(default_factory=list)
-    sumB: list = field(default_factory=list)
-    sumM: list = field(default_factory=list)
-    sumX: list = field(default_factory=list)
-    DB: list = field(default_factory=list)
-    DM: list = field(default_factory=list)
-    DX: list = field(default_factory=list)
-    EB: list = field(default_factory=list)
-    EM: list = field(default_factory=list)
-    EX: list = field(default_factory=list)
-    x_mean: list = field(default_factory=list)
-    dsumBdt: float = 0.0
-    dsumBdto: float = 0.0
-    dsumBdtoo: float = 0.0
-    dsumMdt: float = 0.0
-    dsumMdto: float = 0.0
-    dsumMdtoo: float = 0.0
-    dsumXdt: float = 0.0
-    dsumXdto: float = 0.0
-    dsumXdtoo: float = 0.0
-
-
 def history_sums(phase, fluid):
     c = context()
     out = (_D * 12)()
@@ -91,16 +103,42 @@ def history_sums(phase, fluid):
     return list(out)
 
 
+def history_stats(phase, fluid, ns):
+    """The XC_HIST_NSTATS reductions of record_history! (include/xcore_hip.h, xc_history_stats)."""
+    c = context()
+    out = (_D * XC_HIST_NSTATS)()
+    ps, fs, nss = phase.pack(), fluid.pack(), ns.pack()
+    c.check(lib().xc_history_stats(c.ptr, ctypes.byref(ps), ctypes.byref(fs), ctypes.byref(nss), out),
+            "record_history!")
+    return list(out)
+
+
+def snapshot_(phase, fluid):
+    """diagnose.jl:42-49 ``snapshot!`` into the context (one snapshot per context)."""
+    c = context()
+    ps, fs = phase.pack(), fluid.pack()
+    c.check(lib().xc_snapshot(c.ptr, ctypes.byref(ps), ctypes.byref(fs)), "snapshot!")
+
+
+def compute_resnorm(phase, fluid, dt):
+    """diagnose.jl:57-71 -> (resnorm, res_mass, res_mmnt) against the context's snapshot."""
+    c = context()
+    out = (_D * 3)()
+    ps, fs = phase.pack(), fluid.pack()
+    c.check(lib().xc_compute_resnorm(c.ptr, ctypes.byref(ps), ctypes.byref(fs), float(dt), out), "compute_resnorm")
+    return tuple(out)
+
+
 def record_history_(hst, time, dt, phase, fluid, ns, grid, a1=1.0, a2=1.0, a3=0.0, b1=1.0, b2=0.0, b3=0.0):
-    """history.jl:101-181 (mass sums, boundary-flux drift, conservation errors EB/EM/EX)."""
+    """history.jl:101-222 ``record_history!``: the reductions on the device, the bookkeeping here."""
     h = grid.h
     h2 = h * h
-    s = history_sums(phase, fluid)
+    s = history_stats(phase, fluid, ns)
     hst.dsumBdtoo, hst.dsumBdto = hst.dsumBdto, hst.dsumBdt
     hst.dsumMdtoo, hst.dsumMdto = hst.dsumMdto, hst.dsumMdt
     hst.dsumXdtoo, hst.dsumXdto = hst.dsumXdto, hst.dsumXdt
-    hst.time.append(time)
-    hst.dt.append(dt)
+    hst.time.append(float(time))
+    hst.dt.append(float(dt))
     sumB, sumM, sumX = s[0] * h2 + EPS, s[1] * h2 + EPS, s[2] * h2 + EPS
     hst.sumB.append(sumB); hst.sumM.append(sumM); hst.sumX.append(sumX)
     top_aX, bot_aX, top_aM, bot_aM, top_dX, bot_dX, top_dM, bot_dM = (v * h for v in s[4:12])
@@ -122,48 +160,36 @@ def record_history_(hst, time, dt, phase, fluid, ns, grid, a1=1.0, a2=1.0, a3=0.
     hst.EB.append((sumB - hst.DB[-1] - sumB0) / sumB0)
     hst.EM.append((sumM - hst.DM[-1] - hst.sumM[0]) / sumB0)
     hst.EX.append((sumX - hst.DX[-1] - hst.sumX[0]) / sumB0)
-    hst.x_mean.append(float(phase.x.mean()))
+    n = float(grid.Nz * grid.Nx)
+    for i, f in enumerate(HIST_FIELDS):
+        mn, mx, s1, s2, d1, d2, sl = s[12 + 7 * i: 19 + 7 * i]
+        kind = HIST_KIND.get(f, "gm")
+        getattr(hst, f"{f}_min").append(mn)
+        getattr(hst, f"{f}_max").append(mx)
+        if kind == "std":
+            hst.x_mean.append(s1 / n)
+            hst.x_std.append(math.sqrt(max(d2 - d1 * d1 / n, 0.0) / (n - 1.0)))
+        elif kind == "rms":
+            getattr(hst, f"{f}_rms").append(math.sqrt(s2 / n))
+        else:
+            getattr(hst, f"{f}_gm").append(math.exp(sl / n))
 
 
 def initialize_(phase, fluid, ns, hst, grid, par, scales, xBC="periodic", zBC="closed", x_init=None):
     """run.jl:71-140 ``initialize!`` (fresh start; a restart loads a checkpoint with ``load_checkpoint_``).
 
-    The random perturbation ``rp`` comes from the device generator (seed par.seed, stream 2) in place of
+    The random perturbation ``rp`` comes from the library's generator (seed par.seed, stream 2) in place of
     Julia's MersenneTwister.  ``x_init`` (a device matrix) replaces the perturbed profile when given.
     """
-    Nz, Nx, h, D, L = grid.Nz, grid.Nx, grid.h, grid.D, grid.L
-    dev = phase.x.device
-    dd = torch.float64
-    zf = torch.arange(Nz + 1, dtype=dd, device=dev) * h
-    osgr = 1.0 if par.open_sgr else 0.0
-    taper = 1.0 - (torch.exp(-zf / scales.l0h) + torch.exp(-(D - zf) / scales.l0h)) * (1.0 - osgr)
-    phase.bndtaperw.copy_(taper[:, None].expand(Nz + 1, Nx + 2))
-    zc = (torch.arange(Nz, dtype=dd, device=dev) + 0.5) * h
-    phase.bndshape.copy_(torch.exp((-zc + h / 2) / scales.bnd_w)[:, None].expand(Nz, Nx))
-    if x_init is None:
-        rp = torch.zeros((Nx, Nz), dtype=dd, device=dev).t()
-        normal_fill_(rp, par.seed, stream=2, draw=0)
-        rp = (rp - rp.mean()) / rp.std()
-        xcn = (torch.arange(Nx, dtype=dd, device=dev) + 0.5) * h / L
-        zcn = (torch.arange(Nz, dtype=dd, device=dev) + 0.5) * h / D
-        gp = torch.exp(-((zcn[:, None] - 0.5) / 0.125) ** 2) * torch.exp(-((xcn[None, :] - 0.5) / 0.125) ** 2)
-        xin = par.x0 + (par.Da - par.x0) * phase.bndshape
-        phase.x.copy_(xin * (1.0 + par.dxr * rp + par.dxg * gp))
-    else:
-        phase.x.copy_(x_init)
-    phase.m.copy_(1.0 - phase.x)
-    fluid.eta.fill_(par.etam0)
-    fluid.etaco.fill_(par.etam0)
-    phase.etas.fill_(par.etam0)
-    update_(phase, fluid, grid, par, scales, xBC=xBC, zBC=zBC)
-    phase.X.copy_(fluid.rho * phase.x)
-    phase.M.copy_(fluid.rho * phase.m)
-    noise_(ns, phase, grid, par, scales, scales.dt0, first_iter=True)
-    update_phase_velocities_(phase, fluid, ns, grid, par, xBC=xBC, zBC=zBC)
-    phase.Xo.copy_(phase.X); phase.Xoo.copy_(phase.X)
-    phase.Mo.copy_(phase.M); phase.Moo.copy_(phase.M)
-    fluid.rhoo.copy_(fluid.rho); fluid.rhooo.copy_(fluid.rho)
-    store_noise_(ns)
+    c = context()
+    ps, fs, nss = phase.pack(), fluid.pack(), ns.pack()
+    xi = ctypes.byref(as_mat(x_init, "x_init")) if x_init is not None else None
+    code = lib().xc_initialize(c.ptr, ctypes.byref(ps), ctypes.byref(fs), ctypes.byref(nss),
+                               ctypes.byref(grid_struct(grid)), ctypes.byref(par_struct(par)),
+                               ctypes.byref(scales_struct(scales)), _sym(xBC, BCS, "xBC"), _sym(zBC, BCS, "zBC"),
+                               xi)
+    ns.draws = int(nss.draws)
+    c.check(code, "initialize!")
     if hst is not None:
         record_history_(hst, 0.0, scales.dt0, phase, fluid, ns, grid)
     return 0.0, scales.dt0, 0

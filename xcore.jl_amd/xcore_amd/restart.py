@@ -11,13 +11,11 @@ arrays and numbers only.  Device arrays are staged through host memory; the
 restored run continues bit-for-bit (the step is deterministic given the state).
 """
 import os
-from dataclasses import fields
 
 import numpy as np
 import torch
 
 from .arrays import to_host
-from .driver import History
 
 
 def restart_path(outdir, runID, frame=-1):
@@ -40,8 +38,8 @@ def save_checkpoint(path, phase, fluid, ns, hst, time, dt, step):
             out[f"{prefix}/{n}"] = to_host(a)
     out["noise/seed"] = np.int64(ns.seed)
     out["noise/draws"] = np.int64(ns.draws)
-    for f in fields(History):
-        out[f"hst/{f.name}"] = np.asarray(getattr(hst, f.name), dtype=np.float64)
+    for n, v in list(hst.vectors().items()) + list(hst.rates().items()):
+        out[f"hst/{n}"] = np.asarray(v, dtype=np.float64)
     out["meta/time"] = np.float64(time)
     out["meta/dt"] = np.float64(dt)
     out["meta/step"] = np.int64(step)
@@ -67,12 +65,10 @@ def load_checkpoint_(path, phase, fluid, ns, hst):
                 a.copy_(torch.from_numpy(np.ascontiguousarray(v)).to(dtype=a.dtype))
         ns.seed = int(f["noise/seed"])
         ns.draws = int(f["noise/draws"])
-        for fl in fields(History):
-            v = f[f"hst/{fl.name}"]
-            if isinstance(getattr(hst, fl.name), list):
-                setattr(hst, fl.name, [float(x) for x in v.reshape(-1)])
-            else:
-                setattr(hst, fl.name, float(v))
+        for n in hst.vectors():
+            setattr(hst, n, [float(x) for x in f[f"hst/{n}"].reshape(-1)])
+        for n in hst.rates():
+            setattr(hst, n, float(f[f"hst/{n}"]))
         return float(f["meta/time"]), float(f["meta/dt"]), int(f["meta/step"])
 
 

@@ -219,11 +219,16 @@ class GridStruct(ctypes.Structure):
 class ParStruct(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in ("rhom0", "rhox0", "d0", "g0", "etam0", "etax0")] + \
                [("AA", ctypes.c_double * 4), ("BB", ctypes.c_double * 4), ("CC", ctypes.c_double * 4)] + \
-               [(n, ctypes.c_double) for n in ("Xi", "Ptop", "alpha", "gamma", "CFL", "dtmax", "etacntr")]
+               [(n, ctypes.c_double) for n in ("Xi", "Ptop", "alpha", "gamma", "CFL", "dtmax", "etacntr")] + \
+               [(n, ctypes.c_double) for n in ("x0", "Da", "dxr", "dxg")] + \
+               [("seed", ctypes.c_uint64), ("open_sgr", ctypes.c_int)]
+
+
+SCALES_FIELDS = ("L0", "L0h", "l0", "l0h", "D0", "G0", "txi0", "bnd_w", "dt0")
 
 
 class ScalesStruct(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_double) for n in ("L0", "L0h", "l0", "l0h", "D0", "G0", "txi0")]
+    _fields_ = [(n, ctypes.c_double) for n in SCALES_FIELDS]
 
 
 def grid_struct(g):
@@ -238,11 +243,15 @@ def par_struct(p):
     s.AA = (ctypes.c_double * 4)(*map(float, p.AA))
     s.BB = (ctypes.c_double * 4)(*map(float, p.BB))
     s.CC = (ctypes.c_double * 4)(*map(float, p.CC))
+    for n in ("x0", "Da", "dxr", "dxg"):
+        setattr(s, n, float(getattr(p, n)))
+    s.seed = int(p.seed)
+    s.open_sgr = int(bool(p.open_sgr))
     return s
 
 
 def scales_struct(sc):
-    return ScalesStruct(*(float(getattr(sc, n)) for n in ("L0", "L0h", "l0", "l0h", "D0", "G0", "txi0")))
+    return ScalesStruct(*(float(getattr(sc, n)) for n in SCALES_FIELDS))
 
 
 def _bmat(t):
