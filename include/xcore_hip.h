@@ -342,6 +342,14 @@ int xc_history_sums(xc_ctx* ctx, const xc_phase* phase, const xc_fluid* fluid, d
 #define XC_HIST_NSTATS 145
 int xc_history_stats(xc_ctx* ctx, const xc_phase* phase, const xc_fluid* fluid, const xc_noise* ns,
                      double out[XC_HIST_NSTATS]);
+/* No counterpart in the reference: xc_history_stats for callers that cannot allocate (code under HIP graph
+ * capture, tight loops, harnesses that forbid hipMalloc).  Enqueues on `stream` (a hipStream_t) the same
+ * reductions into out_dev (XC_HIST_NSTATS doubles of device memory) and returns without synchronizing; no
+ * context, no allocation: the partial sums live in a fixed 114.5 KiB device array of the library (one per
+ * device, held until the process ends), whatever the grid.  Grids of more than 536,870,911 cells are
+ * refused (XC_EINVAL_SHAPE).  Calls that overlap on one device race on that array. */
+int xc_history_stats_noalloc(void* stream, const xc_phase* phase, const xc_fluid* fluid, const xc_noise* ns,
+                             double* out_dev);
 /* run.jl:71-140 initialize!(phase, fluid, ns, hst, grid, par, scales; xBC, zBC), fresh start: the
  * boundary taper and shape (run.jl:24-55), x = xin (1 + dxr rp + dxg gp) with rp the normalised white
  * noise (the library's Philox4x32-10 generator, seed par->seed, stream 2, in place of Julia's

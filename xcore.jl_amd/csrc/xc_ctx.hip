@@ -69,6 +69,13 @@ int xc_create(xc_ctx** out, int device, unsigned flags) {
         return xc_fail(nullptr, XC_EHIP, "hipStreamCreate: %s", hipGetErrorString(e));
     }
     c->stream = c->own_stream;
+    // set-up of the no-allocation entry point on this device (the address of its scratch array)
+    const int st = xc_noalloc_init_device();
+    if (st != XC_OK) {
+        hipStreamDestroy(c->own_stream);
+        free(c);
+        return st;
+    }
     *out = c;
     return XC_OK;
 }

@@ -59,6 +59,9 @@ int xc_diffus_centered_kprod(xc_ctx* ctx, xc_mat dff, const xc_mat* qx, const xc
 int xc_history_enqueue(xc_ctx* ctx, hipStream_t st, const xc_phase* ph, const xc_fluid* fl, const xc_noise* ns,
                        double* part, double* out, const char* who);
 
+// looks up the no-allocation scratch array's address on the current device (xc_noalloc.hip; called by xc_create)
+int xc_noalloc_init_device();
+
 // Learned coarse correction (xc_unet.hip), used by the Stokes preconditioner:
 // network forward into the context's own output buffer ([p][3] FP32, valid until the next call),
 // whether parameters were loaded, and the per-channel input/output scales of the trained network.

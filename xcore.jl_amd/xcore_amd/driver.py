@@ -113,6 +113,26 @@ def history_stats(phase, fluid, ns):
     return list(out)
 
 
+_lib.register("xc_history_stats_noalloc", [_P, _P, _P, _P, _P])
+
+
+def history_stats_noalloc(phase, fluid, ns, out, stream=None):
+    """xc_history_stats_noalloc: the same reductions into the device tensor ``out`` (XC_HIST_NSTATS float64)
+    on ``stream`` (torch's current one by default), without a context or any allocation; asynchronous."""
+    import torch
+    if out.dtype != torch.float64 or out.device.type != "cuda" or out.numel() < XC_HIST_NSTATS:
+        raise AssertionError("out must be a float64 device tensor of XC_HIST_NSTATS elements")
+    s = torch.cuda.current_stream() if stream is None else stream
+    ps, fs, nss = phase.pack(), fluid.pack(), ns.pack()
+    code = lib().xc_history_stats_noalloc(_P(s.cuda_stream), ctypes.byref(ps), ctypes.byref(fs),
+                                          ctypes.byref(nss), _P(out.data_ptr()))
+    if code != 0:
+        msg = lib().xc_last_error(None).decode(errors="replace")
+        if code == _lib.XC_EINVAL_SHAPE:
+            raise AssertionError(msg)
+        raise _lib.XCoreError(code, "record_history! (no-alloc): " + msg)
+
+
 def snapshot_(phase, fluid):
     """diagnose.jl:42-49 ``snapshot!`` into the context (one snapshot per context)."""
     c = context()
