@@ -288,6 +288,12 @@ def update_stage(name, ph, fl, g, par, sc, xbc="periodic", zbc="closed"):
         st = L.or_update_rheology(_R(p), _R(opar(par)))
     elif name == "kinematics":
         st = L.or_update_kinematics(_R(p), _R(f), _R(ogrid(g)))
+    elif name == "viscosity":
+        st = L.or_update_viscosity(_R(p), _R(f), _R(ogrid(g)), _R(opar(par)), _R(oscales(sc)))
+    elif name == "segregation_viscosity":
+        st = L.or_update_segregation_viscosity(_R(p), _R(f), _R(opar(par)), _R(oscales(sc)), BCS[zbc])
+    elif name == "stresses":
+        st = L.or_update_stresses(_R(p), _R(f), _R(ogrid(g)), _R(oscales(sc)))
     else:
         raise ValueError(name)
     _chk(st, name)

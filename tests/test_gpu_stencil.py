@@ -57,6 +57,27 @@ def test_advect_centered_with_flux_parity(xc, orc, scheme, shape):
                     assert np.array_equal(g, want), (nm, "not bitwise", np.abs(g - want).max())
 
 
+@pytest.mark.parametrize("scale", [1e-200, 1e30, 1e40, 1e60])
+def test_weno5_large_and_small_magnitudes(xc, orc, scale):
+    # the default WENO5 weights against the reference's (the oracle) for fields of magnitude 1e-200 .. 1e60:
+    # the reference's own weights stay finite up to |df| ~ 1e77; the default form must follow them
+    Nz, Nx = 64, 48
+    h = 1.0 / Nx
+    print(f"WENO5 at field magnitude {scale:g}", flush=True)
+    f, u, w = _inputs(Nz, Nx, 7)
+    f = np.asfortranarray(f * scale)
+    adv_o, qx_o, qz_o = orc.zeros(Nz, Nx), orc.zeros(Nz + 2, Nx + 1), orc.zeros(Nz + 1, Nx + 2)
+    orc.advect_centered(adv_o, f, u, w, h, "weno5", qx=qx_o, qz=qz_o)
+    assert np.all(np.isfinite(adv_o))
+    adv, qx, qz = xc.xcore_zeros(Nz, Nx), xc.xcore_zeros(Nz + 2, Nx + 1), xc.xcore_zeros(Nz + 1, Nx + 2)
+    xc.advect_centered_with_flux_(adv, qx, qz, xc.to_device(f), xc.to_device(u), xc.to_device(w), h, "weno5")
+    torch.cuda.synchronize()
+    for got, want, nm in ((adv, adv_o, "adv"), (qx, qx_o, "qx"), (qz, qz_o, "qz")):
+        g = xc.to_host(got)
+        assert np.all(np.isfinite(g)), nm
+        assert _rel(g, want) <= 1e-12, (nm, scale, _rel(g, want))
+
+
 @pytest.mark.parametrize("scheme", SCHEMES)
 def test_advect_centered_no_flux(xc, orc, scheme):
     Nz, Nx = 47, 90

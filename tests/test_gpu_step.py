@@ -58,7 +58,8 @@ def test_two_steps_vs_oracle(xc, orc, shape):
         src = hph if hasattr(hph, n) else hfl
         worst[n] = _rel(getattr(ph if hasattr(ph, n) else fl, n).cpu().numpy(), getattr(src, n))
     print("relative differences after 2 steps:", worst, flush=True)
-    assert max(worst.values()) <= 1e-6, worst
+    # ~10x the differences measured on an MI355X (P 1.3e-12, MFS 1.1e-12, W/U 8e-14, X 1e-15)
+    assert max(worst.values()) <= 2e-11 and worst["X"] <= 1e-14, worst
 
 
 def test_run_conservation(xc):
@@ -111,7 +112,8 @@ def test_config1_box_free_slip_vs_oracle(xc, orc):
         src = hph if hasattr(hph, n) else hfl
         worst[n] = _rel(getattr(ph if hasattr(ph, n) else fl, n).cpu().numpy(), getattr(src, n))
     print("config-1 relative differences after 3 steps:", worst, flush=True)
-    assert max(worst.values()) <= 1e-6, worst
+    # ~10x the differences measured on an MI355X (P 3.6e-9, U 1.2e-11, W 6.7e-12, X 4.6e-13)
+    assert worst["P"] <= 5e-8 and max(v for k, v in worst.items() if k != "P") <= 2e-10, worst
     # mass sums of the device state equal the oracle's
     s_dev = xc.history_sums(ph, fl)
     assert abs(s_dev[0] - float(hfl.rho.sum())) <= 1e-12 * abs(float(hfl.rho.sum()))

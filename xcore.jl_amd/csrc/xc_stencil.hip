@@ -73,7 +73,11 @@ __device__ __forceinline__ double weno5_poly(double fmm, double fm, double fc, d
 // folded into the polynomial coefficients, the nonlinear weights d_k/(b_k+eps)^2 are scaled by
 // prod_k (b_k+eps)^2 so that one quotient remains, and that quotient is a reciprocal refined by two
 // Newton-Raphson steps plus one residual correction of the product (the denominator is a positive
-// normal number: each b_k + eps >= eps).  FMA contraction allowed.
+// normal number: each b_k + eps >= eps).  FMA contraction allowed.  The (b_k + eps) are first scaled by
+// the power of two that brings the largest into [0.5, 1): the quotient is homogeneous of degree 0 in
+// them and a power-of-two scaling is exact, so the result has the same bits as unscaled wherever the
+// unscaled products stay finite -- and the products of four squares (~ |df|^8) no longer overflow for
+// fields of magnitude ~1e38 and above, up to the ~1e77 where the reference's own weights overflow.
 __device__ __forceinline__ double weno5_poly_fast(double fmm, double fm, double fc, double fp, double fpp) {
 #pragma clang fp contract(fast)
     const double e = 2.220446049250313e-16;
@@ -84,9 +88,12 @@ __device__ __forceinline__ double weno5_poly_fast(double fmm, double fm, double 
     const double s1 = fmm - 2.0 * fm + fc, t1 = fmm - 4.0 * fm + 3.0 * fc;
     const double s2 = fm - 2.0 * fc + fp, t2 = fm - fp;
     const double s3 = fc - 2.0 * fp + fpp, t3 = 3.0 * fc - 4.0 * fp + fpp;
-    const double a1 = (tt * s1) * s1 + ((0.25 * t1) * t1 + e);
-    const double a2 = (tt * s2) * s2 + ((0.25 * t2) * t2 + e);
-    const double a3 = (tt * s3) * s3 + ((0.25 * t3) * t3 + e);
+    const double b1 = (tt * s1) * s1 + ((0.25 * t1) * t1 + e);
+    const double b2 = (tt * s2) * s2 + ((0.25 * t2) * t2 + e);
+    const double b3 = (tt * s3) * s3 + ((0.25 * t3) * t3 + e);
+    const int ex = __builtin_amdgcn_frexp_exp(fmax(b1, fmax(b2, b3)));
+    const double a1 = __builtin_amdgcn_ldexp(b1, -ex), a2 = __builtin_amdgcn_ldexp(b2, -ex),
+                 a3 = __builtin_amdgcn_ldexp(b3, -ex);
     const double q1 = a1 * a1, q2 = a2 * a2, q3 = a3 * a3;
     const double q23 = q2 * q3, q13 = q1 * q3, q12 = q1 * q2;
     const double num = q23 * P1 + q13 * P2 + q12 * P3;
