@@ -146,6 +146,19 @@ typedef struct {  /* scales.jl:290-334 subset */
     double bnd_w, dt0;  /* read by initialize! only */
 } xc_scales;
 
+/* ---- characteristic scales (src/scales.jl) --------------------------------- */
+typedef struct {  /* the scalars scales.jl:88's _compute_scales_inner reads: Parameters + the grid spacing */
+    double D, d0, L0, l0, x0, Da, rhom0, rhox0, etam0, g0, Xi, h;
+} xc_scales_in;
+typedef struct {  /* scales.jl:22-62 Scales{Float64}: every field, in order */
+    double h0, D0, d0, L0, L0h, l0, l0h, bnd_w, rho0, Drho0, chi0, Dchi0, eta0, W0, w0, W0l, w0l, W0t, w0t, W0i,
+           Ri0, ReL0, Rel0, fReL0, fRel0, eII0, ke0, ks0, kx0, tW0, tw0, tk0, ti0, txi0, t0, dt0, taue0, taus0,
+           St0, xie0, xix0, xis0, G0, etae0, etat0, p0, Noe0, Nox0, Nos0, Rc0, Ra0, ReD0, Red0;
+} xc_scales_full;
+/* scales.jl:85-185 compute_scales(par, grid; tol, maxiter) for n parameter sets (ensemble members) at once;
+ * host only (no context, no device work) */
+int xc_compute_scales(int n, const xc_scales_in* in, double tol, int maxiter, xc_scales_full* out);
+
 /* ---- ensembles: many grids of any sizes advance in one launch (no counterpart in the reference;
  * each member is exactly the single-grid advect_centered[_with_flux]! / diffus_centered[_with_flux]!
  * of phsevo.jl:21-149 on its own arrays).  qx.p == NULL on every member: no fluxes. */

@@ -6,6 +6,7 @@ shipped.  Arrays are numpy float64 in Fortran (column-major) order, i.e. the
 memory layout of a Julia ``Matrix{Float64}`` with z as the fast axis.
 """
 import ctypes
+import math
 import os
 
 import numpy as np
@@ -606,3 +607,75 @@ def record_history(hst, time, dt, ph, fl, ns, g, a1=1.0, a2=1.0, a3=0.0, b1=1.0,
         else:
             hst[f"{f}_gm"].append(float(np.exp(np.mean(np.log(np.maximum(a, ep))))))
     return hst
+
+
+# ---- compute_scales (scales.jl:85-185) ----------------------------------------
+
+def compute_scales(par, grid, tol=1e-9, maxiter=200):
+    """scales.jl:85-185 ``compute_scales`` restated in Python (the checker of the library's xc_compute_scales);
+    returns a dict of every Scales field."""
+    D, d0, L0, l0 = par.D, par.d0, par.L0, par.l0
+    x0, Da, rhom0, rhox0 = par.x0, par.Da, par.rhom0, par.rhox0
+    etam0, g0, Xi, h = par.etam0, par.g0, par.Xi, grid.h
+    h0 = h
+    D0 = D / 10
+    L0h = (L0 + h0) / 2
+    l0h = (l0 + h0) / 2
+    bnd_w = l0 / 2 + D / 100
+    rho0 = rhom0
+    Drho0 = rhox0 - rhom0
+    chi0, Dchi0 = (Da, Da / 10) if Da > 0 else (x0, x0 / 10)
+    eta0 = etam0
+    W0l = Dchi0 * Drho0 * g0 * D0 ** 2 / eta0
+    w0l = Drho0 * g0 * d0 ** 2 / eta0
+    w0t = math.sqrt(Drho0 * g0 * d0 ** 2 / (l0 * rho0))
+    W0t = math.sqrt(Dchi0 * Drho0 * g0 * D0 ** 3 / (L0 ** 2 * rho0))
+    W0i = math.sqrt(Dchi0 * Drho0 * g0 * D / rho0)
+    Ri0 = W0i / W0t
+    W0, w0 = W0l, w0l
+    ReL0 = W0 * L0 / (eta0 / rho0)
+    Rel0 = w0 * l0 / (eta0 / rho0)
+    fReL0 = 1 - math.exp(-ReL0)
+    fRel0 = 1 - math.exp(-Rel0)
+    for _ in range(maxiter):
+        W0prv, w0prv = W0, w0
+        ReL0 = W0 * L0 / (eta0 / rho0)
+        Rel0 = w0 * l0 / (eta0 / rho0)
+        fReL0 = 1 - math.exp(-ReL0)
+        fRel0 = 1 - math.exp(-Rel0)
+        W0 = ((math.sqrt(4 / Ri0 ** 2 * Dchi0 * Drho0 * g0 * rho0 * fReL0 * L0 ** 2 * D0 + eta0 ** 2) - eta0)
+              * D0 / (2 * fReL0 * L0 ** 2 * rho0 / Ri0 ** 2))
+        w0 = ((math.sqrt(4 * Drho0 * g0 * rho0 * fRel0 * l0 * d0 ** 2 + eta0 ** 2) - eta0)
+              / (2 * fRel0 * l0 * rho0))
+        res = abs(W0 - W0prv) / W0 + abs(w0 - w0prv) / w0
+        if res <= tol:
+            break
+    eII0 = W0 / D0
+    ke0 = eII0 * L0 ** 2
+    ks0 = w0 * l0
+    kx0 = ks0 + fReL0 * ke0
+    tW0 = D / W0
+    tw0 = D / w0
+    tk0 = D ** 2 / kx0
+    ti0 = D / W0i
+    txi0 = rhox0 * d0 ** 2 / 18 / eta0
+    t0 = 1 / (1 / (ti0 + tW0) + 1 / tw0 + 1 / tk0)
+    dt0 = min((h0 / 2) ** 2 / kx0, (h0 / 2) / (W0 + w0))
+    taue0 = L0 / 2 / W0
+    taus0 = l0 / 2 / w0
+    St0 = txi0 / taue0
+    xie0 = Xi * math.sqrt(fReL0 * ke0 / taue0)
+    xix0 = Xi * math.sqrt(chi0 * fReL0 * ke0 / taue0 * St0 / (1 + St0 ** 2))
+    xis0 = Xi * math.sqrt(chi0 * ks0 / taus0)
+    G0 = Da * rho0 / t0 * D / D0
+    etae0 = fReL0 * ke0 * rho0
+    etat0 = fRel0 * ks0 * rho0
+    p0 = (eta0 + etae0) * eII0
+    vals = dict(h0=h0, D0=D0, d0=d0, L0=L0, L0h=L0h, l0=l0, l0h=l0h, bnd_w=bnd_w, rho0=rho0, Drho0=Drho0,
+                chi0=chi0, Dchi0=Dchi0, eta0=eta0, W0=W0, w0=w0, W0l=W0l, w0l=w0l, W0t=W0t, w0t=w0t, W0i=W0i,
+                Ri0=Ri0, ReL0=ReL0, Rel0=Rel0, fReL0=fReL0, fRel0=fRel0, eII0=eII0, ke0=ke0, ks0=ks0, kx0=kx0,
+                tW0=tW0, tw0=tw0, tk0=tk0, ti0=ti0, txi0=txi0, t0=t0, dt0=dt0, taue0=taue0, taus0=taus0,
+                St0=St0, xie0=xie0, xix0=xix0, xis0=xis0, G0=G0, etae0=etae0, etat0=etat0, p0=p0,
+                Noe0=xie0 / W0, Nox0=xix0 / w0, Nos0=xis0 / w0, Rc0=W0 / w0, Ra0=W0 * D0 / kx0,
+                ReD0=W0 * D0 / ((eta0 + etae0) / rho0), Red0=w0 * d0 / ((eta0 + etat0) / rho0))
+    return vals

@@ -12,6 +12,7 @@ from dataclasses import dataclass, field, fields
 
 import torch
 
+from . import _lib
 from ._lib import XCMat
 from .arrays import DTYPE, as_mat, xcore_zeros
 
@@ -89,73 +90,43 @@ class Scales:
         raise AttributeError(k)
 
 
+SCALES_FULL = ("h0", "D0", "d0", "L0", "L0h", "l0", "l0h", "bnd_w", "rho0", "Drho0", "chi0", "Dchi0", "eta0", "W0",
+               "w0", "W0l", "w0l", "W0t", "w0t", "W0i", "Ri0", "ReL0", "Rel0", "fReL0", "fRel0", "eII0", "ke0", "ks0",
+               "kx0", "tW0", "tw0", "tk0", "ti0", "txi0", "t0", "dt0", "taue0", "taus0", "St0", "xie0", "xix0",
+               "xis0", "G0", "etae0", "etat0", "p0", "Noe0", "Nox0", "Nos0", "Rc0", "Ra0", "ReD0", "Red0")
+SCALES_IN = ("D", "d0", "L0", "l0", "x0", "Da", "rhom0", "rhox0", "etam0", "g0", "Xi", "h")
+
+
+class ScalesIn(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in SCALES_IN]
+
+
+class ScalesFull(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in SCALES_FULL]
+
+
+_lib.register("xc_compute_scales", [ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_void_p])
+
+
+def compute_scales_batch(pars, grids, tol=1e-9, maxiter=200):
+    """scales.jl:85-185 ``compute_scales`` of several (par, grid) pairs in one library call (ensemble members)."""
+    n = len(pars)
+    ins = (ScalesIn * n)()
+    for i, (par, g) in enumerate(zip(pars, grids)):
+        for k in SCALES_IN:
+            setattr(ins[i], k, float(g.h if k == "h" else getattr(par, k)))
+    outs = (ScalesFull * n)()
+    code = _lib.lib().xc_compute_scales(n, ctypes.cast(ins, ctypes.c_void_p), float(tol), int(maxiter),
+                                        ctypes.cast(outs, ctypes.c_void_p))
+    if code != 0:
+        raise ValueError("compute_scales: " + _lib.lib().xc_last_error(None).decode(errors="replace"))
+    return [Scales({k: getattr(o, k) for k in SCALES_FULL}) for o in outs]
+
+
 def compute_scales(par, grid, tol=1e-9, maxiter=200):
-    """scales.jl:349-462 ``compute_scales(par, grid)`` -- fixed point on (W0, w0)."""
-    D, d0, L0, l0 = par.D, par.d0, par.L0, par.l0
-    x0, Da, rhom0, rhox0 = par.x0, par.Da, par.rhom0, par.rhox0
-    etam0, g0, Xi, h = par.etam0, par.g0, par.Xi, grid.h
-    h0 = h
-    D0 = D / 10
-    L0h = (L0 + h0) / 2
-    l0h = (l0 + h0) / 2
-    bnd_w = l0 / 2 + D / 100
-    rho0 = rhom0
-    Drho0 = rhox0 - rhom0
-    chi0, Dchi0 = (Da, Da / 10) if Da > 0 else (x0, x0 / 10)
-    eta0 = etam0
-    W0l = Dchi0 * Drho0 * g0 * D0 ** 2 / eta0
-    w0l = Drho0 * g0 * d0 ** 2 / eta0
-    w0t = math.sqrt(Drho0 * g0 * d0 ** 2 / (l0 * rho0))
-    W0t = math.sqrt(Dchi0 * Drho0 * g0 * D0 ** 3 / (L0 ** 2 * rho0))
-    W0i = math.sqrt(Dchi0 * Drho0 * g0 * D / rho0)
-    Ri0 = W0i / W0t
-    W0, w0 = W0l, w0l
-    ReL0 = W0 * L0 / (eta0 / rho0)
-    Rel0 = w0 * l0 / (eta0 / rho0)
-    fReL0 = 1 - math.exp(-ReL0)
-    fRel0 = 1 - math.exp(-Rel0)
-    for _ in range(maxiter):
-        W0prv, w0prv = W0, w0
-        ReL0 = W0 * L0 / (eta0 / rho0)
-        Rel0 = w0 * l0 / (eta0 / rho0)
-        fReL0 = 1 - math.exp(-ReL0)
-        fRel0 = 1 - math.exp(-Rel0)
-        W0 = ((math.sqrt(4 / Ri0 ** 2 * Dchi0 * Drho0 * g0 * rho0 * fReL0 * L0 ** 2 * D0 + eta0 ** 2) - eta0)
-              * D0 / (2 * fReL0 * L0 ** 2 * rho0 / Ri0 ** 2))
-        w0 = ((math.sqrt(4 * Drho0 * g0 * rho0 * fRel0 * l0 * d0 ** 2 + eta0 ** 2) - eta0)
-              / (2 * fRel0 * l0 * rho0))
-        res = abs(W0 - W0prv) / W0 + abs(w0 - w0prv) / w0
-        if res <= tol:
-            break
-    eII0 = W0 / D0
-    ke0 = eII0 * L0 ** 2
-    ks0 = w0 * l0
-    kx0 = ks0 + fReL0 * ke0
-    tW0 = D / W0
-    tw0 = D / w0
-    tk0 = D ** 2 / kx0
-    ti0 = D / W0i
-    txi0 = rhox0 * d0 ** 2 / 18 / eta0
-    t0 = 1 / (1 / (ti0 + tW0) + 1 / tw0 + 1 / tk0)
-    dt0 = min((h0 / 2) ** 2 / kx0, (h0 / 2) / (W0 + w0))
-    taue0 = L0 / 2 / W0
-    taus0 = l0 / 2 / w0
-    St0 = txi0 / taue0
-    xie0 = Xi * math.sqrt(fReL0 * ke0 / taue0)
-    xix0 = Xi * math.sqrt(chi0 * fReL0 * ke0 / taue0 * St0 / (1 + St0 ** 2))
-    xis0 = Xi * math.sqrt(chi0 * ks0 / taus0)
-    G0 = Da * rho0 / t0 * D / D0
-    etae0 = fReL0 * ke0 * rho0
-    etat0 = fRel0 * ks0 * rho0
-    p0 = (eta0 + etae0) * eII0
-    vals = dict(h0=h0, D0=D0, d0=d0, L0=L0, L0h=L0h, l0=l0, l0h=l0h, bnd_w=bnd_w, rho0=rho0, Drho0=Drho0,
-                chi0=chi0, Dchi0=Dchi0, eta0=eta0, W0=W0, w0=w0, W0l=W0l, w0l=w0l, W0t=W0t, w0t=w0t, W0i=W0i,
-                Ri0=Ri0, ReL0=ReL0, Rel0=Rel0, fReL0=fReL0, fRel0=fRel0, eII0=eII0, ke0=ke0, ks0=ks0, kx0=kx0,
-                tW0=tW0, tw0=tw0, tk0=tk0, ti0=ti0, txi0=txi0, t0=t0, dt0=dt0, taue0=taue0, taus0=taus0,
-                St0=St0, xie0=xie0, xix0=xix0, xis0=xis0, G0=G0, etae0=etae0, etat0=etat0, p0=p0,
-                Noe0=xie0 / W0, Nox0=xix0 / w0, Nos0=xis0 / w0, Rc0=W0 / w0, Ra0=W0 * D0 / kx0,
-                ReD0=W0 * D0 / ((eta0 + etae0) / rho0), Red0=w0 * d0 / ((eta0 + etat0) / rho0))
-    return Scales(vals)
+    """scales.jl:85-185 ``compute_scales(par, grid; tol, maxiter)`` -- fixed point on (W0, w0), computed by the
+    library (host code, no device work)."""
+    return compute_scales_batch([par], [grid], tol, maxiter)[0]
 
 
 # ---- state containers -------------------------------------------------------
