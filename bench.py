@@ -112,6 +112,7 @@ def step_bench(args, learned=False):
     if learned:
         from xcore_amd import train_lcc
         lc_level = train_lcc.load(args.weights)
+        learned = train_lcc.mode()
     with xc.autotune():  # tile shapes for the step's advection calls, chosen once before timing
         xc.run_step_(ph, fl, ns, g, par, sc, xc.StepState(state.time, state.dt, state.step),
                      xc.step_opts(par, tol=1e-8, maxit=2000, fixed_sweeps=1))
@@ -133,6 +134,7 @@ def step_bench(args, learned=False):
     info = {"sweeps_per_step": sweeps / len(log), "fgmres_iters_per_solve": its / max(sweeps, 1),
             "worst_stokes_rel_resid": max(r["stokes_worst_rel_resid"] for r in log),
             "stokes_fallbacks": sum(r["stokes_fallbacks"] for r in log),
+            "stokes_learned_steps": sum(r["stokes_learned_steps"] for r in log),
             "vcycle_levels": [min(r["stokes_mg_levels"] for r in log), max(r["stokes_mg_levels"] for r in log)],
             "inertia_ratio": [min(r["stokes_inertia_ratio"] for r in log), max(r["stokes_inertia_ratio"] for r in log)],
             "fgmres_iters_first_sweep": sum(r["stokes_iters_first"] for r in log) / len(log),
@@ -308,6 +310,40 @@ def ensemble_bench(reps=20, seed=4001):
             "launches_per_step": {"batched": 3, "per_member": 192}}
 
 
+def ensemble_steps_bench(n=64, steps=2, threads=16, seed=7):
+    """Config-4 ensemble of whole time steps: n members cycling through 128^2, 256^2 and 384^2, each with its own
+    noise seed, density contrast (+-20 %) and crystal viscosity 10^e (e uniform in 20..30), each member one full
+    step of run! per ensemble step (adaptive Picard sweeps, FGMRES to 1e-8).  The members' steps run
+    concurrently (xc_ensemble_run_step, `threads` host threads over per-member streams) and, for comparison,
+    one member after another (one thread)."""
+    import torch
+    import xcore_amd as xc
+    from xcore_amd.ensemble import ensemble_members, ensemble_run_step_
+    out = {"members": n, "sizes": "128^2, 256^2, 384^2 (cycled)", "threads": threads}
+    res = {}
+    for mode, nt in (("concurrent", threads), ("one_after_another", 1)):
+        members = ensemble_members(n, seed=seed)
+        cells = sum(m.cells for m in members)
+        opts = xc.step_opts(members[0].par, tol=1e-8, maxit=2000)
+        ensemble_run_step_(members, opts, nthreads=nt)   # warm-up step (solver hierarchies, FFT plans)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sweeps = 0
+        for _ in range(steps):
+            st = ensemble_run_step_(members, opts, nthreads=nt)
+            sweeps += sum(s["sweeps"] for s in st)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        res[mode] = dt
+        out[mode] = {"ms_per_ensemble_step": round(1e3 * dt, 2), "member_steps_per_s": round(n / dt, 1),
+                     "cell_steps_per_s": round(cells / dt, 1), "picard_sweeps_per_member_step": sweeps / (n * steps)}
+        out["cells"] = cells
+        del members
+        torch.cuda.empty_cache()
+    out["concurrent_speedup"] = round(res["one_after_another"] / res["concurrent"], 3)
+    return out
+
+
 def cpu_baseline(N, sweeps_per_step):
     """One Picard sweep of the same step on the CPU oracle (C, OpenMP; SciPy sparse LU for Stokes)."""
     import numpy as np
@@ -429,6 +465,8 @@ def main():
     if not args.no_ensemble:
         print("bench: ensemble of 64 members", file=sys.stderr, flush=True)
         out["ensemble"] = ensemble_bench()
+        print("bench: ensemble of 64 whole runs", file=sys.stderr, flush=True)
+        out["ensemble_steps"] = ensemble_steps_bench()
     if not args.no_cpu:
         print(f"bench: CPU oracle sweep at {args.cpu_n}^2", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, info["plain"]["sweeps_per_step"] if "plain" in info else 1.0)

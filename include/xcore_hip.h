@@ -235,9 +235,12 @@ typedef struct {  /* fluidmech! keywords (fluidmech.jl:71-80) and the iterative-
                                         the coarsest; 0 selects the defaults 3, 3 and, on the coarsest,
                                         20 at the 4 x 4 bottom or 2 where the cycle stopped early */
     double mg_omega;  /* damped-Jacobi weight of the multigrid smoother (0: 0.8) */
-    int learned;      /* 1: add the learned coarse correction (needs xc_unet_set_params) for a first
-                         FGMRES cycle of 6 iterations; if that has not converged the plain
-                         preconditioner continues from the iterate reached (fallback_used = 1) */
+    int learned;      /* the learned correction (needs xc_unet_set_params): 1 the network's coarse estimate e of
+                         LL^-1 v first, z = e + M (v - LL e); 2 the network's estimate of what the plain
+                         preconditioner M misses, z = M v + P N(R v).  A learned step is kept while it cuts the
+                         residual at least as much as plain steps do per unit of cost; otherwise the rest of the
+                         solve is plain (same Krylov basis; fallback_used = 1), and after 3 solves in a row
+                         dropped at their first step the next 16 solves are plain */
     int lc_level;     /* grid of the learned correction: Nz / 2^lc_level x Nx / 2^lc_level cells */
     int mg_sub;       /* pre- and post-smoothing sweeps on the levels between (0: default 1; < 0: as the
                          finest level) */
@@ -258,11 +261,12 @@ typedef struct {
     double rel_resid;      /* final ||RR - LL x|| / ||RR|| */
     double init_rel_resid; /* the same for the initial guess (previous W, U, P) */
     int mg_levels;         /* multigrid levels the velocity preconditioner's V-cycle used */
-    int fallback_used;     /* 1 if the learned coarse correction was rejected (plain multigrid used) */
+    int fallback_used;     /* 1 if the learned correction was dropped during the solve or skipped */
     double inertia_ratio;  /* a1 min(rho) h^2 / (dt max(eta)) on the finest level (0 without inertia or when
                               mg_levels is set): the
                               V-cycle stops at the first level l >= 1 with 4^l times it >= 6 (mg_levels == 0),
                               where two damped-Jacobi sweeps cut the smoothest error mode 4x */
+    int learned_steps;     /* FGMRES steps preconditioned with the learned correction */
 } xc_solve_stats;
 
 /* fluidmech.jl:71-534 fluidmech!(state, grid, par; phase, bnchm_data, sds, top_cnv, bot_cnv, open_cnv,
@@ -324,6 +328,7 @@ typedef struct {
     int stokes_iters_first;         /* FGMRES iterations of the first sweep's solve */
     double stokes_init_rel_first;   /* its initial relative residual (the previous step's solution) */
     double stokes_init_rel_later;   /* the largest initial relative residual of the later sweeps */
+    int stokes_learned_steps;       /* FGMRES steps preconditioned with the learned correction */
 } xc_step_stats;
 
 /* run.jl:204-275: one outer step of run! -- time_coefs, store_previous!, store_noise!, then the
@@ -336,6 +341,25 @@ typedef struct {
 int xc_run_step(xc_ctx* ctx, xc_phase* phase, xc_fluid* fluid, xc_noise* ns, const xc_grid* grid,
                 const xc_par* par, const xc_scales* scales, const xc_step_opts* opts, xc_step_state* state,
                 xc_step_stats* stats);
+/* Ensembles (no reference counterpart): n independent runs, each with its own context (stream, solver
+ * hierarchy), grid, parameters, scales, noise seed and state, advance one step of run! each (as
+ * xc_run_step) with their work running concurrently on the device: nthreads host threads take members in
+ * turn.  Each member's results equal its step run alone, bit for bit.  status: the member's xc_status; the
+ * function returns the first member's error (its message in that member's context), else XC_OK. */
+typedef struct {
+    xc_ctx* ctx;
+    xc_phase* phase;
+    xc_fluid* fluid;
+    xc_noise* ns;
+    const xc_grid* grid;
+    const xc_par* par;
+    const xc_scales* scales;
+    const xc_step_opts* opts;
+    xc_step_state* state;
+    xc_step_stats* stats;
+    int status;
+} xc_ensemble_member;
+int xc_ensemble_run_step(int n, xc_ensemble_member* members, int nthreads);
 /* diagnose.jl:42-49 snapshot!(res, phase, fluid): the context holds one snapshot (of the grid last
  * passed; the reference's StepResidual, diagnose.jl:13-28) */
 int xc_snapshot(xc_ctx* ctx, const xc_phase* phase, const xc_fluid* fluid);

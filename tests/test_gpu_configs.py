@@ -249,7 +249,7 @@ def test_c5_learned_512_residual_on_oracle_system(xc, orc):
     if not os.path.exists(train_lcc.WEIGHTS):
         pytest.skip("no trained weights shipped")
     level = train_lcc.load()
-    for r, stats in _sweep_residuals(xc, orc, learned=True, lc_level=level):
+    for r, stats in _sweep_residuals(xc, orc, learned=train_lcc.mode(), lc_level=level):
         assert stats["converged"] and r <= 1e-8, (r, stats)
 
 

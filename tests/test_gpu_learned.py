@@ -24,7 +24,7 @@ def test_learned_correction_end_to_end(xc, orc):
     from xcore_amd import learned as L
     from xcore_amd import train_lcc as T
     print("generating 48 training solves at 64^2", flush=True)
-    le, rr, ee, its = T.generate(48, N=64, level=2, log_every=0)
+    le, rr, ee, its = T.generate(48, N=64, level=2, log_every=0, mode=2)
     assert min(its) > 0
     inp, tgt, is_, os_ = T.normalise(le, rr, ee)
     L.set_params(L.he_init(0))
@@ -42,7 +42,7 @@ def test_learned_correction_end_to_end(xc, orc):
     res = {}
     for learned in (False, True):
         dph, dfl = to_device_state(xc, ph, "phase"), to_device_state(xc, fl, "fluid")
-        st = xc.fluidmech_(dfl, g, par, phase=dph, tol=1e-8, learned=learned, lc_level=2, **kw)
+        st = xc.fluidmech_(dfl, g, par, phase=dph, tol=1e-8, learned=2 if learned else 0, lc_level=2, **kw)
         sol = np.concatenate([xc.to_host(getattr(dfl, k)).ravel(order="F") for k in ("W", "U", "P")])
         res[learned] = (st, sol)
         print(f"learned={learned}: {st}", flush=True)

@@ -530,7 +530,7 @@ int xc_run_step(xc_ctx* ctx, xc_phase* ph, xc_fluid* fl, xc_noise* ns, const xc_
     if ((st = xc_store_previous(ctx, fl, ph))) return st;
     if ((st = xc_store_noise(ctx, ns))) return st;
     double resnorm = 1.0, resnorm0 = 1.0;
-    int iter = 0, fm_iters = 0, fm_fallbacks = 0, fm_levels = 0, fm_first = 0;
+    int iter = 0, fm_iters = 0, fm_fallbacks = 0, fm_levels = 0, fm_first = 0, fm_learned = 0;
     double fm_worst = 0.0, fm_inertia = 0.0, fm_init_first = 0.0, fm_init_later = 0.0;
     const int fixed = so->fixed_sweeps;
     xc_fm_opts fo = so->fm;
@@ -546,6 +546,7 @@ int xc_run_step(xc_ctx* ctx, xc_phase* ph, xc_fluid* fl, xc_noise* ns, const xc_
         st = xc_fluidmech(ctx, fl, ph, g, par, &fo, nullptr, &fs);
         fm_iters += fs.iters;
         fm_fallbacks += fs.fallback_used;
+        fm_learned += fs.learned_steps;
         fm_levels = fs.mg_levels;
         fm_inertia = fs.inertia_ratio;
         if (iter == 1) {
@@ -577,6 +578,7 @@ int xc_run_step(xc_ctx* ctx, xc_phase* ph, xc_fluid* fl, xc_noise* ns, const xc_
         stats->stokes_iters = fm_iters;
         stats->stokes_worst_rel_resid = fm_worst;
         stats->stokes_fallbacks = fm_fallbacks;
+        stats->stokes_learned_steps = fm_learned;
         stats->stokes_mg_levels = fm_levels;
         stats->stokes_inertia_ratio = fm_inertia;
         stats->stokes_iters_first = fm_first;

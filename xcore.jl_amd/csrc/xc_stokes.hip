@@ -1800,6 +1800,11 @@ struct StokesCtx {
     // depth of the V-cycle for the current solve (levels 0 .. leff-1) and the inertia ratio it came from
     int leff = 0;
     double inertia = 0.0;
+    // the plain preconditioner's mean residual reduction per FGMRES iteration over recent solves (0: none yet),
+    // the yardstick a learned step is measured against; learned solves dropped at their first step in a row
+    // and the number of solves still to run plain before the learned correction is tried again
+    double plain_rate = 0.0;
+    int lc_fail_streak = 0, lc_skip = 0;
 };
 
 // Decide whether the inertial Schur part applies to this solve and (re)build its eigenvalue tables.
@@ -2374,10 +2379,15 @@ int build_rhs(xc_ctx* ctx, StokesCtx* s, xc_fluid* fl, const xc_phase* ph, const
     return XC_OK;
 }
 
-struct SolveOut { int iters; bool conv; double rel, rel0; };
+struct SolveOut { int iters; bool conv; double rel, rel0; int learned_steps; bool dropped; };
+// cost of a learned preconditioner step in plain steps (restriction, network, prolongation besides the V-cycle)
+constexpr double LC_COST = 1.25;
+// a learned correction dropped at its first step in this many solves in a row is left out of the next LC_SKIP
+constexpr int LC_STREAK = 3, LC_SKIP = 16;
 
 struct LcCfg {
     bool on;
+    int mode;  // 1: z = e + M (v - LL e) with e = P N(R v); 2: z = M v + P N(R v)
     int level, f, Hc, Wc;
     float is[4], os[3];
     double eta_ref;
@@ -2394,6 +2404,7 @@ int lc_config(xc_ctx* ctx, StokesCtx* s, const xc_fm_opts* o, const xc_grid* g, 
     if (Hc % 8 || Wc % 8 || Hc < 8 || Wc < 8 || (int64_t)Hc * Wc > s->lc_cap)
         return xc_fail(ctx, XC_EINVAL_SHAPE, "fluidmech!: learned correction grid %d x %d (multiples of 8 needed)", Hc, Wc);
     lc->on = true;
+    lc->mode = o->learned == 2 ? 2 : 1;
     lc->level = lev;
     lc->f = f;
     lc->Hc = Hc;
@@ -2421,10 +2432,11 @@ void lc_restrict(xc_ctx* ctx, StokesCtx* s, const LevD& d, const LcCfg& lc, cons
     hipLaunchKernelGGL(k_lc_norm, dim3(1), dim3(1024), 0, ctx->stream, out, s->lc_part, (int)nbk, nc, sigma, compute);
 }
 
-// z = M^-1 v.  Plain: the block preconditioner.  Learned: the network's coarse estimate
+// z = M^-1 v.  Plain: the block preconditioner.  Learned, mode 1: the network's coarse estimate
 // e = P N(R v) of LL^-1 v first, then the block preconditioner on what it leaves,
 // z = e + M_plain (v - LL e); the multigrid smoothing removes the block-wise roughness
-// of the prolonged estimate.
+// of the prolonged estimate.  Mode 2: the network estimates what the block preconditioner misses,
+// z = M_plain v + P N(R v) (trained on R (LL^-1 v - M_plain v)); no extra operator application.
 int precond_full(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int64_t n, const LcCfg& lc,
                  const double* v, double* z) {
     if (!lc.on) {
@@ -2437,7 +2449,8 @@ int precond_full(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, in
     const float* e;
     int st;
     if ((st = xc_unet_infer(ctx, s->lc_in, 1, lc.Hc, lc.Wc, &e))) return st;
-    hipLaunchKernelGGL(k_zero, dim3(nb), dim3(NT), 0, ctx->stream, z, n);
+    if (lc.mode == 2) precond(ctx, s, F, v, z, mo);
+    else hipLaunchKernelGGL(k_zero, dim3(nb), dim3(NT), 0, ctx->stream, z, n);
     LcProArgs pa{};
     pa.L = dd;
     pa.f = lc.f;
@@ -2449,6 +2462,7 @@ int precond_full(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, in
     pa.z = z;
     pa.n = n;
     hipLaunchKernelGGL(k_lc_prolong, dim3(nb), dim3(NT), 0, ctx->stream, pa);
+    if (lc.mode == 2) return XC_OK;
     hipLaunchKernelGGL(k_ll_cells<true>, dim3(ll_cells_blocks(dd.Nz, dd.Nx)), dim3(NT), 0, ctx->stream, dd, F, z, v,
                        s->tmp);
     precond(ctx, s, F, s->tmp, s->rs, mo);  // rs is free inside an Arnoldi cycle
@@ -2479,6 +2493,14 @@ int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int 
     double* dd1 = s->d3;                  // step reductions (2m + 3)
     double* cf = s->d3 + 3 * (m + 2);     // update coefficients and nu
     volatile double* hout = s->host + 8;  // 2 per step: provisional |g[j+1]|, nu_{j+1}
+    // A learned step is kept only while it cuts the residual at least as much as the plain preconditioner does
+    // per iteration on average (its measured rate), raised to the learned step's relative cost; the first step
+    // that does not switches the rest of the solve to the plain preconditioner.  The Krylov basis is kept:
+    // FGMRES allows a different preconditioner in every step.
+    LcCfg cur = lc;
+    const double lc_bar = pow(s->plain_rate > 0 ? s->plain_rate : 0.2, LC_COST);
+    int lsteps = 0;
+    bool dropped = false;
     while (true) {
         {  // r = b - LL x and ||r||^2 (per-wave partials, their sum written to the mapped host buffer)
             const unsigned nbl = ll_cells_blocks(dd.Nz, dd.Nx);
@@ -2500,7 +2522,7 @@ int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int 
         for (; j < m; ++j) {
             double* vj = s->V + (int64_t)j * n;  // p_j, provisional until this step's update
             double* zj = s->Z + (int64_t)j * n;
-            if ((st = precond_full(ctx, s, F, mo, n, lc, vj, zj))) return st;
+            if ((st = precond_full(ctx, s, F, mo, n, cur, vj, zj))) return st;
             hipLaunchKernelGGL(k_ll_cells<false>, dim3(ll_cells_blocks(dd.Nz, dd.Nx)), dim3(NT), 0, ctx->stream, dd, F,
                                zj, zj, s->ws);
             {
@@ -2517,6 +2539,15 @@ int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int 
             hipLaunchKernelGGL(k_dcgs_update, dim3(nb), dim3(NT), 0, ctx->stream, s->V, n, j, dd1, cf, vj, s->ws,
                                s->V + (int64_t)(j + 1) * n);
             XC_TRY_HIP(ctx, hipEventRecord(s->ev[j & 1], ctx->stream));
+            if (cur.on) {  // judge the learned step now (the pipeline waits for it)
+                XC_TRY_HIP(ctx, hipEventSynchronize(s->ev[j & 1]));
+                const double before = j == 0 ? beta : hout[2 * (j - 1)];
+                ++lsteps;
+                if (!(hout[2 * j] <= before * lc_bar)) {
+                    cur.on = false;
+                    dropped = true;
+                }
+            }
             if (j == 0) continue;
             // step j-1 is complete once its event has fired; step j (enqueued) finalises its column
             XC_TRY_HIP(ctx, hipEventSynchronize(s->ev[(j - 1) & 1]));
@@ -2537,6 +2568,8 @@ int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int 
     out->conv = conv;
     out->rel = rel;
     out->rel0 = rel0;
+    out->learned_steps = lsteps;
+    out->dropped = dropped;
     return XC_OK;
 }
 
@@ -2624,24 +2657,41 @@ int xc_fluidmech(xc_ctx* ctx, xc_fluid* fl, const xc_phase* ph, const xc_grid* g
     bool conv = false;
     LcCfg lc{};
     if ((st = lc_config(ctx, s, o, g, par, &lc))) return st;
-    SolveOut so{};
-    // A learned attempt gets one short cycle (LC_PROBE iterations); if that has not converged the
-    // plain preconditioner continues from the iterate reached (FGMRES minimises the residual, so the
-    // learned directions cannot leave it worse than the start).
-    constexpr int LC_PROBE = 6;
-    const int lmax = lc.on ? (maxit < LC_PROBE ? maxit : LC_PROBE) : maxit;
-    if ((st = fgmres_run(ctx, s, F, mo, m, n, s->bs, s->xs, target, bnorm, lmax, lc, &so))) return st;
     int fallback = 0;
-    if (lc.on && !so.conv) {
-        fallback = 1;
+    if (lc.on && s->lc_skip > 0) {  // recently useless on these systems: plain this time
+        --s->lc_skip;
         lc.on = false;
+        fallback = 1;
+    }
+    SolveOut so{};
+    if ((st = fgmres_run(ctx, s, F, mo, m, n, s->bs, s->xs, target, bnorm, maxit, lc, &so))) return st;
+    if (lc.on && !so.conv && !(so.rel < 1e30)) {  // non-finite iterate: once more, plain, from the initial guess
         const int used = so.iters;
         const double rel0_l = so.rel0;
-        if (!(so.rel < 1e30))  // non-finite iterate: restart from the initial guess
-            hipLaunchKernelGGL(k_gather, dim3(nb), dim3(NT), 0, ctx->stream, fl->W, fl->U, fl->P, s->xs, Nz, Nx);
+        lc.on = false;
+        so.dropped = true;
+        hipLaunchKernelGGL(k_gather, dim3(nb), dim3(NT), 0, ctx->stream, fl->W, fl->U, fl->P, s->xs, Nz, Nx);
+        const int ls = so.learned_steps;
         if ((st = fgmres_run(ctx, s, F, mo, m, n, s->bs, s->xs, target, bnorm, maxit - used, lc, &so))) return st;
         so.iters += used;
         so.rel0 = rel0_l;
+        so.learned_steps = ls;
+        so.dropped = true;
+    }
+    if (lc.on) {
+        fallback |= so.dropped ? 1 : 0;
+        if (so.dropped && so.learned_steps <= 1) {
+            if (++s->lc_fail_streak >= LC_STREAK) {
+                s->lc_skip = LC_SKIP;
+                s->lc_fail_streak = LC_STREAK - 1;
+            }
+        } else {
+            s->lc_fail_streak = 0;
+        }
+    } else if (so.conv && so.iters > 0 && so.rel > 0 && so.rel0 > so.rel) {
+        // the plain preconditioner's measured reduction per iteration (exponential average over solves)
+        const double r = pow(so.rel / so.rel0, 1.0 / so.iters);
+        s->plain_rate = s->plain_rate > 0 ? 0.7 * s->plain_rate + 0.3 * r : r;
     }
     iters = so.iters;
     conv = so.conv;
@@ -2656,6 +2706,7 @@ int xc_fluidmech(xc_ctx* ctx, xc_fluid* fl, const xc_phase* ph, const xc_grid* g
         stats->mg_levels = s->leff;
         stats->inertia_ratio = s->inertia;
         stats->fallback_used = fallback;
+        stats->learned_steps = so.learned_steps;
     }
     if ((st = xc_finish(ctx, "fluidmech!"))) return st;
     if (!conv) return xc_fail(ctx, XC_ENOCONV, "fluidmech!: FGMRES reached %d iterations at relative residual %.3e",
@@ -2721,7 +2772,8 @@ int xc_stokes_solve(xc_ctx* ctx, const xc_fluid* fl, const xc_grid* g, const xc_
         stats->init_rel_resid = so.rel0;
         stats->mg_levels = s->leff;
         stats->inertia_ratio = s->inertia;
-        stats->fallback_used = 0;
+        stats->fallback_used = so.dropped ? 1 : 0;
+        stats->learned_steps = so.learned_steps;
     }
     if ((st = xc_finish(ctx, "stokes_solve"))) return st;
     if (!so.conv) return xc_fail(ctx, XC_ENOCONV, "stokes_solve: FGMRES reached %d iterations at relative residual %.3e",

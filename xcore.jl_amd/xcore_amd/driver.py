@@ -34,7 +34,8 @@ class StepState(ctypes.Structure):
 class StepStats(ctypes.Structure):
     _fields_ = [("sweeps", _I), ("resnorm", _D), ("stokes_iters", _I), ("stokes_worst_rel_resid", _D),
                 ("stokes_fallbacks", _I), ("stokes_mg_levels", _I), ("stokes_inertia_ratio", _D),
-                ("stokes_iters_first", _I), ("stokes_init_rel_first", _D), ("stokes_init_rel_later", _D)]
+                ("stokes_iters_first", _I), ("stokes_init_rel_first", _D), ("stokes_init_rel_later", _D),
+                ("stokes_learned_steps", _I)]
 
 
 # history.jl:184-221: the statistics record_history! keeps, in the order of xc_history_stats
@@ -195,13 +196,13 @@ def record_history_(hst, time, dt, phase, fluid, ns, grid, a1=1.0, a2=1.0, a3=0.
             getattr(hst, f"{f}_gm").append(math.exp(sl / n))
 
 
-def initialize_(phase, fluid, ns, hst, grid, par, scales, xBC="periodic", zBC="closed", x_init=None):
+def initialize_(phase, fluid, ns, hst, grid, par, scales, xBC="periodic", zBC="closed", x_init=None, ctx=None):
     """run.jl:71-140 ``initialize!`` (fresh start; a restart loads a checkpoint with ``load_checkpoint_``).
 
     The random perturbation ``rp`` comes from the library's generator (seed par.seed, stream 2) in place of
     Julia's MersenneTwister.  ``x_init`` (a device matrix) replaces the perturbed profile when given.
     """
-    c = context()
+    c = ctx or context()
     ps, fs, nss = phase.pack(), fluid.pack(), ns.pack()
     xi = ctypes.byref(as_mat(x_init, "x_init")) if x_init is not None else None
     code = lib().xc_initialize(c.ptr, ctypes.byref(ps), ctypes.byref(fs), ctypes.byref(nss),
@@ -210,14 +211,16 @@ def initialize_(phase, fluid, ns, hst, grid, par, scales, xBC="periodic", zBC="c
                                xi)
     ns.draws = int(nss.draws)
     c.check(code, "initialize!")
+    if ctx is not None:
+        ctx.sync()
     if hst is not None:
         record_history_(hst, 0.0, scales.dt0, phase, fluid, ns, grid)
     return 0.0, scales.dt0, 0
 
 
-def run_step_(phase, fluid, ns, grid, par, scales, state, opts):
+def run_step_(phase, fluid, ns, grid, par, scales, state, opts, ctx=None):
     """One outer step of run! (run.jl:205-274) on the device; ``state`` is a StepState."""
-    c = context()
+    c = ctx or context()
     ps, fs, nss = phase.pack(), fluid.pack(), ns.pack()
     st = StepStats()
     code = lib().xc_run_step(c.ptr, ctypes.byref(ps), ctypes.byref(fs), ctypes.byref(nss),

@@ -102,3 +102,88 @@ class BatchPlan:
             self.close()
         except Exception:
             pass
+
+
+# ---- ensembles of whole runs: every member advances one step of run! per call -------------------------
+
+class EnsembleMemberStruct(ctypes.Structure):
+    _fields_ = [("ctx", _P), ("phase", _P), ("fluid", _P), ("ns", _P), ("grid", _P), ("par", _P), ("scales", _P),
+                ("opts", _P), ("state", _P), ("stats", _P), ("status", _I)]
+
+
+_lib.register("xc_ensemble_run_step", [_I, _P, _I])
+
+
+class RunMember:
+    """One run of an ensemble: its own context (stream, solver hierarchy), grid, parameters, scales, noise
+    seed and states, set up by initialize! on its context."""
+
+    def __init__(self, par, grid, scales, seed, x_init=None, device=None):
+        import torch
+        from .driver import StepState
+        from .state import FluidState, NoiseState, PhaseState
+        self.par, self.grid, self.scales = par, grid, scales
+        dev = torch.cuda.current_device() if device is None else int(device)
+        self.ctx = _lib.Context(dev, 0)   # its own stream
+        self.phase, self.fluid = PhaseState(grid.Nz, grid.Nx), FluidState(grid.Nz, grid.Nx)
+        self.ns = NoiseState(grid, scales, seed=seed)
+        torch.cuda.synchronize()          # the zero-filled states are written on torch's stream
+        from .driver import initialize_
+        t, dt, step = initialize_(self.phase, self.fluid, self.ns, None, grid, par, scales, x_init=x_init,
+                                  ctx=self.ctx)
+        self.state = StepState(t, dt, step)
+
+    @property
+    def cells(self):
+        return self.grid.Nz * self.grid.Nx
+
+
+def ensemble_members(n, sizes=(128, 256, 384), seed=0, base=None, drho_span=0.2, log10_etax0=(20.0, 30.0)):
+    """n members cycling through the grid sizes, each with its own noise seed, crystal-melt density contrast
+    drawn uniformly within +-drho_span of base's and crystal viscosity etax0 = 10^e, e uniform in log10_etax0
+    (parameters.jl:50-57); scales of all members from one compute_scales call."""
+    import dataclasses
+
+    import numpy as np
+    from .state import Grid, Parameters, compute_scales_batch
+    base = base or Parameters()
+    rng = np.random.default_rng(seed)
+    pars = []
+    for i in range(n):
+        N = sizes[i % len(sizes)]
+        f = rng.uniform(1.0 - drho_span, 1.0 + drho_span)
+        e = rng.uniform(*log10_etax0)
+        pars.append(dataclasses.replace(base, N=N, L=base.D, rhox0=base.rhom0 + (base.rhox0 - base.rhom0) * f,
+                                        etax0=10.0 ** e, seed=int(seed * 1000 + i)))
+    grids = [Grid.from_par(p) for p in pars]
+    scales = compute_scales_batch(pars, grids)
+    return [RunMember(p, g, s, seed=p.seed) for p, g, s in zip(pars, grids, scales)]
+
+
+def ensemble_run_step_(members, opts, nthreads=16):
+    """One step of run! for every member (xc_ensemble_run_step): the members' work runs concurrently on the
+    device.  opts: one StepOpts for all members or a list.  Returns the members' step statistics."""
+    from .driver import StepStats
+    from .state import grid_struct, par_struct, scales_struct
+    n = len(members)
+    keep = []
+    arr = (EnsembleMemberStruct * n)()
+    stats = [StepStats() for _ in range(n)]
+    nss = []
+    for i, m in enumerate(members):
+        o = opts[i] if isinstance(opts, (list, tuple)) else opts
+        ps, fs, ns = m.phase.pack(), m.fluid.pack(), m.ns.pack()
+        gs, pr, sc = grid_struct(m.grid), par_struct(m.par), scales_struct(m.scales)
+        keep += [ps, fs, ns, gs, pr, sc, o]
+        nss.append(ns)
+        arr[i] = EnsembleMemberStruct(m.ctx.ptr, ctypes.addressof(ps), ctypes.addressof(fs), ctypes.addressof(ns),
+                                      ctypes.addressof(gs), ctypes.addressof(pr), ctypes.addressof(sc),
+                                      ctypes.addressof(o), ctypes.addressof(m.state), ctypes.addressof(stats[i]), 0)
+    code = lib().xc_ensemble_run_step(n, ctypes.cast(arr, _P), int(nthreads))
+    for m, ns in zip(members, nss):
+        m.ns.draws = int(ns.draws)
+    if code != 0:
+        bad = next(i for i in range(n) if arr[i].status != 0)
+        members[bad].ctx.check(arr[bad].status, f"ensemble member {bad}")
+    del keep
+    return [{k: getattr(s, k) for k, _ in StepStats._fields_} for s in stats]

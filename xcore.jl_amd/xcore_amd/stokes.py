@@ -33,7 +33,7 @@ class BnchmStruct(ctypes.Structure):
 
 class SolveStats(ctypes.Structure):
     _fields_ = [("iters", _I), ("converged", _I), ("rel_resid", _D), ("init_rel_resid", _D), ("mg_levels", _I),
-                ("fallback_used", _I), ("inertia_ratio", _D)]
+                ("fallback_used", _I), ("inertia_ratio", _D), ("learned_steps", _I)]
 
 
 _lib.register("xc_fluidmech", [_P, _P, _P, _P, _P, _P, _P, _P])
@@ -65,7 +65,7 @@ def fm_opts(par, sds=-1, top_cnv=1, bot_cnv=1, open_cnv=False, xBC="periodic", z
     return FmOpts(int(sds), int(top_cnv), int(bot_cnv), int(bool(open_cnv)), _sym(xBC, BCS, "xBC"),
                   _sym(zBC, BCS, "zBC"), _sym(par.ADVN, SCHEMES, "scheme"), float(dt), float(a1), float(a2),
                   float(a3), float(b1), float(b2), float(b3), float(tol), int(maxit), int(restart), int(mg_pre),
-                  int(mg_post), int(mg_coarse), float(mg_omega), int(bool(learned)), int(lc_level), int(mg_sub),
+                  int(mg_post), int(mg_coarse), float(mg_omega), int(learned), int(lc_level), int(mg_sub),
                   int(mg_levels), int(bool(mg_fp64)))
 
 

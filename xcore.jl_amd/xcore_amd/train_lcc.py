@@ -24,7 +24,7 @@ import torch
 from . import learned as L
 from .fields import compute_fft_filter, normal_fill_
 from .state import FluidState, Grid, Parameters, compute_scales
-from .stokes import krylov_vector, lc_restrict, stokes_solve, stokes_vector_size
+from .stokes import krylov_vector, lc_restrict, stokes_precond, stokes_solve, stokes_vector_size
 
 WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "lcc_weights.npz")
 
@@ -107,10 +107,11 @@ def zero_mean_pressure(x, g):
     return y
 
 
-def generate(samples, N=256, level=2, seed=0, per_system=12, tol=1e-8, log_every=500, maxit=1000):
+def generate(samples, N=256, level=2, seed=0, per_system=12, tol=1e-8, log_every=500, maxit=1000, mode=2):
     """Training pairs from the library's own solves: for random lognormal-viscosity systems driven by
     buoyancy, the directions FGMRES feeds its preconditioner (the Krylov basis of a plain solve) and
-    their exact corrections e = LL^-1 v (plain FGMRES); returned as coarse-grid block averages."""
+    their exact corrections e = LL^-1 v (plain FGMRES) -- for mode 2 what the plain block preconditioner
+    M misses, e - M v; returned as coarse-grid block averages."""
     L.set_io_scales([1, 1, 1, 1], [1, 1, 1])
     f = 1 << level
     Hc, Wc = N // f, N // f
@@ -138,6 +139,8 @@ def generate(samples, N=256, level=2, seed=0, per_system=12, tol=1e-8, log_every
                 keep[s] = False
                 s += 1
                 continue
+            if mode == 2:
+                e = e - stokes_precond(fl, g, par, v, **kw)
             e = zero_mean_pressure(e, g)
             a, sg = lc_restrict(fl, g, par, level, v)
             b, _ = lc_restrict(fl, g, par, level, e, sigma=sg)
@@ -182,17 +185,27 @@ def train(inp, tgt, steps=2000, batch=32, lr=1e-3, seed=0, log_every=250):
     return hist
 
 
-def save(path, in_scale, out_scale, level, meta=None):
+def save(path, in_scale, out_scale, level, meta=None, mode=2):
     np.savez(path, theta=L.get_params(), in_scale=in_scale, out_scale=out_scale, level=np.int32(level),
-             meta=np.array(str(meta or {})))
+             mode=np.int32(mode), meta=np.array(str(meta or {})))
+
+
+_MODE = [1]
 
 
 def load(path=WEIGHTS):
-    """Load trained weights into the device context; returns the correction level."""
+    """Load trained weights into the device context; returns the correction level (the form the network was
+    trained for -- the fm_opts `learned` value -- is then ``mode()``)."""
     d = np.load(path, allow_pickle=False)
     L.set_params(d["theta"])
     L.set_io_scales(d["in_scale"], d["out_scale"])
+    _MODE[0] = int(d["mode"]) if "mode" in d.files else 1
     return int(d["level"])
+
+
+def mode():
+    """The learned form of the loaded weights: 1 coarse estimate of LL^-1 v, 2 estimate of what M misses."""
+    return _MODE[0]
 
 
 def main():
@@ -204,11 +217,12 @@ def main():
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--out", default=WEIGHTS)
+    ap.add_argument("--mode", type=int, default=2)
     a = ap.parse_args()
     torch.cuda.set_device(0)
     print(f"generating {a.samples} Stokes solves at {a.n}^2", flush=True)
     t0 = time.time()
-    le, rr, ee, its = generate(a.samples, a.n, a.level)
+    le, rr, ee, its = generate(a.samples, a.n, a.level, mode=a.mode)
     tg = time.time() - t0
     inp, tgt, is_, os_ = normalise(le, rr, ee)
     n_val = max(1, a.samples // 20)
@@ -220,10 +234,10 @@ def main():
     for i in range(0, n_val, 16):
         l, _ = L.loss_grad(inp[i:i + 16].contiguous(), tgt[i:i + 16].contiguous())
         vl.append(l)
-    meta = dict(samples=a.samples, n=a.n, level=a.level, steps=a.steps, gen_s=round(tg, 1), train_s=round(tt, 1),
+    meta = dict(samples=a.samples, n=a.n, level=a.level, mode=a.mode, steps=a.steps, gen_s=round(tg, 1), train_s=round(tt, 1),
                 mean_fgmres_iters=float(np.mean(its)), final_train_loss=hist[-1][1], val_loss=float(np.mean(vl)))
     print("result", meta, flush=True)
-    save(a.out, is_, os_, a.level, meta)
+    save(a.out, is_, os_, a.level, meta, mode=a.mode)
 
 
 if __name__ == "__main__":
