@@ -26,8 +26,11 @@ The ensemble section (64 members of 128^2 / 256^2 / 384^2) times one batched lau
 operator -- replayed from a HIP graph -- against one launch per member and operator.
 
 `cpu_baseline` times the CPU oracle (the C restatement of the reference with
-SciPy's sparse LU for the Stokes solve) on one Picard sweep of the same step
-at 256^2 and converts it to steps/s with the mean sweep count of the GPU run.
+SciPy's sparse LU for the Stokes solve) on one Picard sweep of the same 512^2
+step (~2.5 minutes, most of it the sparse LU) and converts it to steps/s with
+the mean sweep count of the GPU run.  Config 1 (100^2, 20 steps) runs on both
+the device and the oracle, the 2048^2 stencil calls on the oracle too, and
+noise! at 1024^2 is timed as the OU microbench.
 """
 import argparse
 import json
@@ -120,13 +123,20 @@ def step_bench(args, learned=False):
     opts = xc.step_opts(par, tol=1e-8, maxit=2000, mg_pre=args.mg_pre, mg_post=args.mg_post, mg_coarse=args.mg_coarse,
                         mg_sub=args.mg_sub, mg_levels=args.mg_levels, mg_omega=args.mg_omega, learned=learned,
                         lc_level=lc_level)
+    def one_step():
+        # run.jl:205-275: the step and its record_history! (nrh = 1, parameters.jl:18)
+        dt_step = state.dt
+        st = xc.run_step_(ph, fl, ns, g, par, sc, state, opts)
+        xc.record_history_(hst, state.time, dt_step, ph, fl, ns, g, *xc.time_coefs(par.TINT, state.step))
+        return st
+
     for _ in range(args.warmup):
-        xc.run_step_(ph, fl, ns, g, par, sc, state, opts)
+        one_step()
     torch.cuda.synchronize()
     log = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        log.append(xc.run_step_(ph, fl, ns, g, par, sc, state, opts))
+        log.append(one_step())
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     sweeps = sum(r["sweeps"] for r in log)
@@ -141,6 +151,7 @@ def step_bench(args, learned=False):
             "init_rel_resid_first_sweep": max(r["stokes_init_rel_first"] for r in log),
             "init_rel_resid_later_sweeps": max(r["stokes_init_rel_later"] for r in log),
             "sim_time": state.time, "dt": state.dt, "step": int(state.step),
+            "history_EB_max": max(abs(e) for e in hst.EB), "history_records": len(hst.EB),
             "x_finite": bool(torch.isfinite(ph.x).all())}
     return 1e3 * elapsed / args.steps, info
 
@@ -228,11 +239,33 @@ def stencil_bench(N, steps, warmup):
         e1.record(stream)
         e1.synchronize()
         floors[kind] = 1e3 * e0.elapsed_time(e1) / reps
+    # the best case of the same traffic: N x N streams (3 + 3, 2 + 3, and a copy), every access an aligned
+    # 16-byte one and no output shifted a row -- what HBM sustains when every line is written whole
+    al = [[xc.xcore_zeros(N, N) for _ in range(6)] for _ in range(NFIELDS)]
+    best = {}
+    for kind, nr, nw in (("advect", 3, 3), ("diffus", 2, 3), ("copy", 1, 1)):
+        nts = 1 | 2
+        for a6 in al:
+            xc.diag_stream_(a6[:nr], a6[3:3 + nw], nts)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for r in range(reps):
+            a6 = al[r % NFIELDS]
+            xc.diag_stream_(a6[:nr], a6[3:3 + nw], nts)
+        e1.record(stream)
+        e1.synchronize()
+        best[kind] = (1e3 * e0.elapsed_time(e1) / reps, 8 * N * N * (nr + nw))
+    del al
     for name, kind in (("weno5_advect_with_flux", "advect"), ("quick_advect_with_flux", "advect"),
                        ("diffus_with_flux", "diffus")):
         o = out[name]
         o["stream_floor_us"] = round(floors[kind], 2)
         o["frac_of_stream_floor"] = round(floors[kind] / o["avg_launch_us"], 4)
+        us_b, by_b = best[kind]
+        o["aligned_stream_GBs"] = round(by_b / (us_b * 1e-6) / 1e9, 1)
+        o["frac_of_aligned_stream"] = round(o["GB/s"] / o["aligned_stream_GBs"], 4)
+    out["aligned_copy_GBs"] = round(best["copy"][1] / (best["copy"][0] * 1e-6) / 1e9, 1)
     wk = out["weno5_advect_with_flux"]
     roofline = {"bound": "hbm", "achieved": wk["GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": wk["hbm_frac"], "traffic": TRAFFIC_WENO5_2048,
@@ -241,7 +274,9 @@ def stencil_bench(N, steps, warmup):
                 "bytes_per_launch": adv_bytes, "avg_launch_us": wk["avg_launch_us"], "launches_timed": reps,
                 "same_traffic_stream_us": wk["stream_floor_us"],
                 "same_traffic_stream_GBs": round(adv_bytes / (wk["stream_floor_us"] * 1e-6) / 1e9, 1),
-                "frac_of_same_traffic_stream": wk["frac_of_stream_floor"]}
+                "frac_of_same_traffic_stream": wk["frac_of_stream_floor"],
+                "aligned_stream_GBs": wk["aligned_stream_GBs"], "frac_of_aligned_stream": wk["frac_of_aligned_stream"],
+                "aligned_copy_GBs": out["aligned_copy_GBs"]}
     return roofline, out
 
 
@@ -344,6 +379,148 @@ def ensemble_steps_bench(n=64, steps=2, threads=16, seed=7):
     return out
 
 
+def _cores():
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except Exception:
+        cores = os.cpu_count() or 1
+    return max(1, min(cores, int(os.environ.get("OMP_NUM_THREADS", cores))))
+
+
+def c1_bench(steps=20):
+    """Config 1, the reference's CPU-runnable case: 10 km box, 100 x 100, free-slip walls, x ~ U(0.04, 0.06),
+    WENO5, CFL 0.5, 20 steps with the adaptive Picard loop -- on the device (FGMRES to 1e-8) and on the C
+    oracle (sparse LU), from the same initial state; steps/s of each and the conservation errors."""
+    import numpy as np
+    import torch
+    import xcore_amd as xc
+    from oracle import oracle as orc
+    N = 100
+    par = xc.Parameters(D=1e4, N=N, L=1e4, CFL=0.5, Da=0.05, x0=0.05)
+    g = xc.Grid.from_par(par)
+    sc = xc.compute_scales(par, g)
+    ph, fl = xc.PhaseState(N, N), xc.FluidState(N, N)
+    ns = xc.NoiseState(g, sc, seed=5)
+    x_init = np.asfortranarray(np.random.default_rng(2024).uniform(0.04, 0.06, (N, N)))
+    hst = xc.History()
+    t, dt, step = xc.initialize_(ph, fl, ns, hst, g, par, sc, x_init=xc.to_device(x_init))
+    hosts = []
+    for kind, dev in (("phase", ph), ("fluid", fl), ("noise", ns)):
+        h = (orc.phase_state(N, N) if kind == "phase" else orc.fluid_state(N, N) if kind == "fluid" else
+             orc.HostNoise(N, N, ns.padL0, ns.padl0, ns.fL, ns.fl))
+        for n, a in h.arrays().items():
+            a[:] = getattr(dev, n).cpu().numpy()
+        hosts.append(h)
+    st_h = dict(time=t, dt=dt, step=step, draws=ns.draws)
+    state = xc.StepState(t, dt, step)
+    opts = xc.step_opts(par, top_cnv=-1, bot_cnv=-1, tol=1e-8, maxit=2000)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sweeps_d = 0
+    for _ in range(steps):
+        dt_step = state.dt
+        r = xc.run_step_(ph, fl, ns, g, par, sc, state, opts)
+        xc.record_history_(hst, state.time, dt_step, ph, fl, ns, g, *xc.time_coefs(par.TINT, state.step))
+        sweeps_d += r["sweeps"]
+    torch.cuda.synchronize()
+    t_gpu = time.perf_counter() - t0
+    cores = _cores()
+    orc.set_threads(cores)
+    hph, hfl, hns = hosts
+    t0 = time.perf_counter()
+    sweeps_h = 0
+    for _ in range(steps):
+        sweeps_h += orc.run_step(hph, hfl, hns, g, par, sc, st_h, top_cnv=-1, bot_cnv=-1, seed=5)
+    t_cpu = time.perf_counter() - t0
+    x_d = ph.x.cpu().numpy()
+    return {"workload": "config 1: 100^2, 10 km box, free slip, 20 steps, adaptive Picard",
+            "gpu_steps_per_s": round(steps / t_gpu, 2), "gpu_picard_sweeps": sweeps_d,
+            "cpu_steps_per_s": round(steps / t_cpu, 3), "cpu_picard_sweeps": sweeps_h, "cpu_cores": cores,
+            "cpu_kind": "port (C oracle, OpenMP; SciPy SuperLU for the Stokes solve)",
+            "gpu_over_cpu": round(t_cpu / t_gpu, 2), "EB_max_gpu": max(abs(e) for e in hst.EB),
+            "x_rel_diff_after_20_steps": float(np.linalg.norm(x_d - hph.x) / np.linalg.norm(hph.x))}
+
+
+def stencil_cpu_bench(N, reps=10):
+    """The 2048^2 stencil calls on the C oracle (OpenMP over the host cores): median seconds per call."""
+    import numpy as np
+    from oracle import oracle as orc
+    fields, u, w, k, h = make_inputs(N)
+    f = fields[0]
+    cores = _cores()
+    orc.set_threads(cores)
+    out = {"cores": cores, "kind": "port (C oracle)", "grid": [N, N], "calls": reps}
+    bufs = (orc.zeros(N, N), orc.zeros(N + 2, N + 1), orc.zeros(N + 1, N + 2))
+    for name, fn in (("weno5_with_flux", lambda: orc.advect_centered(bufs[0], f, u, w, h, "weno5", qx=bufs[1],
+                                                                     qz=bufs[2])),
+                     ("quick_with_flux", lambda: orc.advect_centered(bufs[0], f, u, w, h, "quick", qx=bufs[1],
+                                                                     qz=bufs[2])),
+                     ("diffus_with_flux", lambda: orc.diffus_centered(bufs[0], f, k, h, qx=bufs[1], qz=bufs[2]))):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        out[name + "_ms"] = round(1e3 * float(np.median(ts)), 2)
+    return out
+
+
+def ou_setup(N=1024, seed=4001):
+    """noise! at N^2 with L0h = l0h = 4h (pad 16: FFT length N + 16) and a correlation time of 10 dt after
+    noise.jl:171-172's update (pre-update tau = L0h / (2 V) = 9 / 0.91 dt)."""
+    import torch
+    import xcore_amd as xc
+    D = 1.0
+    h = D / N
+    par = xc.Parameters(D=D, N=N, L=D, L0=7 * h, l0=7 * h, x0=0.05, Da=0.0)
+    g = xc.Grid.from_par(par)
+    sc = xc.compute_scales(par, g)
+    assert abs(sc.L0h - 4 * h) < 1e-15 * D and 4 * math.ceil(sc.L0h / h) == 16
+    dt = 1e-3
+    tau0 = dt / (1.0 / 9.0 - 0.01)
+    ph = xc.PhaseState(N, N)
+    ns = xc.NoiseState(g, sc, seed=seed)
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    ph.V.fill_(sc.L0h / (2 * tau0))
+    ph.vx.fill_(sc.l0h / (2 * tau0))
+    ph.fReL.fill_(1.0)
+    ph.ke.fill_(1e-3)
+    ph.ks.fill_(1e-4)
+    ph.chi.copy_(0.05 + 0.01 * torch.rand((N, N), generator=gen, device="cuda", dtype=torch.float64))
+    for n in ("psieo", "psixo", "psiso"):
+        getattr(ns, n).copy_(1e-3 * torch.randn((N, N), generator=gen, device="cuda", dtype=torch.float64))
+    ph.bndtaperw.fill_(1.0)
+    return par, g, sc, ph, ns, dt
+
+
+def ou_bench(N=1024, reps=20):
+    """noise! (OU update of three potentials, their Gaussian filters with FFT length 1040 x 1024, the noise fluxes
+    and magnitudes) at 1024^2 with fresh white noise every call: time per call from HIP events on the launch
+    stream; algorithmic bytes = every array it reads or writes once."""
+    import torch
+    import xcore_amd as xc
+    par, g, sc, ph, ns, dt = ou_setup(N)
+    xc.noise_(ns, ph, g, par, sc, dt, first_iter=True)
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(s)
+    for _ in range(reps):
+        xc.noise_(ns, ph, g, par, sc, dt, first_iter=True)
+    e1.record(s)
+    torch.cuda.synchronize()
+    us = 1e3 * e0.elapsed_time(e1) / reps
+    cell, wt, ut = N * N, (N + 1) * (N + 2), (N + 2) * (N + 1)
+    # read: psi*o (3), V, vx, fReL, ke, chi, ks (cells), chiw, muw (W faces), chiu, muu (U faces);
+    # written: re, rs, psi* (3), xie/xix/xis (cells), 3 W-face and 3 U-face fluxes
+    nbytes = 8 * (cell * (3 + 6) + 2 * wt + 2 * ut + cell * (2 + 3 + 3) + 3 * wt + 3 * ut)
+    return {"workload": "noise! at 1024^2, Gaussian width 4h (FFT 1040 x 1024), correlation time 10 dt",
+            "us_per_call": round(us, 2), "cells_per_s": round(cell / (us * 1e-6), 1),
+            "algorithmic_bytes_per_call": nbytes, "gbs": round(nbytes / (us * 1e-6) / 1e9, 1),
+            "frac_of_hbm_peak": round(nbytes / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}
+
+
 def cpu_baseline(N, sweeps_per_step):
     """One Picard sweep of the same step on the CPU oracle (C, OpenMP; SciPy sparse LU for Stokes)."""
     import numpy as np
@@ -374,8 +551,8 @@ def cpu_baseline(N, sweeps_per_step):
     spp = max(sweeps_per_step, 1.0)
     return {"value": round(1.0 / (t_sweep * spp), 5), "unit": "steps/s", "cores": cores, "kind": "port",
             "sample": f"one Picard sweep (phsevo!, fluidmech! by sparse LU, noise!, update_phase_velocities!, "
-                      f"update!) at {N}^2 on the C oracle took {t_sweep:.2f} s; steps/s = 1 / (sweep time x "
-                      f"{spp:.2f} sweeps/step of the GPU run); grid {N}^2, not 512^2"}
+                      f"update!) of the {N}^2 step on the C oracle took {t_sweep:.2f} s; steps/s = 1 / (sweep time x "
+                      f"{spp:.2f} sweeps/step of the GPU run)"}
 
 
 def main():
@@ -385,7 +562,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--stencil-n", type=int, default=N_STENCIL)
-    ap.add_argument("--cpu-n", type=int, default=256)
+    ap.add_argument("--cpu-n", type=int, default=512)
     ap.add_argument("--weights", default=os.path.join(ROOT, "xcore.jl_amd", "lcc_weights.npz"),
                     help="trained learned-correction weights; when present the learned step is timed too")
     ap.add_argument("--no-learned", action="store_true")
@@ -467,7 +644,14 @@ def main():
         out["ensemble"] = ensemble_bench()
         print("bench: ensemble of 64 whole runs", file=sys.stderr, flush=True)
         out["ensemble_steps"] = ensemble_steps_bench()
+    print("bench: noise! at 1024^2 (OU microbench)", file=sys.stderr, flush=True)
+    out["ou_noise_1024"] = ou_bench()
     if not args.no_cpu:
+        print("bench: config 1 (100^2, 20 steps) on the device and the CPU oracle", file=sys.stderr, flush=True)
+        out["config1"] = c1_bench()
+        if not args.no_stencil:
+            print(f"bench: CPU oracle stencil calls at {args.stencil_n}^2", file=sys.stderr, flush=True)
+            out["stencil_cpu"] = stencil_cpu_bench(args.stencil_n)
         print(f"bench: CPU oracle sweep at {args.cpu_n}^2", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, info["plain"]["sweeps_per_step"] if "plain" in info else 1.0)
     print(json.dumps(out), flush=True)

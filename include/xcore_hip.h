@@ -188,8 +188,10 @@ int xc_advect_set_tile(xc_ctx* ctx, int scheme, int with_flux, int64_t Nz, int64
 /* Diagnostic, no counterpart in the reference: the memory traffic of a stencil launch without its
  * arithmetic.  out[k](i + (k > 0), j) = sum_r in[r](i, j) for every cell (i, j) of out[0]'s shape
  * (0..3 inputs, 1..3 outputs; outputs k >= 1 one row down, as the padded face fluxes), written with
- * non-temporal stores when nts != 0 -- the bandwidth HBM sustains for that mix of streams, the
- * achievable bound a stencil kernel with the same traffic is measured against. */
+ * non-temporal stores when nts bit 0 is set -- the bandwidth HBM sustains for that mix of streams, the
+ * achievable bound a stencil kernel with the same traffic is measured against.  nts bit 1: the best case of
+ * the same traffic, every stream aligned and not shifted, 16-byte accesses (needs even rows and 16-byte
+ * aligned columns). */
 int xc_diag_stream(xc_ctx* ctx, int nr, const xc_mat* in, int nw, const xc_mat* out, int nts);
 
 /* ---- field updates -------------------------------------------------------- */

@@ -155,34 +155,9 @@ def test_c2_stencils_2048_vs_oracle(xc, orc):
         assert e == 0.0, (n, e)   # bit-identical
 
 
-def ou_1024_setup(xc, N=1024, seed=4001):
-    """noise! at N^2 with L0h = l0h = 4h (pad 16: FFT length N + 16) and tau = 10 dt after noise.jl:171-172's
-    update: the pre-update tau (= L0h / (2 V)) is 9 / 0.91 dt, so 1 / (1/tau + 1/(100 dt)) + dt = 10 dt."""
-    D = 1.0
-    h = D / N
-    par = xc.Parameters(D=D, N=N, L=D, L0=7 * h, l0=7 * h, x0=0.05, Da=0.0)
-    g = xc.Grid.from_par(par)
-    sc = xc.compute_scales(par, g)
-    assert abs(sc.L0h - 4 * h) < 1e-15 * D and 4 * math.ceil(sc.L0h / h) == 16
-    dt = 1e-3
-    tau0 = dt / (1.0 / 9.0 - 0.01)
-    ph = xc.PhaseState(N, N)
-    ns = xc.NoiseState(g, sc, seed=seed)
-    gen = torch.Generator(device="cuda").manual_seed(seed)
-    ph.V.fill_(sc.L0h / (2 * tau0))
-    ph.vx.fill_(sc.l0h / (2 * tau0))
-    ph.fReL.fill_(1.0)
-    ph.ke.fill_(1e-3)
-    ph.ks.fill_(1e-4)
-    ph.chi.copy_(0.05 + 0.01 * torch.rand((N, N), generator=gen, device="cuda", dtype=torch.float64))
-    for n in ("psieo", "psixo", "psiso"):
-        getattr(ns, n).copy_(1e-3 * torch.randn((N, N), generator=gen, device="cuda", dtype=torch.float64))
-    ph.bndtaperw.fill_(1.0)
-    return par, g, sc, ph, ns, dt
-
-
 def test_c2_ou_noise_1024_vs_oracle(xc, orc):
-    par, g, sc, ph, ns, dt = ou_1024_setup(xc)
+    import bench
+    par, g, sc, ph, ns, dt = bench.ou_setup()
     N = g.Nz
     print("C2 OU noise! 1024^2, FFT 1040 x 1024", flush=True)
     hph, hns = _host(orc, ph, "phase"), _host(orc, ns, "noise")

@@ -2,7 +2,7 @@
 // without its arithmetic, to measure the bandwidth HBM sustains for that mix of streams.  Each of
 // nw outputs receives, at every cell (i, j) of out[0]'s shape, the sum of the nr inputs there;
 // outputs k >= 1 are written one row down (the padded flux arrays' layout).  Plain or
-// non-temporal stores.
+// non-temporal stores (mode bit 0); mode bit 1: the aligned 16-byte form, outputs not shifted.
 #include "xc_internal.h"
 
 namespace {
@@ -31,8 +31,38 @@ __global__ __launch_bounds__(256) void k_stream(StreamArgs a) {
     }
 }
 
+// The best case of the same traffic: every stream aligned (outputs k >= 1 not shifted), two consecutive
+// elements a lane as one 16-byte access, 128 rows a wave: what HBM sustains for NR reads and NW writes
+// when every access is a whole, aligned line.
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+template <int NR, int NW, bool NTS>
+__global__ __launch_bounds__(256) void k_stream_vec(StreamArgs a) {
+    const int t = blockIdx.x;
+    const int iz = (t % a.nbz) * 128 + 2 * (threadIdx.x & 63);
+    const int ix = (t / a.nbz) * 4 + (threadIdx.x >> 6);
+    if (iz >= a.Nz || ix >= a.Nx) return;
+    dbl2 s = {0.0, 0.0};
+#pragma unroll
+    for (int r = 0; r < NR; ++r) s += *reinterpret_cast<const dbl2*>(a.in[r] + iz + (int64_t)ix * a.ld_in[r]);
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        dbl2* p = reinterpret_cast<dbl2*>(a.out[w] + iz + (int64_t)ix * a.ld_out[w]);
+        if constexpr (NTS) __builtin_nontemporal_store(s, p);
+        else *p = s;
+    }
+}
+
 template <int NR, int NW>
-void launch_nr_nw(const StreamArgs& a, int nts, hipStream_t st) {
+void launch_nr_nw(const StreamArgs& a, int mode, hipStream_t st) {
+    const bool nts = mode & 1;
+    if (mode & 2) {
+        StreamArgs b = a;
+        b.nbz = (a.Nz + 127) / 128;
+        const dim3 grid((unsigned)(b.nbz * ((a.Nx + 3) / 4)));
+        if (nts) hipLaunchKernelGGL((k_stream_vec<NR, NW, true>), grid, dim3(256), 0, st, b);
+        else hipLaunchKernelGGL((k_stream_vec<NR, NW, false>), grid, dim3(256), 0, st, b);
+        return;
+    }
     const dim3 grid((unsigned)(a.nbz * ((a.Nx + 3) / 4)));
     if (nts) hipLaunchKernelGGL((k_stream<NR, NW, true>), grid, dim3(256), 0, st, a);
     else hipLaunchKernelGGL((k_stream<NR, NW, false>), grid, dim3(256), 0, st, a);
@@ -57,6 +87,12 @@ extern "C" int xc_diag_stream(xc_ctx* ctx, int nr, const xc_mat* in, int nw, con
         if (!xc_mat_ok(out[w]) || out[w].rows < Nz + (w ? 1 : 0) || out[w].cols < Nx)
             return xc_fail(ctx, XC_EINVAL_SHAPE, "diag_stream: output %d too small (outputs 1.. need a row more)", w);
         a.out[w] = out[w].p; a.ld_out[w] = out[w].ld;
+    }
+    if (nts & 2) {  // aligned 16-byte form: even rows, 16-byte aligned columns, outputs not shifted
+        bool ok = Nz % 2 == 0;
+        for (int r = 0; r < nr; ++r) ok = ok && ((uintptr_t)in[r].p % 16 == 0) && in[r].ld % 2 == 0;
+        for (int w = 0; w < nw; ++w) ok = ok && ((uintptr_t)out[w].p % 16 == 0) && out[w].ld % 2 == 0;
+        if (!ok) return xc_fail(ctx, XC_EINVAL_ARG, "diag_stream: the aligned form needs even rows and 16-byte columns");
     }
     a.Nz = (int)Nz; a.Nx = (int)Nx; a.nbz = (int)((Nz + 63) / 64);
     if ((int64_t)a.nbz * ((Nx + 3) / 4) > (1ll << 31) - 1) return xc_fail(ctx, XC_EINVAL_SHAPE, "diag_stream: grid too large");

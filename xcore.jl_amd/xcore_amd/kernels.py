@@ -180,8 +180,9 @@ def set_advect_tile(scheme, Nz, Nx, tz, tx, with_flux=True):
 def diag_stream_(inputs, outputs, nts=False):
     """Diagnostic (no reference counterpart): the stencil traffic without its arithmetic --
     outputs[k][i + (k > 0), j] = sum of inputs[r][i, j] over the cells of outputs[0]'s shape, with
-    non-temporal stores if nts.  Timing it gives the bandwidth HBM sustains for that stream mix."""
+    non-temporal stores if nts bit 0 is set; nts bit 1: every stream aligned and unshifted, 16-byte accesses.
+    Timing it gives the bandwidth HBM sustains for that stream mix."""
     c = context()
     ins = (_lib.XCMat * max(1, len(inputs)))(*[as_mat(t, "in") for t in inputs])
     outs = (_lib.XCMat * len(outputs))(*[as_mat(t, "out") for t in outputs])
-    c.check(lib().xc_diag_stream(c.ptr, len(inputs), ins, len(outputs), outs, int(bool(nts))), "diag_stream")
+    c.check(lib().xc_diag_stream(c.ptr, len(inputs), ins, len(outputs), outs, int(nts)), "diag_stream")
