@@ -84,7 +84,7 @@ extern "C" int xc_diag_stream(xc_ctx* ctx, int nr, const xc_mat* in, int nw, con
         a.in[r] = in[r].p; a.ld_in[r] = in[r].ld;
     }
     for (int w = 0; w < nw; ++w) {
-        if (!xc_mat_ok(out[w]) || out[w].rows < Nz + (w ? 1 : 0) || out[w].cols < Nx)
+        if (!xc_mat_ok(out[w]) || out[w].rows < Nz + ((w && !(nts & 2)) ? 1 : 0) || out[w].cols < Nx)
             return xc_fail(ctx, XC_EINVAL_SHAPE, "diag_stream: output %d too small (outputs 1.. need a row more)", w);
         a.out[w] = out[w].p; a.ld_out[w] = out[w].ld;
     }
