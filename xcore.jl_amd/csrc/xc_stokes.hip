@@ -870,9 +870,8 @@ __global__ __launch_bounds__(NT) void k_sub_grad(LevD L, Flags F, const double* 
 // 5-point operator T = cz (-d_zz) + cx (-d_xx), cz = dt / (a1 h^2),
 // cx = dt / ((a1 + gamma) h^2), periodic in x and zero-flux at the top and
 // bottom rows (W there is a BC row).  T is diagonalised in x by a real FFT and
-// solved in z by a tridiagonal (Thomas) sweep per wavenumber whose pivots are
-// tabulated on the host; the kx = 0 system is singular (constant pressure) and
-// is solved with the mean removed.
+// in z by a cosine transform; the kx = 0 system is singular (constant
+// pressure) and is solved with the mean removed.
 
 // rows of the interior P residual -> half spectra S[r][k], k = 0..Nx/2 (two rows per complex line)
 struct PoisXArgs {
@@ -925,6 +924,16 @@ __global__ __launch_bounds__(FFT_NT) void k_pois_xfwd(PoisXArgs a) {
 // y[src(n)] = v'[n] / Nz.  A lane takes the pair (k, Nz-k) through the
 // forward post-twiddle, the division and the inverse pre-twiddle in registers.
 // This is the mirrored 2 Nz-point FFT solve at half the transform length.
+//
+// open_cnv: the P row Nz is pinned (its DM row plus the pin; the DM row vanishes on the velocities
+// A^-1 GG p produces, because the W row Nz is the same mass balance as a BC row), so the Schur complement
+// is the identity on row Nz and, on rows 1..Nz-1, the operator with the value of row Nz as a fixed
+// (Dirichlet) neighbour: z(Nz) = r(Nz), and rows 1..Nz-1 solve T with y(Nz) held at cz r(Nz) in the
+// spectral units y = cz FFT_x(z).  Per wavenumber that is the zero-flux solve y0 of the column with its
+// last entry zeroed plus a multiple of g_k = (T_N + lam_x[k])^-1 e_last (tabulated on the host) that
+// brings y(last) to the target; for k = 0 (T_N singular: g_0 = T_N^+ e_last) the right-hand side's sum s
+// is moved into the last row (-s g_0) and the constant mode sets y(last).
+constexpr int PZ_NLMAX = 4;  // lines a workgroup of k_pois_zfft takes at most
 struct PoisZArgs {
     double2* S;          // Nz x K, k fastest
     Plan p;              // length Nz
@@ -932,6 +941,8 @@ struct PoisZArgs {
     const double* lamx;  // K: (cx/cz) (2 - 2 cos(2 pi k / Nx))
     const double* lamz;  // Nz: 2 - 2 cos(pi kz / Nz)
     int Nz, K, nl;
+    const double* g;     // open_cnv: K x Nz, g[k * Nz + i] = g_k(i); null otherwise
+    double cz;           // dt / (a1 h^2)
 };
 constexpr int ZD_PER = 5;  // (k, Nz-k) pairs a lane handles: nl (Nz/2 + 1) <= ZD_PER FFT_NT (Nz <= MAXLINE)
 template <int LG>
@@ -944,10 +955,19 @@ __global__ __launch_bounds__(FFT_NT) void k_pois_zfft(PoisZArgs a) {
     double2* A = sm;
     double2* B = sm + nl * n;
     auto src = [&](int m) { return m < (n + 1) / 2 ? 2 * m : 2 * n - 2 * m - 1; };
+    const bool open = a.g != nullptr;
+    const int mlast = (n - 1) % 2 == 0 ? (n - 1) / 2 : n / 2;  // src(mlast) = n - 1
+    __shared__ double2 s_tgt[PZ_NLMAX], s_sum[PZ_NLMAX], s_t[PZ_NLMAX], s_c[PZ_NLMAX];
     for (int i = threadIdx.x; i < nl * n; i += FFT_NT) {
         const int l = i / n, m = i - l * n;
         const int k = k0 + l;
-        A[i] = k < K ? a.S[(int64_t)src(m) * K + k] : make_double2(0.0, 0.0);
+        const double2 v = k < K ? a.S[(int64_t)src(m) * K + k] : make_double2(0.0, 0.0);
+        if (open && m == mlast) {  // open_cnv: the pinned row leaves the column; its target value
+            s_tgt[l] = make_double2(a.cz * v.x, a.cz * v.y);
+            A[i] = make_double2(0.0, 0.0);
+        } else {
+            A[i] = v;
+        }
     }
     // this lane's pairs: the twiddle and both scaled inverse eigenvalues, fetched before the transform
     double2 wq[ZD_PER];
@@ -982,6 +1002,7 @@ __global__ __launch_bounds__(FFT_NT) void k_pois_zfft(PoisZArgs a) {
                                             0.5 * ((wa.x * va.y + wa.y * va.x) + (wa.x * vb.y - wa.y * vb.x)));
             const double2 cb = make_double2(0.5 * ((wb.x * vb.x - wb.y * vb.y) + (wb.x * va.x + wb.y * va.y)),
                                             0.5 * ((wb.x * vb.y + wb.y * vb.x) + (wb.x * va.y - wb.y * va.x)));
+            if (open && kz == 0) s_sum[l] = ca;  // C[0]: the column's sum
             const double2 ea = make_double2(ca.x * fa[q], ca.y * fa[q]);
             const double2 eb = kz == 0 ? make_double2(0.0, 0.0) : make_double2(cb.x * fb[q], cb.y * fb[q]);
             // V'[k] = conj(w_k) (E[k] - i E[Nz-k]), V'[Nz-k] = conj(w_{Nz-k}) (E[Nz-k] - i E[k])
@@ -995,10 +1016,37 @@ __global__ __launch_bounds__(FFT_NT) void k_pois_zfft(PoisZArgs a) {
     const double2* Y;
     if constexpr (LG > 0) Y = lds_fft_p2<true, LG>(X, X == A ? B : A, nl, w);
     else Y = lds_fft<true>(X, X == A ? B : A, nl, a.p);
+    if (!open) {
+        for (int i = threadIdx.x; i < nl * n; i += FFT_NT) {
+            const int l = i / n, m = i - l * n;
+            const int k = k0 + l;
+            if (k < K) a.S[(int64_t)src(m) * K + k] = Y[i];
+        }
+        return;
+    }
+    // open_cnv: y = y0 + t g_k + c per line (c = 0 for k > 0)
+    if (threadIdx.x < nl && k0 + (int)threadIdx.x < K) {
+        const int l = threadIdx.x, k = k0 + l;
+        const double2 y0 = Y[l * n + mlast], tg = s_tgt[l];
+        const double gl = a.g[(int64_t)k * n + n - 1];
+        if (k > 0) {
+            s_t[l] = make_double2((tg.x - y0.x) / gl, (tg.y - y0.y) / gl);
+            s_c[l] = make_double2(0.0, 0.0);
+        } else {
+            const double2 sm0 = s_sum[l];
+            s_t[l] = make_double2(-sm0.x, -sm0.y);
+            s_c[l] = make_double2(tg.x - (y0.x - sm0.x * gl), tg.y - (y0.y - sm0.y * gl));
+        }
+    }
+    __syncthreads();
     for (int i = threadIdx.x; i < nl * n; i += FFT_NT) {
         const int l = i / n, m = i - l * n;
         const int k = k0 + l;
-        if (k < K) a.S[(int64_t)src(m) * K + k] = Y[i];
+        if (k < K) {
+            const double gi = a.g[(int64_t)k * n + src(m)];
+            const double2 t = s_t[l], c = s_c[l], y = Y[i];
+            a.S[(int64_t)src(m) * K + k] = make_double2(y.x + t.x * gi + c.x, y.y + t.y * gi + c.y);
+        }
     }
 }
 
@@ -1047,9 +1095,9 @@ __global__ __launch_bounds__(FFT_NT) void k_pois_xinv(PoisInvArgs a) {
         const int64_t q = (r + 1) + (int64_t)(c + 1) * (Nz + 2);
         const double sv = a.L.RHO(r, c) / a.L.ETA(r, c);
         double z;
-        if (is_pin(a.F, r + 1, c + 1, Nz)) z = a.F.bnchm ? a.rP[q] : a.rP[q] / (1.0 + sv);
-        else z = a.rP[q] / sv;
-        z += a.scale * pv;
+        if (a.F.open_cnv && is_pin(a.F, r + 1, c + 1, Nz)) z = a.rP[q];  // identity row of the Schur complement
+        else if (is_pin(a.F, r + 1, c + 1, Nz)) z = (a.F.bnchm ? a.rP[q] : a.rP[q] / (1.0 + sv)) + a.scale * pv;
+        else z = a.rP[q] / sv + a.scale * pv;
         a.zP[q] = z;
         Zs[rr * n + c] = z;
     }
@@ -1782,6 +1830,8 @@ struct StokesCtx {
     double2* pS = nullptr;      // Nz x pK half spectra
     double* plamx = nullptr;    // pK x-eigenvalues (normalised by cz)
     double* plamz = nullptr;    // Nz z-eigenvalues of the cosine transform
+    double* pg = nullptr;       // open_cnv: K x Nz correction vectors g_k of the pinned bottom row
+    double pgtab = -1.0;        // cx / cz the g table holds
     double2* ptw = nullptr;     // Nx twiddles
     double2* ptwz = nullptr;    // Nz twiddles of the z plan, then Nz cosine-transform twiddles
     std::vector<int> prad, pradz;
@@ -1807,6 +1857,8 @@ struct StokesCtx {
     int lc_fail_streak = 0, lc_skip = 0;
 };
 
+int pois_open_table(xc_ctx* ctx, StokesCtx* s, int Nz, int Nx);
+
 // Decide whether the inertial Schur part applies to this solve and (re)build its eigenvalue tables.
 int pois_table(xc_ctx* ctx, StokesCtx* s, const Flags& F, int Nz, int Nx, double h) {
     s->pratio = -1.0;
@@ -1814,7 +1866,7 @@ int pois_table(xc_ctx* ctx, StokesCtx* s, const Flags& F, int Nz, int Nx, double
     const double ratio = F.a1 / (F.a1 + F.gamma);
     s->pscale = F.a1 * h * h / (F.dt * (double)Nx);
     s->pratio = ratio;
-    if (ratio == s->ptab) return XC_OK;
+    if (ratio == s->ptab) return F.open_cnv ? pois_open_table(ctx, s, Nz, Nx) : XC_OK;
     const int K = s->pK;
     const long double pi = 3.141592653589793238462643383279502884L;
     std::vector<double> lam((size_t)K + (size_t)Nz);
@@ -1825,6 +1877,49 @@ int pois_table(xc_ctx* ctx, StokesCtx* s, const Flags& F, int Nz, int Nx, double
     XC_TRY_HIP(ctx, hipMemcpyAsync(s->plamx, lam.data(), lam.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     s->ptab = ratio;
+    return F.open_cnv ? pois_open_table(ctx, s, Nz, Nx) : XC_OK;
+}
+
+// open_cnv's correction vectors: g_k = (T_N + lam_x[k])^-1 e_last for k > 0 (tridiagonal elimination,
+// diagonally dominant), g_0 = T_N^+ e_last (zero-flux Laplacian, pseudo-inverse: the zero-mean solution of
+// T_N y = e_last - 1/Nz, by its face fluxes f_i = sum_{j <= i} b_j, y_{i+1} = y_i - f_i), in long double
+int pois_open_table(xc_ctx* ctx, StokesCtx* s, int Nz, int Nx) {
+    if (s->pgtab == s->ptab) return XC_OK;
+    const int K = s->pK;
+    const long double pi = 3.141592653589793238462643383279502884L;
+    std::vector<double> g((size_t)K * Nz);
+    std::vector<long double> y((size_t)Nz), cp((size_t)Nz);
+    for (int k = 0; k < K; ++k) {
+        if (k == 0) {
+            long double f = 0.0L, mean = 0.0L;
+            y[0] = 0.0L;
+            for (int i = 0; i + 1 < Nz; ++i) {
+                f += (i == Nz - 1 ? 1.0L : 0.0L) - 1.0L / (long double)Nz;
+                y[i + 1] = y[i] - f;
+            }
+            for (int i = 0; i < Nz; ++i) mean += y[i];
+            mean /= (long double)Nz;
+            for (int i = 0; i < Nz; ++i) g[i] = (double)(y[i] - mean);
+            continue;
+        }
+        const long double lam = (long double)s->ptab * (2.0L - 2.0L * cosl(2.0L * pi * (long double)k / (long double)Nx));
+        // Thomas: diagonal 1 + lam at both ends (2 + lam inside), off-diagonals -1, right-hand side e_last
+        auto dg = [&](int i) { return (i == 0 || i == Nz - 1 ? 1.0L : 2.0L) + lam; };
+        if (Nz == 1) { g[(size_t)k * Nz] = (double)(1.0L / dg(0)); continue; }
+        long double den = dg(0);
+        cp[0] = -1.0L / den;
+        y[0] = 0.0L;
+        for (int i = 1; i < Nz; ++i) {
+            den = dg(i) + cp[i - 1];  // dg(i) - (-1) cp[i-1]
+            cp[i] = -1.0L / den;
+            y[i] = ((i == Nz - 1 ? 1.0L : 0.0L) + y[i - 1]) / den;
+        }
+        for (int i = Nz - 2; i >= 0; --i) y[i] -= cp[i] * y[i + 1];
+        for (int i = 0; i < Nz; ++i) g[(size_t)k * Nz + i] = (double)y[i];
+    }
+    XC_TRY_HIP(ctx, hipMemcpyAsync(s->pg, g.data(), g.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    s->pgtab = s->ptab;
     return XC_OK;
 }
 
@@ -1917,7 +2012,7 @@ int setup(xc_ctx* ctx, int Nz, int Nx, int m, StokesCtx** out, int maxlev = -1) 
     }
     tot += 4 * (size_t)n + (size_t)(m + 1) * n + (size_t)m * n + (size_t)(2 * m + 4) * part_cols + 128 /* dots */ +
            (size_t)n /* tmp */ + 8 * (size_t)(Nz + 2) * (Nx + 3) /* advection scratch */ +
-           2 * (size_t)Nz * K + 2 * (size_t)Nx + 4 * (size_t)Nz + K + 2 * (size_t)Nz + 16 /* Poisson */ +
+           3 * (size_t)Nz * K + 2 * (size_t)Nx + 4 * (size_t)Nz + K + 2 * (size_t)Nz + 16 /* Poisson */ +
            (size_t)(m + 1) * m + 4 * (size_t)m + 8 /* Arnoldi */ + 4 * (size_t)(m + 2) /* dots */;
     const Level& F0 = s->lev[0];
     const size_t ncoef0 = 2 * (size_t)F0.Nz * F0.Nx + (size_t)(F0.Nz + 1) * (F0.Nx + 1) + (size_t)(F0.Nz + 1) * F0.Nx +
@@ -1974,6 +2069,7 @@ int setup(xc_ctx* ctx, int Nz, int Nx, int m, StokesCtx** out, int maxlev = -1) 
     s->ptwz = reinterpret_cast<double2*>(take(4 * (size_t)Nz));
     s->plamx = take((size_t)K + 2 * (size_t)Nz);
     s->plamz = s->plamx + K;
+    s->pg = take((size_t)Nz * K);
     if ((size_t)(p - s->pool) > tot) {  // carving overran the allocation: a sizing bug, never a user error
         stokes_free(ctx, s);
         return xc_fail(ctx, XC_EHIP, "fluidmech!: internal pool sizing (%zu > %zu)", (size_t)(p - s->pool), tot);
@@ -2270,10 +2366,11 @@ void precond(xc_ctx* ctx, StokesCtx* s, const Flags& F, const double* v, double*
         plan_mark_pow2(pz);
         // one line per workgroup up to K = 512 lines; at most ZD_PER (k, Nz-k) pairs a lane
         int nlz = K / 512;
-        int nlzmax = MAXLINE / Nz < 4 ? MAXLINE / Nz : 4;
+        int nlzmax = MAXLINE / Nz < PZ_NLMAX ? MAXLINE / Nz : PZ_NLMAX;
         while (nlzmax > 1 && nlzmax * (Nz / 2 + 1) > ZD_PER * FFT_NT) --nlzmax;
         nlz = nlz < 1 ? 1 : (nlz > nlzmax ? nlzmax : nlz);
-        PoisZArgs za{s->pS, pz, s->ptwz + Nz, s->plamx, s->plamz, Nz, K, nlz};
+        PoisZArgs za{s->pS, pz, s->ptwz + Nz, s->plamx, s->plamz, Nz, K, nlz, F.open_cnv ? s->pg : nullptr,
+                     F.dt / (F.a1 * L.h * L.h)};
         {
             const dim3 gz((unsigned)((K + nlz - 1) / nlz));
             const size_t ldz = 2 * (size_t)nlz * Nz * sizeof(double2);
