@@ -14,14 +14,17 @@
 // parameter vector keeps the layers in order e0a e0b e1a e1b e2a e2b e3a e3b
 // d2a d2b d1a d1b d0a d0b head, with e0a at cin = 4 (the device pads it to 16).
 //
-// Convolution = implicit GEMM on v_mfma_f32_16x16x16_bf16: one wavefront owns
-// 16 output pixels and every output channel (NB blocks of 16), K runs over the
-// 9 taps x cin in steps of 16.  A operand: lane l holds 4 input channels
-// (k = 4 (l / 16) .. +3) of output pixel (l % 16)'s neighbour at the tap, one
-// 8-byte load; B operand: lane l holds w[co = l % 16][t][k .. k+3], one 8-byte
-// load; D: lane l holds channel l % 16 of pixels 4 (l / 16) .. +3.  The
-// backward-data pass is the same kernel on transposed, tap-flipped weights;
-// the weight gradient is a GEMM over pixels accumulated with FP32 atomics.
+// Convolution = implicit GEMM on gfx950's v_mfma_f32_16x16x32_bf16: one
+// wavefront owns 16 output pixels and every output channel (NB blocks of 16), K
+// runs over the 9 taps x cin in steps of 32 (a last step of 16 channels, when
+// cin is an odd multiple of 16, on v_mfma_f32_16x16x16_bf16).  A operand: lane l
+// holds 8 input channels (k = 8 (l / 16) .. +7) of output pixel (l % 16)'s
+// neighbour at the tap, one 16-byte load; B operand: lane l holds
+// w[co = l % 16][t][k .. k+7], one 16-byte load (K is a sum: any assignment of
+// the k slots that A and B share gives the same product); D: lane l holds
+// channel l % 16 of pixels 4 (l / 16) .. +3.  The backward-data pass is the same
+// kernel on transposed, tap-flipped weights; the weight gradient is a GEMM over
+// pixels (32 per MFMA) accumulated with FP32 atomics.
 #include <math.h>
 #include <string.h>
 
@@ -33,6 +36,8 @@ namespace {
 
 typedef unsigned short bf16_t;
 typedef short s4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u8v __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int NT = 256;
@@ -73,7 +78,8 @@ __global__ __launch_bounds__(NT) void k_conv3(ConvArgs a) {
     const int64_t col = pa / a.H;                 // b * W + x
     const int x = (int)(col % a.W);
     const int64_t bW = col - x;                   // b * W
-    const int kq = 4 * (lane >> 4);
+    const int kq = 4 * (lane >> 4), kq8 = 8 * (lane >> 4);
+    const int c32 = a.cin & ~31;  // channels taken 32 at a time; a remaining 16 take the 16-deep MFMA
     f4 acc[NB];
 #pragma unroll
     for (int n = 0; n < NB; ++n) acc[n] = f4{0.f, 0.f, 0.f, 0.f};
@@ -84,13 +90,24 @@ __global__ __launch_bounds__(NT) void k_conv3(ConvArgs a) {
         xx = xx < 0 ? xx + a.W : (xx >= a.W ? xx - a.W : xx);
         const bool ok = va && zz >= 0 && zz < a.H;
         const int64_t q = ((bW + xx) * a.H + zz) * a.in_stride;
-        for (int c0 = 0; c0 < a.cin; c0 += 16) {
-            s4 av = s4{0, 0, 0, 0};
-            if (ok) av = *reinterpret_cast<const s4*>(a.in + q + c0 + kq);
+        for (int c0 = 0; c0 < c32; c0 += 32) {
+            u8v ar = u8v{0, 0, 0, 0, 0, 0, 0, 0};
+            if (ok) ar = *reinterpret_cast<const u8v*>(a.in + q + c0 + kq8);
+            const bf8 av = __builtin_bit_cast(bf8, ar);
 #pragma unroll
             for (int n = 0; n < NB; ++n) {
                 const int co = n * 16 + (lane & 15);
-                const s4 bv = *reinterpret_cast<const s4*>(a.w + ((int64_t)co * 9 + t) * a.cin + c0 + kq);
+                const bf8 bv = __builtin_bit_cast(bf8, *reinterpret_cast<const u8v*>(a.w + ((int64_t)co * 9 + t) * a.cin + c0 + kq8));
+                acc[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[n], 0, 0, 0);
+            }
+        }
+        if (c32 < a.cin) {
+            s4 av = s4{0, 0, 0, 0};
+            if (ok) av = *reinterpret_cast<const s4*>(a.in + q + c32 + kq);
+#pragma unroll
+            for (int n = 0; n < NB; ++n) {
+                const int co = n * 16 + (lane & 15);
+                const s4 bv = *reinterpret_cast<const s4*>(a.w + ((int64_t)co * 9 + t) * a.cin + c32 + kq);
                 acc[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[n], 0, 0, 0);
             }
         }
@@ -120,7 +137,7 @@ struct WGradArgs {
     int cin_w;
     int B, H, W;
     int64_t P;
-    int chunk;                 // pixels per wavefront (multiple of 16)
+    int chunk;                 // pixels per wavefront (multiple of 32)
 };
 __global__ __launch_bounds__(NT) void k_conv3_wgrad(WGradArgs a) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -136,33 +153,33 @@ __global__ __launch_bounds__(NT) void k_conv3_wgrad(WGradArgs a) {
     const int64_t pend = pbeg + a.chunk < a.P ? pbeg + a.chunk : a.P;
     const int co = cb * 16 + (lane & 15);   // A row
     const int ci = ib * 16 + (lane & 15);   // B column
-    const int kq = 4 * (lane >> 4);
+    const int kq8 = 8 * (lane >> 4);
     f4 acc = f4{0.f, 0.f, 0.f, 0.f};
-    for (int64_t pk = pbeg; pk < pend; pk += 16) {
-        s4 av, bv;
+    for (int64_t pk = pbeg; pk < pend; pk += 32) {  // K = 32 pixels a step, 8 a lane
+        u8v ar, br;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int64_t p = pk + kq + i;
-            short ga = 0, xb = 0;
+        for (int i = 0; i < 8; ++i) {
+            const int64_t p = pk + kq8 + i;
+            unsigned short ga = 0, xb = 0;
             if (p < pend) {
-                ga = (short)a.gy[p * a.gy_stride + co];
+                ga = a.gy[p * a.gy_stride + co];
                 const int z = (int)(p % a.H);
                 const int64_t col = p / a.H;
                 const int x = (int)(col % a.W);
                 const int zz = z + dz;
                 int xx = x + dx;
                 xx = xx < 0 ? xx + a.W : (xx >= a.W ? xx - a.W : xx);
-                if (zz >= 0 && zz < a.H) xb = (short)a.x[((col - x + xx) * a.H + zz) * a.x_stride + ci];
+                if (zz >= 0 && zz < a.H) xb = a.x[((col - x + xx) * a.H + zz) * a.x_stride + ci];
             }
-            av[i] = ga;
-            bv[i] = xb;
+            ar[i] = ga;
+            br[i] = xb;
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, ar), __builtin_bit_cast(bf8, br), acc, 0, 0, 0);
     }
-    // D: lane -> column ci = ib*16 + lane%16, rows co = cb*16 + kq + r
+    // D: lane -> column ci = ib*16 + lane%16, rows co = cb*16 + 4 (lane/16) + r
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        const int cor = cb * 16 + kq + r;
+        const int cor = cb * 16 + 4 * (lane >> 4) + r;
         const int cic = ib * 16 + (lane & 15);
         if (cic < a.cin_w) atomicAdd(&a.gw[((int64_t)cor * 9 + t) * a.cin_w + cic], acc[r]);
     }
