@@ -2579,8 +2579,12 @@ int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int 
     // The Arnoldi steps run on the device without host round trips: the Gram-Schmidt
     // coefficients, the Hessenberg column and the Givens rotations never leave the GPU.
     // The host learns the residual estimate |g[j+1]| of step j only after it has
-    // enqueued step j+1, so the device never waits for the host; a converged cycle
-    // therefore computes one basis vector it does not use.
+    // enqueued step j+1, so the device never waits for the host.  Near convergence
+    // (step j's estimate extrapolated from the last two below NEAR_CONV times the bar)
+    // the host waits for step j before enqueuing step j+1 instead: if step j has
+    // converged, its column is closed with its provisional values and the preconditioner,
+    // operator and reduction of a step the solution would not use are never run (the
+    // device idles for one host round trip instead).
     Kry K{};
     K.H = s->kry;
     K.cs = K.H + (size_t)(m + 1) * m;
@@ -2598,6 +2602,7 @@ int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int 
     const double lc_bar = pow(s->plain_rate > 0 ? s->plain_rate : 0.2, LC_COST);
     int lsteps = 0;
     bool dropped = false;
+    constexpr double NEAR_CONV = 3.0;
     while (true) {
         {  // r = b - LL x and ||r||^2 (per-wave partials, their sum written to the mapped host buffer)
             const unsigned nbl = ll_cells_blocks(dd.Nz, dd.Nx);
@@ -2616,9 +2621,26 @@ int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int 
         hipLaunchKernelGGL(k_kry_init, dim3(1), dim3(64), 0, ctx->stream, K, m, beta);
         int used = 0;  // basis columns whose results are final
         int j = 0;
+        bool waited = false;  // step j - 1 was waited for before step j was enqueued
+        bool early = false;   // converged at step j - 1 with step j not enqueued: close column j - 1
         for (; j < m; ++j) {
             double* vj = s->V + (int64_t)j * n;  // p_j, provisional until this step's update
             double* zj = s->Z + (int64_t)j * n;
+            waited = false;
+            if (j >= 2 && !cur.on) {
+                const double r1 = hout[2 * (j - 2)], r0 = j >= 3 ? hout[2 * (j - 3)] : beta;
+                const double pred = r0 > 0.0 ? r1 * (r1 / r0) : r1;
+                if (pred <= 0.5 * target * NEAR_CONV) {
+                    XC_TRY_HIP(ctx, hipEventSynchronize(s->ev[(j - 1) & 1]));
+                    waited = true;
+                    used = j;
+                    ++iters;
+                    if (hout[2 * (j - 1)] <= 0.5 * target || hout[2 * (j - 1) + 1] == 0.0 || iters >= maxit) {
+                        early = true;
+                        break;
+                    }
+                }
+            }
             if ((st = precond_full(ctx, s, F, mo, n, cur, vj, zj))) return st;
             hipLaunchKernelGGL(k_ll_cells<false>, dim3(ll_cells_blocks(dd.Nz, dd.Nx)), dim3(NT), 0, ctx->stream, dd, F,
                                zj, zj, s->ws);
@@ -2645,14 +2667,16 @@ int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int 
                     dropped = true;
                 }
             }
-            if (j == 0) continue;
+            if (j == 0 || waited) continue;
             // step j-1 is complete once its event has fired; step j (enqueued) finalises its column
             XC_TRY_HIP(ctx, hipEventSynchronize(s->ev[(j - 1) & 1]));
             used = j;
             ++iters;
             if (hout[2 * (j - 1)] <= 0.5 * target || hout[2 * (j - 1) + 1] == 0.0 || iters >= maxit) break;
         }
-        if (j == m) {  // the last step of the cycle was not tested yet; its column keeps the provisional values
+        if (early) {  // column j - 1 keeps its provisional values, as the last column of a full cycle does
+            hipLaunchKernelGGL(k_arnoldi_close, dim3(1), dim3(64), 0, ctx->stream, K, m, j - 1);
+        } else if (j == m) {  // the last step of the cycle was not tested yet; its column keeps the provisional values
             XC_TRY_HIP(ctx, hipEventSynchronize(s->ev[(m - 1) & 1]));
             used = m;
             ++iters;
