@@ -459,23 +459,24 @@ __global__ __launch_bounds__(NT) void k_noise_flux(FluxArgs a) {
     xc_dom_ij(a.d, &j, &i);
     const int Nz = a.Nz, Nx = a.Nx;
     if (j >= Nz + 2 || i >= Nx + 2) return;
-    // the rescale of field f (noise.jl:265-267) as (v - fm) k + rm with k = rs / fs divided once here
-    // (identity when fs <= eps), instead of a division per use
-    double rm[3], fm[3], kf[3];
+    // the rescale of field f (noise.jl:265-267), in the reference's operation order ((v - fm) rs) / fs + rm
+    // (identity when fs <= eps), as resc() and k_resc_inplace compute it
+    double rm[3], fm[3], rs[3], fs[3];
     bool on[3];
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
         rm[f] = a.st[f];
         fm[f] = a.st[3 + f];
-        on[f] = a.st[9 + f] > EPS;
-        kf[f] = on[f] ? a.st[6 + f] / a.st[9 + f] : 1.0;
+        rs[f] = a.st[6 + f];
+        fs[f] = a.st[9 + f];
+        on[f] = fs[f] > EPS;
     }
     // ext(f, j, i) = flt_f[icz[j], icx[i]] with grid.icz closed, grid.icx periodic (grid.jl:68-69)
     auto ext = [&](int f, int jj, int ii) -> double {
         const int z = jj - 1 < 0 ? 0 : (jj - 1 >= Nz ? Nz - 1 : jj - 1);
         const int x = ii - 1 < 0 ? Nx - 1 : (ii - 1 >= Nx ? 0 : ii - 1);
         const double v = a.flt[f][z + (int64_t)x * a.ldf[f]];
-        return on[f] ? (v - fm[f]) * kf[f] + rm[f] : v;
+        return on[f] ? (v - fm[f]) * rs[f] / fs[f] + rm[f] : v;
     };
     auto psiec = [&](int jj, int ii) -> double {
         return (ext(0, jj, ii) + ext(0, jj, ii + 1) + ext(0, jj + 1, ii) + ext(0, jj + 1, ii + 1)) / 4.0;
