@@ -570,7 +570,8 @@ def main():
     ap.add_argument("--mg-post", type=int, default=3)
     ap.add_argument("--mg-coarse", type=int, default=0, help="coarsest-level sweeps (0: the library's choice)")
     ap.add_argument("--mg-sub", type=int, default=1)
-    ap.add_argument("--mg-omega", type=float, default=0.8, help="damped-Jacobi weight of the multigrid smoother")
+    ap.add_argument("--mg-omega", type=float, default=0.0,
+                    help="damped-Jacobi weight of the multigrid smoother (0: chosen per level from the inertia ratio)")
     ap.add_argument("--mg-levels", type=int, default=0, help="most multigrid levels (0: down to 4 x 4)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-stencil", action="store_true")

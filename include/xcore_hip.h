@@ -236,7 +236,8 @@ typedef struct {  /* fluidmech! keywords (fluidmech.jl:71-80) and the iterative-
     int mg_pre, mg_post, mg_coarse;  /* multigrid smoothing sweeps on the finest level (pre, post) and on
                                         the coarsest; 0 selects the defaults 3, 3 and, on the coarsest,
                                         20 at the 4 x 4 bottom or 2 where the cycle stopped early */
-    double mg_omega;  /* damped-Jacobi weight of the multigrid smoother (0: 0.8) */
+    double mg_omega;  /* damped-Jacobi weight of the multigrid smoother (0: per level from its inertia ratio
+                         R = a1 rho h^2 / (dt eta): 0.95 where R >= 1, 0.8 elsewhere and without inertia) */
     int learned;      /* the learned correction (needs xc_unet_set_params): 1 the network's coarse estimate e of
                          LL^-1 v first, z = e + M (v - LL e); 2 the network's estimate of what the plain
                          preconditioner M misses, z = M v + P N(R v).  A learned step is kept while it cuts the

@@ -2207,16 +2207,30 @@ int choose_depth(xc_ctx* ctx, StokesCtx* s, const Flags& F, const xc_fm_opts* o,
     return XC_OK;
 }
 
-// sub: pre = post sweeps on levels >= 1 (0: as level 0); f32: the fine level's smoother in single precision
-struct MGOpts { int pre, post, coarse; double omega; int sub; bool f32; };
+// sub: pre = post sweeps on levels >= 1 (0: as level 0); f32: the fine level's smoother in single precision;
+// om[l]: the damped-Jacobi weight on level l
+constexpr int MG_MAXLEV = 16;
+struct MGOpts { int pre, post, coarse; double om[MG_MAXLEV]; int sub; bool f32; };
 
-// solver controls with the defaults measured best on the 512^2 production step (zero fields: default)
-MGOpts mg_opts(const xc_fm_opts* o) {
+// Jacobi weight of a level from its inertia ratio R = a1 rho h^2 / (dt eta).  The eigenvalues of D^-1 A of the
+// velocity block lie in about [R / (R + 7/3), lam_max]: the smoothest mode, and a top that power iteration on the
+// 256^2 production system put at 1.61 for R = 2.4 (its smallest eigenvalue 0.478 by inverse iteration, against
+// 0.509 for R / (R + 7/3)).  Where R >= 1 the whole spectrum is smoothable (smallest eigenvalue >= 0.3) and the
+// weight 0.95 (2 / (lam_min + lam_max) for that range, capped) reduces both ends: 9.0 against 9.4 FGMRES
+// iterations per solve with 0.8 on the 512^2 step (1.0 and above lose).  Where R < 1 the smooth modes are left
+// to the coarser levels and the weight stays the classical 0.8 (0.95 there loses convergence altogether on
+// the inertial manufactured solution, R ~ 0.003-0.04).
+double auto_omega(double R) { return R >= 1.0 ? 0.95 : 0.8; }
+
+// solver controls with the defaults measured best on the 512^2 production step (zero fields: default); after
+// choose_depth, which measures the inertia ratio the automatic Jacobi weights come from (R_l = 4^l R_0)
+MGOpts mg_opts(const xc_fm_opts* o, const StokesCtx* s) {
     MGOpts m;
     m.pre = o->mg_pre > 0 ? o->mg_pre : 3;
     m.post = o->mg_post > 0 ? o->mg_post : 3;
     m.coarse = o->mg_coarse > 0 ? o->mg_coarse : 0;  // 0: coarse_sweeps decides per solve
-    m.omega = o->mg_omega > 0 ? o->mg_omega : 0.8;
+    double R = s->inertia;
+    for (int l = 0; l < MG_MAXLEV; ++l, R *= 4.0) m.om[l] = o->mg_omega > 0 ? o->mg_omega : auto_omega(R);
     m.sub = o->mg_sub > 0 ? o->mg_sub : (o->mg_sub == 0 ? 1 : 0);
     m.f32 = o->mg_fp64 == 0;
     return m;
@@ -2243,7 +2257,7 @@ void descend(xc_ctx* ctx, StokesCtx* s, const Flags& F, size_t l, const MGOpts& 
         return;
     }
     hipLaunchKernelGGL((k_smooth_cells<4, LevD, double, TF, double>), dim3(smooth_cells_blocks(C.Nz, C.Nx)), dim3(NT), 0,
-                       ctx->stream, C.d(), F, C.x, rf, C.t, mo.omega, 0, nullptr, C.b);
+                       ctx->stream, C.d(), F, C.x, rf, C.t, mo.om[l + 1], 0, nullptr, C.b);
     std::swap(C.x, C.t);
     vcycle(ctx, s, F, l + 1, mo, true);
 }
@@ -2260,7 +2274,7 @@ void vcycle(xc_ctx* ctx, StokesCtx* s, const Flags& F, size_t l, const MGOpts& m
         ta.xg = L.x;
         ta.pre = mo.sub > 0 ? mo.sub : mo.pre;
         ta.post = mo.sub > 0 ? mo.sub : mo.post;
-        ta.coarse = coarse_sweeps(s, mo); ta.omega = mo.omega;
+        ta.coarse = coarse_sweeps(s, mo); ta.omega = mo.om[l];
         ta.F = F;
         hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(TAIL_NT), s->tail_lds, ctx->stream, ta);
         return;
@@ -2273,11 +2287,11 @@ void vcycle(xc_ctx* ctx, StokesCtx* s, const Flags& F, size_t l, const MGOpts& m
     const int post = l > 0 && mo.sub > 0 ? mo.sub : mo.post;
     // first sweep from zero (already done by the caller when first_done)
     if (!first_done) {
-        hipLaunchKernelGGL(SMOOTH_D(1), dim3(nbc), dim3(NT), 0, ctx->stream, d, F, L.x, L.b, L.t, mo.omega, fine, nullptr, nullptr);
+        hipLaunchKernelGGL(SMOOTH_D(1), dim3(nbc), dim3(NT), 0, ctx->stream, d, F, L.x, L.b, L.t, mo.om[l], fine, nullptr, nullptr);
         std::swap(L.x, L.t);
     }
     for (int it = 1; it < pre; ++it) {
-        hipLaunchKernelGGL(SMOOTH_D(0), dim3(nbc), dim3(NT), 0, ctx->stream, d, F, L.x, L.b, L.t, mo.omega, fine, nullptr, nullptr);
+        hipLaunchKernelGGL(SMOOTH_D(0), dim3(nbc), dim3(NT), 0, ctx->stream, d, F, L.x, L.b, L.t, mo.om[l], fine, nullptr, nullptr);
         std::swap(L.x, L.t);
     }
     if (coarsest) return;
@@ -2285,7 +2299,7 @@ void vcycle(xc_ctx* ctx, StokesCtx* s, const Flags& F, size_t l, const MGOpts& m
     descend(ctx, s, F, l, mo, (const double*)L.r);
     hipLaunchKernelGGL(k_prolong_cells<double>, dim3(nbc), dim3(NT), 0, ctx->stream, L.Nz, L.Nx, F, s->lev[l + 1].x, L.x);
     for (int it = 0; it < post; ++it) {
-        hipLaunchKernelGGL(SMOOTH_D(0), dim3(nbc), dim3(NT), 0, ctx->stream, d, F, L.x, L.b, L.t, mo.omega, fine, nullptr, nullptr);
+        hipLaunchKernelGGL(SMOOTH_D(0), dim3(nbc), dim3(NT), 0, ctx->stream, d, F, L.x, L.b, L.t, mo.om[l], fine, nullptr, nullptr);
         std::swap(L.x, L.t);
     }
 }
@@ -2304,17 +2318,17 @@ void vcycle_f32(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, con
     float *x = s->fx, *t = s->ft;
     const bool coarsest = s->leff <= 1;
     const int pre = coarsest ? coarse_sweeps(s, mo) : mo.pre, post = mo.post;
-    hipLaunchKernelGGL(SMOOTH_F(3, double, float), dim3(nbc), dim3(NT), 0, ctx->stream, f, F, x, v, x, mo.omega, 1, zP,
+    hipLaunchKernelGGL(SMOOTH_F(3, double, float), dim3(nbc), dim3(NT), 0, ctx->stream, f, F, x, v, x, mo.om[0], 1, zP,
                        s->fb);
     bool in_z = false;
     auto sweep = [&](bool to_z) {
         if (to_z) {
             hipLaunchKernelGGL(SMOOTH_F(0, float, double), dim3(nbc), dim3(NT), 0, ctx->stream, f, F, x, s->fb, z,
-                               mo.omega, 1, nullptr, nullptr);
+                               mo.om[0], 1, nullptr, nullptr);
             in_z = true;
         } else {
             hipLaunchKernelGGL(SMOOTH_F(0, float, float), dim3(nbc), dim3(NT), 0, ctx->stream, f, F, x, s->fb, t,
-                               mo.omega, 1, nullptr, nullptr);
+                               mo.om[0], 1, nullptr, nullptr);
             std::swap(x, t);
         }
     };
@@ -2402,7 +2416,7 @@ void precond(xc_ctx* ctx, StokesCtx* s, const Flags& F, const double* v, double*
     const bool direct = s->leff > 1 && (mo.pre + mo.post) % 2 == 0;
     if (direct) L.x = z;
     hipLaunchKernelGGL(SMOOTH_D(3), dim3(smooth_cells_blocks(L.Nz, L.Nx)), dim3(NT), 0, ctx->stream, d, F, L.x, v, L.t,
-                       mo.omega, 1, zP, L.b);
+                       mo.om[0], 1, zP, L.b);
     std::swap(L.x, L.t);
     vcycle(ctx, s, F, 0, mo, true);
     if (!direct) hipLaunchKernelGGL(k_copyv, dim3(nblk(nv)), dim3(NT), 0, ctx->stream, L.x, z, nv);
@@ -2759,7 +2773,6 @@ int xc_fluidmech(xc_ctx* ctx, xc_fluid* fl, const xc_phase* ph, const xc_grid* g
     const Flags F = make_flags(o, par, Nz, Nx, bn != nullptr);
     const int64_t n = s->lev[0].nv + (int64_t)(Nz + 2) * (Nx + 2);
     const unsigned nb = nblk(n);
-    const MGOpts mo = mg_opts(o);
     // system: coefficients, RR, initial guess
     build_levels(ctx, s, fl, g->h, false);
     const bool depth_pending = depth_launch(ctx, s, F, o);
@@ -2770,6 +2783,7 @@ int xc_fluidmech(xc_ctx* ctx, xc_fluid* fl, const xc_phase* ph, const xc_grid* g
     if ((st = multidot(ctx, s, s->bs, 0, s->bs, n))) return st;
     const double bnorm = sqrt(s->host[0]);
     if ((st = choose_depth(ctx, s, F, o, g->h, depth_pending, true))) return st;  // multidot synchronized
+    const MGOpts mo = mg_opts(o, s);
     build_coarse(ctx, s, s->leff);
     const double target = o->tol * (bnorm > 0 ? bnorm : 1.0);
     const int maxit = o->maxit > 0 ? o->maxit : 500;
@@ -2848,7 +2862,7 @@ static int piece_setup(xc_ctx* ctx, const xc_fluid* fl, const xc_grid* g, const 
     if ((st = choose_depth(ctx, s, *F, o, g->h, depth_launch(ctx, s, *F, o), false))) return st;
     build_coarse(ctx, s, s->leff);
     if ((st = pois_table(ctx, s, *F, (int)g->Nz, (int)g->Nx, g->h))) return st;
-    *mo = mg_opts(o);
+    *mo = mg_opts(o, s);
     *n = s->lev[0].nv + (g->Nz + 2) * (g->Nx + 2);
     return XC_OK;
 }

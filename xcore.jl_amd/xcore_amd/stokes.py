@@ -60,7 +60,7 @@ class BnchmData:
 
 def fm_opts(par, sds=-1, top_cnv=1, bot_cnv=1, open_cnv=False, xBC="periodic", zBC="closed", dt=1e32,
             a1=1.0, a2=1.0, a3=0.0, b1=1.0, b2=0.0, b3=0.0, tol=1e-8, maxit=1000, restart=40,
-            mg_pre=3, mg_post=3, mg_coarse=0, mg_omega=0.8, learned=False, lc_level=2, mg_sub=1, mg_levels=0,
+            mg_pre=3, mg_post=3, mg_coarse=0, mg_omega=0.0, learned=False, lc_level=2, mg_sub=1, mg_levels=0,
             mg_fp64=False):
     return FmOpts(int(sds), int(top_cnv), int(bot_cnv), int(bool(open_cnv)), _sym(xBC, BCS, "xBC"),
                   _sym(zBC, BCS, "zBC"), _sym(par.ADVN, SCHEMES, "scheme"), float(dt), float(a1), float(a2),
