@@ -193,7 +193,7 @@ def test_fine_smoother_precision(xc, orc, open_cnv):
 
 @pytest.mark.parametrize("shape", [(48, 64), (256, 32), (512, 16), (2048, 8)])
 def test_schur_poisson_solve(xc, orc, shape):
-    # the inertial part of the pressure block of z = M^-1 v: with inertia minus without is
+    # the inertial part of the pressure block of z = M^-1 v: with inertia minus (the weighted) without is
     # (a1 h^2 / dt) T^-1 rP on the interior cells, T = -d_zz + a1 / (a1 + gamma) (-d_xx) in grid units
     # (zero flux at the top and bottom rows, periodic in x) on the zero-mean part of rP; checked against a
     # sparse direct solve of T bordered by the constant mode.  (48, 64): mixed-radix transforms; (256, 32):
@@ -211,7 +211,11 @@ def test_schur_poisson_solve(xc, orc, shape):
     z_vis = xc.stokes_precond(dfl, g, par, v, dt=1e32, **kw)
     nv = (Nz + 1) * (Nx + 2) + (Nz + 2) * (Nx + 1)
     blk = lambda t: t[nv:].cpu().numpy().reshape(Nx + 2, Nz + 2).T[1:-1, 1:-1]
-    d = (blk(z_in) - blk(z_vis)).ravel(order="F")
+    # with inertia the viscous part rP eta / rho carries the weight 0.75 (the library's SCHUR_CV), except at the
+    # pressure pin (MapP[round(Nz/2), round(Nx/2)], 1-based, whose row keeps rP / (1 + rho / eta))
+    wv = np.full((Nz, Nx), 0.75)
+    wv[round(Nz / 2) - 2, round(Nx / 2) - 2] = 1.0
+    d = (blk(z_in) - wv * blk(z_vis)).ravel(order="F")
     rP = blk(v).ravel(order="F")
     ratio = 1.5 / (1.5 + par.gamma)
     Dz = sp.diags([-np.ones(Nz - 1), 2 * np.ones(Nz), -np.ones(Nz - 1)], [-1, 0, 1]).tolil()
