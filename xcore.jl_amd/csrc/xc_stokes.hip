@@ -2008,8 +2008,9 @@ int setup(xc_ctx* ctx, int Nz, int Nx, int m, StokesCtx** out, int maxlev = -1) 
     const size_t lcc = (size_t)(Nz / 2) * (Nx / 2);  // learned correction at level >= 1
     tot += 3 * lcc + 16 + (lcc + NT - 1) / NT + 8;
     size_t part_cols = md_blocks(n) > DOT_BLOCKS ? (size_t)md_blocks(n) : (size_t)DOT_BLOCKS;
-    {  // also room for k_ll_cells' per-wave partials
-        const size_t waves = (size_t)ll_cells_blocks(Nz, Nx) * (NT / 64);
+    {  // also room for k_ll_cells' per-wave partials behind the first DOT_BLOCKS (a solve's first residual is
+       // reduced while ||RR||'s partials are)
+        const size_t waves = (size_t)ll_cells_blocks(Nz, Nx) * (NT / 64) + DOT_BLOCKS;
         const size_t rows = 2 * (size_t)m + 4;
         if (rows * part_cols < waves) part_cols = (waves + rows - 1) / rows;
     }
@@ -2585,8 +2586,20 @@ int precond_full(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, in
 }
 
 // right-preconditioned FGMRES(m) on LL x = b from the given x; stops at ||b - LL x|| <= target
+// The residual of the starting x: r0_ready: already in s->rs with its squared norm in s->host[R0_SLOT] (enqueued by
+// the caller before a synchronization it needed anyway, enqueue_r0), else computed here.
+constexpr int R0_SLOT = 4;
+void enqueue_r0(xc_ctx* ctx, StokesCtx* s, const Flags& F, const double* x, const double* b) {
+    const LevD dd = s->lev[0].d();
+    const unsigned nbl = ll_cells_blocks(dd.Nz, dd.Nx);
+    double* part = s->part + DOT_BLOCKS;
+    hipLaunchKernelGGL(k_ll_cells<true>, dim3(nbl), dim3(NT), 0, ctx->stream, dd, F, x, b, s->rs, part);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(NT), 0, ctx->stream, part, (int)(nbl * (NT / 64)), 1,
+                       s->host_dev + R0_SLOT);
+}
 int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int m, int64_t n, const double* b,
-               double* x, double target, double bnorm, int maxit, const LcCfg& lc, SolveOut* out) {
+               double* x, double target, double bnorm, int maxit, const LcCfg& lc, SolveOut* out,
+               bool r0_ready = false) {
     int st;
     const unsigned nb = nblk(n);
     const LevD dd = s->lev[0].d();
@@ -2620,15 +2633,14 @@ int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int 
     int lsteps = 0;
     bool dropped = false;
     constexpr double NEAR_CONV = 3.0;
+    bool first = true;
     while (true) {
-        {  // r = b - LL x and ||r||^2 (per-wave partials, their sum written to the mapped host buffer)
-            const unsigned nbl = ll_cells_blocks(dd.Nz, dd.Nx);
-            hipLaunchKernelGGL(k_ll_cells<true>, dim3(nbl), dim3(NT), 0, ctx->stream, dd, F, x, b, s->rs, s->part);
-            hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(NT), 0, ctx->stream, s->part, (int)(nbl * (NT / 64)), 1,
-                               s->host_dev);
+        if (!(first && r0_ready)) {  // r = b - LL x and ||r||^2 (per-wave partials, summed into the mapped host buffer)
+            enqueue_r0(ctx, s, F, x, b);
             XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
         }
-        const double beta = sqrt(s->host[0]);
+        first = false;
+        const double beta = sqrt(s->host[R0_SLOT]);
         rel = bnorm > 0 ? beta / bnorm : beta;
         if (rel0 < 0) rel0 = rel;
         if (!isfinite(beta)) break;  // not converged; a learned solve falls back to the plain one
@@ -2782,7 +2794,8 @@ int xc_fluidmech(xc_ctx* ctx, xc_fluid* fl, const xc_phase* ph, const xc_grid* g
     if ((st = pois_table(ctx, s, F, Nz, Nx, g->h))) return st;
     if ((st = build_rhs(ctx, s, fl, ph, g, par, o, bn, F, s->bs))) return st;
     hipLaunchKernelGGL(k_gather, dim3(nb), dim3(NT), 0, ctx->stream, fl->W, fl->U, fl->P, s->xs, Nz, Nx);
-    // ||RR||
+    // the starting residual, then ||RR||: one synchronization for both
+    enqueue_r0(ctx, s, F, s->xs, s->bs);
     if ((st = multidot(ctx, s, s->bs, 0, s->bs, n))) return st;
     const double bnorm = sqrt(s->host[0]);
     if ((st = choose_depth(ctx, s, F, o, g->h, depth_pending, true))) return st;  // multidot synchronized
@@ -2802,7 +2815,7 @@ int xc_fluidmech(xc_ctx* ctx, xc_fluid* fl, const xc_phase* ph, const xc_grid* g
         fallback = 1;
     }
     SolveOut so{};
-    if ((st = fgmres_run(ctx, s, F, mo, m, n, s->bs, s->xs, target, bnorm, maxit, lc, &so))) return st;
+    if ((st = fgmres_run(ctx, s, F, mo, m, n, s->bs, s->xs, target, bnorm, maxit, lc, &so, true))) return st;
     if (lc.on && !so.conv && !(so.rel < 1e30)) {  // non-finite iterate: once more, plain, from the initial guess
         const int used = so.iters;
         const double rel0_l = so.rel0;
