@@ -452,70 +452,52 @@ struct FluxArgs {
     double *xiew, *xieu, *xixw, *xixu, *xisw, *xisu;
     int64_t ldew, ldeu, ldxw, ldxu, ldsw, ldsu;
     int Nz, Nx;
+    xc_dom d;
 };
-// One workgroup per 64 x NFX tile of the (Nz+2) x (Nx+2) output positions (lane = row, each wave NFX/4
-// columns).  The three filtered fields are staged in LDS once per tile, rescaled (one division per staged
-// element instead of one per use: each value feeds up to eight outputs), over the field rows
-// clamp(j0-3 .. j0+64) and the field columns (i0-2 .. i0+NFX-1) mod Nx, which hold every value the tile's
-// fluxes read; same arithmetic, in the same order, as the reference's expressions.
-constexpr int NFX = 8;        // output columns per tile
-constexpr int NFR = 64 + 4;   // staged field rows
-constexpr int NFC = NFX + 2;  // staged field columns
 __global__ __launch_bounds__(NT) void k_noise_flux(FluxArgs a) {
-    __shared__ double sf[3][NFC][NFR];
+    int j, i;
+    xc_dom_ij(a.d, &j, &i);
     const int Nz = a.Nz, Nx = a.Nx;
-    const int nbz = (Nz + 2 + 63) >> 6;
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
-    const int j0 = (t % nbz) << 6, i0 = (t / nbz) * NFX;
+    if (j >= Nz + 2 || i >= Nx + 2) return;
     // the rescale of field f (noise.jl:265-267), in the reference's operation order ((v - fm) rs) / fs + rm
     // (identity when fs <= eps), as resc() and k_resc_inplace compute it
-    for (int k = threadIdx.x; k < 3 * NFC * NFR; k += NT) {
-        const int f = k / (NFC * NFR), rem = k - f * (NFC * NFR);
-        const int c = rem / NFR, r = rem - c * NFR;
-        const int fr = min(max(j0 - 3 + r, 0), Nz - 1);
-        const int fc = ((i0 - 2 + c) % Nx + Nx) % Nx;
-        const double v = a.flt[f][fr + (int64_t)fc * a.ldf[f]];
-        const double rm = a.st[f], fm = a.st[3 + f], rs = a.st[6 + f], fs = a.st[9 + f];
-        sf[f][c][r] = fs > EPS ? (v - fm) * rs / fs + rm : v;
+    double rm[3], fm[3], rs[3], fs[3];
+    bool on[3];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        rm[f] = a.st[f];
+        fm[f] = a.st[3 + f];
+        rs[f] = a.st[6 + f];
+        fs[f] = a.st[9 + f];
+        on[f] = fs[f] > EPS;
     }
-    __syncthreads();
-    // ext(f, jj, ii) = the rescaled flt_f[icz[jj], icx[ii]] with grid.icz closed, grid.icx periodic (grid.jl:68-69):
-    // field row clamp(jj - 1), field column xc(ii) = ii - 1 wrapped once; staged at row jj - j0 + 2 and
-    // column (xc(ii) - i0 + 2) mod Nx
+    // ext(f, j, i) = flt_f[icz[j], icx[i]] with grid.icz closed, grid.icx periodic (grid.jl:68-69)
     auto ext = [&](int f, int jj, int ii) -> double {
+        const int z = jj - 1 < 0 ? 0 : (jj - 1 >= Nz ? Nz - 1 : jj - 1);
         const int x = ii - 1 < 0 ? Nx - 1 : (ii - 1 >= Nx ? 0 : ii - 1);
-        int c = x - i0 + 2;  // in (-Nx, Nx + 2): one wrap at most, as selects (no integer division)
-        c += c < 0 ? Nx : 0;
-        c -= c >= Nx ? Nx : 0;
-        return sf[f][c][jj - j0 + 2];
+        const double v = a.flt[f][z + (int64_t)x * a.ldf[f]];
+        return on[f] ? (v - fm[f]) * rs[f] / fs[f] + rm[f] : v;
     };
     auto psiec = [&](int jj, int ii) -> double {
         return (ext(0, jj, ii) + ext(0, jj, ii + 1) + ext(0, jj + 1, ii) + ext(0, jj + 1, ii + 1)) / 4.0;
     };
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int j = j0 + lane;
-#pragma unroll
-    for (int q = 0; q < NFX / 4; ++q) {
-        const int i = i0 + wv + 4 * q;
-        if (j >= Nz + 2 || i >= Nx + 2) continue;
-        if (i <= Nx) {  // U-type (Nz+2, Nx+1)
-            const int z = j - 1 < 0 ? 0 : (j - 1 >= Nz ? Nz - 1 : j - 1);
-            a.xieu[j + (int64_t)i * a.ldeu] = psiec(z + 1, i) - psiec(z, i);
-            const double mu = a.muu[j + (int64_t)i * a.ldmuu] - 0.4;
-            const double tap = (1.0 - exp(-a.chiu[j + (int64_t)i * a.ldchiu] / 1e-4)) *
-                               (1.0 - exp(-(mu > 0.0 ? mu : 0.0) / 0.05));
-            a.xixu[j + (int64_t)i * a.ldxu] = (-(ext(1, j, i + 1) - ext(1, j, i))) * tap;
-            a.xisu[j + (int64_t)i * a.ldsu] = (-(ext(2, j, i + 1) - ext(2, j, i))) * tap;
-        }
-        if (j <= Nz) {  // W-type (Nz+1, Nx+2)
-            const int x = i - 1 < 0 ? Nx - 1 : (i - 1 >= Nx ? 0 : i - 1);
-            a.xiew[j + (int64_t)i * a.ldew] = -(psiec(j, x + 1) - psiec(j, x));
-            const double mu = a.muw[j + (int64_t)i * a.ldmuw] - 0.4;
-            const double tap = (1.0 - exp(-a.chiw[j + (int64_t)i * a.ldchiw] / 1e-4)) *
-                               (1.0 - exp(-(mu > 0.0 ? mu : 0.0) / 0.05));
-            a.xixw[j + (int64_t)i * a.ldxw] = (-(ext(1, j + 1, i) - ext(1, j, i))) * tap;
-            a.xisw[j + (int64_t)i * a.ldsw] = (-(ext(2, j + 1, i) - ext(2, j, i))) * tap;
-        }
+    if (i <= Nx) {  // U-type (Nz+2, Nx+1)
+        const int z = j - 1 < 0 ? 0 : (j - 1 >= Nz ? Nz - 1 : j - 1);
+        a.xieu[j + (int64_t)i * a.ldeu] = psiec(z + 1, i) - psiec(z, i);
+        const double mu = a.muu[j + (int64_t)i * a.ldmuu] - 0.4;
+        const double tap = (1.0 - exp(-a.chiu[j + (int64_t)i * a.ldchiu] / 1e-4)) *
+                           (1.0 - exp(-(mu > 0.0 ? mu : 0.0) / 0.05));
+        a.xixu[j + (int64_t)i * a.ldxu] = (-(ext(1, j, i + 1) - ext(1, j, i))) * tap;
+        a.xisu[j + (int64_t)i * a.ldsu] = (-(ext(2, j, i + 1) - ext(2, j, i))) * tap;
+    }
+    if (j <= Nz) {  // W-type (Nz+1, Nx+2)
+        const int x = i - 1 < 0 ? Nx - 1 : (i - 1 >= Nx ? 0 : i - 1);
+        a.xiew[j + (int64_t)i * a.ldew] = -(psiec(j, x + 1) - psiec(j, x));
+        const double mu = a.muw[j + (int64_t)i * a.ldmuw] - 0.4;
+        const double tap = (1.0 - exp(-a.chiw[j + (int64_t)i * a.ldchiw] / 1e-4)) *
+                           (1.0 - exp(-(mu > 0.0 ? mu : 0.0) / 0.05));
+        a.xixw[j + (int64_t)i * a.ldxw] = (-(ext(1, j + 1, i) - ext(1, j, i))) * tap;
+        a.xisw[j + (int64_t)i * a.ldsw] = (-(ext(2, j + 1, i) - ext(2, j, i))) * tap;
     }
 }
 
@@ -885,9 +867,8 @@ int xc_noise_step(xc_ctx* ctx, xc_noise* ns, const xc_phase* ph, const xc_grid* 
     fa.xiew = ns->xiew.p; fa.ldew = ns->xiew.ld; fa.xieu = ns->xieu.p; fa.ldeu = ns->xieu.ld;
     fa.xixw = ns->xixw.p; fa.ldxw = ns->xixw.ld; fa.xixu = ns->xixu.p; fa.ldxu = ns->xixu.ld;
     fa.xisw = ns->xisw.p; fa.ldsw = ns->xisw.ld; fa.xisu = ns->xisu.p; fa.ldsu = ns->xisu.ld;
-    fa.Nz = (int)Nz; fa.Nx = (int)Nx;
-    const unsigned nft = (unsigned)(((Nz + 2 + 63) / 64) * ((Nx + 2 + NFX - 1) / NFX));
-    hipLaunchKernelGGL(k_noise_flux, dim3(nft), dim3(NT), 0, ctx->stream, fa);
+    fa.Nz = (int)Nz; fa.Nx = (int)Nx; fa.d = xc_make_dom(Nz + 2, Nx + 2);
+    hipLaunchKernelGGL(k_noise_flux, dim3(fa.d.nb), dim3(NT), 0, ctx->stream, fa);
     hipLaunchKernelGGL(k_noise_mag, dim3(d.nb), dim3(NT), 0, ctx->stream, ns->xisw.p, ns->xisw.ld, ns->xisu.p,
                        ns->xisu.ld, ns->xixw.p, ns->xixw.ld, ns->xixu.p, ns->xixu.ld, ns->xiew.p, ns->xiew.ld,
                        ns->xieu.p, ns->xieu.ld, ns->xis.p, ns->xis.ld, ns->xix.p, ns->xix.ld, ns->xie.p, ns->xie.ld,
