@@ -1181,19 +1181,6 @@ __global__ __launch_bounds__(64) void k_eta_rho_final(const double* __restrict__
     rmin = wave_min(rmin);
     if (threadIdx.x == 0) { out[0] = emax; out[1] = rmin; }
 }
-__global__ __launch_bounds__(NT) void k_scale(const double* __restrict__ a, double s, double* __restrict__ b, int64_t n) {
-    const int64_t i = gidx();
-    if (i < n) b[i] = a[i] * s;
-}
-// x += sum_q y[q] Z[q]
-__global__ __launch_bounds__(NT) void k_update_x(const double* __restrict__ Z, int64_t n, int K,
-                                                 const double* __restrict__ y, double* __restrict__ x) {
-    const int64_t i = gidx();
-    if (i >= n) return;
-    double s = x[i];
-    for (int q = 0; q < K; ++q) s += y[q] * Z[(int64_t)q * n + i];
-    x[i] = s;
-}
 // ---- learned coarse correction: restriction to the network's grid and prolongation back ----
 // Cell-centred (W, U, P) of a stacked vector at interior cell (r, c), 0-based
 __device__ __forceinline__ void cell_vals(const LevD& L, const double* v, int r, int c, double* w, double* u,
@@ -1325,8 +1312,13 @@ __global__ __launch_bounds__(NT) void k_lc_prolong(LcProArgs a) {
 struct Kry {
     double *H, *cs, *sn, *g, *y;
 };
-__global__ void k_kry_init(Kry K, int m, double beta) {
-    for (int i = threadIdx.x; i <= m; i += blockDim.x) K.g[i] = i == 0 ? beta : 0.0;
+// the start of a cycle: v_0 = r / beta, and g = beta e_0 (by the first workgroup)
+__global__ __launch_bounds__(NT) void k_cycle_start(const double* __restrict__ r, double inv_beta,
+                                                    double* __restrict__ V, int64_t n, Kry K, int m, double beta) {
+    const int64_t i = gidx();
+    if (i < n) V[i] = r[i] * inv_beta;
+    if (blockIdx.x == 0)
+        for (int q = threadIdx.x; q <= m; q += NT) K.g[q] = q == 0 ? beta : 0.0;
 }
 // ---- Arnoldi with delayed reorthogonalisation (DCGS2, classical Gram-Schmidt twice with the second
 // pass of v_j folded into step j+1: Bielich, Langou, Thomas, Swirydowicz, Yamazaki, Boman, "Low-synch
@@ -1412,25 +1404,6 @@ int md_ept(int64_t n) {
 int64_t md_blocks(int64_t n) {
     const int64_t c = (int64_t)md_ept(n) * NT;
     return (n + c - 1) / c;
-}
-// rotate column Hc (length j + 2, entry j + 1 = the subdiagonal) by the rotations 0..j-1, then form
-// rotation j and apply it to g; Hc[j + 1] becomes 0
-__device__ void givens_column(Kry K, int j, double* Hc) {
-    for (int i = 0; i < j; ++i) {
-        const double a = Hc[i], b = Hc[i + 1];
-        Hc[i] = K.cs[i] * a + K.sn[i] * b;
-        Hc[i + 1] = -K.sn[i] * a + K.cs[i] * b;
-    }
-    const double a = Hc[j], b = Hc[j + 1];
-    const double rr = sqrt(a * a + b * b);
-    const double c = rr > 0 ? a / rr : 1.0, sv = rr > 0 ? b / rr : 0.0;
-    K.cs[j] = c;
-    K.sn[j] = sv;
-    Hc[j] = rr;
-    Hc[j + 1] = 0.0;
-    const double gj = K.g[j];
-    K.g[j + 1] = -sv * gj;
-    K.g[j] = c * gj;
 }
 // part: k_multidot2's nb partials of each row of step j's reduction; their sums d (a[0..j), b[0..j),
 // p.p, p.w, w.w) are formed here (in a fixed order) and stored for the update kernel.  cf: 1/rho, h_jj,
@@ -1552,11 +1525,6 @@ __global__ __launch_bounds__(ARN_NT) void k_arnoldi2(Kry K, int m, int j, const 
     cf[2] = nu > 0.0 ? 1.0 / nu : 0.0;
     cf[3] = nu;
 }
-// the last column of a full cycle, with its provisional subdiagonal as final
-__global__ void k_arnoldi_close(Kry K, int m, int j) {
-    if (threadIdx.x != 0) return;
-    givens_column(K, j, K.H + (int64_t)j * (m + 1));
-}
 // v_j = (p_j - sum_q a_q V_q) / rho in place, p_{j+1} = (w - sum_q b_q V_q - h_jj v_j) / nu_{j+1}
 __global__ __launch_bounds__(NT) void k_dcgs_update(const double* __restrict__ V, int64_t n, int K,
                                                     const double* __restrict__ d, const double* __restrict__ cf,
@@ -1574,22 +1542,55 @@ __global__ __launch_bounds__(NT) void k_dcgs_update(const double* __restrict__ V
     vj[i] = v;
     pnext[i] = (w[i] - sb - cf[1] * v) * cf[2];
 }
-// y = R^-1 g for the j x j upper triangle R of H (upper triangular after the rotations): the wave
-// stages R and g in LDS with every load in flight, then lane 0 back-substitutes (j <= ARN_MMAX)
-__global__ __launch_bounds__(64) void k_trisolve(Kry K, int m, int j) {
-    __shared__ double R[ARN_MMAX * ARN_MMAX], gy[ARN_MMAX];
-    for (int t = threadIdx.x; t < j * j; t += 64) {
-        const int i = t % j, k = t / j;
-        R[t] = i <= k ? K.H[i + (int64_t)k * (m + 1)] : 0.0;
-    }
-    for (int i = threadIdx.x; i < j; i += 64) gy[i] = K.g[i];
+// The end of a cycle in one kernel: every workgroup forms y = R^-1 g for the used x used upper triangle R of H
+// after the rotations, then adds Z y to x over its elements.  close: column used-1 still holds its provisional
+// values (the last column of a full cycle, or a cycle that converged before its next step was enqueued) and is
+// closed first: rotated by rotations 0..used-2, rotation used-1 formed from its provisional subdiagonal,
+// g[used-1] scaled by its cosine.  Every workgroup runs the same operations in the same
+// order, so all hold the same y; the rotations and g are not written back (the next cycle starts from the true
+// residual).  Back substitution by columns on one wave (used <= ARN_MMAX = 64): lane k holds g_k;
+// y_i = g_i / R_ii, then g_k -= R_ki y_i for k < i.  Dynamic LDS: R packed by columns (column k, rows 0..k, at
+// k (k+1) / 2), then y.
+__global__ __launch_bounds__(NT) void k_update_x_solve(Kry K, int m, int used, int close,
+                                                       const double* __restrict__ Z, int64_t n,
+                                                       double* __restrict__ x) {
+    extern __shared__ double sh[];
+    __shared__ double s_c;
+    double* R = sh;
+    double* y = sh + used * (used + 1) / 2;
+    for (int k = 0; k < used; ++k)
+        for (int i = threadIdx.x; i <= k; i += NT) R[k * (k + 1) / 2 + i] = K.H[i + (int64_t)k * (m + 1)];
+    if (threadIdx.x == 0) s_c = 1.0;
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    for (int i = j - 1; i >= 0; --i) {
-        double sum = gy[i];
-        for (int k = i + 1; k < j; ++k) sum -= R[i + k * j] * gy[k];
-        gy[i] = sum / R[i + i * j];
-        K.y[i] = gy[i];
+    if (close && threadIdx.x == 0) {
+        const int j = used - 1;
+        double* Hc = R + j * (j + 1) / 2;
+        for (int i = 0; i < j; ++i) {
+            const double a = Hc[i], b = Hc[i + 1];
+            Hc[i] = K.cs[i] * a + K.sn[i] * b;
+            Hc[i + 1] = -K.sn[i] * a + K.cs[i] * b;
+        }
+        const double a = Hc[j], b = K.H[(j + 1) + (int64_t)j * (m + 1)];
+        const double rr = sqrt(a * a + b * b);
+        Hc[j] = rr;
+        s_c = rr > 0 ? a / rr : 1.0;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        double g = lane < used ? K.g[lane] : 0.0;
+        if (lane == used - 1) g *= s_c;
+        for (int i = used - 1; i >= 0; --i) {
+            const double yi = __shfl(g, i) / R[i * (i + 1) / 2 + i];
+            if (lane == 0) y[i] = yi;
+            if (lane < i) g -= R[i * (i + 1) / 2 + lane] * yi;
+        }
+    }
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+        double sx = x[i];
+        for (int q = 0; q < used; ++q) sx += y[q] * Z[(int64_t)q * n + i];
+        x[i] = sx;
     }
 }
 __global__ __launch_bounds__(NT) void k_zero(double* a, int64_t n) {
@@ -2646,8 +2647,7 @@ int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int 
         if (!isfinite(beta)) break;  // not converged; a learned solve falls back to the plain one
         if (beta <= target) { conv = true; break; }
         if (iters >= maxit) break;
-        hipLaunchKernelGGL(k_scale, dim3(nb), dim3(NT), 0, ctx->stream, s->rs, 1.0 / beta, s->V, n);
-        hipLaunchKernelGGL(k_kry_init, dim3(1), dim3(64), 0, ctx->stream, K, m, beta);
+        hipLaunchKernelGGL(k_cycle_start, dim3(nb), dim3(NT), 0, ctx->stream, s->rs, 1.0 / beta, s->V, n, K, m, beta);
         int used = 0;  // basis columns whose results are final
         int j = 0;
         bool waited = false;  // step j - 1 was waited for before step j was enqueued
@@ -2703,16 +2703,21 @@ int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int 
             ++iters;
             if (hout[2 * (j - 1)] <= 0.5 * target || hout[2 * (j - 1) + 1] == 0.0 || iters >= maxit) break;
         }
-        if (early) {  // column j - 1 keeps its provisional values, as the last column of a full cycle does
-            hipLaunchKernelGGL(k_arnoldi_close, dim3(1), dim3(64), 0, ctx->stream, K, m, j - 1);
-        } else if (j == m) {  // the last step of the cycle was not tested yet; its column keeps the provisional values
+        // column used - 1 keeps its provisional values when the cycle converged before its next step was
+        // enqueued (early) or when the cycle is full (its last step was not tested yet)
+        bool close = early;
+        if (!early && j == m) {
             XC_TRY_HIP(ctx, hipEventSynchronize(s->ev[(m - 1) & 1]));
             used = m;
             ++iters;
-            hipLaunchKernelGGL(k_arnoldi_close, dim3(1), dim3(64), 0, ctx->stream, K, m, m - 1);
+            close = true;
         }
-        hipLaunchKernelGGL(k_trisolve, dim3(1), dim3(64), 0, ctx->stream, K, m, used);
-        hipLaunchKernelGGL(k_update_x, dim3(nb), dim3(NT), 0, ctx->stream, s->Z, n, used, K.y, x);
+        if (used > 0) {
+            const unsigned gu = nb < 1024u ? nb : 1024u;
+            const size_t lds = ((size_t)used * (used + 1) / 2 + (size_t)used) * sizeof(double);
+            hipLaunchKernelGGL(k_update_x_solve, dim3(gu), dim3(NT), lds, ctx->stream, K, m, used, close ? 1 : 0,
+                               s->Z, n, x);
+        }
     }
     out->iters = iters;
     out->conv = conv;
