@@ -690,8 +690,11 @@ int xc_phsevo(xc_ctx* ctx, xc_phase* ph, xc_fluid* fl, const xc_grid* g, const x
     // phsevo.jl:177-188: face views of the phase velocities, two WENO advections with fluxes
     xc_mat Wx = sub(ph->Wx, 0, 1, Nz + 1, Nx), Wm = sub(ph->Wm, 0, 1, Nz + 1, Nx);
     xc_mat Ux = sub(ph->Ux, 1, 0, Nz, Nx + 1), Um = sub(ph->Um, 1, 0, Nz, Nx + 1);
-    st = xc_advect_centered(ctx, scheme, ph->advn_X, &ph->qx_advn_X, &ph->qz_advn_X, ph->X, Ux, Wx, g->h, xbc, zbc);
-    if (!st) st = xc_advect_centered(ctx, scheme, ph->advn_M, &ph->qx_advn_M, &ph->qz_advn_M, ph->M, Um, Wm, g->h, xbc, zbc);
+    {
+        const xc_adv_member pm[2] = {{ph->advn_X, ph->qx_advn_X, ph->qz_advn_X, ph->X, Ux, Wx, g->h},
+                                     {ph->advn_M, ph->qx_advn_M, ph->qz_advn_M, ph->M, Um, Wm, g->h}};
+        st = xc_advect_centered_pair(ctx, scheme, pm, xbc, zbc, "phsevo!");
+    }
     // phsevo.jl:192-194: diffusion of chi with k = X .* kx
     if (!st) st = xc_diffus_centered_kprod(ctx, ph->dffn_X, &ph->qx_dffn_X, &ph->qz_dffn_X, ph->chi, ph->X, ph->kx, g->h, xbc, zbc);
     ctx->flags = saved;

@@ -35,6 +35,10 @@ enum {
 int xc_fail(xc_ctx* ctx, int code, const char* fmt, ...);
 // After enqueueing kernels: report launch errors; synchronize in XC_CTX_SYNC mode.
 int xc_finish(xc_ctx* ctx, const char* what);
+// two centred advections of one scheme in one launch (both with fluxes or both without), by value: the results
+// of two xc_advect_centered calls; name: the caller, for error messages
+extern "C" int xc_advect_centered_pair(xc_ctx* ctx, int scheme, const xc_adv_member* m, int xbc, int zbc,
+                                       const char* name);
 // Device scratch of at least `bytes` in `slot` (contents undefined).
 int xc_scratch(xc_ctx* ctx, int slot, size_t bytes, void** out);
 

@@ -700,6 +700,40 @@ __global__ __launch_bounds__(NT) void k_advect_batch(const AdvArgs* __restrict__
     advect_tile_any<S, false, FLUX, FAST, BZ, BX>(a, b - first[lo]);
 }
 
+// Two grids' tiles in one launch, their arguments by value (phsevo!'s advections of X and M, fluidmech!'s
+// momentum advections of rho W and rho U): workgroup b takes tile b of the first grid or tile b - a0.nb of the
+// second; the same tile body as k_advect, hence the same results.
+template <int S, bool FLUX, bool FAST, int BZ, int BX>
+__global__ __launch_bounds__(NT) void k_advect_pair(AdvArgs a0, AdvArgs a1) {
+    const int b = xcd_tile(blockIdx.x, a0.nb + a1.nb);
+    if (b < a0.nb) advect_tile_any<S, false, FLUX, FAST, BZ, BX>(a0, b);
+    else advect_tile_any<S, false, FLUX, FAST, BZ, BX>(a1, b - a0.nb);
+}
+template <int S, bool FLUX>
+void launch_advect_pair_t(const AdvArgs& a0, const AdvArgs& a1, bool exact, hipStream_t st) {
+    constexpr int BZ = VAR_TZ[2], BX = VAR_TX[2];
+    const dim3 grid((unsigned)(a0.nb + a1.nb));
+    if constexpr (S == XC_WENO5) {
+        if (!exact) {
+            hipLaunchKernelGGL((k_advect_pair<S, FLUX, true, BZ, BX>), grid, dim3(NT), 0, st, a0, a1);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_advect_pair<S, FLUX, false, BZ, BX>), grid, dim3(NT), 0, st, a0, a1);
+}
+template <bool FLUX>
+void launch_advect_pair_s(int scheme, const AdvArgs& a0, const AdvArgs& a1, bool exact, hipStream_t st) {
+    switch (scheme) {
+        case XC_CENTR: launch_advect_pair_t<XC_CENTR, FLUX>(a0, a1, exact, st); break;
+        case XC_UPWD1: launch_advect_pair_t<XC_UPWD1, FLUX>(a0, a1, exact, st); break;
+        case XC_QUICK: launch_advect_pair_t<XC_QUICK, FLUX>(a0, a1, exact, st); break;
+        case XC_FROMM: launch_advect_pair_t<XC_FROMM, FLUX>(a0, a1, exact, st); break;
+        case XC_WENO3: launch_advect_pair_t<XC_WENO3, FLUX>(a0, a1, exact, st); break;
+        case XC_WENO5: launch_advect_pair_t<XC_WENO5, FLUX>(a0, a1, exact, st); break;
+        default: launch_advect_pair_t<XC_TVDIM, FLUX>(a0, a1, exact, st); break;
+    }
+}
+
 template <int S, bool HIN, bool FLUX, bool FAST, int V>
 hipError_t launch_advect_v(const AdvArgs& a0, hipStream_t s) {
     constexpr int BZ = VAR_TZ[V], BX = VAR_TX[V];
@@ -1289,6 +1323,45 @@ int xc_advect_set_tile(xc_ctx* ctx, int scheme, int with_flux, int64_t Nz, int64
     if (te) te->v = v;
     else tune_ctx(ctx)->e.push_back(TuneEntry{scheme, with_flux ? 1 : 0, ex, (int)Nz, (int)Nx, v, {}});
     return XC_OK;
+}
+
+int xc_advect_centered_pair(xc_ctx* ctx, int scheme, const xc_adv_member* m, int xbc, int zbc, const char* name) {
+    XC_CHECK_CTX(ctx);
+    const int g = xc_scheme_halo(scheme);
+    if (g < 0) return xc_fail(ctx, XC_EINVAL_SCHEME, "%s: unsupported scheme = %d", name, scheme);
+    if (xbc != XC_PERIODIC && xbc != XC_CLOSED) return xc_fail(ctx, XC_EINVAL_BC, "%s: unsupported xBC = %d", name, xbc);
+    if (zbc != XC_PERIODIC && zbc != XC_CLOSED) return xc_fail(ctx, XC_EINVAL_BC, "%s: unsupported zBC = %d", name, zbc);
+    const bool flux = m[0].qx.p != nullptr;
+    AdvArgs a[2];
+    for (int i = 0; i < 2; ++i) {
+        const xc_adv_member& q = m[i];
+        const int64_t Nz = q.adv.rows, Nx = q.adv.cols;
+        if (!xc_mat_ok(q.adv) || Nz < g || Nx < g || Nz > (1 << 30) || Nx > (1 << 30) || !xc_shape(q.f, Nz, Nx) ||
+            !xc_shape(q.u, Nz, Nx + 1) || !xc_shape(q.w, Nz + 1, Nx))
+            return xc_fail(ctx, XC_EINVAL_SHAPE, "%s: advection %d has inconsistent shapes", name, i);
+        if ((q.qx.p != nullptr) != flux || (q.qz.p != nullptr) != flux ||
+            (flux && (!xc_shape(q.qx, Nz + 2, Nx + 1) || !xc_shape(q.qz, Nz + 1, Nx + 2))))
+            return xc_fail(ctx, XC_EINVAL_SHAPE, "%s: advection %d flux shapes", name, i);
+        AdvArgs& b = a[i];
+        b = AdvArgs{};
+        b.adv = q.adv.p; b.ld_adv = q.adv.ld;
+        if (flux) { b.qx = q.qx.p; b.ld_qx = q.qx.ld; b.qz = q.qz.p; b.ld_qz = q.qz.ld; }
+        b.f = q.f.p; b.ld_f = q.f.ld;
+        b.u = q.u.p; b.ld_u = q.u.ld;
+        b.w = q.w.p; b.ld_w = q.w.ld;
+        b.Nz = (int)Nz; b.Nx = (int)Nx;
+        b.nbz = (b.Nz + VAR_TZ[2] - 1) / VAR_TZ[2];
+        b.nb = b.nbz * ((b.Nx + VAR_TX[2] - 1) / VAR_TX[2]);
+        b.invh = 1.0 / q.h;
+        b.xbc = xbc; b.zbc = zbc;
+    }
+    if ((int64_t)a[0].nb + a[1].nb > (1 << 30)) return xc_fail(ctx, XC_EINVAL_SHAPE, "%s: too many tiles", name);
+    const bool ex = (ctx->flags & XC_CTX_EXACT) != 0;
+    if (flux) launch_advect_pair_s<true>(scheme, a[0], a[1], ex, ctx->stream);
+    else launch_advect_pair_s<false>(scheme, a[0], a[1], ex, ctx->stream);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return xc_fail(ctx, XC_EHIP, "%s: launch: %s", name, hipGetErrorString(e));
+    return xc_finish(ctx, name);
 }
 
 int xc_advect_centered_batch(xc_ctx* ctx, int scheme, int n, const xc_adv_member* m, int xbc, int zbc) {

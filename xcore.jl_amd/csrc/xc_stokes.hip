@@ -2466,13 +2466,13 @@ int build_rhs(xc_ctx* ctx, StokesCtx* s, xc_fluid* fl, const xc_phase* ph, const
                        fl->rhow, fl->rhou, fz, uz, wz, fx, ux, wx, Nz, Nx);
     const unsigned saved = ctx->flags;
     ctx->flags &= ~XC_CTX_SYNC;
-    int st = xc_advect_centered(ctx, o->scheme, xc_mat{az, Nz - 1, Nx, Nz - 1}, nullptr, nullptr,
-                                xc_mat{fz, Nz - 1, Nx, Nz - 1}, xc_mat{uz, Nz - 1, Nx + 1, Nz - 1},
-                                xc_mat{wz, Nz, Nx, Nz}, g->h, o->xbc, o->zbc);
-    if (!st)
-        st = xc_advect_centered(ctx, o->scheme, xc_mat{ax, Nz, Nx + 1, Nz}, nullptr, nullptr,
-                                xc_mat{fx, Nz, Nx + 1, Nz}, xc_mat{ux, Nz, Nx + 2, Nz}, xc_mat{wx, Nz + 1, Nx + 1, Nz + 1},
-                                g->h, o->xbc, o->zbc);
+    const xc_mat none{nullptr, 0, 0, 0};
+    const xc_adv_member mm[2] = {
+        {xc_mat{az, Nz - 1, Nx, Nz - 1}, none, none, xc_mat{fz, Nz - 1, Nx, Nz - 1}, xc_mat{uz, Nz - 1, Nx + 1, Nz - 1},
+         xc_mat{wz, Nz, Nx, Nz}, g->h},
+        {xc_mat{ax, Nz, Nx + 1, Nz}, none, none, xc_mat{fx, Nz, Nx + 1, Nz}, xc_mat{ux, Nz, Nx + 2, Nz},
+         xc_mat{wx, Nz + 1, Nx + 1, Nz + 1}, g->h}};
+    int st = xc_advect_centered_pair(ctx, o->scheme, mm, o->xbc, o->zbc, "fluidmech!");
     ctx->flags = saved;
     if (st) return st;
     RRArgs a{};
