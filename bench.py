@@ -345,7 +345,7 @@ def ensemble_bench(reps=20, seed=4001):
             "launches_per_step": {"batched": 3, "per_member": 192}}
 
 
-def ensemble_steps_bench(n=64, steps=2, threads=16, seed=7):
+def ensemble_steps_bench(n=64, steps=2, threads=4, seed=7):
     """Config-4 ensemble of whole time steps: n members cycling through 128^2, 256^2 and 384^2, each with its own
     noise seed, density contrast (+-20 %) and crystal viscosity 10^e (e uniform in 20..30), each member one full
     step of run! per ensemble step (adaptive Picard sweeps, FGMRES to 1e-8).  The members' steps run

@@ -20,7 +20,9 @@ extern "C" int xc_ensemble_run_step(int n, xc_ensemble_member* members, int nthr
         for (int j = 0; j < i; ++j)
             if (members[i].ctx == members[j].ctx)
                 return xc_fail(nullptr, XC_EINVAL_ARG, "ensemble: members %d and %d share a context", j, i);
-    const int nt = nthreads < 1 ? 1 : (nthreads > n ? (n > 0 ? n : 1) : nthreads);
+    // default: one thread per hardware queue (4 by default); more threads only contend for the same queues
+    const int want = nthreads < 1 ? 4 : nthreads;
+    const int nt = want > n ? (n > 0 ? n : 1) : want;
     std::atomic<int> next{0};
     auto work = [&]() {
         for (int i = next.fetch_add(1); i < n; i = next.fetch_add(1)) {

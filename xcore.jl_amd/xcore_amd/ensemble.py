@@ -160,9 +160,10 @@ def ensemble_members(n, sizes=(128, 256, 384), seed=0, base=None, drho_span=0.2,
     return [RunMember(p, g, s, seed=p.seed) for p, g, s in zip(pars, grids, scales)]
 
 
-def ensemble_run_step_(members, opts, nthreads=16):
+def ensemble_run_step_(members, opts, nthreads=0):
     """One step of run! for every member (xc_ensemble_run_step): the members' work runs concurrently on the
-    device.  opts: one StepOpts for all members or a list.  Returns the members' step statistics."""
+    device (nthreads host threads; 0: the library's default, 4).  opts: one StepOpts for all members or a list.
+    Returns the members' step statistics."""
     from .driver import StepStats
     from .state import grid_struct, par_struct, scales_struct
     n = len(members)
