@@ -15,8 +15,9 @@
 //   2^-54 of the kernel's mass, run instead as direct convolutions with the
 //   2R+1 taps of the same circular convolution (k_gconv_x for R <= 43,
 //   k_gconv_z for R <= 128; lines of at least 64 points).
-// * Mean/std rescale (noise.jl:261-267): two-pass reductions (mean, then the
-//   centred sum of squares) over all six raw/filtered fields.
+// * Mean/std rescale (noise.jl:261-267): one pass over all six raw/filtered
+//   fields summing x - x[0] and its square (the shift keeps the one-pass
+//   variance free of cancellation), then one wave per field.
 // * Flux differencing, tapers and magnitudes (noise.jl:195-236) as two
 //   elementwise kernels that apply the rescale on the fly.
 #include <math.h>
@@ -394,43 +395,51 @@ struct StatArgs {
     int64_t ld[6];
     int Nz, Nx;
     xc_dom d;
-    double* part;   // [6][nb]
-    double* stats;  // [6] means, [6] sums of squares -> see k_stat_final
+    double* part;   // [12][nb]: sums of d = x - x[0], then of d^2
+    double* stats;  // [6] means, [6] stds (n - 1)
 };
-template <bool CENTRED>
+// One pass over the six arrays: per workgroup, the sums of d = x - x[0] and of d^2.  The shift by the array's
+// first element keeps the one-pass variance below free of cancellation where the mean is large against the
+// spread (the two-pass form read every array twice).
 __global__ __launch_bounds__(NT) void k_stat_partial(StatArgs s) {
     int j, i;
     xc_dom_ij(s.d, &j, &i);
     const bool in = j < s.Nz && i < s.Nx;
-    __shared__ double red[6][NT / 64];
+    __shared__ double red[12][NT / 64];
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
-        double v = 0.0;
-        if (in) {
-            v = s.a[q][j + (int64_t)i * s.ld[q]];
-            if (CENTRED) {
-                v -= s.stats[q];
-                v = v * v;
-            }
+        const double K = s.a[q][0];
+        const double d = in ? s.a[q][j + (int64_t)i * s.ld[q]] - K : 0.0;
+        const double d1 = wave_total(d);  // DPP wave sums, then the NT / 64 wave sums
+        const double d2 = wave_total(d * d);
+        if ((threadIdx.x & 63) == 0) {
+            red[q][threadIdx.x >> 6] = d1;
+            red[6 + q][threadIdx.x >> 6] = d2;
         }
-        v = wave_total(v);  // DPP wave sum, then the NT / 64 wave sums
-        if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = v;
     }
     __syncthreads();
-    if (threadIdx.x < 6) {
+    if (threadIdx.x < 12) {
         double t = 0.0;
 #pragma unroll
         for (int w = 0; w < NT / 64; ++w) t += red[threadIdx.x][w];
         s.part[(int64_t)threadIdx.x * s.d.nb + blockIdx.x] = t;
     }
 }
-// sums partials; mode 0: stats[q] = mean; mode 1: stats[6+q] = std (n-1).  One wave per statistic.
-__global__ __launch_bounds__(6 * 64) void k_stat_final(const double* part, int nb, double n, int mode, double* stats) {
+// stats[q] = mean = x[0] + sum d / n, stats[6 + q] = std (n - 1) = sqrt((sum d^2 - (sum d)^2 / n) / (n - 1))
+// (0 where that is not positive; noise.jl:261-267 rescales only where the std exceeds eps).  One wave per row of
+// partials.
+__global__ __launch_bounds__(12 * 64) void k_stat_final(StatArgs s, double n) {
+    __shared__ double t[12];
     const int q = threadIdx.x >> 6;
-    const double t = xc_rows_reduce<false>(part, nb, q);
-    if ((threadIdx.x & 63) == 0) {
-        if (mode == 0) stats[q] = t / n;
-        else stats[6 + q] = sqrt(t / (n - 1.0));
+    const double v = xc_rows_reduce<false>(s.part, s.d.nb, q);
+    if ((threadIdx.x & 63) == 0) t[q] = v;
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int f = threadIdx.x;
+        const double t1 = t[f], t2 = t[6 + f];
+        s.stats[f] = s.a[f][0] + t1 / n;
+        const double var = (t2 - t1 * t1 / n) / (n - 1.0);
+        s.stats[6 + f] = sqrt(var > 0.0 ? var : 0.0);
     }
 }
 
@@ -751,7 +760,7 @@ int xc_fft_filter(xc_ctx* ctx, xc_mat psi, xc_mat out, int64_t pad, double h, do
     const xc_dom d = xc_make_dom(Nz, Nx);
     const size_t ncell = (size_t)Nz * Nx;
     int st;
-    if ((st = xc_scratch(ctx, XC_SLOT_NOISE, (ncell + 6 * (size_t)d.nb + 12) * sizeof(double), (void**)&sc))) return st;
+    if ((st = xc_scratch(ctx, XC_SLOT_NOISE, (ncell + 12 * (size_t)d.nb + 12) * sizeof(double), (void**)&sc))) return st;
     xc_mat tmp{sc, Nz, Nx, Nz};
     if ((st = filter_fields(ctx, 1, &psi, &out, &tmp, &pad, h, &sigma))) return st;
     // statistics over (raw, filtered) pairs; unused slots point at the same arrays
@@ -762,11 +771,9 @@ int xc_fft_filter(xc_ctx* ctx, xc_mat psi, xc_mat out, int64_t pad, double h, do
     }
     sa.Nz = Nz; sa.Nx = Nx; sa.d = d;
     sa.part = sc + ncell;
-    sa.stats = sa.part + 6 * d.nb;
-    hipLaunchKernelGGL(k_stat_partial<false>, dim3(d.nb), dim3(NT), 0, ctx->stream, sa);
-    hipLaunchKernelGGL(k_stat_final, dim3(1), dim3(6 * 64), 0, ctx->stream, sa.part, d.nb, (double)ncell, 0, sa.stats);
-    hipLaunchKernelGGL(k_stat_partial<true>, dim3(d.nb), dim3(NT), 0, ctx->stream, sa);
-    hipLaunchKernelGGL(k_stat_final, dim3(1), dim3(6 * 64), 0, ctx->stream, sa.part, d.nb, (double)ncell, 1, sa.stats);
+    sa.stats = sa.part + 12 * d.nb;
+    hipLaunchKernelGGL(k_stat_partial, dim3(d.nb), dim3(NT), 0, ctx->stream, sa);
+    hipLaunchKernelGGL(k_stat_final, dim3(1), dim3(12 * 64), 0, ctx->stream, sa, (double)ncell);
     // rescale to the raw mean/std (field 0's statistics), elementwise in place
     hipLaunchKernelGGL(k_resc_inplace, dim3(d.nb), dim3(NT), 0, ctx->stream, out.p, out.ld, sa.stats, Nz, Nx, d);
     return xc_finish(ctx, "compute_fft_filter");
@@ -835,7 +842,7 @@ int xc_noise_step(xc_ctx* ctx, xc_noise* ns, const xc_phase* ph, const xc_grid* 
     const size_t ncell = (size_t)Nz * Nx;
     double* sb;
     int st;
-    if ((st = xc_scratch(ctx, XC_SLOT_NOISE, (6 * ncell + 6 * (size_t)d.nb + 12) * sizeof(double), (void**)&sb)))
+    if ((st = xc_scratch(ctx, XC_SLOT_NOISE, (6 * ncell + 12 * (size_t)d.nb + 12) * sizeof(double), (void**)&sb)))
         return st;
     xc_mat in[3] = {ns->psie, ns->psix, ns->psis};
     xc_mat tmp[3], flt[3];
@@ -853,11 +860,9 @@ int xc_noise_step(xc_ctx* ctx, xc_noise* ns, const xc_phase* ph, const xc_grid* 
     }
     sa.Nz = (int)Nz; sa.Nx = (int)Nx; sa.d = d;
     sa.part = sb + 6 * ncell;
-    sa.stats = sa.part + 6 * d.nb;
-    hipLaunchKernelGGL(k_stat_partial<false>, dim3(d.nb), dim3(NT), 0, ctx->stream, sa);
-    hipLaunchKernelGGL(k_stat_final, dim3(1), dim3(6 * 64), 0, ctx->stream, sa.part, d.nb, (double)ncell, 0, sa.stats);
-    hipLaunchKernelGGL(k_stat_partial<true>, dim3(d.nb), dim3(NT), 0, ctx->stream, sa);
-    hipLaunchKernelGGL(k_stat_final, dim3(1), dim3(6 * 64), 0, ctx->stream, sa.part, d.nb, (double)ncell, 1, sa.stats);
+    sa.stats = sa.part + 12 * d.nb;
+    hipLaunchKernelGGL(k_stat_partial, dim3(d.nb), dim3(NT), 0, ctx->stream, sa);
+    hipLaunchKernelGGL(k_stat_final, dim3(1), dim3(12 * 64), 0, ctx->stream, sa, (double)ncell);
     // 4. fluxes and magnitudes (noise.jl:190-236)
     FluxArgs fa{};
     for (int f = 0; f < 3; ++f) { fa.flt[f] = flt[f].p; fa.ldf[f] = flt[f].ld; }
