@@ -177,12 +177,13 @@ int xc_batch_run(xc_ctx* ctx, const xc_batch_plan* plan);
 int xc_batch_destroy(xc_ctx* ctx, xc_batch_plan* plan);  /* synchronizes the context's stream */
 
 /* Tile shape the advection kernels use at this (scheme, fluxes, grid): height tz (z) x width tx (tx < 0:
- * the persistent, software-pipelined 64 x |tx| kernel), and the measured time per launch of every
- * candidate -- 64x16, 128x8, 64x8, 32x32, 64x8 pipelined -- in us[5] (if non-NULL).  No counterpart in the
- * reference (the autotuner of the tile shapes).  XC_ENOCONV: not tuned yet (the default shape). */
+ * the persistent, software-pipelined 64 x |tx| kernel that streams each next halo tile into LDS by DMA),
+ * and the measured time per launch of every candidate -- 64x16, 128x8, 64x8, 32x32, 64x16 pipelined,
+ * 64x8 pipelined -- in us[6] (if non-NULL).  No counterpart in the reference (the autotuner of the tile
+ * shapes).  XC_ENOCONV: not tuned yet (the default shape). */
 int xc_advect_tile(xc_ctx* ctx, int scheme, int with_flux, int64_t Nz, int64_t Nx, int* tz, int* tx, float* us);
 /* pin the tile shape for (scheme, fluxes, grid), e.g. a choice tuned in an earlier run: 64x16, 128x8, 64x8,
- * 32x32, or 64x-8 for the pipelined kernel (not for tvdim) */
+ * 32x32, or 64x-16 / 64x-8 for the pipelined kernels (not for tvdim) */
 int xc_advect_set_tile(xc_ctx* ctx, int scheme, int with_flux, int64_t Nz, int64_t Nx, int tz, int tx);
 
 /* Diagnostic, no counterpart in the reference: the memory traffic of a stencil launch without its
@@ -191,7 +192,7 @@ int xc_advect_set_tile(xc_ctx* ctx, int scheme, int with_flux, int64_t Nz, int64
  * non-temporal stores when nts bit 0 is set -- the bandwidth HBM sustains for that mix of streams, the
  * achievable bound a stencil kernel with the same traffic is measured against.  nts bit 1: the best case of
  * the same traffic, every stream aligned and not shifted, 16-byte accesses (needs even rows and 16-byte
- * aligned columns). */
+ * aligned columns).  nts bit 2: non-temporal loads. */
 int xc_diag_stream(xc_ctx* ctx, int nr, const xc_mat* in, int nw, const xc_mat* out, int nts);
 
 /* ---- field updates -------------------------------------------------------- */

@@ -1,8 +1,9 @@
 // Diagnostic streaming kernel (no reference counterpart): the memory traffic of a stencil launch
 // without its arithmetic, to measure the bandwidth HBM sustains for that mix of streams.  Each of
 // nw outputs receives, at every cell (i, j) of out[0]'s shape, the sum of the nr inputs there;
-// outputs k >= 1 are written one row down (the padded flux arrays' layout).  Plain or
-// non-temporal stores (mode bit 0); mode bit 1: the aligned 16-byte form, outputs not shifted.
+// outputs k >= 1 are written one row down (the padded flux arrays' layout).  Mode bit 0: non-temporal
+// stores (as the stencils use for launches larger than the last-level cache); bit 1: the aligned 16-byte
+// form, outputs not shifted; bit 2: non-temporal loads.
 #include "xc_internal.h"
 
 namespace {
@@ -14,7 +15,7 @@ struct StreamArgs {
     int Nz, Nx, nbz;
 };
 
-template <int NR, int NW, bool NTS>
+template <int NR, int NW, bool NTS, bool NTL>
 __global__ __launch_bounds__(256) void k_stream(StreamArgs a) {
     const int t = blockIdx.x;
     const int iz = (t % a.nbz) * 64 + (threadIdx.x & 63);
@@ -22,7 +23,10 @@ __global__ __launch_bounds__(256) void k_stream(StreamArgs a) {
     if (iz >= a.Nz || ix >= a.Nx) return;
     double s = 0.0;
 #pragma unroll
-    for (int r = 0; r < NR; ++r) s += a.in[r][iz + (int64_t)ix * a.ld_in[r]];
+    for (int r = 0; r < NR; ++r) {
+        const double* p = &a.in[r][iz + (int64_t)ix * a.ld_in[r]];
+        s += NTL ? __builtin_nontemporal_load(p) : *p;
+    }
 #pragma unroll
     for (int w = 0; w < NW; ++w) {
         double* p = &a.out[w][iz + (w ? 1 : 0) + (int64_t)ix * a.ld_out[w]];
@@ -35,7 +39,7 @@ __global__ __launch_bounds__(256) void k_stream(StreamArgs a) {
 // elements a lane as one 16-byte access, 128 rows a wave: what HBM sustains for NR reads and NW writes
 // when every access is a whole, aligned line.
 typedef double dbl2 __attribute__((ext_vector_type(2)));
-template <int NR, int NW, bool NTS>
+template <int NR, int NW, bool NTS, bool NTL>
 __global__ __launch_bounds__(256) void k_stream_vec(StreamArgs a) {
     const int t = blockIdx.x;
     const int iz = (t % a.nbz) * 128 + 2 * (threadIdx.x & 63);
@@ -43,7 +47,10 @@ __global__ __launch_bounds__(256) void k_stream_vec(StreamArgs a) {
     if (iz >= a.Nz || ix >= a.Nx) return;
     dbl2 s = {0.0, 0.0};
 #pragma unroll
-    for (int r = 0; r < NR; ++r) s += *reinterpret_cast<const dbl2*>(a.in[r] + iz + (int64_t)ix * a.ld_in[r]);
+    for (int r = 0; r < NR; ++r) {
+        const dbl2* p = reinterpret_cast<const dbl2*>(a.in[r] + iz + (int64_t)ix * a.ld_in[r]);
+        s += NTL ? __builtin_nontemporal_load(p) : *p;
+    }
 #pragma unroll
     for (int w = 0; w < NW; ++w) {
         dbl2* p = reinterpret_cast<dbl2*>(a.out[w] + iz + (int64_t)ix * a.ld_out[w]);
@@ -52,20 +59,25 @@ __global__ __launch_bounds__(256) void k_stream_vec(StreamArgs a) {
     }
 }
 
-template <int NR, int NW>
-void launch_nr_nw(const StreamArgs& a, int mode, hipStream_t st) {
+template <int NR, int NW, bool NTL>
+void launch_nr_nw_l(const StreamArgs& a, int mode, hipStream_t st) {
     const bool nts = mode & 1;
     if (mode & 2) {
         StreamArgs b = a;
         b.nbz = (a.Nz + 127) / 128;
         const dim3 grid((unsigned)(b.nbz * ((a.Nx + 3) / 4)));
-        if (nts) hipLaunchKernelGGL((k_stream_vec<NR, NW, true>), grid, dim3(256), 0, st, b);
-        else hipLaunchKernelGGL((k_stream_vec<NR, NW, false>), grid, dim3(256), 0, st, b);
+        if (nts) hipLaunchKernelGGL((k_stream_vec<NR, NW, true, NTL>), grid, dim3(256), 0, st, b);
+        else hipLaunchKernelGGL((k_stream_vec<NR, NW, false, NTL>), grid, dim3(256), 0, st, b);
         return;
     }
     const dim3 grid((unsigned)(a.nbz * ((a.Nx + 3) / 4)));
-    if (nts) hipLaunchKernelGGL((k_stream<NR, NW, true>), grid, dim3(256), 0, st, a);
-    else hipLaunchKernelGGL((k_stream<NR, NW, false>), grid, dim3(256), 0, st, a);
+    if (nts) hipLaunchKernelGGL((k_stream<NR, NW, true, NTL>), grid, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_stream<NR, NW, false, NTL>), grid, dim3(256), 0, st, a);
+}
+template <int NR, int NW>
+void launch_nr_nw(const StreamArgs& a, int mode, hipStream_t st) {
+    if (mode & 4) launch_nr_nw_l<NR, NW, true>(a, mode, st);
+    else launch_nr_nw_l<NR, NW, false>(a, mode, st);
 }
 
 }  // namespace

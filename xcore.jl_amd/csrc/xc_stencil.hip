@@ -29,10 +29,10 @@ constexpr int TZ = 64;   // default tile height (z, contiguous)
 constexpr int TX = 16;   // default tile width (x)
 constexpr int NT = 256;  // lanes per workgroup
 // advection tile shapes the autotuner chooses from (variant 0 is the default)
-constexpr int NVAR = 5;
-constexpr int VAR_TZ[NVAR] = {64, 128, 64, 32, 64};
-constexpr int VAR_TX[NVAR] = {16, 8, 8, 32, 8};   // variant 4: 64 x 8, persistent and pipelined
-constexpr int VAR_PIPE = 4;
+constexpr int NVAR = 6;
+constexpr int VAR_TZ[NVAR] = {64, 128, 64, 32, 64, 64};
+constexpr int VAR_TX[NVAR] = {16, 8, 8, 32, 16, 8};   // variants 4, 5: persistent, LDS-DMA pipelined
+constexpr int VAR_PIPE = 4;                           // the first persistent variant
 
 __host__ __device__ constexpr int halo_of(int s) {
     return s == XC_CENTR ? 1 : s == XC_UPWD1 ? 1 : s == XC_WENO5 ? 3 : 2;
@@ -73,11 +73,13 @@ __device__ __forceinline__ double weno5_poly(double fmm, double fm, double fc, d
 // folded into the polynomial coefficients, the nonlinear weights d_k/(b_k+eps)^2 are scaled by
 // prod_k (b_k+eps)^2 so that one quotient remains, and that quotient is a reciprocal refined by two
 // Newton-Raphson steps plus one residual correction of the product (the denominator is a positive
-// normal number: each b_k + eps >= eps).  FMA contraction allowed.  The (b_k + eps) are first scaled by
-// the power of two that brings the largest into [0.5, 1): the quotient is homogeneous of degree 0 in
-// them and a power-of-two scaling is exact, so the result has the same bits as unscaled wherever the
-// unscaled products stay finite -- and the products of four squares (~ |df|^8) no longer overflow for
-// fields of magnitude ~1e38 and above, up to the ~1e77 where the reference's own weights overflow.
+// normal number: each b_k + eps >= eps).  FMA contraction allowed.  When the largest (b_k + eps) exceeds
+// 2^200 they are first scaled by the power of two that brings the largest into [0.5, 1): the quotient is
+// homogeneous of degree 0 in them and a power-of-two scaling is exact (the hardware reciprocal depends
+// on the mantissa only), so the result has the same bits as unscaled wherever the unscaled products stay
+// finite -- and the products of four squares (~ |df|^8) no longer overflow for fields of magnitude ~1e38
+// and above, up to the ~1e77 where the reference's own weights overflow.  Below 2^200 the products stay
+// under 2^800 and the scaling is skipped (the same bits).
 __device__ __forceinline__ double weno5_poly_fast(double fmm, double fm, double fc, double fp, double fpp) {
 #pragma clang fp contract(fast)
     const double e = 2.220446049250313e-16;
@@ -91,9 +93,14 @@ __device__ __forceinline__ double weno5_poly_fast(double fmm, double fm, double 
     const double b1 = (tt * s1) * s1 + ((0.25 * t1) * t1 + e);
     const double b2 = (tt * s2) * s2 + ((0.25 * t2) * t2 + e);
     const double b3 = (tt * s3) * s3 + ((0.25 * t3) * t3 + e);
-    const int ex = __builtin_amdgcn_frexp_exp(fmax(b1, fmax(b2, b3)));
-    const double a1 = __builtin_amdgcn_ldexp(b1, -ex), a2 = __builtin_amdgcn_ldexp(b2, -ex),
-                 a3 = __builtin_amdgcn_ldexp(b3, -ex);
+    double a1 = b1, a2 = b2, a3 = b3;
+    const double bm = fmax(b1, fmax(b2, b3));
+    if (bm > 0x1p200) {  // products of four squares would pass 2^800: rescale (rare; exact either way)
+        const int ex = __builtin_amdgcn_frexp_exp(bm);
+        a1 = __builtin_amdgcn_ldexp(b1, -ex);
+        a2 = __builtin_amdgcn_ldexp(b2, -ex);
+        a3 = __builtin_amdgcn_ldexp(b3, -ex);
+    }
     const double q1 = a1 * a1, q2 = a2 * a2, q3 = a3 * a3;
     const double q23 = q2 * q3, q13 = q1 * q3, q12 = q1 * q2;
     const double num = q23 * P1 + q13 * P2 + q12 * P3;
@@ -384,21 +391,166 @@ __device__ __forceinline__ void advect_tile(const AdvArgs& a, int t) {
 // takes whole columns (of the f halo tile, of x-faces, of z-faces, of cells), so column indexes and
 // column addresses are wave-uniform (scalar registers), only the row offset is per lane, and every
 // LDS access of a wave touches 64 consecutive doubles.  Same loads, face arithmetic and stores as
-// advect_tile, hence bitwise the same results.
+// advect_tile, hence bitwise the same results.  The tile is split in two parts shared by the one-tile
+// kernel and the persistent one: LrVel holds the face velocities a wave needs (loaded into registers),
+// lr_faces computes and stores the faces and the divergence from an f halo tile already in LDS.
+template <int S, int BX>
+struct LrShape {
+    static constexpr int BZ = 64;
+    static constexpr int G = halo_of(S);
+    static constexpr int PZ = BZ + 2 * G;
+    static constexpr int PX = BX + 2 * G;
+    static constexpr int NWV = NT / 64;                  // waves per workgroup
+    static constexpr int CF = (PX + NWV - 1) / NWV;      // halo-tile columns per wave
+    static constexpr int CU = (BX + 1 + NWV - 1) / NWV;  // x-face columns per wave
+    static constexpr int CW = (BX + NWV - 1) / NWV;      // z-face (and cell) columns per wave
+    static constexpr int NV = S == XC_TVDIM ? 3 : 1;     // velocities per face (TVD: left, own, right)
+    static constexpr int VI = S == XC_TVDIM ? 1 : 0;
+    static constexpr int WC = 1 % NWV;                   // the wave that also takes the closing z-faces
+    static_assert(BX <= 64 && NT % 64 == 0, "lane-row tiles");
+};
+
+template <int S, int BX>
+struct LrVel {
+    using L = LrShape<S, BX>;
+    double ru[L::CU][L::NV];  // x-face velocities of the wave's x-face columns, row of the lane
+    double rw[L::CW][L::NV];  // z-face velocities of the wave's cell columns, face row of the lane
+    double rc[L::NV];         // closing z-faces (tile row BZ) of the columns x0 + l, lanes l < BX
+};
+
+// Velocity load; NTL: non-temporal (each velocity is read by one tile only, so the line need not stay cached)
+template <bool NTL>
+__device__ __forceinline__ double get(const double* p) {
+    if constexpr (NTL) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
+template <int S, bool HIN, int BX, bool NTL = false>
+__device__ __forceinline__ void lr_load_vel(const AdvArgs& a, int z0, int x0, int l, int wv, LrVel<S, BX>& v) {
+    using L = LrShape<S, BX>;
+    constexpr int BZ = L::BZ, NWV = L::NWV;
+    const int Nz = a.Nz, Nx = a.Nx;
+    const int izc = min(z0 + l, Nz - 1);
+#pragma unroll
+    for (int j = 0; j < L::CU; ++j) {
+        const int F = min(x0 + min(wv + j * NWV, BX), Nx);
+        if constexpr (S == XC_TVDIM) {
+            int c0 = F, c2 = F + 2, c1 = F + 1;
+            if constexpr (!HIN) {
+                c0 = map_face(F - 1, Nx, a.xbc);
+                c1 = F;
+                c2 = map_face(F + 1, Nx, a.xbc);
+            }
+            v.ru[j][0] = a.u[izc + (int64_t)c0 * a.ld_u];
+            v.ru[j][1] = a.u[izc + (int64_t)c1 * a.ld_u];
+            v.ru[j][2] = a.u[izc + (int64_t)c2 * a.ld_u];
+        } else {
+            v.ru[j][0] = get<NTL>(&a.u[izc + (int64_t)F * a.ld_u]);
+        }
+    }
+    auto wload = [&](double* r, int F, int ix) {  // F: face row, 0..Nz; ix: cell column, 0..Nx-1
+        const double* col = a.w + (int64_t)ix * a.ld_w;
+        if constexpr (S == XC_TVDIM) {
+            if constexpr (HIN) {
+                r[0] = col[F];
+                r[1] = col[F + 1];
+                r[2] = col[F + 2];
+            } else {
+                r[0] = col[map_face(F - 1, Nz, a.zbc)];
+                r[1] = col[F];
+                r[2] = col[map_face(F + 1, Nz, a.zbc)];
+            }
+        } else {
+            r[0] = get<NTL>(&col[F]);
+        }
+    };
+#pragma unroll
+    for (int j = 0; j < L::CW; ++j)
+        wload(v.rw[j], min(z0 + l, Nz), min(x0 + min(wv + j * NWV, BX - 1), Nx - 1));
+    if (wv == L::WC) wload(v.rc, min(z0 + BZ, Nz), min(x0 + min(l, BX - 1), Nx - 1));
+}
+
+// Faces and divergence of the tile at (z0, x0) from the f halo tile fs (PX columns of PZ rows, the
+// BCs already applied) and the velocities v; qxs (BX + 1 columns of BZ) and qzc (BX) are workgroup
+// LDS scratch.  Ends after the last LDS read of fs, qxs and qzc but before any barrier.
+template <int S, bool FLUX, bool FAST, int BX, bool NTS>
+__device__ __forceinline__ void lr_faces(const AdvArgs& a, int z0, int x0, int l, int wv, const double* fs,
+                                         double* qxs, double* qzc, const LrVel<S, BX>& v) {
+    using L = LrShape<S, BX>;
+    constexpr int BZ = L::BZ, G = L::G, PZ = L::PZ, NWV = L::NWV, NV = L::NV, VI = L::VI;
+    const int Nz = a.Nz, Nx = a.Nx;
+    const bool zper = a.zbc == XC_PERIODIC, xper = a.xbc == XC_PERIODIC;
+    const int qx_top = (zper && Nz > 1) ? Nz - 1 : 0;  // source row of qx[0, :]
+    const int qx_bot = (zper && Nz > 1) ? 0 : Nz - 1;  // source row of qx[Nz+1, :]
+    const int qz_lft = (xper && Nx > 1) ? Nx - 1 : 0;
+    const int qz_rgt = (xper && Nx > 1) ? 0 : Nx - 1;
+    const int iz = z0 + l;
+
+    // ---- x-faces: column F = x0 + fx of the wave, row iz of the lane
+#pragma unroll
+    for (int j = 0; j < L::CU; ++j) {
+        const int fx = wv + j * NWV;
+        if (fx <= BX) {
+            const int F = x0 + fx;
+            double q = 0.0;
+            if (F <= Nx && iz < Nz) {
+                q = face_flux<S, FAST>(&fs[(fx + G) * PZ + l + G], PZ, v.ru[j][0], v.ru[j][VI], v.ru[j][NV - 1]);
+                if constexpr (FLUX) {
+                    if (fx < BX || F == Nx) {
+                        double* col = a.qx + (int64_t)F * a.ld_qx;
+                        put<NTS>(&col[iz + 1], q);
+                        if (iz == qx_top) put<NTS>(&col[0], q);
+                        if (iz == qx_bot) put<NTS>(&col[Nz + 1], q);
+                    }
+                }
+            }
+            qxs[fx * BZ + l] = q;
+        }
+    }
+    // ---- z-faces: face row F = z0 + l of the wave's column ix
+    auto zface = [&](const double* r, int lx, int fz) -> double {
+        const int F = z0 + fz, ix = x0 + lx;
+        double q = 0.0;
+        if (ix < Nx && F <= Nz) {
+            q = face_flux<S, FAST>(&fs[(lx + G) * PZ + fz + G], 1, r[0], r[VI], r[NV - 1]);
+            if constexpr (FLUX) {
+                if (fz < BZ || F == Nz) {
+                    put<NTS>(&a.qz[F + (int64_t)(ix + 1) * a.ld_qz], q);
+                    if (ix == qz_lft) put<NTS>(&a.qz[F], q);
+                    if (ix == qz_rgt) put<NTS>(&a.qz[F + (int64_t)(Nx + 1) * a.ld_qz], q);
+                }
+            }
+        }
+        return q;
+    };
+    double zq[L::CW];
+#pragma unroll
+    for (int j = 0; j < L::CW; ++j) {
+        const int lx = wv + j * NWV;
+        zq[j] = lx < BX ? zface(v.rw[j], lx, l) : 0.0;
+    }
+    if (wv == L::WC && l < BX) qzc[l] = zface(v.rc, l, BZ);
+    lds_barrier();
+
+    // ---- divergence, advect.jl:94: (qxr - qxl + qzb - qzt) * invh; the face below row l is lane
+    // l + 1's z-face (a wave shift), below row 63 the closing face
+#pragma unroll
+    for (int j = 0; j < L::CW; ++j) {
+        const int lx = wv + j * NWV;
+        const int ix = x0 + lx;
+        const double below = lane_next(zq[j]);
+        if (lx < BX && ix < Nx && iz < Nz) {
+            const double qxr = qxs[(lx + 1) * BZ + l], qxl = qxs[lx * BZ + l];
+            const double qzb = l == 63 ? qzc[lx] : below, qzt = zq[j];
+            put<NTS>(&a.adv[iz + (int64_t)ix * a.ld_adv], (qxr - qxl + qzb - qzt) * a.invh);
+        }
+    }
+}
+
 template <int S, bool HIN, bool FLUX, bool FAST, int BX, bool NTS = false>
 __device__ __forceinline__ void advect_tile_lr(const AdvArgs& a, int t) {
-    constexpr int BZ = 64;
-    constexpr int G = halo_of(S);
-    constexpr int PZ = BZ + 2 * G;
-    constexpr int PX = BX + 2 * G;
-    constexpr int NWV = NT / 64;                  // waves per workgroup
-    constexpr int CF = (PX + NWV - 1) / NWV;      // halo-tile columns per wave
-    constexpr int CU = (BX + 1 + NWV - 1) / NWV;  // x-face columns per wave
-    constexpr int CW = (BX + NWV - 1) / NWV;      // z-face (and cell) columns per wave
-    constexpr int NV = S == XC_TVDIM ? 3 : 1;     // velocities per face (TVD: left, own, right)
-    constexpr int VI = S == XC_TVDIM ? 1 : 0;
-    constexpr int WC = 1 % NWV;                   // the wave that also takes the closing z-faces
-    static_assert(BX <= 64 && NT % 64 == 0, "lane-row tiles");
+    using L = LrShape<S, BX>;
+    constexpr int BZ = L::BZ, G = L::G, PZ = L::PZ, PX = L::PX, NWV = L::NWV, CF = L::CF;
     __shared__ double fs[PX * PZ];
     __shared__ double qxs[(BX + 1) * BZ];
     __shared__ double qzc[BX];  // closing z-faces; the other z-faces stay in registers
@@ -430,50 +582,18 @@ __device__ __forceinline__ void advect_tile_lr(const AdvArgs& a, int t) {
         if constexpr (HIN) hx = gx + G;
         else hx = map_cell_sel(gx, Nx, xper);
         const double* col = a.f + (int64_t)hx * a.ld_f;
-        rf[j][0] = col[hzA];
-        rf[j][1] = col[hzB];
-    }
-    const int izc = min(z0 + l, Nz - 1);
-    double ru[CU][NV];
-#pragma unroll
-    for (int j = 0; j < CU; ++j) {
-        const int F = min(x0 + min(wv + j * NWV, BX), Nx);
-        if constexpr (S == XC_TVDIM) {
-            int c0 = F, c2 = F + 2, c1 = F + 1;
-            if constexpr (!HIN) {
-                c0 = map_face(F - 1, Nx, a.xbc);
-                c1 = F;
-                c2 = map_face(F + 1, Nx, a.xbc);
-            }
-            ru[j][0] = a.u[izc + (int64_t)c0 * a.ld_u];
-            ru[j][1] = a.u[izc + (int64_t)c1 * a.ld_u];
-            ru[j][2] = a.u[izc + (int64_t)c2 * a.ld_u];
+        // NTS launches: the columns no neighbouring tile reads (more than G from either edge of the tile)
+        // are loaded non-temporally, so they do not displace the shared halo columns from the cache
+        if (NTS && c >= 2 * G && c < PX - 2 * G) {
+            rf[j][0] = get<true>(&col[hzA]);
+            rf[j][1] = get<true>(&col[hzB]);
         } else {
-            ru[j][0] = a.u[izc + (int64_t)F * a.ld_u];
+            rf[j][0] = col[hzA];
+            rf[j][1] = col[hzB];
         }
     }
-    auto wload = [&](double* r, int F, int ix) {  // F: face row, 0..Nz; ix: cell column, 0..Nx-1
-        const double* col = a.w + (int64_t)ix * a.ld_w;
-        if constexpr (S == XC_TVDIM) {
-            if constexpr (HIN) {
-                r[0] = col[F];
-                r[1] = col[F + 1];
-                r[2] = col[F + 2];
-            } else {
-                r[0] = col[map_face(F - 1, Nz, a.zbc)];
-                r[1] = col[F];
-                r[2] = col[map_face(F + 1, Nz, a.zbc)];
-            }
-        } else {
-            r[0] = col[F];
-        }
-    };
-    double rw[CW][NV];
-#pragma unroll
-    for (int j = 0; j < CW; ++j)
-        wload(rw[j], min(z0 + l, Nz), min(x0 + min(wv + j * NWV, BX - 1), Nx - 1));
-    double rc[NV];  // closing z-faces (tile row BZ) of the columns x0 + l, lanes l < BX
-    if (wv == WC) wload(rc, min(z0 + BZ, Nz), min(x0 + min(l, BX - 1), Nx - 1));
+    LrVel<S, BX> v;
+    lr_load_vel<S, HIN, BX, NTS>(a, z0, x0, l, wv, v);
 
 #pragma unroll
     for (int j = 0; j < CF; ++j) {
@@ -484,72 +604,7 @@ __device__ __forceinline__ void advect_tile_lr(const AdvArgs& a, int t) {
         }
     }
     lds_barrier();
-
-    const int qx_top = (zper && Nz > 1) ? Nz - 1 : 0;  // source row of qx[0, :]
-    const int qx_bot = (zper && Nz > 1) ? 0 : Nz - 1;  // source row of qx[Nz+1, :]
-    const int qz_lft = (xper && Nx > 1) ? Nx - 1 : 0;
-    const int qz_rgt = (xper && Nx > 1) ? 0 : Nx - 1;
-    const int iz = z0 + l;
-
-    // ---- x-faces: column F = x0 + fx of the wave, row iz of the lane
-#pragma unroll
-    for (int j = 0; j < CU; ++j) {
-        const int fx = wv + j * NWV;
-        if (fx <= BX) {
-            const int F = x0 + fx;
-            double q = 0.0;
-            if (F <= Nx && iz < Nz) {
-                q = face_flux<S, FAST>(&fs[(fx + G) * PZ + l + G], PZ, ru[j][0], ru[j][VI], ru[j][NV - 1]);
-                if constexpr (FLUX) {
-                    if (fx < BX || F == Nx) {
-                        double* col = a.qx + (int64_t)F * a.ld_qx;
-                        put<NTS>(&col[iz + 1], q);
-                        if (iz == qx_top) put<NTS>(&col[0], q);
-                        if (iz == qx_bot) put<NTS>(&col[Nz + 1], q);
-                    }
-                }
-            }
-            qxs[fx * BZ + l] = q;
-        }
-    }
-    // ---- z-faces: face row F = z0 + l of the wave's column ix
-    auto zface = [&](const double* r, int lx, int fz) -> double {
-        const int F = z0 + fz, ix = x0 + lx;
-        double q = 0.0;
-        if (ix < Nx && F <= Nz) {
-            q = face_flux<S, FAST>(&fs[(lx + G) * PZ + fz + G], 1, r[0], r[VI], r[NV - 1]);
-            if constexpr (FLUX) {
-                if (fz < BZ || F == Nz) {
-                    put<NTS>(&a.qz[F + (int64_t)(ix + 1) * a.ld_qz], q);
-                    if (ix == qz_lft) put<NTS>(&a.qz[F], q);
-                    if (ix == qz_rgt) put<NTS>(&a.qz[F + (int64_t)(Nx + 1) * a.ld_qz], q);
-                }
-            }
-        }
-        return q;
-    };
-    double zq[CW];
-#pragma unroll
-    for (int j = 0; j < CW; ++j) {
-        const int lx = wv + j * NWV;
-        zq[j] = lx < BX ? zface(rw[j], lx, l) : 0.0;
-    }
-    if (wv == WC && l < BX) qzc[l] = zface(rc, l, BZ);
-    lds_barrier();
-
-    // ---- divergence, advect.jl:94: (qxr - qxl + qzb - qzt) * invh; the face below row l is lane
-    // l + 1's z-face (a wave shift), below row 63 the closing face
-#pragma unroll
-    for (int j = 0; j < CW; ++j) {
-        const int lx = wv + j * NWV;
-        const int ix = x0 + lx;
-        const double below = lane_next(zq[j]);
-        if (lx < BX && ix < Nx && iz < Nz) {
-            const double qxr = qxs[(lx + 1) * BZ + l], qxl = qxs[lx * BZ + l];
-            const double qzb = l == 63 ? qzc[lx] : below, qzt = zq[j];
-            put<NTS>(&a.adv[iz + (int64_t)ix * a.ld_adv], (qxr - qxl + qzb - qzt) * a.invh);
-        }
-    }
+    lr_faces<S, FLUX, FAST, BX, NTS>(a, z0, x0, l, wv, fs, qxs, qzc, v);
 }
 
 // the tile body a kernel uses: lane-row form for 64-row tiles, the generic form otherwise
@@ -559,123 +614,99 @@ __device__ __forceinline__ void advect_tile_any(const AdvArgs& a, int t) {
     else advect_tile<S, HIN, FLUX, FAST, BZ, BX, NTS>(a, t);
 }
 
-// Persistent, software-pipelined form (centred input, schemes without padded velocities): each
-// workgroup walks its XCD's tiles; while tile t is reconstructed from LDS, the f halo tile and the
-// face velocities of its next tile are already in flight into registers, so the memory system is
-// not idle during the reconstruction arithmetic.  Same face arithmetic as advect_tile.
-template <int S, bool FLUX, bool FAST, int BZ, int BX>
-__global__ __launch_bounds__(NT) void k_advect_pipe(AdvArgs a) {
+typedef __attribute__((address_space(3))) char lds_char_t;
+
+// LDS DMA of one dword a lane: lane k of the wave copies the dword at gbase + voff (its own offset) to
+// LDS byte lds + 4 k.  Written as inline asm so that the compiler's wait-count insertion does not treat
+// it as a pending write to every LDS array (with the builtin it waits for the DMA before each LDS read,
+// including reads of the other buffer); the caller waits for it explicitly (vmcnt) before reading the
+// destination.  M0 (the LDS base) is saved and restored around it.
+__device__ __forceinline__ void dma_dword(const char* gbase, unsigned voff, unsigned lds) {
+    unsigned saved;
+    asm volatile("s_mov_b32 %0, m0\n\ts_mov_b32 m0, %1\n\tglobal_load_lds_dword %2, %3\n\ts_mov_b32 m0, %0"
+                 : "=&s"(saved) : "s"(lds), "v"(voff), "s"(gbase) : "memory");
+}
+__device__ __forceinline__ unsigned lds_offset(void* p) {
+    return (unsigned)(uintptr_t)(lds_char_t*)p;
+}
+
+// Persistent, software-pipelined lane-row form (centred input, schemes without padded velocities).
+// Each workgroup walks its XCD's tiles.  The f halo tile of the next tile is copied from global memory
+// straight into the other half of a double-buffered LDS tile by the LDS DMA path (global_load_lds:
+// lane k of a wave writes dword k after the wave-uniform LDS base, no VGPRs involved), and the next
+// tile's face velocities are loaded into a second register set, both issued before the current tile's
+// reconstructions: the memory system streams the next tile while the arithmetic of this one runs.
+// A halo column of PZ doubles is 2 PZ dwords, ND dword-DMA instructions of 64 lanes; each lane's
+// address is its row's mapped source (the BCs as an index map, at 4-byte granularity), so clamped and
+// wrapped halo rows need no fix-up.  Same face arithmetic (lr_faces) as k_advect: bitwise the same.
+template <int S, bool FLUX, bool FAST, int BX, bool NTS>
+__global__ __launch_bounds__(NT) void k_advect_dma(AdvArgs a) {
     static_assert(S != XC_TVDIM, "tvdim needs its padded velocities");
-    constexpr bool NTS = false;
-    constexpr int G = halo_of(S);
-    constexpr int PZ = BZ + 2 * G;
-    constexpr int PX = BX + 2 * G;
-    constexpr int NF = (PX * PZ + NT - 1) / NT;
-    constexpr int NU = (BZ * (BX + 1) + NT - 1) / NT;
-    constexpr int NW = ((BZ + 1) * BX + NT - 1) / NT;
-    __shared__ double fs[PX * PZ];
-    __shared__ double us[BZ * (BX + 1)];
-    __shared__ double ws[(BZ + 1) * BX];
+    using L = LrShape<S, BX>;
+    constexpr int BZ = L::BZ, G = L::G, PZ = L::PZ, PX = L::PX, NWV = L::NWV, CF = L::CF;
+    constexpr int ND = (2 * PZ + 63) / 64;
+    __shared__ double fs[2][PX * PZ];
     __shared__ double qxs[(BX + 1) * BZ];
-    __shared__ double qzs[BX * (BZ + 1)];
-    const int tid = threadIdx.x;
+    __shared__ double qzc[BX];
+    const int l = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int Nz = a.Nz, Nx = a.Nx;
-    double rf[NF], ru[NU], rw[NW];
-    auto fetch = [&](int t) {
+    const bool zper = a.zbc == XC_PERIODIC, xper = a.xbc == XC_PERIODIC;
+
+    auto dma_f = [&](int t, int b) {
         const int z0 = (t % a.nbz) * BZ, x0 = (t / a.nbz) * BX;
+        int off[ND];  // byte offset in a column of the dword this lane copies, per DMA instruction
 #pragma unroll
-        for (int i = 0; i < NF; ++i) {
-            const int k = tid + i * NT;
-            if (k < PX * PZ) {
-                const int lz = k % PZ, lx = k / PZ;
-                int gz = z0 + lz - G, gx = x0 + lx - G;
-                gz = gz < Nz + G - 1 ? gz : Nz + G - 1;
-                gx = gx < Nx + G - 1 ? gx : Nx + G - 1;
-                rf[i] = a.f[map_cell(gz, Nz, a.zbc) + (int64_t)map_cell(gx, Nx, a.xbc) * a.ld_f];
+        for (int k = 0; k < ND; ++k) {
+            const int d = 64 * k + l;
+            const int r = min(d >> 1, PZ - 1);
+            const int gz = min(z0 + r - G, Nz + G - 1);
+            off[k] = map_cell_sel(gz, Nz, zper) * 8 + (d & 1) * 4;
+        }
+#pragma unroll
+        for (int j = 0; j < CF; ++j) {
+            const int c = wv + j * NWV;
+            if (c < PX) {
+                const int gx = min(x0 + c - G, Nx + G - 1);
+                const char* col = reinterpret_cast<const char*>(a.f + (int64_t)map_cell_sel(gx, Nx, xper) * a.ld_f);
+                const unsigned dst = lds_offset(&fs[b][c * PZ]);
+#pragma unroll
+                for (int k = 0; k < ND; ++k)
+                    if (64 * k + l < 2 * PZ)  // lanes past the column's last dword stay off
+                        dma_dword(col, (unsigned)off[k], dst + 256 * k);
             }
-        }
-#pragma unroll
-        for (int i = 0; i < NU; ++i) {
-            const int k = tid + i * NT;
-            const int lz = k % BZ, fx = k / BZ;
-            const int iz = z0 + lz, F = x0 + fx;
-            ru[i] = (k < BZ * (BX + 1) && iz < Nz && F <= Nx) ? a.u[iz + (int64_t)F * a.ld_u] : 0.0;
-        }
-#pragma unroll
-        for (int i = 0; i < NW; ++i) {
-            const int k = tid + i * NT;
-            const int fz = k % (BZ + 1), lx = k / (BZ + 1);
-            const int F = z0 + fz, ix = x0 + lx;
-            rw[i] = (k < (BZ + 1) * BX && ix < Nx && F <= Nz) ? a.w[F + (int64_t)ix * a.ld_w] : 0.0;
         }
     };
-    const int qx_top = (a.zbc == XC_PERIODIC && Nz > 1) ? Nz - 1 : 0;
-    const int qx_bot = (a.zbc == XC_PERIODIC && Nz > 1) ? 0 : Nz - 1;
-    const int qz_lft = (a.xbc == XC_PERIODIC && Nx > 1) ? Nx - 1 : 0;
-    const int qz_rgt = (a.xbc == XC_PERIODIC && Nx > 1) ? 0 : Nx - 1;
+    auto tile_of = [&](int it) { return xcd_tile(it, a.nb); };
+
     int it = blockIdx.x;
-    if (it < a.nb) fetch(xcd_tile(it, a.nb));
+    if (it >= a.nb) return;
+    LrVel<S, BX> v;
+    {
+        const int t = tile_of(it);
+        dma_f(t, 0);
+        lr_load_vel<S, false, BX, NTS>(a, (t % a.nbz) * BZ, (t / a.nbz) * BX, l, wv, v);
+    }
+    int b = 0;
     for (; it < a.nb; it += gridDim.x) {
-        const int t = xcd_tile(it, a.nb);
+        const int t = tile_of(it);
         const int z0 = (t % a.nbz) * BZ, x0 = (t / a.nbz) * BX;
-#pragma unroll
-        for (int i = 0; i < NF; ++i)
-            if (tid + i * NT < PX * PZ) fs[tid + i * NT] = rf[i];
-#pragma unroll
-        for (int i = 0; i < NU; ++i)
-            if (tid + i * NT < BZ * (BX + 1)) us[tid + i * NT] = ru[i];
-#pragma unroll
-        for (int i = 0; i < NW; ++i)
-            if (tid + i * NT < (BZ + 1) * BX) ws[tid + i * NT] = rw[i];
-        lds_barrier();
-        if (it + (int)gridDim.x < a.nb) fetch(xcd_tile(it + gridDim.x, a.nb));  // next tile, in flight
-#pragma unroll
-        for (int k = tid; k < BZ * (BX + 1); k += NT) {
-            const int lz = k % BZ, fx = k / BZ;
-            const int iz = z0 + lz, F = x0 + fx;
-            double q = 0.0;
-            if (iz < Nz && F <= Nx) {
-                q = face_flux<S, FAST>(&fs[(fx + G) * PZ + lz + G], PZ, 0.0, us[k], 0.0);
-                if constexpr (FLUX) {
-                    if (fx < BX || F == Nx) {
-                        double* col = a.qx + (int64_t)F * a.ld_qx;
-                        put<NTS>(&col[iz + 1], q);
-                        if (iz == qx_top) put<NTS>(&col[0], q);
-                        if (iz == qx_bot) put<NTS>(&col[Nz + 1], q);
-                    }
-                }
-            }
-            qxs[fx * BZ + lz] = q;
+        // this tile's halo has landed in fs[b] (every wave's share: wait, then meet), and every wave is
+        // done with fs[b ^ 1], qxs and qzc of the previous tile
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_waitcnt(0x0070);  // vmcnt(0) expcnt(7) lgkmcnt(0)
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        LrVel<S, BX> vn;
+        const int itn = it + (int)gridDim.x;
+        if (itn < a.nb) {
+            const int tn = tile_of(itn);
+            dma_f(tn, b ^ 1);
+            lr_load_vel<S, false, BX, NTS>(a, (tn % a.nbz) * BZ, (tn / a.nbz) * BX, l, wv, vn);
         }
-#pragma unroll
-        for (int k = tid; k < (BZ + 1) * BX; k += NT) {
-            const int fz = k % (BZ + 1), lx = k / (BZ + 1);
-            const int F = z0 + fz, ix = x0 + lx;
-            double q = 0.0;
-            if (ix < Nx && F <= Nz) {
-                q = face_flux<S, FAST>(&fs[(lx + G) * PZ + fz + G], 1, 0.0, ws[k], 0.0);
-                if constexpr (FLUX) {
-                    if (fz < BZ || F == Nz) {
-                        put<NTS>(&a.qz[F + (int64_t)(ix + 1) * a.ld_qz], q);
-                        if (ix == qz_lft) put<NTS>(&a.qz[F], q);
-                        if (ix == qz_rgt) put<NTS>(&a.qz[F + (int64_t)(Nx + 1) * a.ld_qz], q);
-                    }
-                }
-            }
-            qzs[lx * (BZ + 1) + fz] = q;
-        }
-        lds_barrier();
-#pragma unroll
-        for (int k = tid; k < BZ * BX; k += NT) {
-            const int lz = k % BZ, lx = k / BZ;
-            const int iz = z0 + lz, ix = x0 + lx;
-            if (iz < Nz && ix < Nx) {
-                const double qxr = qxs[(lx + 1) * BZ + lz], qxl = qxs[lx * BZ + lz];
-                const double qzb = qzs[lx * (BZ + 1) + lz + 1], qzt = qzs[lx * (BZ + 1) + lz];
-                put<NTS>(&a.adv[iz + (int64_t)ix * a.ld_adv], (qxr - qxl + qzb - qzt) * a.invh);
-            }
-        }
-        lds_barrier();  // fs/us/ws/qxs/qzs are rewritten for the next tile
+        lr_faces<S, FLUX, FAST, BX, NTS>(a, z0, x0, l, wv, fs[b], qxs, qzc, v);
+        v = vn;
+        b ^= 1;
     }
 }
 
@@ -756,20 +787,29 @@ int pipe_grid(K kernel, int nb) {
     return nb < g ? nb : (g > 0 ? g : nb);
 }
 
+template <int S, bool FLUX, bool FAST, int BX>
+hipError_t launch_advect_dma(const AdvArgs& a0, hipStream_t s) {
+    AdvArgs a = a0;
+    a.nbz = (a.Nz + 63) / 64;
+    a.nb = a.nbz * ((a.Nx + BX - 1) / BX);
+    if (a.nts) {
+        auto kern = k_advect_dma<S, FLUX, FAST, BX, true>;
+        hipLaunchKernelGGL(kern, dim3(pipe_grid(kern, a.nb)), dim3(NT), 0, s, a);
+    } else {
+        auto kern = k_advect_dma<S, FLUX, FAST, BX, false>;
+        hipLaunchKernelGGL(kern, dim3(pipe_grid(kern, a.nb)), dim3(NT), 0, s, a);
+    }
+    return hipGetLastError();
+}
+
 template <int S, bool HIN, bool FLUX, bool FAST>
 hipError_t launch_advect_f(const AdvArgs& a, hipStream_t s, int v) {
     if constexpr (HIN) {  // halo'd-input entry points keep the default tile
         return launch_advect_v<S, HIN, FLUX, FAST, 0>(a, s);
     } else {
         if constexpr (S != XC_TVDIM) {
-            if (v == VAR_PIPE) {
-                AdvArgs b = a;
-                b.nbz = (b.Nz + VAR_TZ[VAR_PIPE] - 1) / VAR_TZ[VAR_PIPE];
-                b.nb = b.nbz * ((b.Nx + VAR_TX[VAR_PIPE] - 1) / VAR_TX[VAR_PIPE]);
-                auto kern = k_advect_pipe<S, FLUX, FAST, VAR_TZ[VAR_PIPE], VAR_TX[VAR_PIPE]>;
-                hipLaunchKernelGGL(kern, dim3(pipe_grid(kern, b.nb)), dim3(NT), 0, s, b);
-                return hipGetLastError();
-            }
+            if (v >= VAR_PIPE) return v == VAR_PIPE ? launch_advect_dma<S, FLUX, FAST, VAR_TX[VAR_PIPE]>(a, s)
+                                                    : launch_advect_dma<S, FLUX, FAST, VAR_TX[VAR_PIPE + 1]>(a, s);
         }
         switch (v) {
             case 1: return launch_advect_v<S, HIN, FLUX, FAST, 1>(a, s);
@@ -829,7 +869,7 @@ int tune_advect(xc_ctx* ctx, int scheme, bool flux, bool ex, const AdvArgs& a, i
     XC_TRY_HIP(ctx, hipEventCreate(&e1));
     float best = 1e30f;
     for (int v = 0; v < NVAR; ++v) {
-        if (v == VAR_PIPE && scheme == XC_TVDIM) { t.us[v] = 1e30f; continue; }
+        if (v >= VAR_PIPE && scheme == XC_TVDIM) { t.us[v] = 1e30f; continue; }
         hipError_t e = hipSuccess;
         for (int r = 0; r < 4 && e == hipSuccess; ++r) {  // first launch warms up
             if (r == 1) (void)hipEventRecord(e0, ctx->stream);
@@ -1032,9 +1072,15 @@ __device__ __forceinline__ void diffus_tile(const DifArgs& a, int t) {
         // rows/columns past the grid end map like the one after the last
         const int gz = min(z0 + lz - 1, Nz), gx = min(x0 + lx - 1, Nx);
         const int64_t hz = map_cell_sel(gz, Nz, zper), hx = map_cell_sel(gx, Nx, xper);
-        rf[i] = a.f[hz + hx * a.ld_f];
-        rk[i] = a.k[hz + hx * a.ld_k];
-        if (has_k2) rk2[i] = a.k2[hz + hx * a.ld_k2];
+        if (NTS && lx >= 2 && lx < PX - 2) {  // columns no neighbouring tile reads: non-temporal
+            rf[i] = get<true>(&a.f[hz + hx * a.ld_f]);
+            rk[i] = get<true>(&a.k[hz + hx * a.ld_k]);
+            if (has_k2) rk2[i] = get<true>(&a.k2[hz + hx * a.ld_k2]);
+        } else {
+            rf[i] = a.f[hz + hx * a.ld_f];
+            rk[i] = a.k[hz + hx * a.ld_k];
+            if (has_k2) rk2[i] = a.k2[hz + hx * a.ld_k2];
+        }
     }
 #pragma unroll
     for (int i = 0; i < NF; ++i) {
@@ -1305,7 +1351,7 @@ int xc_advect_tile(xc_ctx* ctx, int scheme, int with_flux, int64_t Nz, int64_t N
     const TuneEntry* te = tune_find(ctx, scheme, with_flux ? 1 : 0, (ctx->flags & XC_CTX_EXACT) ? 1 : 0, (int)Nz, (int)Nx);
     const int v = te ? te->v : 0;
     *tz = VAR_TZ[v];
-    *tx = v == VAR_PIPE ? -VAR_TX[v] : VAR_TX[v];
+    *tx = v >= VAR_PIPE ? -VAR_TX[v] : VAR_TX[v];
     if (us)
         for (int k = 0; k < NVAR; ++k) us[k] = te ? te->us[k] : 0.f;
     return te ? XC_OK : XC_ENOCONV;
@@ -1314,9 +1360,8 @@ int xc_advect_tile(xc_ctx* ctx, int scheme, int with_flux, int64_t Nz, int64_t N
 int xc_advect_set_tile(xc_ctx* ctx, int scheme, int with_flux, int64_t Nz, int64_t Nx, int tz, int tx) {
     XC_CHECK_CTX(ctx);
     int v = -1;
-    for (int k = 0; k < VAR_PIPE; ++k)
-        if (VAR_TZ[k] == tz && VAR_TX[k] == tx) v = k;
-    if (tz == VAR_TZ[VAR_PIPE] && tx == -VAR_TX[VAR_PIPE] && scheme != XC_TVDIM) v = VAR_PIPE;
+    for (int k = 0; k < NVAR; ++k)
+        if (VAR_TZ[k] == tz && (k < VAR_PIPE ? VAR_TX[k] : -VAR_TX[k]) == tx && (k < VAR_PIPE || scheme != XC_TVDIM)) v = k;
     if (v < 0) return xc_fail(ctx, XC_EINVAL_ARG, "advect_set_tile: no %d x %d tile", tz, tx);
     const int ex = (ctx->flags & XC_CTX_EXACT) ? 1 : 0;
     TuneEntry* te = tune_find(ctx, scheme, with_flux ? 1 : 0, ex, (int)Nz, (int)Nx);

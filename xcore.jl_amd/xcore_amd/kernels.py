@@ -55,10 +55,10 @@ def advect_tile(scheme, Nz, Nx, with_flux=True):
     """(tz, tx, {tile: us}) the advection kernels use at this size; tuned=False if not measured yet."""
     c = context()
     tz, tx = ctypes.c_int(0), ctypes.c_int(0)
-    us = (ctypes.c_float * 5)()
+    us = (ctypes.c_float * 6)()
     code = lib().xc_advect_tile(c.ptr, _sym(scheme, SCHEMES, "scheme"), int(bool(with_flux)), int(Nz), int(Nx),
                                 ctypes.byref(tz), ctypes.byref(tx), us)
-    names = ["64x16", "128x8", "64x8", "32x32", "64x8p"]
+    names = ["64x16", "128x8", "64x8", "32x32", "64x16p", "64x8p"]
     cand = {n: round(float(u), 2) for n, u in zip(names, us) if float(u) < 1e29}
     return tz.value, tx.value, cand, code == 0
 
@@ -171,7 +171,7 @@ def diffus_centered_with_flux_(dff, qx, qz, f, k, h, xBC="periodic", zBC="closed
 
 def set_advect_tile(scheme, Nz, Nx, tz, tx, with_flux=True):
     """Pin the advection tile shape for this (scheme, fluxes, grid): one of 64x16, 128x8, 64x8, 32x32, or
-    (64, -8) for the persistent pipelined kernel."""
+    (64, -16) / (64, -8) for the persistent pipelined kernels."""
     c = context()
     c.check(lib().xc_advect_set_tile(c.ptr, _sym(scheme, SCHEMES, "scheme"), int(bool(with_flux)), int(Nz), int(Nx),
                                      int(tz), int(tx)), "advect_set_tile")
@@ -180,7 +180,8 @@ def set_advect_tile(scheme, Nz, Nx, tz, tx, with_flux=True):
 def diag_stream_(inputs, outputs, nts=False):
     """Diagnostic (no reference counterpart): the stencil traffic without its arithmetic --
     outputs[k][i + (k > 0), j] = sum of inputs[r][i, j] over the cells of outputs[0]'s shape, with
-    non-temporal stores if nts bit 0 is set; nts bit 1: every stream aligned and unshifted, 16-byte accesses.
+    non-temporal stores if nts bit 0 is set; nts bit 1: every stream aligned and unshifted, 16-byte accesses;
+    nts bit 2: non-temporal loads.
     Timing it gives the bandwidth HBM sustains for that stream mix."""
     c = context()
     ins = (_lib.XCMat * max(1, len(inputs)))(*[as_mat(t, "in") for t in inputs])
