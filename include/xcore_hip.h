@@ -253,6 +253,13 @@ typedef struct {  /* fluidmech! keywords (fluidmech.jl:71-80) and the iterative-
     int mg_fp64;      /* 1: the finest level's smoother in double precision; 0 (default): in single precision
                          (its vectors and a copy of its coefficients; FGMRES, the operator and the coarser
                          levels stay in double precision, and so does the convergence test) */
+    int xmean;        /* horizontal-mean correction (outside the benchmark mode): every FGMRES cycle's starting
+                         guess is first corrected exactly on the fields that do not vary in x,
+                         x += R (R^T LL R)^-1 R^T (RR - LL x) with R the x-uniform extension of one value per row
+                         of W, U and P (the Krylov method converges slowest on that subspace).  -1: off; 1: probe
+                         R^T LL R in this solve (9 operator applications); 2: reuse the operator the last probing
+                         solve on this grid left (probe if none); 0 (default): off in xc_fluidmech, and in
+                         xc_run_step probed in each time step's first Picard sweep and reused in the others */
 } xc_fm_opts;
 
 typedef struct {  /* state.jl:131-134 BnchmData */

@@ -543,6 +543,8 @@ int xc_run_step(xc_ctx* ctx, xc_phase* ph, xc_fluid* fl, xc_noise* ns, const xc_
         if ((st = xc_snapshot(ctx, ph, fl))) return st;
         if ((st = xc_phsevo(ctx, ph, fl, g, par, sc, so->scheme, so->xbc, so->zbc, c, dt, par->alpha))) return st;
         xc_solve_stats fs{};
+        // the horizontal-mean correction's operator: probed in the step's first sweep, reused in the others
+        if (so->fm.xmean == 0) fo.xmean = iter == 1 ? 1 : 2;
         st = xc_fluidmech(ctx, fl, ph, g, par, &fo, nullptr, &fs);
         fm_iters += fs.iters;
         fm_fallbacks += fs.fallback_used;

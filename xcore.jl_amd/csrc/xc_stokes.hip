@@ -24,6 +24,9 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
+#include <cmath>
+
 #include <vector>
 
 #include "xc_fft.h"
@@ -32,6 +35,9 @@
 namespace {
 
 constexpr int NT = 256;
+
+constexpr int XM_CH = 16;      // columns per partial x-sum of the horizontal-mean correction
+constexpr int XM_PROBES = 9;   // comb probes of its Galerkin operator (3 fields x 3 residues of z mod 3)
 
 struct Flags {
     int bnchm, sds, top_cnv, bot_cnv, open_cnv;
@@ -1859,6 +1865,19 @@ struct StokesCtx {
     // and the number of solves still to run plain before the learned correction is tried again
     double plain_rate = 0.0;
     int lc_fail_streak = 0, lc_skip = 0;
+    // horizontal-mean correction of each FGMRES cycle's starting guess (see xm_build): the Galerkin operator
+    // R^T LL R on the x-uniform fields, factorized on the host (banded LU, rows ordered W, U, P per z), its
+    // mapped host scratch (q = R^T r, delta, the probe sums) and device buffers (partial sums, delta)
+    int xm_n0 = 0, xm_nch = 0;
+    double* xm_host = nullptr;
+    double* xm_hdev = nullptr;
+    double* xm_part = nullptr;
+    double* xm_delta = nullptr;
+    std::vector<double> xm_lu;
+    std::vector<int> xm_piv;
+    bool xm_built = false;   // a probing solve has run on this grid
+    bool xm_ok = false;      // ... and its factorization is held
+    bool xm_active = false;  // this solve corrects its cycles' starting guesses
 };
 
 int pois_open_table(xc_ctx* ctx, StokesCtx* s, int Nz, int Nx);
@@ -1931,6 +1950,8 @@ void stokes_free(xc_ctx*, void* p) {
     StokesCtx* s = (StokesCtx*)p;
     if (s->pool) (void)hipFree(s->pool);
     if (s->host) (void)hipHostFree(s->host);
+    if (s->xm_host) (void)hipHostFree(s->xm_host);
+    if (s->xm_part) (void)hipFree(s->xm_part);
     for (auto& e : s->ev)
         if (e) (void)hipEventDestroy(e);
     delete s;
@@ -2104,6 +2125,16 @@ int setup(xc_ctx* ctx, int Nz, int Nx, int m, StokesCtx** out, int maxlev = -1) 
     if (e != hipSuccess) { stokes_free(ctx, s); return xc_fail(ctx, XC_ENOMEM, "fluidmech!: pinned scratch"); }
     e = hipHostGetDevicePointer((void**)&s->host_dev, s->host, 0);
     if (e != hipSuccess) { stokes_free(ctx, s); return xc_fail(ctx, XC_EHIP, "fluidmech!: %s", hipGetErrorString(e)); }
+    s->xm_n0 = 3 * Nz + 5;
+    s->xm_nch = (Nx + 2 + XM_CH - 1) / XM_CH;
+    e = hipHostMalloc((void**)&s->xm_host, (size_t)(2 + XM_PROBES) * s->xm_n0 * sizeof(double),
+                      hipHostMallocMapped | hipHostMallocCoherent);
+    if (e != hipSuccess) { stokes_free(ctx, s); return xc_fail(ctx, XC_ENOMEM, "fluidmech!: pinned scratch"); }
+    e = hipHostGetDevicePointer((void**)&s->xm_hdev, s->xm_host, 0);
+    if (e != hipSuccess) { stokes_free(ctx, s); return xc_fail(ctx, XC_EHIP, "fluidmech!: %s", hipGetErrorString(e)); }
+    e = hipMalloc((void**)&s->xm_part, (size_t)(s->xm_nch + 1) * s->xm_n0 * sizeof(double));
+    if (e != hipSuccess) { stokes_free(ctx, s); return xc_fail(ctx, XC_ENOMEM, "fluidmech!: horizontal-mean scratch"); }
+    s->xm_delta = s->xm_part + (size_t)s->xm_nch * s->xm_n0;
     for (auto& ev : s->ev) {
         e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
         if (e != hipSuccess) { stokes_free(ctx, s); return xc_fail(ctx, XC_EHIP, "fluidmech!: event"); }
@@ -2587,6 +2618,182 @@ int precond_full(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, in
 }
 
 // right-preconditioned FGMRES(m) on LL x = b from the given x; stops at ||b - LL x|| <= target
+// ---- horizontal-mean correction of a cycle's starting guess
+// The Krylov solve converges slowly in one subspace: the fields that do not vary in x (k_x = 0 modes --
+// the horizontally averaged mass flux through each row of faces and the pressure profile that carries it),
+// which the preconditioner's Schur approximation treats least accurately; FGMRES then stalls for ~8
+// iterations at a residual of ~1e-7 of ||RR|| before it resolves them.  Before every cycle the starting guess
+// is corrected on that subspace exactly (a Galerkin, two-level step): with R the map from profiles (one value
+// per row of W, U and P) to x-uniform fields (every column, ghost columns included, equal to the profile),
+// x += R L0^-1 R^T (b - LL x) with L0 = R^T LL R, so that the corrected residual has no x-uniform component.
+// L0 couples rows z - 1 .. z + 1 only; it is probed with 9 x-uniform comb vectors (one field, rows of one
+// residue mod 3), each response row decoded to the comb row within one of it, and factorized on the host
+// (LU with partial pivoting, band 5 with rows ordered W, U, P per z).  Profiles are [W rows 0..Nz][U rows
+// 0..Nz+1][P rows 0..Nz+1].
+__device__ __forceinline__ void xm_locate(int64_t i, int Nz, int Nx, int& f, int& row) {
+    const int64_t nW = (int64_t)(Nz + 1) * (Nx + 2), nU = (int64_t)(Nz + 2) * (Nx + 1);
+    if (i < nW) { f = 0; row = (int)(i % (Nz + 1)); }
+    else if (i < nW + nU) { f = 1; row = (int)((i - nW) % (Nz + 2)); }
+    else { f = 2; row = (int)((i - nW - nU) % (Nz + 2)); }
+}
+// comb_f < 0: x += R p; otherwise x = R c with c the comb (field comb_f, rows = comb_k mod 3)
+__global__ __launch_bounds__(NT) void k_xm_expand(const double* __restrict__ p, double* __restrict__ x, int64_t n,
+                                                  int Nz, int Nx, int comb_f, int comb_k) {
+    const int64_t i = gidx();
+    if (i >= n) return;
+    int f, row;
+    xm_locate(i, Nz, Nx, f, row);
+    if (comb_f >= 0) x[i] = (f == comb_f && row % 3 == comb_k) ? 1.0 : 0.0;
+    else x[i] += p[row + (f == 0 ? 0 : f == 1 ? Nz + 1 : 2 * Nz + 3)];
+}
+// part[c][q] = the sum of profile row q's entries in columns c XM_CH .. c XM_CH + XM_CH - 1 of its field
+__global__ __launch_bounds__(NT) void k_xm_part(const double* __restrict__ v, double* __restrict__ part, int Nz,
+                                                int Nx, int n0) {
+    const int q = blockIdx.x * NT + threadIdx.x, c = blockIdx.y;
+    if (q >= n0) return;
+    const int64_t nW = (int64_t)(Nz + 1) * (Nx + 2), nU = (int64_t)(Nz + 2) * (Nx + 1);
+    int rows, cols, row;
+    int64_t base;
+    if (q <= Nz) { rows = Nz + 1; cols = Nx + 2; row = q; base = 0; }
+    else if (q <= 2 * Nz + 2) { rows = Nz + 2; cols = Nx + 1; row = q - (Nz + 1); base = nW; }
+    else { rows = Nz + 2; cols = Nx + 2; row = q - (2 * Nz + 3); base = nW + nU; }
+    const int c0 = c * XM_CH, c1 = min(cols, c0 + XM_CH);
+    double sum = 0.0;
+    for (int col = c0; col < c1; ++col) sum += v[base + row + (int64_t)col * rows];
+    part[(int64_t)c * n0 + q] = sum;
+}
+__global__ __launch_bounds__(NT) void k_xm_final(const double* __restrict__ part, int nch, int n0, double* out) {
+    const int q = blockIdx.x * NT + threadIdx.x;
+    if (q >= n0) return;
+    double sum = 0.0;
+    for (int c = 0; c < nch; ++c) sum += part[(int64_t)c * n0 + q];
+    out[q] = sum;
+}
+// out (n0, device or mapped host) = R^T v
+void xm_sum(xc_ctx* ctx, StokesCtx* s, const double* v, double* out) {
+    const int n0 = s->xm_n0;
+    hipLaunchKernelGGL(k_xm_part, dim3((n0 + NT - 1) / NT, s->xm_nch), dim3(NT), 0, ctx->stream, v, s->xm_part,
+                       s->Nz, s->Nx, n0);
+    hipLaunchKernelGGL(k_xm_final, dim3((n0 + NT - 1) / NT), dim3(NT), 0, ctx->stream, s->xm_part, s->xm_nch, n0,
+                       out);
+}
+
+// LU with partial pivoting of a band matrix (kl sub-, ku super-diagonals); row i of a holds columns
+// i - kl .. i + kl + ku (the room the row interchanges fill), W = 2 kl + ku + 1 entries a row
+constexpr int XM_KL = 5, XM_KU = 5, XM_W = 2 * XM_KL + XM_KU + 1;
+inline double& xm_at(std::vector<double>& a, int i, int j) { return a[(size_t)i * XM_W + (j - i + XM_KL)]; }
+bool xm_band_lu(int n, std::vector<double>& a, std::vector<int>& piv) {
+    piv.assign(n, 0);
+    for (int k = 0; k < n; ++k) {
+        const int last = std::min(n - 1, k + XM_KL), right = std::min(n - 1, k + XM_KL + XM_KU);
+        int p = k;
+        for (int i = k + 1; i <= last; ++i)
+            if (fabs(xm_at(a, i, k)) > fabs(xm_at(a, p, k))) p = i;
+        const double d = xm_at(a, p, k);
+        if (!(d != 0.0) || !std::isfinite(d)) return false;
+        piv[k] = p;
+        if (p != k)
+            for (int j = k; j <= right; ++j) std::swap(xm_at(a, k, j), xm_at(a, p, j));
+        for (int i = k + 1; i <= last; ++i) {
+            const double l = xm_at(a, i, k) / xm_at(a, k, k);
+            xm_at(a, i, k) = l;
+            if (l != 0.0)
+                for (int j = k + 1; j <= right; ++j) xm_at(a, i, j) -= l * xm_at(a, k, j);
+        }
+    }
+    return true;
+}
+void xm_band_solve(int n, std::vector<double>& a, const std::vector<int>& piv, double* b) {
+    for (int k = 0; k < n; ++k) {
+        if (piv[k] != k) std::swap(b[k], b[piv[k]]);
+        for (int i = k + 1; i <= std::min(n - 1, k + XM_KL); ++i) b[i] -= xm_at(a, i, k) * b[k];
+    }
+    for (int k = n - 1; k >= 0; --k) {
+        double v = b[k];
+        for (int j = k + 1; j <= std::min(n - 1, k + XM_KL + XM_KU); ++j) v -= xm_at(a, k, j) * b[j];
+        b[k] = v / xm_at(a, k, k);
+    }
+}
+// profile index q of (field f, row z) and its position in the banded ordering (3 z + f; W has no row Nz + 1,
+// whose position holds an identity row)
+inline int xm_prof(int f, int z, int Nz) { return z + (f == 0 ? 0 : f == 1 ? Nz + 1 : 2 * Nz + 3); }
+
+// Probe and factorize L0 for the current coefficients and flags (uses s->ws and s->tmp; synchronizes).
+int xm_build(xc_ctx* ctx, StokesCtx* s, const Flags& F) {
+    const int Nz = s->Nz, Nx = s->Nx, n0 = s->xm_n0;
+    const LevD dd = s->lev[0].d();
+    const int64_t n = s->lev[0].nv + (int64_t)(Nz + 2) * (Nx + 2);
+    s->xm_ok = false;
+    for (int f = 0; f < 3; ++f)
+        for (int k = 0; k < 3; ++k) {
+            hipLaunchKernelGGL(k_xm_expand, dim3(nblk(n)), dim3(NT), 0, ctx->stream, (const double*)nullptr, s->ws, n,
+                               Nz, Nx, f, k);
+            hipLaunchKernelGGL(k_ll_cells<false>, dim3(ll_cells_blocks(dd.Nz, dd.Nx)), dim3(NT), 0, ctx->stream, dd,
+                               F, s->ws, s->ws, s->tmp);
+            xm_sum(ctx, s, s->tmp, s->xm_hdev + (size_t)(2 + 3 * f + k) * n0);
+        }
+    XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const int nb = 3 * (Nz + 2);
+    const int rows[3] = {Nz + 1, Nz + 2, Nz + 2};
+    std::vector<double>& a = s->xm_lu;
+    a.assign((size_t)nb * XM_W, 0.0);
+    xm_at(a, 3 * (Nz + 1), 3 * (Nz + 1)) = 1.0;  // W has no row Nz + 1
+    for (int f = 0; f < 3; ++f)
+        for (int k = 0; k < 3; ++k) {
+            const double* resp = s->xm_host + (size_t)(2 + 3 * f + k) * n0;
+            for (int fr = 0; fr < 3; ++fr)
+                for (int zr = 0; zr < rows[fr]; ++zr) {
+                    const double v = resp[xm_prof(fr, zr, Nz)];
+                    if (v == 0.0) continue;
+                    const int zt = zr - 1 + (((k - (zr - 1)) % 3) + 3) % 3;  // the comb row within one of zr
+                    const int i = 3 * zr + fr, j = 3 * zt + f;
+                    if (!std::isfinite(v) || zt < 0 || zt >= rows[f] || j - i > XM_KU || i - j > XM_KL) return XC_OK;
+                    xm_at(a, i, j) += v;
+                }
+        }
+    s->xm_ok = xm_band_lu(nb, a, s->xm_piv);
+    return XC_OK;
+}
+
+// Decide for this solve (fluidmech!) from o->xmean: -1 or 0 off; 1 probe the operator now; 2 reuse the operator
+// the last probing solve of this grid left (probe if there is none).  Reuse keeps the results of a solve a
+// function of the calls of the current time step only (run! probes in each step's first Picard sweep), so a
+// resumed run repeats the uninterrupted one bit for bit.
+int xm_prepare(xc_ctx* ctx, StokesCtx* s, const Flags& F, const xc_fm_opts* o) {
+    s->xm_active = false;
+    if (F.bnchm || o->xmean <= 0) return XC_OK;
+    if (o->xmean == 1 || !s->xm_built) {
+        int st;
+        if ((st = xm_build(ctx, s, F))) return st;
+        s->xm_built = true;
+    }
+    s->xm_active = s->xm_ok;
+    return XC_OK;
+}
+
+// x += R L0^-1 q with q = R^T r of the residual of x (in s->xm_host, reduced with that residual); false if the
+// correction is not finite (x unchanged)
+bool xm_correct(xc_ctx* ctx, StokesCtx* s, double* x, int64_t n) {
+    const int Nz = s->Nz, n0 = s->xm_n0, nb = 3 * (Nz + 2);
+    const int rows[3] = {Nz + 1, Nz + 2, Nz + 2};
+    std::vector<double> b((size_t)nb, 0.0);
+    for (int f = 0; f < 3; ++f)
+        for (int z = 0; z < rows[f]; ++z) b[3 * z + f] = s->xm_host[xm_prof(f, z, Nz)];
+    xm_band_solve(nb, s->xm_lu, s->xm_piv, b.data());
+    double* d = s->xm_host + n0;
+    for (int f = 0; f < 3; ++f)
+        for (int z = 0; z < rows[f]; ++z) {
+            const double v = b[3 * z + f];
+            if (!std::isfinite(v)) return false;
+            d[xm_prof(f, z, Nz)] = v;
+        }
+    if (hipMemcpyAsync(s->xm_delta, d, (size_t)n0 * sizeof(double), hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+        return false;
+    hipLaunchKernelGGL(k_xm_expand, dim3(nblk(n)), dim3(NT), 0, ctx->stream, (const double*)s->xm_delta, x, n, Nz,
+                       s->Nx, -1, 0);
+    return true;
+}
+
 // The residual of the starting x: r0_ready: already in s->rs with its squared norm in s->host[R0_SLOT] (enqueued by
 // the caller before a synchronization it needed anyway, enqueue_r0), else computed here.
 constexpr int R0_SLOT = 4;
@@ -2597,6 +2804,7 @@ void enqueue_r0(xc_ctx* ctx, StokesCtx* s, const Flags& F, const double* x, cons
     hipLaunchKernelGGL(k_ll_cells<true>, dim3(nbl), dim3(NT), 0, ctx->stream, dd, F, x, b, s->rs, part);
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(NT), 0, ctx->stream, part, (int)(nbl * (NT / 64)), 1,
                        s->host_dev + R0_SLOT);
+    if (s->xm_active) xm_sum(ctx, s, s->rs, s->xm_hdev);  // R^T r for the cycle start's correction
 }
 int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int m, int64_t n, const double* b,
                double* x, double target, double bnorm, int maxit, const LcCfg& lc, SolveOut* out,
@@ -2641,6 +2849,10 @@ int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int 
             XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
         }
         first = false;
+        if (s->xm_active && iters < maxit && xm_correct(ctx, s, x, n)) {  // the x-uniform part of the residual, exactly
+            enqueue_r0(ctx, s, F, x, b);
+            XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        }
         const double beta = sqrt(s->host[R0_SLOT]);
         rel = bnorm > 0 ? beta / bnorm : beta;
         if (rel0 < 0) rel0 = rel;
@@ -2797,6 +3009,7 @@ int xc_fluidmech(xc_ctx* ctx, xc_fluid* fl, const xc_phase* ph, const xc_grid* g
     build_levels(ctx, s, fl, g->h, false);
     const bool depth_pending = depth_launch(ctx, s, F, o);
     if ((st = pois_table(ctx, s, F, Nz, Nx, g->h))) return st;
+    if ((st = xm_prepare(ctx, s, F, o))) return st;
     if ((st = build_rhs(ctx, s, fl, ph, g, par, o, bn, F, s->bs))) return st;
     hipLaunchKernelGGL(k_gather, dim3(nb), dim3(NT), 0, ctx->stream, fl->W, fl->U, fl->P, s->xs, Nz, Nx);
     // the starting residual, then ||RR||: one synchronization for both
@@ -2853,6 +3066,7 @@ int xc_fluidmech(xc_ctx* ctx, xc_fluid* fl, const xc_phase* ph, const xc_grid* g
     conv = so.conv;
     rel = so.rel;
     rel0 = so.rel0;
+    s->xm_active = false;
     hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(NT), 0, ctx->stream, s->xs, fl->W, fl->U, fl->P, Nz, Nx);
     if (stats) {
         stats->iters = iters;
@@ -2878,6 +3092,7 @@ static int piece_setup(xc_ctx* ctx, const xc_fluid* fl, const xc_grid* g, const 
     if ((st = check_fm(ctx, fl, g, o))) return st;
     if ((st = setup(ctx, (int)g->Nz, (int)g->Nx, m, sp, o->mg_levels))) return st;
     StokesCtx* s = *sp;
+    s->xm_active = false;  // the solver pieces run the plain method
     build_levels(ctx, s, fl, g->h, false);
     *F = make_flags(o, par, (int)g->Nz, (int)g->Nx, bnchm != 0);
     if ((st = choose_depth(ctx, s, *F, o, g->h, depth_launch(ctx, s, *F, o), false))) return st;
