@@ -36,7 +36,7 @@ namespace {
 
 constexpr int NT = 256;
 
-constexpr int XM_CH = 16;      // columns per partial x-sum of the horizontal-mean correction
+constexpr int XM_CH = 64;      // columns per partial x-sum of the horizontal-mean correction
 constexpr int XM_PROBES = 9;   // comb probes of its Galerkin operator (3 fields x 3 residues of z mod 3)
 
 struct Flags {
@@ -2132,9 +2132,9 @@ int setup(xc_ctx* ctx, int Nz, int Nx, int m, StokesCtx** out, int maxlev = -1) 
     if (e != hipSuccess) { stokes_free(ctx, s); return xc_fail(ctx, XC_ENOMEM, "fluidmech!: pinned scratch"); }
     e = hipHostGetDevicePointer((void**)&s->xm_hdev, s->xm_host, 0);
     if (e != hipSuccess) { stokes_free(ctx, s); return xc_fail(ctx, XC_EHIP, "fluidmech!: %s", hipGetErrorString(e)); }
-    e = hipMalloc((void**)&s->xm_part, (size_t)(s->xm_nch + 1) * s->xm_n0 * sizeof(double));
+    e = hipMalloc((void**)&s->xm_part, (size_t)(XM_PROBES * s->xm_nch + 1) * s->xm_n0 * sizeof(double));
     if (e != hipSuccess) { stokes_free(ctx, s); return xc_fail(ctx, XC_ENOMEM, "fluidmech!: horizontal-mean scratch"); }
-    s->xm_delta = s->xm_part + (size_t)s->xm_nch * s->xm_n0;
+    s->xm_delta = s->xm_part + (size_t)XM_PROBES * s->xm_nch * s->xm_n0;
     for (auto& ev : s->ev) {
         e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
         if (e != hipSuccess) { stokes_free(ctx, s); return xc_fail(ctx, XC_EHIP, "fluidmech!: event"); }
@@ -2646,11 +2646,14 @@ __global__ __launch_bounds__(NT) void k_xm_expand(const double* __restrict__ p, 
     if (comb_f >= 0) x[i] = (f == comb_f && row % 3 == comb_k) ? 1.0 : 0.0;
     else x[i] += p[row + (f == 0 ? 0 : f == 1 ? Nz + 1 : 2 * Nz + 3)];
 }
-// part[c][q] = the sum of profile row q's entries in columns c XM_CH .. c XM_CH + XM_CH - 1 of its field
-__global__ __launch_bounds__(NT) void k_xm_part(const double* __restrict__ v, double* __restrict__ part, int Nz,
-                                                int Nx, int n0) {
+// part[p][c][q] = the sum of profile row q's entries in columns c XM_CH .. c XM_CH + XM_CH - 1 of its field, for
+// vector p = blockIdx.z (v + p vstride)
+__global__ __launch_bounds__(NT) void k_xm_part(const double* __restrict__ v, int64_t vstride,
+                                                double* __restrict__ part, int Nz, int Nx, int n0) {
     const int q = blockIdx.x * NT + threadIdx.x, c = blockIdx.y;
     if (q >= n0) return;
+    v += (int64_t)blockIdx.z * vstride;
+    part += (int64_t)blockIdx.z * gridDim.y * n0;
     const int64_t nW = (int64_t)(Nz + 1) * (Nx + 2), nU = (int64_t)(Nz + 2) * (Nx + 1);
     int rows, cols, row;
     int64_t base;
@@ -2658,24 +2661,35 @@ __global__ __launch_bounds__(NT) void k_xm_part(const double* __restrict__ v, do
     else if (q <= 2 * Nz + 2) { rows = Nz + 2; cols = Nx + 1; row = q - (Nz + 1); base = nW; }
     else { rows = Nz + 2; cols = Nx + 2; row = q - (2 * Nz + 3); base = nW + nU; }
     const int c0 = c * XM_CH, c1 = min(cols, c0 + XM_CH);
+    const double* col = v + base + row;
     double sum = 0.0;
-    for (int col = c0; col < c1; ++col) sum += v[base + row + (int64_t)col * rows];
+    int cc = c0;
+    for (; cc + 8 <= c1; cc += 8) {  // eight loads in flight, summed in column order
+        double t[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) t[u] = col[(int64_t)(cc + u) * rows];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) sum += t[u];
+    }
+    for (; cc < c1; ++cc) sum += col[(int64_t)cc * rows];
     part[(int64_t)c * n0 + q] = sum;
 }
+// out[p][q] = sum over the nch partials of vector p (fixed order)
 __global__ __launch_bounds__(NT) void k_xm_final(const double* __restrict__ part, int nch, int n0, double* out) {
     const int q = blockIdx.x * NT + threadIdx.x;
     if (q >= n0) return;
+    part += (int64_t)blockIdx.y * nch * n0;
     double sum = 0.0;
     for (int c = 0; c < nch; ++c) sum += part[(int64_t)c * n0 + q];
-    out[q] = sum;
+    out[(int64_t)blockIdx.y * n0 + q] = sum;
 }
-// out (n0, device or mapped host) = R^T v
-void xm_sum(xc_ctx* ctx, StokesCtx* s, const double* v, double* out) {
+// out[p] (n0 each, device or mapped host) = R^T (v + p vstride) for p < nv
+void xm_sum(xc_ctx* ctx, StokesCtx* s, const double* v, double* out, int nv = 1, int64_t vstride = 0) {
     const int n0 = s->xm_n0;
-    hipLaunchKernelGGL(k_xm_part, dim3((n0 + NT - 1) / NT, s->xm_nch), dim3(NT), 0, ctx->stream, v, s->xm_part,
-                       s->Nz, s->Nx, n0);
-    hipLaunchKernelGGL(k_xm_final, dim3((n0 + NT - 1) / NT), dim3(NT), 0, ctx->stream, s->xm_part, s->xm_nch, n0,
-                       out);
+    hipLaunchKernelGGL(k_xm_part, dim3((n0 + NT - 1) / NT, s->xm_nch, nv), dim3(NT), 0, ctx->stream, v, vstride,
+                       s->xm_part, s->Nz, s->Nx, n0);
+    hipLaunchKernelGGL(k_xm_final, dim3((n0 + NT - 1) / NT, nv), dim3(NT), 0, ctx->stream, s->xm_part, s->xm_nch,
+                       n0, out);
 }
 
 // LU with partial pivoting of a band matrix (kl sub-, ku super-diagonals); row i of a holds columns
@@ -2724,14 +2738,17 @@ int xm_build(xc_ctx* ctx, StokesCtx* s, const Flags& F) {
     const LevD dd = s->lev[0].d();
     const int64_t n = s->lev[0].nv + (int64_t)(Nz + 2) * (Nx + 2);
     s->xm_ok = false;
+    const bool batch = s->m >= XM_PROBES;  // the responses side by side in the (idle) basis Z, summed together
     for (int f = 0; f < 3; ++f)
         for (int k = 0; k < 3; ++k) {
+            double* resp = batch ? s->Z + (int64_t)(3 * f + k) * n : s->tmp;
             hipLaunchKernelGGL(k_xm_expand, dim3(nblk(n)), dim3(NT), 0, ctx->stream, (const double*)nullptr, s->ws, n,
                                Nz, Nx, f, k);
             hipLaunchKernelGGL(k_ll_cells<false>, dim3(ll_cells_blocks(dd.Nz, dd.Nx)), dim3(NT), 0, ctx->stream, dd,
-                               F, s->ws, s->ws, s->tmp);
-            xm_sum(ctx, s, s->tmp, s->xm_hdev + (size_t)(2 + 3 * f + k) * n0);
+                               F, s->ws, s->ws, resp);
+            if (!batch) xm_sum(ctx, s, resp, s->xm_hdev + (size_t)(2 + 3 * f + k) * n0);
         }
+    if (batch) xm_sum(ctx, s, s->Z, s->xm_hdev + 2 * (size_t)n0, XM_PROBES, n);
     XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const int nb = 3 * (Nz + 2);
     const int rows[3] = {Nz + 1, Nz + 2, Nz + 2};
@@ -2797,14 +2814,14 @@ bool xm_correct(xc_ctx* ctx, StokesCtx* s, double* x, int64_t n) {
 // The residual of the starting x: r0_ready: already in s->rs with its squared norm in s->host[R0_SLOT] (enqueued by
 // the caller before a synchronization it needed anyway, enqueue_r0), else computed here.
 constexpr int R0_SLOT = 4;
-void enqueue_r0(xc_ctx* ctx, StokesCtx* s, const Flags& F, const double* x, const double* b) {
+void enqueue_r0(xc_ctx* ctx, StokesCtx* s, const Flags& F, const double* x, const double* b, bool with_q = true) {
     const LevD dd = s->lev[0].d();
     const unsigned nbl = ll_cells_blocks(dd.Nz, dd.Nx);
     double* part = s->part + DOT_BLOCKS;
     hipLaunchKernelGGL(k_ll_cells<true>, dim3(nbl), dim3(NT), 0, ctx->stream, dd, F, x, b, s->rs, part);
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(NT), 0, ctx->stream, part, (int)(nbl * (NT / 64)), 1,
                        s->host_dev + R0_SLOT);
-    if (s->xm_active) xm_sum(ctx, s, s->rs, s->xm_hdev);  // R^T r for the cycle start's correction
+    if (s->xm_active && with_q) xm_sum(ctx, s, s->rs, s->xm_hdev);  // R^T r for the cycle start's correction
 }
 int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int m, int64_t n, const double* b,
                double* x, double target, double bnorm, int maxit, const LcCfg& lc, SolveOut* out,
@@ -2850,7 +2867,7 @@ int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int 
         }
         first = false;
         if (s->xm_active && iters < maxit && xm_correct(ctx, s, x, n)) {  // the x-uniform part of the residual, exactly
-            enqueue_r0(ctx, s, F, x, b);
+            enqueue_r0(ctx, s, F, x, b, false);
             XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
         }
         const double beta = sqrt(s->host[R0_SLOT]);
