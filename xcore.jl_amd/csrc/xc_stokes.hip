@@ -2814,7 +2814,7 @@ bool xm_correct(xc_ctx* ctx, StokesCtx* s, double* x, int64_t n) {
 // The residual of the starting x: r0_ready: already in s->rs with its squared norm in s->host[R0_SLOT] (enqueued by
 // the caller before a synchronization it needed anyway, enqueue_r0), else computed here.
 constexpr int R0_SLOT = 4;
-void enqueue_r0(xc_ctx* ctx, StokesCtx* s, const Flags& F, const double* x, const double* b, bool with_q = true) {
+void enqueue_r0(xc_ctx* ctx, StokesCtx* s, const Flags& F, const double* x, const double* b, bool with_q = false) {
     const LevD dd = s->lev[0].d();
     const unsigned nbl = ll_cells_blocks(dd.Nz, dd.Nx);
     double* part = s->part + DOT_BLOCKS;
@@ -2861,14 +2861,23 @@ int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int 
     constexpr double NEAR_CONV = 3.0;
     bool first = true;
     while (true) {
+        const bool q_ready = first && r0_ready;  // the first residual came with R^T r (see xc_fluidmech)
         if (!(first && r0_ready)) {  // r = b - LL x and ||r||^2 (per-wave partials, summed into the mapped host buffer)
             enqueue_r0(ctx, s, F, x, b);
             XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
         }
         first = false;
-        if (s->xm_active && iters < maxit && xm_correct(ctx, s, x, n)) {  // the x-uniform part of the residual, exactly
-            enqueue_r0(ctx, s, F, x, b, false);
-            XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        // a cycle that will run: its starting guess corrected on the x-uniform fields first (R^T r is reduced here
+        // unless it came with the first residual, so a converged solve's final check does not pay for it)
+        if (s->xm_active && iters < maxit && isfinite(s->host[R0_SLOT]) && sqrt(s->host[R0_SLOT]) > target) {
+            if (!q_ready) {
+                xm_sum(ctx, s, s->rs, s->xm_hdev);
+                XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            }
+            if (xm_correct(ctx, s, x, n)) {
+                enqueue_r0(ctx, s, F, x, b);
+                XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            }
         }
         const double beta = sqrt(s->host[R0_SLOT]);
         rel = bnorm > 0 ? beta / bnorm : beta;
@@ -3029,8 +3038,8 @@ int xc_fluidmech(xc_ctx* ctx, xc_fluid* fl, const xc_phase* ph, const xc_grid* g
     if ((st = xm_prepare(ctx, s, F, o))) return st;
     if ((st = build_rhs(ctx, s, fl, ph, g, par, o, bn, F, s->bs))) return st;
     hipLaunchKernelGGL(k_gather, dim3(nb), dim3(NT), 0, ctx->stream, fl->W, fl->U, fl->P, s->xs, Nz, Nx);
-    // the starting residual, then ||RR||: one synchronization for both
-    enqueue_r0(ctx, s, F, s->xs, s->bs);
+    // the starting residual (and its x-uniform part R^T r), then ||RR||: one synchronization for all
+    enqueue_r0(ctx, s, F, s->xs, s->bs, true);
     if ((st = multidot(ctx, s, s->bs, 0, s->bs, n))) return st;
     const double bnorm = sqrt(s->host[0]);
     if ((st = choose_depth(ctx, s, F, o, g->h, depth_pending, true))) return st;  // multidot synchronized
