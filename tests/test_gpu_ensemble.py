@@ -152,6 +152,7 @@ def test_run_ensemble_members_equal_single_runs_and_oracle(xc, orc):
             want = getattr(src, n)
             worst[n] = float(np.linalg.norm(got - want) / max(np.linalg.norm(want), 1e-300))
         print(f"  member {i}: relative differences {worst}", flush=True)
-        # ~10x the differences measured on an MI355X (P 3.3e-10, MFS 3.2e-12, U 3.6e-13, X 2e-16)
-        assert worst["P"] <= 3e-9 and max(v for k, v in worst.items() if k != "P") <= 5e-11, worst
+        # ~5-10x the differences measured on an MI355X (P 7.6e-10, MFS 8.0e-11, U 3.8e-12, W 2.3e-12, X 2e-16)
+        assert worst["P"] <= 5e-9 and worst["MFS"] <= 5e-10, worst
+        assert max(v for k, v in worst.items() if k not in ("P", "MFS")) <= 5e-11, worst
         assert abs(m.state.dt - st["dt"]) <= 1e-10 * st["dt"]

@@ -37,7 +37,7 @@ namespace {
 constexpr int NT = 256;
 
 constexpr int XM_CH = 64;      // columns per partial x-sum of the horizontal-mean correction
-constexpr int XM_PROBES = 9;   // comb probes of its Galerkin operator (3 fields x 3 residues of z mod 3)
+constexpr int XM_PROBES = 6;   // comb probes of its Galerkin operator (W and P x 3 residues of z mod 3)
 
 struct Flags {
     int bnchm, sds, top_cnv, bot_cnv, open_cnv;
@@ -373,7 +373,7 @@ __global__ __launch_bounds__(NT) void k_smooth_cells(LV L, Flags F, const TX* __
 template <bool RESID>
 __global__ __launch_bounds__(NT) void k_ll_cells(LevD L, Flags F, const double* __restrict__ x,
                                                  const double* __restrict__ b, double* __restrict__ y,
-                                                 double* __restrict__ sq = nullptr) {
+                                                 double* __restrict__ sq = nullptr, double* __restrict__ pinc = nullptr) {
     const int Nz = L.Nz, Nx = L.Nx;
     const int segs = (Nz + 2 + 63) >> 6;
     const int task = __builtin_amdgcn_readfirstlane(xcd_tile(blockIdx.x, gridDim.x) * (NT / 64) + (threadIdx.x >> 6));
@@ -499,7 +499,14 @@ __global__ __launch_bounds__(NT) void k_ll_cells(LevD L, Flags F, const double* 
             v = -rul * ih * Ul + ru0 * ih * Ur - rwm * ih * Wn + rw0 * ih * Wr;
             if (pin) v += Pr;
         }
-        const double o = RESID ? ldr(b + offP + (int64_t)c * ldP, ru) - v : v;
+        double o = RESID ? ldr(b + offP + (int64_t)c * ldP, ru) - v : v;
+        if (!RESID && pinc && pin && live) {
+            // constant-pressure fix of an FGMRES step (b = the step's Krylov vector): LL 1_P = e_pin, so with
+            // c = b_pin - (LL x)_pin the step uses x + c 1_P, whose image has b's value in the pin row
+            const double bp = b[kp];
+            *pinc = bp - v;
+            o = bp;
+        }
         if (live) {
             y[kp] = o;
             ss = fma(o, o, ss);
@@ -1559,7 +1566,8 @@ __global__ __launch_bounds__(NT) void k_dcgs_update(const double* __restrict__ V
 // k (k+1) / 2), then y.
 __global__ __launch_bounds__(NT) void k_update_x_solve(Kry K, int m, int used, int close,
                                                        const double* __restrict__ Z, int64_t n,
-                                                       double* __restrict__ x) {
+                                                       double* __restrict__ x, const double* __restrict__ pinc,
+                                                       int64_t offP) {
     extern __shared__ double sh[];
     __shared__ double s_c;
     double* R = sh;
@@ -1593,11 +1601,21 @@ __global__ __launch_bounds__(NT) void k_update_x_solve(Kry K, int m, int used, i
         }
     }
     __syncthreads();
+    // the steps' constant-pressure fixes (pinc: c_q of step q, or none): x_P += sum_q c_q y_q
+    double pc = 0.0;
+    if (pinc)
+        for (int q = 0; q < used; ++q) pc += pinc[q] * y[q];
     for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
         double sx = x[i];
         for (int q = 0; q < used; ++q) sx += y[q] * Z[(int64_t)q * n + i];
+        if (pinc && i >= offP) sx += pc;
         x[i] = sx;
     }
+}
+// x_P += c for every pressure entry (LL 1_P = e_pin: only the pin row's residual moves, by -c)
+__global__ __launch_bounds__(NT) void k_p_shift(double* __restrict__ xP, int64_t nP, double c) {
+    const int64_t i = gidx();
+    if (i < nP) xP[i] += c;
 }
 __global__ __launch_bounds__(NT) void k_zero(double* a, int64_t n) {
     const int64_t i = gidx();
@@ -1873,6 +1891,7 @@ struct StokesCtx {
     double* xm_hdev = nullptr;
     double* xm_part = nullptr;
     double* xm_delta = nullptr;
+    double* xm_pinc = nullptr;  // the FGMRES steps' constant-pressure fixes (m)
     std::vector<double> xm_lu;
     std::vector<int> xm_piv;
     bool xm_built = false;   // a probing solve has run on this grid
@@ -2132,9 +2151,10 @@ int setup(xc_ctx* ctx, int Nz, int Nx, int m, StokesCtx** out, int maxlev = -1) 
     if (e != hipSuccess) { stokes_free(ctx, s); return xc_fail(ctx, XC_ENOMEM, "fluidmech!: pinned scratch"); }
     e = hipHostGetDevicePointer((void**)&s->xm_hdev, s->xm_host, 0);
     if (e != hipSuccess) { stokes_free(ctx, s); return xc_fail(ctx, XC_EHIP, "fluidmech!: %s", hipGetErrorString(e)); }
-    e = hipMalloc((void**)&s->xm_part, (size_t)(XM_PROBES * s->xm_nch + 1) * s->xm_n0 * sizeof(double));
+    e = hipMalloc((void**)&s->xm_part, ((size_t)(XM_PROBES * s->xm_nch + 1) * s->xm_n0 + (size_t)m + 8) * sizeof(double));
     if (e != hipSuccess) { stokes_free(ctx, s); return xc_fail(ctx, XC_ENOMEM, "fluidmech!: horizontal-mean scratch"); }
     s->xm_delta = s->xm_part + (size_t)XM_PROBES * s->xm_nch * s->xm_n0;
+    s->xm_pinc = s->xm_delta + s->xm_n0;
     for (auto& ev : s->ev) {
         e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
         if (e != hipSuccess) { stokes_free(ctx, s); return xc_fail(ctx, XC_EHIP, "fluidmech!: event"); }
@@ -2739,14 +2759,15 @@ int xm_build(xc_ctx* ctx, StokesCtx* s, const Flags& F) {
     const int64_t n = s->lev[0].nv + (int64_t)(Nz + 2) * (Nx + 2);
     s->xm_ok = false;
     const bool batch = s->m >= XM_PROBES;  // the responses side by side in the (idle) basis Z, summed together
-    for (int f = 0; f < 3; ++f)
+    for (int pf = 0; pf < 2; ++pf)
         for (int k = 0; k < 3; ++k) {
-            double* resp = batch ? s->Z + (int64_t)(3 * f + k) * n : s->tmp;
+            const int f = 2 * pf, pr = 3 * pf + k;  // fields W and P
+            double* resp = batch ? s->Z + (int64_t)pr * n : s->tmp;
             hipLaunchKernelGGL(k_xm_expand, dim3(nblk(n)), dim3(NT), 0, ctx->stream, (const double*)nullptr, s->ws, n,
                                Nz, Nx, f, k);
             hipLaunchKernelGGL(k_ll_cells<false>, dim3(ll_cells_blocks(dd.Nz, dd.Nx)), dim3(NT), 0, ctx->stream, dd,
                                F, s->ws, s->ws, resp);
-            if (!batch) xm_sum(ctx, s, resp, s->xm_hdev + (size_t)(2 + 3 * f + k) * n0);
+            if (!batch) xm_sum(ctx, s, resp, s->xm_hdev + (size_t)(2 + pr) * n0);
         }
     if (batch) xm_sum(ctx, s, s->Z, s->xm_hdev + 2 * (size_t)n0, XM_PROBES, n);
     XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -2755,10 +2776,12 @@ int xm_build(xc_ctx* ctx, StokesCtx* s, const Flags& F) {
     std::vector<double>& a = s->xm_lu;
     a.assign((size_t)nb * XM_W, 0.0);
     xm_at(a, 3 * (Nz + 1), 3 * (Nz + 1)) = 1.0;  // W has no row Nz + 1
-    for (int f = 0; f < 3; ++f)
+    for (int z = 0; z < Nz + 2; ++z) xm_at(a, 3 * z + 1, 3 * z + 1) = 1.0;  // U is not corrected
+    for (int pf = 0; pf < 2; ++pf)
         for (int k = 0; k < 3; ++k) {
-            const double* resp = s->xm_host + (size_t)(2 + 3 * f + k) * n0;
-            for (int fr = 0; fr < 3; ++fr)
+            const int f = 2 * pf;
+            const double* resp = s->xm_host + (size_t)(2 + 3 * pf + k) * n0;
+            for (int fr = 0; fr < 3; fr += 2)
                 for (int zr = 0; zr < rows[fr]; ++zr) {
                     const double v = resp[xm_prof(fr, zr, Nz)];
                     if (v == 0.0) continue;
@@ -2794,7 +2817,7 @@ bool xm_correct(xc_ctx* ctx, StokesCtx* s, double* x, int64_t n) {
     const int Nz = s->Nz, n0 = s->xm_n0, nb = 3 * (Nz + 2);
     const int rows[3] = {Nz + 1, Nz + 2, Nz + 2};
     std::vector<double> b((size_t)nb, 0.0);
-    for (int f = 0; f < 3; ++f)
+    for (int f = 0; f < 3; f += 2)  // the U rows stay 0 (identity rows): U is not corrected
         for (int z = 0; z < rows[f]; ++z) b[3 * z + f] = s->xm_host[xm_prof(f, z, Nz)];
     xm_band_solve(nb, s->xm_lu, s->xm_piv, b.data());
     double* d = s->xm_host + n0;
@@ -2856,34 +2879,47 @@ int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int 
     // FGMRES allows a different preconditioner in every step.
     LcCfg cur = lc;
     const double lc_bar = pow(s->plain_rate > 0 ? s->plain_rate : 0.2, LC_COST);
+    // With the horizontal-mean correction every step also fixes its constant-pressure component exactly (the single
+    // interior pin: LL 1_P = e_pin): the preconditioner leaves errors in the pressure level that the residual hardly
+    // sees, and without the fix they differ from one Picard sweep to the next, which the Picard test reads as a
+    // change of P (config 1: 210 sweeps in 20 steps instead of the 160 of the sparse-LU oracle, with it 160).
+    const bool pinfix = s->xm_active && F.pin_kind == 1 && !F.bnchm;
     int lsteps = 0;
     bool dropped = false;
     constexpr double NEAR_CONV = 3.0;
     bool first = true;
     while (true) {
-        const bool q_ready = first && r0_ready;  // the first residual came with R^T r (see xc_fluidmech)
         if (!(first && r0_ready)) {  // r = b - LL x and ||r||^2 (per-wave partials, summed into the mapped host buffer)
-            enqueue_r0(ctx, s, F, x, b);
+            enqueue_r0(ctx, s, F, x, b, s->xm_active);  // with R^T r (the first residual has it: xc_fluidmech)
             XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
         }
         first = false;
-        // a cycle that will run: its starting guess corrected on the x-uniform fields first (R^T r is reduced here
-        // unless it came with the first residual, so a converged solve's final check does not pay for it)
-        if (s->xm_active && iters < maxit && isfinite(s->host[R0_SLOT]) && sqrt(s->host[R0_SLOT]) > target) {
-            if (!q_ready) {
-                xm_sum(ctx, s, s->rs, s->xm_hdev);
-                XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-            }
-            if (xm_correct(ctx, s, x, n)) {
-                enqueue_r0(ctx, s, F, x, b);
-                XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-            }
+        // a cycle that will run: its starting guess corrected on the x-uniform fields first
+        if (s->xm_active && iters < maxit && isfinite(s->host[R0_SLOT]) && sqrt(s->host[R0_SLOT]) > target &&
+            xm_correct(ctx, s, x, n)) {
+            enqueue_r0(ctx, s, F, x, b);
+            XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
         }
         const double beta = sqrt(s->host[R0_SLOT]);
         rel = bnorm > 0 ? beta / bnorm : beta;
         if (rel0 < 0) rel0 = rel;
         if (!isfinite(beta)) break;  // not converged; a learned solve falls back to the plain one
-        if (beta <= target) { conv = true; break; }
+        if (beta <= target) {
+            conv = true;
+            if (pinfix) {
+                // The steps fixed the pressure level through the pin row, where it rests on the divergence of one
+                // cell; set it instead from the global mass balance, the sum of every divergence row, which the
+                // boundary fluxes fix: x_P += c with c the sum of the final residual over the interior P rows (its
+                // row sums R^T r came with it)
+                const int Nz = dd.Nz;
+                double c = 0.0;
+                for (int z = 1; z <= Nz; ++z) c += s->xm_host[xm_prof(2, z, Nz)];
+                const int64_t nP = (int64_t)(Nz + 2) * (dd.Nx + 2);
+                if (isfinite(c))
+                    hipLaunchKernelGGL(k_p_shift, dim3(nblk(nP)), dim3(NT), 0, ctx->stream, x + (n - nP), nP, c);
+            }
+            break;
+        }
         if (iters >= maxit) break;
         hipLaunchKernelGGL(k_cycle_start, dim3(nb), dim3(NT), 0, ctx->stream, s->rs, 1.0 / beta, s->V, n, K, m, beta);
         int used = 0;  // basis columns whose results are final
@@ -2910,7 +2946,7 @@ int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int 
             }
             if ((st = precond_full(ctx, s, F, mo, n, cur, vj, zj))) return st;
             hipLaunchKernelGGL(k_ll_cells<false>, dim3(ll_cells_blocks(dd.Nz, dd.Nx)), dim3(NT), 0, ctx->stream, dd, F,
-                               zj, zj, s->ws);
+                               zj, pinfix ? vj : zj, s->ws, (double*)nullptr, pinfix ? s->xm_pinc + j : (double*)nullptr);
             {
                 const dim3 g((unsigned)md_blocks(n)), b(NT), g1(1);
                 switch (md_ept(n)) {
@@ -2954,7 +2990,8 @@ int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int 
             const unsigned gu = nb < 1024u ? nb : 1024u;
             const size_t lds = ((size_t)used * (used + 1) / 2 + (size_t)used) * sizeof(double);
             hipLaunchKernelGGL(k_update_x_solve, dim3(gu), dim3(NT), lds, ctx->stream, K, m, used, close ? 1 : 0,
-                               s->Z, n, x);
+                               s->Z, n, x, pinfix ? (const double*)s->xm_pinc : (const double*)nullptr,
+                               n - (int64_t)(dd.Nz + 2) * (dd.Nx + 2));
         }
     }
     out->iters = iters;
