@@ -52,11 +52,11 @@ NFIELDS = 4
 NT_STORE_BYTES = 64 << 20
 TILE_SHAPES = {"64x16": (64, 16), "128x8": (128, 8), "64x8": (64, 8), "32x32": (32, 32), "64x16p": (64, -16),
                "64x8p": (64, -8)}
-# HBM bytes per launch of the WENO5 stencil at 2048^2 (64 x 16 tiles, non-temporal stores), measured with
+# HBM bytes per launch of the WENO5 stencil at 2048^2 (64 x 16 tiles, non-temporal loads and stores), measured with
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE in separate runs of `bench.py --no-step --no-cpu --no-ensemble
-# --calib-copy-mb 1024`: 49,715 KiB x 2.0 read + 103,563 KiB x 1.0 written, the factors being what a 1 GiB
-# device copy reads on the same counters (524,309 and 1,048,576 KiB).  1.032 x the algorithmic bytes.
-TRAFFIC_WENO5_2048 = 207861103
+# --calib-copy-mb 1024`: 51,778 KiB x 2.0 read + 103,914 KiB x 1.0 written, the factors being what a 1 GiB
+# device copy reads on the same counters (524,302 and 1,048,576 KiB).  1.055 x the algorithmic bytes.
+TRAFFIC_WENO5_2048 = 212446368
 
 
 def make_inputs(N, seed_base=1001):
