@@ -501,23 +501,29 @@ def ou_setup(N=1024, seed=4001):
     return par, g, sc, ph, ns, dt
 
 
-def ou_bench(N=1024, reps=20):
+def ou_bench(N=1024, reps=20, batches=5, warm=5):
     """noise! (OU update of three potentials, their Gaussian filters with FFT length 1040 x 1024, the noise fluxes
     and magnitudes) at 1024^2 with fresh white noise every call: time per call from HIP events on the launch
-    stream; algorithmic bytes = every array it reads or writes once."""
+    stream, the median of `batches` batches of `reps` calls after `warm` untimed calls (single batches of a fresh
+    process have read anywhere from 178 to 230 us per call); algorithmic bytes = every array it reads or writes
+    once."""
     import torch
     import xcore_amd as xc
     par, g, sc, ph, ns, dt = ou_setup(N)
-    xc.noise_(ns, ph, g, par, sc, dt, first_iter=True)
+    for _ in range(warm):
+        xc.noise_(ns, ph, g, par, sc, dt, first_iter=True)
     s = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    e0.record(s)
-    for _ in range(reps):
-        xc.noise_(ns, ph, g, par, sc, dt, first_iter=True)
-    e1.record(s)
-    torch.cuda.synchronize()
-    us = 1e3 * e0.elapsed_time(e1) / reps
+    per = []
+    for _ in range(batches):
+        torch.cuda.synchronize()
+        e0.record(s)
+        for _ in range(reps):
+            xc.noise_(ns, ph, g, par, sc, dt, first_iter=True)
+        e1.record(s)
+        torch.cuda.synchronize()
+        per.append(1e3 * e0.elapsed_time(e1) / reps)
+    us = sorted(per)[len(per) // 2]
     cell, wt, ut = N * N, (N + 1) * (N + 2), (N + 2) * (N + 1)
     # read: psi*o (3), V, vx, fReL, ke, chi, ks (cells), chiw, muw (W faces), chiu, muu (U faces);
     # written: re, rs, psi* (3), xie/xix/xis (cells), 3 W-face and 3 U-face fluxes

@@ -490,21 +490,46 @@ __global__ __launch_bounds__(NT) void k_noise_flux(FluxArgs a) {
     auto psiec = [&](int jj, int ii) -> double {
         return (ext(0, jj, ii) + ext(0, jj, ii + 1) + ext(0, jj + 1, ii) + ext(0, jj + 1, ii + 1)) / 4.0;
     };
+    auto taper = [](double chi, double mu) {
+        const double m = mu - 0.4;
+        return (1.0 - exp(-chi / 1e-4)) * (1.0 - exp(-(m > 0.0 ? m : 0.0) / 0.05));
+    };
+    if (j >= 2 && j <= Nz - 1 && i >= 2 && i <= Nx - 1) {
+        // away from the ghost rows and columns every ext(f, jj, ii) above is the rescaled value at
+        // (jj - 1, ii - 1) itself: the same terms in the same order, each distinct one evaluated once (8 of
+        // psi's 16, 3 of the 4 of each other field), psiec(j, i) shared by both face types
+        auto g = [&](int f, int z, int x) -> double {
+            const double v = a.flt[f][z + (int64_t)x * a.ldf[f]];
+            return on[f] ? (v - fm[f]) * rs[f] / fs[f] + rm[f] : v;
+        };
+        const double p11 = g(0, j - 1, i - 1), p12 = g(0, j - 1, i), p21 = g(0, j, i - 1), p22 = g(0, j, i);
+        const double p01 = g(0, j - 2, i - 1), p02 = g(0, j - 2, i), p10 = g(0, j - 1, i - 2), p20 = g(0, j, i - 2);
+        const double ps = (p11 + p12 + p21 + p22) / 4.0;           // psiec(j, i)
+        const double pu = (p01 + p02 + p11 + p12) / 4.0;           // psiec(j - 1, i)
+        const double pw = (p10 + p11 + p20 + p21) / 4.0;           // psiec(j, i - 1)
+        const double x11 = g(1, j - 1, i - 1), x12 = g(1, j - 1, i), x21 = g(1, j, i - 1);
+        const double s11 = g(2, j - 1, i - 1), s12 = g(2, j - 1, i), s21 = g(2, j, i - 1);
+        const double tu = taper(a.chiu[j + (int64_t)i * a.ldchiu], a.muu[j + (int64_t)i * a.ldmuu]);
+        const double tw = taper(a.chiw[j + (int64_t)i * a.ldchiw], a.muw[j + (int64_t)i * a.ldmuw]);
+        a.xieu[j + (int64_t)i * a.ldeu] = ps - pu;
+        a.xixu[j + (int64_t)i * a.ldxu] = (-(x12 - x11)) * tu;
+        a.xisu[j + (int64_t)i * a.ldsu] = (-(s12 - s11)) * tu;
+        a.xiew[j + (int64_t)i * a.ldew] = -(ps - pw);
+        a.xixw[j + (int64_t)i * a.ldxw] = (-(x21 - x11)) * tw;
+        a.xisw[j + (int64_t)i * a.ldsw] = (-(s21 - s11)) * tw;
+        return;
+    }
     if (i <= Nx) {  // U-type (Nz+2, Nx+1)
         const int z = j - 1 < 0 ? 0 : (j - 1 >= Nz ? Nz - 1 : j - 1);
         a.xieu[j + (int64_t)i * a.ldeu] = psiec(z + 1, i) - psiec(z, i);
-        const double mu = a.muu[j + (int64_t)i * a.ldmuu] - 0.4;
-        const double tap = (1.0 - exp(-a.chiu[j + (int64_t)i * a.ldchiu] / 1e-4)) *
-                           (1.0 - exp(-(mu > 0.0 ? mu : 0.0) / 0.05));
+        const double tap = taper(a.chiu[j + (int64_t)i * a.ldchiu], a.muu[j + (int64_t)i * a.ldmuu]);
         a.xixu[j + (int64_t)i * a.ldxu] = (-(ext(1, j, i + 1) - ext(1, j, i))) * tap;
         a.xisu[j + (int64_t)i * a.ldsu] = (-(ext(2, j, i + 1) - ext(2, j, i))) * tap;
     }
     if (j <= Nz) {  // W-type (Nz+1, Nx+2)
         const int x = i - 1 < 0 ? Nx - 1 : (i - 1 >= Nx ? 0 : i - 1);
         a.xiew[j + (int64_t)i * a.ldew] = -(psiec(j, x + 1) - psiec(j, x));
-        const double mu = a.muw[j + (int64_t)i * a.ldmuw] - 0.4;
-        const double tap = (1.0 - exp(-a.chiw[j + (int64_t)i * a.ldchiw] / 1e-4)) *
-                           (1.0 - exp(-(mu > 0.0 ? mu : 0.0) / 0.05));
+        const double tap = taper(a.chiw[j + (int64_t)i * a.ldchiw], a.muw[j + (int64_t)i * a.ldmuw]);
         a.xixw[j + (int64_t)i * a.ldxw] = (-(ext(1, j + 1, i) - ext(1, j, i))) * tap;
         a.xisw[j + (int64_t)i * a.ldsw] = (-(ext(2, j + 1, i) - ext(2, j, i))) * tap;
     }

@@ -277,7 +277,8 @@ struct StepCtx {
     int snap_Nz = -1, snap_Nx = -1;  // grid of the snapshot held (-1: none)
     double* part = nullptr;   // reduction partials (resnorm, history sums, statistics, rp)
     double* out = nullptr;    // reduction results
-    double* host = nullptr;   // pinned copy of out
+    double* host = nullptr;   // pinned copy of out (mapped, coherent)
+    double* host_dev = nullptr;  // the same memory as seen by kernels
     double* rp = nullptr;     // initialize!'s random perturbation
     size_t rp_n = 0;
 };
@@ -304,7 +305,9 @@ int step_ctx(xc_ctx* ctx, StepCtx** out) {
         ctx->ext_free[XC_EXT_STEP] = step_free;
         XC_TRY_HIP(ctx, hipMalloc(&s->part, PART_N * sizeof(double)));
         XC_TRY_HIP(ctx, hipMalloc(&s->out, OUT_N * sizeof(double)));
-        XC_TRY_HIP(ctx, hipHostMalloc((void**)&s->host, OUT_N * sizeof(double), hipHostMallocDefault));
+        XC_TRY_HIP(ctx, hipHostMalloc((void**)&s->host, OUT_N * sizeof(double),
+                                      hipHostMallocMapped | hipHostMallocCoherent));
+        XC_TRY_HIP(ctx, hipHostGetDevicePointer((void**)&s->host_dev, s->host, 0));
     }
     *out = s;
     return XC_OK;
@@ -420,8 +423,8 @@ int xc_compute_resnorm(xc_ctx* ctx, const xc_phase* ph, const xc_fluid* fl, doub
                        s->snap_Nz, s->snap_Nx, Nz, Nx);
     hipLaunchKernelGGL(k_resnorm_part, dim3(RED_BLOCKS), dim3(NT), 0, ctx->stream, fl->W, fl->U, fl->P, ph->X,
                        fl->MFS, fl->rho, s->snap, Nz, Nx, s->part);
-    hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(9 * 64), 0, ctx->stream, s->part, RED_BLOCKS, 9, s->out);
-    XC_TRY_HIP(ctx, hipMemcpyAsync(s->host, s->out, 9 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    // the nine sums straight into the mapped host buffer
+    hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(9 * 64), 0, ctx->stream, s->part, RED_BLOCKS, 9, s->host_dev);
     XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const double* a = s->host;
     const double ep = 2.220446049250313e-16;

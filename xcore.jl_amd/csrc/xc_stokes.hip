@@ -1173,6 +1173,28 @@ __global__ __launch_bounds__(NT) void k_dot_final(const double* __restrict__ par
     const double t = xc_rows_reduce<false>(part, nb, q);
     if ((threadIdx.x & 63) == 0) out[q] = t;
 }
+// out[0] = the sum of nb partials (fixed order) by the whole workgroup, sixteen loads in flight per lane: for
+// the thousands of per-wave partials of the residual kernel, which one wave would read in many dependent rounds
+__global__ __launch_bounds__(NT) void k_sum_final1(const double* __restrict__ part, int nb, double* out) {
+    __shared__ double red[NT / 64];
+    constexpr int U = 16;
+    double acc = 0.0;
+    for (int b0 = threadIdx.x; b0 < nb; b0 += NT * U) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int b = b0 + NT * u;
+            v[u] = part[b < nb ? b : nb - 1];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (b0 + NT * u < nb) acc += v[u];
+    }
+    acc = wave_total(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) out[0] = (red[0] + red[1]) + (red[2] + red[3]);
+}
 // largest viscosity and smallest density of the fine level from k_pack5's per-block partials (one wave)
 __global__ __launch_bounds__(64) void k_eta_rho_final(const double* __restrict__ part, int nb, double* out) {
     double emax = 0.0, rmin = 1e308;
@@ -1332,6 +1354,20 @@ __global__ __launch_bounds__(NT) void k_cycle_start(const double* __restrict__ r
     if (i < n) V[i] = r[i] * inv_beta;
     if (blockIdx.x == 0)
         for (int q = threadIdx.x; q <= m; q += NT) K.g[q] = q == 0 ? beta : 0.0;
+}
+// the same with beta^2 = ||r||^2 still on the device (b2), so that the host need not wait for it; the first
+// workgroup also hands beta^2 to the host (hb2)
+__global__ __launch_bounds__(NT) void k_cycle_start_dev(const double* __restrict__ r, const double* __restrict__ b2,
+                                                        double* __restrict__ V, int64_t n, Kry K, int m,
+                                                        double* hb2) {
+    const double bb = *b2;
+    const double beta = sqrt(bb), inv_beta = 1.0 / beta;
+    const int64_t i = gidx();
+    if (i < n) V[i] = r[i] * inv_beta;
+    if (blockIdx.x == 0) {
+        for (int q = threadIdx.x; q <= m; q += NT) K.g[q] = q == 0 ? beta : 0.0;
+        if (threadIdx.x == 0) *hb2 = bb;
+    }
 }
 // ---- Arnoldi with delayed reorthogonalisation (DCGS2, classical Gram-Schmidt twice with the second
 // pass of v_j folded into step j+1: Bielich, Langou, Thomas, Swirydowicz, Yamazaki, Boman, "Low-synch
@@ -1892,6 +1928,7 @@ struct StokesCtx {
     double* xm_part = nullptr;
     double* xm_delta = nullptr;
     double* xm_pinc = nullptr;  // the FGMRES steps' constant-pressure fixes (m)
+    double* xm_b2 = nullptr;    // ||r||^2 of the corrected starting guess (device; the cycle start reads it)
     std::vector<double> xm_lu;
     std::vector<int> xm_piv;
     bool xm_built = false;   // a probing solve has run on this grid
@@ -2155,6 +2192,7 @@ int setup(xc_ctx* ctx, int Nz, int Nx, int m, StokesCtx** out, int maxlev = -1) 
     if (e != hipSuccess) { stokes_free(ctx, s); return xc_fail(ctx, XC_ENOMEM, "fluidmech!: horizontal-mean scratch"); }
     s->xm_delta = s->xm_part + (size_t)XM_PROBES * s->xm_nch * s->xm_n0;
     s->xm_pinc = s->xm_delta + s->xm_n0;
+    s->xm_b2 = s->xm_pinc + m;
     for (auto& ev : s->ev) {
         e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
         if (e != hipSuccess) { stokes_free(ctx, s); return xc_fail(ctx, XC_EHIP, "fluidmech!: event"); }
@@ -2837,13 +2875,15 @@ bool xm_correct(xc_ctx* ctx, StokesCtx* s, double* x, int64_t n) {
 // The residual of the starting x: r0_ready: already in s->rs with its squared norm in s->host[R0_SLOT] (enqueued by
 // the caller before a synchronization it needed anyway, enqueue_r0), else computed here.
 constexpr int R0_SLOT = 4;
-void enqueue_r0(xc_ctx* ctx, StokesCtx* s, const Flags& F, const double* x, const double* b, bool with_q = false) {
+// b2: where ||r||^2 goes (default: the host's slot)
+void enqueue_r0(xc_ctx* ctx, StokesCtx* s, const Flags& F, const double* x, const double* b, bool with_q = false,
+                double* b2 = nullptr) {
     const LevD dd = s->lev[0].d();
     const unsigned nbl = ll_cells_blocks(dd.Nz, dd.Nx);
     double* part = s->part + DOT_BLOCKS;
     hipLaunchKernelGGL(k_ll_cells<true>, dim3(nbl), dim3(NT), 0, ctx->stream, dd, F, x, b, s->rs, part);
-    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(NT), 0, ctx->stream, part, (int)(nbl * (NT / 64)), 1,
-                       s->host_dev + R0_SLOT);
+    hipLaunchKernelGGL(k_sum_final1, dim3(1), dim3(NT), 0, ctx->stream, (const double*)part, (int)(nbl * (NT / 64)),
+                       b2 ? b2 : s->host_dev + R0_SLOT);
     if (s->xm_active && with_q) xm_sum(ctx, s, s->rs, s->xm_hdev);  // R^T r for the cycle start's correction
 }
 int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int m, int64_t n, const double* b,
@@ -2894,17 +2934,28 @@ int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int 
             XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
         }
         first = false;
-        // a cycle that will run: its starting guess corrected on the x-uniform fields first
-        if (s->xm_active && iters < maxit && isfinite(s->host[R0_SLOT]) && sqrt(s->host[R0_SLOT]) > target &&
-            xm_correct(ctx, s, x, n)) {
-            enqueue_r0(ctx, s, F, x, b);
-            XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        // a cycle that will run: its starting guess corrected on the x-uniform fields first.  The corrected
+        // residual's norm stays on the device (the cycle start reads it); the host reads it once step 0 is done,
+        // which it waits for anyway.  (Should the correction have converged the solve, the cycle's first step
+        // finds that out.)
+        const bool dev_beta = s->xm_active && iters < maxit && isfinite(s->host[R0_SLOT]) &&
+                              sqrt(s->host[R0_SLOT]) > target && xm_correct(ctx, s, x, n);
+        if (dev_beta) enqueue_r0(ctx, s, F, x, b, false, s->xm_b2);
+        double beta = sqrt(s->host[R0_SLOT]);
+        bool beta_known = !dev_beta;
+        auto know_beta = [&]() {  // once step 0 has completed (its event waited for)
+            if (beta_known) return;
+            beta_known = true;
+            beta = sqrt(s->host[R0_SLOT]);
+            rel = bnorm > 0 ? beta / bnorm : beta;
+            if (rel0 < 0) rel0 = rel;
+        };
+        if (beta_known) {
+            rel = bnorm > 0 ? beta / bnorm : beta;
+            if (rel0 < 0) rel0 = rel;
         }
-        const double beta = sqrt(s->host[R0_SLOT]);
-        rel = bnorm > 0 ? beta / bnorm : beta;
-        if (rel0 < 0) rel0 = rel;
-        if (!isfinite(beta)) break;  // not converged; a learned solve falls back to the plain one
-        if (beta <= target) {
+        if (beta_known && !isfinite(beta)) break;  // not converged; a learned solve falls back to the plain one
+        if (beta_known && beta <= target) {
             conv = true;
             if (pinfix) {
                 // The steps fixed the pressure level through the pin row, where it rests on the divergence of one
@@ -2921,7 +2972,12 @@ int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int 
             break;
         }
         if (iters >= maxit) break;
-        hipLaunchKernelGGL(k_cycle_start, dim3(nb), dim3(NT), 0, ctx->stream, s->rs, 1.0 / beta, s->V, n, K, m, beta);
+        if (dev_beta)
+            hipLaunchKernelGGL(k_cycle_start_dev, dim3(nb), dim3(NT), 0, ctx->stream, s->rs, (const double*)s->xm_b2,
+                               s->V, n, K, m, s->host_dev + R0_SLOT);
+        else
+            hipLaunchKernelGGL(k_cycle_start, dim3(nb), dim3(NT), 0, ctx->stream, s->rs, 1.0 / beta, s->V, n, K, m,
+                               beta);
         int used = 0;  // basis columns whose results are final
         int j = 0;
         bool waited = false;  // step j - 1 was waited for before step j was enqueued
@@ -2963,6 +3019,8 @@ int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int 
             XC_TRY_HIP(ctx, hipEventRecord(s->ev[j & 1], ctx->stream));
             if (cur.on) {  // judge the learned step now (the pipeline waits for it)
                 XC_TRY_HIP(ctx, hipEventSynchronize(s->ev[j & 1]));
+                know_beta();
+                if (!isfinite(beta)) break;
                 const double before = j == 0 ? beta : hout[2 * (j - 1)];
                 ++lsteps;
                 if (!(hout[2 * j] <= before * lc_bar)) {
@@ -2973,12 +3031,19 @@ int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int 
             if (j == 0 || waited) continue;
             // step j-1 is complete once its event has fired; step j (enqueued) finalises its column
             XC_TRY_HIP(ctx, hipEventSynchronize(s->ev[(j - 1) & 1]));
+            know_beta();
+            if (!isfinite(beta)) break;
             used = j;
             ++iters;
             if (hout[2 * (j - 1)] <= 0.5 * target || hout[2 * (j - 1) + 1] == 0.0 || iters >= maxit) break;
         }
         // column used - 1 keeps its provisional values when the cycle converged before its next step was
         // enqueued (early) or when the cycle is full (its last step was not tested yet)
+        if (!beta_known) {  // the cycle ended before step 0 was waited for (m = 1)
+            XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            know_beta();
+        }
+        if (!isfinite(beta)) break;  // a non-finite corrected residual: not converged
         bool close = early;
         if (!early && j == m) {
             XC_TRY_HIP(ctx, hipEventSynchronize(s->ev[(m - 1) & 1]));
