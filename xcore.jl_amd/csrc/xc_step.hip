@@ -281,6 +281,8 @@ struct StepCtx {
     double* host_dev = nullptr;  // the same memory as seen by kernels
     double* rp = nullptr;     // initialize!'s random perturbation
     size_t rp_n = 0;
+    hipStream_t side = nullptr;  // run!'s second stream (noise!), default priority
+    hipEvent_t ev_go = nullptr, ev_noise = nullptr;
 };
 // partials: the history sums (NHS x RED_BLOCKS) followed by the field statistics (NHF x NHQ x HST_NB); the
 // residual norms (9 x RED_BLOCKS) and initialize!'s two sums use the front of the same buffer
@@ -294,6 +296,12 @@ void step_free(xc_ctx*, void* p) {
     if (s->out) (void)hipFree(s->out);
     if (s->host) (void)hipHostFree(s->host);
     if (s->rp) (void)hipFree(s->rp);
+    if (s->side) {
+        (void)hipStreamSynchronize(s->side);
+        (void)hipStreamDestroy(s->side);
+    }
+    if (s->ev_go) (void)hipEventDestroy(s->ev_go);
+    if (s->ev_noise) (void)hipEventDestroy(s->ev_noise);
     delete s;
 }
 // the context's reduction buffers (the snapshot buffer is made by xc_snapshot only)
@@ -310,6 +318,27 @@ int step_ctx(xc_ctx* ctx, StepCtx** out) {
         XC_TRY_HIP(ctx, hipHostGetDevicePointer((void**)&s->host_dev, s->host, 0));
     }
     *out = s;
+    return XC_OK;
+}
+// noise! of one Picard sweep on run!'s second stream, after the sweep's start (ev_go), its end marked by ev_noise
+// (recorded on every path: the caller waits for it before it returns)
+int noise_side(xc_ctx* ctx, StepCtx* sx, xc_noise* ns, const xc_phase* ph, const xc_grid* g, const xc_par* par,
+               const xc_scales* sc, double dt, bool first) {
+    XC_TRY_HIP(ctx, hipStreamWaitEvent(sx->side, sx->ev_go, 0));
+    const hipStream_t main = ctx->stream;
+    ctx->stream = sx->side;
+    const int st = xc_noise_step(ctx, ns, ph, g, par, sc, dt, first ? 1 : 0);
+    ctx->stream = main;
+    XC_TRY_HIP(ctx, hipEventRecord(sx->ev_noise, sx->side));
+    if (st) XC_TRY_HIP(ctx, hipStreamWaitEvent(ctx->stream, sx->ev_noise, 0));
+    return st;
+}
+// run!'s second stream and its two events (made once per context)
+int step_side(xc_ctx* ctx, StepCtx* s) {
+    if (s->side) return XC_OK;
+    XC_TRY_HIP(ctx, hipEventCreateWithFlags(&s->ev_go, hipEventDisableTiming));
+    XC_TRY_HIP(ctx, hipEventCreateWithFlags(&s->ev_noise, hipEventDisableTiming));
+    XC_TRY_HIP(ctx, hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));
     return XC_OK;
 }
 
@@ -540,11 +569,24 @@ int xc_run_step(xc_ctx* ctx, xc_phase* ph, xc_fluid* fl, xc_noise* ns, const xc_
     fo.dt = dt;
     fo.a1 = c[0]; fo.a2 = c[1]; fo.a3 = c[2]; fo.b1 = c[3]; fo.b2 = c[4]; fo.b3 = c[5];
     fo.xbc = so->xbc; fo.zbc = so->zbc; fo.scheme = so->scheme;
+    StepCtx* sx;
+    if ((st = step_ctx(ctx, &sx)) || (st = step_side(ctx, sx))) return st;
     while (fixed > 0 ? iter < fixed
                      : ((resnorm / resnorm0 >= so->rtol && resnorm >= so->atol && iter < so->maxit) || iter < 3)) {
         ++iter;
+        // noise! reads what the previous sweep's update! wrote and its own state only, and snapshot!, phsevo! and
+        // fluidmech! neither read nor write its fields: it runs on the second stream beside them (results
+        // identical), from the start of the sweep (ev_go), enqueued after phsevo! so that the device starts on the
+        // sweep's own chain first, and joined before update_phase_velocities!, which reads both
+        XC_TRY_HIP(ctx, hipEventRecord(sx->ev_go, ctx->stream));
         if ((st = xc_snapshot(ctx, ph, fl))) return st;
         if ((st = xc_phsevo(ctx, ph, fl, g, par, sc, so->scheme, so->xbc, so->zbc, c, dt, par->alpha))) return st;
+        if ((st = noise_side(ctx, sx, ns, ph, g, par, sc, dt, iter <= 1))) return st;
+        // every later return leaves the context's stream ordered after the noise work
+        auto join = [&]() -> int {
+            const hipError_t e = hipStreamWaitEvent(ctx->stream, sx->ev_noise, 0);
+            return e == hipSuccess ? XC_OK : xc_fail(ctx, XC_EHIP, "run!: %s", hipGetErrorString(e));
+        };
         xc_solve_stats fs{};
         // the horizontal-mean correction's operator: probed in the step's first sweep, reused in the others
         if (so->fm.xmean == 0) fo.xmean = iter == 1 ? 1 : 2;
@@ -561,8 +603,9 @@ int xc_run_step(xc_ctx* ctx, xc_phase* ph, xc_fluid* fl, xc_noise* ns, const xc_
             fm_init_later = fs.init_rel_resid;
         }
         if (fs.rel_resid > fm_worst) fm_worst = fs.rel_resid;
+        const int sj = join();
         if (st) return st;
-        if ((st = xc_noise_step(ctx, ns, ph, g, par, sc, dt, iter <= 1))) return st;
+        if (sj) return sj;
         if ((st = xc_update_phase_velocities(ctx, ph, fl, ns, g, par, so->xbc, so->zbc))) return st;
         if ((st = xc_update(ctx, ph, fl, g, par, sc, so->xbc, so->zbc))) return st;
         double rn[3];
