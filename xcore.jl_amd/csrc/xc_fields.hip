@@ -618,8 +618,11 @@ int xc_time_coefs(int tint, int64_t step, double c[6]) {
     return XC_OK;
 }
 
-int xc_update(xc_ctx* ctx, xc_phase* ph, xc_fluid* fl, const xc_grid* g, const xc_par* par, const xc_scales* sc,
-              int xbc, int zbc) {
+// update! with the lithostatic chain (row sums of rhow, their scan, Drho / Pl / Pt), which nothing else in
+// update! reads, on the stream `side` beside the cell kernels (ev_a: its start after k_interp, ev_b: its end,
+// which the context's stream waits for before returning); side == nullptr: all on the context's stream
+int xc_update_side(xc_ctx* ctx, xc_phase* ph, xc_fluid* fl, const xc_grid* g, const xc_par* par,
+                   const xc_scales* sc, int xbc, int zbc, hipStream_t side, hipEvent_t ev_a, hipEvent_t ev_b) {
     XC_CHECK_CTX(ctx);
     int st;
     if ((st = check_grid(ctx, g)) || (st = check_bc(ctx, xbc, zbc))) return st;
@@ -649,14 +652,18 @@ int xc_update(xc_ctx* ctx, xc_phase* ph, xc_fluid* fl, const xc_grid* g, const x
                   dm(ph->x_w), dm(ph->m_w), dm(ph->x_u), dm(ph->m_u), db(ph->hasx), db(ph->hasm), p,
                   xc_make_dom(Nz + 2, Nx + 2)};
     LAUNCH(k_interp, ia.d, ia);
-    hipLaunchKernelGGL(k_rowsum, dim3((rows + 63) / 64, nchunk), dim3(64), 0, ctx->stream, dm(fl->rhow), rows,
-                       (int)Nx, part);
-    hipLaunchKernelGGL(k_rowmean_scan, dim3(1), dim3(RM_NT), 2 * (((rows + 15) & ~15) + 16) * sizeof(double), ctx->stream,
-                       part, rows, nchunk,
-                       (int)Nx, par->g0, g->h, par->Ptop, ref, plcol);
+    const hipStream_t ls = side ? side : ctx->stream;
+    if (side) {
+        XC_TRY_HIP(ctx, hipEventRecord(ev_a, ctx->stream));
+        XC_TRY_HIP(ctx, hipStreamWaitEvent(side, ev_a, 0));
+    }
+    hipLaunchKernelGGL(k_rowsum, dim3((rows + 63) / 64, nchunk), dim3(64), 0, ls, dm(fl->rhow), rows, (int)Nx, part);
+    hipLaunchKernelGGL(k_rowmean_scan, dim3(1), dim3(RM_NT), 2 * (((rows + 15) & ~15) + 16) * sizeof(double), ls,
+                       part, rows, nchunk, (int)Nx, par->g0, g->h, par->Ptop, ref, plcol);
     xc_dom dZ = xc_make_dom(Nz + 1, Nx);
-    LAUNCH(k_drho_pt, dZ, dm(fl->rhow), dm(fl->Drho), dm(ph->Pl), dm(ph->Pt), dm(fl->P), ref, plcol,
-           par->Ptop / 100.0, (int)Nz, (int)Nx, dZ);
+    hipLaunchKernelGGL(k_drho_pt, dim3(dZ.nb), dim3(NT), 0, ls, dm(fl->rhow), dm(fl->Drho), dm(ph->Pl), dm(ph->Pt),
+                       dm(fl->P), (const double*)ref, (const double*)plcol, par->Ptop / 100.0, (int)Nz, (int)Nx, dZ);
+    if (side) XC_TRY_HIP(ctx, hipEventRecord(ev_b, side));
     CellArgs ca{dm(ph->chi), dm(ph->mu), dm(ph->etamix), dm(fl->W), dm(fl->U), dm(ph->exx), dm(ph->ezz),
                 dm(ph->Div_V), dm(ph->exz), dm(ph->eII), dm(ph->V), dm(ph->wx), dm(ph->wm), dm(ph->vx),
                 dm(ph->vm), dm(fl->rho), dm(ph->etae), dm(ph->ke), dm(ph->kx), dm(ph->ReL), dm(ph->fReL),
@@ -669,7 +676,12 @@ int xc_update(xc_ctx* ctx, xc_phase* ph, xc_fluid* fl, const xc_grid* g, const x
     LAUNCH(k_corner, dK, dm(fl->eta), dm(fl->etaco), dm(ph->etas), dm(ph->etasw), (int)Nz, (int)Nx, zbc, dK);
     LAUNCH(k_stress, dK, dm(fl->etaco), dm(ph->exz), dm(ph->txz), dm(fl->eta), dm(ph->exx), dm(ph->ezz),
            dm(ph->txx), dm(ph->tzz), dm(ph->tII), dm(ph->Ra), dm(ph->V), dm(ph->kx), sc->D0, (int)Nz, (int)Nx, dK);
+    if (side) XC_TRY_HIP(ctx, hipStreamWaitEvent(ctx->stream, ev_b, 0));
     return xc_finish(ctx, "update!");
+}
+int xc_update(xc_ctx* ctx, xc_phase* ph, xc_fluid* fl, const xc_grid* g, const xc_par* par, const xc_scales* sc,
+              int xbc, int zbc) {
+    return xc_update_side(ctx, ph, fl, g, par, sc, xbc, zbc, nullptr, nullptr, nullptr);
 }
 
 int xc_phsevo(xc_ctx* ctx, xc_phase* ph, xc_fluid* fl, const xc_grid* g, const xc_par* par, const xc_scales* sc,

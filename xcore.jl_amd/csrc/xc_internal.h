@@ -39,6 +39,10 @@ int xc_finish(xc_ctx* ctx, const char* what);
 // of two xc_advect_centered calls; name: the caller, for error messages
 extern "C" int xc_advect_centered_pair(xc_ctx* ctx, int scheme, const xc_adv_member* m, int xbc, int zbc,
                                        const char* name);
+// update! with its lithostatic chain on a second stream (xc_fields.hip; side == nullptr: xc_update)
+extern "C" int xc_update_side(xc_ctx* ctx, xc_phase* ph, xc_fluid* fl, const xc_grid* g, const xc_par* par,
+                              const xc_scales* sc, int xbc, int zbc, hipStream_t side, hipEvent_t ev_a,
+                              hipEvent_t ev_b);
 // Device scratch of at least `bytes` in `slot` (contents undefined).
 int xc_scratch(xc_ctx* ctx, int slot, size_t bytes, void** out);
 

@@ -282,7 +282,7 @@ struct StepCtx {
     double* rp = nullptr;     // initialize!'s random perturbation
     size_t rp_n = 0;
     hipStream_t side = nullptr;  // run!'s second stream (noise!), default priority
-    hipEvent_t ev_go = nullptr, ev_noise = nullptr;
+    hipEvent_t ev_go = nullptr, ev_noise = nullptr, ev_la = nullptr, ev_lb = nullptr;
 };
 // partials: the history sums (NHS x RED_BLOCKS) followed by the field statistics (NHF x NHQ x HST_NB); the
 // residual norms (9 x RED_BLOCKS) and initialize!'s two sums use the front of the same buffer
@@ -302,6 +302,8 @@ void step_free(xc_ctx*, void* p) {
     }
     if (s->ev_go) (void)hipEventDestroy(s->ev_go);
     if (s->ev_noise) (void)hipEventDestroy(s->ev_noise);
+    if (s->ev_la) (void)hipEventDestroy(s->ev_la);
+    if (s->ev_lb) (void)hipEventDestroy(s->ev_lb);
     delete s;
 }
 // the context's reduction buffers (the snapshot buffer is made by xc_snapshot only)
@@ -335,10 +337,9 @@ int noise_side(xc_ctx* ctx, StepCtx* sx, xc_noise* ns, const xc_phase* ph, const
 }
 // run!'s second stream and its two events (made once per context)
 int step_side(xc_ctx* ctx, StepCtx* s) {
-    if (s->side) return XC_OK;
-    XC_TRY_HIP(ctx, hipEventCreateWithFlags(&s->ev_go, hipEventDisableTiming));
-    XC_TRY_HIP(ctx, hipEventCreateWithFlags(&s->ev_noise, hipEventDisableTiming));
-    XC_TRY_HIP(ctx, hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));
+    for (hipEvent_t* e : {&s->ev_go, &s->ev_noise, &s->ev_la, &s->ev_lb})
+        if (!*e) XC_TRY_HIP(ctx, hipEventCreateWithFlags(e, hipEventDisableTiming));
+    if (!s->side) XC_TRY_HIP(ctx, hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));
     return XC_OK;
 }
 
@@ -607,7 +608,8 @@ int xc_run_step(xc_ctx* ctx, xc_phase* ph, xc_fluid* fl, xc_noise* ns, const xc_
         if (st) return st;
         if (sj) return sj;
         if ((st = xc_update_phase_velocities(ctx, ph, fl, ns, g, par, so->xbc, so->zbc))) return st;
-        if ((st = xc_update(ctx, ph, fl, g, par, sc, so->xbc, so->zbc))) return st;
+        if ((st = xc_update_side(ctx, ph, fl, g, par, sc, so->xbc, so->zbc, sx->side, sx->ev_la, sx->ev_lb)))
+            return st;
         double rn[3];
         if ((st = xc_compute_resnorm(ctx, ph, fl, dt, rn))) return st;
         if (!isfinite(rn[0])) return xc_fail(ctx, XC_ENOCONV, "run!: NaN Picard residual at step %lld sweep %d",
