@@ -123,7 +123,7 @@ def step_bench(args, learned=False):
     par, g, sc, ph, fl, ns, hst, state = c3_setup(args.n)
     opts = xc.step_opts(par, tol=1e-8, maxit=2000, mg_pre=args.mg_pre, mg_post=args.mg_post, mg_coarse=args.mg_coarse,
                         mg_sub=args.mg_sub, mg_levels=args.mg_levels, mg_omega=args.mg_omega, learned=learned,
-                        lc_level=lc_level)
+                        lc_level=lc_level, mg_jacobi=args.mg_jacobi)
     def one_step():
         # run.jl:205-275: the step and its record_history! (nrh = 1, parameters.jl:18)
         dt_step = state.dt
@@ -591,6 +591,8 @@ def main():
     ap.add_argument("--mg-omega", type=float, default=0.0,
                     help="damped-Jacobi weight of the multigrid smoother (0: chosen per level from the inertia ratio)")
     ap.add_argument("--mg-levels", type=int, default=0, help="most multigrid levels (0: down to 4 x 4)")
+    ap.add_argument("--mg-jacobi", action="store_true",
+                    help="damped-Jacobi fine-level smoothing instead of the default Chebyshev passes")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-stencil", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")

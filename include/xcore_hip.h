@@ -238,7 +238,8 @@ typedef struct {  /* fluidmech! keywords (fluidmech.jl:71-80) and the iterative-
                                         the coarsest; 0 selects the defaults 3, 3 and, on the coarsest,
                                         20 at the 4 x 4 bottom or 2 where the cycle stopped early */
     double mg_omega;  /* damped-Jacobi weight of the multigrid smoother (0: per level from its inertia ratio
-                         R = a1 rho h^2 / (dt eta): 0.95 where R >= 1, 0.8 elsewhere and without inertia) */
+                         R = a1 rho h^2 / (dt eta): 0.95 where R >= 1, 0.8 elsewhere and without inertia); the
+                         finest level's Chebyshev passes (mg_jacobi) set their own weights */
     int learned;      /* the learned correction (needs xc_unet_set_params): 1 the network's coarse estimate e of
                          LL^-1 v first, z = e + M (v - LL e); 2 the network's estimate of what the plain
                          preconditioner M misses, z = M v + P N(R v).  A learned step is kept while it cuts the
@@ -260,6 +261,10 @@ typedef struct {  /* fluidmech! keywords (fluidmech.jl:71-80) and the iterative-
                          R^T LL R in this solve (9 operator applications); 2: reuse the operator the last probing
                          solve on this grid left (probe if none); 0 (default): off in xc_fluidmech, and in
                          xc_run_step probed in each time step's first Picard sweep and reused in the others */
+    int mg_jacobi;    /* 1: the finest level's pre- and post-smoothing are damped Jacobi sweeps (mg_omega) in every
+                         case; 0 (default): with the single-precision fine smoother (mg_fp64 = 0), Chebyshev passes
+                         (the same sweeps, each adding a multiple of the last update) where the finest level's
+                         inertia ratio is >= 1 and it is not the only level, damped Jacobi elsewhere */
 } xc_fm_opts;
 
 typedef struct {  /* state.jl:131-134 BnchmData */

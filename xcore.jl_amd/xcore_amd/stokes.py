@@ -25,7 +25,7 @@ class FmOpts(ctypes.Structure):
                [(n, _D) for n in ("dt", "a1", "a2", "a3", "b1", "b2", "b3", "tol")] + \
                [(n, _I) for n in ("maxit", "restart", "mg_pre", "mg_post", "mg_coarse")] + [("mg_omega", _D)] + \
                [("learned", _I), ("lc_level", _I), ("mg_sub", _I), ("mg_levels", _I), ("mg_fp64", _I),
-                ("xmean", _I)]
+                ("xmean", _I), ("mg_jacobi", _I)]
 
 
 class BnchmStruct(ctypes.Structure):
@@ -62,12 +62,12 @@ class BnchmData:
 def fm_opts(par, sds=-1, top_cnv=1, bot_cnv=1, open_cnv=False, xBC="periodic", zBC="closed", dt=1e32,
             a1=1.0, a2=1.0, a3=0.0, b1=1.0, b2=0.0, b3=0.0, tol=1e-8, maxit=1000, restart=40,
             mg_pre=3, mg_post=3, mg_coarse=0, mg_omega=0.0, learned=False, lc_level=2, mg_sub=1, mg_levels=0,
-            mg_fp64=False, xmean=0):
+            mg_fp64=False, xmean=0, mg_jacobi=False):
     return FmOpts(int(sds), int(top_cnv), int(bot_cnv), int(bool(open_cnv)), _sym(xBC, BCS, "xBC"),
                   _sym(zBC, BCS, "zBC"), _sym(par.ADVN, SCHEMES, "scheme"), float(dt), float(a1), float(a2),
                   float(a3), float(b1), float(b2), float(b3), float(tol), int(maxit), int(restart), int(mg_pre),
                   int(mg_post), int(mg_coarse), float(mg_omega), int(learned), int(lc_level), int(mg_sub),
-                  int(mg_levels), int(bool(mg_fp64)), int(xmean))
+                  int(mg_levels), int(bool(mg_fp64)), int(xmean), int(bool(mg_jacobi)))
 
 
 def fluidmech_(state, grid, par, phase=None, bnchm_data=None, raise_on_noconv=True, **kw):
