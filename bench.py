@@ -597,6 +597,7 @@ def main():
     ap.add_argument("--no-stencil", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-step", action="store_true", help="skip the full-step bench (profiling runs)")
+    ap.add_argument("--no-ou", action="store_true", help="skip the noise! microbench (profiling runs)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="before the stencil bench, copy a buffer of this many MiB three times (a known-size "
                          "kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -665,8 +666,9 @@ def main():
         out["ensemble"] = ensemble_bench()
         print("bench: ensemble of 64 whole runs", file=sys.stderr, flush=True)
         out["ensemble_steps"] = ensemble_steps_bench()
-    print("bench: noise! at 1024^2 (OU microbench)", file=sys.stderr, flush=True)
-    out["ou_noise_1024"] = ou_bench()
+    if not args.no_ou:
+        print("bench: noise! at 1024^2 (OU microbench)", file=sys.stderr, flush=True)
+        out["ou_noise_1024"] = ou_bench()
     if not args.no_cpu:
         print("bench: config 1 (100^2, 20 steps) on the device and the CPU oracle", file=sys.stderr, flush=True)
         out["config1"] = c1_bench()
