@@ -245,7 +245,8 @@ typedef struct {  /* fluidmech! keywords (fluidmech.jl:71-80) and the iterative-
                          preconditioner M misses, z = M v + P N(R v).  A learned step is kept while it cuts the
                          residual at least as much as plain steps do per unit of cost; otherwise the rest of the
                          solve is plain (same Krylov basis; fallback_used = 1), and after 3 solves in a row
-                         dropped at their first step the next 16 solves are plain */
+                         dropped at their first step the next 16 solves are plain (32, 64, ... up to 1024 each
+                         time the solve that retries it is dropped again) */
     int lc_level;     /* grid of the learned correction: Nz / 2^lc_level x Nx / 2^lc_level cells */
     int mg_sub;       /* pre- and post-smoothing sweeps on the levels between (0: default 1; < 0: as the
                          finest level) */

@@ -1929,6 +1929,7 @@ struct StokesCtx {
     // and the number of solves still to run plain before the learned correction is tried again
     double plain_rate = 0.0;
     int lc_fail_streak = 0, lc_skip = 0;
+    int lc_backoff = 0;  // skip periods in a row that ended in another dropped retry (the next one is doubled)
     // horizontal-mean correction of each FGMRES cycle's starting guess (see xm_build): the Galerkin operator
     // R^T LL R on the x-uniform fields, factorized on the host (banded LU, rows ordered W, U, P per z), its
     // mapped host scratch (q = R^T r, delta, the probe sums) and device buffers (partial sums, delta)
@@ -2637,7 +2638,8 @@ struct SolveOut { int iters; bool conv; double rel, rel0; int learned_steps; boo
 // cost of a learned preconditioner step in plain steps (restriction, network, prolongation besides the V-cycle)
 constexpr double LC_COST = 1.25;
 // a learned correction dropped at its first step in this many solves in a row is left out of the next LC_SKIP
-constexpr int LC_STREAK = 3, LC_SKIP = 16;
+// solves, twice as many each time the retry after such a period is dropped again (up to LC_SKIP << LC_BACKOFF)
+constexpr int LC_STREAK = 3, LC_SKIP = 16, LC_BACKOFF = 6;
 
 struct LcCfg {
     bool on;
@@ -3228,11 +3230,13 @@ int xc_fluidmech(xc_ctx* ctx, xc_fluid* fl, const xc_phase* ph, const xc_grid* g
         fallback |= so.dropped ? 1 : 0;
         if (so.dropped && so.learned_steps <= 1) {
             if (++s->lc_fail_streak >= LC_STREAK) {
-                s->lc_skip = LC_SKIP;
+                s->lc_skip = LC_SKIP << (s->lc_backoff < LC_BACKOFF ? s->lc_backoff : LC_BACKOFF);
+                ++s->lc_backoff;
                 s->lc_fail_streak = LC_STREAK - 1;
             }
         } else {
             s->lc_fail_streak = 0;
+            s->lc_backoff = 0;
         }
     } else if (so.conv && so.iters > 0 && so.rel > 0 && so.rel0 > so.rel) {
         // the plain preconditioner's measured reduction per iteration (exponential average over solves)
