@@ -147,3 +147,47 @@ def test_checkpoint_resume_bitwise(xc, tmp_path):
         for n, x in a.arrays().items():
             assert torch.equal(x, b.arrays()[n]), n
     assert ns.draws == ns2.draws and hst.EB == hst2.EB and hst.time == hst2.time
+
+
+def test_run_step_equals_its_entry_points(xc):
+    # xc_run_step puts noise! (and update!'s lithostatic chain) on a second stream beside phsevo! and
+    # fluidmech!; the same step composed from the public entry points, one after another on one stream in
+    # the reference's order (run.jl:204-275), must give the same bits in every field
+    Nz, Nx = 48, 40
+    par = xc.Parameters(D=10.0, N=Nz, L=10.0 * Nx / Nz, etam0=10.0, x0=0.1, Da=0.0)
+    g = xc.Grid.from_par(par)
+    sc = xc.compute_scales(par, g)
+    states = []
+    for _ in range(2):
+        ph, fl = xc.PhaseState(Nz, Nx), xc.FluidState(Nz, Nx)
+        ns = xc.NoiseState(g, sc, seed=3)
+        t, dt, step = xc.initialize_(ph, fl, ns, xc.History(), g, par, sc)
+        states.append((ph, fl, ns, xc.StepState(t, dt, step)))
+    sweeps = 3
+    fm = dict(tol=1e-10, maxit=2000)
+    (ph, fl, ns, st), (ph2, fl2, ns2, st2) = states
+    xc.run_step_(ph, fl, ns, g, par, sc, st, xc.step_opts(par, fixed_sweeps=sweeps, **fm))
+    # the same step by hand
+    step = st2.step + 1
+    a1, a2, a3, b1, b2, b3 = xc.time_coefs(par.TINT, step)
+    dt = st2.dt
+    tc = dict(a1=a1, a2=a2, a3=a3, b1=b1, b2=b2, b3=b3)
+    xc.store_previous_(fl2, ph2)
+    xc.store_noise_(ns2)
+    for it in range(1, sweeps + 1):
+        xc.driver.snapshot_(ph2, fl2)
+        xc.phsevo_(ph2, fl2, g, par, sc, ADVN=par.ADVN, dt=dt, **tc)
+        xc.fluidmech_(fl2, g, par, phase=ph2, dt=dt, xmean=1 if it == 1 else 2, **tc, **fm)
+        xc.noise_(ns2, ph2, g, par, sc, dt, first_iter=it <= 1)
+        xc.update_phase_velocities_(ph2, fl2, ns2, g, par)
+        xc.update_(ph2, fl2, g, par, sc)
+        xc.driver.compute_resnorm(ph2, fl2, dt)
+    dtn = xc.update_dt(ph2, fl2, g, par, dt)
+    torch.cuda.synchronize()
+    assert st.step == step and st.dt == dtn and st.time == st2.time + dt
+    bad = []
+    for a, b in ((ph, ph2), (fl, fl2), (ns, ns2)):
+        for n, x in a.arrays().items():
+            if not torch.equal(x, b.arrays()[n]):
+                bad.append(n)
+    assert ns.draws == ns2.draws and not bad, bad

@@ -191,6 +191,28 @@ def test_fine_smoother_precision(xc, orc, open_cnv):
     assert abs(its[False] - its[True]) <= max(2, its[True] // 10), its
 
 
+@pytest.mark.parametrize("shape", [(64, 64), (128, 96)])
+def test_chebyshev_smoothing(xc, orc, shape):
+    # the fine level's Chebyshev passes (default where its inertia ratio is >= 1) and damped-Jacobi sweeps
+    # (mg_jacobi): both meet the residual bar on the oracle's assembled system, Chebyshev in no more iterations
+    Nz, Nx = shape
+    par, g, sc, ph, fl, ns = plausible_state(orc, xc, Nz, Nx, seed=13)
+    orc.update(ph, fl, g, par, sc)
+    orc.phsevo(ph, fl, g, par, sc, coef=orc.time_coefs("bd2im", 3), dt=sc.dt0)
+    kw = dict(dt=sc.dt0, a1=1.5, a2=2.0, a3=-0.5, b1=1, b2=0, b3=0)
+    LL, RR = orc.fluidmech_assemble(copy.deepcopy(fl), copy.deepcopy(ph), g, orc.fm_opts(par, **kw))
+    its = {}
+    for jacobi in (True, False):
+        dph, dfl = to_device_state(xc, ph, "phase"), to_device_state(xc, fl, "fluid")
+        stats = xc.fluidmech_(dfl, g, par, phase=dph, tol=1e-8, mg_jacobi=jacobi, **kw)
+        print(f"{shape} mg_jacobi={jacobi}: {stats}", flush=True)
+        assert stats["converged"] and stats["inertia_ratio"] >= 1.0 and stats["mg_levels"] >= 2
+        sol = np.concatenate([xc.to_host(getattr(dfl, k)).ravel(order="F") for k in ("W", "U", "P")])
+        assert np.linalg.norm(LL @ sol - RR) / np.linalg.norm(RR) <= 1e-8
+        its[jacobi] = stats["iters"]
+    assert its[False] <= its[True], its
+
+
 @pytest.mark.parametrize("shape", [(48, 64), (256, 32), (512, 16), (2048, 8)])
 def test_schur_poisson_solve(xc, orc, shape):
     # the inertial part of the pressure block of z = M^-1 v: with inertia minus (the weighted) without is
