@@ -350,7 +350,10 @@ typedef struct {
 
 /* run.jl:204-275: one outer step of run! -- time_coefs, store_previous!, store_noise!, then the
  * Picard loop (>= 3 sweeps of snapshot!, phsevo!, fluidmech!, noise!, update_phase_velocities!,
- * update!, compute_resnorm), time += dt, dt = update_dt(...).  state->step/time/dt advance only when
+ * update!, compute_resnorm), time += dt, dt = update_dt(...).  noise! and update!'s lithostatic chain run on
+ * a second stream the context owns (default priority), ordered after the work queued before the call and before
+ * all work queued after it on the context's stream; the results are those of the entry points called one after
+ * another.  state->step/time/dt advance only when
  * the step succeeds.  A failed step (XC_ENOCONV from a Stokes solve or a NaN Picard residual, or any
  * other error) returns with state unchanged but the fields part-way through the step: they are not a
  * state of the run, which has to resume from a checkpoint (the reference raises at the same point,
