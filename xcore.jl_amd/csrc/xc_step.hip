@@ -281,7 +281,8 @@ struct StepCtx {
     double* host_dev = nullptr;  // the same memory as seen by kernels
     double* rp = nullptr;     // initialize!'s random perturbation
     size_t rp_n = 0;
-    hipStream_t side = nullptr;  // run!'s second stream (noise!), default priority
+    hipStream_t side = nullptr;  // run!'s second stream (noise!, update!'s lithostatic chain), default priority
+    // the sweep's start and noise!'s end on it; the lithostatic chain's start and end (xc_update_side)
     hipEvent_t ev_go = nullptr, ev_noise = nullptr, ev_la = nullptr, ev_lb = nullptr;
 };
 // partials: the history sums (NHS x RED_BLOCKS) followed by the field statistics (NHF x NHQ x HST_NB); the
@@ -335,7 +336,7 @@ int noise_side(xc_ctx* ctx, StepCtx* sx, xc_noise* ns, const xc_phase* ph, const
     if (st) XC_TRY_HIP(ctx, hipStreamWaitEvent(ctx->stream, sx->ev_noise, 0));
     return st;
 }
-// run!'s second stream and its two events (made once per context)
+// run!'s second stream and its four events (made once per context)
 int step_side(xc_ctx* ctx, StepCtx* s) {
     for (hipEvent_t* e : {&s->ev_go, &s->ev_noise, &s->ev_la, &s->ev_lb})
         if (!*e) XC_TRY_HIP(ctx, hipEventCreateWithFlags(e, hipEventDisableTiming));
