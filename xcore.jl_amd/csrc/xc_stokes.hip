@@ -1651,11 +1651,33 @@ __global__ __launch_bounds__(NT) void k_update_x_solve(Kry K, int m, int used, i
     double pc = 0.0;
     if (pinc)
         for (int q = 0; q < used; ++q) pc += pinc[q] * y[q];
-    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
-        double sx = x[i];
-        for (int q = 0; q < used; ++q) sx += y[q] * Z[(int64_t)q * n + i];
-        if (pinc && i >= offP) sx += pc;
-        x[i] = sx;
+    // UX elements per thread, their loads of each Z column in flight together (each element's sum in the
+    // order x + y_0 z_0 + y_1 z_1 + ...)
+    constexpr int UX = 4;
+    const int64_t stride = (int64_t)gridDim.x * NT;
+    for (int64_t i0 = (int64_t)blockIdx.x * NT + threadIdx.x; i0 < n; i0 += UX * stride) {
+        double sx[UX];
+        int64_t ix[UX];
+#pragma unroll
+        for (int e = 0; e < UX; ++e) {
+            ix[e] = i0 + e * stride < n ? i0 + e * stride : n - 1;
+            sx[e] = x[ix[e]];
+        }
+        for (int q = 0; q < used; ++q) {
+            const double yq = y[q];
+            double zq[UX];
+#pragma unroll
+            for (int e = 0; e < UX; ++e) zq[e] = Z[(int64_t)q * n + ix[e]];
+#pragma unroll
+            for (int e = 0; e < UX; ++e) sx[e] += yq * zq[e];
+        }
+#pragma unroll
+        for (int e = 0; e < UX; ++e) {
+            if (i0 + e * stride >= n) continue;
+            double v = sx[e];
+            if (pinc && ix[e] >= offP) v += pc;
+            x[ix[e]] = v;
+        }
     }
 }
 // x_P += c for every pressure entry (LL 1_P = e_pin: only the pin row's residual moves, by -c)
