@@ -1185,7 +1185,7 @@ __global__ __launch_bounds__(NT) void k_dot_final(const double* __restrict__ par
 }
 // out[0] = the sum of nb partials (fixed order) by the whole workgroup, sixteen loads in flight per lane: for
 // the thousands of per-wave partials of the residual kernel, which one wave would read in many dependent rounds
-__global__ __launch_bounds__(NT) void k_sum_final1(const double* __restrict__ part, int nb, double* out) {
+__device__ __forceinline__ void block_sum1(const double* __restrict__ part, int nb, double* out) {
     __shared__ double red[NT / 64];
     constexpr int U = 16;
     double acc = 0.0;
@@ -1204,6 +1204,9 @@ __global__ __launch_bounds__(NT) void k_sum_final1(const double* __restrict__ pa
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
     __syncthreads();
     if (threadIdx.x == 0) out[0] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+__global__ __launch_bounds__(NT) void k_sum_final1(const double* __restrict__ part, int nb, double* out) {
+    block_sum1(part, nb, out);
 }
 // largest viscosity and smallest density of the fine level from k_pack5's per-block partials (one wave)
 __global__ __launch_bounds__(64) void k_eta_rho_final(const double* __restrict__ part, int nb, double* out) {
@@ -2779,8 +2782,16 @@ __global__ __launch_bounds__(NT) void k_xm_expand(const double* __restrict__ p, 
 }
 // part[p][c][q] = the sum of profile row q's entries in columns c XM_CH .. c XM_CH + XM_CH - 1 of its field, for
 // vector p = blockIdx.z (v + p vstride)
+// sq (optional): the launch has one more z plane, whose first workgroup sums the nsq partials sq into sq_out
+// (k_sum_final1's work, in the same launch)
 __global__ __launch_bounds__(NT) void k_xm_part(const double* __restrict__ v, int64_t vstride,
-                                                double* __restrict__ part, int Nz, int Nx, int n0) {
+                                                double* __restrict__ part, int Nz, int Nx, int n0,
+                                                const double* __restrict__ sq = nullptr, int nsq = 0,
+                                                double* sq_out = nullptr) {
+    if (sq && blockIdx.z == gridDim.z - 1) {
+        if (blockIdx.x == 0 && blockIdx.y == 0) block_sum1(sq, nsq, sq_out);
+        return;
+    }
     const int q = blockIdx.x * NT + threadIdx.x, c = blockIdx.y;
     if (q >= n0) return;
     v += (int64_t)blockIdx.z * vstride;
@@ -2815,10 +2826,12 @@ __global__ __launch_bounds__(NT) void k_xm_final(const double* __restrict__ part
     out[(int64_t)blockIdx.y * n0 + q] = sum;
 }
 // out[p] (n0 each, device or mapped host) = R^T (v + p vstride) for p < nv
-void xm_sum(xc_ctx* ctx, StokesCtx* s, const double* v, double* out, int nv = 1, int64_t vstride = 0) {
+// sq, nsq, sq_out: also the sum of nsq partials, in the same launch (k_xm_part)
+void xm_sum(xc_ctx* ctx, StokesCtx* s, const double* v, double* out, int nv = 1, int64_t vstride = 0,
+            const double* sq = nullptr, int nsq = 0, double* sq_out = nullptr) {
     const int n0 = s->xm_n0;
-    hipLaunchKernelGGL(k_xm_part, dim3((n0 + NT - 1) / NT, s->xm_nch, nv), dim3(NT), 0, ctx->stream, v, vstride,
-                       s->xm_part, s->Nz, s->Nx, n0);
+    hipLaunchKernelGGL(k_xm_part, dim3((n0 + NT - 1) / NT, s->xm_nch, nv + (sq ? 1 : 0)), dim3(NT), 0, ctx->stream,
+                       v, vstride, s->xm_part, s->Nz, s->Nx, n0, sq, nsq, sq_out);
     hipLaunchKernelGGL(k_xm_final, dim3((n0 + NT - 1) / NT, nv), dim3(NT), 0, ctx->stream, s->xm_part, s->xm_nch,
                        n0, out);
 }
@@ -2955,9 +2968,12 @@ void enqueue_r0(xc_ctx* ctx, StokesCtx* s, const Flags& F, const double* x, cons
     const unsigned nbl = ll_cells_blocks(dd.Nz, dd.Nx);
     double* part = s->part + DOT_BLOCKS;
     hipLaunchKernelGGL(k_ll_cells<true>, dim3(nbl), dim3(NT), 0, ctx->stream, dd, F, x, b, s->rs, part);
-    hipLaunchKernelGGL(k_sum_final1, dim3(1), dim3(NT), 0, ctx->stream, (const double*)part, (int)(nbl * (NT / 64)),
-                       b2 ? b2 : s->host_dev + R0_SLOT);
-    if (s->xm_active && with_q) xm_sum(ctx, s, s->rs, s->xm_hdev);  // R^T r for the cycle start's correction
+    double* b2out = b2 ? b2 : s->host_dev + R0_SLOT;
+    if (s->xm_active && with_q)  // R^T r for the cycle start's correction, ||r||^2 in the same launch
+        xm_sum(ctx, s, s->rs, s->xm_hdev, 1, 0, part, (int)(nbl * (NT / 64)), b2out);
+    else
+        hipLaunchKernelGGL(k_sum_final1, dim3(1), dim3(NT), 0, ctx->stream, (const double*)part,
+                           (int)(nbl * (NT / 64)), b2out);
 }
 int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int m, int64_t n, const double* b,
                double* x, double target, double bnorm, int maxit, const LcCfg& lc, SolveOut* out,
