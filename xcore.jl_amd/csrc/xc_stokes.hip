@@ -380,10 +380,12 @@ __global__ __launch_bounds__(NT) void k_smooth_cells(LV L, Flags F, const TX* __
 // of the reference's rows (fluidmech.jl:145-488) in the order vel_row and the pressure rows use, from
 // scalar column bases plus the lane's row offset.
 // sq: when given, wave w (of 4 gridDim.x) writes the sum of squares of the values its lanes stored to sq[w]
+// bq (RESID): likewise the sums of squares of the b entries of those rows (||b||^2 over the whole vector)
 template <bool RESID>
 __global__ __launch_bounds__(NT) void k_ll_cells(LevD L, Flags F, const double* __restrict__ x,
                                                  const double* __restrict__ b, double* __restrict__ y,
-                                                 double* __restrict__ sq = nullptr, double* __restrict__ pinc = nullptr) {
+                                                 double* __restrict__ sq = nullptr, double* __restrict__ pinc = nullptr,
+                                                 double* __restrict__ bq = nullptr) {
     const int Nz = L.Nz, Nx = L.Nx;
     const int segs = (Nz + 2 + 63) >> 6;
     const int task = __builtin_amdgcn_readfirstlane(xcd_tile(blockIdx.x, gridDim.x) * (NT / 64) + (threadIdx.x >> 6));
@@ -391,12 +393,13 @@ __global__ __launch_bounds__(NT) void k_ll_cells(LevD L, Flags F, const double* 
     const int lane = threadIdx.x & 63;
     if (c > Nx + 1) {
         if (sq && lane == 0) sq[task] = 0.0;
+        if (bq && lane == 0) bq[task] = 0.0;
         return;
     }
     const int r0 = ((task - c * segs) << 6) + lane;
     const bool live = r0 <= Nz + 1;
     const int r = live ? r0 : Nz + 1;  // lanes past the last row repeat it and store nothing
-    double ss = 0.0;
+    double ss = 0.0, sb = 0.0;
     const int ru = r;                            // U and P row
     const int rw = r <= Nz ? r : Nz;             // W row
     const int um = ru > 0 ? ru - 1 : 0, up = ru <= Nz ? ru + 1 : Nz + 1;
@@ -466,10 +469,12 @@ __global__ __launch_bounds__(NT) void k_ll_cells(LevD L, Flags F, const double* 
                 (0.5 * C2 - T1) * i2 * Ur - (0.5 * C2 - T2) * i2 * Us;
         }
         if (!bc) v += -ih * Pr + ih * Ps;
-        const double o = RESID ? ldr(b + (int64_t)cw * ldW, rw) - v : v;
+        const double bv = RESID ? ldr(b + (int64_t)cw * ldW, rw) : 0.0;
+        const double o = RESID ? bv - v : v;
         if (live) {
             y[kw] = o;
             ss = fma(o, o, ss);
+            if (RESID) sb = fma(bv, bv, sb);
         }
     }
     // U row (r, c) (u_row), then GG on the interior rows
@@ -490,10 +495,12 @@ __global__ __launch_bounds__(NT) void k_ll_cells(LevD L, Flags F, const double* 
                 (0.5 * C2 - T1) * i2 * Wr - (0.5 * C2 - T2) * i2 * We;
         }
         if (!bc) v += -ih * Pr + ih * Pe;
-        const double o = RESID ? ldr(b + offU + (int64_t)cu * ldU, ru) - v : v;
+        const double bv = RESID ? ldr(b + offU + (int64_t)cu * ldU, ru) : 0.0;
+        const double o = RESID ? bv - v : v;
         if (live) {
             y[ku] = o;
             ss = fma(o, o, ss);
+            if (RESID) sb = fma(bv, bv, sb);
         }
     }
     // P row (r, c): DM, the BC rows and the pins (fluidmech.jl:357-488)
@@ -509,7 +516,9 @@ __global__ __launch_bounds__(NT) void k_ll_cells(LevD L, Flags F, const double* 
             v = -rul * ih * Ul + ru0 * ih * Ur - rwm * ih * Wn + rw0 * ih * Wr;
             if (pin) v += Pr;
         }
-        double o = RESID ? ldr(b + offP + (int64_t)c * ldP, ru) - v : v;
+        const double bv = RESID ? ldr(b + offP + (int64_t)c * ldP, ru) : 0.0;
+        double o = RESID ? bv - v : v;
+        if (RESID && live) sb = fma(bv, bv, sb);
         if (!RESID && pinc && pin && live) {
             // constant-pressure fix of an FGMRES step (b = the step's Krylov vector): LL 1_P = e_pin, so with
             // c = b_pin - (LL x)_pin the step uses x + c 1_P, whose image has b's value in the pin row
@@ -525,6 +534,10 @@ __global__ __launch_bounds__(NT) void k_ll_cells(LevD L, Flags F, const double* 
     if (sq) {
         const double t = wave_total(ss);
         if (lane == 0) sq[task] = t;
+    }
+    if (RESID && bq) {
+        const double t = wave_total(sb);
+        if (lane == 0) bq[task] = t;
     }
 }
 inline unsigned ll_cells_blocks(int Nz, int Nx) {
@@ -1205,8 +1218,13 @@ __device__ __forceinline__ void block_sum1(const double* __restrict__ part, int 
     __syncthreads();
     if (threadIdx.x == 0) out[0] = (red[0] + red[1]) + (red[2] + red[3]);
 }
-__global__ __launch_bounds__(NT) void k_sum_final1(const double* __restrict__ part, int nb, double* out) {
+__global__ __launch_bounds__(NT) void k_sum_final1(const double* __restrict__ part, int nb, double* out,
+                                                   const double* __restrict__ part2 = nullptr, double* out2 = nullptr) {
     block_sum1(part, nb, out);
+    if (part2) {
+        __syncthreads();  // block_sum1's LDS is read by thread 0 before it is reused
+        block_sum1(part2, nb, out2);
+    }
 }
 // largest viscosity and smallest density of the fine level from k_pack5's per-block partials (one wave)
 __global__ __launch_bounds__(64) void k_eta_rho_final(const double* __restrict__ part, int nb, double* out) {
@@ -2124,7 +2142,7 @@ int setup(xc_ctx* ctx, int Nz, int Nx, int m, StokesCtx** out, int maxlev = -1) 
     size_t part_cols = md_blocks(n) > DOT_BLOCKS ? (size_t)md_blocks(n) : (size_t)DOT_BLOCKS;
     {  // also room for k_ll_cells' per-wave partials behind the first DOT_BLOCKS (a solve's first residual is
        // reduced while ||RR||'s partials are)
-        const size_t waves = (size_t)ll_cells_blocks(Nz, Nx) * (NT / 64) + DOT_BLOCKS;
+        const size_t waves = 2 * (size_t)ll_cells_blocks(Nz, Nx) * (NT / 64) + DOT_BLOCKS;  // r and b sums
         const size_t rows = 2 * (size_t)m + 4;
         if (rows * part_cols < waves) part_cols = (waves + rows - 1) / rows;
     }
@@ -2783,13 +2801,20 @@ __global__ __launch_bounds__(NT) void k_xm_expand(const double* __restrict__ p, 
 // part[p][c][q] = the sum of profile row q's entries in columns c XM_CH .. c XM_CH + XM_CH - 1 of its field, for
 // vector p = blockIdx.z (v + p vstride)
 // sq (optional): the launch has one more z plane, whose first workgroup sums the nsq partials sq into sq_out
-// (k_sum_final1's work, in the same launch)
+// (k_sum_final1's work, in the same launch), and bq (optional) into bq_out
 __global__ __launch_bounds__(NT) void k_xm_part(const double* __restrict__ v, int64_t vstride,
                                                 double* __restrict__ part, int Nz, int Nx, int n0,
                                                 const double* __restrict__ sq = nullptr, int nsq = 0,
-                                                double* sq_out = nullptr) {
+                                                double* sq_out = nullptr, const double* __restrict__ bq = nullptr,
+                                                double* bq_out = nullptr) {
     if (sq && blockIdx.z == gridDim.z - 1) {
-        if (blockIdx.x == 0 && blockIdx.y == 0) block_sum1(sq, nsq, sq_out);
+        if (blockIdx.x == 0 && blockIdx.y == 0) {
+            block_sum1(sq, nsq, sq_out);
+            if (bq) {
+                __syncthreads();  // block_sum1's LDS is read by thread 0 before it is reused
+                block_sum1(bq, nsq, bq_out);
+            }
+        }
         return;
     }
     const int q = blockIdx.x * NT + threadIdx.x, c = blockIdx.y;
@@ -2826,12 +2851,13 @@ __global__ __launch_bounds__(NT) void k_xm_final(const double* __restrict__ part
     out[(int64_t)blockIdx.y * n0 + q] = sum;
 }
 // out[p] (n0 each, device or mapped host) = R^T (v + p vstride) for p < nv
-// sq, nsq, sq_out: also the sum of nsq partials, in the same launch (k_xm_part)
+// sq, nsq, sq_out (and bq, bq_out): also the sums of nsq partials, in the same launch (k_xm_part)
 void xm_sum(xc_ctx* ctx, StokesCtx* s, const double* v, double* out, int nv = 1, int64_t vstride = 0,
-            const double* sq = nullptr, int nsq = 0, double* sq_out = nullptr) {
+            const double* sq = nullptr, int nsq = 0, double* sq_out = nullptr, const double* bq = nullptr,
+            double* bq_out = nullptr) {
     const int n0 = s->xm_n0;
     hipLaunchKernelGGL(k_xm_part, dim3((n0 + NT - 1) / NT, s->xm_nch, nv + (sq ? 1 : 0)), dim3(NT), 0, ctx->stream,
-                       v, vstride, s->xm_part, s->Nz, s->Nx, n0, sq, nsq, sq_out);
+                       v, vstride, s->xm_part, s->Nz, s->Nx, n0, sq, nsq, sq_out, bq, bq_out);
     hipLaunchKernelGGL(k_xm_final, dim3((n0 + NT - 1) / NT, nv), dim3(NT), 0, ctx->stream, s->xm_part, s->xm_nch,
                        n0, out);
 }
@@ -2961,19 +2987,22 @@ bool xm_correct(xc_ctx* ctx, StokesCtx* s, double* x, int64_t n) {
 // The residual of the starting x: r0_ready: already in s->rs with its squared norm in s->host[R0_SLOT] (enqueued by
 // the caller before a synchronization it needed anyway, enqueue_r0), else computed here.
 constexpr int R0_SLOT = 4;
-// b2: where ||r||^2 goes (default: the host's slot)
+// b2: where ||r||^2 goes (default: the host's slot); bb: where ||b||^2 goes (none by default)
 void enqueue_r0(xc_ctx* ctx, StokesCtx* s, const Flags& F, const double* x, const double* b, bool with_q = false,
-                double* b2 = nullptr) {
+                double* b2 = nullptr, double* bb = nullptr) {
     const LevD dd = s->lev[0].d();
     const unsigned nbl = ll_cells_blocks(dd.Nz, dd.Nx);
+    const int nw = (int)(nbl * (NT / 64));
     double* part = s->part + DOT_BLOCKS;
-    hipLaunchKernelGGL(k_ll_cells<true>, dim3(nbl), dim3(NT), 0, ctx->stream, dd, F, x, b, s->rs, part);
+    double* bpart = bb ? part + nw : nullptr;
+    hipLaunchKernelGGL(k_ll_cells<true>, dim3(nbl), dim3(NT), 0, ctx->stream, dd, F, x, b, s->rs, part,
+                       (double*)nullptr, bpart);
     double* b2out = b2 ? b2 : s->host_dev + R0_SLOT;
-    if (s->xm_active && with_q)  // R^T r for the cycle start's correction, ||r||^2 in the same launch
-        xm_sum(ctx, s, s->rs, s->xm_hdev, 1, 0, part, (int)(nbl * (NT / 64)), b2out);
+    if (s->xm_active && with_q)  // R^T r for the cycle start's correction, the norms in the same launch
+        xm_sum(ctx, s, s->rs, s->xm_hdev, 1, 0, part, nw, b2out, bpart, bb);
     else
-        hipLaunchKernelGGL(k_sum_final1, dim3(1), dim3(NT), 0, ctx->stream, (const double*)part,
-                           (int)(nbl * (NT / 64)), b2out);
+        hipLaunchKernelGGL(k_sum_final1, dim3(1), dim3(NT), 0, ctx->stream, (const double*)part, nw, b2out,
+                           (const double*)bpart, bb);
 }
 int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int m, int64_t n, const double* b,
                double* x, double target, double bnorm, int maxit, const LcCfg& lc, SolveOut* out,
@@ -3229,11 +3258,11 @@ int xc_fluidmech(xc_ctx* ctx, xc_fluid* fl, const xc_phase* ph, const xc_grid* g
     if ((st = xm_prepare(ctx, s, F, o))) return st;
     if ((st = build_rhs(ctx, s, fl, ph, g, par, o, bn, F, s->bs))) return st;
     hipLaunchKernelGGL(k_gather, dim3(nb), dim3(NT), 0, ctx->stream, fl->W, fl->U, fl->P, s->xs, Nz, Nx);
-    // the starting residual (and its x-uniform part R^T r), then ||RR||: one synchronization for all
-    enqueue_r0(ctx, s, F, s->xs, s->bs, true);
-    if ((st = multidot(ctx, s, s->bs, 0, s->bs, n))) return st;
+    // the starting residual, its x-uniform part R^T r and ||RR|| (summed beside ||r||): one synchronization
+    enqueue_r0(ctx, s, F, s->xs, s->bs, true, nullptr, s->host_dev);
+    XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
     const double bnorm = sqrt(s->host[0]);
-    if ((st = choose_depth(ctx, s, F, o, g->h, depth_pending, true))) return st;  // multidot synchronized
+    if ((st = choose_depth(ctx, s, F, o, g->h, depth_pending, true))) return st;  // synchronized above
     const MGOpts mo = mg_opts(o, s);
     build_coarse(ctx, s, s->leff);
     const double target = o->tol * (bnorm > 0 ? bnorm : 1.0);
