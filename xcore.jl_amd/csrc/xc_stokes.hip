@@ -1728,12 +1728,17 @@ __global__ __launch_bounds__(NT) void k_gather(xc_mat W, xc_mat U, xc_mat P, dou
     else if (k < nW + nU) { const int64_t q = k - nW; x[k] = U.p[(q % (Nz + 2)) + (q / (Nz + 2)) * U.ld]; }
     else if (k < nW + nU + nP) { const int64_t q = k - nW - nU; x[k] = P.p[(q % (Nz + 2)) + (q / (Nz + 2)) * P.ld]; }
 }
-__global__ __launch_bounds__(NT) void k_scatter(const double* x, xc_mat W, xc_mat U, xc_mat P, int Nz, int Nx) {
+// shift: P = x_P + cP (the pressure level of a pin-fixed solve, fgmres_run's pshift)
+__global__ __launch_bounds__(NT) void k_scatter(const double* x, xc_mat W, xc_mat U, xc_mat P, int Nz, int Nx,
+                                                int shift = 0, double cP = 0.0) {
     const int64_t k = gidx();
     const int64_t nW = (int64_t)(Nz + 1) * (Nx + 2), nU = (int64_t)(Nz + 2) * (Nx + 1), nP = (int64_t)(Nz + 2) * (Nx + 2);
     if (k < nW) W.p[(k % (Nz + 1)) + (k / (Nz + 1)) * W.ld] = x[k];
     else if (k < nW + nU) { const int64_t q = k - nW; U.p[(q % (Nz + 2)) + (q / (Nz + 2)) * U.ld] = x[k]; }
-    else if (k < nW + nU + nP) { const int64_t q = k - nW - nU; P.p[(q % (Nz + 2)) + (q / (Nz + 2)) * P.ld] = x[k]; }
+    else if (k < nW + nU + nP) {
+        const int64_t q = k - nW - nU;
+        P.p[(q % (Nz + 2)) + (q / (Nz + 2)) * P.ld] = shift ? x[k] + cP : x[k];
+    }
 }
 // dense copy of a strided matrix
 
@@ -2677,7 +2682,9 @@ int build_rhs(xc_ctx* ctx, StokesCtx* s, xc_fluid* fl, const xc_phase* ph, const
     return XC_OK;
 }
 
-struct SolveOut { int iters; bool conv; double rel, rel0; int learned_steps; bool dropped; };
+// pshift: the pressure level a converged solve with the pin fix sets (x_P += pshift), left to the caller
+// (xc_fluidmech adds it as it scatters the solution; has_pshift: there is one)
+struct SolveOut { int iters; bool conv; double rel, rel0; int learned_steps; bool dropped; bool has_pshift; double pshift; };
 // cost of a learned preconditioner step in plain steps (restriction, network, prolongation besides the V-cycle)
 constexpr double LC_COST = 1.25;
 // a learned correction dropped at its first step in this many solves in a row is left out of the next LC_SKIP
@@ -3042,6 +3049,8 @@ int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int 
     // sees, and without the fix they differ from one Picard sweep to the next, which the Picard test reads as a
     // change of P (config 1: 210 sweeps in 20 steps instead of the 160 of the sparse-LU oracle, with it 160).
     const bool pinfix = s->xm_active && F.pin_kind == 1 && !F.bnchm;
+    bool has_pshift = false;
+    double pshift = 0.0;
     int lsteps = 0;
     bool dropped = false;
     constexpr double NEAR_CONV = 3.0;
@@ -3083,9 +3092,10 @@ int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int 
                 const int Nz = dd.Nz;
                 double c = 0.0;
                 for (int z = 1; z <= Nz; ++z) c += s->xm_host[xm_prof(2, z, Nz)];
-                const int64_t nP = (int64_t)(Nz + 2) * (dd.Nx + 2);
-                if (isfinite(c))
-                    hipLaunchKernelGGL(k_p_shift, dim3(nblk(nP)), dim3(NT), 0, ctx->stream, x + (n - nP), nP, c);
+                if (isfinite(c)) {
+                    has_pshift = true;
+                    pshift = c;
+                }
             }
             break;
         }
@@ -3183,6 +3193,8 @@ int fgmres_run(xc_ctx* ctx, StokesCtx* s, const Flags& F, const MGOpts& mo, int 
     out->rel0 = rel0;
     out->learned_steps = lsteps;
     out->dropped = dropped;
+    out->has_pshift = has_pshift;
+    out->pshift = pshift;
     return XC_OK;
 }
 
@@ -3315,7 +3327,8 @@ int xc_fluidmech(xc_ctx* ctx, xc_fluid* fl, const xc_phase* ph, const xc_grid* g
     rel = so.rel;
     rel0 = so.rel0;
     s->xm_active = false;
-    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(NT), 0, ctx->stream, s->xs, fl->W, fl->U, fl->P, Nz, Nx);
+    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(NT), 0, ctx->stream, s->xs, fl->W, fl->U, fl->P, Nz, Nx,
+                       so.has_pshift ? 1 : 0, so.pshift);
     if (stats) {
         stats->iters = iters;
         stats->converged = conv;
@@ -3384,6 +3397,10 @@ int xc_stokes_solve(xc_ctx* ctx, const xc_fluid* fl, const xc_grid* g, const xc_
     if ((st = lc_config(ctx, s, o, g, par, &lc))) return st;
     SolveOut so{};
     if ((st = fgmres_run(ctx, s, F, mo, m, n, rhs, x, target, bnorm, maxit, lc, &so))) return st;
+    if (so.has_pshift) {  // (not with the solver pieces, which run without the horizontal-mean correction)
+        const int64_t nP = (int64_t)(g->Nz + 2) * (g->Nx + 2);
+        hipLaunchKernelGGL(k_p_shift, dim3(nblk(nP)), dim3(NT), 0, ctx->stream, x + (n - nP), nP, so.pshift);
+    }
     if (stats) {
         stats->iters = so.iters;
         stats->converged = so.conv;
