@@ -1863,6 +1863,9 @@ struct RRArgs {
     xc_mat srcW, srcU, srcP;     // bnchm only
     double a2, a3, dt, g0, D, h, oc;
     int bnchm;
+    // optional: the starting guess gathered from the state in the same pass (k_gather's work)
+    xc_mat gW, gU, gP;
+    double* gx;
 };
 __global__ __launch_bounds__(NT) void k_rr(RRArgs a, int Nz, int Nx, double* RR) {
 #pragma clang fp contract(off)
@@ -1901,6 +1904,11 @@ __global__ __launch_bounds__(NT) void k_rr(RRArgs a, int Nz, int Nx, double* RR)
         }
     }
     RR[k] = v;
+    if (a.gx) {
+        if (k < nW) a.gx[k] = at(a.gW, (int)(k % (Nz + 1)), (int)(k / (Nz + 1)));
+        else if (k < nW + nU) { const int64_t q = k - nW; a.gx[k] = at(a.gU, (int)(q % (Nz + 2)), (int)(q / (Nz + 2))); }
+        else { const int64_t q = k - nW - nU; a.gx[k] = at(a.gP, (int)(q % (Nz + 2)), (int)(q / (Nz + 2))); }
+    }
 }
 // MMS pins of the right-hand side (fluidmech.jl:468, 475)
 __global__ void k_rr_pins(double* RR, int64_t kp, xc_mat Pex, int pr, int pc, int64_t ku, xc_mat Uex, int ur, int uc) {
@@ -2626,8 +2634,9 @@ int multidot(xc_ctx* ctx, StokesCtx* s, const double* V, int K, const double* w,
 }
 
 
+// x0: also gather the state's (W, U, P) into x0 (the solve's starting guess) in the right-hand side's pass
 int build_rhs(xc_ctx* ctx, StokesCtx* s, xc_fluid* fl, const xc_phase* ph, const xc_grid* g, const xc_par* par,
-              const xc_fm_opts* o, const xc_bnchm* bn, const Flags& F, double* RR) {
+              const xc_fm_opts* o, const xc_bnchm* bn, const Flags& F, double* RR, double* x0 = nullptr) {
     const int Nz = (int)g->Nz, Nx = (int)g->Nx;
     // MFS update + mean (fluidmech.jl:115-127)
     RhsArgs ra{};
@@ -2670,6 +2679,7 @@ int build_rhs(xc_ctx* ctx, StokesCtx* s, xc_fluid* fl, const xc_phase* ph, const
     a.a2 = o->a2; a.a3 = o->a3; a.dt = o->dt; a.g0 = par->g0; a.D = g->D; a.h = g->h;
     a.oc = o->open_cnv ? 1.0 : 0.0;
     a.bnchm = bn != nullptr;
+    if (x0) { a.gW = fl->W; a.gU = fl->U; a.gP = fl->P; a.gx = x0; }
     const int64_t n = (int64_t)(Nz + 1) * (Nx + 2) + (int64_t)(Nz + 2) * (Nx + 1) + (int64_t)(Nz + 2) * (Nx + 2);
     hipLaunchKernelGGL(k_rr, dim3(nblk(n)), dim3(NT), 0, ctx->stream, a, Nz, Nx, RR);
     if (bn) {
@@ -3268,8 +3278,7 @@ int xc_fluidmech(xc_ctx* ctx, xc_fluid* fl, const xc_phase* ph, const xc_grid* g
     const bool depth_pending = depth_launch(ctx, s, F, o);
     if ((st = pois_table(ctx, s, F, Nz, Nx, g->h))) return st;
     if ((st = xm_prepare(ctx, s, F, o))) return st;
-    if ((st = build_rhs(ctx, s, fl, ph, g, par, o, bn, F, s->bs))) return st;
-    hipLaunchKernelGGL(k_gather, dim3(nb), dim3(NT), 0, ctx->stream, fl->W, fl->U, fl->P, s->xs, Nz, Nx);
+    if ((st = build_rhs(ctx, s, fl, ph, g, par, o, bn, F, s->bs, s->xs))) return st;  // and the starting guess
     // the starting residual, its x-uniform part R^T r and ||RR|| (summed beside ||r||): one synchronization
     enqueue_r0(ctx, s, F, s->xs, s->bs, true, nullptr, s->host_dev);
     XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
