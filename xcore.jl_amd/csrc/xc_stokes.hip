@@ -1226,27 +1226,6 @@ __global__ __launch_bounds__(NT) void k_sum_final1(const double* __restrict__ pa
         block_sum1(part2, nb, out2);
     }
 }
-// largest viscosity and smallest density of the fine level from k_pack5's per-block partials (one wave)
-__global__ __launch_bounds__(64) void k_eta_rho_final(const double* __restrict__ part, int nb, double* out) {
-    double emax = 0.0, rmin = 1e308;
-    for (int b0 = threadIdx.x; b0 < nb; b0 += 64 * 8) {
-        double e[8], r[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int b = b0 + 64 * u < nb ? b0 + 64 * u : nb - 1;
-            e[u] = part[2 * b];
-            r[u] = part[2 * b + 1];
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            emax = fmax(emax, e[u]);
-            rmin = fmin(rmin, r[u]);
-        }
-    }
-    emax = -wave_min(-emax);
-    rmin = wave_min(rmin);
-    if (threadIdx.x == 0) { out[0] = emax; out[1] = rmin; }
-}
 // ---- learned coarse correction: restriction to the network's grid and prolongation back ----
 // Cell-centred (W, U, P) of a stacked vector at interior cell (r, c), 0-based
 __device__ __forceinline__ void cell_vals(const LevD& L, const double* v, int r, int c, double* w, double* u,
@@ -1952,6 +1931,8 @@ struct StokesCtx {
     double *xs, *bs, *rs, *ws, *V, *Z, *part, *dots, *tmp, *adv_buf;
     double* host = nullptr;   // pinned host scratch for reductions (mapped, coherent)
     double* host_dev = nullptr;  // the same memory as seen by kernels
+    double* pk_host = nullptr;   // k_pack5's per-block max(eta) / min(rho) partials (mapped, coherent), read by
+    double* pk_dev = nullptr;    // choose_depth on the host after the solve's first synchronization
     double* kry = nullptr;    // Arnoldi H, Givens, g, y (device)
     double* d3 = nullptr;     // dot-product results: 2m + 3 of one Arnoldi step, then 4 coefficients
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -2073,6 +2054,7 @@ void stokes_free(xc_ctx*, void* p) {
     StokesCtx* s = (StokesCtx*)p;
     if (s->pool) (void)hipFree(s->pool);
     if (s->host) (void)hipHostFree(s->host);
+    if (s->pk_host) (void)hipHostFree(s->pk_host);
     if (s->xm_host) (void)hipHostFree(s->xm_host);
     if (s->xm_part) (void)hipFree(s->xm_part);
     for (auto& e : s->ev)
@@ -2244,6 +2226,14 @@ int setup(xc_ctx* ctx, int Nz, int Nx, int m, StokesCtx** out, int maxlev = -1) 
         }
         if (e != hipSuccess) { stokes_free(ctx, s); return xc_fail(ctx, XC_EHIP, "fluidmech!: %s", hipGetErrorString(e)); }
     }
+    {  // every k_pack5 launch of this grid: its blocks cover the largest of the five coefficient arrays
+        const int64_t most = (int64_t)(Nz + 1) * (Nx + 2) > (int64_t)(Nz + 2) * (Nx + 1) ? (int64_t)(Nz + 1) * (Nx + 2)
+                                                                                         : (int64_t)(Nz + 2) * (Nx + 1);
+        e = hipHostMalloc((void**)&s->pk_host, 2 * (size_t)nblk(most) * sizeof(double),
+                          hipHostMallocMapped | hipHostMallocCoherent);
+        if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&s->pk_dev, s->pk_host, 0);
+        if (e != hipSuccess) { stokes_free(ctx, s); return xc_fail(ctx, XC_ENOMEM, "fluidmech!: pinned scratch"); }
+    }
     e = hipHostMalloc((void**)&s->host, 256 * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent);
     if (e != hipSuccess) { stokes_free(ctx, s); return xc_fail(ctx, XC_ENOMEM, "fluidmech!: pinned scratch"); }
     e = hipHostGetDevicePointer((void**)&s->host_dev, s->host, 0);
@@ -2304,7 +2294,7 @@ void build_levels(xc_ctx* ctx, StokesCtx* s, const xc_fluid* fl, double h, bool 
     Level& L0 = s->lev[0];
     L0.h = h;
     Pack5 pk{{fl->eta, fl->rho, fl->etaco, fl->rhow, fl->rhou}, {L0.eta, L0.rho, L0.etaco, L0.rhow, L0.rhou},
-             s->f_coef, L0.eta, s->part};
+             s->f_coef, L0.eta, s->pk_dev};
     int64_t most = 0;
     for (int q = 0; q < 5; ++q) most = pk.a[q].rows * pk.a[q].cols > most ? pk.a[q].rows * pk.a[q].cols : most;
     s->pack_blocks = nblk(most);
@@ -2337,14 +2327,12 @@ void build_coarse(xc_ctx* ctx, StokesCtx* s, int upto) {
 // Without inertia (dt = Inf, the benchmark mode) every level is kept.  mg_levels > 0 in the options caps
 // the depth instead.
 constexpr double MG_INERTIA_STOP = 6.0;
-// the reduction choose_depth needs, enqueued (results in s->host[200..201] once the stream has reached it);
-// returns whether it was needed
+// whether choose_depth needs the fine level's largest viscosity and smallest density (k_pack5 writes its per-block
+// partials into mapped host memory; choose_depth reduces them once the stream has passed it)
 bool depth_launch(xc_ctx* ctx, StokesCtx* s, const Flags& F, const xc_fm_opts* o) {
-    if (o->mg_levels > 0 || F.bnchm || !(F.dt < 1e30) || !(F.a1 > 0)) return false;
-    // the per-block partials were written by the fine level's packing kernel
-    hipLaunchKernelGGL(k_eta_rho_final, dim3(1), dim3(64), 0, ctx->stream, s->part, (int)s->pack_blocks,
-                       s->host_dev + 200);
-    return true;
+    (void)ctx;
+    (void)s;
+    return !(o->mg_levels > 0 || F.bnchm || !(F.dt < 1e30) || !(F.a1 > 0));
 }
 // sets s->leff; launched: depth_launch enqueued the reduction; synced: the stream has been synchronized since
 int choose_depth(xc_ctx* ctx, StokesCtx* s, const Flags& F, const xc_fm_opts* o, double h, bool launched,
@@ -2358,7 +2346,11 @@ int choose_depth(xc_ctx* ctx, StokesCtx* s, const Flags& F, const xc_fm_opts* o,
     }
     if (!launched) return XC_OK;
     if (!synced) XC_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    const double emax = s->host[200], rmin = s->host[201];
+    double emax = 0.0, rmin = 1e308;  // max and min are exact in any order
+    for (unsigned b = 0; b < s->pack_blocks; ++b) {
+        emax = fmax(emax, s->pk_host[2 * b]);
+        rmin = fmin(rmin, s->pk_host[2 * b + 1]);
+    }
     if (!(emax > 0) || !(rmin > 0)) return XC_OK;
     const double R0 = F.a1 * rmin * h * h / (F.dt * emax);
     s->inertia = R0;
